@@ -1,0 +1,274 @@
+"""ctypes binding of ``libbmg.so`` (the C ABI declared in ``include/bmg.h``).
+
+There is no fallback: if the HIP library is missing or no GPU is visible the
+calls raise.  Arrays use the reference layout -- ghost-padded
+``(Nx+2, Ny+2, Nz+2)`` float64 blocks, z contiguous; stencils carry a trailing
+axis of 7 ``[C,L,R,B,T,A,F]`` coefficients.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbmg.so")
+
+NEUMANN, DIRICHLET, PERIODIC = 0, 1, 2
+VCYCLE, WCYCLE, FCYCLE = 0, 1, 2
+COARSE_SMOOTH, COARSE_DIRECT = 0, 1
+RBGS, JAC = 0, 1
+FP64, MIXED = 0, 1
+X, B, R = 0, 1, 2
+
+EXPORTS = [
+    "bmg_default_opts", "bmg_last_error", "bmg_device_count", "bmg_create",
+    "bmg_destroy", "bmg_set_opts", "bmg_num_levels", "bmg_level_shape",
+    "bmg_set_operator_stencil", "bmg_set_operator_stencil_device",
+    "bmg_set_operator_laplacian", "bmg_get_operator_stencil", "bmg_solve",
+    "bmg_solve_device", "bmg_upload_field", "bmg_download_field",
+    "bmg_op_prepare", "bmg_op_smooth", "bmg_op_residual", "bmg_op_restrict",
+    "bmg_op_prolong", "bmg_op_correct_bcs", "bmg_op_coarse_solve",
+    "bmg_op_cycle", "bmg_op_sum_sq_residual", "bmg_profile_smoother",
+    "bmg_profile_read",
+]
+
+
+class BmgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"bmg error {code}: {msg}")
+        self.code = code
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int * 3),
+                ("vertices", ctypes.POINTER(ctypes.c_double) * 3),
+                ("bc", ctypes.c_int * 6),
+                ("bc_value", ctypes.c_double * 6)]
+
+
+class Opts(ctypes.Structure):
+    _fields_ = [("rel_tol", ctypes.c_double), ("tolerance", ctypes.c_double),
+                ("min_iter", ctypes.c_int), ("max_iter", ctypes.c_int),
+                ("n_sweeps_pre", ctypes.c_int), ("n_sweeps_post", ctypes.c_int),
+                ("cycle", ctypes.c_int), ("coarse_mode", ctypes.c_int),
+                ("coarse_level", ctypes.c_int), ("smoother", ctypes.c_int),
+                ("precision", ctypes.c_int), ("reserved", ctypes.c_int * 5)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("iters", ctypes.c_int), ("status", ctypes.c_int),
+                ("initial_residual", ctypes.c_double),
+                ("final_residual", ctypes.c_double),
+                ("solve_ms", ctypes.c_double),
+                ("reserved", ctypes.c_double * 4)]
+
+
+_lib = None
+
+
+def load():
+    """Load libbmg.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: run __graft_entry__.build()")
+    lib = ctypes.CDLL(LIB_PATH)
+    P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    DP = ctypes.POINTER(ctypes.c_double)
+    IP = ctypes.POINTER(ctypes.c_int)
+    sig = {
+        "bmg_default_opts": (None, [ctypes.POINTER(Opts)]),
+        "bmg_last_error": (ctypes.c_char_p, []),
+        "bmg_device_count": (I, [IP]),
+        "bmg_create": (I, [ctypes.POINTER(P), ctypes.POINTER(Grid),
+                           ctypes.POINTER(Opts)]),
+        "bmg_destroy": (I, [P]),
+        "bmg_set_opts": (I, [P, ctypes.POINTER(Opts)]),
+        "bmg_num_levels": (I, [P]),
+        "bmg_level_shape": (I, [P, I, IP]),
+        "bmg_set_operator_stencil": (I, [P, I, DP, I]),
+        "bmg_set_operator_stencil_device": (I, [P, I, P, I]),
+        "bmg_set_operator_laplacian": (I, [P, DP, DP, DP]),
+        "bmg_get_operator_stencil": (I, [P, I, DP]),
+        "bmg_solve": (I, [P, DP, DP, I, ctypes.POINTER(Stats)]),
+        "bmg_solve_device": (I, [P, P, P, I, ctypes.POINTER(Stats)]),
+        "bmg_upload_field": (I, [P, I, I, DP]),
+        "bmg_download_field": (I, [P, I, I, DP]),
+        "bmg_op_prepare": (I, [P]),
+        "bmg_op_smooth": (I, [P, I, I]),
+        "bmg_op_residual": (I, [P, I, I]),
+        "bmg_op_restrict": (I, [P, I]),
+        "bmg_op_prolong": (I, [P, I]),
+        "bmg_op_correct_bcs": (I, [P, I]),
+        "bmg_op_coarse_solve": (I, [P]),
+        "bmg_op_cycle": (I, [P]),
+        "bmg_op_sum_sq_residual": (I, [P, DP]),
+        "bmg_profile_smoother": (I, [P, I, I]),
+        "bmg_profile_read": (I, [P, DP, ctypes.POINTER(ctypes.c_longlong), DP]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(code):
+    if code != 0:
+        raise BmgError(code, load().bmg_last_error().decode())
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def default_opts(**kw):
+    o = Opts()
+    load().bmg_default_opts(ctypes.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise KeyError(k)
+        setattr(o, k, v)
+    return o
+
+
+class Context:
+    """One solver instance: hierarchy, operator, device fields."""
+
+    def __init__(self, vertices, bc, bc_value=(0.0,) * 6, **opts):
+        lib = load()
+        self.lib = lib
+        self._verts = [np.ascontiguousarray(v, dtype=np.float64) for v in vertices]
+        g = Grid()
+        for d in range(3):
+            g.n[d] = len(self._verts[d]) - 1
+            g.vertices[d] = _dp(self._verts[d])
+        for f in range(6):
+            g.bc[f] = int(bc[f])
+            g.bc_value[f] = float(bc_value[f])
+        self.opts = default_opts(**opts)
+        ptr = ctypes.c_void_p()
+        _check(lib.bmg_create(ctypes.byref(ptr), ctypes.byref(g),
+                              ctypes.byref(self.opts)))
+        self.ptr = ptr
+        self.nlevels = lib.bmg_num_levels(ptr)
+        self.shapes = []
+        for l in range(self.nlevels):
+            s = (ctypes.c_int * 3)()
+            _check(lib.bmg_level_shape(ptr, l, s))
+            self.shapes.append(tuple(s))
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            self.lib.bmg_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        self.close()
+
+    def padded(self, l):
+        return tuple(q + 2 for q in self.shapes[l])
+
+    def set_opts(self, **kw):
+        for k, v in kw.items():
+            if not hasattr(self.opts, k):
+                raise KeyError(k)
+            setattr(self.opts, k, v)
+        _check(self.lib.bmg_set_opts(self.ptr, ctypes.byref(self.opts)))
+
+    def set_stencil(self, l, A, symmetric=True):
+        A = np.ascontiguousarray(A, dtype=np.float64)
+        if A.shape != self.padded(l) + (7,):
+            raise ValueError(f"stencil shape {A.shape} != {self.padded(l) + (7,)}")
+        _check(self.lib.bmg_set_operator_stencil(self.ptr, l, _dp(A), int(symmetric)))
+
+    def set_laplacian(self, lam=None):
+        if lam is None:
+            null = ctypes.POINTER(ctypes.c_double)()
+            _check(self.lib.bmg_set_operator_laplacian(self.ptr, null, null, null))
+        else:
+            self._lam = [np.ascontiguousarray(a, dtype=np.float64) for a in lam]
+            for a in self._lam:
+                if a.shape != self.padded(0):
+                    raise ValueError("lambda arrays must have the padded level-0 shape")
+            _check(self.lib.bmg_set_operator_laplacian(
+                self.ptr, *[_dp(a) for a in self._lam]))
+
+    def stencil(self, l):
+        A = np.zeros(self.padded(l) + (7,))
+        _check(self.lib.bmg_get_operator_stencil(self.ptr, l, _dp(A)))
+        return A
+
+    def solve(self, b, x=None, const_matrix=False):
+        """MG::solve on host arrays; b and x are updated in place."""
+        if x is None:
+            x = np.zeros(self.padded(0))
+        for a in (b, x):
+            if a.dtype != np.float64 or not a.flags.c_contiguous or a.shape != self.padded(0):
+                raise ValueError("b and x must be C-contiguous float64 padded level-0 arrays")
+        st = Stats()
+        _check(self.lib.bmg_solve(self.ptr, _dp(b), _dp(x), int(const_matrix),
+                                  ctypes.byref(st)))
+        return x, st
+
+    def solve_device(self, b_ptr, x_ptr, const_matrix=False):
+        """MG::solve on device pointers (e.g. torch tensor .data_ptr())."""
+        st = Stats()
+        _check(self.lib.bmg_solve_device(self.ptr, ctypes.c_void_p(b_ptr),
+                                         ctypes.c_void_p(x_ptr), int(const_matrix),
+                                         ctypes.byref(st)))
+        return st
+
+    # level operations (kernel-level parity)
+    def upload(self, which, l, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        if v.shape != self.padded(l):
+            raise ValueError("field shape mismatch")
+        _check(self.lib.bmg_upload_field(self.ptr, which, l, _dp(v)))
+
+    def download(self, which, l):
+        v = np.zeros(self.padded(l))
+        _check(self.lib.bmg_download_field(self.ptr, which, l, _dp(v)))
+        return v
+
+    def prepare(self):
+        _check(self.lib.bmg_op_prepare(self.ptr))
+
+    def smooth(self, l, sweeps):
+        _check(self.lib.bmg_op_smooth(self.ptr, l, sweeps))
+
+    def residual(self, l, corr_bcs=True):
+        _check(self.lib.bmg_op_residual(self.ptr, l, int(corr_bcs)))
+
+    def restrict(self, l):
+        _check(self.lib.bmg_op_restrict(self.ptr, l))
+
+    def prolong(self, l):
+        _check(self.lib.bmg_op_prolong(self.ptr, l))
+
+    def correct_bcs(self, l):
+        _check(self.lib.bmg_op_correct_bcs(self.ptr, l))
+
+    def coarse_solve(self):
+        _check(self.lib.bmg_op_coarse_solve(self.ptr))
+
+    def cycle(self):
+        _check(self.lib.bmg_op_cycle(self.ptr))
+
+    def profile_smoother(self, enable=True, level=0):
+        _check(self.lib.bmg_profile_smoother(self.ptr, int(enable), level))
+
+    def profile_read(self):
+        ms = ctypes.c_double(0.0)
+        n = ctypes.c_longlong(0)
+        by = ctypes.c_double(0.0)
+        _check(self.lib.bmg_profile_read(self.ptr, ctypes.byref(ms), ctypes.byref(n),
+                                         ctypes.byref(by)))
+        return ms.value, n.value, by.value
+
+    def sum_sq_residual(self):
+        out = np.zeros(1)
+        _check(self.lib.bmg_op_sum_sq_residual(self.ptr, _dp(out)))
+        return float(out[0])

@@ -1,0 +1,1608 @@
+// Host side of the MI355X multigrid solver: level hierarchy, metrics,
+// transfer weights, the MG::solve / Solve / cycle control flow and the C ABI
+// declared in include/bmg.h.  All numerical work runs in the HIP kernels of
+// bmg_kernels.hip on the context's stream.
+#include "bmg.h"
+#include "bmg_internal.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <exception>
+#include <functional>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+using namespace bmg;
+
+namespace {
+
+thread_local std::string g_error = "";
+
+struct Error : std::runtime_error
+{
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void hipCheck(hipError_t e, const char* what, int line)
+{
+    if (e != hipSuccess)
+        throw Error(BMG_ERR_HIP, std::string(what) + " (line " +
+                                     std::to_string(line) + "): " +
+                                     hipGetErrorString(e));
+}
+#define HIPCK(x) hipCheck((x), #x, __LINE__)
+
+// Levels with more cells than this keep a matrix-free / face-coefficient
+// operator; smaller ones (and the pinned coarsest level) store the stencil.
+constexpr long MATERIALIZE_CELLS = 32768;
+
+// utils.H:1010-1021
+double trimPrecision(double s)
+{
+    if (s != 0.0)
+    {
+        const double factor =
+            std::pow(10.0, 14 - 1 - static_cast<int>(std::log10(std::fabs(s))));
+        s = std::round(s*factor)/factor;
+    }
+    return s;
+}
+
+// vertexVector.C:183-188
+double round12(double u) { return std::round(u/1e-12)*1e-12; }
+
+// hexa.H:120-128, one axis of a rectilinear cell
+double hexaCentre(double lo, double hi, int axis)
+{
+    static const int pat[3][8] = {{0,1,0,1,0,1,0,1}, {0,0,1,1,0,0,1,1},
+                                  {0,0,0,0,1,1,1,1}};
+    double c = (pat[axis][0] ? hi : lo)/8.0;
+    for (int i = 1; i < 8; i++) c += (pat[axis][i] ? hi : lo)/8.0;
+    return c;
+}
+
+struct HostAxis
+{
+    int N = 0, R = 0;
+    std::vector<double> v, cc, w, dl;
+};
+
+template <typename T>
+T* devAlloc(size_t n)
+{
+    if (n == 0) return nullptr;
+    void* p = nullptr;
+    HIPCK(hipMalloc(&p, n*sizeof(T)));
+    return static_cast<T*>(p);
+}
+
+template <typename T>
+void devFree(T*& p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+template <typename T>
+T* devUpload(const std::vector<T>& h)
+{
+    T* d = devAlloc<T>(h.size());
+    HIPCK(hipMemcpy(d, h.data(), h.size()*sizeof(T), hipMemcpyHostToDevice));
+    return d;
+}
+
+struct Level
+{
+    LevelDims L{};
+    HostAxis ax[3];
+    int opkind = OP_STENCIL;
+    bool haveOp = false;
+    // FP64 fields
+    double* x[2] = {nullptr, nullptr};
+    double* b[2] = {nullptr, nullptr};
+    double* r[2] = {nullptr, nullptr};
+    double* tmp[2] = {nullptr, nullptr};
+    // FP32 fields (mixed precision)
+    float* xf[2] = {nullptr, nullptr};
+    float* bf[2] = {nullptr, nullptr};
+    float* rf[2] = {nullptr, nullptr};
+    float* tmpf[2] = {nullptr, nullptr};
+    // operator
+    double* a7[2] = {nullptr, nullptr};
+    float* a7f[2] = {nullptr, nullptr};
+    double* wf[2] = {nullptr, nullptr};
+    float* wff[2] = {nullptr, nullptr};
+    // metrics and transfer data
+    double* dw[3] = {nullptr, nullptr, nullptr};
+    double* ddl[3] = {nullptr, nullptr, nullptr};
+    float* fw[3] = {nullptr, nullptr, nullptr};
+    float* fdl[3] = {nullptr, nullptr, nullptr};
+    double* ru[3] = {nullptr, nullptr, nullptr};
+    double* pu[3] = {nullptr, nullptr, nullptr};
+    float* ruf[3] = {nullptr, nullptr, nullptr};
+    float* puf[3] = {nullptr, nullptr, nullptr};
+
+    long cells() const { return long(L.N[0])*L.N[1]*L.N[2]; }
+
+    void freeAll()
+    {
+        for (int c = 0; c < 2; c++)
+        {
+            if (rf[c] == bf[c]) rf[c] = nullptr;   // level-0 alias
+            devFree(x[c]); devFree(b[c]); devFree(r[c]); devFree(tmp[c]);
+            devFree(xf[c]); devFree(bf[c]); devFree(rf[c]); devFree(tmpf[c]);
+            devFree(a7[c]); devFree(a7f[c]); devFree(wf[c]); devFree(wff[c]);
+        }
+        for (int d = 0; d < 3; d++)
+        {
+            devFree(dw[d]); devFree(ddl[d]); devFree(fw[d]); devFree(fdl[d]);
+            devFree(ru[d]); devFree(pu[d]); devFree(ruf[d]); devFree(puf[d]);
+        }
+    }
+};
+
+template <typename T>
+Pair<T> pr(T* const (&a)[2])
+{
+    return Pair<T>{{a[0], a[1]}};
+}
+
+template <typename T>
+Pair<const T> cpr(T* const (&a)[2])
+{
+    return Pair<const T>{{a[0], a[1]}};
+}
+
+}  // namespace
+
+struct bmg_ctx
+{
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int bc[6];
+    double bcval[6];
+    bmg_opts opts;
+    std::vector<Level> lv;
+    bool symmetric = true;
+    bool eliminated = false;
+    bool prepared = false;       // coarse factorisation valid
+    int preparedCoarse = -1;
+    // coarse LU
+    double* dLU = nullptr;
+    int* dPiv = nullptr;
+    int nc = 0;
+    // reductions
+    double* partials = nullptr;
+    size_t npartials = 0;
+    double* dScalar = nullptr;
+    double* hScalar = nullptr;   // pinned
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool mixedReady = false;
+    // smoother profiling (bmg_profile_smoother)
+    bool profOn = false;
+    int profLevel = 0;
+    std::vector<hipEvent_t> profEv;     // start/stop pairs
+    int profPending = 0;
+    std::vector<double> profBytesPending;
+    double profMs = 0.0, profBytes = 0.0;
+    long long profLaunches = 0;
+
+    int nLevels() const { return int(lv.size()); }
+    int coarseLevel() const { return nLevels() - opts.coarse_level - 1; }
+    bool periodic(int f) const { return bc[f] == BMG_PERIODIC; }
+    bool singular() const
+    {
+        if (!symmetric) return false;
+        for (int f = 0; f < 6; f++)
+            if (bc[f] == BMG_DIRICHLET) return false;
+        return true;
+    }
+    bool anyPeriodic() const
+    {
+        for (int f = 0; f < 6; f++)
+            if (periodic(f)) return true;
+        return false;
+    }
+
+    ~bmg_ctx()
+    {
+        for (auto& l : lv) l.freeAll();
+        devFree(dLU);
+        devFree(dPiv);
+        devFree(partials);
+        devFree(dScalar);
+        if (hScalar) (void)hipHostFree(hScalar);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        for (hipEvent_t e : profEv) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    // ---------------------------------------------------------------- views
+    template <typename T>
+    OpView<T> op(int l) const;
+
+    template <typename T>
+    TransferView<T> transfer(int lf) const;
+
+    template <typename T>
+    Pair<T> X(int l);
+    template <typename T>
+    Pair<T> B(int l);
+    template <typename T>
+    Pair<T> Rr(int l);
+    template <typename T>
+    Pair<T> Tmp(int l);
+
+    // -------------------------------------------------------------- helpers
+    double reduceFinal(int nblocks)
+    {
+        launch_reduce_final(partials, nblocks, dScalar, stream);
+        HIPCK(hipGetLastError());
+        HIPCK(hipMemcpyAsync(hScalar, dScalar, sizeof(double),
+                             hipMemcpyDeviceToHost, stream));
+        HIPCK(hipStreamSynchronize(stream));
+        harvestProfile();
+        return *hScalar;
+    }
+
+    static constexpr int PROF_RING = 256;
+
+    // events of completed launches -> totals (call after a stream sync)
+    void harvestProfile()
+    {
+        for (int i = 0; i < profPending; i++)
+        {
+            float ms = 0.0f;
+            HIPCK(hipEventElapsedTime(&ms, profEv[2*i], profEv[2*i + 1]));
+            profMs += ms;
+            profBytes += profBytesPending[i];
+            profLaunches++;
+        }
+        profPending = 0;
+    }
+
+    template <typename T>
+    void profiledRbgs(int l, const OpView<T>& o, Pair<T> x, Pair<const T> b,
+                      int colour, const Region& rg)
+    {
+        const bool prof = profOn && l == profLevel;
+        if (prof && profPending == PROF_RING)
+        {
+            HIPCK(hipStreamSynchronize(stream));
+            harvestProfile();
+        }
+        if (prof)
+        {
+            if (profEv.empty())
+            {
+                profEv.resize(2*PROF_RING);
+                for (auto& e : profEv) HIPCK(hipEventCreate(&e));
+                profBytesPending.resize(PROF_RING);
+            }
+            HIPCK(hipEventRecord(profEv[2*profPending], stream));
+        }
+        launch_rbgs(lv[l].L, o, x, b, colour, rg, stream);
+        if (prof)
+        {
+            HIPCK(hipEventRecord(profEv[2*profPending + 1], stream));
+            long m[3];
+            for (int d = 0; d < 3; d++) m[d] = std::max(0, rg.E[d] - rg.S[d]);
+            const long tot = m[0]*m[1]*m[2];
+            const int cs = (rg.S[0] + rg.S[1] + rg.S[2]) & 1;
+            const long cells = (tot % 2 == 0) ? tot/2
+                             : (colour == cs ? (tot + 1)/2 : tot/2);
+            const double per = o.kind == OP_CONST ? 3.0
+                             : o.kind == OP_FACES ? 9.0 : 10.0;
+            profBytesPending[profPending] = double(cells)*per*sizeof(T);
+            profPending++;
+        }
+    }
+
+    Region interior(int l, int padding) const
+    {
+        Region rg;
+        for (int d = 0; d < 3; d++)
+        {
+            const int padLo = periodic(2*d), padHi = periodic(2*d + 1);
+            rg.S[d] = 1 + padding*padLo;
+            rg.E[d] = lv[l].L.N[d] + 1 - padding*padHi;
+        }
+        return rg;
+    }
+
+    Region slab(int l, int f) const
+    {
+        Region rg;
+        for (int d = 0; d < 3; d++) { rg.S[d] = 1; rg.E[d] = lv[l].L.N[d] + 1; }
+        const int fd = f/2;
+        if (f % 2 == 0) rg.E[fd] = rg.S[fd] + 1;
+        else rg.S[fd] = rg.E[fd] - 1;
+        return rg;
+    }
+
+    template <typename T>
+    void smooth(int l, int sweeps);
+    template <typename T>
+    void residual(int l, bool corrBcs, bool norm, int* nblocks);
+    template <typename T>
+    void restrictTo(int l);
+    template <typename T>
+    void prolongFrom(int l);
+    template <typename T>
+    void coarseSolve();
+    template <typename T>
+    void cycle(std::vector<int>& visits, int l);
+
+    void prepare(bool constMatrix);
+    void prepareCoarse();
+    void ensureMixed();
+    double normFactor();
+    void runSolve(bool constMatrix, bmg_stats* st);
+};
+
+template <>
+OpView<double> bmg_ctx::op<double>(int l) const
+{
+    const Level& q = lv[l];
+    OpView<double> o{};
+    o.kind = q.opkind;
+    for (int c = 0; c < 2; c++) { o.a7[c] = q.a7[c]; o.wf[c] = q.wf[c]; }
+    for (int d = 0; d < 3; d++) { o.w[d] = q.dw[d]; o.dl[d] = q.ddl[d]; }
+    return o;
+}
+
+template <>
+OpView<float> bmg_ctx::op<float>(int l) const
+{
+    const Level& q = lv[l];
+    OpView<float> o{};
+    o.kind = q.opkind;
+    for (int c = 0; c < 2; c++) { o.a7[c] = q.a7f[c]; o.wf[c] = q.wff[c]; }
+    for (int d = 0; d < 3; d++) { o.w[d] = q.fw[d]; o.dl[d] = q.fdl[d]; }
+    return o;
+}
+
+template <>
+TransferView<double> bmg_ctx::transfer<double>(int lf) const
+{
+    TransferView<double> t{};
+    for (int d = 0; d < 3; d++)
+    {
+        t.ru[d] = lv[lf + 1].ru[d];
+        t.pu[d] = lv[lf].pu[d];
+        t.fw[d] = lv[lf].dw[d];
+        t.cw[d] = lv[lf + 1].dw[d];
+    }
+    return t;
+}
+
+template <>
+TransferView<float> bmg_ctx::transfer<float>(int lf) const
+{
+    TransferView<float> t{};
+    for (int d = 0; d < 3; d++)
+    {
+        t.ru[d] = lv[lf + 1].ruf[d];
+        t.pu[d] = lv[lf].puf[d];
+        t.fw[d] = lv[lf].fw[d];
+        t.cw[d] = lv[lf + 1].fw[d];
+    }
+    return t;
+}
+
+template <> Pair<double> bmg_ctx::X<double>(int l) { return pr(lv[l].x); }
+template <> Pair<double> bmg_ctx::B<double>(int l) { return pr(lv[l].b); }
+template <> Pair<double> bmg_ctx::Rr<double>(int l) { return pr(lv[l].r); }
+template <> Pair<double> bmg_ctx::Tmp<double>(int l)
+{
+    Level& q = lv[l];
+    if (!q.tmp[0])
+        for (int c = 0; c < 2; c++) q.tmp[c] = devAlloc<double>(q.L.ncol);
+    return pr(q.tmp);
+}
+template <> Pair<float> bmg_ctx::X<float>(int l) { return pr(lv[l].xf); }
+template <> Pair<float> bmg_ctx::B<float>(int l) { return pr(lv[l].bf); }
+template <> Pair<float> bmg_ctx::Rr<float>(int l) { return pr(lv[l].rf); }
+template <> Pair<float> bmg_ctx::Tmp<float>(int l)
+{
+    Level& q = lv[l];
+    if (!q.tmpf[0])
+        for (int c = 0; c < 2; c++) q.tmpf[c] = devAlloc<float>(q.L.ncol);
+    return pr(q.tmpf);
+}
+
+template <typename T>
+static Pair<const T> cst(Pair<T> p)
+{
+    return Pair<const T>{{p.c[0], p.c[1]}};
+}
+
+// rbgsSmoother::Smooth (rbgsSmoother.C:376-437) / jacSmoother::Smooth.  The
+// four counter-relative phases of every region (rbgsSmoother.C:65-183) are
+// turned into colours of the padded index: forward even = colour of S,
+// forward odd = the other, backward even/odd anchored at E-1.  A phase that
+// repeats the phase right before it on the same region is skipped: it reads
+// only the other colour, which did not change, so it would rewrite the same
+// values.
+template <typename T>
+void bmg_ctx::smooth(int l, int sweeps)
+{
+    Level& q = lv[l];
+    const OpView<T> o = op<T>(l);
+    Pair<T> x = X<T>(l);
+    Pair<const T> b = cst(B<T>(l));
+    if (opts.smoother == BMG_JAC)
+    {
+        Pair<T> tmp = Tmp<T>(l);
+        for (int s = 0; s < sweeps; s++)
+        {
+            launch_jacobi(q.L, o, x, b, tmp, interior(l, 1), stream);
+            if (anyPeriodic())
+            {
+                launch_bc_fill(q.L, o, x, 0, l == 0, bcval, stream);
+                for (int f = 0; f < 6; f++)
+                    if (periodic(f))
+                        launch_jacobi(q.L, o, x, b, tmp, slab(l, f), stream);
+            }
+        }
+        return;
+    }
+    struct OpRec { int kind; Region rg; int colour; };
+    std::vector<OpRec> ops;
+    auto phases = [&](const Region& rg)
+    {
+        const int Ss = rg.S[0] + rg.S[1] + rg.S[2];
+        const int Es = rg.E[0] + rg.E[1] + rg.E[2];
+        const int cols[4] = {Ss & 1, (Ss + 1) & 1, (Es + 1) & 1, Es & 1};
+        for (int c : cols) ops.push_back({0, rg, c});
+    };
+    for (int s = 0; s < sweeps; s++)
+    {
+        phases(interior(l, 1));
+        if (anyPeriodic())
+        {
+            ops.push_back({1, Region{}, 0});
+            for (int f = 0; f < 6; f++)
+                if (periodic(f)) phases(slab(l, f));
+        }
+    }
+    const OpRec* prev = nullptr;
+    for (const OpRec& r : ops)
+    {
+        if (r.kind == 0)
+        {
+            const bool same = prev && prev->kind == 0 &&
+                              prev->colour == r.colour &&
+                              std::memcmp(&prev->rg, &r.rg, sizeof(Region)) == 0;
+            if (!same)
+                profiledRbgs<T>(l, o, x, b, r.colour, r.rg);
+        }
+        else
+            launch_bc_fill(q.L, o, x, 0, l == 0, bcval, stream);
+        prev = &r;
+    }
+}
+
+// linearSystem::residual (linearSystemResidual.C:280-332).  With BC
+// correction the periodic ghosts are refreshed first: the interior part of
+// the reference's split never reads them, so the values are the same.
+template <typename T>
+void bmg_ctx::residual(int l, bool corrBcs, bool norm, int* nblocks)
+{
+    Level& q = lv[l];
+    const OpView<T> o = op<T>(l);
+    if (corrBcs && anyPeriodic())
+        launch_bc_fill(q.L, o, X<T>(l), 0, l == 0, bcval, stream);
+    Region rg;
+    for (int d = 0; d < 3; d++) { rg.S[d] = 1; rg.E[d] = q.L.N[d] + 1; }
+    launch_residual(q.L, o, cst(X<T>(l)), cst(B<T>(l)), Rr<T>(l), rg,
+                    norm ? partials : nullptr, nblocks, stream);
+}
+
+template <typename T>
+void bmg_ctx::restrictTo(int l)
+{
+    launch_restrict(lv[l].L, lv[l + 1].L, transfer<T>(l), cst(Rr<T>(l)),
+                    B<T>(l + 1), stream);
+}
+
+template <typename T>
+void bmg_ctx::prolongFrom(int l)
+{
+    // the coarse ghosts are used, so correct them first
+    // (linearProlongationScheme.C:128)
+    launch_bc_fill(lv[l + 1].L, op<T>(l + 1), X<T>(l + 1), 1, (l + 1) == 0,
+                   bcval, stream);
+    launch_prolong(lv[l].L, lv[l + 1].L, transfer<T>(l), cst(X<T>(l + 1)),
+                   X<T>(l), stream);
+}
+
+template <typename T>
+void bmg_ctx::coarseSolve()
+{
+    const int l = coarseLevel();
+    if (!prepared || preparedCoarse != l)
+        throw Error(BMG_ERR_STATE, "coarse level not factorised");
+    launch_coarse_solve(lv[l].L, op<T>(l), X<T>(l), cst(B<T>(l)), dLU, dPiv,
+                        nc, l == 0, bcval, stream);
+}
+
+// MG::cycle (MG.C:19-129).  On the coarsest level in direct mode the
+// pre-smoothing is skipped: APLU::solve overwrites every interior value and
+// reads ghosts only through eliminated (zero) couplings, so the result is the
+// same.
+template <typename T>
+void bmg_ctx::cycle(std::vector<int>& visits, int l)
+{
+    Level& q = lv[l];
+    const int lc = coarseLevel();
+    if (singular() && l == lc)
+        launch_set_cell<T>(B<T>(l), ((long)1*q.L.P[1] + 1)*q.L.Hz + 0,
+                           (1 + 1 + 1) & 1, T(0), stream);
+    if (l > 0)
+        launch_fill<T>(X<T>(l), q.L.ncol, T(0), stream);
+    const bool direct = opts.coarse_mode == BMG_COARSE_DIRECT;
+    if (opts.n_sweeps_pre > 0 && !(l == lc && direct))
+        smooth<T>(l, opts.n_sweeps_pre);
+    if (l < lc)
+    {
+        int reps = 1;
+        if (opts.cycle == BMG_WCYCLE) reps = 1 + (l > 0);
+        else if (opts.cycle == BMG_FCYCLE) reps = 1 + (l > 0 && visits[l] == 0);
+        for (int rep = 0; rep < reps; rep++)
+        {
+            if (rep > 0 || l > 0)
+                residual<T>(l, true, false, nullptr);
+            restrictTo<T>(l);
+            cycle<T>(visits, l + 1);
+            prolongFrom<T>(l);
+            if (opts.n_sweeps_post > 0)
+                smooth<T>(l, opts.n_sweeps_post);
+        }
+    }
+    else
+    {
+        if (direct)
+            coarseSolve<T>();
+        else
+            smooth<T>(l, std::max(opts.n_sweeps_post, 2));
+    }
+    visits[l]++;
+}
+
+// APLU::prepare (APLU.C:26-89): dense matrix of the coarse level from the
+// CSR rows of linearSystemAggregation (out-of-system couplings removed,
+// repeated columns merged, linearSystemAggregation.C:260-285), then Crout LU
+// with implicit partial pivoting (Press et al., ludcmp), the algorithm of
+// OpenFOAM's LUDecompose.
+void bmg_ctx::prepareCoarse()
+{
+    const int l = coarseLevel();
+    Level& q = lv[l];
+    const LevelDims& L = q.L;
+    const int n = L.N[0]*L.N[1]*L.N[2];
+    if (n > 4096)
+        throw Error(BMG_ERR_ARG, "coarse level too large for the direct "
+                                 "solver (more than 4096 cells)");
+    std::vector<double> h0(L.ncol*7), h1(L.ncol*7);
+    HIPCK(hipStreamSynchronize(stream));
+    HIPCK(hipMemcpy(h0.data(), q.a7[0], h0.size()*sizeof(double),
+                    hipMemcpyDeviceToHost));
+    HIPCK(hipMemcpy(h1.data(), q.a7[1], h1.size()*sizeof(double),
+                    hipMemcpyDeviceToHost));
+    auto number = [&](int I, int J, int K) -> long
+    {
+        int g[3] = {I - 1, J - 1, K - 1};
+        for (int d = 0; d < 3; d++)
+        {
+            if (g[d] < 0 || g[d] >= L.N[d])
+            {
+                if (!periodic(2*d + (g[d] >= L.N[d]))) return -1;
+                g[d] = (g[d] + L.N[d]) % L.N[d];
+            }
+        }
+        return (long(g[0])*L.N[1] + g[1])*L.N[2] + g[2];
+    };
+    static const int off[7][3] = {{0,0,0},{-1,0,0},{1,0,0},{0,-1,0},{0,1,0},
+                                  {0,0,-1},{0,0,1}};
+    std::vector<double> D(size_t(n)*n, 0.0);
+    for (int I = 1; I <= L.N[0]; I++)
+    for (int J = 1; J <= L.N[1]; J++)
+    for (int K = 1; K <= L.N[2]; K++)
+    {
+        const long row = number(I, J, K);
+        const int colour = (I + J + K) & 1;
+        const long idx = ((long)I*L.P[1] + J)*L.Hz + (K >> 1);
+        const std::vector<double>& h = colour ? h1 : h0;
+        double a[7];
+        long col[7];
+        for (int s = 0; s < 7; s++)
+        {
+            a[s] = h[s*L.ncol + idx];
+            col[s] = number(I + off[s][0], J + off[s][1], K + off[s][2]);
+            if (col[s] < 0 || col[s] >= n) a[s] = 0;
+        }
+        for (int s = 1; s < 7; s++)
+        {
+            int t = 0;
+            while (col[t] != col[s]) t++;
+            if (t < s) { a[t] += a[s]; a[s] = 0; }
+        }
+        for (int s = 0; s < 7; s++)
+            if (a[s] != 0) D[size_t(row)*n + col[s]] = a[s];
+    }
+    // Crout LU, implicit pivoting
+    std::vector<double> vv(n);
+    std::vector<int> piv(n, 0);
+    for (int i = 0; i < n; i++)
+    {
+        double big = 0.0;
+        for (int j = 0; j < n; j++) big = std::max(big, std::fabs(D[size_t(i)*n + j]));
+        if (big == 0.0)
+            throw Error(BMG_ERR_SINGULAR, "coarse matrix has an empty row");
+        vv[i] = 1.0/big;
+    }
+    for (int j = 0; j < n; j++)
+    {
+        for (int i = 0; i < j; i++)
+        {
+            double sum = D[size_t(i)*n + j];
+            for (int k = 0; k < i; k++) sum -= D[size_t(i)*n + k]*D[size_t(k)*n + j];
+            D[size_t(i)*n + j] = sum;
+        }
+        int imax = j;
+        double big = 0.0;
+        for (int i = j; i < n; i++)
+        {
+            double sum = D[size_t(i)*n + j];
+            for (int k = 0; k < j; k++) sum -= D[size_t(i)*n + k]*D[size_t(k)*n + j];
+            D[size_t(i)*n + j] = sum;
+            const double t = vv[i]*std::fabs(sum);
+            if (t >= big) { big = t; imax = i; }
+        }
+        piv[j] = imax;
+        if (j != imax)
+        {
+            for (int k = 0; k < n; k++)
+                std::swap(D[size_t(imax)*n + k], D[size_t(j)*n + k]);
+            vv[imax] = vv[j];
+        }
+        if (D[size_t(j)*n + j] == 0.0) D[size_t(j)*n + j] = 1e-15;
+        if (j != n - 1)
+        {
+            const double rr = 1.0/D[size_t(j)*n + j];
+            for (int i = j + 1; i < n; i++) D[size_t(i)*n + j] *= rr;
+        }
+    }
+    devFree(dLU);
+    devFree(dPiv);
+    dLU = devUpload(D);
+    dPiv = devUpload(piv);
+    nc = n;
+    prepared = true;
+    preparedCoarse = l;
+}
+
+// MG.C:431-459: singularity pin of the coarse level, ghost elimination
+// (with the level-0 inhomogeneous source terms), coarse factorisation.
+void bmg_ctx::prepare(bool constMatrix)
+{
+    for (auto& q : lv)
+        if (!q.haveOp)
+            throw Error(BMG_ERR_STATE, "operator not set on every level");
+    const int lc = coarseLevel();
+    if (lc < 1 && nLevels() > 1)
+        throw Error(BMG_ERR_ARG, "coarse level is set too high");
+    Level& qc = lv[lc];
+    if (qc.opkind != OP_STENCIL)
+    {
+        // the pinned coarse level needs the stored stencil
+        for (int c = 0; c < 2; c++)
+        {
+            qc.a7[c] = devAlloc<double>(qc.L.ncol*7);
+            HIPCK(hipMemsetAsync(qc.a7[c], 0, qc.L.ncol*7*sizeof(double),
+                                 stream));
+        }
+        launch_materialize_stencil(qc.L, op<double>(lc), pr(qc.a7), stream);
+        qc.opkind = OP_STENCIL;
+        mixedReady = false;
+    }
+    if (singular())
+        launch_pin(qc.L, pr(qc.a7), stream);
+    if (!eliminated)
+    {
+        bool inhom = false;
+        for (int f = 0; f < 6; f++)
+            if (!periodic(f) && bcval[f] != 0.0) inhom = true;
+        if (inhom)
+            launch_bc_source(lv[0].L, op<double>(0), pr(lv[0].b), bcval, stream);
+        for (auto& q : lv)
+            if (q.opkind == OP_STENCIL)
+                launch_eliminate(q.L, pr(q.a7), stream);
+        eliminated = true;
+        mixedReady = false;
+    }
+    HIPCK(hipGetLastError());
+    if (opts.coarse_mode == BMG_COARSE_DIRECT &&
+        (!constMatrix || !prepared || preparedCoarse != lc))
+        prepareCoarse();
+}
+
+// FP32 copies of the operator and transfer data for mixed-precision cycles.
+void bmg_ctx::ensureMixed()
+{
+    if (mixedReady) return;
+    for (int l = 0; l < nLevels(); l++)
+    {
+        Level& q = lv[l];
+        const long n = q.L.ncol;
+        for (int c = 0; c < 2; c++)
+        {
+            if (!q.xf[c]) q.xf[c] = devAlloc<float>(n);
+            if (!q.bf[c]) q.bf[c] = devAlloc<float>(n);
+            if (l > 0 && !q.rf[c]) q.rf[c] = devAlloc<float>(n);
+        }
+        if (l == 0)
+            for (int c = 0; c < 2; c++) q.rf[c] = q.bf[c];  // r aliases b
+        launch_fill<float>(pr(q.xf), n, 0.0f, stream);
+        launch_fill<float>(pr(q.bf), n, 0.0f, stream);
+        if (q.opkind == OP_STENCIL)
+        {
+            for (int c = 0; c < 2; c++)
+                if (!q.a7f[c]) q.a7f[c] = devAlloc<float>(n*7);
+            Pair<const double> in{{q.a7[0], q.a7[1]}};
+            launch_to_float(n*7, in, pr(q.a7f), stream);
+        }
+        else if (q.opkind == OP_FACES)
+        {
+            for (int c = 0; c < 2; c++)
+                if (!q.wff[c]) q.wff[c] = devAlloc<float>(n*3);
+            Pair<const double> in{{q.wf[0], q.wf[1]}};
+            launch_to_float(n*3, in, pr(q.wff), stream);
+        }
+        for (int d = 0; d < 3; d++)
+        {
+            auto f32 = [](const std::vector<double>& v)
+            {
+                std::vector<float> o(v.size());
+                for (size_t i = 0; i < v.size(); i++) o[i] = float(v[i]);
+                return o;
+            };
+            if (!q.fw[d]) q.fw[d] = devUpload(f32(q.ax[d].w));
+            if (!q.fdl[d]) q.fdl[d] = devUpload(f32(q.ax[d].dl));
+        }
+    }
+    // transfer u arrays in FP32 (host copies were uploaded in FP64)
+    for (int l = 0; l < nLevels(); l++)
+    {
+        Level& q = lv[l];
+        for (int d = 0; d < 3; d++)
+        {
+            auto conv = [&](double* src, float*& dst, int n)
+            {
+                if (!src || dst) return;
+                std::vector<double> h(n);
+                HIPCK(hipMemcpy(h.data(), src, n*sizeof(double),
+                                hipMemcpyDeviceToHost));
+                std::vector<float> f(n);
+                for (int i = 0; i < n; i++) f[i] = float(h[i]);
+                dst = devUpload(f);
+            };
+            conv(q.ru[d], q.ruf[d], q.L.P[d]);
+            conv(q.pu[d], q.puf[d], q.L.P[d]);
+        }
+    }
+    HIPCK(hipGetLastError());
+    mixedReady = true;
+}
+
+// solver::normFactors (solver.C:13-61)
+double bmg_ctx::normFactor()
+{
+    Level& q = lv[0];
+    const double n = double(q.cells());
+    int nb = 0;
+    launch_sum<double>(q.L, cpr(q.x), partials, &nb, stream);
+    const double sum = reduceFinal(nb);
+    const double y = sum/n;
+    launch_norm_factor_terms<double>(q.L, op<double>(0), cpr(q.r), cpr(q.b), y,
+                                     partials, &nb, stream);
+    double f = reduceFinal(nb);
+    f = std::sqrt(f);
+    return std::max(f, n*1e-12);
+}
+
+// MG::Solve (MG.C:132-254) around the cycles; FP64 or mixed precision.
+void bmg_ctx::runSolve(bool constMatrix, bmg_stats* st)
+{
+    prepare(constMatrix);
+    Level& q0 = lv[0];
+    const bool mixed = opts.precision == BMG_MIXED;
+    if (mixed) ensureMixed();
+
+    launch_fill<double>(pr(q0.r), q0.L.ncol, 0.0, stream);
+    int nb = 0;
+    Region full;
+    for (int d = 0; d < 3; d++) { full.S[d] = 1; full.E[d] = q0.L.N[d] + 1; }
+    launch_residual(q0.L, op<double>(0), cpr(q0.x), cpr(q0.b), pr(q0.r), full,
+                    partials, &nb, stream);
+    const double ss0 = reduceFinal(nb);
+    const double nf = normFactor();
+    const double init = std::sqrt(ss0)/nf;
+    if (!std::isfinite(init))
+        throw Error(BMG_ERR_NUMERIC, "initial residual is not finite");
+    double cur = init;
+    auto check = [&](double c) -> int
+    {
+        return (init > 0.0 && c > opts.tolerance &&
+                c/(init + 1e-20) > opts.rel_tol) ? 0 : 1;
+    };
+    int conv = check(cur);
+    int iter = 0;
+    std::vector<int> visits(nLevels(), 0);
+    while ((iter < opts.max_iter && conv == 0) || iter < opts.min_iter)
+    {
+        if (iter < opts.min_iter) conv = 0;
+        std::fill(visits.begin(), visits.end(), 0);
+        if (!mixed)
+            cycle<double>(visits, 0);
+        else
+        {
+            // defect equation A e = r in FP32, e0 = 0; x += e in FP64
+            Pair<const double> r64{{q0.r[0], q0.r[1]}};
+            launch_to_float(q0.L.ncol, r64, pr(q0.bf), stream);
+            launch_fill<float>(pr(q0.xf), q0.L.ncol, 0.0f, stream);
+            cycle<float>(visits, 0);
+            launch_add_correction(q0.L, pr(q0.x), cpr(q0.xf), stream);
+        }
+        residual<double>(0, true, true, &nb);
+        const double ss = reduceFinal(nb);
+        cur = std::sqrt(ss)/nf;
+        if (!std::isfinite(cur))
+            throw Error(BMG_ERR_NUMERIC, "residual norm is not finite");
+        conv = check(cur);
+        iter++;
+    }
+    // x[0].correct<eliminatedBcs>() (MG.C:475)
+    launch_bc_fill(q0.L, op<double>(0), pr(q0.x), 2, 1, bcval, stream);
+    HIPCK(hipGetLastError());
+    if (st)
+    {
+        st->iters = iter;
+        st->initial_residual = init;
+        st->final_residual = cur;
+    }
+}
+
+namespace {
+
+int run(const std::function<void()>& f)
+{
+    try
+    {
+        f();
+        return BMG_OK;
+    }
+    catch (const Error& e)
+    {
+        g_error = e.what();
+        return e.code;
+    }
+    catch (const std::exception& e)
+    {
+        g_error = e.what();
+        return BMG_ERR_ARG;
+    }
+    catch (...)
+    {
+        g_error = "unknown error";
+        return BMG_ERR_ARG;
+    }
+}
+
+void buildAxis(HostAxis& a, const std::vector<double>& verts, bool periodic,
+               int axis)
+{
+    const int N = a.N;
+    a.v.assign(N + 3, 0.0);
+    for (int c = 0; c <= N; c++) a.v[c + 1] = verts[c];
+    // levelPoints.C:230-237 (mirrored) / :389-401 (periodic)
+    if (!periodic)
+    {
+        a.v[0] = a.v[1] + (a.v[1] - a.v[2]);
+        a.v[N + 2] = a.v[N + 1] + (a.v[N + 1] - a.v[N]);
+    }
+    else
+    {
+        a.v[0] = a.v[1] - (a.v[N + 1] - a.v[N]);
+        a.v[N + 2] = a.v[N + 1] + (a.v[2] - a.v[1]);
+    }
+    for (auto& s : a.v) s = trimPrecision(s);
+    a.cc.assign(N + 2, 0.0);
+    a.w.assign(N + 2, 0.0);
+    a.dl.assign(N + 2, 0.0);
+    for (int p = 0; p < N + 2; p++)
+    {
+        a.cc[p] = hexaCentre(a.v[p], a.v[p + 1], axis);
+        a.w[p] = a.v[p + 1] - a.v[p];
+    }
+    // fvMeshMetrics.C:876
+    for (int p = 1; p < N + 2; p++)
+        a.dl[p] = 1.0/std::fabs(a.cc[p] - a.cc[p - 1]);
+}
+
+void checkLevel(const bmg_ctx* c, int level)
+{
+    if (!c) throw Error(BMG_ERR_ARG, "null context");
+    if (level < 0 || level >= c->nLevels())
+        throw Error(BMG_ERR_ARG, "level out of range");
+}
+
+}  // namespace
+
+extern "C" {
+
+void bmg_default_opts(bmg_opts* o)
+{
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->rel_tol = 0.0;
+    o->tolerance = 0.0;
+    o->min_iter = 0;
+    o->max_iter = 999;
+    o->n_sweeps_pre = 0;
+    o->n_sweeps_post = 2;
+    o->cycle = BMG_FCYCLE;
+    o->coarse_mode = BMG_COARSE_DIRECT;
+    o->coarse_level = 0;
+    o->smoother = BMG_RBGS;
+    o->precision = BMG_FP64;
+}
+
+const char* bmg_last_error(void) { return g_error.c_str(); }
+
+int bmg_device_count(int* count)
+{
+    return run([&] {
+        if (!count) throw Error(BMG_ERR_ARG, "null pointer");
+        HIPCK(hipGetDeviceCount(count));
+    });
+}
+
+static void validateOpts(const bmg_opts* o)
+{
+    if (o->max_iter < 0 || o->min_iter < 0 || o->n_sweeps_pre < 0 ||
+        o->n_sweeps_post < 0 || o->coarse_level < 0)
+        throw Error(BMG_ERR_ARG, "negative iteration/sweep/level option");
+    if (o->cycle < 0 || o->cycle > 2 || o->coarse_mode < 0 ||
+        o->coarse_mode > 1 || o->smoother < 0 || o->smoother > 1 ||
+        o->precision < 0 || o->precision > 1)
+        throw Error(BMG_ERR_ARG, "unknown enum value in options");
+}
+
+int bmg_create(bmg_ctx** out, const bmg_grid* g, const bmg_opts* opts)
+{
+    return run([&] {
+        if (!out || !g) throw Error(BMG_ERR_ARG, "null pointer");
+        *out = nullptr;
+        for (int d = 0; d < 3; d++)
+        {
+            if (g->n[d] < 1 || g->n[d] > (1 << 14) || !g->vertices[d])
+                throw Error(BMG_ERR_ARG, "invalid grid shape or vertices");
+            for (int q = 0; q < g->n[d]; q++)
+                if (!(g->vertices[d][q + 1] > g->vertices[d][q]))
+                    throw Error(BMG_ERR_ARG,
+                                "vertex coordinates must increase");
+        }
+        for (int f = 0; f < 6; f++)
+            if (g->bc[f] < 0 || g->bc[f] > 2)
+                throw Error(BMG_ERR_ARG, "unknown boundary kind");
+        for (int d = 0; d < 3; d++)
+            if ((g->bc[2*d] == BMG_PERIODIC) != (g->bc[2*d + 1] == BMG_PERIODIC))
+                throw Error(BMG_ERR_ARG,
+                            "periodic faces must come in opposite pairs");
+        std::unique_ptr<bmg_ctx> c(new bmg_ctx);
+        HIPCK(hipGetDevice(&c->device));
+        HIPCK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        HIPCK(hipEventCreate(&c->ev0));
+        HIPCK(hipEventCreate(&c->ev1));
+        for (int f = 0; f < 6; f++) { c->bc[f] = g->bc[f]; c->bcval[f] = g->bc_value[f]; }
+        bmg_opts o;
+        bmg_default_opts(&o);
+        if (opts) o = *opts;
+        validateOpts(&o);
+        c->opts = o;
+
+        // mesh::generateLevels (mesh.C:19-53), level::NandR (level.C:11-53)
+        int N[3] = {g->n[0], g->n[1], g->n[2]};
+        int R[3] = {0, 0, 0};
+        while (true)
+        {
+            Level q;
+            for (int d = 0; d < 3; d++)
+            {
+                q.L.N[d] = N[d];
+                q.L.P[d] = N[d] + 2;
+                q.L.R[d] = R[d];
+                q.ax[d].N = N[d];
+                q.ax[d].R = R[d];
+                std::vector<double> v(N[d] + 1);
+                if (c->lv.empty())
+                    for (int i = 0; i <= N[d]; i++) v[i] = g->vertices[d][i];
+                else
+                {
+                    const HostAxis& pa = c->lv.back().ax[d];
+                    for (int i = 0; i <= N[d]; i++) v[i] = pa.v[i*R[d] + 1];
+                }
+                buildAxis(q.ax[d], v, c->periodic(2*d), d);
+            }
+            for (int f = 0; f < 6; f++) q.L.bc[f] = c->bc[f];
+            q.L.Hz = (q.L.P[2] + 1)/2;
+            q.L.ncol = long(q.L.P[0])*q.L.P[1]*q.L.Hz;
+            c->lv.push_back(q);
+            int Nc[3];
+            bool coarsenable = false;
+            for (int d = 0; d < 3; d++)
+            {
+                // utils.H:1156-1161 rejects these under FULLDEBUG
+                if (N[d] > 3 && N[d] % 2)
+                    throw Error(BMG_ERR_ARG, "non-coarsenable grid: a level "
+                                             "has an odd size above 3");
+                Nc[d] = N[d] <= 3 ? N[d] : N[d]/2;
+                if (Nc[d] != N[d]) coarsenable = true;
+            }
+            if (!coarsenable) break;
+            for (int d = 0; d < 3; d++) { R[d] = N[d]/Nc[d]; N[d] = Nc[d]; }
+        }
+        if (c->opts.coarse_level > c->nLevels() - 2 && c->nLevels() > 1)
+            throw Error(BMG_ERR_ARG, "coarse level is set too high");
+
+        // device storage
+        size_t maxBlocks = 1024;
+        for (auto& q : c->lv)
+        {
+            for (int cc = 0; cc < 2; cc++)
+            {
+                q.x[cc] = devAlloc<double>(q.L.ncol);
+                q.b[cc] = devAlloc<double>(q.L.ncol);
+                q.r[cc] = devAlloc<double>(q.L.ncol);
+                HIPCK(hipMemset(q.x[cc], 0, q.L.ncol*sizeof(double)));
+                HIPCK(hipMemset(q.b[cc], 0, q.L.ncol*sizeof(double)));
+                HIPCK(hipMemset(q.r[cc], 0, q.L.ncol*sizeof(double)));
+            }
+            for (int d = 0; d < 3; d++)
+            {
+                q.dw[d] = devUpload(q.ax[d].w);
+                q.ddl[d] = devUpload(q.ax[d].dl);
+            }
+            const size_t nbx = size_t((q.L.P[2] + 2)/2 + 1 + 63)/64 + 1;
+            maxBlocks = std::max(maxBlocks,
+                                 nbx*size_t(q.L.P[1]/4 + 2)*size_t(q.L.P[0]));
+        }
+        // transfer weights (linearRestrictionScheme.C:15-80,
+        // linearProlongationScheme.C:12-110), per axis for rectilinear meshes
+        for (int l = 0; l + 1 < c->nLevels(); l++)
+        {
+            Level& f = c->lv[l];
+            Level& q = c->lv[l + 1];
+            for (int d = 0; d < 3; d++)
+            {
+                const HostAxis& fa = f.ax[d];
+                const HostAxis& ca = q.ax[d];
+                std::vector<double> ru(q.L.P[d], 0.0);
+                for (int I = 1; I <= q.L.N[d]; I++)
+                {
+                    const int f0 = (I - 1)*q.L.R[d] + 1;
+                    const double Px = ca.cc[I] - fa.cc[f0];
+                    const double Dx = fa.cc[f0 + 1] - fa.cc[f0];
+                    ru[I] = round12((Px*Dx)/(Dx*Dx));
+                }
+                q.ru[d] = devUpload(ru);
+                std::vector<double> pu(f.L.P[d], 0.0);
+                for (int I = 1; I <= f.L.N[d]; I++)
+                {
+                    const int i = I - 1;
+                    const int off = (i % 2) ? 1 : -1;
+                    const int base = i/q.L.R[d];
+                    const double x0 = ca.cc[base + 1];
+                    const double x1 = ca.cc[base + off + 1];
+                    const double Px = fa.cc[I] - x0;
+                    const double Dx = x1 - x0;
+                    pu[I] = round12((Px*Dx)/(Dx*Dx));
+                }
+                f.pu[d] = devUpload(pu);
+            }
+        }
+        c->npartials = maxBlocks;
+        c->partials = devAlloc<double>(maxBlocks);
+        c->dScalar = devAlloc<double>(1);
+        HIPCK(hipHostMalloc(&c->hScalar, sizeof(double)));
+        *out = c.release();
+    });
+}
+
+int bmg_destroy(bmg_ctx* c)
+{
+    return run([&] {
+        if (c)
+        {
+            (void)hipStreamSynchronize(c->stream);
+            delete c;
+        }
+    });
+}
+
+int bmg_set_opts(bmg_ctx* c, const bmg_opts* o)
+{
+    return run([&] {
+        if (!c || !o) throw Error(BMG_ERR_ARG, "null pointer");
+        validateOpts(o);
+        if (o->coarse_level > c->nLevels() - 2 && c->nLevels() > 1)
+            throw Error(BMG_ERR_ARG, "coarse level is set too high");
+        if (o->coarse_level != c->opts.coarse_level ||
+            o->coarse_mode != c->opts.coarse_mode)
+            c->prepared = false;
+        c->opts = *o;
+    });
+}
+
+int bmg_num_levels(const bmg_ctx* c) { return c ? c->nLevels() : BMG_ERR_ARG; }
+
+int bmg_level_shape(const bmg_ctx* c, int level, int n[3])
+{
+    return run([&] {
+        checkLevel(c, level);
+        for (int d = 0; d < 3; d++) n[d] = c->lv[level].L.N[d];
+    });
+}
+
+static void setStencil(bmg_ctx* c, int level, const double* A7, bool device,
+                       int symmetric)
+{
+    checkLevel(c, level);
+    if (!A7) throw Error(BMG_ERR_ARG, "null stencil");
+    Level& q = c->lv[level];
+    const size_t n = size_t(q.L.P[0])*q.L.P[1]*q.L.P[2]*7;
+    double* d = nullptr;
+    if (!device)
+    {
+        d = devAlloc<double>(n);
+        HIPCK(hipMemcpyAsync(d, A7, n*sizeof(double), hipMemcpyHostToDevice,
+                             c->stream));
+    }
+    for (int cc = 0; cc < 2; cc++)
+    {
+        devFree(q.wf[cc]);
+        devFree(q.wff[cc]);
+        if (!q.a7[cc]) q.a7[cc] = devAlloc<double>(q.L.ncol*7);
+        HIPCK(hipMemsetAsync(q.a7[cc], 0, q.L.ncol*7*sizeof(double),
+                             c->stream));
+    }
+    launch_stencil_ref_to_split(q.L, device ? A7 : d, pr(q.a7), c->stream);
+    HIPCK(hipGetLastError());
+    HIPCK(hipStreamSynchronize(c->stream));
+    if (d) devFree(d);
+    q.opkind = OP_STENCIL;
+    q.haveOp = true;
+    c->symmetric = symmetric != 0;
+    c->eliminated = false;
+    c->prepared = false;
+    c->mixedReady = false;
+}
+
+int bmg_set_operator_stencil(bmg_ctx* c, int level, const double* A7,
+                             int symmetric)
+{
+    return run([&] { setStencil(c, level, A7, false, symmetric); });
+}
+
+int bmg_set_operator_stencil_device(bmg_ctx* c, int level, const double* A7,
+                                    int symmetric)
+{
+    return run([&] { setStencil(c, level, A7, true, symmetric); });
+}
+
+// linearGaussLaplacianScheme::imLaplacian (linearGaussLaplacianScheme.C:
+// 22-102) on every level: face coefficients area*delta*lambda (lambda
+// restricted area-weighted to coarse levels) or the lambda-free operator.
+int bmg_set_operator_laplacian(bmg_ctx* c, const double* lx, const double* ly,
+                               const double* lz)
+{
+    return run([&] {
+        if (!c) throw Error(BMG_ERR_ARG, "null context");
+        const bool withLambda = lx != nullptr;
+        if (withLambda && (!ly || !lz))
+            throw Error(BMG_ERR_ARG, "need all three lambda arrays");
+        std::vector<double*> lam(3*c->nLevels(), nullptr);
+        if (withLambda)
+        {
+            const Level& q0 = c->lv[0];
+            const size_t n0 = size_t(q0.L.P[0])*q0.L.P[1]*q0.L.P[2];
+            const double* h[3] = {lx, ly, lz};
+            for (int fd = 0; fd < 3; fd++)
+            {
+                lam[fd] = devAlloc<double>(n0);
+                HIPCK(hipMemcpyAsync(lam[fd], h[fd], n0*sizeof(double),
+                                     hipMemcpyHostToDevice, c->stream));
+            }
+            for (int l = 0; l + 1 < c->nLevels(); l++)
+            {
+                const Level& f = c->lv[l];
+                const Level& q = c->lv[l + 1];
+                const size_t nc = size_t(q.L.P[0])*q.L.P[1]*q.L.P[2];
+                for (int fd = 0; fd < 3; fd++)
+                {
+                    lam[3*(l + 1) + fd] = devAlloc<double>(nc);
+                    HIPCK(hipMemsetAsync(lam[3*(l + 1) + fd], 0,
+                                         nc*sizeof(double), c->stream));
+                    launch_lambda_restrict(f.L, q.L, f.dw[0], f.dw[1], f.dw[2],
+                                           lam[3*l + fd], lam[3*(l + 1) + fd],
+                                           fd, c->stream);
+                }
+            }
+        }
+        for (int l = 0; l < c->nLevels(); l++)
+        {
+            Level& q = c->lv[l];
+            for (int cc = 0; cc < 2; cc++)
+            {
+                devFree(q.a7[cc]); devFree(q.a7f[cc]);
+                devFree(q.wf[cc]); devFree(q.wff[cc]);
+            }
+            if (withLambda)
+            {
+                for (int cc = 0; cc < 2; cc++)
+                {
+                    q.wf[cc] = devAlloc<double>(q.L.ncol*3);
+                    HIPCK(hipMemsetAsync(q.wf[cc], 0,
+                                         q.L.ncol*3*sizeof(double), c->stream));
+                }
+                for (int fd = 0; fd < 3; fd++)
+                    launch_face_coeffs(q.L, q.dw[0], q.dw[1], q.dw[2],
+                                       q.ddl[fd], lam[3*l + fd], fd,
+                                       pr(q.wf), c->stream);
+                q.opkind = OP_FACES;
+            }
+            else
+                q.opkind = OP_CONST;
+            if (q.cells() <= MATERIALIZE_CELLS)
+            {
+                for (int cc = 0; cc < 2; cc++)
+                {
+                    q.a7[cc] = devAlloc<double>(q.L.ncol*7);
+                    HIPCK(hipMemsetAsync(q.a7[cc], 0,
+                                         q.L.ncol*7*sizeof(double), c->stream));
+                }
+                launch_materialize_stencil(q.L, c->op<double>(l), pr(q.a7),
+                                           c->stream);
+                q.opkind = OP_STENCIL;
+                for (int cc = 0; cc < 2; cc++) devFree(q.wf[cc]);
+            }
+            q.haveOp = true;
+        }
+        HIPCK(hipGetLastError());
+        HIPCK(hipStreamSynchronize(c->stream));
+        for (double* p : lam) if (p) (void)hipFree(p);
+        c->symmetric = true;
+        c->eliminated = false;
+        c->prepared = false;
+        c->mixedReady = false;
+    });
+}
+
+int bmg_get_operator_stencil(bmg_ctx* c, int level, double* A7)
+{
+    return run([&] {
+        checkLevel(c, level);
+        if (!A7) throw Error(BMG_ERR_ARG, "null output");
+        Level& q = c->lv[level];
+        const size_t n = size_t(q.L.P[0])*q.L.P[1]*q.L.P[2]*7;
+        Pair<double> tmp{{nullptr, nullptr}};
+        bool own = false;
+        if (q.opkind != OP_STENCIL)
+        {
+            for (int cc = 0; cc < 2; cc++)
+            {
+                tmp.c[cc] = devAlloc<double>(q.L.ncol*7);
+                HIPCK(hipMemsetAsync(tmp.c[cc], 0, q.L.ncol*7*sizeof(double),
+                                     c->stream));
+            }
+            launch_materialize_stencil(q.L, c->op<double>(level), tmp,
+                                       c->stream);
+            own = true;
+            if (c->eliminated) launch_eliminate(q.L, tmp, c->stream);
+        }
+        else
+            tmp = pr(q.a7);
+        double* d = devAlloc<double>(n);
+        HIPCK(hipMemsetAsync(d, 0, n*sizeof(double), c->stream));
+        launch_stencil_split_to_ref(q.L, Pair<const double>{{tmp.c[0], tmp.c[1]}},
+                                    d, c->stream);
+        HIPCK(hipGetLastError());
+        HIPCK(hipMemcpyAsync(A7, d, n*sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
+        HIPCK(hipStreamSynchronize(c->stream));
+        devFree(d);
+        if (own) { devFree(tmp.c[0]); devFree(tmp.c[1]); }
+    });
+}
+
+static void solveImpl(bmg_ctx* c, double* b, double* x, bool device,
+                      int constMatrix, bmg_stats* st)
+{
+    if (!c || !b || !x) throw Error(BMG_ERR_ARG, "null pointer");
+    Level& q = c->lv[0];
+    const size_t n = size_t(q.L.P[0])*q.L.P[1]*q.L.P[2];
+    double* db = b;
+    double* dx = x;
+    if (!device)
+    {
+        db = devAlloc<double>(n);
+        dx = devAlloc<double>(n);
+        HIPCK(hipMemcpyAsync(db, b, n*sizeof(double), hipMemcpyHostToDevice,
+                             c->stream));
+        HIPCK(hipMemcpyAsync(dx, x, n*sizeof(double), hipMemcpyHostToDevice,
+                             c->stream));
+    }
+    bmg_stats local{};
+    std::exception_ptr err;
+    try
+    {
+        HIPCK(hipEventRecord(c->ev0, c->stream));
+        launch_ref_to_split<double>(q.L, db, pr(q.b), c->stream);
+        launch_ref_to_split<double>(q.L, dx, pr(q.x), c->stream);
+        c->runSolve(constMatrix != 0, &local);
+        launch_split_to_ref<double>(q.L, cpr(q.x), dx, c->stream);
+        launch_split_to_ref<double>(q.L, cpr(q.b), db, c->stream);
+        HIPCK(hipEventRecord(c->ev1, c->stream));
+        HIPCK(hipEventSynchronize(c->ev1));
+        c->harvestProfile();
+        float ms = 0.0f;
+        HIPCK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        local.solve_ms = ms;
+        if (!device)
+        {
+            HIPCK(hipMemcpyAsync(x, dx, n*sizeof(double),
+                                 hipMemcpyDeviceToHost, c->stream));
+            HIPCK(hipMemcpyAsync(b, db, n*sizeof(double),
+                                 hipMemcpyDeviceToHost, c->stream));
+            HIPCK(hipStreamSynchronize(c->stream));
+        }
+    }
+    catch (...)
+    {
+        err = std::current_exception();
+    }
+    if (!device)
+    {
+        (void)hipStreamSynchronize(c->stream);
+        devFree(db);
+        devFree(dx);
+    }
+    if (err)
+    {
+        if (st)
+        {
+            *st = local;
+            st->status = BMG_ERR_ARG;
+            try { std::rethrow_exception(err); }
+            catch (const Error& e) { st->status = e.code; }
+            catch (...) {}
+        }
+        std::rethrow_exception(err);
+    }
+    local.status = BMG_OK;
+    if (st) *st = local;
+}
+
+int bmg_solve(bmg_ctx* c, double* b, double* x, int constMatrix, bmg_stats* st)
+{
+    return run([&] { solveImpl(c, b, x, false, constMatrix, st); });
+}
+
+int bmg_solve_device(bmg_ctx* c, double* db, double* dx, int constMatrix,
+                     bmg_stats* st)
+{
+    return run([&] { solveImpl(c, db, dx, true, constMatrix, st); });
+}
+
+static double* fieldPtr(bmg_ctx* c, int which, int level, int colour)
+{
+    Level& q = c->lv[level];
+    switch (which)
+    {
+    case 0: return q.x[colour];
+    case 1: return q.b[colour];
+    case 2: return q.r[colour];
+    default: throw Error(BMG_ERR_ARG, "unknown field");
+    }
+}
+
+int bmg_upload_field(bmg_ctx* c, int which, int level, const double* v)
+{
+    return run([&] {
+        checkLevel(c, level);
+        if (!v) throw Error(BMG_ERR_ARG, "null pointer");
+        Level& q = c->lv[level];
+        const size_t n = size_t(q.L.P[0])*q.L.P[1]*q.L.P[2];
+        double* d = devAlloc<double>(n);
+        HIPCK(hipMemcpyAsync(d, v, n*sizeof(double), hipMemcpyHostToDevice,
+                             c->stream));
+        Pair<double> f{{fieldPtr(c, which, level, 0),
+                        fieldPtr(c, which, level, 1)}};
+        launch_ref_to_split<double>(q.L, d, f, c->stream);
+        HIPCK(hipGetLastError());
+        HIPCK(hipStreamSynchronize(c->stream));
+        devFree(d);
+    });
+}
+
+int bmg_download_field(bmg_ctx* c, int which, int level, double* v)
+{
+    return run([&] {
+        checkLevel(c, level);
+        if (!v) throw Error(BMG_ERR_ARG, "null pointer");
+        Level& q = c->lv[level];
+        const size_t n = size_t(q.L.P[0])*q.L.P[1]*q.L.P[2];
+        double* d = devAlloc<double>(n);
+        Pair<const double> f{{fieldPtr(c, which, level, 0),
+                              fieldPtr(c, which, level, 1)}};
+        launch_split_to_ref<double>(q.L, f, d, c->stream);
+        HIPCK(hipGetLastError());
+        HIPCK(hipMemcpyAsync(v, d, n*sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
+        HIPCK(hipStreamSynchronize(c->stream));
+        devFree(d);
+    });
+}
+
+int bmg_op_prepare(bmg_ctx* c)
+{
+    return run([&] {
+        if (!c) throw Error(BMG_ERR_ARG, "null context");
+        c->prepare(false);
+        HIPCK(hipStreamSynchronize(c->stream));
+    });
+}
+
+static void needEliminated(bmg_ctx* c)
+{
+    if (!c->eliminated)
+        throw Error(BMG_ERR_STATE, "call bmg_op_prepare first");
+}
+
+int bmg_op_smooth(bmg_ctx* c, int level, int sweeps)
+{
+    return run([&] {
+        checkLevel(c, level);
+        needEliminated(c);
+        c->smooth<double>(level, sweeps);
+        HIPCK(hipGetLastError());
+        HIPCK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int bmg_op_residual(bmg_ctx* c, int level, int corr)
+{
+    return run([&] {
+        checkLevel(c, level);
+        needEliminated(c);
+        if (corr)
+            c->residual<double>(level, true, false, nullptr);
+        else
+        {
+            Level& q = c->lv[level];
+            Region rg;
+            for (int d = 0; d < 3; d++) { rg.S[d] = 1; rg.E[d] = q.L.N[d] + 1; }
+            launch_residual(q.L, c->op<double>(level), cpr(q.x), cpr(q.b),
+                            pr(q.r), rg, nullptr, nullptr, c->stream);
+        }
+        HIPCK(hipGetLastError());
+        HIPCK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int bmg_op_restrict(bmg_ctx* c, int level)
+{
+    return run([&] {
+        checkLevel(c, level);
+        if (level + 1 >= c->nLevels()) throw Error(BMG_ERR_ARG, "no coarser level");
+        c->restrictTo<double>(level);
+        HIPCK(hipGetLastError());
+        HIPCK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int bmg_op_prolong(bmg_ctx* c, int level)
+{
+    return run([&] {
+        checkLevel(c, level);
+        if (level + 1 >= c->nLevels()) throw Error(BMG_ERR_ARG, "no coarser level");
+        if (!c->lv[level + 1].haveOp)
+            throw Error(BMG_ERR_STATE, "operator not set");
+        c->prolongFrom<double>(level);
+        HIPCK(hipGetLastError());
+        HIPCK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int bmg_op_correct_bcs(bmg_ctx* c, int level)
+{
+    return run([&] {
+        checkLevel(c, level);
+        launch_bc_fill(c->lv[level].L, c->op<double>(level),
+                       pr(c->lv[level].x), 1, level == 0, c->bcval, c->stream);
+        HIPCK(hipGetLastError());
+        HIPCK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int bmg_op_coarse_solve(bmg_ctx* c)
+{
+    return run([&] {
+        if (!c) throw Error(BMG_ERR_ARG, "null context");
+        needEliminated(c);
+        c->coarseSolve<double>();
+        HIPCK(hipGetLastError());
+        HIPCK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int bmg_op_cycle(bmg_ctx* c)
+{
+    return run([&] {
+        if (!c) throw Error(BMG_ERR_ARG, "null context");
+        needEliminated(c);
+        std::vector<int> visits(c->nLevels(), 0);
+        c->cycle<double>(visits, 0);
+        HIPCK(hipGetLastError());
+        HIPCK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int bmg_op_sum_sq_residual(bmg_ctx* c, double* out)
+{
+    return run([&] {
+        if (!c || !out) throw Error(BMG_ERR_ARG, "null pointer");
+        int nb = 0;
+        launch_sum_sq<double>(c->lv[0].L, cpr(c->lv[0].r), c->partials, &nb,
+                              c->stream);
+        *out = c->reduceFinal(nb);
+    });
+}
+
+int bmg_profile_smoother(bmg_ctx* c, int enable, int level)
+{
+    return run([&] {
+        checkLevel(c, level);
+        HIPCK(hipStreamSynchronize(c->stream));
+        c->harvestProfile();
+        c->profOn = enable != 0;
+        c->profLevel = level;
+        c->profMs = 0.0;
+        c->profBytes = 0.0;
+        c->profLaunches = 0;
+    });
+}
+
+int bmg_profile_read(bmg_ctx* c, double* total_ms, long long* launches,
+                     double* bytes)
+{
+    return run([&] {
+        if (!c) throw Error(BMG_ERR_ARG, "null context");
+        HIPCK(hipStreamSynchronize(c->stream));
+        c->harvestProfile();
+        if (total_ms) *total_ms = c->profMs;
+        if (launches) *launches = c->profLaunches;
+        if (bytes) *bytes = c->profBytes;
+    });
+}
+
+}  // extern "C"

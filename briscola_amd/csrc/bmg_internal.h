@@ -1,0 +1,144 @@
+// Internal declarations shared by the HIP kernels and the host solver.
+//
+// Device layout ("colour-split"): every level field is stored as two arrays,
+// one per red-black colour c = (I+J+K) & 1 of the padded cell index.  Cell
+// (I,J,K) lives at position ((I*P1 + J)*Hz + (K>>1)) of its colour array,
+// Hz = (P2+1)/2.  A red-black phase then streams only the cells of its
+// colour, and the six neighbours of a cell are the same position in the
+// rows (I+-1,J) and (I,J+-1) of the other colour plus positions
+// p-1+q / p+q (q = K&1) of its own row there.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace bmg {
+
+enum { NEU = 0, DIR = 1, PER = 2 };
+
+// Operator representations (see DESIGN.md of the repository):
+//   OP_STENCIL: the reference AoS 7-point stencil, split by colour and stored
+//               component-major (7 arrays per colour);
+//   OP_CONST  : lambda-free rectilinear Laplacian, coefficients rebuilt from
+//               1-D width/delta arrays on the fly (no per-cell bytes);
+//   OP_FACES  : three lower-face coefficients per cell (area*delta*lambda).
+enum { OP_STENCIL = 0, OP_CONST = 1, OP_FACES = 2 };
+
+struct LevelDims {
+    int N[3];
+    int P[3];
+    int Hz;
+    long ncol;     // elements per colour array = P0*P1*Hz
+    int bc[6];
+    int R[3];      // refinement ratio of this level vs its parent
+};
+
+template <typename T>
+struct OpView {
+    int kind;
+    const T* a7[2];   // OP_STENCIL: 7*ncol per colour, component-major
+    const T* wf[2];   // OP_FACES: 3*ncol per colour, component-major
+    const T* w[3];    // cell widths per padded index
+    const T* dl[3];   // 1/|dcc| of the lower face of each padded index
+};
+
+template <typename T>
+struct Pair {
+    T* c[2];
+};
+
+// Restriction / prolongation per-axis data (in T).
+template <typename T>
+struct TransferView {
+    const T* ru[3];   // coarse padded index -> u of the restriction weights
+    const T* pu[3];   // fine padded index   -> u of the prolongation weights
+    const T* fw[3];   // fine widths
+    const T* cw[3];   // coarse widths
+};
+
+struct Region {
+    int S[3];
+    int E[3];
+};
+
+// ---- launchers (bmg_kernels.hip) ----
+template <typename T>
+void launch_rbgs(const LevelDims& L, const OpView<T>& op, Pair<T> x,
+                 Pair<const T> b, int colour, const Region& rg,
+                 hipStream_t s);
+template <typename T>
+void launch_jacobi(const LevelDims& L, const OpView<T>& op, Pair<T> x,
+                   Pair<const T> b, Pair<T> tmp, const Region& rg,
+                   hipStream_t s);
+template <typename T>
+void launch_residual(const LevelDims& L, const OpView<T>& op, Pair<const T> x,
+                     Pair<const T> b, Pair<T> r, const Region& rg,
+                     double* partials, int* nblocks, hipStream_t s);
+template <typename T>
+void launch_restrict(const LevelDims& Lf, const LevelDims& Lc,
+                     const TransferView<T>& tv, Pair<const T> rf, Pair<T> bc,
+                     hipStream_t s);
+template <typename T>
+void launch_prolong(const LevelDims& Lf, const LevelDims& Lc,
+                    const TransferView<T>& tv, Pair<const T> xc, Pair<T> xf,
+                    hipStream_t s);
+// mode 0: periodic faces only (evaluate<nonEliminatedBcs,1>);
+// mode 1: full correctBoundaryConditions; mode 2: Neumann/Dirichlet faces
+// only (correct<eliminatedBcs>); level0 enables the inhomogeneous terms.
+template <typename T>
+void launch_bc_fill(const LevelDims& L, const OpView<T>& op, Pair<T> x,
+                    int mode, int level0, const double* bcval, hipStream_t s);
+template <typename T>
+void launch_fill(Pair<T> x, long ncol, T v, hipStream_t s);
+template <typename T>
+void launch_ref_to_split(const LevelDims& L, const double* ref, Pair<T> out,
+                         hipStream_t s);
+template <typename T>
+void launch_split_to_ref(const LevelDims& L, Pair<const T> in, double* ref,
+                         hipStream_t s);
+void launch_stencil_ref_to_split(const LevelDims& L, const double* ref7,
+                                 Pair<double> a7, hipStream_t s);
+void launch_stencil_split_to_ref(const LevelDims& L, Pair<const double> a7,
+                                 double* ref7, hipStream_t s);
+template <typename T>
+void launch_materialize_stencil(const LevelDims& L, const OpView<T>& op,
+                                Pair<T> a7, hipStream_t s);
+void launch_pin(const LevelDims& L, Pair<double> a7, hipStream_t s);
+void launch_eliminate(const LevelDims& L, Pair<double> a7, hipStream_t s);
+// b[0] -= inhomogeneous boundary terms of eliminateGhosts (level 0 only)
+void launch_bc_source(const LevelDims& L, const OpView<double>& op,
+                      Pair<double> b, const double* bcval, hipStream_t s);
+void launch_lambda_restrict(const LevelDims& Lf, const LevelDims& Lc,
+                            const double* fw0, const double* fw1,
+                            const double* fw2, const double* lamf,
+                            double* lamc, int fd, hipStream_t s);
+void launch_face_coeffs(const LevelDims& L, const double* w0,
+                        const double* w1, const double* w2, const double* dl,
+                        const double* lam, int fd, Pair<double> wf,
+                        hipStream_t s);
+template <typename T>
+void launch_coarse_solve(const LevelDims& L, const OpView<T>& op, Pair<T> x,
+                         Pair<const T> b, const double* lu, const int* piv,
+                         int n, int level0, const double* bcval,
+                         hipStream_t s);
+template <typename T>
+void launch_sum_sq(const LevelDims& L, Pair<const T> f, double* partials,
+                   int* nblocks, hipStream_t s);
+template <typename T>
+void launch_sum(const LevelDims& L, Pair<const T> f, double* partials,
+                int* nblocks, hipStream_t s);
+template <typename T>
+void launch_norm_factor_terms(const LevelDims& L, const OpView<T>& op,
+                              Pair<const T> r, Pair<const T> b, double y,
+                              double* partials, int* nblocks, hipStream_t s);
+void launch_reduce_final(const double* partials, int n, double* out,
+                         hipStream_t s);
+// x64 += e32 (interior), r32 = (float) r64, etc.
+void launch_add_correction(const LevelDims& L, Pair<double> x,
+                           Pair<const float> e, hipStream_t s);
+void launch_to_float(long ncol, Pair<const double> in, Pair<float> out,
+                     hipStream_t s);
+template <typename T>
+void launch_set_cell(Pair<T> f, long idx, int colour, T v, hipStream_t s);
+
+}  // namespace bmg

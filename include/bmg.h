@@ -1,0 +1,188 @@
+/*
+ * bmg.h -- C ABI of the MI355X geometric-multigrid Poisson solver.
+ *
+ * Drop-in for Briscola's MG linear solver:
+ *   template<class SType,class Type,class MeshType> class MG
+ *       : public solver<SType,Type,MeshType>
+ *   virtual void solve(linearSystem<SType,Type,MeshType>& sys,
+ *                      const bool constMatrix = false);
+ * (src/briscolaFiniteVolume/solvers/MG/MG.H:135-139, MG.C:402-483; the
+ * abstract interface is solvers/solver/solver.H:280-284).  The hot
+ * instantiation is <stencil, scalar, colocated>.
+ *
+ * Layout at the boundary is the reference's own (no conversion needed by the
+ * caller):
+ *   - a level field is a ghost-padded block of (Nx+2)(Ny+2)(Nz+2) doubles,
+ *     lin(i,j,k) = (i*(Ny+2)+j)*(Nz+2)+k, k (z) contiguous
+ *     (src/briscolaCore/include/utils.H:559-570, block.H:37-393);
+ *   - a stencil field stores 7 doubles [C,L,R,B,T,A,F] per padded cell
+ *     (src/briscolaCore/primitives/StencilSpace/stencil/stencil.H:21).
+ *
+ * Conventions: every entry point returns BMG_OK (0) or a negative status;
+ * bmg_last_error() gives the message of the last failure on the calling
+ * thread.  No exception crosses the ABI.  A context owns its device memory
+ * and one HIP stream; distinct contexts may be used from different threads,
+ * one context may not.
+ */
+#ifndef BMG_H
+#define BMG_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bmg_ctx bmg_ctx;
+
+enum bmg_status {
+    BMG_OK = 0,
+    BMG_ERR_ARG = -1,        /* invalid argument / shape */
+    BMG_ERR_HIP = -2,        /* HIP runtime error */
+    BMG_ERR_STATE = -3,      /* call out of order (e.g. no operator) */
+    BMG_ERR_SINGULAR = -4,   /* coarse matrix could not be factorised */
+    BMG_ERR_NUMERIC = -5     /* NaN/Inf residual norm */
+};
+
+/* Boundary kinds per face, order -x,+x,-y,+y,-z,+z (faceOffsets,
+ * src/briscolaCore/include/utils.H:191-241). */
+enum bmg_bc { BMG_NEUMANN = 0, BMG_DIRICHLET = 1, BMG_PERIODIC = 2 };
+/* MGCycleType, MG.H:18-23 */
+enum bmg_cycle { BMG_VCYCLE = 0, BMG_WCYCLE = 1, BMG_FCYCLE = 2 };
+/* MGCoarseMode, MG.H:26-30 */
+enum bmg_coarse_mode { BMG_COARSE_SMOOTH = 0, BMG_COARSE_DIRECT = 1 };
+/* smoother TypeNames rbgs / jac (rbgsSmoother.H:56, jacSmoother.H) */
+enum bmg_smoother { BMG_RBGS = 0, BMG_JAC = 1 };
+/* BMG_FP64: every pass in FP64 (the reference's arithmetic).
+ * BMG_MIXED: FP32 cycles on the defect equation inside FP64 defect
+ * correction; FP64 residual, update and coarsest direct solve. */
+enum bmg_precision { BMG_FP64 = 0, BMG_MIXED = 1 };
+
+/* Mesh + boundary conditions: what MG reads from fvMesh and the BCs of
+ * sys.x() (mesh.C:19-53 hierarchy, fvMeshMetrics.C metrics,
+ * Neumann/Dirichlet/periodicBoundaryCondition.C). */
+typedef struct bmg_grid {
+    int n[3];                   /* level-0 cells per axis (i,j,k) = (x,y,z) */
+    const double* vertices[3];  /* n[d]+1 vertex coordinates per axis       */
+    int bc[6];                  /* enum bmg_bc per face                     */
+    double bc_value[6];         /* Neumann gradient / Dirichlet value,
+                                   uniform per face, level 0 only           */
+} bmg_grid;
+
+/* The MG solver dictionary (MG.C:257-304, solver.C:114-123). */
+typedef struct bmg_opts {
+    double rel_tol;         /* relTol (required)            */
+    double tolerance;       /* tolerance (required)         */
+    int min_iter;           /* minIter, default 0           */
+    int max_iter;           /* maxIter, default 999         */
+    int n_sweeps_pre;       /* nSweepsPre, default 0        */
+    int n_sweeps_post;      /* nSweepsPost, default 2       */
+    int cycle;              /* cycleType, default F         */
+    int coarse_mode;        /* coarseMode, default direct   */
+    int coarse_level;       /* coarseLevel, default 0       */
+    int smoother;           /* smoother, default rbgs       */
+    int precision;          /* BMG_FP64 (default) / BMG_MIXED */
+    int reserved[5];
+} bmg_opts;
+
+typedef struct bmg_stats {
+    int iters;              /* nIter of printSolverStats (solver.C:97-112) */
+    int status;             /* BMG_OK or the failing status               */
+    double initial_residual;
+    double final_residual;
+    double solve_ms;        /* device time of the whole solve             */
+    double reserved[4];
+} bmg_stats;
+
+/* Fill opts with the reference defaults (rel_tol/tolerance set to 0). */
+void bmg_default_opts(bmg_opts* opts);
+
+const char* bmg_last_error(void);
+int bmg_device_count(int* count);
+
+/* Builds the level hierarchy (mesh::generateLevels, mesh.C:19-53; coarsen,
+ * utils.H:1154-1171), level metrics and transfer weights
+ * (linearRestrictionScheme.C:15-80, linearProlongationScheme.C:12-110) and
+ * allocates device storage on the current HIP device. */
+int bmg_create(bmg_ctx** ctx, const bmg_grid* grid, const bmg_opts* opts);
+int bmg_destroy(bmg_ctx* ctx);
+int bmg_set_opts(bmg_ctx* ctx, const bmg_opts* opts);
+
+int bmg_num_levels(const bmg_ctx* ctx);
+int bmg_level_shape(const bmg_ctx* ctx, int level, int n[3]);
+
+/* Operator input, drop-in form: sys.A()[level] as assembled (before ghost
+ * elimination), reference stencil layout, host memory.  Every level must be
+ * set before a solve.  symmetric: linearSystem::symmetric_ (decides
+ * singular(), linearSystem.H:227-230). */
+int bmg_set_operator_stencil(bmg_ctx* ctx, int level, const double* A7,
+                             int symmetric);
+/* Same, device memory. */
+int bmg_set_operator_stencil_device(bmg_ctx* ctx, int level,
+                                    const double* d_A7, int symmetric);
+
+/* Operator input, native form: the solver assembles
+ * laplacian(lambda, x) = div(lambda grad x) itself on every level
+ * (linearGaussLaplacianScheme.C:22-102).  lambda: NULL for the lambda-free
+ * operator, else three host arrays of level-0 lower-face coefficients in
+ * padded cell layout, restricted to coarse levels area-weighted
+ * (areaWeightedFaceRestrictionScheme.C:19-56). */
+int bmg_set_operator_laplacian(bmg_ctx* ctx, const double* lambda_x,
+                               const double* lambda_y,
+                               const double* lambda_z);
+
+/* Read back sys.A()[level] after the solver's ghost elimination and
+ * singularity pin (the reference mutates the caller's A, MG.C:433-447). */
+int bmg_get_operator_stencil(bmg_ctx* ctx, int level, double* A7);
+
+/* MG::solve: b (level-0 source, in/out: inhomogeneous boundary terms are
+ * folded in as eliminateGhosts does) and x (initial guess in, solution out,
+ * eliminated ghosts refreshed on exit, MG.C:475).  Host pointers, reference
+ * layout.  const_matrix: reuse the coarse factorisation (MG.C:449-459). */
+int bmg_solve(bmg_ctx* ctx, double* b, double* x, int const_matrix,
+              bmg_stats* stats);
+/* Same with device pointers on the context's device; runs on the context's
+ * stream and returns after the solve completed. */
+int bmg_solve_device(bmg_ctx* ctx, double* d_b, double* d_x,
+                     int const_matrix, bmg_stats* stats);
+
+/* ---- Level operations for kernel-level parity tests.  Fields are host
+ * arrays in reference layout; 'which' is 0 = x, 1 = b, 2 = r. ---- */
+int bmg_upload_field(bmg_ctx* ctx, int which, int level, const double* v);
+int bmg_download_field(bmg_ctx* ctx, int which, int level, double* v);
+/* MG.C:431-447 setSingularityConstraint + eliminateGhosts (+ coarse LU). */
+int bmg_op_prepare(bmg_ctx* ctx);
+/* rbgsSmoother::Smooth / jacSmoother::Smooth (rbgsSmoother.C:376-437). */
+int bmg_op_smooth(bmg_ctx* ctx, int level, int sweeps);
+/* linearSystem::residual (linearSystemResidual.C:280-332). */
+int bmg_op_residual(bmg_ctx* ctx, int level, int corr_bcs);
+/* b[level+1] = scaled linear restriction of r[level]
+ * (linearRestrictionScheme.C:103-160). */
+int bmg_op_restrict(bmg_ctx* ctx, int level);
+/* x[level] += linear prolongation of x[level+1]
+ * (linearProlongationScheme.C:112-147). */
+int bmg_op_prolong(bmg_ctx* ctx, int level);
+/* meshLevel::correctBoundaryConditions on x[level] (meshLevel.C:231-254). */
+int bmg_op_correct_bcs(bmg_ctx* ctx, int level);
+/* APLU::solve on the coarse level (APLU.C:91-128). */
+int bmg_op_coarse_solve(bmg_ctx* ctx);
+/* One MG::cycle from level 0 (MG.C:19-129). */
+int bmg_op_cycle(bmg_ctx* ctx);
+/* sum of r[0]^2 over interior cells. */
+int bmg_op_sum_sq_residual(bmg_ctx* ctx, double* out);
+
+/* ---- Measurement hook (no counterpart in the reference): while enabled,
+ * every launch of the red-black smoother kernel on 'level' is bracketed by
+ * HIP events on the context's stream.  bmg_profile_read returns the summed
+ * device time of those launches, their number, and their algorithmic HBM
+ * bytes (cells updated x bytes per update of the operator representation:
+ * b, the other colour of x and the write of x, plus the coefficients). ---- */
+int bmg_profile_smoother(bmg_ctx* ctx, int enable, int level);
+int bmg_profile_read(bmg_ctx* ctx, double* total_ms, long long* launches,
+                     double* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

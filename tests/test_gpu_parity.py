@@ -1,0 +1,169 @@
+"""GPU parity: every HIP kernel and the whole solve against the CPU oracle.
+
+The FP64 path keeps the reference's operation order (built with
+-ffp-contract=off on both sides), so results are compared bit for bit
+(``np.array_equal``; +0.0 and -0.0 compare equal).  Cases cover the three
+boundary families of the workloads (all-Neumann, periodic+Neumann on a
+stretched mesh, variable coefficients), both operator inputs (the reference
+stencil layout and the native Laplacian assembly) and a Dirichlet/mixed case.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from briscola_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+OPTS = dict(rel_tol=1e-10, tolerance=0.0, n_sweeps_pre=2, n_sweeps_post=2,
+            cycle=oracle.VCYCLE, coarse_mode=oracle.COARSE_DIRECT,
+            coarse_level=1)
+
+
+def mixed_problem(n=16, seed=0):
+    """Dirichlet on x faces, periodic y, Neumann z, uniform mesh."""
+    verts = [np.arange(q + 1) / q for q in (n, n, n)]
+    rng = np.random.default_rng(seed)
+    b = np.zeros((n + 2,) * 3)
+    b[1:-1, 1:-1, 1:-1] = rng.uniform(-1, 1, (n, n, n)) / n ** 3
+    return W.Problem("mixed", verts,
+                     (W.DIRICHLET, W.DIRICHLET, W.PERIODIC, W.PERIODIC,
+                      W.NEUMANN, W.NEUMANN), b)
+
+
+CASES = {
+    "neumann16": lambda: W.uniform_neumann(16, seed=1),
+    "channel16": lambda: W.channel(16, seed=1),
+    "spheres16": lambda: W.spheres(16, seed=1, nspheres=6),
+    "mixed16": lambda: mixed_problem(16, seed=1),
+    "aniso": lambda: W.uniform_neumann((16, 8, 32), seed=2),
+    # level 0 above 32^3 cells: matrix-free / face-coefficient kernels
+    "neumann48": lambda: W.uniform_neumann(48, seed=5),
+    "channel48": lambda: W.channel(48, seed=5),
+    "spheres48": lambda: W.spheres(48, seed=5, nspheres=8),
+}
+
+
+def build_pair(gpu, prob, native, **opts):
+    o = oracle.Oracle(prob.vertices, prob.bc, prob.bc_value)
+    o.set_opts(**opts)
+    o.assemble(prob.lam)
+    ctx = gpu.Context(prob.vertices, prob.bc, prob.bc_value, **opts)
+    assert ctx.shapes == o.shapes
+    if native:
+        ctx.set_laplacian(prob.lam)
+    else:
+        for l in range(o.nlevels):
+            ctx.set_stencil(l, o.stencil(l))
+    return o, ctx
+
+
+def inner(a):
+    return a[1:-1, 1:-1, 1:-1]
+
+
+def assert_same(a, b, what):
+    if not np.array_equal(a, b):
+        d = np.abs(a - b)
+        raise AssertionError(f"{what}: max |diff| {d.max():.3e} at "
+                             f"{np.unravel_index(np.argmax(d), d.shape)}")
+
+
+@pytest.mark.parametrize("native", [False, True], ids=["stencil", "native"])
+@pytest.mark.parametrize("case", list(CASES))
+def test_level_ops_bitwise(gpu, case, native):
+    prob = CASES[case]()
+    print(f"[level ops] case={case} native={native}", flush=True)
+    o, ctx = build_pair(gpu, prob, native, **OPTS)
+    o.pin_and_eliminate()
+    o.prepare_coarse()
+    ctx.prepare()
+    rng = np.random.default_rng(7)
+    for l in range(o.nlevels):
+        A = o.stencil(l)
+        assert_same(inner(ctx.stencil(l)), inner(A), f"eliminated stencil l={l}")
+    for l in range(o.nlevels - 1):
+        shp = o.padded(l)
+        x = rng.uniform(-1, 1, shp)
+        b = rng.uniform(-1, 1, shp)
+        for which, v in ((oracle.X, x), (oracle.B, b)):
+            o.set_field(which, l, v)
+            ctx.upload(which, l, v)
+        o.smooth(l, 2)
+        ctx.smooth(l, 2)
+        assert_same(ctx.download(0, l), o.field(oracle.X, l), f"smooth l={l}")
+        o.residual(l, True)
+        ctx.residual(l, True)
+        assert_same(inner(ctx.download(2, l)), inner(o.field(oracle.R, l)),
+                    f"residual l={l}")
+        o.restrict(l)
+        ctx.restrict(l)
+        assert_same(inner(ctx.download(1, l + 1)), inner(o.field(oracle.B, l + 1)),
+                    f"restrict l={l}")
+        xc = rng.uniform(-1, 1, o.padded(l + 1))
+        o.set_field(oracle.X, l + 1, xc)
+        ctx.upload(0, l + 1, xc)
+        o.prolong(l)
+        ctx.prolong(l)
+        assert_same(ctx.download(0, l + 1), o.field(oracle.X, l + 1),
+                    f"coarse ghosts l={l + 1}")
+        assert_same(ctx.download(0, l), o.field(oracle.X, l), f"prolong l={l}")
+    lc = o.nlevels - 1 - OPTS["coarse_level"]
+    bc_ = rng.uniform(-1, 1, o.padded(lc))
+    xc_ = np.zeros(o.padded(lc))
+    for which, v in ((oracle.X, xc_), (oracle.B, bc_)):
+        o.set_field(which, lc, v)
+        ctx.upload(which, lc, v)
+    o.coarse_solve()
+    ctx.coarse_solve()
+    assert_same(ctx.download(0, lc), o.field(oracle.X, lc), "coarse solve")
+
+
+@pytest.mark.parametrize("native", [False, True], ids=["stencil", "native"])
+@pytest.mark.parametrize("case", ["neumann16", "channel16", "spheres16",
+                                  "mixed16", "aniso", "neumann48",
+                                  "channel48", "spheres48"])
+def test_solve_bitwise(gpu, case, native):
+    prob = CASES[case]()
+    print(f"[solve] case={case} native={native}", flush=True)
+    opts = dict(OPTS, rel_tol=1e-8, max_iter=400)
+    o, ctx = build_pair(gpu, prob, native, **opts)
+    o.set_field(oracle.B, 0, prob.b)
+    it, r0, r1 = o.solve()
+    b = prob.b.copy()
+    x, st = ctx.solve(b)
+    assert st.iters == it
+    assert abs(st.initial_residual - r0) <= 1e-12 * abs(r0)
+    assert abs(st.final_residual - r1) <= 1e-10 * abs(r1)
+    assert_same(x, o.field(oracle.X, 0), "solution")
+    assert_same(b, o.field(oracle.B, 0), "b after the solve")
+
+
+@pytest.mark.parametrize("cycle", [oracle.VCYCLE, oracle.WCYCLE, oracle.FCYCLE],
+                         ids=["V", "W", "F"])
+@pytest.mark.parametrize("smoother", [oracle.RBGS, oracle.JAC], ids=["rbgs", "jac"])
+def test_cycle_variants(gpu, cycle, smoother):
+    prob = W.channel(16, seed=3)
+    print(f"[cycles] cycle={cycle} smoother={smoother}", flush=True)
+    opts = dict(rel_tol=1e-6, tolerance=0.0, max_iter=60, n_sweeps_pre=1,
+                n_sweeps_post=2, cycle=cycle, coarse_mode=oracle.COARSE_DIRECT,
+                coarse_level=0, smoother=smoother)
+    o, ctx = build_pair(gpu, prob, True, **opts)
+    o.set_field(oracle.B, 0, prob.b)
+    it, r0, r1 = o.solve()
+    x, st = ctx.solve(prob.b.copy())
+    assert st.iters == it
+    assert_same(x, o.field(oracle.X, 0), "solution")
+
+
+def test_coarse_smooth_mode(gpu):
+    prob = W.uniform_neumann(16, seed=4)
+    opts = dict(rel_tol=1e-8, tolerance=0.0, max_iter=100, n_sweeps_pre=0,
+                n_sweeps_post=2, cycle=oracle.FCYCLE,
+                coarse_mode=oracle.COARSE_SMOOTH, coarse_level=0)
+    o, ctx = build_pair(gpu, prob, True, **opts)
+    o.set_field(oracle.B, 0, prob.b)
+    it, r0, r1 = o.solve()
+    x, st = ctx.solve(prob.b.copy())
+    assert st.iters == it
+    assert_same(x, o.field(oracle.X, 0), "solution")
