@@ -28,8 +28,8 @@ EXPORTS = [
     "bmg_solve_device", "bmg_upload_field", "bmg_download_field",
     "bmg_op_prepare", "bmg_op_smooth", "bmg_op_residual", "bmg_op_restrict",
     "bmg_op_prolong", "bmg_op_correct_bcs", "bmg_op_coarse_solve",
-    "bmg_op_cycle", "bmg_op_sum_sq_residual", "bmg_profile_smoother",
-    "bmg_profile_read",
+    "bmg_op_cycle", "bmg_op_sum_sq_residual", "bmg_time_smoother",
+    "bmg_set_graphs", "bmg_solve_diagonal",
 ]
 
 
@@ -104,8 +104,9 @@ def load():
         "bmg_op_coarse_solve": (I, [P]),
         "bmg_op_cycle": (I, [P]),
         "bmg_op_sum_sq_residual": (I, [P, DP]),
-        "bmg_profile_smoother": (I, [P, I, I]),
-        "bmg_profile_read": (I, [P, DP, ctypes.POINTER(ctypes.c_longlong), DP]),
+        "bmg_time_smoother": (I, [P, I, I, I, DP, DP]),
+        "bmg_set_graphs": (I, [P, I]),
+        "bmg_solve_diagonal": (I, [P, DP, DP, DP]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -213,6 +214,12 @@ class Context:
                                   ctypes.byref(st)))
         return x, st
 
+    def solve_diagonal(self, A0, b, x):
+        """MG::solve of a diagonal system (level-0 stencil A0)."""
+        A0 = np.ascontiguousarray(A0, dtype=np.float64)
+        _check(self.lib.bmg_solve_diagonal(self.ptr, _dp(A0), _dp(b), _dp(x)))
+        return x
+
     def solve_device(self, b_ptr, x_ptr, const_matrix=False):
         """MG::solve on device pointers (e.g. torch tensor .data_ptr())."""
         st = Stats()
@@ -257,16 +264,17 @@ class Context:
     def cycle(self):
         _check(self.lib.bmg_op_cycle(self.ptr))
 
-    def profile_smoother(self, enable=True, level=0):
-        _check(self.lib.bmg_profile_smoother(self.ptr, int(enable), level))
-
-    def profile_read(self):
+    def time_smoother(self, level=0, precision=FP64, reps=50):
+        """Average duration (ms) and algorithmic bytes of one red-black phase
+        launch on ``level`` (HIP events on the solver's stream)."""
         ms = ctypes.c_double(0.0)
-        n = ctypes.c_longlong(0)
         by = ctypes.c_double(0.0)
-        _check(self.lib.bmg_profile_read(self.ptr, ctypes.byref(ms), ctypes.byref(n),
-                                         ctypes.byref(by)))
-        return ms.value, n.value, by.value
+        _check(self.lib.bmg_time_smoother(self.ptr, level, precision, reps,
+                                          ctypes.byref(ms), ctypes.byref(by)))
+        return ms.value, by.value
+
+    def set_graphs(self, enable):
+        _check(self.lib.bmg_set_graphs(self.ptr, int(enable)))
 
     def sum_sq_residual(self):
         out = np.zeros(1)

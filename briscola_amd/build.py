@@ -21,6 +21,9 @@ FLAGS = [
 ]
 
 
+CPP_TEST = os.path.join(ROOT, "tests", "cpp", "test_hipmg")
+
+
 def build(verbose=False):
     cmd = (["hipcc"] + FLAGS + ["-I" + os.path.join(ROOT, "include"),
                                 "-o", LIB + ".tmp"]
@@ -31,6 +34,13 @@ def build(verbose=False):
     os.replace(LIB + ".tmp", LIB)
     if verbose:
         print(" ".join(cmd))
+    # C++ program driving the adapter of host/hipMG.H (a GPU test runs it)
+    cmd = ["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"),
+           "-I" + os.path.join(HERE, "host"), CPP_TEST + ".cpp", "-o", CPP_TEST,
+           "-L" + HERE, "-lbmg", "-Wl,-rpath,$ORIGIN/../../briscola_amd"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("g++ failed:\n" + r.stderr)
     return LIB
 
 

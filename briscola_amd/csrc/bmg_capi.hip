@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <exception>
 #include <functional>
@@ -184,14 +185,13 @@ struct bmg_ctx
     double* hScalar = nullptr;   // pinned
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool mixedReady = false;
-    // smoother profiling (bmg_profile_smoother)
-    bool profOn = false;
-    int profLevel = 0;
-    std::vector<hipEvent_t> profEv;     // start/stop pairs
-    int profPending = 0;
-    std::vector<double> profBytesPending;
-    double profMs = 0.0, profBytes = 0.0;
-    long long profLaunches = 0;
+    // one MG iteration captured as a HIP graph and replayed
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graphExec = nullptr;
+    int graphMixed = -1;
+    long gen = 0;          // bumped whenever device buffers are (re)allocated
+    long graphGen = -1;
+    bool useGraphs = true;
 
     int nLevels() const { return int(lv.size()); }
     int coarseLevel() const { return nLevels() - opts.coarse_level - 1; }
@@ -220,7 +220,8 @@ struct bmg_ctx
         if (hScalar) (void)hipHostFree(hScalar);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
-        for (hipEvent_t e : profEv) (void)hipEventDestroy(e);
+        if (graphExec) (void)hipGraphExecDestroy(graphExec);
+        if (graph) (void)hipGraphDestroy(graph);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -248,61 +249,7 @@ struct bmg_ctx
         HIPCK(hipMemcpyAsync(hScalar, dScalar, sizeof(double),
                              hipMemcpyDeviceToHost, stream));
         HIPCK(hipStreamSynchronize(stream));
-        harvestProfile();
         return *hScalar;
-    }
-
-    static constexpr int PROF_RING = 256;
-
-    // events of completed launches -> totals (call after a stream sync)
-    void harvestProfile()
-    {
-        for (int i = 0; i < profPending; i++)
-        {
-            float ms = 0.0f;
-            HIPCK(hipEventElapsedTime(&ms, profEv[2*i], profEv[2*i + 1]));
-            profMs += ms;
-            profBytes += profBytesPending[i];
-            profLaunches++;
-        }
-        profPending = 0;
-    }
-
-    template <typename T>
-    void profiledRbgs(int l, const OpView<T>& o, Pair<T> x, Pair<const T> b,
-                      int colour, const Region& rg)
-    {
-        const bool prof = profOn && l == profLevel;
-        if (prof && profPending == PROF_RING)
-        {
-            HIPCK(hipStreamSynchronize(stream));
-            harvestProfile();
-        }
-        if (prof)
-        {
-            if (profEv.empty())
-            {
-                profEv.resize(2*PROF_RING);
-                for (auto& e : profEv) HIPCK(hipEventCreate(&e));
-                profBytesPending.resize(PROF_RING);
-            }
-            HIPCK(hipEventRecord(profEv[2*profPending], stream));
-        }
-        launch_rbgs(lv[l].L, o, x, b, colour, rg, stream);
-        if (prof)
-        {
-            HIPCK(hipEventRecord(profEv[2*profPending + 1], stream));
-            long m[3];
-            for (int d = 0; d < 3; d++) m[d] = std::max(0, rg.E[d] - rg.S[d]);
-            const long tot = m[0]*m[1]*m[2];
-            const int cs = (rg.S[0] + rg.S[1] + rg.S[2]) & 1;
-            const long cells = (tot % 2 == 0) ? tot/2
-                             : (colour == cs ? (tot + 1)/2 : tot/2);
-            const double per = o.kind == OP_CONST ? 3.0
-                             : o.kind == OP_FACES ? 9.0 : 10.0;
-            profBytesPending[profPending] = double(cells)*per*sizeof(T);
-            profPending++;
-        }
     }
 
     Region interior(int l, int padding) const
@@ -340,6 +287,8 @@ struct bmg_ctx
     template <typename T>
     void cycle(std::vector<int>& visits, int l);
 
+    void enqueueIteration(bool mixed);
+    double iterate(bool mixed);
     void prepare(bool constMatrix);
     void prepareCoarse();
     void ensureMixed();
@@ -404,7 +353,10 @@ template <> Pair<double> bmg_ctx::Tmp<double>(int l)
 {
     Level& q = lv[l];
     if (!q.tmp[0])
+    {
         for (int c = 0; c < 2; c++) q.tmp[c] = devAlloc<double>(q.L.ncol);
+        gen++;
+    }
     return pr(q.tmp);
 }
 template <> Pair<float> bmg_ctx::X<float>(int l) { return pr(lv[l].xf); }
@@ -414,7 +366,10 @@ template <> Pair<float> bmg_ctx::Tmp<float>(int l)
 {
     Level& q = lv[l];
     if (!q.tmpf[0])
+    {
         for (int c = 0; c < 2; c++) q.tmpf[c] = devAlloc<float>(q.L.ncol);
+        gen++;
+    }
     return pr(q.tmpf);
 }
 
@@ -430,7 +385,8 @@ static Pair<const T> cst(Pair<T> p)
 // forward odd = the other, backward even/odd anchored at E-1.  A phase that
 // repeats the phase right before it on the same region is skipped: it reads
 // only the other colour, which did not change, so it would rewrite the same
-// values.
+// values.  The boundary slabs of each periodic face pair run as one
+// temporally blocked launch (k_rbgs_slabs).
 template <typename T>
 void bmg_ctx::smooth(int l, int sweeps)
 {
@@ -454,39 +410,51 @@ void bmg_ctx::smooth(int l, int sweeps)
         }
         return;
     }
-    struct OpRec { int kind; Region rg; int colour; };
-    std::vector<OpRec> ops;
-    auto phases = [&](const Region& rg)
+    auto colours = [](const Region& rg, int out[4]) -> int
     {
         const int Ss = rg.S[0] + rg.S[1] + rg.S[2];
         const int Es = rg.E[0] + rg.E[1] + rg.E[2];
-        const int cols[4] = {Ss & 1, (Ss + 1) & 1, (Es + 1) & 1, Es & 1};
-        for (int c : cols) ops.push_back({0, rg, c});
+        const int c4[4] = {Ss & 1, (Ss + 1) & 1, (Es + 1) & 1, Es & 1};
+        int n = 0;
+        for (int i = 0; i < 4; i++)
+            if (n == 0 || out[n - 1] != c4[i]) out[n++] = c4[i];
+        return n;
     };
+    // the previous phase launched on this level: a phase that repeats it on
+    // the same region would rewrite the same values and is skipped
+    int prevColour = -1;
     for (int s = 0; s < sweeps; s++)
     {
-        phases(interior(l, 1));
-        if (anyPeriodic())
+        // interior region, padded on the periodic faces
+        const Region rg = interior(l, 1);
+        int cols[4];
+        const int nc = colours(rg, cols);
+        for (int i = 0; i < nc; i++)
         {
-            ops.push_back({1, Region{}, 0});
-            for (int f = 0; f < 6; f++)
-                if (periodic(f)) phases(slab(l, f));
+            if (cols[i] != prevColour)
+                launch_rbgs(q.L, o, x, b, cols[i], rg, stream);
+            prevColour = cols[i];
         }
-    }
-    const OpRec* prev = nullptr;
-    for (const OpRec& r : ops)
-    {
-        if (r.kind == 0)
+        if (!anyPeriodic()) continue;
+        launch_bc_fill(q.L, o, x, 0, l == 0, bcval, stream);
+        // width-1 slabs of the periodic faces, face pairs in order
+        for (int d = 0; d < 3; d++)
         {
-            const bool same = prev && prev->kind == 0 &&
-                              prev->colour == r.colour &&
-                              std::memcmp(&prev->rg, &r.rg, sizeof(Region)) == 0;
-            if (!same)
-                profiledRbgs<T>(l, o, x, b, r.colour, r.rg);
+            if (!periodic(2*d)) continue;
+            int planes[2] = {1, q.L.N[d]};
+            int cl[8];
+            const int n0 = colours(slab(l, 2*d), cl);
+            const int n1 = colours(slab(l, 2*d + 1), cl + 4);
+            if (q.L.N[d] >= 3 && n0 == n1)
+                launch_rbgs_slabs(q.L, o, x, b, d, planes, 2, cl, n0, stream);
+            else
+            {
+                launch_rbgs_slabs(q.L, o, x, b, d, planes, 1, cl, n0, stream);
+                launch_rbgs_slabs(q.L, o, x, b, d, planes + 1, 1, cl + 4, n1,
+                                  stream);
+            }
         }
-        else
-            launch_bc_fill(q.L, o, x, 0, l == 0, bcval, stream);
-        prev = &r;
+        prevColour = -1;
     }
 }
 
@@ -681,10 +649,21 @@ void bmg_ctx::prepareCoarse()
             for (int i = j + 1; i < n; i++) D[size_t(i)*n + j] *= rr;
         }
     }
-    devFree(dLU);
-    devFree(dPiv);
-    dLU = devUpload(D);
-    dPiv = devUpload(piv);
+    if (dLU && nc == n)
+    {
+        HIPCK(hipMemcpy(dLU, D.data(), D.size()*sizeof(double),
+                        hipMemcpyHostToDevice));
+        HIPCK(hipMemcpy(dPiv, piv.data(), piv.size()*sizeof(int),
+                        hipMemcpyHostToDevice));
+    }
+    else
+    {
+        devFree(dLU);
+        devFree(dPiv);
+        dLU = devUpload(D);
+        dPiv = devUpload(piv);
+        gen++;
+    }
     nc = n;
     prepared = true;
     preparedCoarse = l;
@@ -713,6 +692,7 @@ void bmg_ctx::prepare(bool constMatrix)
         launch_materialize_stencil(qc.L, op<double>(lc), pr(qc.a7), stream);
         qc.opkind = OP_STENCIL;
         mixedReady = false;
+        gen++;
     }
     if (singular())
         launch_pin(qc.L, pr(qc.a7), stream);
@@ -739,6 +719,7 @@ void bmg_ctx::prepare(bool constMatrix)
 void bmg_ctx::ensureMixed()
 {
     if (mixedReady) return;
+    gen++;
     for (int l = 0; l < nLevels(); l++)
     {
         Level& q = lv[l];
@@ -803,6 +784,68 @@ void bmg_ctx::ensureMixed()
     mixedReady = true;
 }
 
+// One MG iteration (MG.C:207-230): the cycle, the level-0 residual with its
+// norm, and the copy of the norm to pinned host memory.  No synchronisation
+// inside, so the sequence can be captured into a HIP graph.
+void bmg_ctx::enqueueIteration(bool mixed)
+{
+    Level& q0 = lv[0];
+    std::vector<int> visits(nLevels(), 0);
+    if (!mixed)
+        cycle<double>(visits, 0);
+    else
+    {
+        // defect equation A e = r in FP32, e0 = 0; x += e in FP64
+        Pair<const double> r64{{q0.r[0], q0.r[1]}};
+        launch_to_float(q0.L.ncol, r64, pr(q0.bf), stream);
+        launch_fill<float>(pr(q0.xf), q0.L.ncol, 0.0f, stream);
+        cycle<float>(visits, 0);
+        launch_add_correction(q0.L, pr(q0.x), cpr(q0.xf), stream);
+    }
+    int nb = 0;
+    residual<double>(0, true, true, &nb);
+    launch_reduce_final(partials, nb, dScalar, stream);
+    HIPCK(hipMemcpyAsync(hScalar, dScalar, sizeof(double),
+                         hipMemcpyDeviceToHost, stream));
+}
+
+// Runs one iteration and returns the sum of squared level-0 residuals.
+double bmg_ctx::iterate(bool mixed)
+{
+    if (useGraphs)
+    {
+        if (opts.smoother == BMG_JAC)
+            for (int l = 0; l < nLevels(); l++)
+                mixed ? (void)Tmp<float>(l) : (void)Tmp<double>(l);
+        if (!graphExec || graphGen != gen || graphMixed != int(mixed))
+        {
+            if (graphExec) { HIPCK(hipGraphExecDestroy(graphExec)); graphExec = nullptr; }
+            if (graph) { HIPCK(hipGraphDestroy(graph)); graph = nullptr; }
+            HIPCK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            try
+            {
+                enqueueIteration(mixed);
+            }
+            catch (...)
+            {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            HIPCK(hipStreamEndCapture(stream, &graph));
+            HIPCK(hipGraphInstantiate(&graphExec, graph, nullptr, nullptr, 0));
+            graphGen = gen;
+            graphMixed = int(mixed);
+        }
+        HIPCK(hipGraphLaunch(graphExec, stream));
+    }
+    else
+        enqueueIteration(mixed);
+    HIPCK(hipStreamSynchronize(stream));
+    return *hScalar;
+}
+
 // solver::normFactors (solver.C:13-61)
 double bmg_ctx::normFactor()
 {
@@ -846,24 +889,10 @@ void bmg_ctx::runSolve(bool constMatrix, bmg_stats* st)
     };
     int conv = check(cur);
     int iter = 0;
-    std::vector<int> visits(nLevels(), 0);
     while ((iter < opts.max_iter && conv == 0) || iter < opts.min_iter)
     {
         if (iter < opts.min_iter) conv = 0;
-        std::fill(visits.begin(), visits.end(), 0);
-        if (!mixed)
-            cycle<double>(visits, 0);
-        else
-        {
-            // defect equation A e = r in FP32, e0 = 0; x += e in FP64
-            Pair<const double> r64{{q0.r[0], q0.r[1]}};
-            launch_to_float(q0.L.ncol, r64, pr(q0.bf), stream);
-            launch_fill<float>(pr(q0.xf), q0.L.ncol, 0.0f, stream);
-            cycle<float>(visits, 0);
-            launch_add_correction(q0.L, pr(q0.x), cpr(q0.xf), stream);
-        }
-        residual<double>(0, true, true, &nb);
-        const double ss = reduceFinal(nb);
+        const double ss = iterate(mixed);
         cur = std::sqrt(ss)/nf;
         if (!std::isfinite(cur))
             throw Error(BMG_ERR_NUMERIC, "residual norm is not finite");
@@ -1009,6 +1038,8 @@ int bmg_create(bmg_ctx** out, const bmg_grid* g, const bmg_opts* opts)
                 throw Error(BMG_ERR_ARG,
                             "periodic faces must come in opposite pairs");
         std::unique_ptr<bmg_ctx> c(new bmg_ctx);
+        if (const char* e = std::getenv("BMG_NO_GRAPHS"))
+            c->useGraphs = e[0] == '0';
         HIPCK(hipGetDevice(&c->device));
         HIPCK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         HIPCK(hipEventCreate(&c->ev0));
@@ -1045,7 +1076,11 @@ int bmg_create(bmg_ctx** out, const bmg_grid* g, const bmg_opts* opts)
             }
             for (int f = 0; f < 6; f++) q.L.bc[f] = c->bc[f];
             q.L.Hz = (q.L.P[2] + 1)/2;
+            q.L.sI = q.L.P[1]*q.L.Hz;
             q.L.ncol = long(q.L.P[0])*q.L.P[1]*q.L.Hz;
+            if (q.L.ncol*8 >= (1L << 31))
+                throw Error(BMG_ERR_ARG, "level too large for 32-bit colour "
+                                         "indexing");
             c->lv.push_back(q);
             int Nc[3];
             bool coarsenable = false;
@@ -1150,6 +1185,7 @@ int bmg_set_opts(bmg_ctx* c, const bmg_opts* o)
             o->coarse_mode != c->opts.coarse_mode)
             c->prepared = false;
         c->opts = *o;
+        c->gen++;
     });
 }
 
@@ -1191,6 +1227,7 @@ static void setStencil(bmg_ctx* c, int level, const double* A7, bool device,
     if (d) devFree(d);
     q.opkind = OP_STENCIL;
     q.haveOp = true;
+    c->gen++;
     c->symmetric = symmetric != 0;
     c->eliminated = false;
     c->prepared = false;
@@ -1290,6 +1327,7 @@ int bmg_set_operator_laplacian(bmg_ctx* c, const double* lx, const double* ly,
         HIPCK(hipGetLastError());
         HIPCK(hipStreamSynchronize(c->stream));
         for (double* p : lam) if (p) (void)hipFree(p);
+        c->gen++;
         c->symmetric = true;
         c->eliminated = false;
         c->prepared = false;
@@ -1363,7 +1401,6 @@ static void solveImpl(bmg_ctx* c, double* b, double* x, bool device,
         launch_split_to_ref<double>(q.L, cpr(q.b), db, c->stream);
         HIPCK(hipEventRecord(c->ev1, c->stream));
         HIPCK(hipEventSynchronize(c->ev1));
-        c->harvestProfile();
         float ms = 0.0f;
         HIPCK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
         local.solve_ms = ms;
@@ -1411,6 +1448,45 @@ int bmg_solve_device(bmg_ctx* c, double* db, double* dx, int constMatrix,
                      bmg_stats* st)
 {
     return run([&] { solveImpl(c, db, dx, true, constMatrix, st); });
+}
+
+// MG::solve of a diagonal system (MG.C:404-428): level-0 stencil, ghost
+// elimination, x = b/A_C, level-0 boundary update.
+int bmg_solve_diagonal(bmg_ctx* c, const double* A7, double* b, double* x)
+{
+    return run([&] {
+        if (!c || !A7 || !b || !x) throw Error(BMG_ERR_ARG, "null pointer");
+        setStencil(c, 0, A7, false, 1);
+        Level& q = c->lv[0];
+        const size_t n = size_t(q.L.P[0])*q.L.P[1]*q.L.P[2];
+        double* db = devAlloc<double>(n);
+        double* dx = devAlloc<double>(n);
+        HIPCK(hipMemcpyAsync(db, b, n*sizeof(double), hipMemcpyHostToDevice,
+                             c->stream));
+        HIPCK(hipMemcpyAsync(dx, x, n*sizeof(double), hipMemcpyHostToDevice,
+                             c->stream));
+        launch_ref_to_split<double>(q.L, db, pr(q.b), c->stream);
+        launch_ref_to_split<double>(q.L, dx, pr(q.x), c->stream);
+        bool inhom = false;
+        for (int f = 0; f < 6; f++)
+            if (!c->periodic(f) && c->bcval[f] != 0.0) inhom = true;
+        if (inhom)
+            launch_bc_source(q.L, c->op<double>(0), pr(q.b), c->bcval, c->stream);
+        launch_eliminate(q.L, pr(q.a7), c->stream);
+        c->eliminated = true;
+        launch_diag(q.L, cpr(q.a7), pr(q.x), cpr(q.b), c->stream);
+        launch_bc_fill(q.L, c->op<double>(0), pr(q.x), 1, 1, c->bcval, c->stream);
+        launch_split_to_ref<double>(q.L, cpr(q.x), dx, c->stream);
+        launch_split_to_ref<double>(q.L, cpr(q.b), db, c->stream);
+        HIPCK(hipGetLastError());
+        HIPCK(hipMemcpyAsync(x, dx, n*sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
+        HIPCK(hipMemcpyAsync(b, db, n*sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream));
+        HIPCK(hipStreamSynchronize(c->stream));
+        devFree(db);
+        devFree(dx);
+    });
 }
 
 static double* fieldPtr(bmg_ctx* c, int which, int level, int colour)
@@ -1578,30 +1654,49 @@ int bmg_op_sum_sq_residual(bmg_ctx* c, double* out)
     });
 }
 
-int bmg_profile_smoother(bmg_ctx* c, int enable, int level)
+int bmg_time_smoother(bmg_ctx* c, int level, int precision, int reps,
+                      double* avg_ms, double* bytes)
 {
     return run([&] {
         checkLevel(c, level);
+        needEliminated(c);
+        if (reps < 1) throw Error(BMG_ERR_ARG, "reps must be positive");
+        if (precision == BMG_MIXED) c->ensureMixed();
+        Level& q = c->lv[level];
+        const Region rg = c->interior(level, 1);
         HIPCK(hipStreamSynchronize(c->stream));
-        c->harvestProfile();
-        c->profOn = enable != 0;
-        c->profLevel = level;
-        c->profMs = 0.0;
-        c->profBytes = 0.0;
-        c->profLaunches = 0;
+        HIPCK(hipEventRecord(c->ev0, c->stream));
+        for (int r = 0; r < reps; r++)
+        {
+            if (precision == BMG_MIXED)
+                launch_rbgs<float>(q.L, c->op<float>(level), pr(q.xf), cpr(q.bf),
+                                   r & 1, rg, c->stream);
+            else
+                launch_rbgs<double>(q.L, c->op<double>(level), pr(q.x), cpr(q.b),
+                                    r & 1, rg, c->stream);
+        }
+        HIPCK(hipEventRecord(c->ev1, c->stream));
+        HIPCK(hipEventSynchronize(c->ev1));
+        HIPCK(hipGetLastError());
+        float ms = 0.0f;
+        HIPCK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        long m[3];
+        for (int d = 0; d < 3; d++) m[d] = std::max(0, rg.E[d] - rg.S[d]);
+        const double cells = 0.5*double(m[0]*m[1]*m[2]);
+        const int kind = q.opkind;
+        const double per = kind == OP_CONST ? 3.0 : kind == OP_FACES ? 9.0 : 10.0;
+        if (avg_ms) *avg_ms = double(ms)/reps;
+        if (bytes)
+            *bytes = cells*per*(precision == BMG_MIXED ? sizeof(float)
+                                                       : sizeof(double));
     });
 }
 
-int bmg_profile_read(bmg_ctx* c, double* total_ms, long long* launches,
-                     double* bytes)
+int bmg_set_graphs(bmg_ctx* c, int enable)
 {
     return run([&] {
         if (!c) throw Error(BMG_ERR_ARG, "null context");
-        HIPCK(hipStreamSynchronize(c->stream));
-        c->harvestProfile();
-        if (total_ms) *total_ms = c->profMs;
-        if (launches) *launches = c->profLaunches;
-        if (bytes) *bytes = c->profBytes;
+        c->useGraphs = enable != 0;
     });
 }
 

@@ -28,7 +28,8 @@ struct LevelDims {
     int N[3];
     int P[3];
     int Hz;
-    long ncol;     // elements per colour array = P0*P1*Hz
+    int sI;        // colour-array stride of one I plane = P1*Hz
+    long ncol;     // elements per colour array = P0*P1*Hz (< 2^31/8)
     int bc[6];
     int R[3];      // refinement ratio of this level vs its parent
 };
@@ -66,6 +67,12 @@ template <typename T>
 void launch_rbgs(const LevelDims& L, const OpView<T>& op, Pair<T> x,
                  Pair<const T> b, int colour, const Region& rg,
                  hipStream_t s);
+// All phases of the boundary slabs of one face pair (both planes when they
+// are independent, nslab = 2) in one launch; colours[4*slab + phase].
+template <typename T>
+void launch_rbgs_slabs(const LevelDims& L, const OpView<T>& op, Pair<T> x,
+                       Pair<const T> b, int d, const int* planes, int nslab,
+                       const int* colours, int nphase, hipStream_t s);
 template <typename T>
 void launch_jacobi(const LevelDims& L, const OpView<T>& op, Pair<T> x,
                    Pair<const T> b, Pair<T> tmp, const Region& rg,
@@ -133,6 +140,8 @@ void launch_norm_factor_terms(const LevelDims& L, const OpView<T>& op,
                               double* partials, int* nblocks, hipStream_t s);
 void launch_reduce_final(const double* partials, int n, double* out,
                          hipStream_t s);
+void launch_diag(const LevelDims& L, Pair<const double> a7, Pair<double> x,
+                 Pair<const double> b, hipStream_t s);
 // x64 += e32 (interior), r32 = (float) r64, etc.
 void launch_add_correction(const LevelDims& L, Pair<double> x,
                            Pair<const float> e, hipStream_t s);

@@ -20,10 +20,12 @@ struct BcVals {
     double v[6];
 };
 
-__device__ __forceinline__ long cidx(const LevelDims& L, int I, int J, int K)
+__device__ __forceinline__ int cidx(const LevelDims& L, int I, int J, int K)
 {
-    return ((long)I*L.P[1] + J)*L.Hz + (K >> 1);
+    return (I*L.P[1] + J)*L.Hz + (K >> 1);
 }
+
+__device__ __forceinline__ int nci(const LevelDims& L) { return int(L.ncol); }
 
 // Stencil coefficients [C,L,R,B,T,A,F] of padded cell (I,J,K), as assembled
 // by linearGaussLaplacianScheme.C:61-94 (centre = 0 - v_xl - v_xu - v_yl -
@@ -31,14 +33,15 @@ __device__ __forceinline__ long cidx(const LevelDims& L, int I, int J, int K)
 template <typename T, int OP>
 __device__ __forceinline__ void raw_coeffs(const LevelDims& L,
                                            const OpView<T>& op, int colour,
-                                           long idx, long row, int p, int q,
+                                           int idx, int row, int p, int q,
                                            int I, int J, int K, T a[7])
 {
     if constexpr (OP == OP_STENCIL)
     {
         const T* s = op.a7[colour];
+        const int n = nci(L);
 #pragma unroll
-        for (int c = 0; c < 7; c++) a[c] = s[c*L.ncol + idx];
+        for (int c = 0; c < 7; c++) a[c] = s[c*n + idx];
     }
     else
     {
@@ -57,15 +60,15 @@ __device__ __forceinline__ void raw_coeffs(const LevelDims& L,
         }
         else
         {
-            const long sI = (long)L.P[1]*L.Hz;
+            const int n = nci(L);
             const T* wc = op.wf[colour];
             const T* wo = op.wf[colour ^ 1];
             a[1] = wc[idx];
-            a[2] = wo[idx + sI];
-            a[3] = wc[L.ncol + idx];
-            a[4] = wo[L.ncol + idx + L.Hz];
-            a[5] = wc[2*L.ncol + idx];
-            a[6] = wo[2*L.ncol + row + p + q];
+            a[2] = wo[idx + L.sI];
+            a[3] = wc[n + idx];
+            a[4] = wo[n + idx + L.Hz];
+            a[5] = wc[2*n + idx];
+            a[6] = wo[2*n + row + p + q];
         }
         T c0 = T(0) - a[1];
         c0 = c0 - a[2];
@@ -100,7 +103,7 @@ __device__ __forceinline__ void eliminate_cell(const LevelDims& L, int I,
 
 template <typename T, int OP>
 __device__ __forceinline__ void coeffs(const LevelDims& L, const OpView<T>& op,
-                                       int colour, long idx, long row, int p,
+                                       int colour, int idx, int row, int p,
                                        int q, int I, int J, int K, T a[7])
 {
     raw_coeffs<T, OP>(L, op, colour, idx, row, p, q, I, J, K, a);
@@ -143,9 +146,9 @@ __global__ __launch_bounds__(BX*BY) void k_rbgs(LevelDims L, OpView<T> op,
     const int p = pmin + blockIdx.x*BX + threadIdx.x;
     const int K = 2*p + q;
     if (K >= rg.E[2]) return;
-    const long sI = (long)L.P[1]*L.Hz;
-    const long row = ((long)I*L.P[1] + J)*L.Hz;
-    const long idx = row + p;
+    const int sI = L.sI;
+    const int row = (I*L.P[1] + J)*L.Hz;
+    const int idx = row + p;
     T a[7];
     coeffs<T, OP>(L, op, colour, idx, row, p, q, I, J, K, a);
     const T P =
@@ -158,6 +161,130 @@ __global__ __launch_bounds__(BX*BY) void k_rbgs(LevelDims L, OpView<T> op,
     xc[idx] = (bcl[idx] - P)/a[0];
 }
 
+// All red-black phases of the width-1 boundary slabs of one face pair
+// (rbgsSmoother::SweepBoundary, rbgsSmoother.C:186-374) in one launch.  A
+// slab is a plane of cells whose out-of-plane neighbours (the interior layer
+// and the ghost layer filled just before) stay fixed while its phases run,
+// so every phase is a 2-D red-black update of the plane.  Each workgroup
+// owns a TT x TT tile, loads the tile grown by H cells (H = number of
+// phases) into LDS and runs the phases there: phase p updates the tile grown
+// by H-1-p cells, whose neighbours are exact after phase p-1, so the tile is
+// exact after the last phase and the halo work is redundant but identical.
+// Opposite slabs of a pair are independent (they are at least two cells
+// apart) and run as blockIdx.z = 0/1.  The update is the row product in the
+// reference order with the two out-of-plane terms read from global memory.
+struct SlabArgs {
+    int d;            // slab normal axis
+    int plane[2];     // padded index of each slab plane
+    int nphase;       // H
+    int colours[2][4];
+    int tt;           // output tile edge
+};
+
+template <typename T, int OP>
+__global__ __launch_bounds__(256) void k_rbgs_slabs(LevelDims L, OpView<T> op,
+                                                    Pair<T> x, Pair<const T> b,
+                                                    SlabArgs sa)
+{
+    extern __shared__ unsigned char slab_smem[];
+    T* xs = reinterpret_cast<T*>(slab_smem);
+    const int d = sa.d;
+    const int a1 = d == 0 ? 1 : 0;
+    const int a2 = d == 2 ? 1 : 2;
+    const int H = sa.nphase;
+    const int W = sa.tt + 2*H;
+    const int Ip = sa.plane[blockIdx.z];
+    const int o1 = 1 + blockIdx.y*sa.tt - H;
+    const int o2 = 1 + blockIdx.x*sa.tt - H;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int e = tid; e < W*W; e += nt)
+    {
+        const int c1 = o1 + e/W, c2 = o2 + e % W;
+        T v = T(0);
+        if (c1 >= 0 && c1 < L.P[a1] && c2 >= 0 && c2 < L.P[a2])
+        {
+            int g[3];
+            g[d] = Ip; g[a1] = c1; g[a2] = c2;
+            v = x.c[(g[0] + g[1] + g[2]) & 1][cidx(L, g[0], g[1], g[2])];
+        }
+        xs[e] = v;
+    }
+    __syncthreads();
+    for (int ph = 0; ph < H; ph++)
+    {
+        const int colour = sa.colours[blockIdx.z][ph];
+        const int grow = H - 1 - ph;
+        const int lo = H - grow, hi = W - (H - grow);   // LDS window
+        for (int e = tid; e < W*W; e += nt)
+        {
+            const int u1 = e/W, u2 = e % W;
+            if (u1 < lo || u1 >= hi || u2 < lo || u2 >= hi) continue;
+            const int c1 = o1 + u1, c2 = o2 + u2;
+            if (c1 < 1 || c1 > L.N[a1] || c2 < 1 || c2 > L.N[a2]) continue;
+            int g[3];
+            g[d] = Ip; g[a1] = c1; g[a2] = c2;
+            const int I = g[0], J = g[1], K = g[2];
+            if (((I + J + K) & 1) != colour) continue;
+            const int q = K & 1;
+            const int p = K >> 1;
+            const int row = (I*L.P[1] + J)*L.Hz;
+            const int idx = row + p;
+            T a[7];
+            coeffs<T, OP>(L, op, colour, idx, row, p, q, I, J, K, a);
+            const T* xo = x.c[colour ^ 1];
+            // neighbour values in the reference order -i,+i,-j,+j,-k,+k
+            T nb[6];
+            const int e1 = (u1)*W, e1m = (u1 - 1)*W, e1p = (u1 + 1)*W;
+            if (d == 0)
+            {
+                nb[0] = xo[idx - L.sI];
+                nb[1] = xo[idx + L.sI];
+                nb[2] = xs[e1m + u2];
+                nb[3] = xs[e1p + u2];
+                nb[4] = xs[e1 + u2 - 1];
+                nb[5] = xs[e1 + u2 + 1];
+            }
+            else if (d == 1)
+            {
+                nb[0] = xs[e1m + u2];
+                nb[1] = xs[e1p + u2];
+                nb[2] = xo[idx - L.Hz];
+                nb[3] = xo[idx + L.Hz];
+                nb[4] = xs[e1 + u2 - 1];
+                nb[5] = xs[e1 + u2 + 1];
+            }
+            else
+            {
+                nb[0] = xs[e1m + u2];
+                nb[1] = xs[e1p + u2];
+                nb[2] = xs[e1 + u2 - 1];
+                nb[3] = xs[e1 + u2 + 1];
+                nb[4] = xo[row + p - 1 + q];
+                nb[5] = xo[row + p + q];
+            }
+            const T P =
+                a[1]*nb[0]
+              + a[2]*nb[1]
+              + a[3]*nb[2]
+              + a[4]*nb[3]
+              + a[5]*nb[4]
+              + a[6]*nb[5];
+            xs[e] = (b.c[colour][idx] - P)/a[0];
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < W*W; e += nt)
+    {
+        const int u1 = e/W, u2 = e % W;
+        if (u1 < H || u1 >= W - H || u2 < H || u2 >= W - H) continue;
+        const int c1 = o1 + u1, c2 = o2 + u2;
+        if (c1 < 1 || c1 > L.N[a1] || c2 < 1 || c2 > L.N[a2]) continue;
+        int g[3];
+        g[d] = Ip; g[a1] = c1; g[a2] = c2;
+        x.c[(g[0] + g[1] + g[2]) & 1][cidx(L, g[0], g[1], g[2])] = xs[e];
+    }
+}
+
 // Damped Jacobi (jacSmoother.C:18-127, omega 0.8): y from x, then copy back.
 template <typename T, int OP>
 __global__ __launch_bounds__(BX*BY) void k_jacobi(LevelDims L, OpView<T> op,
@@ -168,9 +295,9 @@ __global__ __launch_bounds__(BX*BY) void k_jacobi(LevelDims L, OpView<T> op,
     const int J = rg.S[1] + blockIdx.y*BY + threadIdx.y;
     const int p = (rg.S[2] >> 1) + blockIdx.x*BX + threadIdx.x;
     if (J >= rg.E[1]) return;
-    const long sI = (long)L.P[1]*L.Hz;
-    const long row = ((long)I*L.P[1] + J)*L.Hz;
-    const long idx = row + p;
+    const int sI = L.sI;
+    const int row = (I*L.P[1] + J)*L.Hz;
+    const int idx = row + p;
     const T omega = T(0.8);
     const T om1 = T(1.0) - T(0.8);
 #pragma unroll
@@ -203,7 +330,7 @@ __global__ __launch_bounds__(BX*BY) void k_copy_region(LevelDims L,
     const int J = rg.S[1] + blockIdx.y*BY + threadIdx.y;
     const int p = (rg.S[2] >> 1) + blockIdx.x*BX + threadIdx.x;
     if (J >= rg.E[1]) return;
-    const long idx = ((long)I*L.P[1] + J)*L.Hz + p;
+    const int idx = (I*L.P[1] + J)*L.Hz + p;
 #pragma unroll
     for (int h = 0; h < 2; h++)
     {
@@ -232,9 +359,9 @@ __global__ __launch_bounds__(BX*BY) void k_residual(LevelDims L, OpView<T> op,
     double acc = 0.0;
     if (J < rg.E[1])
     {
-        const long sI = (long)L.P[1]*L.Hz;
-        const long row = ((long)I*L.P[1] + J)*L.Hz;
-        const long idx = row + p;
+        const int sI = L.sI;
+        const int row = (I*L.P[1] + J)*L.Hz;
+        const int idx = row + p;
 #pragma unroll
         for (int h = 0; h < 2; h++)
         {
@@ -322,7 +449,7 @@ __global__ __launch_bounds__(BX*BY) void k_restrict(LevelDims Lf, LevelDims Lc,
         cvc = cvc*tv.cw[2][K];
         cvc = cvc > T(1e-16) ? cvc : T(1e-16);
         v = v*cvc;
-        bc.c[(I + J + K) & 1][((long)I*Lc.P[1] + J)*Lc.Hz + p] = v;
+        bc.c[(I + J + K) & 1][(I*Lc.P[1] + J)*Lc.Hz + p] = v;
     }
 }
 
@@ -331,6 +458,12 @@ __global__ __launch_bounds__(BX*BY) void k_restrict(LevelDims Lf, LevelDims Lc,
 // fastest, idx = i/R + abc*off, off = i odd ? +1 : -1, w_v = (gx*gy)*gz with
 // g = 1-u (abc 0) or u (abc 1); u = 1/2 on all axes when any u lies outside
 // [0,1) (:89-93).
+//
+// One thread per pair of fine cells K = 2p, 2p+1.  Along z the two cells use
+// the same two coarse indices (for R = 2 the odd cell's parent is the even
+// cell's neighbour and vice versa; for R = 1 both are {2p, 2p+1}), so the 8
+// coarse values are loaded once per pair and each cell sums them in its own
+// parent/neighbour order.
 template <typename T>
 __global__ __launch_bounds__(BX*BY) void k_prolong(LevelDims Lf, LevelDims Lc,
                                                    TransferView<T> tv,
@@ -341,26 +474,50 @@ __global__ __launch_bounds__(BX*BY) void k_prolong(LevelDims Lf, LevelDims Lc,
     const int J = 1 + blockIdx.y*BY + threadIdx.y;
     const int p = blockIdx.x*BX + threadIdx.x;
     if (J > Lf.N[1]) return;
+    const int K0 = 2*p, K1 = 2*p + 1;
+    const bool v0 = K0 >= 1 && K0 <= Lf.N[2];
+    const bool v1 = K1 >= 1 && K1 <= Lf.N[2];
+    if (!v0 && !v1) return;
     const int i = I - 1, j = J - 1;
     const int offx = (i & 1) ? 1 : -1, offy = (j & 1) ? 1 : -1;
-    const int bx = i/Lc.R[0], by = j/Lc.R[1];
-    T ux = tv.pu[0][I], uy = tv.pu[1][J];
+    const int ci[2] = {i/Lc.R[0] + 1, i/Lc.R[0] + 1 + offx};
+    const int cj[2] = {j/Lc.R[1] + 1, j/Lc.R[1] + 1 + offy};
+    // coarse z indices shared by both cells: zl < zh
+    const int zl = Lc.R[2] == 1 ? 2*p : p;
+    const int zh = zl + 1;
+    T c[2][2][2];   // [cz: zl/zh][cy][cx]
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int bb = 0; bb < 2; bb++)
+    {
+        const int base = (ci[a]*Lc.P[1] + cj[bb])*Lc.Hz;
+        const int par = ci[a] + cj[bb];
+        c[0][bb][a] = xc.c[(par + zl) & 1][base + (zl >> 1)];
+        c[1][bb][a] = xc.c[(par + zh) & 1][base + (zh >> 1)];
+    }
+    const T ux = tv.pu[0][I], uy = tv.pu[1][J];
+    const int colour0 = (I + J + K0) & 1;
+    const int idx = (I*Lf.P[1] + J)*Lf.Hz + p;
 #pragma unroll
     for (int h = 0; h < 2; h++)
     {
+        if (h == 0 ? !v0 : !v1) continue;
         const int K = 2*p + h;
-        if (K < 1 || K > Lf.N[2]) continue;
         const int k = K - 1;
+        // parent / neighbour along z as indices into c[0..1]
         const int offz = (k & 1) ? 1 : -1;
-        const int bz = k/Lc.R[2];
-        T uz = tv.pu[2][K];
-        T u0 = ux, u1 = uy, u2 = uz;
+        const int bz = k/Lc.R[2] + 1;
+        const int zpar = bz == zl ? 0 : 1;
+        const int znb = (bz + offz) == zl ? 0 : 1;
+        T u0 = ux, u1 = uy, u2 = tv.pu[2][K];
         const bool inside = !(u0 < T(0) || u0 >= T(1) || u1 < T(0) ||
                               u1 >= T(1) || u2 < T(0) || u2 >= T(1));
         if (!inside) { u0 = T(0.5); u1 = T(0.5); u2 = T(0.5); }
         const T gx[2] = {T(1.0) - u0, u0};
         const T gy[2] = {T(1.0) - u1, u1};
         const T gz[2] = {T(1.0) - u2, u2};
+        const int zsel[2] = {zpar, znb};
         T value = T(0);
 #pragma unroll
         for (int cz = 0; cz < 2; cz++)
@@ -373,16 +530,12 @@ __global__ __launch_bounds__(BX*BY) void k_prolong(LevelDims Lf, LevelDims Lc,
             w = w*gz[cz];
             if (w != T(0))
             {
-                const int ci = bx + cx*offx + 1;
-                const int cj = by + cy*offy + 1;
-                const int ck = bz + cz*offz + 1;
-                const T t = w*xc.c[(ci + cj + ck) & 1][cidx(Lc, ci, cj, ck)];
+                const T t = w*c[zsel[cz]][cy][cx];
                 value = value + t;
             }
         }
-        const int colour = (I + J + K) & 1;
-        const long idx = ((long)I*Lf.P[1] + J)*Lf.Hz + p;
-        xf.c[colour][idx] = xf.c[colour][idx] + value;
+        T* dst = xf.c[colour0 ^ h];
+        dst[idx] = dst[idx] + value;
     }
 }
 
@@ -519,7 +672,7 @@ __global__ __launch_bounds__(BX*BY) void k_ref_to_split(LevelDims L,
     const int J = blockIdx.y*BY + threadIdx.y;
     const int p = blockIdx.x*BX + threadIdx.x;
     if (J >= L.P[1] || p >= L.Hz) return;
-    const long idx = ((long)I*L.P[1] + J)*L.Hz + p;
+    const int idx = (I*L.P[1] + J)*L.Hz + p;
     for (int h = 0; h < 2; h++)
     {
         const int K = 2*p + h;
@@ -538,7 +691,7 @@ __global__ __launch_bounds__(BX*BY) void k_split_to_ref(LevelDims L,
     const int J = blockIdx.y*BY + threadIdx.y;
     const int p = blockIdx.x*BX + threadIdx.x;
     if (J >= L.P[1] || p >= L.Hz) return;
-    const long idx = ((long)I*L.P[1] + J)*L.Hz + p;
+    const int idx = (I*L.P[1] + J)*L.Hz + p;
     for (int h = 0; h < 2; h++)
     {
         const int K = 2*p + h;
@@ -555,7 +708,7 @@ __global__ __launch_bounds__(BX*BY) void k_stencil_ref_to_split(
     const int J = blockIdx.y*BY + threadIdx.y;
     const int p = blockIdx.x*BX + threadIdx.x;
     if (J >= L.P[1] || p >= L.Hz) return;
-    const long idx = ((long)I*L.P[1] + J)*L.Hz + p;
+    const int idx = (I*L.P[1] + J)*L.Hz + p;
     for (int h = 0; h < 2; h++)
     {
         const int K = 2*p + h;
@@ -573,7 +726,7 @@ __global__ __launch_bounds__(BX*BY) void k_stencil_split_to_ref(
     const int J = blockIdx.y*BY + threadIdx.y;
     const int p = blockIdx.x*BX + threadIdx.x;
     if (J >= L.P[1] || p >= L.Hz) return;
-    const long idx = ((long)I*L.P[1] + J)*L.Hz + p;
+    const int idx = (I*L.P[1] + J)*L.Hz + p;
     for (int h = 0; h < 2; h++)
     {
         const int K = 2*p + h;
@@ -595,8 +748,8 @@ __global__ __launch_bounds__(BX*BY) void k_materialize(LevelDims L,
     const int J = 1 + blockIdx.y*BY + threadIdx.y;
     const int p = blockIdx.x*BX + threadIdx.x;
     if (J > L.N[1]) return;
-    const long row = ((long)I*L.P[1] + J)*L.Hz;
-    const long idx = row + p;
+    const int row = (I*L.P[1] + J)*L.Hz;
+    const int idx = row + p;
     for (int h = 0; h < 2; h++)
     {
         const int K = 2*p + h;
@@ -635,7 +788,7 @@ __global__ __launch_bounds__(BX*BY) void k_pin(LevelDims L, Pair<double> a7)
     const int J = 1 + blockIdx.y*BY + threadIdx.y;
     const int p = blockIdx.x*BX + threadIdx.x;
     if (J > L.N[1]) return;
-    const long idx = ((long)I*L.P[1] + J)*L.Hz + p;
+    const int idx = (I*L.P[1] + J)*L.Hz + p;
     const int off[7][3] = {{0,0,0},{-1,0,0},{1,0,0},{0,-1,0},{0,1,0},
                            {0,0,-1},{0,0,1}};
     for (int h = 0; h < 2; h++)
@@ -663,7 +816,7 @@ __global__ __launch_bounds__(BX*BY) void k_eliminate(LevelDims L,
     const int J = 1 + blockIdx.y*BY + threadIdx.y;
     const int p = blockIdx.x*BX + threadIdx.x;
     if (J > L.N[1]) return;
-    const long idx = ((long)I*L.P[1] + J)*L.Hz + p;
+    const int idx = (I*L.P[1] + J)*L.Hz + p;
     for (int h = 0; h < 2; h++)
     {
         const int K = 2*p + h;
@@ -688,8 +841,8 @@ __global__ __launch_bounds__(BX*BY) void k_bc_source(LevelDims L,
     const int J = 1 + blockIdx.y*BY + threadIdx.y;
     const int p = blockIdx.x*BX + threadIdx.x;
     if (J > L.N[1]) return;
-    const long row = ((long)I*L.P[1] + J)*L.Hz;
-    const long idx = row + p;
+    const int row = (I*L.P[1] + J)*L.Hz;
+    const int idx = row + p;
     for (int h = 0; h < 2; h++)
     {
         const int K = 2*p + h;
@@ -777,17 +930,26 @@ __global__ void k_face_coeffs(LevelDims L, const double* w0, const double* w1,
 // the unit-lower forward substitution runs column by column; for each row
 // the subtractions still happen in column order, exactly as the row-wise
 // LUBacksubstitute.  The upper back substitution is row-wise in one lane.
-// Then the full ghost update of the level (APLU.C:126).
-template <typename T>
+// The factors are staged in LDS when they fit.  Then the full ghost update
+// of the level (APLU.C:126).
+template <typename T, bool LDS>
 __global__ __launch_bounds__(256) void k_coarse_solve(LevelDims L,
                                                       OpView<T> op, Pair<T> x,
                                                       Pair<const T> b,
-                                                      const double* lu,
+                                                      const double* lug,
                                                       const int* piv, int n,
                                                       int level0, BcVals bv)
 {
-    extern __shared__ double y[];
+    extern __shared__ double smem[];
+    double* y = smem;
+    const double* lu = lug;
     const int tid = threadIdx.x, nt = blockDim.x;
+    if constexpr (LDS)
+    {
+        double* l = smem + n;
+        for (int t = tid; t < n*n; t += nt) l[t] = lug[t];
+        lu = l;
+    }
     const int N1 = L.N[1], N2 = L.N[2];
     const int off[7][3] = {{0,0,0},{-1,0,0},{1,0,0},{0,-1,0},{0,1,0},
                            {0,0,-1},{0,0,1}};
@@ -795,7 +957,7 @@ __global__ __launch_bounds__(256) void k_coarse_solve(LevelDims L,
     {
         const int I = 1 + q/(N1*N2), J = 1 + (q/N2) % N1, K = 1 + q % N2;
         const int colour = (I + J + K) & 1;
-        const long idx = cidx(L, I, J, K);
+        const int idx = cidx(L, I, J, K);
         double v = double(b.c[colour][idx]);
         for (int s = 1; s < 7; s++)
         {
@@ -807,7 +969,7 @@ __global__ __launch_bounds__(256) void k_coarse_solve(LevelDims L,
                     out = true;
             if (out)
             {
-                const double a = double(op.a7[colour][s*L.ncol + idx]);
+                const double a = double(op.a7[colour][s*nci(L) + idx]);
                 const double t = a*double(x.c[colour ^ 1][cidx(L, g[0], g[1], g[2])]);
                 v = v - t;
             }
@@ -829,7 +991,7 @@ __global__ __launch_bounds__(256) void k_coarse_solve(LevelDims L,
         const double yj = y[j];
         for (int i = j + 1 + tid; i < n; i += nt)
         {
-            const double t = lu[(long)i*n + j]*yj;
+            const double t = lu[i*n + j]*yj;
             y[i] = y[i] - t;
         }
         __syncthreads();
@@ -837,13 +999,14 @@ __global__ __launch_bounds__(256) void k_coarse_solve(LevelDims L,
     if (tid == 0)
         for (int i = n - 1; i >= 0; i--)
         {
+            const double* row = lu + i*n;
             double sum = y[i];
             for (int j = i + 1; j < n; j++)
             {
-                const double t = lu[(long)i*n + j]*y[j];
+                const double t = row[j]*y[j];
                 sum = sum - t;
             }
-            y[i] = sum/lu[(long)i*n + i];
+            y[i] = sum/row[i];
         }
     __syncthreads();
     for (int q = tid; q < n; q += nt)
@@ -875,7 +1038,7 @@ __global__ __launch_bounds__(BX*BY) void k_reduce_field(LevelDims L,
     double acc = 0.0;
     if (J <= L.N[1])
     {
-        const long idx = ((long)I*L.P[1] + J)*L.Hz + p;
+        const int idx = (I*L.P[1] + J)*L.Hz + p;
         for (int h = 0; h < 2; h++)
         {
             const int K = 2*p + h;
@@ -904,8 +1067,8 @@ __global__ __launch_bounds__(BX*BY) void k_nf_terms(LevelDims L, OpView<T> op,
     double acc = 0.0;
     if (J <= L.N[1])
     {
-        const long row = ((long)I*L.P[1] + J)*L.Hz;
-        const long idx = row + p;
+        const int row = (I*L.P[1] + J)*L.Hz;
+        const int idx = row + p;
         for (int h = 0; h < 2; h++)
         {
             const int K = 2*p + h;
@@ -927,14 +1090,41 @@ __global__ __launch_bounds__(BX*BY) void k_nf_terms(LevelDims L, OpView<T> op,
         partials[blockIdx.x + gridDim.x*(blockIdx.y + gridDim.y*blockIdx.z)] = s;
 }
 
-__global__ __launch_bounds__(256) void k_reduce_final(const double* partials,
-                                                      int n, double* out)
+__global__ __launch_bounds__(1024) void k_reduce_final(const double* partials,
+                                                       int n, double* out)
 {
-    __shared__ double sm[4];
-    double acc = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) acc += partials[i];
-    const double s = block_sum(acc, sm);
+    __shared__ double sm[16];
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int i = threadIdx.x;
+    for (; i + 3*1024 < n; i += 4*1024)
+    {
+        a0 += partials[i];
+        a1 += partials[i + 1024];
+        a2 += partials[i + 2*1024];
+        a3 += partials[i + 3*1024];
+    }
+    for (; i < n; i += 1024) a0 += partials[i];
+    const double s = block_sum((a0 + a1) + (a2 + a3), sm);
     if (threadIdx.x == 0) out[0] = s;
+}
+
+// diagonalSmoother::Sweep (diagonalSmoother.C:13-69): x = b / A_C.
+__global__ __launch_bounds__(BX*BY) void k_diag(LevelDims L, Pair<const double> a7,
+                                                Pair<double> x,
+                                                Pair<const double> b)
+{
+    const int I = 1 + blockIdx.z;
+    const int J = 1 + blockIdx.y*BY + threadIdx.y;
+    const int p = blockIdx.x*BX + threadIdx.x;
+    if (J > L.N[1]) return;
+    const int idx = (I*L.P[1] + J)*L.Hz + p;
+    for (int h = 0; h < 2; h++)
+    {
+        const int K = 2*p + h;
+        if (K < 1 || K > L.N[2]) continue;
+        const int c = (I + J + K) & 1;
+        x.c[c][idx] = b.c[c][idx]/a7.c[c][idx];
+    }
 }
 
 __global__ __launch_bounds__(BX*BY) void k_add_correction(LevelDims L,
@@ -945,7 +1135,7 @@ __global__ __launch_bounds__(BX*BY) void k_add_correction(LevelDims L,
     const int J = 1 + blockIdx.y*BY + threadIdx.y;
     const int p = blockIdx.x*BX + threadIdx.x;
     if (J > L.N[1]) return;
-    const long idx = ((long)I*L.P[1] + J)*L.Hz + p;
+    const int idx = (I*L.P[1] + J)*L.Hz + p;
     for (int h = 0; h < 2; h++)
     {
         const int K = 2*p + h;
@@ -1027,6 +1217,43 @@ void launch_rbgs(const LevelDims& L, const OpView<T>& op, Pair<T> x,
     default:
         hipLaunchKernelGGL((k_rbgs<T, OP_FACES>), g, BLK, 0, s, L, op, xc, xo,
                            bc, colour, rg);
+        break;
+    }
+}
+
+template <typename T>
+void launch_rbgs_slabs(const LevelDims& L, const OpView<T>& op, Pair<T> x,
+                       Pair<const T> b, int d, const int* planes, int nslab,
+                       const int* colours, int nphase, hipStream_t s)
+{
+    SlabArgs sa;
+    sa.d = d;
+    sa.plane[0] = planes[0];
+    sa.plane[1] = nslab > 1 ? planes[1] : planes[0];
+    sa.nphase = nphase;
+    for (int z = 0; z < 2; z++)
+        for (int i = 0; i < 4; i++)
+            sa.colours[z][i] = i < nphase ? colours[(nslab > 1 ? z : 0)*4 + i] : 0;
+    const int a1 = d == 0 ? 1 : 0;
+    const int a2 = d == 2 ? 1 : 2;
+    const int n1 = L.N[a1], n2 = L.N[a2];
+    sa.tt = (n1*n2 >= 128*128) ? 16 : 8;
+    const int W = sa.tt + 2*nphase;
+    const dim3 g((n2 + sa.tt - 1)/sa.tt, (n1 + sa.tt - 1)/sa.tt, nslab);
+    const size_t lds = size_t(W)*W*sizeof(T);
+    switch (op.kind)
+    {
+    case OP_STENCIL:
+        hipLaunchKernelGGL((k_rbgs_slabs<T, OP_STENCIL>), g, dim3(256), lds, s,
+                           L, op, x, b, sa);
+        break;
+    case OP_CONST:
+        hipLaunchKernelGGL((k_rbgs_slabs<T, OP_CONST>), g, dim3(256), lds, s,
+                           L, op, x, b, sa);
+        break;
+    default:
+        hipLaunchKernelGGL((k_rbgs_slabs<T, OP_FACES>), g, dim3(256), lds, s,
+                           L, op, x, b, sa);
         break;
     }
 }
@@ -1210,9 +1437,14 @@ void launch_coarse_solve(const LevelDims& L, const OpView<T>& op, Pair<T> x,
                          int n, int level0, const double* bcval,
                          hipStream_t s)
 {
-    hipLaunchKernelGGL((k_coarse_solve<T>), dim3(1), dim3(256),
-                       size_t(n)*sizeof(double), s, L, op, x, b, lu, piv, n,
-                       level0, bcvals(bcval));
+    if (size_t(n)*(n + 1)*sizeof(double) <= 56*1024)
+        hipLaunchKernelGGL((k_coarse_solve<T, true>), dim3(1), dim3(256),
+                           size_t(n)*(n + 1)*sizeof(double), s, L, op, x, b,
+                           lu, piv, n, level0, bcvals(bcval));
+    else
+        hipLaunchKernelGGL((k_coarse_solve<T, false>), dim3(1), dim3(256),
+                           size_t(n)*sizeof(double), s, L, op, x, b, lu, piv,
+                           n, level0, bcvals(bcval));
 }
 
 template <typename T>
@@ -1254,8 +1486,14 @@ void launch_norm_factor_terms(const LevelDims& L, const OpView<T>& op,
 void launch_reduce_final(const double* partials, int n, double* out,
                          hipStream_t s)
 {
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(256), 0, s, partials, n,
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(1024), 0, s, partials, n,
                        out);
+}
+
+void launch_diag(const LevelDims& L, Pair<const double> a7, Pair<double> x,
+                 Pair<const double> b, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_diag, grid_interior(L), BLK, 0, s, L, a7, x, b);
 }
 
 void launch_add_correction(const LevelDims& L, Pair<double> x,
@@ -1282,6 +1520,10 @@ void launch_set_cell(Pair<T> f, long idx, int colour, T v, hipStream_t s)
     template void launch_rbgs<T>(const LevelDims&, const OpView<T>&, Pair<T>, \
                                  Pair<const T>, int, const Region&,           \
                                  hipStream_t);                                \
+    template void launch_rbgs_slabs<T>(const LevelDims&, const OpView<T>&,    \
+                                       Pair<T>, Pair<const T>, int,           \
+                                       const int*, int, const int*, int,      \
+                                       hipStream_t);                          \
     template void launch_jacobi<T>(const LevelDims&, const OpView<T>&,        \
                                    Pair<T>, Pair<const T>, Pair<T>,           \
                                    const Region&, hipStream_t);               \

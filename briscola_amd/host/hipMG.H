@@ -1,0 +1,233 @@
+// hipMG.H -- C++ face of the MI355X multigrid solver with the interface of
+// Briscola's MG linear solver, for host code that calls it the way Briscola
+// calls its solvers:
+//
+//   solver<stencil,scalar,colocated>::solve(linearSystem&, bool constMatrix)
+//   (src/briscolaFiniteVolume/solvers/solver/solver.H:280-284,
+//    solvers/MG/MG.H:135-139, MG.C:402-483)
+//
+// Header-only; everything numerical happens behind the C ABI of bmg.h.
+//
+// Mirrored semantics:
+//   * the dictionary keys and defaults of MG (MG.C:257-304) and solver
+//     (relTol and tolerance required, minIter 0, maxIter 999; solver.C:
+//     114-123);
+//   * solve() eliminates the Neumann/Dirichlet ghosts into A and the level-0
+//     inhomogeneous terms into b, pins the coarsest level of singular systems
+//     and overwrites x (MG.C:431-481); the mutated A and b are copied back
+//     into the linearSystem, as the reference mutates the caller's objects;
+//   * the coarse factorisation is redone unless constMatrix (MG.C:449-459);
+//   * diagonal systems are solved by one diagonal sweep (MG.C:408-428);
+//   * printSolverStats line "MG: Solving for colocated <name>, initial
+//     residual = ..., final residual = ..., nIter = N" (solver.C:97-112);
+//   * non-convergence is silent; configuration errors raise FatalError (the
+//     reference aborts through FatalErrorInFunction).
+#ifndef BRISCOLA_AMD_HIPMG_H
+#define BRISCOLA_AMD_HIPMG_H
+
+#include "bmg.h"
+
+#include <cstdio>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace briscola_amd
+{
+
+struct FatalError : std::runtime_error
+{
+    using std::runtime_error::runtime_error;
+};
+
+// The subset of an OpenFOAM dictionary MG reads: word/scalar/label entries.
+class dictionary
+{
+    std::map<std::string, std::string> e_;
+
+public:
+    dictionary() = default;
+    dictionary(std::initializer_list<std::pair<const std::string, std::string>> l)
+    : e_(l) {}
+
+    void add(const std::string& k, const std::string& v) { e_[k] = v; }
+    bool found(const std::string& k) const { return e_.count(k) > 0; }
+
+    double lookupScalar(const std::string& k) const
+    {
+        auto it = e_.find(k);
+        if (it == e_.end())
+            throw FatalError("keyword " + k + " is undefined in dictionary");
+        return std::stod(it->second);
+    }
+    int lookupOrDefault(const std::string& k, int d) const
+    {
+        auto it = e_.find(k);
+        return it == e_.end() ? d : std::stoi(it->second);
+    }
+    std::string lookupOrDefault(const std::string& k, const char* d) const
+    {
+        auto it = e_.find(k);
+        return it == e_.end() ? std::string(d) : it->second;
+    }
+};
+
+// What MG reads from fvMesh: level-0 shape, rectilinear vertex coordinates
+// and the boundary condition of every face of x (-x,+x,-y,+y,-z,+z).
+struct fvMesh
+{
+    int N[3];
+    std::vector<double> vertices[3];
+    int bc[6];                 // BMG_NEUMANN / BMG_DIRICHLET / BMG_PERIODIC
+    double bcValue[6] = {0, 0, 0, 0, 0, 0};
+};
+
+// linearSystem<stencil,scalar,colocated>: unknown x, deep stencil field A
+// (every level, ghost-padded [C,L,R,B,T,A,F]) and source b (level 0).
+struct linearSystem
+{
+    std::string name = "p";
+    std::vector<std::vector<double>> A;   // A[level]: (Nx+2)(Ny+2)(Nz+2)*7
+    std::vector<double> b;                // level 0, ghost-padded
+    std::vector<double> x;                // level 0, ghost-padded
+    bool symmetric = true;
+    bool diagonal = false;
+};
+
+struct solverStats
+{
+    int nIter = 0;
+    double initialResidual = 0.0;
+    double finalResidual = 0.0;
+};
+
+class solver
+{
+public:
+    virtual ~solver() = default;
+    virtual void solve(linearSystem& sys, bool constMatrix = false) = 0;
+};
+
+class MG : public solver
+{
+    bmg_ctx* ctx_ = nullptr;
+    bmg_opts opts_;
+    std::vector<double> verts_[3];
+    bool operatorSet_ = false;
+    bool verbose_;
+    solverStats last_;
+
+    static void check(int code)
+    {
+        if (code != BMG_OK) throw FatalError(bmg_last_error());
+    }
+
+public:
+    static constexpr const char* typeName = "MG";
+
+    // MG(const dictionary&, const fvMesh&) -- MG.C:257-399
+    MG(const dictionary& dict, const fvMesh& mesh, bool verbose = true)
+    : verbose_(verbose)
+    {
+        bmg_default_opts(&opts_);
+        opts_.rel_tol = dict.lookupScalar("relTol");
+        opts_.tolerance = dict.lookupScalar("tolerance");
+        opts_.min_iter = dict.lookupOrDefault("minIter", 0);
+        opts_.max_iter = dict.lookupOrDefault("maxIter", 999);
+        opts_.n_sweeps_pre = dict.lookupOrDefault("nSweepsPre", 0);
+        opts_.n_sweeps_post = dict.lookupOrDefault("nSweepsPost", 2);
+        const std::string cyc = dict.lookupOrDefault("cycleType", "F");
+        if (cyc == "V") opts_.cycle = BMG_VCYCLE;
+        else if (cyc == "W") opts_.cycle = BMG_WCYCLE;
+        else if (cyc == "F") opts_.cycle = BMG_FCYCLE;
+        else throw FatalError("Unknown cycleType " + cyc);
+        const std::string cm = dict.lookupOrDefault("coarseMode", "direct");
+        if (cm == "direct") opts_.coarse_mode = BMG_COARSE_DIRECT;
+        else if (cm == "smooth") opts_.coarse_mode = BMG_COARSE_SMOOTH;
+        else throw FatalError("Unknown coarseMode " + cm);
+        opts_.coarse_level = dict.lookupOrDefault("coarseLevel", 0);
+        const std::string sm = dict.lookupOrDefault("smoother", "rbgs");
+        if (sm == "rbgs") opts_.smoother = BMG_RBGS;
+        else if (sm == "jac") opts_.smoother = BMG_JAC;
+        else throw FatalError("Unsupported smoother type " + sm);
+        if (dict.lookupOrDefault("prolong", "linear") != "linear" ||
+            dict.lookupOrDefault("restrict", "linear") != "linear")
+            throw FatalError("only the linear transfer schemes are provided");
+        const std::string prec = dict.lookupOrDefault("precision", "FP64");
+        opts_.precision = prec == "mixed" ? BMG_MIXED : BMG_FP64;
+
+        bmg_grid g;
+        for (int d = 0; d < 3; d++)
+        {
+            g.n[d] = mesh.N[d];
+            verts_[d] = mesh.vertices[d];
+            g.vertices[d] = verts_[d].data();
+        }
+        for (int f = 0; f < 6; f++)
+        {
+            g.bc[f] = mesh.bc[f];
+            g.bc_value[f] = mesh.bcValue[f];
+        }
+        check(bmg_create(&ctx_, &g, &opts_));
+    }
+
+    ~MG() override { if (ctx_) bmg_destroy(ctx_); }
+    MG(const MG&) = delete;
+    MG& operator=(const MG&) = delete;
+
+    int nLevels() const { return bmg_num_levels(ctx_); }
+
+    std::vector<int> levelShape(int l) const
+    {
+        int n[3];
+        check(bmg_level_shape(ctx_, l, n));
+        return {n[0], n[1], n[2]};
+    }
+
+    const solverStats& lastStats() const { return last_; }
+
+    // MG::solve (MG.C:402-483)
+    void solve(linearSystem& sys, bool constMatrix = false) override
+    {
+        const int L = nLevels();
+        if (sys.diagonal)
+        {
+            check(bmg_solve_diagonal(ctx_, sys.A.at(0).data(), sys.b.data(),
+                                     sys.x.data()));
+            last_ = {0, 1.0, 0.0};
+            print("diagonal", sys.name);
+            return;
+        }
+        if (int(sys.A.size()) != L)
+            throw FatalError("linearSystem must hold A on every level");
+        if (!constMatrix || !operatorSet_)
+        {
+            for (int l = 0; l < L; l++)
+                check(bmg_set_operator_stencil(ctx_, l, sys.A[l].data(),
+                                               sys.symmetric));
+            operatorSet_ = true;
+        }
+        bmg_stats st;
+        check(bmg_solve(ctx_, sys.b.data(), sys.x.data(), constMatrix, &st));
+        // the reference eliminates ghosts and pins the coarse level in place
+        for (int l = 0; l < L; l++)
+            check(bmg_get_operator_stencil(ctx_, l, sys.A[l].data()));
+        last_ = {st.iters, st.initial_residual, st.final_residual};
+        print(typeName, sys.name);
+    }
+
+private:
+    void print(const char* solverName, const std::string& field) const
+    {
+        if (!verbose_) return;
+        std::printf("%s: Solving for colocated %s, initial residual = %g, "
+                    "final residual = %g, nIter = %d\n", solverName,
+                    field.c_str(), last_.initialResidual, last_.finalResidual,
+                    last_.nIter);
+    }
+};
+
+}  // namespace briscola_amd
+
+#endif

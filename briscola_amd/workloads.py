@@ -132,15 +132,22 @@ def channel(n=256, seed=0, beta=2.0):
     Lx, Lz = 4.0 * math.pi, 2.0 * math.pi
     modes = _modes(8)
     ph = uniform01(CASE_CHANNEL, seed, 1, 2 * len(modes)) * (2.0 * math.pi)
-    # f = sum_m a_m cos(kx x + phx) cos(pi my (y+1)/2) cos(kz z + phz)
-    # evaluated as a separable sum over (mx, mz) per my.
-    f = np.zeros((nx, ny, nz))
-    for q, (mx, my, mz) in enumerate(modes):
-        amp = (mx * mx + my * my + mz * mz) ** (-5.0 / 6.0)
-        cx = np.cos(2.0 * math.pi * mx * xc / Lx + ph[2 * q])
-        cy = np.cos(math.pi * my * (yc + 1.0) / 2.0)
-        cz = np.cos(2.0 * math.pi * mz * zc / Lz + ph[2 * q + 1])
-        f += amp * (cx[:, None, None] * cy[None, :, None]) * cz[None, None, :]
+    # f = sum_m a_m cos(kx x + phx) cos(pi my (y+1)/2) cos(kz z + phz),
+    # summed per my as G_my(x,z) = CX^T diag(a) CZ, then over my.
+    G = np.zeros((9, nx, nz))
+    amp = np.array([(mx * mx + my * my + mz * mz) ** (-5.0 / 6.0)
+                    for (mx, my, mz) in modes])
+    for my in range(9):
+        sel = [q for q, m in enumerate(modes) if m[1] == my]
+        if not sel:
+            continue
+        CX = np.stack([np.cos(2.0 * math.pi * modes[q][0] * xc / Lx + ph[2 * q])
+                       for q in sel])
+        CZ = np.stack([np.cos(2.0 * math.pi * modes[q][2] * zc / Lz + ph[2 * q + 1])
+                       for q in sel])
+        G[my] = CX.T @ (amp[sel][:, None] * CZ)
+    CY = np.stack([np.cos(math.pi * my * (yc + 1.0) / 2.0) for my in range(9)])
+    f = np.einsum("mxz,my->xyz", G, CY, optimize=True)
     f += 1e-3 * uniform_pm1(CASE_CHANNEL, seed, 2, nx * ny * nz).reshape(nx, ny, nz)
     dx, dy, dz = (np.diff(v) for v in verts)
     vol = dx[:, None, None] * dy[None, :, None] * dz[None, None, :]

@@ -146,6 +146,12 @@ int bmg_solve(bmg_ctx* ctx, double* b, double* x, int const_matrix,
 int bmg_solve_device(bmg_ctx* ctx, double* d_b, double* d_x,
                      int const_matrix, bmg_stats* stats);
 
+/* MG::solve of a diagonal system (sys.diagonal(), MG.C:404-428): level-0
+ * stencil A7 (reference layout, only the centre is used after ghost
+ * elimination), x = b / A_C on every interior cell, then the level-0
+ * boundary update.  b and x as in bmg_solve. */
+int bmg_solve_diagonal(bmg_ctx* ctx, const double* A7, double* b, double* x);
+
 /* ---- Level operations for kernel-level parity tests.  Fields are host
  * arrays in reference layout; 'which' is 0 = x, 1 = b, 2 = r. ---- */
 int bmg_upload_field(bmg_ctx* ctx, int which, int level, const double* v);
@@ -171,15 +177,19 @@ int bmg_op_cycle(bmg_ctx* ctx);
 /* sum of r[0]^2 over interior cells. */
 int bmg_op_sum_sq_residual(bmg_ctx* ctx, double* out);
 
-/* ---- Measurement hook (no counterpart in the reference): while enabled,
- * every launch of the red-black smoother kernel on 'level' is bracketed by
- * HIP events on the context's stream.  bmg_profile_read returns the summed
- * device time of those launches, their number, and their algorithmic HBM
- * bytes (cells updated x bytes per update of the operator representation:
- * b, the other colour of x and the write of x, plus the coefficients). ---- */
-int bmg_profile_smoother(bmg_ctx* ctx, int enable, int level);
-int bmg_profile_read(bmg_ctx* ctx, double* total_ms, long long* launches,
-                     double* bytes);
+/* ---- Measurement hooks (no counterpart in the reference). ----
+ * bmg_time_smoother: launches the red-black phase kernel of 'level' (the
+ * interior region, colours alternating) 'reps' times between two HIP events
+ * on the context's stream and returns the average launch duration and the
+ * algorithmic HBM bytes of one launch (cells updated x bytes per update of
+ * the operator representation: b, the other colour of x, the write of x,
+ * plus stored coefficients).  precision: BMG_FP64 or BMG_MIXED (FP32 arrays).
+ * Needs bmg_op_prepare or a solve first; overwrites x of that level. */
+int bmg_time_smoother(bmg_ctx* ctx, int level, int precision, int reps,
+                      double* avg_ms, double* bytes);
+/* Replay each MG iteration as a captured HIP graph (default on; the
+ * environment variable BMG_NO_GRAPHS=1 turns it off at creation). */
+int bmg_set_graphs(bmg_ctx* ctx, int enable);
 
 #ifdef __cplusplus
 }

@@ -79,6 +79,7 @@ def _load():
         "oracle_norm_factor": (None, [P, D]),
         "oracle_solve": (None, [P, c_int, D]),
         "oracle_coarse_dense": (c_int, [P, D, D, I]),
+        "oracle_solve_diagonal": (None, [P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -218,6 +219,9 @@ class Oracle:
         st = np.zeros(3)
         self.lib.oracle_solve(self.ptr, int(const_matrix), _dp(st))
         return int(st[0]), float(st[1]), float(st[2])
+
+    def solve_diagonal(self):
+        self.lib.oracle_solve_diagonal(self.ptr)
 
     def coarse_dense(self):
         l = self.nlevels - 1

@@ -1325,6 +1325,23 @@ void oracle_solve(oracle_ctx* c, int const_matrix, double stats[3])
     stats[2] = cur;
 }
 
+/* MG::solve for a diagonal system (MG.C:404-428): ghost elimination, one
+ * diagonalSmoother::Sweep (x = b/A_C, diagonalSmoother.C:13-69) and the
+ * level-0 boundary update. */
+void oracle_solve_diagonal(oracle_ctx* c)
+{
+    eliminateGhosts(*c);
+    Level& lv = c->L[0];
+    for (int i = 1; i <= lv.N[0]; i++)
+    for (int j = 1; j <= lv.N[1]; j++)
+    for (int k = 1; k <= lv.N[2]; k++)
+    {
+        const long p = lv.lin(i, j, k);
+        lv.x[p] = lv.b[p]/lv.A[7*p];
+    }
+    correctBoundaryConditions(*c, 0, lv, lv.x);
+}
+
 int oracle_coarse_dense(const oracle_ctx* c, double* dense, double* lu,
                         int* pivots)
 {

@@ -79,6 +79,9 @@ void oracle_correct_eliminated(oracle_ctx*);  /* x[0].correct<eliminatedBcs> */
 double oracle_sum_sq_residual(const oracle_ctx*);
 void oracle_norm_factor(const oracle_ctx*, double* nf);
 
+/* MG::solve of a diagonal system (MG.C:404-428). */
+void oracle_solve_diagonal(oracle_ctx*);
+
 /* Whole MG::solve: stats = {iters, initial, final}. */
 void oracle_solve(oracle_ctx*, int const_matrix, double stats[3]);
 

@@ -1,0 +1,92 @@
+// Drives the C++ adapter (briscola_amd/host/hipMG.H) the way a Briscola
+// caller drives MG: build fvMesh and the solver dictionary, fill a
+// linearSystem (A on every level, b, x) and call solve(sys, constMatrix).
+//
+//   test_hipmg <dir>
+// reads  dir/shape.txt   "nx ny nz nlevels bc0..bc5 relTol coarseLevel"
+//        dir/vertices.bin, dir/A<l>.bin, dir/b.bin
+// writes dir/x.bin, dir/b_out.bin, dir/A0_out.bin, dir/stats.txt and prints
+// the solver statistics line.
+#include "hipMG.H"
+
+#include <cstdio>
+#include <fstream>
+#include <string>
+#include <vector>
+
+using namespace briscola_amd;
+
+static std::vector<double> readBin(const std::string& p, size_t n)
+{
+    std::vector<double> v(n);
+    std::ifstream f(p, std::ios::binary);
+    f.read(reinterpret_cast<char*>(v.data()), n*sizeof(double));
+    if (!f) throw FatalError("cannot read " + p);
+    return v;
+}
+
+static void writeBin(const std::string& p, const std::vector<double>& v)
+{
+    std::ofstream f(p, std::ios::binary);
+    f.write(reinterpret_cast<const char*>(v.data()), v.size()*sizeof(double));
+}
+
+int main(int argc, char** argv)
+{
+    if (argc != 2) { std::fprintf(stderr, "usage: test_hipmg <dir>\n"); return 2; }
+    const std::string dir = argv[1];
+    std::ifstream sh(dir + "/shape.txt");
+    fvMesh mesh;
+    int nlev = 0;
+    double relTol = 0;
+    int coarseLevel = 0;
+    sh >> mesh.N[0] >> mesh.N[1] >> mesh.N[2] >> nlev;
+    for (int f = 0; f < 6; f++) sh >> mesh.bc[f];
+    sh >> relTol >> coarseLevel;
+    std::vector<std::vector<int>> shapes(nlev, std::vector<int>(3));
+    for (int l = 0; l < nlev; l++) sh >> shapes[l][0] >> shapes[l][1] >> shapes[l][2];
+    const size_t nv = size_t(mesh.N[0] + 1) + mesh.N[1] + 1 + mesh.N[2] + 1;
+    std::vector<double> v = readBin(dir + "/vertices.bin", nv);
+    size_t o = 0;
+    for (int d = 0; d < 3; d++)
+    {
+        mesh.vertices[d].assign(v.begin() + o, v.begin() + o + mesh.N[d] + 1);
+        o += mesh.N[d] + 1;
+    }
+    dictionary dict{{"relTol", std::to_string(relTol)}, {"tolerance", "0"},
+                    {"cycleType", "V"}, {"nSweepsPre", "2"},
+                    {"nSweepsPost", "2"},
+                    {"coarseLevel", std::to_string(coarseLevel)}};
+    try
+    {
+        MG mg(dict, mesh);
+        if (mg.nLevels() != nlev) throw FatalError("level count differs");
+        linearSystem sys;
+        for (int l = 0; l < nlev; l++)
+        {
+            const size_t n = size_t(shapes[l][0] + 2)*(shapes[l][1] + 2)*(shapes[l][2] + 2);
+            sys.A.push_back(readBin(dir + "/A" + std::to_string(l) + ".bin", 7*n));
+        }
+        const size_t n0 = size_t(mesh.N[0] + 2)*(mesh.N[1] + 2)*(mesh.N[2] + 2);
+        sys.b = readBin(dir + "/b.bin", n0);
+        sys.x.assign(n0, 0.0);
+        mg.solve(sys, false);
+        writeBin(dir + "/x.bin", sys.x);
+        writeBin(dir + "/b_out.bin", sys.b);
+        writeBin(dir + "/A0_out.bin", sys.A[0]);
+        std::ofstream st(dir + "/stats.txt");
+        st << mg.lastStats().nIter << " " << mg.lastStats().initialResidual
+           << " " << mg.lastStats().finalResidual << "\n";
+        // a wrong dictionary is a FatalError, as in the reference
+        bool threw = false;
+        try { MG bad(dictionary{{"relTol", "1e-6"}}, mesh); }
+        catch (const FatalError&) { threw = true; }
+        if (!threw) throw FatalError("missing tolerance was accepted");
+    }
+    catch (const FatalError& e)
+    {
+        std::fprintf(stderr, "FatalError: %s\n", e.what());
+        return 1;
+    }
+    return 0;
+}

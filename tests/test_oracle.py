@@ -1,0 +1,579 @@
+"""Pins the CPU oracle (the parity checker) to the reference semantics.
+
+Every oracle function is checked against something other than its own
+formula: the reference's test suite invariants (restriction/prolongation
+reproduce constants and linear functions, Test-restriction.C and
+Test-prolongation.C; boundary ghosts, Test-boundaryConditions.C; metrics,
+Test-metrics.C; dense coarse solve vs numpy, externalSolver/check.py), closed
+forms of the finite-volume operator (Test-FFTPoissonSolver.C:28-79),
+an independent pure-Python restatement of the red-black phase order on tiny
+grids, and FP64 direct solvers for whole solves.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import direct
+from briscola_amd import workloads as W
+
+N, D, P = oracle.NEUMANN, oracle.DIRICHLET, oracle.PERIODIC
+
+
+def stretched(n, beta=1.5):
+    j = np.arange(n + 1)
+    return np.tanh(beta * (2.0 * j / n - 1.0)) / math.tanh(beta)
+
+
+def make(verts, bc, lam=None, **opts):
+    o = oracle.Oracle(verts, bc)
+    o.set_opts(**opts)
+    o.assemble(lam)
+    return o
+
+
+def inner(a):
+    return a[1:-1, 1:-1, 1:-1]
+
+
+# ---------------------------------------------------------------- hierarchy
+
+def test_level_shapes_follow_coarsen():
+    o = make([np.linspace(0, 1, 65)] * 3, (N,) * 6)
+    assert o.shapes == [(64,) * 3, (32,) * 3, (16,) * 3, (8,) * 3, (4,) * 3, (2,) * 3]
+    o = make([np.linspace(0, 1, q + 1) for q in (64, 32, 128)], (N,) * 6)
+    assert o.shapes == [(64, 32, 128), (32, 16, 64), (16, 8, 32), (8, 4, 16),
+                        (4, 2, 8), (2, 2, 4), (2, 2, 2)]
+    assert o.ratios[-1] == (1, 1, 2)
+    o = make([np.linspace(0, 1, 193)] * 3, (N,) * 6)
+    assert [s[0] for s in o.shapes] == [192, 96, 48, 24, 12, 6, 3]
+
+
+# ------------------------------------------------------------------ metrics
+
+def test_metrics_box():
+    """Test-metrics.C: centres/widths of a box match the analytic values."""
+    v = [np.linspace(0.0, 2.0, 9), np.linspace(-1.0, 1.0, 5), np.linspace(0.0, 0.5, 3)]
+    o = make(v, (N,) * 6)
+    for d in range(3):
+        h = v[d][1] - v[d][0]
+        n = len(v[d]) - 1
+        cc = o.centres(0, d)
+        expect = v[d][0] + h * (np.arange(n + 2) - 0.5)
+        assert np.allclose(cc, expect, rtol=0, atol=1e-12)
+        assert np.allclose(o.widths(0, d), h, rtol=1e-12)
+        # coarse level: every second vertex
+        cc1 = o.centres(1, d)
+        h1 = 2 * h if n > 3 else h
+        n1 = n // 2 if n > 3 else n
+        assert np.allclose(cc1, v[d][0] + h1 * (np.arange(n1 + 2) - 0.5), atol=1e-12)
+
+
+def test_periodic_ghost_width():
+    """levelPoints.C:389-401: periodic ghosts take the opposite cell width."""
+    vy = stretched(8)
+    o = make([np.linspace(0, 1, 9), vy, np.linspace(0, 1, 9)], (N, N, P, P, N, N))
+    w = o.widths(0, 1)
+    assert np.isclose(w[0], vy[-1] - vy[-2], rtol=1e-12)
+    assert np.isclose(w[-1], vy[1] - vy[0], rtol=1e-12)
+    o = make([np.linspace(0, 1, 9), vy, np.linspace(0, 1, 9)], (N,) * 6)
+    w = o.widths(0, 1)
+    assert np.isclose(w[0], vy[1] - vy[0], rtol=1e-12)      # mirrored
+
+
+# ------------------------------------------------------------------ operator
+
+def test_uniform_operator_closed_form():
+    """Uniform h: off-diagonals h, centre -6h; Neumann adds +h per face and
+    Dirichlet -h per face after elimination (A.1/A.2)."""
+    n = 8
+    h = 1.0 / n
+    o = make([np.linspace(0, 1, n + 1)] * 3, (N, D, N, N, D, N))
+    A = o.stencil(0)
+    Ai = inner(A)
+    assert np.allclose(Ai[1:-1, 1:-1, 1:-1, 1:], h, rtol=1e-14)
+    assert np.allclose(Ai[..., 0], -6 * h, rtol=1e-14)
+    o.pin_and_eliminate()
+    Ae = inner(o.stencil(0))
+    # cell (0,0,0): Neumann -x, Neumann -y, Dirichlet -z
+    assert Ae[0, 0, 0, 0] == -6 * h + h + h - h
+    assert Ae[0, 0, 0, 1] == 0 and Ae[0, 0, 0, 3] == 0 and Ae[0, 0, 0, 5] == 0
+    # cell (n-1, 3, n-1): Dirichlet +x, Neumann +z
+    assert Ae[n - 1, 3, n - 1, 0] == -6 * h - h + h
+    assert Ae[n - 1, 3, n - 1, 2] == 0 and Ae[n - 1, 3, n - 1, 6] == 0
+    # interior cell untouched
+    assert Ae[3, 3, 3, 0] == -6 * h
+    # not singular (Dirichlet present): no pin on the coarse level
+    assert not o.singular()
+
+
+def test_stretched_operator_matches_explicit_stencil():
+    """Test-FFTPoissonSolver.C:28-79: on a stretched rectilinear mesh the
+    assembled operator divided by the cell volume is the explicit stencil
+    (p_{i-1}-p_i)/(dxw dx_i) - ... with dxw the centre distance."""
+    vx = np.linspace(0, 4 * math.pi, 9)
+    vy = stretched(8, 2.0)
+    vz = np.linspace(0, 2 * math.pi, 7)
+    o = make([vx, vy, vz], (P, P, N, N, P, P))
+    A = o.stencil(0)
+    dx, dy, dz = np.diff(vx), np.diff(vy), np.diff(vz)
+    yc = 0.5 * (vy[:-1] + vy[1:])
+    V = dx[:, None, None] * dy[None, :, None] * dz[None, None, :]
+    Ai = inner(A)
+    for j in range(1, len(dy) - 1):
+        dys = yc[j] - yc[j - 1]
+        dyn = yc[j + 1] - yc[j]
+        expect_B = 1.0 / (dys * dy[j])
+        expect_T = 1.0 / (dyn * dy[j])
+        assert np.allclose(Ai[:, j, :, 3] / V[:, j, :], expect_B, rtol=1e-12)
+        assert np.allclose(Ai[:, j, :, 4] / V[:, j, :], expect_T, rtol=1e-12)
+    expect_L = 1.0 / (dx[0] * dx[0])
+    assert np.allclose(Ai[..., 1] / V, expect_L, rtol=1e-12)
+    # symmetry A_R(i) = A_L(i+1) exactly, interior
+    assert np.array_equal(Ai[:-1, :, :, 2], Ai[1:, :, :, 1])
+    assert np.array_equal(Ai[:, :-1, :, 4], Ai[:, 1:, :, 3])
+    # rows sum to ~0 (Laplacian of a constant)
+    rs = Ai.sum(axis=-1)
+    assert np.abs(rs).max() <= 1e-13 * np.abs(Ai[..., 0]).max()
+
+
+def test_lambda_operator_and_area_weighted_restriction():
+    """Face coefficients area*delta*lambda; coarse lambda is the
+    area-weighted mean of the fine faces (areaWeightedFaceRestriction)."""
+    n = 8
+    v = [np.linspace(0, 1, n + 1)] * 3
+    rng = np.random.default_rng(3)
+    lam = [rng.uniform(0.5, 2.0, (n + 2,) * 3) for _ in range(3)]
+    o = make(v, (N,) * 6, lam=lam)
+    h = 1.0 / n
+    A0 = inner(o.stencil(0))
+    # lower x face of interior cell (i,j,k): lam[0][i+1, j+1, k+1]
+    assert np.allclose(A0[..., 1], h * inner(lam[0]), rtol=1e-14)
+    assert np.allclose(A0[..., 4], h * lam[1][1:-1, 2:, 1:-1], rtol=1e-14)
+    # coarse level: uniform grid -> plain mean of the 4 fine faces
+    A1 = inner(o.stencil(1))
+    lx = lam[0]
+    coarse = 0.25 * (lx[1:-1:2, 1:-1:2, 1:-1:2] + lx[1:-1:2, 2::2, 1:-1:2]
+                     + lx[1:-1:2, 1:-1:2, 2::2] + lx[1:-1:2, 2::2, 2::2])
+    assert np.allclose(A1[..., 1], 2 * h * coarse, rtol=1e-13)
+
+
+# --------------------------------------------------------- boundary ghosts
+
+def test_boundary_ghosts():
+    """Test-boundaryConditions.C:26-157 plus periodic/edge rules: Dirichlet(0)
+    ghost = -interior, Neumann(0) ghost = interior, periodic ghost = opposite
+    interior, unset edge/corner ghosts copy the adjacent interior cell."""
+    n = 6
+    o = make([np.linspace(0, 1, n + 1)] * 3, (D, N, P, P, N, D))
+    rng = np.random.default_rng(1)
+    for l in range(o.nlevels):
+        x = rng.uniform(-1, 1, o.padded(l))
+        o.set_field(oracle.X, l, x)
+        o.correct_bcs(l)
+        g = o.field(oracle.X, l)
+        m = o.shapes[l]
+        assert np.array_equal(g[0, 1:-1, 1:-1], -x[1, 1:-1, 1:-1])          # Dirichlet -x
+        assert np.array_equal(g[-1, 1:-1, 1:-1], x[-2, 1:-1, 1:-1])         # Neumann +x
+        assert np.array_equal(g[1:-1, 0, 1:-1], x[1:-1, m[1], 1:-1])        # periodic -y
+        assert np.array_equal(g[1:-1, -1, 1:-1], x[1:-1, 1, 1:-1])          # periodic +y
+        assert np.array_equal(g[1:-1, 1:-1, 0], x[1:-1, 1:-1, 1])           # Neumann -z
+        assert np.array_equal(g[1:-1, 1:-1, -1], -x[1:-1, 1:-1, -2])        # Dirichlet +z
+        # edge (-x,-y): x not periodic -> unset -> copy of cell (0,0,k)
+        assert np.array_equal(g[0, 0, 1:-1], x[1, 1, 1:-1])
+        # edge (-y,+y)-free corner (+x,+y,+z) -> copy of the corner cell
+        assert g[-1, -1, -1] == x[-2, -2, -2]
+
+
+def test_periodic_edges_and_corners():
+    n = 4
+    o = make([np.linspace(0, 1, n + 1)] * 3, (P,) * 6)
+    x = np.random.default_rng(2).uniform(-1, 1, o.padded(0))
+    o.set_field(oracle.X, 0, x)
+    o.correct_bcs(0)
+    g = o.field(oracle.X, 0)
+    assert g[0, 0, 0] == x[n, n, n]
+    assert np.array_equal(g[0, 0, 1:-1], x[n, n, 1:-1])
+    assert g[-1, 0, -1] == x[1, n, 1]
+
+
+# ------------------------------------------------------------------- residual
+
+def test_residual_dense():
+    """r = b - A x with the row product in C,L,R,B,T,A,F order, bitwise."""
+    v = [np.linspace(0, 1, 7), stretched(6), np.linspace(0, 2, 9)]
+    o = make(v, (P, P, N, D, N, N))
+    o.pin_and_eliminate()
+    rng = np.random.default_rng(4)
+    x = rng.uniform(-1, 1, o.padded(0))
+    b = rng.uniform(-1, 1, o.padded(0))
+    o.set_field(oracle.X, 0, x)
+    o.set_field(oracle.B, 0, b)
+    o.residual(0, corr_bcs=True)
+    r = inner(o.field(oracle.R, 0))
+    xg = o.field(oracle.X, 0)           # periodic ghosts refreshed by the call
+    A = inner(o.stencil(0))
+    c = xg[1:-1, 1:-1, 1:-1]
+    terms = [c, xg[:-2, 1:-1, 1:-1], xg[2:, 1:-1, 1:-1], xg[1:-1, :-2, 1:-1],
+             xg[1:-1, 2:, 1:-1], xg[1:-1, 1:-1, :-2], xg[1:-1, 1:-1, 2:]]
+    acc = A[..., 0] * terms[0]
+    for s in range(1, 7):
+        acc = acc + A[..., s] * terms[s]
+    assert np.array_equal(r, inner(b) - acc)
+    # periodic ghosts equal the opposite interior after the BC update
+    assert np.array_equal(xg[0, 1:-1, 1:-1], xg[-2, 1:-1, 1:-1])
+
+
+# --------------------------------------------------------------- smoothers
+
+def py_rbgs_smooth(A, x, b, bc, sweeps):
+    """Independent restatement of rbgsSmoother::Smooth on a padded block:
+    interior region padded on periodic faces, four counter-parity phases,
+    periodic ghost fill, width-1 boundary slabs (rbgsSmoother.C:14-437)."""
+    Np = x.shape
+    n = [s - 2 for s in Np]
+
+    def phases(S, E):
+        M = [E[d] - S[d] for d in range(3)]
+        for phase in range(4):
+            fwd = phase < 2
+            want = phase % 2
+            for i in range(M[0]):
+                for j in range(M[1]):
+                    for k in range(M[2]):
+                        if (i + j + k) % 2 != want:
+                            continue
+                        if fwd:
+                            I, J, K = S[0] + i, S[1] + j, S[2] + k
+                        else:
+                            I, J, K = E[0] - 1 - i, E[1] - 1 - j, E[2] - 1 - k
+                        a = A[I, J, K]
+                        p = (a[1] * x[I - 1, J, K] + a[2] * x[I + 1, J, K]
+                             + a[3] * x[I, J - 1, K] + a[4] * x[I, J + 1, K]
+                             + a[5] * x[I, J, K - 1] + a[6] * x[I, J, K + 1])
+                        x[I, J, K] = (b[I, J, K] - p) / a[0]
+
+    per = [bc[f] == P for f in range(6)]
+    for _ in range(sweeps):
+        S = [1 + per[2 * d] for d in range(3)]
+        E = [n[d] + 1 - per[2 * d + 1] for d in range(3)]
+        phases(S, E)
+        # periodic face ghosts
+        for d in range(3):
+            if per[2 * d]:
+                sl_lo = [slice(1, -1)] * 3
+                sl_hi = [slice(1, -1)] * 3
+                src_lo = [slice(1, -1)] * 3
+                src_hi = [slice(1, -1)] * 3
+                sl_lo[d], src_lo[d] = 0, n[d]
+                sl_hi[d], src_hi[d] = n[d] + 1, 1
+                x[tuple(sl_lo)] = x[tuple(src_lo)]
+                x[tuple(sl_hi)] = x[tuple(src_hi)]
+        for f in range(6):
+            if not per[f]:
+                continue
+            d = f // 2
+            S = [1, 1, 1]
+            E = [n[0] + 1, n[1] + 1, n[2] + 1]
+            if f % 2 == 0:
+                E[d] = S[d] + 1
+            else:
+                S[d] = E[d] - 1
+            phases(S, E)
+
+
+@pytest.mark.parametrize("bc", [(N,) * 6, (P, P, N, N, P, P), (D, N, P, P, N, D)],
+                         ids=["neumann", "channel", "mixed"])
+def test_rbgs_matches_python_restatement(bc):
+    v = [np.linspace(0, 1, 7), stretched(4), np.linspace(0, 1, 6)]
+    o = make(v, bc)
+    o.pin_and_eliminate()
+    rng = np.random.default_rng(5)
+    x = rng.uniform(-1, 1, o.padded(0))
+    b = rng.uniform(-1, 1, o.padded(0))
+    o.set_field(oracle.X, 0, x)
+    o.set_field(oracle.B, 0, b)
+    o.smooth(0, 2)
+    xp = x.copy()
+    py_rbgs_smooth(o.stencil(0), xp, b, bc, 2)
+    assert np.array_equal(o.field(oracle.X, 0), xp)
+
+
+def test_rbgs_fixed_point_solves_system():
+    """Iterating the smoother converges to A x = b (Dirichlet, non-singular)."""
+    v = [np.linspace(0, 1, 5), stretched(4), np.linspace(0, 1, 5)]
+    o = make(v, (D,) * 6)
+    o.pin_and_eliminate()
+    rng = np.random.default_rng(6)
+    b = rng.uniform(-1, 1, o.padded(0))
+    o.set_field(oracle.B, 0, b)
+    o.set_field(oracle.X, 0, np.zeros(o.padded(0)))
+    o.smooth(0, 800)
+    o.residual(0, True)
+    r = inner(o.field(oracle.R, 0))
+    assert np.linalg.norm(r) <= 1e-13 * np.linalg.norm(inner(b))
+
+
+def test_rbgs_1d_is_textbook_gauss_seidel():
+    """1 x 1 x n Dirichlet line: red-black order is odd-even Gauss-Seidel on
+    the tridiagonal system."""
+    n = 6
+    o = make([np.array([0.0, 1.0]), np.array([0.0, 1.0]), np.linspace(0, 1, n + 1)],
+             (N, N, N, N, D, D))
+    o.pin_and_eliminate()
+    A = o.stencil(0)[1, 1, 1:-1]
+    rng = np.random.default_rng(8)
+    b = rng.uniform(-1, 1, o.padded(0))
+    x0 = np.zeros(o.padded(0))
+    o.set_field(oracle.B, 0, b)
+    o.set_field(oracle.X, 0, x0)
+    o.smooth(0, 1)
+    got = o.field(oracle.X, 0)[1, 1, 1:-1]
+    # M = (1,1,6), sum even: phases even, odd, odd(idempotent), even
+    y = np.zeros(n + 2)
+    bb = b[1, 1, :]
+    for parity in (0, 1, 1, 0):
+        for k in range(n):
+            if k % 2 == parity:
+                y[k + 1] = (bb[k + 1] - (A[k, 5] * y[k] + A[k, 6] * y[k + 2])) / A[k, 0]
+    assert np.allclose(got, y[1:-1], rtol=1e-15, atol=0)
+
+
+def test_jacobi_sweep():
+    """jacSmoother.C:18-127: y = 0.2 x + 0.8 (b - offdiag)/A0 on the region."""
+    v = [np.linspace(0, 1, 5)] * 3
+    o = make(v, (N,) * 6, smoother=oracle.JAC)
+    o.pin_and_eliminate()
+    rng = np.random.default_rng(9)
+    x = rng.uniform(-1, 1, o.padded(0))
+    b = rng.uniform(-1, 1, o.padded(0))
+    o.set_field(oracle.X, 0, x)
+    o.set_field(oracle.B, 0, b)
+    o.smooth(0, 1)
+    A = inner(o.stencil(0))
+    off = (A[..., 1] * x[:-2, 1:-1, 1:-1] + A[..., 2] * x[2:, 1:-1, 1:-1]
+           + A[..., 3] * x[1:-1, :-2, 1:-1] + A[..., 4] * x[1:-1, 2:, 1:-1]
+           + A[..., 5] * x[1:-1, 1:-1, :-2] + A[..., 6] * x[1:-1, 1:-1, 2:])
+    expect = (1.0 - 0.8) * inner(x) + 0.8 * (inner(b) - off) / A[..., 0]
+    assert np.array_equal(inner(o.field(oracle.X, 0)), expect)
+
+
+# --------------------------------------------------------- grid transfers
+
+@pytest.mark.parametrize("shape", [(8, 8, 8), (8, 16, 4), (12, 6, 3)])
+def test_restriction_reproduces_linear(shape):
+    """Scaled restriction of r = V_f g(x_f), g linear, gives V_c g(x_c)
+    (Test-restriction.C invariants, volume-scaled form used by MG.C:77)."""
+    v = [np.linspace(0, 1, shape[0] + 1), stretched(shape[1]),
+         np.linspace(-1, 2, shape[2] + 1)]
+    o = make(v, (N,) * 6)
+    for l in range(o.nlevels - 1):
+        cf = [o.centres(l, d) for d in range(3)]
+        wf = [o.widths(l, d) for d in range(3)]
+        cc = [o.centres(l + 1, d) for d in range(3)]
+        wc = [o.widths(l + 1, d) for d in range(3)]
+        for g in (lambda x, y, z: 2.0 + 0 * x + 0 * y + 0 * z,
+                  lambda x, y, z: 1.0 + 3 * x,
+                  lambda x, y, z: 0.5 - 2 * y + z):
+            X, Y, Z = np.meshgrid(*cf, indexing="ij")
+            Vf = wf[0][:, None, None] * wf[1][None, :, None] * wf[2][None, None, :]
+            o.set_field(oracle.R, l, Vf * g(X, Y, Z))
+            o.restrict(l)
+            Xc, Yc, Zc = np.meshgrid(*cc, indexing="ij")
+            Vc = wc[0][:, None, None] * wc[1][None, :, None] * wc[2][None, None, :]
+            got = inner(o.field(oracle.B, l + 1))
+            want = inner(Vc * g(Xc, Yc, Zc))
+            # u is rounded to a 1e-12 grid (vertexVector.C:183-188), so on
+            # the stretched axis reproduction is exact to ~1e-12 of the range
+            assert np.allclose(got, want, rtol=0, atol=2e-12 * np.abs(want).max())
+
+
+@pytest.mark.parametrize("shape", [(8, 8, 8), (8, 16, 4)])
+def test_prolongation_reproduces_constants_and_linear(shape):
+    """Test-prolongation.C: constants everywhere; linear functions of the
+    cell centres wherever the stencil stays inside the coarse interior."""
+    v = [np.linspace(0, 1, shape[0] + 1), np.linspace(0, 2, shape[1] + 1),
+         np.linspace(0, 1, shape[2] + 1)]
+    o = make(v, (P, P, N, N, P, P))
+    for l in range(o.nlevels - 1):
+        o.set_field(oracle.X, l + 1, np.full(o.padded(l + 1), 2.0))
+        o.set_field(oracle.X, l, np.zeros(o.padded(l)))
+        o.prolong(l)
+        assert np.allclose(inner(o.field(oracle.X, l)), 2.0, rtol=1e-14)
+        cc = [o.centres(l + 1, d) for d in range(3)]
+        cf = [o.centres(l, d) for d in range(3)]
+        Xc, Yc, Zc = np.meshgrid(*cc, indexing="ij")
+        o.set_field(oracle.X, l + 1, 1.0 + 2 * Xc - Yc + 0.5 * Zc)
+        o.set_field(oracle.X, l, np.zeros(o.padded(l)))
+        o.prolong(l)
+        X, Y, Z = np.meshgrid(*cf, indexing="ij")
+        want = inner(1.0 + 2 * X - Y + 0.5 * Z)
+        got = inner(o.field(oracle.X, l))
+        keep = [slice(1, -1) if o.ratios[l + 1][d] > 1 else slice(None) for d in range(3)]
+        assert np.allclose(got[tuple(keep)], want[tuple(keep)], rtol=0, atol=1e-13)
+
+
+def test_uniform_transfer_weights():
+    """Uniform grid: restriction = sum of the 8 children (weights 1/8 scaled
+    by volume ratio 8); prolongation weights 27/64, 9/64, 3/64, 1/64."""
+    n = 8
+    o = make([np.linspace(0, 1, n + 1)] * 3, (N,) * 6)
+    rng = np.random.default_rng(10)
+    r = rng.uniform(-1, 1, o.padded(0))
+    o.set_field(oracle.R, 0, r)
+    o.restrict(0)
+    ri = inner(r)
+    s = sum(ri[a::2, b::2, c::2] for a in (0, 1) for b in (0, 1) for c in (0, 1))
+    assert np.allclose(inner(o.field(oracle.B, 1)), s, rtol=1e-14)
+    xc = np.zeros(o.padded(1))
+    xc[2, 2, 2] = 64.0
+    o.set_field(oracle.X, 1, xc)
+    o.set_field(oracle.X, 0, np.zeros(o.padded(0)))
+    o.prolong(0)
+    xf = inner(o.field(oracle.X, 0))
+    # coarse interior cell (1,1,1) spans fine 2..3; fine 2 is the lower half
+    assert xf[2, 2, 2] == 27.0 and xf[3, 3, 3] == 27.0
+    assert xf[1, 2, 2] == 9.0 and xf[1, 1, 2] == 3.0 and xf[1, 1, 1] == 1.0
+
+
+# -------------------------------------------------------------- coarse solve
+
+def test_coarse_lu_matches_numpy():
+    """APLU vs numpy.linalg.solve on the same dense matrix
+    (externalSolver/check.py:53-79 checks RMS <= 1e-3; we ask 1e-13)."""
+    for bc in ((N,) * 6, (P, P, N, N, P, P), (D, D, P, P, N, N)):
+        v = [np.linspace(0, 1, 9), stretched(8), np.linspace(0, 1, 9)]
+        o = make(v, bc, coarse_level=1)
+        o.pin_and_eliminate()
+        o.prepare_coarse()
+        dense, lu, piv = o.coarse_dense()
+        lc = o.nlevels - 2
+        # dense matrix rebuilt independently from the eliminated stencil
+        A = inner(o.stencil(lc))
+        m = o.shapes[lc]
+        nn = int(np.prod(m))
+        M = np.zeros((nn, nn))
+        idx = np.arange(nn).reshape(m)
+        offs = [(0, 0, 0), (-1, 0, 0), (1, 0, 0), (0, -1, 0), (0, 1, 0), (0, 0, -1), (0, 0, 1)]
+        for i in range(m[0]):
+            for j in range(m[1]):
+                for k in range(m[2]):
+                    for s, (a, bb, c) in enumerate(offs):
+                        q = [i + a, j + bb, k + c]
+                        ok = True
+                        for d in range(3):
+                            if q[d] < 0 or q[d] >= m[d]:
+                                if bc[2 * d] == P:
+                                    q[d] %= m[d]
+                                else:
+                                    ok = False
+                        if ok and A[i, j, k, s] != 0:
+                            M[idx[i, j, k], idx[tuple(q)]] += A[i, j, k, s]
+        assert np.allclose(dense, M, rtol=1e-15, atol=0)
+        rng = np.random.default_rng(11)
+        bc_ = rng.uniform(-1, 1, o.padded(lc))
+        o.set_field(oracle.B, lc, bc_)
+        o.set_field(oracle.X, lc, np.zeros(o.padded(lc)))
+        o.coarse_solve()
+        x = inner(o.field(oracle.X, lc)).ravel()
+        xe = np.linalg.solve(M, inner(bc_).ravel())
+        assert np.allclose(x, xe, rtol=1e-12, atol=1e-13 * np.abs(xe).max())
+
+
+def test_singular_pin():
+    """solver.C:200-231: coarse A(0,0,0) = diag(1), couplings to cell 0 gone
+    (including the periodic wrap)."""
+    o = make([np.linspace(0, 1, 9)] * 3, (P, P, N, N, N, N), coarse_level=0)
+    assert o.singular()
+    o.pin_and_eliminate()
+    lc = o.nlevels - 1
+    A = o.stencil(lc)
+    assert np.array_equal(A[1, 1, 1], [1, 0, 0, 0, 0, 0, 0])
+    n = o.shapes[lc][0]
+    assert A[2, 1, 1, 1] == 0          # L of cell (1,0,0)
+    assert A[n, 1, 1, 2] == 0          # periodic R of cell (n-1,0,0)
+    assert A[1, 2, 1, 3] == 0 and A[1, 1, 2, 5] == 0
+
+
+# ------------------------------------------------------------ whole solves
+
+def test_norm_factor_zero_guess():
+    """x0 = 0 => normFactor = ||b|| and the initial residual is 1."""
+    p = W.uniform_neumann(8, seed=0)
+    o = make(p.vertices, p.bc, rel_tol=1e-6)
+    o.set_field(oracle.B, 0, p.b)
+    o.pin_and_eliminate()
+    o.residual(0, False)
+    assert np.isclose(o.norm_factor(), np.linalg.norm(inner(p.b)), rtol=1e-14)
+    it, r0, r1 = o.solve()
+    assert r0 == pytest.approx(1.0, rel=1e-14)
+
+
+def test_solve_c1_vs_dct():
+    """64^3 all-Neumann, V(2,2), 4^3 direct coarse, relTol 1e-10."""
+    p = W.uniform_neumann(64, seed=0)
+    o = make(p.vertices, p.bc, rel_tol=1e-10, n_sweeps_pre=2, n_sweeps_post=2,
+             cycle=oracle.VCYCLE, coarse_level=1)
+    o.set_field(oracle.B, 0, p.b)
+    it, r0, r1 = o.solve()
+    assert r1 <= 1e-10 and 5 <= it <= 20
+    xe = direct.solve_dct(p.b, 1.0 / 64)
+    assert direct.rel_error(inner(o.field(oracle.X, 0)), xe, True) < 1e-9
+
+
+def test_solve_channel_vs_fft_thomas():
+    p = W.channel(24, seed=1)
+    o = make(p.vertices, p.bc, rel_tol=1e-10, n_sweeps_pre=2, n_sweeps_post=2,
+             cycle=oracle.VCYCLE, coarse_level=1, max_iter=2000)
+    o.set_field(oracle.B, 0, p.b)
+    it, r0, r1 = o.solve()
+    assert r1 <= 1e-10
+    xe = direct.solve_fft_thomas(p.b, p.vertices)
+    assert direct.rel_error(inner(o.field(oracle.X, 0)), xe, True) < 1e-8
+    # the two direct solvers agree
+    xs = direct.solve_sparse(o.stencil(0), p.b, p.bc)
+    assert direct.rel_error(xs, xe, True) < 1e-12
+
+
+def test_solve_spheres_vs_sparse():
+    p = W.spheres(16, seed=2, nspheres=5)
+    o = make(p.vertices, p.bc, lam=p.lam, rel_tol=1e-11, n_sweeps_pre=2,
+             n_sweeps_post=2, cycle=oracle.VCYCLE, coarse_level=0)
+    o.set_field(oracle.B, 0, p.b)
+    it, r0, r1 = o.solve()
+    assert r1 <= 1e-11
+    xs = direct.solve_sparse(o.stencil(0), p.b, p.bc)
+    assert direct.rel_error(inner(o.field(oracle.X, 0)), xs, True) < 1e-8
+
+
+def test_solve_dirichlet_inhomogeneous():
+    """Dirichlet value v on -x/+x: solution of laplace(x) = 0 is linear
+    between the walls; b picks up -2 g v (DirichletBoundaryCondition.C:45)."""
+    n = 8
+    v = [np.linspace(0, 1, n + 1)] * 3
+    o = oracle.Oracle(v, (D, D, N, N, N, N), (1.0, 3.0, 0, 0, 0, 0))
+    o.set_opts(rel_tol=1e-12, n_sweeps_pre=2, n_sweeps_post=2, cycle=oracle.VCYCLE,
+               coarse_level=1)
+    o.assemble()
+    o.set_field(oracle.B, 0, np.zeros(o.padded(0)))
+    it, r0, r1 = o.solve()
+    x = inner(o.field(oracle.X, 0))
+    xc = o.centres(0, 0)[1:-1]
+    assert np.allclose(x, (1.0 + 2.0 * xc)[:, None, None], rtol=1e-10)
+    # eliminated ghosts refreshed on exit (MG.C:475): ghost = 2v - interior
+    g = o.field(oracle.X, 0)
+    assert np.allclose(g[0, 1:-1, 1:-1], 2 * 1.0 - g[1, 1:-1, 1:-1])
+
+
+def test_iteration_count_reported_w_f_cycles():
+    p = W.uniform_neumann(16, seed=3)
+    its = {}
+    for cyc in (oracle.VCYCLE, oracle.WCYCLE, oracle.FCYCLE):
+        o = make(p.vertices, p.bc, rel_tol=1e-8, n_sweeps_pre=1, n_sweeps_post=1,
+                 cycle=cyc, coarse_level=0)
+        o.set_field(oracle.B, 0, p.b)
+        its[cyc] = o.solve()[0]
+    # W and F visit coarse levels more often than V: never slower to converge
+    assert its[oracle.WCYCLE] <= its[oracle.VCYCLE]
+    assert its[oracle.FCYCLE] <= its[oracle.VCYCLE]
