@@ -36,11 +36,15 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def solver_opts(precision):
+MAX_ITER = 999
+
+
+def solver_opts(precision, krylov=0):
     from briscola_amd import bmg
-    return dict(rel_tol=1e-8, tolerance=0.0, max_iter=999, n_sweeps_pre=2,
+    return dict(rel_tol=1e-8, tolerance=0.0, max_iter=MAX_ITER, n_sweeps_pre=2,
                 n_sweeps_post=2, cycle=bmg.VCYCLE, coarse_mode=bmg.COARSE_DIRECT,
-                coarse_level=1, smoother=bmg.RBGS, precision=precision)
+                coarse_level=1, smoother=bmg.RBGS, precision=precision,
+                krylov=krylov)
 
 
 def traffic_from_profiles():
@@ -55,10 +59,11 @@ def traffic_from_profiles():
     return None
 
 
-def run_gpu(prob, precision, steps, warmup, profile, reps=50):
+def run_gpu(prob, precision, krylov, steps, warmup, profile, reps=50):
     import torch
     from briscola_amd import bmg
-    ctx = bmg.Context(prob.vertices, prob.bc, prob.bc_value, **solver_opts(precision))
+    ctx = bmg.Context(prob.vertices, prob.bc, prob.bc_value,
+                      **solver_opts(precision, krylov))
     ctx.set_laplacian(prob.lam)
     b_host = torch.from_numpy(prob.b)
     b_dev = b_host.to("cuda:0")
@@ -100,6 +105,7 @@ def cpu_baseline_and_check(prob, results, iters_fp64, sample_iters):
     o = oracle.Oracle(prob.vertices, prob.bc)
     opts = solver_opts(0)
     opts.pop("precision")
+    opts.pop("krylov")
     o.set_opts(**opts)
     o.assemble()
     o.set_field(oracle.B, 0, prob.b)
@@ -135,8 +141,13 @@ def main():
     ap.add_argument("--cpu-sample-iters", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true",
                     help="skip the CPU baseline and the FP64 direct check")
-    ap.add_argument("--only", choices=["fp64", "mixed"], default=None)
+    ap.add_argument("--only", choices=["fp64", "mixed", "fcg_fp64", "fcg_mixed"],
+                    default=None)
+    ap.add_argument("--max-iter", type=int, default=999,
+                    help="cap on MG iterations (profiling runs only)")
     args = ap.parse_args()
+    global MAX_ITER
+    MAX_ITER = args.max_iter
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -147,13 +158,14 @@ def main():
     log(f"[bench] generating channel {args.n}^3")
     prob = workloads.channel(args.n, seed=0)
     results = {}
-    plan = [("fp64", bmg.FP64), ("mixed", bmg.MIXED)]
+    plan = [("fp64", bmg.FP64, bmg.KRYLOV_NONE), ("mixed", bmg.MIXED, bmg.KRYLOV_NONE),
+            ("fcg_fp64", bmg.FP64, bmg.KRYLOV_FCG), ("fcg_mixed", bmg.MIXED, bmg.KRYLOV_FCG)]
     if args.only:
         plan = [p for p in plan if p[0] == args.only]
-    for name, prec in plan:
+    for name, prec, kry in plan:
         log(f"[bench] {name}: warmup {args.warmup}, steps {args.steps}")
-        results[name] = run_gpu(prob, prec, args.steps, args.warmup,
-                                profile=True)
+        results[name] = run_gpu(prob, prec, kry, args.steps, args.warmup,
+                                profile=(kry == bmg.KRYLOV_NONE))
         r = results[name]
         log(f"[bench] {name}: {r['ms']:.2f} ms/solve, {r['iters']} iterations, "
             f"final rel. residual {r['final_residual']:.3e}")
@@ -188,7 +200,7 @@ def main():
         "higher_is_better": False,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "f64" if best == "fp64" else "f32+f64",
+        "dtype": "f64" if best in ("fp64", "fcg_fp64") else "f32+f64",
         "data": "synthetic",
         "config": {"workload": (f"channel {args.n}^3: periodic x,z, Neumann y, tanh "
                                 f"beta=2, Fourier RHS; MG V(2,2) rbgs, linear "
@@ -202,8 +214,9 @@ def main():
                        "err_vs_fp64_oracle": v.get("err_vs_fp64_oracle")}
                    for k, v in results.items()},
     }
-    if "fp64" in results and "mixed" in results:
-        line["speedup_mixed_vs_fp64"] = round(results["fp64"]["ms"] / results["mixed"]["ms"], 3)
+    if "fp64" in results:
+        line["speedup_vs_plain_fp64_mg"] = {
+            k: round(results["fp64"]["ms"] / v["ms"], 3) for k, v in results.items()}
     print(json.dumps(line), flush=True)
     return 0
 

@@ -18,6 +18,7 @@ VCYCLE, WCYCLE, FCYCLE = 0, 1, 2
 COARSE_SMOOTH, COARSE_DIRECT = 0, 1
 RBGS, JAC = 0, 1
 FP64, MIXED = 0, 1
+KRYLOV_NONE, KRYLOV_FCG = 0, 1
 X, B, R = 0, 1, 2
 
 EXPORTS = [
@@ -52,7 +53,8 @@ class Opts(ctypes.Structure):
                 ("n_sweeps_pre", ctypes.c_int), ("n_sweeps_post", ctypes.c_int),
                 ("cycle", ctypes.c_int), ("coarse_mode", ctypes.c_int),
                 ("coarse_level", ctypes.c_int), ("smoother", ctypes.c_int),
-                ("precision", ctypes.c_int), ("reserved", ctypes.c_int * 5)]
+                ("precision", ctypes.c_int), ("krylov", ctypes.c_int),
+                ("reserved", ctypes.c_int * 4)]
 
 
 class Stats(ctypes.Structure):

@@ -185,6 +185,17 @@ struct bmg_ctx
     double* hScalar = nullptr;   // pinned
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool mixedReady = false;
+    // flexible CG state (level-0 FP64 fields) and the captured preconditioner
+    double* kx[2] = {nullptr, nullptr};
+    double* kb[2] = {nullptr, nullptr};
+    double* kr[2] = {nullptr, nullptr};
+    double* kp[2] = {nullptr, nullptr};
+    double* kq[2] = {nullptr, nullptr};
+    double* kz[2] = {nullptr, nullptr};
+    hipGraph_t pgraph = nullptr;
+    hipGraphExec_t pgraphExec = nullptr;
+    int pgraphMixed = -1;
+    long pgraphGen = -1;
     // one MG iteration captured as a HIP graph and replayed
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
@@ -222,6 +233,13 @@ struct bmg_ctx
         if (ev1) (void)hipEventDestroy(ev1);
         if (graphExec) (void)hipGraphExecDestroy(graphExec);
         if (graph) (void)hipGraphDestroy(graph);
+        if (pgraphExec) (void)hipGraphExecDestroy(pgraphExec);
+        if (pgraph) (void)hipGraphDestroy(pgraph);
+        for (int c = 0; c < 2; c++)
+        {
+            devFree(kx[c]); devFree(kb[c]); devFree(kr[c]);
+            devFree(kp[c]); devFree(kq[c]); devFree(kz[c]);
+        }
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -289,6 +307,12 @@ struct bmg_ctx
 
     void enqueueIteration(bool mixed);
     double iterate(bool mixed);
+    void enqueuePrecondition(bool mixed);
+    void precondition(bool mixed);
+    double dots(int nd, const Pair<const double>* ab, double* out);
+    double normFactorOf(Pair<const double> x, Pair<const double> r,
+                        Pair<const double> b);
+    void runFCG(bool constMatrix, bmg_stats* st);
     void prepare(bool constMatrix);
     void prepareCoarse();
     void ensureMixed();
@@ -846,6 +870,204 @@ double bmg_ctx::iterate(bool mixed)
     return *hScalar;
 }
 
+// z = M(r): one MG cycle on A z = r with z0 = 0, in FP64 or in FP32.  The
+// level-0 work arrays of the cycle receive r as right-hand side and as the
+// residual the first repetition restricts (MG.C:72-77).
+void bmg_ctx::enqueuePrecondition(bool mixed)
+{
+    Level& q0 = lv[0];
+    const long n = q0.L.ncol;
+    Pair<const double> r{{kr[0], kr[1]}};
+    std::vector<int> visits(nLevels(), 0);
+    if (!mixed)
+    {
+        launch_copy<double>(n, r, pr(q0.b), stream);
+        launch_copy<double>(n, r, pr(q0.r), stream);
+        launch_fill<double>(pr(q0.x), n, 0.0, stream);
+        cycle<double>(visits, 0);
+        launch_copy<double>(n, cpr(q0.x), pr(kz), stream);
+    }
+    else
+    {
+        launch_to_float(n, r, pr(q0.bf), stream);
+        launch_fill<float>(pr(q0.xf), n, 0.0f, stream);
+        cycle<float>(visits, 0);
+        launch_to_double(n, Pair<const float>{{q0.xf[0], q0.xf[1]}}, pr(kz),
+                         stream);
+    }
+}
+
+void bmg_ctx::precondition(bool mixed)
+{
+    if (!useGraphs)
+    {
+        enqueuePrecondition(mixed);
+        return;
+    }
+    if (opts.smoother == BMG_JAC)
+        for (int l = 0; l < nLevels(); l++)
+            mixed ? (void)Tmp<float>(l) : (void)Tmp<double>(l);
+    if (!pgraphExec || pgraphGen != gen || pgraphMixed != int(mixed))
+    {
+        if (pgraphExec) { HIPCK(hipGraphExecDestroy(pgraphExec)); pgraphExec = nullptr; }
+        if (pgraph) { HIPCK(hipGraphDestroy(pgraph)); pgraph = nullptr; }
+        HIPCK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        try
+        {
+            enqueuePrecondition(mixed);
+        }
+        catch (...)
+        {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        HIPCK(hipStreamEndCapture(stream, &pgraph));
+        HIPCK(hipGraphInstantiate(&pgraphExec, pgraph, nullptr, nullptr, 0));
+        pgraphGen = gen;
+        pgraphMixed = int(mixed);
+    }
+    HIPCK(hipGraphLaunch(pgraphExec, stream));
+}
+
+// Up to three dot products over the level-0 interior; returns the first.
+double bmg_ctx::dots(int nd, const Pair<const double>* ab, double* out)
+{
+    const int nb = launch_dots(lv[0].L, nd, ab, partials, stream);
+    for (int d = 0; d < nd; d++)
+        launch_reduce_final(partials + size_t(d)*nb, nb, dScalar + d, stream);
+    HIPCK(hipGetLastError());
+    HIPCK(hipMemcpyAsync(hScalar, dScalar, nd*sizeof(double),
+                         hipMemcpyDeviceToHost, stream));
+    HIPCK(hipStreamSynchronize(stream));
+    for (int d = 0; d < nd; d++) out[d] = hScalar[d];
+    return out[0];
+}
+
+// Flexible conjugate gradients (Notay 2000, one stored direction) with one
+// MG cycle as the variable preconditioner; the stopping test is the
+// reference's (normFactors/checkConvergence, solver.C:13-95) on the updated
+// residual, confirmed on the true residual b - A x before returning.
+void bmg_ctx::runFCG(bool constMatrix, bmg_stats* st)
+{
+    prepare(constMatrix);
+    const bool mixed = opts.precision == BMG_MIXED;
+    if (mixed) ensureMixed();
+    Level& q0 = lv[0];
+    const long n = q0.L.ncol;
+    if (!kx[0])
+    {
+        for (int c = 0; c < 2; c++)
+        {
+            kx[c] = devAlloc<double>(n); kb[c] = devAlloc<double>(n);
+            kr[c] = devAlloc<double>(n); kp[c] = devAlloc<double>(n);
+            kq[c] = devAlloc<double>(n); kz[c] = devAlloc<double>(n);
+        }
+        gen++;
+        for (double** f : {kr, kp, kq, kz})
+            launch_fill<double>(Pair<double>{{f[0], f[1]}}, n, 0.0, stream);
+    }
+    const OpView<double> o = op<double>(0);
+    Pair<double> X{{kx[0], kx[1]}}, Bv{{kb[0], kb[1]}}, Rv{{kr[0], kr[1]}},
+        Pv{{kp[0], kp[1]}}, Qv{{kq[0], kq[1]}}, Zv{{kz[0], kz[1]}};
+    launch_copy<double>(n, cpr(q0.x), X, stream);
+    launch_copy<double>(n, cpr(q0.b), Bv, stream);
+    Region full;
+    for (int d = 0; d < 3; d++) { full.S[d] = 1; full.E[d] = q0.L.N[d] + 1; }
+    int nb = 0;
+    // initial residual without boundary update (MG.C:163)
+    launch_residual(q0.L, o, cst(X), cst(Bv), Rv, full, partials, &nb, stream);
+    double ss = reduceFinal(nb);
+    const double nf = normFactorOf(cst(X), cst(Rv), cst(Bv));
+    const double init = std::sqrt(ss)/nf;
+    if (!std::isfinite(init))
+        throw Error(BMG_ERR_NUMERIC, "initial residual is not finite");
+    auto check = [&](double cc) -> int
+    {
+        return (init > 0.0 && cc > opts.tolerance &&
+                cc/(init + 1e-20) > opts.rel_tol) ? 0 : 1;
+    };
+    double cur = init;
+    int conv = check(cur);
+    int iter = 0;
+    bool restart = true;
+    double pq = 0.0;
+    while ((iter < opts.max_iter && conv == 0) || iter < opts.min_iter)
+    {
+        if (restart)
+        {
+            precondition(mixed);
+            launch_copy<double>(n, cst(Zv), Pv, stream);
+            restart = false;
+        }
+        if (anyPeriodic())
+            launch_bc_fill(q0.L, o, Pv, 0, 1, bcval, stream);
+        launch_apply(q0.L, o, cst(Pv), Qv, stream);
+        const Pair<const double> ab[4] = {cst(Pv), cst(Qv), cst(Pv), cst(Rv)};
+        double d2[2];
+        dots(2, ab, d2);
+        pq = d2[0];
+        if (!(pq != 0.0) || !std::isfinite(pq))
+            throw Error(BMG_ERR_NUMERIC, "flexible CG breakdown (p.Ap = 0)");
+        const double alpha = d2[1]/pq;
+        nb = launch_cg_update(q0.L, alpha, X, Rv, cst(Pv), cst(Qv), partials,
+                              stream);
+        ss = reduceFinal(nb);
+        cur = std::sqrt(ss)/nf;
+        if (!std::isfinite(cur))
+            throw Error(BMG_ERR_NUMERIC, "residual norm is not finite");
+        iter++;
+        conv = check(cur);
+        if (conv && iter >= opts.min_iter)
+        {
+            // confirm on the true residual (with the boundary update)
+            if (anyPeriodic())
+                launch_bc_fill(q0.L, o, X, 0, 1, bcval, stream);
+            launch_residual(q0.L, o, cst(X), cst(Bv), Rv, full, partials, &nb,
+                            stream);
+            ss = reduceFinal(nb);
+            cur = std::sqrt(ss)/nf;
+            conv = check(cur);
+            if (conv) break;
+            restart = true;
+            continue;
+        }
+        precondition(mixed);
+        const Pair<const double> zq[2] = {cst(Zv), cst(Qv)};
+        double d1[1];
+        dots(1, zq, d1);
+        const double beta = -d1[0]/pq;
+        launch_xpby(n, cst(Zv), beta, Pv, stream);
+    }
+    launch_copy<double>(n, cst(X), pr(q0.x), stream);
+    launch_copy<double>(n, cst(Bv), pr(q0.b), stream);
+    launch_bc_fill(q0.L, op<double>(0), pr(q0.x), 2, 1, bcval, stream);
+    HIPCK(hipGetLastError());
+    if (st)
+    {
+        st->iters = iter;
+        st->initial_residual = init;
+        st->final_residual = cur;
+    }
+}
+
+double bmg_ctx::normFactorOf(Pair<const double> x, Pair<const double> r,
+                             Pair<const double> b)
+{
+    Level& q = lv[0];
+    const double n = double(q.cells());
+    int nb = 0;
+    launch_sum<double>(q.L, x, partials, &nb, stream);
+    const double sum = reduceFinal(nb);
+    const double y = sum/n;
+    launch_norm_factor_terms<double>(q.L, op<double>(0), r, b, y, partials,
+                                     &nb, stream);
+    double f = reduceFinal(nb);
+    f = std::sqrt(f);
+    return std::max(f, n*1e-12);
+}
+
 // solver::normFactors (solver.C:13-61)
 double bmg_ctx::normFactor()
 {
@@ -865,6 +1087,11 @@ double bmg_ctx::normFactor()
 // MG::Solve (MG.C:132-254) around the cycles; FP64 or mixed precision.
 void bmg_ctx::runSolve(bool constMatrix, bmg_stats* st)
 {
+    if (opts.krylov == BMG_KRYLOV_FCG)
+    {
+        runFCG(constMatrix, st);
+        return;
+    }
     prepare(constMatrix);
     Level& q0 = lv[0];
     const bool mixed = opts.precision == BMG_MIXED;
@@ -1012,7 +1239,8 @@ static void validateOpts(const bmg_opts* o)
         throw Error(BMG_ERR_ARG, "negative iteration/sweep/level option");
     if (o->cycle < 0 || o->cycle > 2 || o->coarse_mode < 0 ||
         o->coarse_mode > 1 || o->smoother < 0 || o->smoother > 1 ||
-        o->precision < 0 || o->precision > 1)
+        o->precision < 0 || o->precision > 1 || o->krylov < 0 ||
+        o->krylov > 1)
         throw Error(BMG_ERR_ARG, "unknown enum value in options");
 }
 
@@ -1155,10 +1383,10 @@ int bmg_create(bmg_ctx** out, const bmg_grid* g, const bmg_opts* opts)
                 f.pu[d] = devUpload(pu);
             }
         }
-        c->npartials = maxBlocks;
-        c->partials = devAlloc<double>(maxBlocks);
-        c->dScalar = devAlloc<double>(1);
-        HIPCK(hipHostMalloc(&c->hScalar, sizeof(double)));
+        c->npartials = 3*maxBlocks;
+        c->partials = devAlloc<double>(3*maxBlocks);
+        c->dScalar = devAlloc<double>(4);
+        HIPCK(hipHostMalloc(&c->hScalar, 4*sizeof(double)));
         *out = c.release();
     });
 }

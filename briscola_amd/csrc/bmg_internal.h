@@ -142,6 +142,21 @@ void launch_reduce_final(const double* partials, int n, double* out,
                          hipStream_t s);
 void launch_diag(const LevelDims& L, Pair<const double> a7, Pair<double> x,
                  Pair<const double> b, hipStream_t s);
+// flexible CG building blocks (FP64, level 0)
+void launch_apply(const LevelDims& L, const OpView<double>& op,
+                  Pair<const double> p, Pair<double> q, hipStream_t s);
+// dot products of ab[0]*ab[1], ab[2]*ab[3], ab[4]*ab[5]; returns blocks
+int launch_dots(const LevelDims& L, int nd, const Pair<const double>* ab,
+                double* partials, hipStream_t s);
+int launch_cg_update(const LevelDims& L, double alpha, Pair<double> x,
+                     Pair<double> r, Pair<const double> p,
+                     Pair<const double> q, double* partials, hipStream_t s);
+void launch_xpby(long n, Pair<const double> z, double beta, Pair<double> p,
+                 hipStream_t s);
+template <typename T>
+void launch_copy(long n, Pair<const T> a, Pair<T> b, hipStream_t s);
+void launch_to_double(long n, Pair<const float> a, Pair<double> b,
+                      hipStream_t s);
 // x64 += e32 (interior), r32 = (float) r64, etc.
 void launch_add_correction(const LevelDims& L, Pair<double> x,
                            Pair<const float> e, hipStream_t s);

@@ -1108,6 +1108,135 @@ __global__ __launch_bounds__(1024) void k_reduce_final(const double* partials,
     if (threadIdx.x == 0) out[0] = s;
 }
 
+// q = A p (row product in the C,L,R,B,T,A,F order), interior cells.
+template <int OP>
+__global__ __launch_bounds__(BX*BY) void k_apply(LevelDims L,
+                                                 OpView<double> op,
+                                                 Pair<const double> p,
+                                                 Pair<double> q)
+{
+    const int I = 1 + blockIdx.z;
+    const int J = 1 + blockIdx.y*BY + threadIdx.y;
+    const int pp = blockIdx.x*BX + threadIdx.x;
+    if (J > L.N[1]) return;
+    const int row = (I*L.P[1] + J)*L.Hz;
+    const int idx = row + pp;
+    for (int h = 0; h < 2; h++)
+    {
+        const int K = 2*pp + h;
+        if (K < 1 || K > L.N[2]) continue;
+        const int colour = (I + J + K) & 1;
+        const double* xc = p.c[colour];
+        const double* xo = p.c[colour ^ 1];
+        double a[7];
+        coeffs<double, OP>(L, op, colour, idx, row, pp, h, I, J, K, a);
+        q.c[colour][idx] =
+            a[0]*xc[idx]
+          + a[1]*xo[idx - L.sI]
+          + a[2]*xo[idx + L.sI]
+          + a[3]*xo[idx - L.Hz]
+          + a[4]*xo[idx + L.Hz]
+          + a[5]*xo[row + pp - 1 + h]
+          + a[6]*xo[row + pp + h];
+    }
+}
+
+// Up to three dot products over interior cells in one pass; per-block
+// partials at partials[d*nblocks + block].
+__global__ __launch_bounds__(BX*BY) void k_dots(LevelDims L, int nd,
+                                                Pair<const double> a0,
+                                                Pair<const double> b0,
+                                                Pair<const double> a1,
+                                                Pair<const double> b1,
+                                                Pair<const double> a2,
+                                                Pair<const double> b2,
+                                                double* partials)
+{
+    __shared__ double sm[BX*BY/64];
+    const int I = 1 + blockIdx.z;
+    const int J = 1 + blockIdx.y*BY + threadIdx.y;
+    const int pp = blockIdx.x*BX + threadIdx.x;
+    double acc[3] = {0.0, 0.0, 0.0};
+    if (J <= L.N[1])
+    {
+        const int idx = (I*L.P[1] + J)*L.Hz + pp;
+        for (int h = 0; h < 2; h++)
+        {
+            const int K = 2*pp + h;
+            if (K < 1 || K > L.N[2]) continue;
+            const int c = (I + J + K) & 1;
+            acc[0] += a0.c[c][idx]*b0.c[c][idx];
+            if (nd > 1) acc[1] += a1.c[c][idx]*b1.c[c][idx];
+            if (nd > 2) acc[2] += a2.c[c][idx]*b2.c[c][idx];
+        }
+    }
+    const int nb = gridDim.x*gridDim.y*gridDim.z;
+    const int blk = blockIdx.x + gridDim.x*(blockIdx.y + gridDim.y*blockIdx.z);
+    for (int d = 0; d < nd; d++)
+    {
+        const double s = block_sum(acc[d], sm);
+        if (threadIdx.x == 0 && threadIdx.y == 0) partials[d*nb + blk] = s;
+        __syncthreads();
+    }
+}
+
+// x += alpha p; r -= alpha q (interior) and per-block partial sums of r^2.
+__global__ __launch_bounds__(BX*BY) void k_cg_update(LevelDims L, double alpha,
+                                                     Pair<double> x,
+                                                     Pair<double> r,
+                                                     Pair<const double> p,
+                                                     Pair<const double> q,
+                                                     double* partials)
+{
+    __shared__ double sm[BX*BY/64];
+    const int I = 1 + blockIdx.z;
+    const int J = 1 + blockIdx.y*BY + threadIdx.y;
+    const int pp = blockIdx.x*BX + threadIdx.x;
+    double acc = 0.0;
+    if (J <= L.N[1])
+    {
+        const int idx = (I*L.P[1] + J)*L.Hz + pp;
+        for (int h = 0; h < 2; h++)
+        {
+            const int K = 2*pp + h;
+            if (K < 1 || K > L.N[2]) continue;
+            const int c = (I + J + K) & 1;
+            x.c[c][idx] = x.c[c][idx] + alpha*p.c[c][idx];
+            const double rn = r.c[c][idx] - alpha*q.c[c][idx];
+            r.c[c][idx] = rn;
+            acc += rn*rn;
+        }
+    }
+    const double s = block_sum(acc, sm);
+    if (threadIdx.x == 0 && threadIdx.y == 0)
+        partials[blockIdx.x + gridDim.x*(blockIdx.y + gridDim.y*blockIdx.z)] = s;
+}
+
+// p = z + beta p over the whole padded arrays (ghosts refreshed later).
+__global__ void k_xpby(long n, Pair<const double> z, double beta,
+                       Pair<double> p)
+{
+    const long t = (long)blockIdx.x*blockDim.x + threadIdx.x;
+    if (t < n)
+    {
+        p.c[0][t] = z.c[0][t] + beta*p.c[0][t];
+        p.c[1][t] = z.c[1][t] + beta*p.c[1][t];
+    }
+}
+
+template <typename T>
+__global__ void k_copy(long n, Pair<const T> a, Pair<T> b)
+{
+    const long t = (long)blockIdx.x*blockDim.x + threadIdx.x;
+    if (t < n) { b.c[0][t] = a.c[0][t]; b.c[1][t] = a.c[1][t]; }
+}
+
+__global__ void k_to_double(long n, Pair<const float> a, Pair<double> b)
+{
+    const long t = (long)blockIdx.x*blockDim.x + threadIdx.x;
+    if (t < n) { b.c[0][t] = double(a.c[0][t]); b.c[1][t] = double(a.c[1][t]); }
+}
+
 // diagonalSmoother::Sweep (diagonalSmoother.C:13-69): x = b / A_C.
 __global__ __launch_bounds__(BX*BY) void k_diag(LevelDims L, Pair<const double> a7,
                                                 Pair<double> x,
@@ -1494,6 +1623,64 @@ void launch_diag(const LevelDims& L, Pair<const double> a7, Pair<double> x,
                  Pair<const double> b, hipStream_t s)
 {
     hipLaunchKernelGGL(k_diag, grid_interior(L), BLK, 0, s, L, a7, x, b);
+}
+
+void launch_apply(const LevelDims& L, const OpView<double>& op,
+                  Pair<const double> p, Pair<double> q, hipStream_t s)
+{
+    if (op.kind == OP_STENCIL)
+        hipLaunchKernelGGL((k_apply<OP_STENCIL>), grid_interior(L), BLK, 0, s,
+                           L, op, p, q);
+    else if (op.kind == OP_CONST)
+        hipLaunchKernelGGL((k_apply<OP_CONST>), grid_interior(L), BLK, 0, s, L,
+                           op, p, q);
+    else
+        hipLaunchKernelGGL((k_apply<OP_FACES>), grid_interior(L), BLK, 0, s, L,
+                           op, p, q);
+}
+
+int launch_dots(const LevelDims& L, int nd, const Pair<const double>* ab,
+                double* partials, hipStream_t s)
+{
+    const dim3 g = grid_interior(L);
+    Pair<const double> z{{nullptr, nullptr}};
+    hipLaunchKernelGGL(k_dots, g, BLK, 0, s, L, nd, ab[0], ab[1],
+                       nd > 1 ? ab[2] : z, nd > 1 ? ab[3] : z,
+                       nd > 2 ? ab[4] : z, nd > 2 ? ab[5] : z, partials);
+    return int(g.x*g.y*g.z);
+}
+
+int launch_cg_update(const LevelDims& L, double alpha, Pair<double> x,
+                     Pair<double> r, Pair<const double> p,
+                     Pair<const double> q, double* partials, hipStream_t s)
+{
+    const dim3 g = grid_interior(L);
+    hipLaunchKernelGGL(k_cg_update, g, BLK, 0, s, L, alpha, x, r, p, q,
+                       partials);
+    return int(g.x*g.y*g.z);
+}
+
+void launch_xpby(long n, Pair<const double> z, double beta, Pair<double> p,
+                 hipStream_t s)
+{
+    hipLaunchKernelGGL(k_xpby, dim3((n + 255)/256), dim3(256), 0, s, n, z,
+                       beta, p);
+}
+
+template <typename T>
+void launch_copy(long n, Pair<const T> a, Pair<T> b, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_copy<T>), dim3((n + 255)/256), dim3(256), 0, s, n,
+                       a, b);
+}
+template void launch_copy<double>(long, Pair<const double>, Pair<double>,
+                                  hipStream_t);
+
+void launch_to_double(long n, Pair<const float> a, Pair<double> b,
+                      hipStream_t s)
+{
+    hipLaunchKernelGGL(k_to_double, dim3((n + 255)/256), dim3(256), 0, s, n,
+                       a, b);
 }
 
 void launch_add_correction(const LevelDims& L, Pair<double> x,

@@ -57,6 +57,11 @@ enum bmg_smoother { BMG_RBGS = 0, BMG_JAC = 1 };
  * BMG_MIXED: FP32 cycles on the defect equation inside FP64 defect
  * correction; FP64 residual, update and coarsest direct solve. */
 enum bmg_precision { BMG_FP64 = 0, BMG_MIXED = 1 };
+/* Outer iteration: BMG_KRYLOV_NONE repeats MG cycles as MG::Solve does
+ * (MG.C:191-242); BMG_KRYLOV_FCG runs flexible conjugate gradients (Notay,
+ * FCG with one stored direction) preconditioned by one MG cycle of the
+ * chosen precision, with the same stopping test. */
+enum bmg_krylov { BMG_KRYLOV_NONE = 0, BMG_KRYLOV_FCG = 1 };
 
 /* Mesh + boundary conditions: what MG reads from fvMesh and the BCs of
  * sys.x() (mesh.C:19-53 hierarchy, fvMeshMetrics.C metrics,
@@ -82,7 +87,8 @@ typedef struct bmg_opts {
     int coarse_level;       /* coarseLevel, default 0       */
     int smoother;           /* smoother, default rbgs       */
     int precision;          /* BMG_FP64 (default) / BMG_MIXED */
-    int reserved[5];
+    int krylov;             /* BMG_KRYLOV_NONE (default) / BMG_KRYLOV_FCG */
+    int reserved[4];
 } bmg_opts;
 
 typedef struct bmg_stats {

@@ -38,7 +38,7 @@ int main(int argc, char** argv)
     std::ifstream sh(dir + "/shape.txt");
     fvMesh mesh;
     int nlev = 0;
-    double relTol = 0;
+    std::string relTol;
     int coarseLevel = 0;
     sh >> mesh.N[0] >> mesh.N[1] >> mesh.N[2] >> nlev;
     for (int f = 0; f < 6; f++) sh >> mesh.bc[f];
@@ -53,7 +53,7 @@ int main(int argc, char** argv)
         mesh.vertices[d].assign(v.begin() + o, v.begin() + o + mesh.N[d] + 1);
         o += mesh.N[d] + 1;
     }
-    dictionary dict{{"relTol", std::to_string(relTol)}, {"tolerance", "0"},
+    dictionary dict{{"relTol", relTol}, {"tolerance", "0"},
                     {"cycleType", "V"}, {"nSweepsPre", "2"},
                     {"nSweepsPost", "2"},
                     {"coarseLevel", std::to_string(coarseLevel)}};
