@@ -1,0 +1,36 @@
+"""Counter-collection driver: runs the kernels whose HBM traffic the
+benchmark reports, for rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE).
+
+  rocprofv3 --pmc FETCH_SIZE --output-format csv -d <dir> -o run -- \
+      python3 -m briscola_amd.measure
+
+Launches, in order: 5 level-0 red-black phases of the 256^3 channel operator
+(k_rbgs<double, 1>, 198 MB algorithmic each) and one plain device-to-device
+copy of 128 MiB (the calibration kernel: 128 MiB read, 128 MiB written).
+"""
+import sys
+
+
+def main(n=256, reps=5):
+    import torch
+    from briscola_amd import bmg, workloads
+    p = workloads.channel(n, seed=0)
+    ctx = bmg.Context(p.vertices, p.bc, rel_tol=1e-8, tolerance=0.0,
+                      cycle=bmg.VCYCLE, n_sweeps_pre=2, n_sweeps_post=2,
+                      coarse_level=1)
+    ctx.set_laplacian()
+    ctx.upload(bmg.B, 0, p.b)
+    ctx.prepare()
+    ms, by = ctx.time_smoother(0, bmg.FP64, reps)
+    print(f"phase: {ms * 1e3:.2f} us, {by / 1e6:.1f} MB algorithmic", flush=True)
+    a = torch.empty(128 << 20, dtype=torch.uint8, device="cuda:0").fill_(1)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print("copy: 134217728 bytes read, 134217728 bytes written", flush=True)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
