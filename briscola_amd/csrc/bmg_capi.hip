@@ -128,6 +128,17 @@ struct Level
     double* pu[3] = {nullptr, nullptr, nullptr};
     float* ruf[3] = {nullptr, nullptr, nullptr};
     float* puf[3] = {nullptr, nullptr, nullptr};
+    // snapshots of the slab planes 1 and N_d of every periodic axis d
+    // (P_a1 x P_a2 values each), read by the slab smoother
+    double* snap[3][2] = {{nullptr, nullptr}, {nullptr, nullptr},
+                          {nullptr, nullptr}};
+    float* snapf[3][2] = {{nullptr, nullptr}, {nullptr, nullptr},
+                          {nullptr, nullptr}};
+
+    size_t planeCells(int d) const
+    {
+        return size_t(d == 0 ? L.P[1] : L.P[0])*size_t(d == 2 ? L.P[1] : L.P[2]);
+    }
 
     long cells() const { return long(L.N[0])*L.N[1]*L.N[2]; }
 
@@ -144,6 +155,7 @@ struct Level
         {
             devFree(dw[d]); devFree(ddl[d]); devFree(fw[d]); devFree(fdl[d]);
             devFree(ru[d]); devFree(pu[d]); devFree(ruf[d]); devFree(puf[d]);
+            for (int k = 0; k < 2; k++) { devFree(snap[d][k]); devFree(snapf[d][k]); }
         }
     }
 };
@@ -258,6 +270,8 @@ struct bmg_ctx
     Pair<T> Rr(int l);
     template <typename T>
     Pair<T> Tmp(int l);
+    template <typename T>
+    T* const (*Snap(int l))[2];
 
     // -------------------------------------------------------------- helpers
     double reduceFinal(int nblocks)
@@ -383,6 +397,14 @@ template <> Pair<double> bmg_ctx::Tmp<double>(int l)
     }
     return pr(q.tmp);
 }
+template <> double* const (*bmg_ctx::Snap<double>(int l))[2]
+{
+    return lv[l].snap;
+}
+template <> float* const (*bmg_ctx::Snap<float>(int l))[2]
+{
+    return lv[l].snapf;
+}
 template <> Pair<float> bmg_ctx::X<float>(int l) { return pr(lv[l].xf); }
 template <> Pair<float> bmg_ctx::B<float>(int l) { return pr(lv[l].bf); }
 template <> Pair<float> bmg_ctx::Rr<float>(int l) { return pr(lv[l].rf); }
@@ -460,23 +482,60 @@ void bmg_ctx::smooth(int l, int sweeps)
             prevColour = cols[i];
         }
         if (!anyPeriodic()) continue;
-        launch_bc_fill(q.L, o, x, 0, l == 0, bcval, stream);
-        // width-1 slabs of the periodic faces, face pairs in order
+        // periodic ghosts, plus snapshots of the slab planes
+        T* const (*sn)[2] = Snap<T>(l);
+        launch_bc_fill_snap(q.L, o, x, 0, l == 0, bcval, sn, stream);
+        // width-1 slabs of the periodic faces, face pairs in order: one
+        // launch per pair when its two planes are independent, else one per
+        // plane
+        struct SlabLaunch { int d, slot, nslab, ncol; int cl[8]; };
+        std::vector<SlabLaunch> seq;
         for (int d = 0; d < 3; d++)
         {
             if (!periodic(2*d)) continue;
-            int planes[2] = {1, q.L.N[d]};
-            int cl[8];
-            const int n0 = colours(slab(l, 2*d), cl);
-            const int n1 = colours(slab(l, 2*d + 1), cl + 4);
+            SlabLaunch s0{d, 0, 2, 0, {}};
+            const int n0 = colours(slab(l, 2*d), s0.cl);
+            const int n1 = colours(slab(l, 2*d + 1), s0.cl + 4);
             if (q.L.N[d] >= 3 && n0 == n1)
-                launch_rbgs_slabs(q.L, o, x, b, d, planes, 2, cl, n0, stream);
+            {
+                s0.ncol = n0;
+                seq.push_back(s0);
+            }
             else
             {
-                launch_rbgs_slabs(q.L, o, x, b, d, planes, 1, cl, n0, stream);
-                launch_rbgs_slabs(q.L, o, x, b, d, planes + 1, 1, cl + 4, n1,
-                                  stream);
+                SlabLaunch s1 = s0;
+                s0.nslab = 1;
+                s0.ncol = n0;
+                s1.slot = 1;
+                s1.nslab = 1;
+                s1.ncol = n1;
+                for (int i = 0; i < 4; i++) s1.cl[i] = s0.cl[4 + i];
+                seq.push_back(s0);
+                seq.push_back(s1);
             }
+        }
+        for (size_t i = 0; i < seq.size(); i++)
+        {
+            const SlabLaunch& sl = seq[i];
+            const int d = sl.d;
+            const int planes[2] = {1, q.L.N[d]};
+            SlabIO<T> io{};
+            io.src[0] = sn[d][sl.slot];
+            io.src[1] = sn[d][1];
+            // later launches whose planes meet this one
+            for (size_t j = i + 1; j < seq.size(); j++)
+                for (int k = 0; k < seq[j].nslab; k++)
+                {
+                    const int dj = seq[j].d, sj = seq[j].slot + k;
+                    const int pj = sj ? q.L.N[dj] : 1;
+                    if (dj == d && pj != planes[sl.slot]) continue;
+                    io.tptr[io.ntarget] = sn[dj][sj];
+                    io.tdim[io.ntarget] = dj;
+                    io.tplane[io.ntarget] = pj;
+                    io.ntarget++;
+                }
+            launch_rbgs_slabs(q.L, o, x, b, d, planes + sl.slot, sl.nslab,
+                              sl.cl, sl.ncol, io, stream);
         }
         prevColour = -1;
     }
@@ -492,9 +551,7 @@ void bmg_ctx::residual(int l, bool corrBcs, bool norm, int* nblocks)
     const OpView<T> o = op<T>(l);
     if (corrBcs && anyPeriodic())
         launch_bc_fill(q.L, o, X<T>(l), 0, l == 0, bcval, stream);
-    Region rg;
-    for (int d = 0; d < 3; d++) { rg.S[d] = 1; rg.E[d] = q.L.N[d] + 1; }
-    launch_residual(q.L, o, cst(X<T>(l)), cst(B<T>(l)), Rr<T>(l), rg,
+    launch_residual(q.L, o, cst(X<T>(l)), cst(B<T>(l)), Rr<T>(l),
                     norm ? partials : nullptr, nblocks, stream);
 }
 
@@ -756,6 +813,11 @@ void bmg_ctx::ensureMixed()
         }
         if (l == 0)
             for (int c = 0; c < 2; c++) q.rf[c] = q.bf[c];  // r aliases b
+        for (int d = 0; d < 3; d++)
+            if (periodic(2*d))
+                for (int k = 0; k < 2; k++)
+                    if (!q.snapf[d][k])
+                        q.snapf[d][k] = devAlloc<float>(q.planeCells(d));
         launch_fill<float>(pr(q.xf), n, 0.0f, stream);
         launch_fill<float>(pr(q.bf), n, 0.0f, stream);
         if (q.opkind == OP_STENCIL)
@@ -973,11 +1035,9 @@ void bmg_ctx::runFCG(bool constMatrix, bmg_stats* st)
         Pv{{kp[0], kp[1]}}, Qv{{kq[0], kq[1]}}, Zv{{kz[0], kz[1]}};
     launch_copy<double>(n, cpr(q0.x), X, stream);
     launch_copy<double>(n, cpr(q0.b), Bv, stream);
-    Region full;
-    for (int d = 0; d < 3; d++) { full.S[d] = 1; full.E[d] = q0.L.N[d] + 1; }
     int nb = 0;
     // initial residual without boundary update (MG.C:163)
-    launch_residual(q0.L, o, cst(X), cst(Bv), Rv, full, partials, &nb, stream);
+    launch_residual(q0.L, o, cst(X), cst(Bv), Rv, partials, &nb, stream);
     double ss = reduceFinal(nb);
     const double nf = normFactorOf(cst(X), cst(Rv), cst(Bv));
     const double init = std::sqrt(ss)/nf;
@@ -1024,7 +1084,7 @@ void bmg_ctx::runFCG(bool constMatrix, bmg_stats* st)
             // confirm on the true residual (with the boundary update)
             if (anyPeriodic())
                 launch_bc_fill(q0.L, o, X, 0, 1, bcval, stream);
-            launch_residual(q0.L, o, cst(X), cst(Bv), Rv, full, partials, &nb,
+            launch_residual(q0.L, o, cst(X), cst(Bv), Rv, partials, &nb,
                             stream);
             ss = reduceFinal(nb);
             cur = std::sqrt(ss)/nf;
@@ -1099,9 +1159,7 @@ void bmg_ctx::runSolve(bool constMatrix, bmg_stats* st)
 
     launch_fill<double>(pr(q0.r), q0.L.ncol, 0.0, stream);
     int nb = 0;
-    Region full;
-    for (int d = 0; d < 3; d++) { full.S[d] = 1; full.E[d] = q0.L.N[d] + 1; }
-    launch_residual(q0.L, op<double>(0), cpr(q0.x), cpr(q0.b), pr(q0.r), full,
+    launch_residual(q0.L, op<double>(0), cpr(q0.x), cpr(q0.b), pr(q0.r),
                     partials, &nb, stream);
     const double ss0 = reduceFinal(nb);
     const double nf = normFactor();
@@ -1344,10 +1402,14 @@ int bmg_create(bmg_ctx** out, const bmg_grid* g, const bmg_opts* opts)
             {
                 q.dw[d] = devUpload(q.ax[d].w);
                 q.ddl[d] = devUpload(q.ax[d].dl);
+                if (c->periodic(2*d))
+                    for (int k = 0; k < 2; k++)
+                        q.snap[d][k] = devAlloc<double>(q.planeCells(d));
             }
             const size_t nbx = size_t((q.L.P[2] + 2)/2 + 1 + 63)/64 + 1;
             maxBlocks = std::max(maxBlocks,
                                  nbx*size_t(q.L.P[1]/4 + 2)*size_t(q.L.P[0]));
+            maxBlocks = std::max(maxBlocks, size_t(pair_tile_count(q.L)));
         }
         // transfer weights (linearRestrictionScheme.C:15-80,
         // linearProlongationScheme.C:12-110), per axis for rectilinear meshes
@@ -1803,10 +1865,8 @@ int bmg_op_residual(bmg_ctx* c, int level, int corr)
         else
         {
             Level& q = c->lv[level];
-            Region rg;
-            for (int d = 0; d < 3; d++) { rg.S[d] = 1; rg.E[d] = q.L.N[d] + 1; }
             launch_residual(q.L, c->op<double>(level), cpr(q.x), cpr(q.b),
-                            pr(q.r), rg, nullptr, nullptr, c->stream);
+                            pr(q.r), nullptr, nullptr, c->stream);
         }
         HIPCK(hipGetLastError());
         HIPCK(hipStreamSynchronize(c->stream));

@@ -16,7 +16,7 @@ namespace bmg {
 
 enum { NEU = 0, DIR = 1, PER = 2 };
 
-// Operator representations (see DESIGN.md of the repository):
+// Operator representations:
 //   OP_STENCIL: the reference AoS 7-point stencil, split by colour and stored
 //               component-major (7 arrays per colour);
 //   OP_CONST  : lambda-free rectilinear Laplacian, coefficients rebuilt from
@@ -63,24 +63,40 @@ struct Region {
 };
 
 // ---- launchers (bmg_kernels.hip) ----
+// number of per-tile partial sums a cell-pair reduction over level L writes
+int pair_tile_count(const LevelDims& L);
 template <typename T>
 void launch_rbgs(const LevelDims& L, const OpView<T>& op, Pair<T> x,
                  Pair<const T> b, int colour, const Region& rg,
                  hipStream_t s);
+// Slab-plane snapshots and the snapshots a slab launch writes on to: src
+// holds the snapshot of each of its planes; every target is the snapshot of
+// a later slab launch, of plane tplane normal to axis tdim.
+template <typename T>
+struct SlabIO {
+    const T* src[2];
+    int ntarget;
+    T* tptr[6];
+    int tdim[6];
+    int tplane[6];
+};
 // All phases of the boundary slabs of one face pair (both planes when they
 // are independent, nslab = 2) in one launch; colours[4*slab + phase].
 template <typename T>
 void launch_rbgs_slabs(const LevelDims& L, const OpView<T>& op, Pair<T> x,
                        Pair<const T> b, int d, const int* planes, int nslab,
-                       const int* colours, int nphase, hipStream_t s);
+                       const int* colours, int nphase, const SlabIO<T>& io,
+                       hipStream_t s);
 template <typename T>
 void launch_jacobi(const LevelDims& L, const OpView<T>& op, Pair<T> x,
                    Pair<const T> b, Pair<T> tmp, const Region& rg,
                    hipStream_t s);
+// r = b - A x over the interior (and per-tile partial sums of r^2 when
+// partials is not null; *nblocks receives their number)
 template <typename T>
 void launch_residual(const LevelDims& L, const OpView<T>& op, Pair<const T> x,
-                     Pair<const T> b, Pair<T> r, const Region& rg,
-                     double* partials, int* nblocks, hipStream_t s);
+                     Pair<const T> b, Pair<T> r, double* partials,
+                     int* nblocks, hipStream_t s);
 template <typename T>
 void launch_restrict(const LevelDims& Lf, const LevelDims& Lc,
                      const TransferView<T>& tv, Pair<const T> rf, Pair<T> bc,
@@ -95,6 +111,13 @@ void launch_prolong(const LevelDims& Lf, const LevelDims& Lc,
 template <typename T>
 void launch_bc_fill(const LevelDims& L, const OpView<T>& op, Pair<T> x,
                     int mode, int level0, const double* bcval, hipStream_t s);
+// The same, also copying the slab planes of every periodic axis d into
+// snap[d][0] (plane 1) and snap[d][1] (plane N_d), P_a1 x P_a2 each
+// (snap == nullptr or a null row: none for that axis).
+template <typename T>
+void launch_bc_fill_snap(const LevelDims& L, const OpView<T>& op, Pair<T> x,
+                         int mode, int level0, const double* bcval,
+                         T* const (*snap)[2], hipStream_t s);
 template <typename T>
 void launch_fill(Pair<T> x, long ncol, T v, hipStream_t s);
 template <typename T>

@@ -27,6 +27,32 @@ __device__ __forceinline__ int cidx(const LevelDims& L, int I, int J, int K)
 
 __device__ __forceinline__ int nci(const LevelDims& L) { return int(L.ncol); }
 
+// Element i of an array addressed by a 32-bit byte offset from a base held
+// in scalar registers (every array is below 2^31 bytes), so that a load is
+// one global_load with a 32-bit VGPR offset instead of a 64-bit address add.
+template <typename T>
+__device__ __forceinline__ T ld(const T* base, unsigned i)
+{
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) +
+                                       size_t(i*unsigned(sizeof(T))));
+}
+
+// Array of colour c of a field.  A select between the two kernel-argument
+// pointers (rather than an index into the argument) keeps the result a
+// global-memory pointer.
+template <typename T>
+__device__ __forceinline__ T* col(const Pair<T>& x, int c)
+{
+    return c ? x.c[1] : x.c[0];
+}
+
+template <typename T>
+__device__ __forceinline__ void st(T* base, unsigned i, T v)
+{
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) +
+                          size_t(i*unsigned(sizeof(T)))) = v;
+}
+
 // Stencil coefficients [C,L,R,B,T,A,F] of padded cell (I,J,K), as assembled
 // by linearGaussLaplacianScheme.C:61-94 (centre = 0 - v_xl - v_xu - v_yl -
 // v_yu - v_zl - v_zu, faces in loop order), before ghost elimination.
@@ -41,34 +67,35 @@ __device__ __forceinline__ void raw_coeffs(const LevelDims& L,
         const T* s = op.a7[colour];
         const int n = nci(L);
 #pragma unroll
-        for (int c = 0; c < 7; c++) a[c] = s[c*n + idx];
+        for (int c = 0; c < 7; c++) a[c] = ld(s, unsigned(c*n + idx));
     }
     else
     {
         if constexpr (OP == OP_CONST)
         {
-            const T wx = op.w[0][I], wy = op.w[1][J], wz = op.w[2][K];
+            const T wx = ld(op.w[0], I), wy = ld(op.w[1], J);
+            const T wz = ld(op.w[2], K);
             const T ax = wy*wz;
             const T ay = wx*wz;
             const T az = wx*wy;
-            a[1] = ax*op.dl[0][I];
-            a[2] = ax*op.dl[0][I+1];
-            a[3] = ay*op.dl[1][J];
-            a[4] = ay*op.dl[1][J+1];
-            a[5] = az*op.dl[2][K];
-            a[6] = az*op.dl[2][K+1];
+            a[1] = ax*ld(op.dl[0], I);
+            a[2] = ax*ld(op.dl[0], I + 1);
+            a[3] = ay*ld(op.dl[1], J);
+            a[4] = ay*ld(op.dl[1], J + 1);
+            a[5] = az*ld(op.dl[2], K);
+            a[6] = az*ld(op.dl[2], K + 1);
         }
         else
         {
             const int n = nci(L);
             const T* wc = op.wf[colour];
             const T* wo = op.wf[colour ^ 1];
-            a[1] = wc[idx];
-            a[2] = wo[idx + L.sI];
-            a[3] = wc[n + idx];
-            a[4] = wo[n + idx + L.Hz];
-            a[5] = wc[2*n + idx];
-            a[6] = wo[2*n + row + p + q];
+            a[1] = ld(wc, idx);
+            a[2] = ld(wo, idx + L.sI);
+            a[3] = ld(wc, n + idx);
+            a[4] = ld(wo, n + idx + L.Hz);
+            a[5] = ld(wc, 2*n + idx);
+            a[6] = ld(wo, 2*n + row + p + q);
         }
         T c0 = T(0) - a[1];
         c0 = c0 - a[2];
@@ -89,6 +116,11 @@ __device__ __forceinline__ void eliminate_cell(const LevelDims& L, int I,
 {
     const bool on[6] = {I == 1, I == L.N[0], J == 1, J == L.N[1],
                         K == 1, K == L.N[2]};
+    // most cells touch no eliminated face: branch around the whole block
+    const bool faceI = (on[0] && L.bc[0] != PER) || (on[1] && L.bc[1] != PER);
+    const bool faceJ = (on[2] && L.bc[2] != PER) || (on[3] && L.bc[3] != PER);
+    const bool faceK = (on[4] && L.bc[4] != PER) || (on[5] && L.bc[5] != PER);
+    if (!(faceI || faceJ || faceK)) return;
 #pragma unroll
     for (int f = 0; f < 6; f++)
     {
@@ -125,163 +157,335 @@ __device__ __forceinline__ double block_sum(double v, double* sm)
     return s;
 }
 
+// A logical 3-D grid of tiles launched as a 1-D grid.  The device hands
+// workgroups to its eight XCDs in turn (workgroup w to XCD w mod 8); tile
+// (w mod 8)*per + w/8 gives every XCD one contiguous run of tiles, so the
+// planes a stencil reads again for the next plane are still in the L2 of
+// the XCD that reads them.  Tile order is x fastest, then y, then z.
+struct Tiles {
+    int gx, gy, gz;
+    int per;   // tiles per XCD
+};
+
+__device__ __forceinline__ bool tile_of(const Tiles& t, int& bx, int& by,
+                                        int& bz)
+{
+    const unsigned w = blockIdx.x;
+    const unsigned lin = (w & 7u)*unsigned(t.per) + (w >> 3);
+    if (lin >= unsigned(t.gx*t.gy*t.gz)) return false;
+    const unsigned r = lin/unsigned(t.gx);
+    bx = int(lin - r*unsigned(t.gx));
+    bz = int(r/unsigned(t.gy));
+    by = int(r - unsigned(bz)*unsigned(t.gy));
+    return true;
+}
+
+__device__ __forceinline__ int tile_index(const Tiles& t, int bx, int by,
+                                          int bz)
+{
+    return bx + t.gx*(by + t.gy*bz);
+}
+
+// Cell-pair kernels: thread (p, J) owns the two cells K = 2p+1 and
+// K = 2p+2 of padded row (I,J).  Cell A (K = 2p+1) sits at position p of
+// colour cA = (I+J+1)&1, cell B (K = 2p+2) at position p+1 of colour
+// cB = (I+J)&1; p runs over (N2+1)/2 pairs, so every lane of a full row
+// holds two interior cells.  Tiles are BX pairs x BY rows, one tile row per
+// plane I: a wavefront is 64 pairs of one row, so J, cA and cB are uniform.
+struct PairPos {
+    int I, J, p;
+    int cA, cB;
+    int row;          // first position of row (I,J)
+    unsigned idx;     // row + p: cell A at idx of colour cA, cell B at idx+1
+                      // of colour cB
+    bool vB;          // K = 2p+2 is an interior cell
+};
+
+__device__ __forceinline__ bool pair_pos(const LevelDims& L, int bx, int by,
+                                         int bz, PairPos& c)
+{
+    c.I = 1 + bz;
+    c.J = __builtin_amdgcn_readfirstlane(1 + by*BY + int(threadIdx.y));
+    c.p = bx*BX + int(threadIdx.x);
+    if (c.J > L.N[1] || c.p >= (L.N[2] + 1)/2) return false;
+    c.cB = (c.I + c.J) & 1;
+    c.cA = c.cB ^ 1;
+    c.row = (c.I*L.P[1] + c.J)*L.Hz;
+    c.idx = unsigned(c.row + c.p);
+    c.vB = 2*c.p + 2 <= L.N[2];
+    return true;
+}
+
+// A x for both cells of a pair (row product in the C,L,R,B,T,A,F order,
+// linearSystemFunctions.H:42-49).  All twelve x values are loaded before any
+// arithmetic.
+template <typename T, int OP>
+__device__ __forceinline__ void pair_Ax(const LevelDims& L,
+                                        const OpView<T>& op, Pair<const T> x,
+                                        const PairPos& c, T Ax[2])
+{
+    const unsigned sI = L.sI, Hz = L.Hz, i = c.idx;
+    // uniform selects between the two kernel-argument pointers keep the
+    // loads in the global address space
+    const T* xa = col(x, c.cA);
+    const T* xb = col(x, c.cB);
+    // cell A (K = 2p+1): own xa[i]; neighbours in xb: -i,+i,-j,+j at i,
+    // -k = K 2p at xb[i], +k = K 2p+2 at xb[i+1] (cell B)
+    const T uA = ld(xa, i), uB = ld(xb, i + 1);
+    const T nA[6] = {ld(xb, i - sI), ld(xb, i + sI), ld(xb, i - Hz),
+                     ld(xb, i + Hz), ld(xb, i), uB};
+    // cell B (K = 2p+2): own xb[i+1]; neighbours in xa at i+1, -k = cell A,
+    // +k = K 2p+3 at xa[i+1]
+    const T nB[6] = {ld(xa, i + 1 - sI), ld(xa, i + 1 + sI),
+                     ld(xa, i + 1 - Hz), ld(xa, i + 1 + Hz), uA,
+                     ld(xa, i + 1)};
+    T a[7];
+    coeffs<T, OP>(L, op, c.cA, int(i), c.row, c.p, 1, c.I, c.J, 2*c.p + 1, a);
+    Ax[0] = a[0]*uA + a[1]*nA[0] + a[2]*nA[1] + a[3]*nA[2] + a[4]*nA[3]
+          + a[5]*nA[4] + a[6]*nA[5];
+    Ax[1] = T(0);
+    if (c.vB)
+    {
+        coeffs<T, OP>(L, op, c.cB, int(i) + 1, c.row, c.p + 1, 0, c.I, c.J,
+                      2*c.p + 2, a);
+        Ax[1] = a[0]*uB + a[1]*nB[0] + a[2]*nB[1] + a[3]*nB[2] + a[4]*nB[3]
+              + a[5]*nB[4] + a[6]*nB[5];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Red-black Gauss-Seidel phase: update every cell of one colour inside
 // region [S,E) (rbgsSmoother.C:65-183):
 //   x_c = (b_c - (A1 x_-i + A2 x_+i + A3 x_-j + A4 x_+j + A5 x_-k + A6 x_+k))
 //         / A0
-// One thread per cell of the colour; lanes run along the contiguous axis.
+// One thread per cell of the colour; lanes run along the contiguous axis;
+// tiles of BX positions x BY rows, one tile row per plane, dealt to the
+// XCDs in contiguous runs of planes.
 template <typename T, int OP>
 __global__ __launch_bounds__(BX*BY) void k_rbgs(LevelDims L, OpView<T> op,
                                                 T* __restrict__ xc,
                                                 const T* __restrict__ xo,
                                                 const T* __restrict__ bcl,
-                                                int colour, Region rg)
+                                                int colour, Region rg,
+                                                Tiles tl)
 {
-    const int I = rg.S[0] + blockIdx.z;
-    const int J = rg.S[1] + blockIdx.y*BY + threadIdx.y;
+    int bx, by, bz;
+    if (!tile_of(tl, bx, by, bz)) return;
+    const int I = rg.S[0] + bz;
+    // a wavefront is one row (BX = 64), so J and everything derived from it
+    // is uniform and lives in scalar registers
+    const int J = __builtin_amdgcn_readfirstlane(rg.S[1] + by*BY +
+                                                 int(threadIdx.y));
     if (J >= rg.E[1]) return;
     const int q = (colour - I - J) & 1;
     const int pmin = (rg.S[2] - q + 1) >> 1;
-    const int p = pmin + blockIdx.x*BX + threadIdx.x;
+    const int p = pmin + bx*BX + int(threadIdx.x);
     const int K = 2*p + q;
     if (K >= rg.E[2]) return;
-    const int sI = L.sI;
+    const unsigned sI = L.sI, Hz = L.Hz;
     const int row = (I*L.P[1] + J)*L.Hz;
-    const int idx = row + p;
+    const unsigned idx = unsigned(row + p);
+    // issue every streamed load before the coefficient arithmetic
+    const T nb[6] = {ld(xo, idx - sI), ld(xo, idx + sI), ld(xo, idx - Hz),
+                     ld(xo, idx + Hz), ld(xo, idx - 1 + q), ld(xo, idx + q)};
+    const T bv = ld(bcl, idx);
     T a[7];
-    coeffs<T, OP>(L, op, colour, idx, row, p, q, I, J, K, a);
+    coeffs<T, OP>(L, op, colour, int(idx), row, p, q, I, J, K, a);
     const T P =
-        a[1]*xo[idx - sI]
-      + a[2]*xo[idx + sI]
-      + a[3]*xo[idx - L.Hz]
-      + a[4]*xo[idx + L.Hz]
-      + a[5]*xo[row + p - 1 + q]
-      + a[6]*xo[row + p + q];
-    xc[idx] = (bcl[idx] - P)/a[0];
+        a[1]*nb[0]
+      + a[2]*nb[1]
+      + a[3]*nb[2]
+      + a[4]*nb[3]
+      + a[5]*nb[4]
+      + a[6]*nb[5];
+    st(xc, idx, (bv - P)/a[0]);
 }
 
 // All red-black phases of the width-1 boundary slabs of one face pair
 // (rbgsSmoother::SweepBoundary, rbgsSmoother.C:186-374) in one launch.  A
 // slab is a plane of cells whose out-of-plane neighbours (the interior layer
 // and the ghost layer filled just before) stay fixed while its phases run,
-// so every phase is a 2-D red-black update of the plane.  Each workgroup
-// owns a TT x TT tile, loads the tile grown by H cells (H = number of
-// phases) into LDS and runs the phases there: phase p updates the tile grown
-// by H-1-p cells, whose neighbours are exact after phase p-1, so the tile is
-// exact after the last phase and the halo work is redundant but identical.
+// so every phase is a 2-D red-black update of the plane.
+//
+// A workgroup owns a TT x TT tile (or the whole plane when it fits in
+// 32 x 32 with its ghost ring), loads the tile grown by H cells (H = number
+// of phases) into LDS and runs the phases there: phase p updates the tile
+// grown by H-1-p cells, whose neighbours are exact after phase p-1, so the
+// tile is exact after the last phase and the halo work is redundant but
+// identical.  The in-plane values are read from a snapshot of the plane
+// taken when the ghosts were filled (or written into it by an earlier slab
+// launch), never from x, which the neighbouring workgroups overwrite while
+// this one runs.  Everything else a cell needs (b, its two out-of-plane
+// neighbours, its coefficients) is loaded once before the first phase.
 // Opposite slabs of a pair are independent (they are at least two cells
-// apart) and run as blockIdx.z = 0/1.  The update is the row product in the
-// reference order with the two out-of-plane terms read from global memory.
+// apart) and run as blockIdx.z = 0/1.  Results go to x and to the snapshots
+// of the slab launches that follow (tptr/tdim/tplane), whose planes meet
+// this one along a line.
+template <typename T>
 struct SlabArgs {
-    int d;            // slab normal axis
-    int plane[2];     // padded index of each slab plane
-    int nphase;       // H
+    int plane[2];      // padded index of each slab plane
+    int nphase;        // H
     int colours[2][4];
-    int tt;           // output tile edge
+    int tt;            // output tile edge; 0: one tile holds the whole plane
+    const T* src[2];   // snapshot of each slab plane, P_a1 x P_a2
+    int ntarget;
+    T* tptr[6];
+    int tdim[6];
+    int tplane[6];
 };
 
-template <typename T, int OP>
+constexpr int SLAB_W = 32;   // LDS tile edge
+
+template <typename T, int OP, int D>
 __global__ __launch_bounds__(256) void k_rbgs_slabs(LevelDims L, OpView<T> op,
                                                     Pair<T> x, Pair<const T> b,
-                                                    SlabArgs sa)
+                                                    SlabArgs<T> sa)
 {
-    extern __shared__ unsigned char slab_smem[];
-    T* xs = reinterpret_cast<T*>(slab_smem);
-    const int d = sa.d;
-    const int a1 = d == 0 ? 1 : 0;
-    const int a2 = d == 2 ? 1 : 2;
+    constexpr int A1 = D == 0 ? 1 : 0;
+    constexpr int A2 = D == 2 ? 1 : 2;
+    __shared__ T xs[SLAB_W][SLAB_W + 1];
+    const int slab = blockIdx.z;
     const int H = sa.nphase;
-    const int W = sa.tt + 2*H;
-    const int Ip = sa.plane[blockIdx.z];
-    const int o1 = 1 + blockIdx.y*sa.tt - H;
-    const int o2 = 1 + blockIdx.x*sa.tt - H;
-    const int tid = threadIdx.x, nt = blockDim.x;
-    for (int e = tid; e < W*W; e += nt)
+    const int n1 = L.N[A1], n2 = L.N[A2];
+    const bool whole = sa.tt == 0;
+    const int W1 = whole ? n1 + 2 : sa.tt + 2*H;
+    const int W2 = whole ? n2 + 2 : sa.tt + 2*H;
+    const int o1 = whole ? 0 : 1 + int(blockIdx.y)*sa.tt - H;
+    const int o2 = whole ? 0 : 1 + int(blockIdx.x)*sa.tt - H;
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const T* src = sa.src[slab];
+    const int Ps = L.P[A2];
+
+    // the four cells of this thread: u1 = ty + 16*(k>>1), u2 = tx + 16*(k&1)
+    bool act[4];
+    int colk[4];
+    unsigned idxk[4];
+    T a[4][7], bv[4], om[4], opl[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++)
     {
-        const int c1 = o1 + e/W, c2 = o2 + e % W;
+        const int u1 = ty + 16*(k >> 1), u2 = tx + 16*(k & 1);
+        const int c1 = o1 + u1, c2 = o2 + u2;
+        const bool inTile = u1 < W1 && u2 < W2;
         T v = T(0);
-        if (c1 >= 0 && c1 < L.P[a1] && c2 >= 0 && c2 < L.P[a2])
+        if (inTile && c1 >= 0 && c1 < L.P[A1] && c2 >= 0 && c2 < Ps)
+            v = src[c1*Ps + c2];
+        if (inTile) xs[u1][u2] = v;
+        act[k] = inTile && c1 >= 1 && c1 <= n1 && c2 >= 1 && c2 <= n2;
+        colk[k] = 0;
+        idxk[k] = 0;
+        bv[k] = om[k] = opl[k] = T(0);
+#pragma unroll
+        for (int m = 0; m < 7; m++) a[k][m] = T(0);
+        if (act[k])
         {
             int g[3];
-            g[d] = Ip; g[a1] = c1; g[a2] = c2;
-            v = x.c[(g[0] + g[1] + g[2]) & 1][cidx(L, g[0], g[1], g[2])];
+            g[D] = sa.plane[slab];
+            g[A1] = c1;
+            g[A2] = c2;
+            const int I = g[0], J = g[1], K = g[2];
+            const int colour = (I + J + K) & 1;
+            const int q = K & 1, p = K >> 1;
+            const int row = (I*L.P[1] + J)*L.Hz;
+            const unsigned idx = unsigned(row + p);
+            const T* xo = col(x, colour ^ 1);
+            if (D == 0)
+            {
+                om[k] = ld(xo, idx - unsigned(L.sI));
+                opl[k] = ld(xo, idx + unsigned(L.sI));
+            }
+            else if (D == 1)
+            {
+                om[k] = ld(xo, idx - unsigned(L.Hz));
+                opl[k] = ld(xo, idx + unsigned(L.Hz));
+            }
+            else
+            {
+                om[k] = ld(xo, idx - 1 + q);
+                opl[k] = ld(xo, idx + q);
+            }
+            bv[k] = ld(col(b, colour), idx);
+            coeffs<T, OP>(L, op, colour, int(idx), row, p, q, I, J, K, a[k]);
+            colk[k] = colour;
+            idxk[k] = idx;
         }
-        xs[e] = v;
     }
     __syncthreads();
     for (int ph = 0; ph < H; ph++)
     {
-        const int colour = sa.colours[blockIdx.z][ph];
-        const int grow = H - 1 - ph;
-        const int lo = H - grow, hi = W - (H - grow);   // LDS window
-        for (int e = tid; e < W*W; e += nt)
+        const int colour = sa.colours[slab][ph];
+        const int lo = whole ? 0 : ph + 1;
+        const int hi1 = whole ? W1 : W1 - ph - 1;
+        const int hi2 = whole ? W2 : W2 - ph - 1;
+#pragma unroll
+        for (int k = 0; k < 4; k++)
         {
-            const int u1 = e/W, u2 = e % W;
-            if (u1 < lo || u1 >= hi || u2 < lo || u2 >= hi) continue;
-            const int c1 = o1 + u1, c2 = o2 + u2;
-            if (c1 < 1 || c1 > L.N[a1] || c2 < 1 || c2 > L.N[a2]) continue;
-            int g[3];
-            g[d] = Ip; g[a1] = c1; g[a2] = c2;
-            const int I = g[0], J = g[1], K = g[2];
-            if (((I + J + K) & 1) != colour) continue;
-            const int q = K & 1;
-            const int p = K >> 1;
-            const int row = (I*L.P[1] + J)*L.Hz;
-            const int idx = row + p;
-            T a[7];
-            coeffs<T, OP>(L, op, colour, idx, row, p, q, I, J, K, a);
-            const T* xo = x.c[colour ^ 1];
+            const int u1 = ty + 16*(k >> 1), u2 = tx + 16*(k & 1);
+            if (!act[k] || colk[k] != colour || u1 < lo || u1 >= hi1 ||
+                u2 < lo || u2 >= hi2)
+                continue;
             // neighbour values in the reference order -i,+i,-j,+j,-k,+k
             T nb[6];
-            const int e1 = (u1)*W, e1m = (u1 - 1)*W, e1p = (u1 + 1)*W;
-            if (d == 0)
+            if (D == 0)
             {
-                nb[0] = xo[idx - L.sI];
-                nb[1] = xo[idx + L.sI];
-                nb[2] = xs[e1m + u2];
-                nb[3] = xs[e1p + u2];
-                nb[4] = xs[e1 + u2 - 1];
-                nb[5] = xs[e1 + u2 + 1];
+                nb[0] = om[k];
+                nb[1] = opl[k];
+                nb[2] = xs[u1 - 1][u2];
+                nb[3] = xs[u1 + 1][u2];
+                nb[4] = xs[u1][u2 - 1];
+                nb[5] = xs[u1][u2 + 1];
             }
-            else if (d == 1)
+            else if (D == 1)
             {
-                nb[0] = xs[e1m + u2];
-                nb[1] = xs[e1p + u2];
-                nb[2] = xo[idx - L.Hz];
-                nb[3] = xo[idx + L.Hz];
-                nb[4] = xs[e1 + u2 - 1];
-                nb[5] = xs[e1 + u2 + 1];
+                nb[0] = xs[u1 - 1][u2];
+                nb[1] = xs[u1 + 1][u2];
+                nb[2] = om[k];
+                nb[3] = opl[k];
+                nb[4] = xs[u1][u2 - 1];
+                nb[5] = xs[u1][u2 + 1];
             }
             else
             {
-                nb[0] = xs[e1m + u2];
-                nb[1] = xs[e1p + u2];
-                nb[2] = xs[e1 + u2 - 1];
-                nb[3] = xs[e1 + u2 + 1];
-                nb[4] = xo[row + p - 1 + q];
-                nb[5] = xo[row + p + q];
+                nb[0] = xs[u1 - 1][u2];
+                nb[1] = xs[u1 + 1][u2];
+                nb[2] = xs[u1][u2 - 1];
+                nb[3] = xs[u1][u2 + 1];
+                nb[4] = om[k];
+                nb[5] = opl[k];
             }
             const T P =
-                a[1]*nb[0]
-              + a[2]*nb[1]
-              + a[3]*nb[2]
-              + a[4]*nb[3]
-              + a[5]*nb[4]
-              + a[6]*nb[5];
-            xs[e] = (b.c[colour][idx] - P)/a[0];
+                a[k][1]*nb[0]
+              + a[k][2]*nb[1]
+              + a[k][3]*nb[2]
+              + a[k][4]*nb[3]
+              + a[k][5]*nb[4]
+              + a[k][6]*nb[5];
+            xs[u1][u2] = (bv[k] - P)/a[k][0];
         }
         __syncthreads();
     }
-    for (int e = tid; e < W*W; e += nt)
+#pragma unroll
+    for (int k = 0; k < 4; k++)
     {
-        const int u1 = e/W, u2 = e % W;
-        if (u1 < H || u1 >= W - H || u2 < H || u2 >= W - H) continue;
-        const int c1 = o1 + u1, c2 = o2 + u2;
-        if (c1 < 1 || c1 > L.N[a1] || c2 < 1 || c2 > L.N[a2]) continue;
+        const int u1 = ty + 16*(k >> 1), u2 = tx + 16*(k & 1);
+        if (!act[k]) continue;
+        if (!whole && (u1 < H || u1 >= H + sa.tt || u2 < H || u2 >= H + sa.tt))
+            continue;
+        const T v = xs[u1][u2];
+        st(col(x, colk[k]), idxk[k], v);
         int g[3];
-        g[d] = Ip; g[a1] = c1; g[a2] = c2;
-        x.c[(g[0] + g[1] + g[2]) & 1][cidx(L, g[0], g[1], g[2])] = xs[e];
+        g[D] = sa.plane[slab];
+        g[A1] = o1 + u1;
+        g[A2] = o2 + u2;
+        for (int t = 0; t < sa.ntarget; t++)
+        {
+            const int d = sa.tdim[t];
+            if (g[d] != sa.tplane[t]) continue;
+            const int e1 = d == 0 ? g[1] : g[0];
+            const int e2 = d == 2 ? g[1] : g[2];
+            const int pe = d == 2 ? L.P[1] : L.P[2];
+            sa.tptr[t][e1*pe + e2] = v;
+        }
     }
 }
 
@@ -343,54 +547,39 @@ __global__ __launch_bounds__(BX*BY) void k_copy_region(LevelDims L,
 
 // ---------------------------------------------------------------------------
 // Residual r = b - A x (linearSystemResidual.C:75-95; row product in
-// C,L,R,B,T,A,F order, linearSystemFunctions.H:42-49).  Optionally writes
-// per-block partial sums of r^2 (deterministic order).
+// C,L,R,B,T,A,F order, linearSystemFunctions.H:42-49) over the interior.
+// Optionally writes per-tile partial sums of r^2 (deterministic order).
 template <typename T, int OP, bool NORM>
 __global__ __launch_bounds__(BX*BY) void k_residual(LevelDims L, OpView<T> op,
                                                     Pair<const T> x,
                                                     Pair<const T> b,
-                                                    Pair<T> r, Region rg,
+                                                    Pair<T> r, Tiles tl,
                                                     double* partials)
 {
     __shared__ double sm[BX*BY/64];
-    const int I = rg.S[0] + blockIdx.z;
-    const int J = rg.S[1] + blockIdx.y*BY + threadIdx.y;
-    const int p = (rg.S[2] >> 1) + blockIdx.x*BX + threadIdx.x;
+    int bx, by, bz;
+    if (!tile_of(tl, bx, by, bz)) return;
+    PairPos c;
     double acc = 0.0;
-    if (J < rg.E[1])
+    if (pair_pos(L, bx, by, bz, c))
     {
-        const int sI = L.sI;
-        const int row = (I*L.P[1] + J)*L.Hz;
-        const int idx = row + p;
-#pragma unroll
-        for (int h = 0; h < 2; h++)
+        const T bA = ld(col(b, c.cA), c.idx), bB = ld(col(b, c.cB), c.idx + 1);
+        T Ax[2];
+        pair_Ax<T, OP>(L, op, x, c, Ax);
+        const T rA = bA - Ax[0], rB = bB - Ax[1];
+        st(col(r, c.cA), c.idx, rA);
+        if (NORM) acc += (double)rA*(double)rA;
+        if (c.vB)
         {
-            const int K = 2*p + h;
-            if (K < rg.S[2] || K >= rg.E[2]) continue;
-            const int colour = (I + J + K) & 1;
-            const T* xc = x.c[colour];
-            const T* xo = x.c[colour ^ 1];
-            T a[7];
-            coeffs<T, OP>(L, op, colour, idx, row, p, h, I, J, K, a);
-            const T Ax =
-                a[0]*xc[idx]
-              + a[1]*xo[idx - sI]
-              + a[2]*xo[idx + sI]
-              + a[3]*xo[idx - L.Hz]
-              + a[4]*xo[idx + L.Hz]
-              + a[5]*xo[row + p - 1 + h]
-              + a[6]*xo[row + p + h];
-            const T res = b.c[colour][idx] - Ax;
-            r.c[colour][idx] = res;
-            if (NORM) acc += (double)res*(double)res;
+            st(col(r, c.cB), c.idx + 1, rB);
+            if (NORM) acc += (double)rB*(double)rB;
         }
     }
     if (NORM)
     {
         const double s = block_sum(acc, sm);
         if (threadIdx.x == 0 && threadIdx.y == 0)
-            partials[blockIdx.x + gridDim.x*(blockIdx.y + gridDim.y*blockIdx.z)]
-                = s;
+            partials[tile_index(tl, bx, by, bz)] = s;
     }
 }
 
@@ -398,29 +587,58 @@ __global__ __launch_bounds__(BX*BY) void k_residual(LevelDims L, OpView<T> op,
 // Scaled linear restriction (linearRestrictionScheme.C:103-160, weights
 // :15-80): b_c = cvc * sum_q (w_q * r_q) * (1/cvf_q), q over the 8 children
 // (z outer, x inner), w_q = ((fx*fy)*fz) zeroed when <= 1e-12.
+//
+// One thread per pair of coarse cells K = 2p+1, 2p+2 (cell-pair tiles of
+// the coarse level); the 16 fine values are loaded before any arithmetic.
 template <typename T>
-__global__ __launch_bounds__(BX*BY) void k_restrict(LevelDims Lf, LevelDims Lc,
+__global__ __launch_bounds__(BX*BY) void k_restrict(LevelDims Lf,
+                                                    LevelDims Lc,
                                                     TransferView<T> tv,
                                                     Pair<const T> rf,
-                                                    Pair<T> bc)
+                                                    Pair<T> bc, Tiles tl)
 {
-    const int I = 1 + blockIdx.z;
-    const int J = 1 + blockIdx.y*BY + threadIdx.y;
-    const int p = blockIdx.x*BX + threadIdx.x;
-    if (J > Lc.N[1]) return;
+    int bx, by, bz;
+    if (!tile_of(tl, bx, by, bz)) return;
+    PairPos c;
+    if (!pair_pos(Lc, bx, by, bz, c)) return;
+    const int I = c.I, J = c.J;
     const int R0 = Lc.R[0], R1 = Lc.R[1], R2 = Lc.R[2];
     const int ox = R0 > 1, oy = R1 > 1, oz = R2 > 1;
+    int fi[2], fj[2];
+#pragma unroll
+    for (int k = 0; k < 2; k++)
+    {
+        fi[k] = (I - 1)*R0 + k*ox + 1;
+        fj[k] = (J - 1)*R1 + k*oy + 1;
+    }
+    // cell h = 0: K = 2p+1, h = 1: K = 2p+2 (a missing B reuses A's K)
+    const int Kh[2] = {2*c.p + 1, c.vB ? 2*c.p + 2 : 2*c.p + 1};
+    // fine values [h][cz][cy][cx]
+    T rv[2][2][2][2];
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+    for (int cz = 0; cz < 2; cz++)
+#pragma unroll
+    for (int cy = 0; cy < 2; cy++)
+#pragma unroll
+    for (int cx = 0; cx < 2; cx++)
+    {
+        const int fk = (Kh[h] - 1)*R2 + cz*oz + 1;
+        rv[h][cz][cy][cx] = ld(rf.c[(fi[cx] + fj[cy] + fk) & 1],
+                               unsigned(cidx(Lf, fi[cx], fj[cy], fk)));
+    }
     const T ux = tv.ru[0][I], uy = tv.ru[1][J];
     const T fx[2] = {T(1.0) - ux, ux};
     const T fy[2] = {T(1.0) - uy, uy};
+    T out[2];
 #pragma unroll
     for (int h = 0; h < 2; h++)
     {
-        const int K = 2*p + h;
-        if (K < 1 || K > Lc.N[2]) continue;
+        const int K = Kh[h];
         const T uz = tv.ru[2][K];
         const T fz[2] = {T(1.0) - uz, uz};
-        T v = T(0);
+        T sum = T(0);
 #pragma unroll
         for (int cz = 0; cz < 2; cz++)
 #pragma unroll
@@ -433,24 +651,22 @@ __global__ __launch_bounds__(BX*BY) void k_restrict(LevelDims Lf, LevelDims Lc,
             if (w <= T(1e-12)) w = T(0);
             if (w != T(0))
             {
-                const int fi = (I - 1)*R0 + cx*ox + 1;
-                const int fj = (J - 1)*R1 + cy*oy + 1;
                 const int fk = (K - 1)*R2 + cz*oz + 1;
-                T cvf = tv.fw[0][fi]*tv.fw[1][fj];
+                T cvf = tv.fw[0][fi[cx]]*tv.fw[1][fj[cy]];
                 cvf = cvf*tv.fw[2][fk];
                 cvf = cvf > T(1e-16) ? cvf : T(1e-16);
                 const T icvf = T(1.0)/cvf;
-                const int fc = (fi + fj + fk) & 1;
-                const T t = w*rf.c[fc][cidx(Lf, fi, fj, fk)];
-                v = v + t*icvf;
+                const T t = w*rv[h][cz][cy][cx];
+                sum = sum + t*icvf;
             }
         }
         T cvc = tv.cw[0][I]*tv.cw[1][J];
         cvc = cvc*tv.cw[2][K];
         cvc = cvc > T(1e-16) ? cvc : T(1e-16);
-        v = v*cvc;
-        bc.c[(I + J + K) & 1][(I*Lc.P[1] + J)*Lc.Hz + p] = v;
+        out[h] = sum*cvc;
     }
+    st(col(bc, c.cA), c.idx, out[0]);
+    if (c.vB) st(col(bc, c.cB), c.idx + 1, out[1]);
 }
 
 // Linear prolongation with plusEqOp (linearProlongationScheme.C:112-147,
@@ -459,33 +675,31 @@ __global__ __launch_bounds__(BX*BY) void k_restrict(LevelDims Lf, LevelDims Lc,
 // g = 1-u (abc 0) or u (abc 1); u = 1/2 on all axes when any u lies outside
 // [0,1) (:89-93).
 //
-// One thread per pair of fine cells K = 2p, 2p+1.  Along z the two cells use
-// the same two coarse indices (for R = 2 the odd cell's parent is the even
-// cell's neighbour and vice versa; for R = 1 both are {2p, 2p+1}), so the 8
-// coarse values are loaded once per pair and each cell sums them in its own
-// parent/neighbour order.
+// One thread per pair of fine cells K = 2p+1 (k = 2p even: neighbour
+// below) and K = 2p+2 (k = 2p+1 odd: neighbour above).  Their coarse z
+// indices lie in zbase..zbase+nz-1 (R = 2: parents p+1, neighbours p and
+// p+2; R = 1: cells 2p+1, 2p+2 with neighbours 2p and 2p+3), loaded once
+// for the 4 coarse columns.
 template <typename T>
 __global__ __launch_bounds__(BX*BY) void k_prolong(LevelDims Lf, LevelDims Lc,
                                                    TransferView<T> tv,
                                                    Pair<const T> xc,
-                                                   Pair<T> xf)
+                                                   Pair<T> xf, Tiles tl)
 {
-    const int I = 1 + blockIdx.z;
-    const int J = 1 + blockIdx.y*BY + threadIdx.y;
-    const int p = blockIdx.x*BX + threadIdx.x;
-    if (J > Lf.N[1]) return;
-    const int K0 = 2*p, K1 = 2*p + 1;
-    const bool v0 = K0 >= 1 && K0 <= Lf.N[2];
-    const bool v1 = K1 >= 1 && K1 <= Lf.N[2];
-    if (!v0 && !v1) return;
+    int bx, by, bz;
+    if (!tile_of(tl, bx, by, bz)) return;
+    PairPos c;
+    if (!pair_pos(Lf, bx, by, bz, c)) return;
+    const int I = c.I, J = c.J, p = c.p;
     const int i = I - 1, j = J - 1;
     const int offx = (i & 1) ? 1 : -1, offy = (j & 1) ? 1 : -1;
     const int ci[2] = {i/Lc.R[0] + 1, i/Lc.R[0] + 1 + offx};
     const int cj[2] = {j/Lc.R[1] + 1, j/Lc.R[1] + 1 + offy};
-    // coarse z indices shared by both cells: zl < zh
-    const int zl = Lc.R[2] == 1 ? 2*p : p;
-    const int zh = zl + 1;
-    T c[2][2][2];   // [cz: zl/zh][cy][cx]
+    const int R2 = Lc.R[2];
+    const int zbase = R2 == 1 ? 2*p : p;
+    const int nz = R2 == 1 ? 4 : 3;
+    // coarse values [z - zbase][cy][cx]
+    T cv[4][2][2];
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
@@ -493,23 +707,27 @@ __global__ __launch_bounds__(BX*BY) void k_prolong(LevelDims Lf, LevelDims Lc,
     {
         const int base = (ci[a]*Lc.P[1] + cj[bb])*Lc.Hz;
         const int par = ci[a] + cj[bb];
-        c[0][bb][a] = xc.c[(par + zl) & 1][base + (zl >> 1)];
-        c[1][bb][a] = xc.c[(par + zh) & 1][base + (zh >> 1)];
+#pragma unroll
+        for (int z = 0; z < 4; z++)
+        {
+            // for nz = 3 the fourth value repeats the third and is unused
+            const int zz = zbase + (z < nz ? z : nz - 1);
+            cv[z][bb][a] = ld(xc.c[(par + zz) & 1], unsigned(base + (zz >> 1)));
+        }
     }
+    const T old[2] = {ld(col(xf, c.cA), c.idx), ld(col(xf, c.cB), c.idx + 1)};
     const T ux = tv.pu[0][I], uy = tv.pu[1][J];
-    const int colour0 = (I + J + K0) & 1;
-    const int idx = (I*Lf.P[1] + J)*Lf.Hz + p;
+    T add[2] = {T(0), T(0)};
 #pragma unroll
     for (int h = 0; h < 2; h++)
     {
-        if (h == 0 ? !v0 : !v1) continue;
-        const int K = 2*p + h;
-        const int k = K - 1;
-        // parent / neighbour along z as indices into c[0..1]
-        const int offz = (k & 1) ? 1 : -1;
-        const int bz = k/Lc.R[2] + 1;
-        const int zpar = bz == zl ? 0 : 1;
-        const int znb = (bz + offz) == zl ? 0 : 1;
+        if (h == 1 && !c.vB) continue;
+        const int K = 2*p + 1 + h;
+        // parent and neighbour as indices into cv: cell A (k = 2p even)
+        // has parent 1, neighbour 0; cell B (k = 2p+1 odd) parent 1,
+        // neighbour 2 for R = 2, parent 2, neighbour 3 for R = 1
+        const int zpar = h == 0 ? 1 : (R2 == 1 ? 2 : 1);
+        const int znb = h == 0 ? 0 : zpar + 1;
         T u0 = ux, u1 = uy, u2 = tv.pu[2][K];
         const bool inside = !(u0 < T(0) || u0 >= T(1) || u1 < T(0) ||
                               u1 >= T(1) || u2 < T(0) || u2 >= T(1));
@@ -530,13 +748,14 @@ __global__ __launch_bounds__(BX*BY) void k_prolong(LevelDims Lf, LevelDims Lc,
             w = w*gz[cz];
             if (w != T(0))
             {
-                const T t = w*c[zsel[cz]][cy][cx];
+                const T t = w*cv[zsel[cz]][cy][cx];
                 value = value + t;
             }
         }
-        T* dst = xf.c[colour0 ^ h];
-        dst[idx] = dst[idx] + value;
+        add[h] = value;
     }
+    st(col(xf, c.cA), c.idx, old[0] + add[0]);
+    if (c.vB) st(col(xf, c.cB), c.idx + 1, old[1] + add[1]);
 }
 
 // ---------------------------------------------------------------------------
@@ -599,59 +818,255 @@ __device__ __forceinline__ void ghost_cell(const LevelDims& L,
     x.c[(I + J + K) & 1][cidx(L, I, J, K)] = val;
 }
 
-__device__ __forceinline__ bool shell_cell(const LevelDims& L, long t, int& I,
+// The ghost shell as the coarse solve walks it: flat 32-bit index, K
+// fastest.  A: planes I = 0 and I = P0-1; B: rows J = 0 and J = P1-1 of
+// planes I = 1..P0-2; C: cells K = 0 and K = P2-1 of the remaining rows.
+struct Shell {
+    int nA, nB, nC;
+};
+
+__device__ __forceinline__ bool shell_cell(const LevelDims& L,
+                                           const Shell& sh, int t, int& I,
                                            int& J, int& K)
 {
-    const long P0 = L.P[0], P1 = L.P[1], P2 = L.P[2];
-    const long nA = 2*P1*P2;
-    const long nB = 2*(P0 - 2)*P2;
-    const long nC = 2*(P0 - 2)*(P1 - 2);
-    if (t < nA)
+    const int P0 = L.P[0], P1 = L.P[1], P2 = L.P[2];
+    if (t < sh.nA)
     {
-        I = t < P1*P2 ? 0 : int(P0 - 1);
-        const long u = t % (P1*P2);
-        J = int(u/P2);
-        K = int(u % P2);
+        const int half = P1*P2;
+        const int side = t >= half;
+        const int u = t - side*half;
+        I = side ? P0 - 1 : 0;
+        J = u/P2;
+        K = u - J*P2;
         return true;
     }
-    t -= nA;
-    if (t < nB)
+    t -= sh.nA;
+    if (t < sh.nB)
     {
-        const long half = (P0 - 2)*P2;
-        J = t < half ? 0 : int(P1 - 1);
-        const long u = t % half;
-        I = 1 + int(u/P2);
-        K = int(u % P2);
+        const int half = (P0 - 2)*P2;
+        const int side = t >= half;
+        const int u = t - side*half;
+        J = side ? P1 - 1 : 0;
+        const int i = u/P2;
+        I = 1 + i;
+        K = u - i*P2;
         return true;
     }
-    t -= nB;
-    if (t < nC)
+    t -= sh.nB;
+    if (t < sh.nC)
     {
-        const long half = (P0 - 2)*(P1 - 2);
-        K = t < half ? 0 : int(P2 - 1);
-        const long u = t % half;
-        I = 1 + int(u/(P1 - 2));
-        J = 1 + int(u % (P1 - 2));
+        const int half = (P0 - 2)*(P1 - 2);
+        const int side = t >= half;
+        const int u = t - side*half;
+        K = side ? P2 - 1 : 0;
+        const int i = u/(P1 - 2);
+        I = 1 + i;
+        J = 1 + (u - i*(P1 - 2));
         return true;
     }
     return false;
 }
 
-__host__ __device__ inline long shell_count(const LevelDims& L)
+__host__ __device__ inline Shell shell_all(const LevelDims& L)
 {
-    const long P0 = L.P[0], P1 = L.P[1], P2 = L.P[2];
-    return 2*P1*P2 + 2*(P0 - 2)*P2 + 2*(P0 - 2)*(P1 - 2);
+    const int P0 = L.P[0], P1 = L.P[1], P2 = L.P[2];
+    Shell sh;
+    sh.nA = 2*P1*P2;
+    sh.nB = 2*(P0 - 2)*P2;
+    sh.nC = 2*(P0 - 2)*(P1 - 2);
+    return sh;
 }
+
+// Face ghost of face 2D+side at in-face interior cell (c1, c2) of the two
+// other axes, by the rules of ghost_cell with every index static.
+template <typename T, int D>
+__device__ __forceinline__ void face_ghost(const LevelDims& L,
+                                           const OpView<T>& op, Pair<T> x,
+                                           int mode, int level0,
+                                           const BcVals& bv, int side, int c1,
+                                           int c2)
+{
+    constexpr int A1 = D == 0 ? 1 : 0;
+    constexpr int A2 = D == 2 ? 1 : 2;
+    const int bcf = side ? L.bc[2*D + 1] : L.bc[2*D];
+    const bool per = bcf == PER;
+    if (mode == 0 && !per) return;
+    if (mode == 2 && per) return;
+    const int N = L.N[D];
+    int g[3], s[3];
+    g[D] = side ? N + 1 : 0;
+    g[A1] = c1;
+    g[A2] = c2;
+    s[A1] = c1;
+    s[A2] = c2;
+    s[D] = per ? (side ? 1 : N) : (side ? N : 1);
+    const T xs = ld(col(x, (s[0] + s[1] + s[2]) & 1),
+                    unsigned(cidx(L, s[0], s[1], s[2])));
+    T val;
+    if (per)
+        val = xs;
+    else
+    {
+        const T H = T(level0 ? 1.0 : 0.0);
+        const T bvf = T(side ? bv.v[2*D + 1] : bv.v[2*D]);
+        if (bcf == NEU)
+        {
+            const T delta = ld(op.dl[D], unsigned(side ? N + 1 : 1));
+            val = xs + H*bvf/delta;
+        }
+        else
+            val = H*T(2.0)*bvf - xs;
+    }
+    st(col(x, (g[0] + g[1] + g[2]) & 1), unsigned(cidx(L, g[0], g[1], g[2])),
+       val);
+}
+
+// Edge or corner ghost (two or three indices outside): periodic image when
+// every face it borders is periodic, else a copy of the adjacent interior
+// cell.
+template <typename T>
+__device__ __forceinline__ void edge_ghost(const LevelDims& L, Pair<T> x,
+                                           int I, int J, int K)
+{
+    const int g[3] = {I, J, K};
+    int bo[3];
+    bool allPer = true;
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+    {
+        bo[d] = g[d] == 0 ? -1 : (g[d] == L.P[d] - 1 ? 1 : 0);
+        const int bcf = bo[d] < 0 ? L.bc[2*d] : L.bc[2*d + 1];
+        if (bo[d] != 0 && bcf != PER) allPer = false;
+    }
+    int s[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+        s[d] = allPer ? g[d] - bo[d]*L.N[d] : g[d] - bo[d];
+    const T v = ld(col(x, (s[0] + s[1] + s[2]) & 1),
+                   unsigned(cidx(L, s[0], s[1], s[2])));
+    st(col(x, (I + J + K) & 1), unsigned(cidx(L, I, J, K)), v);
+}
+
+// Ghost fill in up to four groups of threads, each padded to whole blocks
+// so that a block serves one group: group D < 3 the two faces normal to
+// axis D (face ghosts only), group 3 the edges and corners (mode 1).
+struct BcGroups {
+    int start[8];     // first thread of each group; start[7] = total
+    int count[7];     // cells in each group (0: not filled)
+};
+
+// Snapshot cell (c1, c2) of the slab plane of axis D in slot `slot` (plane 1
+// or N_D) as x holds it once the periodic face ghosts are filled: an
+// in-plane face ghost of a periodic face takes its periodic image, every
+// other cell its current value.  Only cells this kernel never writes are
+// read, so the snapshot does not depend on the order of the threads.
+template <typename T, int D>
+__device__ __forceinline__ void snap_cell(const LevelDims& L, Pair<T> x,
+                                          int slot, int c1, int c2, T* dst)
+{
+    constexpr int A1 = D == 0 ? 1 : 0;
+    constexpr int A2 = D == 2 ? 1 : 2;
+    int g[3];
+    g[D] = slot ? L.N[D] : 1;
+    g[A1] = c1;
+    g[A2] = c2;
+    const bool out1 = c1 == 0 || c1 == L.P[A1] - 1;
+    const bool out2 = c2 == 0 || c2 == L.P[A2] - 1;
+    if (out1 && !out2)
+    {
+        const int bcf = c1 == 0 ? L.bc[2*A1] : L.bc[2*A1 + 1];
+        if (bcf == PER) g[A1] = c1 == 0 ? L.N[A1] : 1;
+    }
+    else if (out2 && !out1)
+    {
+        const int bcf = c2 == 0 ? L.bc[2*A2] : L.bc[2*A2 + 1];
+        if (bcf == PER) g[A2] = c2 == 0 ? L.N[A2] : 1;
+    }
+    dst[c1*L.P[A2] + c2] = ld(col(x, (g[0] + g[1] + g[2]) & 1),
+                              unsigned(cidx(L, g[0], g[1], g[2])));
+}
+
+template <typename T>
+struct SnapPtrs {
+    T* p[3][2];
+};
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_bc_fill(LevelDims L, OpView<T> op,
                                                  Pair<T> x, int mode,
-                                                 int level0, BcVals bv)
+                                                 int level0, BcVals bv,
+                                                 BcGroups gr, SnapPtrs<T> sn)
 {
-    const long t = (long)blockIdx.x*blockDim.x + threadIdx.x;
+    const int b0 = blockIdx.x*256;
+    int g = 0;
+    while (g < 6 && b0 >= gr.start[g + 1]) g++;
+    const int t = b0 + int(threadIdx.x) - gr.start[g];
+    if (t >= gr.count[g]) return;
+    if (g >= 4)
+    {
+        // slab-plane snapshots of axis g-4, both slots
+        const int d = g - 4;
+        const int p1 = d == 0 ? L.P[1] : L.P[0];
+        const int p2 = d == 2 ? L.P[1] : L.P[2];
+        const int slot = t >= p1*p2;
+        const int u = t - slot*p1*p2;
+        const int c1 = u/p2;
+        const int c2 = u - c1*p2;
+        if (d == 0) snap_cell<T, 0>(L, x, slot, c1, c2, sn.p[0][slot]);
+        else if (d == 1) snap_cell<T, 1>(L, x, slot, c1, c2, sn.p[1][slot]);
+        else snap_cell<T, 2>(L, x, slot, c1, c2, sn.p[2][slot]);
+        return;
+    }
+    if (g < 3)
+    {
+        const int n1 = g == 0 ? L.N[1] : L.N[0];
+        const int n2 = g == 2 ? L.N[1] : L.N[2];
+        const int half = n1*n2;
+        const int side = t >= half;
+        const int u = t - side*half;
+        const int c1 = u/n2;
+        const int c2 = u - c1*n2;
+        if (g == 0)
+            face_ghost<T, 0>(L, op, x, mode, level0, bv, side, 1 + c1, 1 + c2);
+        else if (g == 1)
+            face_ghost<T, 1>(L, op, x, mode, level0, bv, side, 1 + c1, 1 + c2);
+        else
+            face_ghost<T, 2>(L, op, x, mode, level0, bv, side, 1 + c1, 1 + c2);
+        return;
+    }
+    // edges along axis 0, 1, 2 (4 each, interior range), then 8 corners
     int I, J, K;
-    if (!shell_cell(L, t, I, J, K)) return;
-    ghost_cell(L, op, x, mode, level0, bv, I, J, K);
+    int u = t;
+    const int e0 = 4*L.N[0], e1 = 4*L.N[1], e2 = 4*L.N[2];
+    if (u < e0)
+    {
+        const int c = u/L.N[0];
+        I = 1 + (u - c*L.N[0]);
+        J = (c & 1) ? L.P[1] - 1 : 0;
+        K = (c & 2) ? L.P[2] - 1 : 0;
+    }
+    else if ((u -= e0) < e1)
+    {
+        const int c = u/L.N[1];
+        J = 1 + (u - c*L.N[1]);
+        I = (c & 1) ? L.P[0] - 1 : 0;
+        K = (c & 2) ? L.P[2] - 1 : 0;
+    }
+    else if ((u -= e1) < e2)
+    {
+        const int c = u/L.N[2];
+        K = 1 + (u - c*L.N[2]);
+        I = (c & 1) ? L.P[0] - 1 : 0;
+        J = (c & 2) ? L.P[1] - 1 : 0;
+    }
+    else
+    {
+        u -= e2;
+        I = (u & 1) ? L.P[0] - 1 : 0;
+        J = (u & 2) ? L.P[1] - 1 : 0;
+        K = (u & 4) ? L.P[2] - 1 : 0;
+    }
+    edge_ghost(L, x, I, J, K);
 }
 
 template <typename T>
@@ -1015,11 +1430,12 @@ __global__ __launch_bounds__(256) void k_coarse_solve(LevelDims L,
         x.c[(I + J + K) & 1][cidx(L, I, J, K)] = T(y[q]);
     }
     __syncthreads();
-    const long ns = shell_count(L);
-    for (long t = tid; t < ns; t += nt)
+    const Shell sh = shell_all(L);
+    const int ns = sh.nA + sh.nB + sh.nC;
+    for (int t = tid; t < ns; t += nt)
     {
         int I, J, K;
-        if (shell_cell(L, t, I, J, K))
+        if (shell_cell(L, sh, t, I, J, K))
             ghost_cell(L, op, x, 1, level0, bv, I, J, K);
     }
 }
@@ -1029,27 +1445,24 @@ __global__ __launch_bounds__(256) void k_coarse_solve(LevelDims L,
 template <typename T, int MODE>
 __global__ __launch_bounds__(BX*BY) void k_reduce_field(LevelDims L,
                                                         Pair<const T> f,
+                                                        Tiles tl,
                                                         double* partials)
 {
     __shared__ double sm[BX*BY/64];
-    const int I = 1 + blockIdx.z;
-    const int J = 1 + blockIdx.y*BY + threadIdx.y;
-    const int p = blockIdx.x*BX + threadIdx.x;
+    int bx, by, bz;
+    if (!tile_of(tl, bx, by, bz)) return;
+    PairPos c;
     double acc = 0.0;
-    if (J <= L.N[1])
+    if (pair_pos(L, bx, by, bz, c))
     {
-        const int idx = (I*L.P[1] + J)*L.Hz + p;
-        for (int h = 0; h < 2; h++)
-        {
-            const int K = 2*p + h;
-            if (K < 1 || K > L.N[2]) continue;
-            const double v = double(f.c[(I + J + K) & 1][idx]);
-            acc += MODE == 0 ? v*v : v;
-        }
+        const double uA = double(ld(col(f, c.cA), c.idx));
+        const double uB = double(ld(col(f, c.cB), c.idx + 1));
+        acc += MODE == 0 ? uA*uA : uA;
+        if (c.vB) acc += MODE == 0 ? uB*uB : uB;
     }
     const double s = block_sum(acc, sm);
     if (threadIdx.x == 0 && threadIdx.y == 0)
-        partials[blockIdx.x + gridDim.x*(blockIdx.y + gridDim.y*blockIdx.z)] = s;
+        partials[tile_index(tl, bx, by, bz)] = s;
 }
 
 // Terms of normFactors (solver.C:35-46): (|r - b + Ay| + |b - Ay|)^2 with
@@ -1058,36 +1471,35 @@ template <typename T, int OP>
 __global__ __launch_bounds__(BX*BY) void k_nf_terms(LevelDims L, OpView<T> op,
                                                     Pair<const T> r,
                                                     Pair<const T> b, double y,
-                                                    double* partials)
+                                                    Tiles tl, double* partials)
 {
     __shared__ double sm[BX*BY/64];
-    const int I = 1 + blockIdx.z;
-    const int J = 1 + blockIdx.y*BY + threadIdx.y;
-    const int p = blockIdx.x*BX + threadIdx.x;
+    int bx, by, bz;
+    if (!tile_of(tl, bx, by, bz)) return;
+    PairPos c;
     double acc = 0.0;
-    if (J <= L.N[1])
+    if (pair_pos(L, bx, by, bz, c))
     {
-        const int row = (I*L.P[1] + J)*L.Hz;
-        const int idx = row + p;
         for (int h = 0; h < 2; h++)
         {
-            const int K = 2*p + h;
-            if (K < 1 || K > L.N[2]) continue;
-            const int colour = (I + J + K) & 1;
+            if (h == 1 && !c.vB) continue;
+            const int colour = h ? c.cB : c.cA;
+            const unsigned idx = c.idx + h;
             T a[7];
-            coeffs<T, OP>(L, op, colour, idx, row, p, h, I, J, K, a);
+            coeffs<T, OP>(L, op, colour, int(idx), c.row, c.p + h, 1 - h, c.I,
+                          c.J, 2*c.p + 1 + h, a);
             double rs = double(a[0]);
-            for (int s = 1; s < 7; s++) rs = rs + double(a[s]);
+            for (int k = 1; k < 7; k++) rs = rs + double(a[k]);
             const double Ay = rs*y;
-            const double rv = double(r.c[colour][idx]);
-            const double bv = double(b.c[colour][idx]);
+            const double rv = double(ld(col(r, colour), idx));
+            const double bv = double(ld(col(b, colour), idx));
             const double t = fabs((rv - bv) + Ay) + fabs(bv - Ay);
             acc += t*t;
         }
     }
     const double s = block_sum(acc, sm);
     if (threadIdx.x == 0 && threadIdx.y == 0)
-        partials[blockIdx.x + gridDim.x*(blockIdx.y + gridDim.y*blockIdx.z)] = s;
+        partials[tile_index(tl, bx, by, bz)] = s;
 }
 
 __global__ __launch_bounds__(1024) void k_reduce_final(const double* partials,
@@ -1113,36 +1525,20 @@ template <int OP>
 __global__ __launch_bounds__(BX*BY) void k_apply(LevelDims L,
                                                  OpView<double> op,
                                                  Pair<const double> p,
-                                                 Pair<double> q)
+                                                 Pair<double> q, Tiles tl)
 {
-    const int I = 1 + blockIdx.z;
-    const int J = 1 + blockIdx.y*BY + threadIdx.y;
-    const int pp = blockIdx.x*BX + threadIdx.x;
-    if (J > L.N[1]) return;
-    const int row = (I*L.P[1] + J)*L.Hz;
-    const int idx = row + pp;
-    for (int h = 0; h < 2; h++)
-    {
-        const int K = 2*pp + h;
-        if (K < 1 || K > L.N[2]) continue;
-        const int colour = (I + J + K) & 1;
-        const double* xc = p.c[colour];
-        const double* xo = p.c[colour ^ 1];
-        double a[7];
-        coeffs<double, OP>(L, op, colour, idx, row, pp, h, I, J, K, a);
-        q.c[colour][idx] =
-            a[0]*xc[idx]
-          + a[1]*xo[idx - L.sI]
-          + a[2]*xo[idx + L.sI]
-          + a[3]*xo[idx - L.Hz]
-          + a[4]*xo[idx + L.Hz]
-          + a[5]*xo[row + pp - 1 + h]
-          + a[6]*xo[row + pp + h];
-    }
+    int bx, by, bz;
+    if (!tile_of(tl, bx, by, bz)) return;
+    PairPos c;
+    if (!pair_pos(L, bx, by, bz, c)) return;
+    double Ax[2];
+    pair_Ax<double, OP>(L, op, p, c, Ax);
+    st(col(q, c.cA), c.idx, Ax[0]);
+    if (c.vB) st(col(q, c.cB), c.idx + 1, Ax[1]);
 }
 
-// Up to three dot products over interior cells in one pass; per-block
-// partials at partials[d*nblocks + block].
+// Up to three dot products over interior cells in one pass; per-tile
+// partials at partials[d*ntiles + tile].
 __global__ __launch_bounds__(BX*BY) void k_dots(LevelDims L, int nd,
                                                 Pair<const double> a0,
                                                 Pair<const double> b0,
@@ -1150,28 +1546,27 @@ __global__ __launch_bounds__(BX*BY) void k_dots(LevelDims L, int nd,
                                                 Pair<const double> b1,
                                                 Pair<const double> a2,
                                                 Pair<const double> b2,
-                                                double* partials)
+                                                Tiles tl, double* partials)
 {
     __shared__ double sm[BX*BY/64];
-    const int I = 1 + blockIdx.z;
-    const int J = 1 + blockIdx.y*BY + threadIdx.y;
-    const int pp = blockIdx.x*BX + threadIdx.x;
+    int bx, by, bz;
+    if (!tile_of(tl, bx, by, bz)) return;
+    PairPos c;
     double acc[3] = {0.0, 0.0, 0.0};
-    if (J <= L.N[1])
+    if (pair_pos(L, bx, by, bz, c))
     {
-        const int idx = (I*L.P[1] + J)*L.Hz + pp;
         for (int h = 0; h < 2; h++)
         {
-            const int K = 2*pp + h;
-            if (K < 1 || K > L.N[2]) continue;
-            const int c = (I + J + K) & 1;
-            acc[0] += a0.c[c][idx]*b0.c[c][idx];
-            if (nd > 1) acc[1] += a1.c[c][idx]*b1.c[c][idx];
-            if (nd > 2) acc[2] += a2.c[c][idx]*b2.c[c][idx];
+            if (h == 1 && !c.vB) continue;
+            const int k = h ? c.cB : c.cA;
+            const unsigned i = c.idx + h;
+            acc[0] += ld(col(a0, k), i)*ld(col(b0, k), i);
+            if (nd > 1) acc[1] += ld(col(a1, k), i)*ld(col(b1, k), i);
+            if (nd > 2) acc[2] += ld(col(a2, k), i)*ld(col(b2, k), i);
         }
     }
-    const int nb = gridDim.x*gridDim.y*gridDim.z;
-    const int blk = blockIdx.x + gridDim.x*(blockIdx.y + gridDim.y*blockIdx.z);
+    const int nb = tl.gx*tl.gy*tl.gz;
+    const int blk = tile_index(tl, bx, by, bz);
     for (int d = 0; d < nd; d++)
     {
         const double s = block_sum(acc[d], sm);
@@ -1180,36 +1575,47 @@ __global__ __launch_bounds__(BX*BY) void k_dots(LevelDims L, int nd,
     }
 }
 
-// x += alpha p; r -= alpha q (interior) and per-block partial sums of r^2.
-__global__ __launch_bounds__(BX*BY) void k_cg_update(LevelDims L, double alpha,
+// x += alpha p; r -= alpha q (interior) and per-tile partial sums of r^2.
+__global__ __launch_bounds__(BX*BY) void k_cg_update(LevelDims L,
+                                                     double alpha,
                                                      Pair<double> x,
                                                      Pair<double> r,
                                                      Pair<const double> p,
                                                      Pair<const double> q,
+                                                     Tiles tl,
                                                      double* partials)
 {
     __shared__ double sm[BX*BY/64];
-    const int I = 1 + blockIdx.z;
-    const int J = 1 + blockIdx.y*BY + threadIdx.y;
-    const int pp = blockIdx.x*BX + threadIdx.x;
+    int bx, by, bz;
+    if (!tile_of(tl, bx, by, bz)) return;
+    PairPos c;
     double acc = 0.0;
-    if (J <= L.N[1])
+    if (pair_pos(L, bx, by, bz, c))
     {
-        const int idx = (I*L.P[1] + J)*L.Hz + pp;
+        double xv[2], rv[2], pv[2], qv[2];
         for (int h = 0; h < 2; h++)
         {
-            const int K = 2*pp + h;
-            if (K < 1 || K > L.N[2]) continue;
-            const int c = (I + J + K) & 1;
-            x.c[c][idx] = x.c[c][idx] + alpha*p.c[c][idx];
-            const double rn = r.c[c][idx] - alpha*q.c[c][idx];
-            r.c[c][idx] = rn;
+            const int k = h ? c.cB : c.cA;
+            const unsigned i = c.idx + h;
+            xv[h] = ld(col(x, k), i);
+            rv[h] = ld(col(r, k), i);
+            pv[h] = ld(col(p, k), i);
+            qv[h] = ld(col(q, k), i);
+        }
+        for (int h = 0; h < 2; h++)
+        {
+            if (h == 1 && !c.vB) continue;
+            const int k = h ? c.cB : c.cA;
+            const unsigned i = c.idx + h;
+            st(col(x, k), i, xv[h] + alpha*pv[h]);
+            const double rn = rv[h] - alpha*qv[h];
+            st(col(r, k), i, rn);
             acc += rn*rn;
         }
     }
     const double s = block_sum(acc, sm);
     if (threadIdx.x == 0 && threadIdx.y == 0)
-        partials[blockIdx.x + gridDim.x*(blockIdx.y + gridDim.y*blockIdx.z)] = s;
+        partials[tile_index(tl, bx, by, bz)] = s;
 }
 
 // p = z + beta p over the whole padded arrays (ghosts refreshed later).
@@ -1258,20 +1664,18 @@ __global__ __launch_bounds__(BX*BY) void k_diag(LevelDims L, Pair<const double> 
 
 __global__ __launch_bounds__(BX*BY) void k_add_correction(LevelDims L,
                                                           Pair<double> x,
-                                                          Pair<const float> e)
+                                                          Pair<const float> e,
+                                                          Tiles tl)
 {
-    const int I = 1 + blockIdx.z;
-    const int J = 1 + blockIdx.y*BY + threadIdx.y;
-    const int p = blockIdx.x*BX + threadIdx.x;
-    if (J > L.N[1]) return;
-    const int idx = (I*L.P[1] + J)*L.Hz + p;
-    for (int h = 0; h < 2; h++)
-    {
-        const int K = 2*p + h;
-        if (K < 1 || K > L.N[2]) continue;
-        const int c = (I + J + K) & 1;
-        x.c[c][idx] = x.c[c][idx] + double(e.c[c][idx]);
-    }
+    int bx, by, bz;
+    if (!tile_of(tl, bx, by, bz)) return;
+    PairPos c;
+    if (!pair_pos(L, bx, by, bz, c)) return;
+    const double xA = ld(col(x, c.cA), c.idx), xB = ld(col(x, c.cB), c.idx + 1);
+    const double eA = double(ld(col(e, c.cA), c.idx));
+    const double eB = double(ld(col(e, c.cB), c.idx + 1));
+    st(col(x, c.cA), c.idx, xA + eA);
+    if (c.vB) st(col(x, c.cB), c.idx + 1, xB + eB);
 }
 
 __global__ void k_to_float(long n, Pair<const double> in, Pair<float> out)
@@ -1311,6 +1715,27 @@ inline dim3 grid_padded(const LevelDims& L)
 
 const dim3 BLK(BX, BY, 1);
 
+Tiles make_tiles(int gx, int gy, int gz)
+{
+    Tiles t;
+    t.gx = gx;
+    t.gy = gy;
+    t.gz = gz;
+    t.per = (gx*gy*gz + 7)/8;
+    return t;
+}
+
+inline dim3 tiles_grid(const Tiles& t) { return dim3(8*t.per); }
+
+inline int tiles_count(const Tiles& t) { return t.gx*t.gy*t.gz; }
+
+// tiles of the cell-pair kernels over the interior of level L
+Tiles pair_tiles(const LevelDims& L)
+{
+    return make_tiles(((L.N[2] + 1)/2 + BX - 1)/BX, (L.N[1] + BY - 1)/BY,
+                      L.N[0]);
+}
+
 BcVals bcvals(const double* v)
 {
     BcVals b;
@@ -1322,14 +1747,18 @@ BcVals bcvals(const double* v)
 
 // ---------------------------------------------------------------------------
 // launchers
+int pair_tile_count(const LevelDims& L) { return tiles_count(pair_tiles(L)); }
+
 template <typename T>
 void launch_rbgs(const LevelDims& L, const OpView<T>& op, Pair<T> x,
                  Pair<const T> b, int colour, const Region& rg, hipStream_t s)
 {
     if (rg.E[0] <= rg.S[0] || rg.E[1] <= rg.S[1] || rg.E[2] <= rg.S[2]) return;
     const int nK = rg.E[2] - rg.S[2];
-    const dim3 g(((nK + 1)/2 + 1 + BX - 1)/BX, (rg.E[1] - rg.S[1] + BY - 1)/BY,
-                 rg.E[0] - rg.S[0]);
+    const Tiles tl = make_tiles(((nK + 1)/2 + 1 + BX - 1)/BX,
+                                (rg.E[1] - rg.S[1] + BY - 1)/BY,
+                                rg.E[0] - rg.S[0]);
+    const dim3 g = tiles_grid(tl);
     T* xc = x.c[colour];
     const T* xo = x.c[colour ^ 1];
     const T* bc = b.c[colour];
@@ -1337,15 +1766,15 @@ void launch_rbgs(const LevelDims& L, const OpView<T>& op, Pair<T> x,
     {
     case OP_STENCIL:
         hipLaunchKernelGGL((k_rbgs<T, OP_STENCIL>), g, BLK, 0, s, L, op, xc,
-                           xo, bc, colour, rg);
+                           xo, bc, colour, rg, tl);
         break;
     case OP_CONST:
         hipLaunchKernelGGL((k_rbgs<T, OP_CONST>), g, BLK, 0, s, L, op, xc, xo,
-                           bc, colour, rg);
+                           bc, colour, rg, tl);
         break;
     default:
         hipLaunchKernelGGL((k_rbgs<T, OP_FACES>), g, BLK, 0, s, L, op, xc, xo,
-                           bc, colour, rg);
+                           bc, colour, rg, tl);
         break;
     }
 }
@@ -1353,38 +1782,50 @@ void launch_rbgs(const LevelDims& L, const OpView<T>& op, Pair<T> x,
 template <typename T>
 void launch_rbgs_slabs(const LevelDims& L, const OpView<T>& op, Pair<T> x,
                        Pair<const T> b, int d, const int* planes, int nslab,
-                       const int* colours, int nphase, hipStream_t s)
+                       const int* colours, int nphase, const SlabIO<T>& io,
+                       hipStream_t s)
 {
-    SlabArgs sa;
-    sa.d = d;
+    SlabArgs<T> sa;
     sa.plane[0] = planes[0];
     sa.plane[1] = nslab > 1 ? planes[1] : planes[0];
     sa.nphase = nphase;
     for (int z = 0; z < 2; z++)
         for (int i = 0; i < 4; i++)
             sa.colours[z][i] = i < nphase ? colours[(nslab > 1 ? z : 0)*4 + i] : 0;
+    sa.src[0] = io.src[0];
+    sa.src[1] = nslab > 1 ? io.src[1] : io.src[0];
+    sa.ntarget = io.ntarget;
+    for (int t = 0; t < 6; t++)
+    {
+        sa.tptr[t] = t < io.ntarget ? io.tptr[t] : nullptr;
+        sa.tdim[t] = t < io.ntarget ? io.tdim[t] : 0;
+        sa.tplane[t] = t < io.ntarget ? io.tplane[t] : -1;
+    }
     const int a1 = d == 0 ? 1 : 0;
     const int a2 = d == 2 ? 1 : 2;
     const int n1 = L.N[a1], n2 = L.N[a2];
-    sa.tt = (n1*n2 >= 128*128) ? 16 : 8;
-    const int W = sa.tt + 2*nphase;
-    const dim3 g((n2 + sa.tt - 1)/sa.tt, (n1 + sa.tt - 1)/sa.tt, nslab);
-    const size_t lds = size_t(W)*W*sizeof(T);
-    switch (op.kind)
+    dim3 g(1, 1, nslab);
+    if (n1 + 2 <= SLAB_W && n2 + 2 <= SLAB_W)
+        sa.tt = 0;
+    else
     {
-    case OP_STENCIL:
-        hipLaunchKernelGGL((k_rbgs_slabs<T, OP_STENCIL>), g, dim3(256), lds, s,
-                           L, op, x, b, sa);
-        break;
-    case OP_CONST:
-        hipLaunchKernelGGL((k_rbgs_slabs<T, OP_CONST>), g, dim3(256), lds, s,
-                           L, op, x, b, sa);
-        break;
-    default:
-        hipLaunchKernelGGL((k_rbgs_slabs<T, OP_FACES>), g, dim3(256), lds, s,
-                           L, op, x, b, sa);
-        break;
+        sa.tt = 16;
+        g = dim3((n2 + sa.tt - 1)/sa.tt, (n1 + sa.tt - 1)/sa.tt, nslab);
     }
+#define BMG_SLAB(OPK, DD)                                                     \
+    hipLaunchKernelGGL((k_rbgs_slabs<T, OPK, DD>), g, dim3(256), 0, s, L, op, \
+                       x, b, sa)
+#define BMG_SLAB_D(OPK)                                                       \
+    do {                                                                      \
+        if (d == 0) BMG_SLAB(OPK, 0);                                         \
+        else if (d == 1) BMG_SLAB(OPK, 1);                                    \
+        else BMG_SLAB(OPK, 2);                                                \
+    } while (0)
+    if (op.kind == OP_STENCIL) BMG_SLAB_D(OP_STENCIL);
+    else if (op.kind == OP_CONST) BMG_SLAB_D(OP_CONST);
+    else BMG_SLAB_D(OP_FACES);
+#undef BMG_SLAB_D
+#undef BMG_SLAB
 }
 
 template <typename T>
@@ -1415,15 +1856,15 @@ void launch_jacobi(const LevelDims& L, const OpView<T>& op, Pair<T> x,
 
 template <typename T>
 void launch_residual(const LevelDims& L, const OpView<T>& op, Pair<const T> x,
-                     Pair<const T> b, Pair<T> r, const Region& rg,
-                     double* partials, int* nblocks, hipStream_t s)
+                     Pair<const T> b, Pair<T> r, double* partials,
+                     int* nblocks, hipStream_t s)
 {
-    const dim3 g = grid3(rg);
-    if (nblocks) *nblocks = int(g.x*g.y*g.z);
-    if (rg.E[0] <= rg.S[0] || rg.E[1] <= rg.S[1] || rg.E[2] <= rg.S[2]) return;
+    const Tiles tl = pair_tiles(L);
+    const dim3 g = tiles_grid(tl);
+    if (nblocks) *nblocks = tiles_count(tl);
 #define BMG_RES(OPK, NRM)                                                     \
-    hipLaunchKernelGGL((k_residual<T, OPK, NRM>), g, BLK, 0, s, L, op, x, b, \
-                       r, rg, partials)
+    hipLaunchKernelGGL((k_residual<T, OPK, NRM>), g, BLK, 0, s, L, op,   \
+                       x, b, r, tl, partials)
     if (partials)
     {
         if (op.kind == OP_STENCIL) BMG_RES(OP_STENCIL, true);
@@ -1444,8 +1885,9 @@ void launch_restrict(const LevelDims& Lf, const LevelDims& Lc,
                      const TransferView<T>& tv, Pair<const T> rf, Pair<T> bc,
                      hipStream_t s)
 {
-    hipLaunchKernelGGL((k_restrict<T>), grid_interior(Lc), BLK, 0, s, Lf, Lc,
-                       tv, rf, bc);
+    const Tiles tl = pair_tiles(Lc);
+    hipLaunchKernelGGL((k_restrict<T>), tiles_grid(tl), BLK, 0, s, Lf,
+                       Lc, tv, rf, bc, tl);
 }
 
 template <typename T>
@@ -1453,17 +1895,56 @@ void launch_prolong(const LevelDims& Lf, const LevelDims& Lc,
                     const TransferView<T>& tv, Pair<const T> xc, Pair<T> xf,
                     hipStream_t s)
 {
-    hipLaunchKernelGGL((k_prolong<T>), grid_interior(Lf), BLK, 0, s, Lf, Lc,
-                       tv, xc, xf);
+    const Tiles tl = pair_tiles(Lf);
+    hipLaunchKernelGGL((k_prolong<T>), tiles_grid(tl), BLK, 0, s, Lf, Lc,
+                       tv, xc, xf, tl);
+}
+
+template <typename T>
+void launch_bc_fill_snap(const LevelDims& L, const OpView<T>& op, Pair<T> x,
+                         int mode, int level0, const double* bcval,
+                         T* const (*snap)[2], hipStream_t s)
+{
+    // groups holding ghosts this mode writes: mode 0 the periodic faces,
+    // mode 2 the other faces, mode 1 every ghost; then the snapshots
+    auto wanted = [&](int f) {
+        return mode == 1 || (mode == 0) == (L.bc[f] == PER);
+    };
+    BcGroups gr;
+    SnapPtrs<T> sn;
+    const int faceCells[3] = {2*L.N[1]*L.N[2], 2*L.N[0]*L.N[2],
+                              2*L.N[0]*L.N[1]};
+    const int planeCells[3] = {2*L.P[1]*L.P[2], 2*L.P[0]*L.P[2],
+                               2*L.P[0]*L.P[1]};
+    int total = 0;
+    for (int g = 0; g < 7; g++)
+    {
+        int n = 0;
+        if (g < 3) n = (wanted(2*g) || wanted(2*g + 1)) ? faceCells[g] : 0;
+        else if (g == 3) n = mode == 1 ? 4*(L.N[0] + L.N[1] + L.N[2]) + 8 : 0;
+        else
+        {
+            const int d = g - 4;
+            const bool on = snap && snap[d][0] && snap[d][1];
+            sn.p[d][0] = on ? snap[d][0] : nullptr;
+            sn.p[d][1] = on ? snap[d][1] : nullptr;
+            n = on ? planeCells[d] : 0;
+        }
+        gr.start[g] = total;
+        gr.count[g] = n;
+        total += (n + 255)/256*256;
+    }
+    gr.start[7] = total;
+    if (total == 0) return;
+    hipLaunchKernelGGL((k_bc_fill<T>), dim3(total/256), dim3(256), 0, s, L,
+                       op, x, mode, level0, bcvals(bcval), gr, sn);
 }
 
 template <typename T>
 void launch_bc_fill(const LevelDims& L, const OpView<T>& op, Pair<T> x,
                     int mode, int level0, const double* bcval, hipStream_t s)
 {
-    const long n = shell_count(L);
-    hipLaunchKernelGGL((k_bc_fill<T>), dim3((n + 255)/256), dim3(256), 0, s, L,
-                       op, x, mode, level0, bcvals(bcval));
+    launch_bc_fill_snap<T>(L, op, x, mode, level0, bcval, nullptr, s);
 }
 
 template <typename T>
@@ -1580,18 +2061,20 @@ template <typename T>
 void launch_sum_sq(const LevelDims& L, Pair<const T> f, double* partials,
                    int* nblocks, hipStream_t s)
 {
-    const dim3 g = grid_interior(L);
-    *nblocks = int(g.x*g.y*g.z);
-    hipLaunchKernelGGL((k_reduce_field<T, 0>), g, BLK, 0, s, L, f, partials);
+    const Tiles tl = pair_tiles(L);
+    *nblocks = tiles_count(tl);
+    hipLaunchKernelGGL((k_reduce_field<T, 0>), tiles_grid(tl), BLK, 0, s,
+                       L, f, tl, partials);
 }
 
 template <typename T>
 void launch_sum(const LevelDims& L, Pair<const T> f, double* partials,
                 int* nblocks, hipStream_t s)
 {
-    const dim3 g = grid_interior(L);
-    *nblocks = int(g.x*g.y*g.z);
-    hipLaunchKernelGGL((k_reduce_field<T, 1>), g, BLK, 0, s, L, f, partials);
+    const Tiles tl = pair_tiles(L);
+    *nblocks = tiles_count(tl);
+    hipLaunchKernelGGL((k_reduce_field<T, 1>), tiles_grid(tl), BLK, 0, s,
+                       L, f, tl, partials);
 }
 
 template <typename T>
@@ -1599,17 +2082,18 @@ void launch_norm_factor_terms(const LevelDims& L, const OpView<T>& op,
                               Pair<const T> r, Pair<const T> b, double y,
                               double* partials, int* nblocks, hipStream_t s)
 {
-    const dim3 g = grid_interior(L);
-    *nblocks = int(g.x*g.y*g.z);
+    const Tiles tl = pair_tiles(L);
+    const dim3 g = tiles_grid(tl);
+    *nblocks = tiles_count(tl);
     if (op.kind == OP_STENCIL)
-        hipLaunchKernelGGL((k_nf_terms<T, OP_STENCIL>), g, BLK, 0, s, L, op, r,
-                           b, y, partials);
+        hipLaunchKernelGGL((k_nf_terms<T, OP_STENCIL>), g, BLK, 0, s, L,
+                           op, r, b, y, tl, partials);
     else if (op.kind == OP_CONST)
-        hipLaunchKernelGGL((k_nf_terms<T, OP_CONST>), g, BLK, 0, s, L, op, r, b,
-                           y, partials);
+        hipLaunchKernelGGL((k_nf_terms<T, OP_CONST>), g, BLK, 0, s, L, op,
+                           r, b, y, tl, partials);
     else
-        hipLaunchKernelGGL((k_nf_terms<T, OP_FACES>), g, BLK, 0, s, L, op, r, b,
-                           y, partials);
+        hipLaunchKernelGGL((k_nf_terms<T, OP_FACES>), g, BLK, 0, s, L, op,
+                           r, b, y, tl, partials);
 }
 
 void launch_reduce_final(const double* partials, int n, double* out,
@@ -1628,36 +2112,38 @@ void launch_diag(const LevelDims& L, Pair<const double> a7, Pair<double> x,
 void launch_apply(const LevelDims& L, const OpView<double>& op,
                   Pair<const double> p, Pair<double> q, hipStream_t s)
 {
+    const Tiles tl = pair_tiles(L);
+    const dim3 g = tiles_grid(tl);
     if (op.kind == OP_STENCIL)
-        hipLaunchKernelGGL((k_apply<OP_STENCIL>), grid_interior(L), BLK, 0, s,
-                           L, op, p, q);
+        hipLaunchKernelGGL((k_apply<OP_STENCIL>), g, BLK, 0, s, L, op, p,
+                           q, tl);
     else if (op.kind == OP_CONST)
-        hipLaunchKernelGGL((k_apply<OP_CONST>), grid_interior(L), BLK, 0, s, L,
-                           op, p, q);
+        hipLaunchKernelGGL((k_apply<OP_CONST>), g, BLK, 0, s, L, op, p, q,
+                           tl);
     else
-        hipLaunchKernelGGL((k_apply<OP_FACES>), grid_interior(L), BLK, 0, s, L,
-                           op, p, q);
+        hipLaunchKernelGGL((k_apply<OP_FACES>), g, BLK, 0, s, L, op, p, q,
+                           tl);
 }
 
 int launch_dots(const LevelDims& L, int nd, const Pair<const double>* ab,
                 double* partials, hipStream_t s)
 {
-    const dim3 g = grid_interior(L);
+    const Tiles tl = pair_tiles(L);
     Pair<const double> z{{nullptr, nullptr}};
-    hipLaunchKernelGGL(k_dots, g, BLK, 0, s, L, nd, ab[0], ab[1],
-                       nd > 1 ? ab[2] : z, nd > 1 ? ab[3] : z,
-                       nd > 2 ? ab[4] : z, nd > 2 ? ab[5] : z, partials);
-    return int(g.x*g.y*g.z);
+    hipLaunchKernelGGL(k_dots, tiles_grid(tl), BLK, 0, s, L, nd, ab[0],
+                       ab[1], nd > 1 ? ab[2] : z, nd > 1 ? ab[3] : z,
+                       nd > 2 ? ab[4] : z, nd > 2 ? ab[5] : z, tl, partials);
+    return tiles_count(tl);
 }
 
 int launch_cg_update(const LevelDims& L, double alpha, Pair<double> x,
                      Pair<double> r, Pair<const double> p,
                      Pair<const double> q, double* partials, hipStream_t s)
 {
-    const dim3 g = grid_interior(L);
-    hipLaunchKernelGGL(k_cg_update, g, BLK, 0, s, L, alpha, x, r, p, q,
-                       partials);
-    return int(g.x*g.y*g.z);
+    const Tiles tl = pair_tiles(L);
+    hipLaunchKernelGGL(k_cg_update, tiles_grid(tl), BLK, 0, s, L, alpha,
+                       x, r, p, q, tl, partials);
+    return tiles_count(tl);
 }
 
 void launch_xpby(long n, Pair<const double> z, double beta, Pair<double> p,
@@ -1686,7 +2172,9 @@ void launch_to_double(long n, Pair<const float> a, Pair<double> b,
 void launch_add_correction(const LevelDims& L, Pair<double> x,
                            Pair<const float> e, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_add_correction, grid_interior(L), BLK, 0, s, L, x, e);
+    const Tiles tl = pair_tiles(L);
+    hipLaunchKernelGGL(k_add_correction, tiles_grid(tl), BLK, 0, s, L, x,
+                       e, tl);
 }
 
 void launch_to_float(long ncol, Pair<const double> in, Pair<float> out,
@@ -1710,14 +2198,13 @@ void launch_set_cell(Pair<T> f, long idx, int colour, T v, hipStream_t s)
     template void launch_rbgs_slabs<T>(const LevelDims&, const OpView<T>&,    \
                                        Pair<T>, Pair<const T>, int,           \
                                        const int*, int, const int*, int,      \
-                                       hipStream_t);                          \
+                                       const SlabIO<T>&, hipStream_t);                          \
     template void launch_jacobi<T>(const LevelDims&, const OpView<T>&,        \
                                    Pair<T>, Pair<const T>, Pair<T>,           \
                                    const Region&, hipStream_t);               \
     template void launch_residual<T>(const LevelDims&, const OpView<T>&,      \
                                      Pair<const T>, Pair<const T>, Pair<T>,   \
-                                     const Region&, double*, int*,            \
-                                     hipStream_t);                            \
+                                     double*, int*, hipStream_t);             \
     template void launch_restrict<T>(const LevelDims&, const LevelDims&,      \
                                      const TransferView<T>&, Pair<const T>,   \
                                      Pair<T>, hipStream_t);                   \
@@ -1726,7 +2213,10 @@ void launch_set_cell(Pair<T> f, long idx, int colour, T v, hipStream_t s)
                                     Pair<T>, hipStream_t);                    \
     template void launch_bc_fill<T>(const LevelDims&, const OpView<T>&,       \
                                     Pair<T>, int, int, const double*,         \
-                                    hipStream_t);                             \
+                                    hipStream_t);                                                                              \
+    template void launch_bc_fill_snap<T>(const LevelDims&, const OpView<T>&,  \
+                                         Pair<T>, int, int, const double*,    \
+                                         T* const (*)[2], hipStream_t);                             \
     template void launch_fill<T>(Pair<T>, long, T, hipStream_t);              \
     template void launch_ref_to_split<T>(const LevelDims&, const double*,     \
                                          Pair<T>, hipStream_t);               \

@@ -31,6 +31,16 @@ def mixed_problem(n=16, seed=0):
                       W.NEUMANN, W.NEUMANN), b)
 
 
+def thin_periodic(n, bc, seed=0):
+    """Uniform mesh with thin periodic axes: slab planes that coincide
+    (N_d = 1) or touch (N_d = 2) are swept one launch per plane."""
+    verts = [np.arange(q + 1) / q for q in n]
+    rng = np.random.default_rng(seed)
+    b = np.zeros(tuple(q + 2 for q in n))
+    b[1:-1, 1:-1, 1:-1] = rng.uniform(-1, 1, n) / np.prod(n)
+    return W.Problem("thin", verts, bc, b)
+
+
 CASES = {
     "neumann16": lambda: W.uniform_neumann(16, seed=1),
     "channel16": lambda: W.channel(16, seed=1),
@@ -41,6 +51,12 @@ CASES = {
     "neumann48": lambda: W.uniform_neumann(48, seed=5),
     "channel48": lambda: W.channel(48, seed=5),
     "spheres48": lambda: W.spheres(48, seed=5, nspheres=8),
+    "thin2": lambda: thin_periodic(
+        (2, 16, 32), (W.PERIODIC, W.PERIODIC, W.NEUMANN, W.NEUMANN,
+                      W.PERIODIC, W.PERIODIC), seed=6),
+    "thin1": lambda: thin_periodic(
+        (8, 12, 1), (W.PERIODIC, W.PERIODIC, W.DIRICHLET, W.DIRICHLET,
+                     W.PERIODIC, W.PERIODIC), seed=7),
 }
 
 
