@@ -1,21 +1,28 @@
 #!/usr/bin/env python
-"""Benchmark: 256^3 channel-flow pressure Poisson solve to 1e-8 rel-res.
+"""Benchmark: 256^3 pressure Poisson solve to 1e-8 relative residual.
 
-Workload: periodic in x and z, Neumann in y, tanh (beta = 2) wall-normal
-stretching, Fourier-mode right-hand side (see briscola_amd/workloads.py),
-Briscola MG settings V(2,2) red-black Gauss-Seidel, linear transfers, 4^3
-coarse level solved directly, relTol 1e-8.  One step = one complete solve
-from b resident in HBM (x0 = 0) to x with ||r|| / ||r0|| <= 1e-8, run both in
-FP64 (the reference arithmetic, bitwise equal to the CPU restatement) and in
-mixed precision (FP32 cycles inside FP64 defect correction).  ``value`` is the
-faster of the two; both are reported.
+Headline workload (the north-star case): the variable-coefficient pressure
+equation div(rho^-1 grad p) = f on a 256^3 Neumann unit box with density
+ratio 1000 (rho = 1000 outside, 1 inside 32 seeded spheres, interface smoothed
+over 1.5 cells; f = divergence of a seeded random face velocity).  Secondary
+workload: the 256^3 channel (periodic x,z, Neumann y, tanh beta=2 stretching,
+Fourier-mode RHS).  Both use Briscola MG settings V(2,2) red-black
+Gauss-Seidel, linear transfers, coarse level 1 (4^3) solved directly,
+relTol 1e-8.  See briscola_amd/workloads.py for the generators.
 
-The roofline object covers the dominant kernel of the FP64 solve, the
-level-0 red-black smoother phase, timed live with HIP events on the
-library's stream over the timed steps.  The CPU baseline times the CPU
+One step = one complete solve from b resident in HBM (x0 = 0) to x with
+||r|| / ||r0|| <= 1e-8.  Methods: plain FP64 MG (the reference algorithm,
+bitwise equal to the CPU restatement), mixed-precision MG (FP32 cycles inside
+FP64 defect correction), and flexible CG preconditioned by one FP64 or mixed
+MG cycle.  ``value`` is the fastest method on the headline workload; every
+method is reported with iterations, final residual and its error against an
+FP64 reference solution.
+
+The roofline object covers the dominant kernel of the plain FP64 solve, the
+level-0 red-black smoother phase, timed live with HIP events on the library's
+stream right after the timed steps.  The CPU baseline times the CPU
 restatement of the reference multigrid (one core) on a bounded sample of the
-same solve and scales it by the iteration count; the same leg checks each GPU
-solution against an FP64 direct solve (FFT in x,z + Thomas in y).
+same solve and scales it by the iteration count.
 """
 import argparse
 import json
@@ -30,27 +37,55 @@ sys.path.insert(0, ROOT)
 METRIC = ("256³ Poisson time to 1e-8 rel-res: learned/mixed vs plain FP64 MG; "
           "HBM GB/s")
 HBM_PEAK_GBS = 8000.0
+MAX_ITER = 999
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-MAX_ITER = 999
-
-
-def solver_opts(precision, krylov=0):
+def solver_opts(precision, krylov=0, rel_tol=1e-8, max_iter=None):
     from briscola_amd import bmg
-    return dict(rel_tol=1e-8, tolerance=0.0, max_iter=MAX_ITER, n_sweeps_pre=2,
-                n_sweeps_post=2, cycle=bmg.VCYCLE, coarse_mode=bmg.COARSE_DIRECT,
-                coarse_level=1, smoother=bmg.RBGS, precision=precision,
-                krylov=krylov)
+    return dict(rel_tol=rel_tol, tolerance=0.0,
+                max_iter=MAX_ITER if max_iter is None else max_iter,
+                n_sweeps_pre=2, n_sweeps_post=2, cycle=bmg.VCYCLE,
+                coarse_mode=bmg.COARSE_DIRECT, coarse_level=1,
+                smoother=bmg.RBGS, precision=precision, krylov=krylov)
 
 
-def traffic_from_profiles():
+WORKLOADS = {
+    "spheres": dict(
+        title="spheres 256^3",
+        desc=("variable-coefficient VOF pressure div(rho^-1 grad p) = f, {n}^3 "
+              "Neumann unit box, rho 1000 outside / 1 inside 32 seeded spheres "
+              "(radius 0.03-0.1, interface over 1.5 cells), f = div of a random "
+              "face velocity; MG V(2,2) rbgs, linear transfers, 4^3 direct "
+              "coarse, relTol 1e-8"),
+        kernel="k_rbgs<double, 2> (level-0 red-black phase, face-coefficient "
+               "operator, 72 B per updated cell)",
+        traffic="rbgs_phase_spheres256_traffic.txt"),
+    "channel": dict(
+        title="channel 256^3",
+        desc=("channel {n}^3: periodic x,z, Neumann y, tanh beta=2, Fourier "
+              "RHS; MG V(2,2) rbgs, linear transfers, 4^3 direct coarse, "
+              "relTol 1e-8"),
+        kernel="k_rbgs<double, 1> (level-0 red-black phase, interior region, "
+               "matrix-free operator, 24 B per updated cell)",
+        traffic="rbgs_phase_channel256_traffic.txt"),
+}
+
+
+def make_problem(name, n):
+    from briscola_amd import workloads
+    if name == "spheres":
+        return workloads.spheres(n, seed=0)
+    return workloads.channel(n, seed=0)
+
+
+def traffic_from_profiles(fname):
     """HBM bytes per launch of the smoother phase measured with rocprofv3
-    counters (written by hand into profiles/ after the PMC runs)."""
-    path = os.path.join(ROOT, "profiles", "rbgs_phase_channel256_traffic.txt")
+    counters (the summary committed under profiles/)."""
+    path = os.path.join(ROOT, "profiles", fname)
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -59,14 +94,14 @@ def traffic_from_profiles():
     return None
 
 
-def run_gpu(prob, precision, krylov, steps, warmup, profile, reps=50):
+def run_gpu(prob, precision, krylov, steps, warmup, profile, reps=50,
+            rel_tol=1e-8, max_iter=None):
     import torch
     from briscola_amd import bmg
     ctx = bmg.Context(prob.vertices, prob.bc, prob.bc_value,
-                      **solver_opts(precision, krylov))
+                      **solver_opts(precision, krylov, rel_tol, max_iter))
     ctx.set_laplacian(prob.lam)
-    b_host = torch.from_numpy(prob.b)
-    b_dev = b_host.to("cuda:0")
+    b_dev = torch.from_numpy(prob.b).to("cuda:0")
     x_dev = torch.zeros_like(b_dev)
     stats = None
     for _ in range(warmup):
@@ -88,26 +123,38 @@ def run_gpu(prob, precision, krylov, steps, warmup, profile, reps=50):
     # device, which was copied out above)
     prof = ctx.time_smoother(0, precision, reps) if profile else None
     ctx.close()
-    return dict(ms=1e3 * wall / steps, device_ms=dev_ms / steps, iters=stats.iters,
-                final_residual=stats.final_residual, x=x, prof=prof)
+    return dict(ms=1e3 * wall / steps, device_ms=dev_ms / steps,
+                iters=stats.iters, final_residual=stats.final_residual, x=x,
+                prof=prof)
 
 
-def cpu_baseline_and_check(prob, results, iters_fp64, sample_iters):
-    """CPU restatement of the reference MG on a bounded sample, and the FP64
-    direct-solve error of each GPU solution."""
-    import numpy as np
-    import oracle
+def reference_solution(name, prob):
+    """FP64 reference solution and how it was obtained: the channel by the
+    direct FFT (x,z) + Thomas (y) solve on the host; the variable-coefficient
+    case (no direct solver at 256^3) by FP64 flexible CG with the FP64 MG
+    cycle to 1e-13 on the device."""
     from oracle import direct
-    out = {}
-    xe = direct.solve_fft_thomas(prob.b, prob.vertices, bc_y=(0, 0))
-    for name, r in results.items():
-        r["err_vs_fp64_oracle"] = direct.rel_error(r["x"][1:-1, 1:-1, 1:-1], xe, True)
-    o = oracle.Oracle(prob.vertices, prob.bc)
+    if name == "channel":
+        return (direct.solve_fft_thomas(prob.b, prob.vertices, bc_y=(0, 0)),
+                "host FFT(x,z) + Thomas(y) direct solve, FP64")
+    from briscola_amd import bmg
+    r = run_gpu(prob, bmg.FP64, bmg.KRYLOV_FCG, 1, 0, False, rel_tol=1e-13,
+                max_iter=3000)
+    return (r["x"][1:-1, 1:-1, 1:-1],
+            f"FP64 flexible CG + FP64 MG cycle to {r['final_residual']:.1e} "
+            f"relative residual ({r['iters']} iterations) on the device")
+
+
+def cpu_baseline(prob, iters_fp64, sample_iters, n):
+    """The CPU restatement of the reference MG (one core) on a bounded
+    sample of the same FP64 solve."""
+    import oracle
+    o = oracle.Oracle(prob.vertices, prob.bc, prob.bc_value)
     opts = solver_opts(0)
     opts.pop("precision")
     opts.pop("krylov")
     o.set_opts(**opts)
-    o.assemble()
+    o.assemble(prob.lam)
     o.set_field(oracle.B, 0, prob.b)
     o.pin_and_eliminate()
     o.prepare_coarse()
@@ -118,18 +165,85 @@ def cpu_baseline_and_check(prob, results, iters_fp64, sample_iters):
         o.residual(0, True)
         o.sum_sq_residual()
     per_iter = (time.perf_counter() - t0) / sample_iters
-    out["cpu"] = {
+    del o
+    return {
         "value": round(per_iter * iters_fp64 * 1e3, 1),
         "unit": "ms",
         "cores": 1,
         "kind": "port",
-        "sample": (f"{sample_iters} V(2,2) iterations (cycle + residual + norm) of "
-                   f"the same 256^3 FP64 solve on one host core "
+        "sample": (f"{sample_iters} V(2,2) iterations (cycle + residual + norm) "
+                   f"of the same {n}^3 FP64 solve on one host core "
                    f"({per_iter * 1e3:.0f} ms each), scaled by the "
                    f"{iters_fp64} iterations the solve takes; "
                    f"host {platform.processor() or platform.machine()}"),
     }
-    return out
+
+
+def roofline(result, kernel, traffic):
+    if not result or not result.get("prof"):
+        return None
+    avg_ms, per_launch = result["prof"]
+    achieved = per_launch / (avg_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic, "kernel": kernel,
+            "avg_launch_us": round(avg_ms * 1e3, 2),
+            "bytes_per_launch": per_launch}
+
+
+def run_workload(name, args):
+    import numpy as np
+    from briscola_amd import bmg
+    from oracle import direct
+    w = WORKLOADS[name]
+    log(f"[bench] {name}: generating {args.n}^3")
+    prob = make_problem(name, args.n)
+    plan = [("fp64", bmg.FP64, bmg.KRYLOV_NONE),
+            ("mixed", bmg.MIXED, bmg.KRYLOV_NONE),
+            ("fcg_fp64", bmg.FP64, bmg.KRYLOV_FCG),
+            ("fcg_mixed", bmg.MIXED, bmg.KRYLOV_FCG)]
+    if args.only:
+        plan = [p for p in plan if p[0] == args.only]
+    results = {}
+    for mname, prec, kry in plan:
+        log(f"[bench] {name} {mname}: warmup {args.warmup}, steps {args.steps}")
+        results[mname] = run_gpu(prob, prec, kry, args.steps, args.warmup,
+                                 profile=(kry == bmg.KRYLOV_NONE))
+        r = results[mname]
+        log(f"[bench] {name} {mname}: {r['ms']:.2f} ms/solve, {r['iters']} "
+            f"iterations, final rel. residual {r['final_residual']:.3e}")
+    ref_note = None
+    if not args.no_check:
+        log(f"[bench] {name}: FP64 reference solution")
+        xe, ref_note = reference_solution(name, prob)
+        for r in results.values():
+            r["err_vs_fp64_reference"] = direct.rel_error(
+                r["x"][1:-1, 1:-1, 1:-1], xe, True)
+    cpu = None
+    if not args.no_cpu:
+        log(f"[bench] {name}: CPU baseline sample")
+        iters = results.get("fp64", next(iter(results.values())))["iters"]
+        cpu = cpu_baseline(prob, iters, args.cpu_sample_iters, args.n)
+    best = min(results, key=lambda k: results[k]["ms"])
+    out = {
+        "workload": w["desc"].format(n=args.n),
+        "best": best,
+        "roofline": roofline(results.get("fp64"), w["kernel"],
+                             traffic_from_profiles(w["traffic"])),
+        "cpu_baseline": cpu,
+        "fp64_reference": ref_note,
+        "solves": {k: {"ms": round(v["ms"], 3),
+                       "device_ms": round(v["device_ms"], 3),
+                       "iters": v["iters"],
+                       "final_rel_residual": v["final_residual"],
+                       "err_vs_fp64_reference": v.get("err_vs_fp64_reference")}
+                   for k, v in results.items()},
+    }
+    if "fp64" in results:
+        out["speedup_vs_plain_fp64_mg"] = {
+            k: round(results["fp64"]["ms"] / v["ms"], 3)
+            for k, v in results.items()}
+    return out, results
 
 
 def main():
@@ -138,9 +252,14 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--workload", choices=["spheres", "channel", "both"],
+                    default="both",
+                    help="headline is spheres; 'both' adds the channel")
     ap.add_argument("--cpu-sample-iters", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true",
-                    help="skip the CPU baseline and the FP64 direct check")
+                    help="skip the CPU baseline")
+    ap.add_argument("--no-check", action="store_true",
+                    help="skip the FP64 reference solution")
     ap.add_argument("--only", choices=["fp64", "mixed", "fcg_fp64", "fcg_mixed"],
                     default=None)
     ap.add_argument("--max-iter", type=int, default=999,
@@ -151,44 +270,18 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(each MI355X runs one brick; see the sharding design).", file=sys.stderr)
+              "(each MI355X runs one brick; see the sharding design).",
+              file=sys.stderr)
         return 2
 
-    from briscola_amd import bmg, workloads
-    log(f"[bench] generating channel {args.n}^3")
-    prob = workloads.channel(args.n, seed=0)
-    results = {}
-    plan = [("fp64", bmg.FP64, bmg.KRYLOV_NONE), ("mixed", bmg.MIXED, bmg.KRYLOV_NONE),
-            ("fcg_fp64", bmg.FP64, bmg.KRYLOV_FCG), ("fcg_mixed", bmg.MIXED, bmg.KRYLOV_FCG)]
-    if args.only:
-        plan = [p for p in plan if p[0] == args.only]
-    for name, prec, kry in plan:
-        log(f"[bench] {name}: warmup {args.warmup}, steps {args.steps}")
-        results[name] = run_gpu(prob, prec, kry, args.steps, args.warmup,
-                                profile=(kry == bmg.KRYLOV_NONE))
-        r = results[name]
-        log(f"[bench] {name}: {r['ms']:.2f} ms/solve, {r['iters']} iterations, "
-            f"final rel. residual {r['final_residual']:.3e}")
+    head = "spheres" if args.workload in ("spheres", "both") else "channel"
+    main_out, main_res = run_workload(head, args)
+    extra = None
+    if args.workload == "both":
+        extra, _ = run_workload("channel", args)
 
-    cpu = None
-    if not args.no_cpu:
-        log("[bench] CPU baseline sample and FP64 direct check")
-        iters = results.get("fp64", next(iter(results.values())))["iters"]
-        cpu = cpu_baseline_and_check(prob, results, iters,
-                                     args.cpu_sample_iters)["cpu"]
-
-    best = min(results, key=lambda k: results[k]["ms"])
-    rb = results[best]
-    roof = None
-    if "fp64" in results and results["fp64"]["prof"]:
-        avg_ms, per_launch = results["fp64"]["prof"]
-        achieved = per_launch / (avg_ms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic_from_profiles(),
-                "kernel": "k_rbgs<double, 1> (level-0 red-black phase, interior region)",
-                "avg_launch_us": round(avg_ms * 1e3, 2),
-                "bytes_per_launch": per_launch}
+    best = main_out["best"]
+    rb = main_res[best]
     line = {
         "metric": METRIC,
         "value": round(rb["ms"], 3),
@@ -202,21 +295,16 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if best in ("fp64", "fcg_fp64") else "f32+f64",
         "data": "synthetic",
-        "config": {"workload": (f"channel {args.n}^3: periodic x,z, Neumann y, tanh "
-                                f"beta=2, Fourier RHS; MG V(2,2) rbgs, linear "
-                                f"transfers, 4^3 direct coarse, relTol 1e-8"),
-                   "method": best},
-        "roofline": roof,
-        "cpu_baseline": cpu,
-        "solves": {k: {"ms": round(v["ms"], 3), "device_ms": round(v["device_ms"], 3),
-                       "iters": v["iters"],
-                       "final_rel_residual": v["final_residual"],
-                       "err_vs_fp64_oracle": v.get("err_vs_fp64_oracle")}
-                   for k, v in results.items()},
+        "config": {"workload": main_out["workload"], "method": best},
+        "roofline": main_out["roofline"],
+        "cpu_baseline": main_out["cpu_baseline"],
+        "fp64_reference": main_out["fp64_reference"],
+        "solves": main_out["solves"],
     }
-    if "fp64" in results:
-        line["speedup_vs_plain_fp64_mg"] = {
-            k: round(results["fp64"]["ms"] / v["ms"], 3) for k, v in results.items()}
+    if "speedup_vs_plain_fp64_mg" in main_out:
+        line["speedup_vs_plain_fp64_mg"] = main_out["speedup_vs_plain_fp64_mg"]
+    if extra is not None:
+        line["channel256"] = extra
     print(json.dumps(line), flush=True)
     return 0
 

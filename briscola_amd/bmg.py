@@ -18,7 +18,7 @@ VCYCLE, WCYCLE, FCYCLE = 0, 1, 2
 COARSE_SMOOTH, COARSE_DIRECT = 0, 1
 RBGS, JAC = 0, 1
 FP64, MIXED = 0, 1
-KRYLOV_NONE, KRYLOV_FCG = 0, 1
+KRYLOV_NONE, KRYLOV_FCG, KRYLOV_FCG_LEARNED = 0, 1, 2
 X, B, R = 0, 1, 2
 
 EXPORTS = [
@@ -62,7 +62,10 @@ class Stats(ctypes.Structure):
                 ("initial_residual", ctypes.c_double),
                 ("final_residual", ctypes.c_double),
                 ("solve_ms", ctypes.c_double),
-                ("reserved", ctypes.c_double * 4)]
+                ("learned_iters", ctypes.c_int),
+                ("learned_fallback", ctypes.c_int),
+                ("learned_guess_residual", ctypes.c_double),
+                ("reserved", ctypes.c_double * 2)]
 
 
 _lib = None

@@ -3,6 +3,8 @@
 // declared in include/bmg.h.  All numerical work runs in the HIP kernels of
 // bmg_kernels.hip on the context's stream.
 #include "bmg.h"
+#include "bmg_unet.h"
+#include "unet_internal.h"
 #include "bmg_internal.h"
 
 #include <hip/hip_runtime.h>
@@ -203,11 +205,26 @@ struct bmg_ctx
     double* kr[2] = {nullptr, nullptr};
     double* kp[2] = {nullptr, nullptr};
     double* kq[2] = {nullptr, nullptr};
-    double* kz[2] = {nullptr, nullptr};
     hipGraph_t pgraph = nullptr;
     hipGraphExec_t pgraphExec = nullptr;
     int pgraphMixed = -1;
     long pgraphGen = -1;
+    // one flexible-CG iteration (operator, updates, preconditioner, next
+    // direction) captured as a HIP graph
+    hipGraph_t fgraph = nullptr;
+    hipGraphExec_t fgraphExec = nullptr;
+    int fgraphMode = -1;
+    long fgraphGen = -1;
+    // learned preconditioner: network (not owned), dense FP32 buffers of the
+    // level-0 interior (input, coefficient channel, output) and the host
+    // copy of the coefficient channel of the current operator
+    bmg_unet* unet = nullptr;
+    float* uIn = nullptr;
+    float* uCoef = nullptr;
+    float* uOut = nullptr;
+    std::vector<float> coefChannel;
+    double coefScale = 1.0;   // 10^mean(log10 lambda) of the cells
+    bool coefUploaded = false;
     // one MG iteration captured as a HIP graph and replayed
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
@@ -247,10 +264,15 @@ struct bmg_ctx
         if (graph) (void)hipGraphDestroy(graph);
         if (pgraphExec) (void)hipGraphExecDestroy(pgraphExec);
         if (pgraph) (void)hipGraphDestroy(pgraph);
+        if (fgraphExec) (void)hipGraphExecDestroy(fgraphExec);
+        if (fgraph) (void)hipGraphDestroy(fgraph);
+        devFree(uIn);
+        devFree(uCoef);
+        devFree(uOut);
         for (int c = 0; c < 2; c++)
         {
             devFree(kx[c]); devFree(kb[c]); devFree(kr[c]);
-            devFree(kp[c]); devFree(kq[c]); devFree(kz[c]);
+            devFree(kp[c]); devFree(kq[c]);
         }
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -322,8 +344,12 @@ struct bmg_ctx
     void enqueueIteration(bool mixed);
     double iterate(bool mixed);
     void enqueuePrecondition(bool mixed);
+    // FCG preconditioner modes: 0 FP64 cycle, 1 mixed cycle, 2 learned
+    void enqueueFCGIteration(int mode);
+    void fcgIteration(int mode);
+    void enqueueLearned(const double* ssDev, double ssHost, Pair<double> z);
+    void prepareLearned();
     void precondition(bool mixed);
-    double dots(int nd, const Pair<const double>* ab, double* out);
     double normFactorOf(Pair<const double> x, Pair<const double> r,
                         Pair<const double> b);
     void runFCG(bool constMatrix, bmg_stats* st);
@@ -933,30 +959,158 @@ double bmg_ctx::iterate(bool mixed)
 }
 
 // z = M(r): one MG cycle on A z = r with z0 = 0, in FP64 or in FP32.  The
-// level-0 work arrays of the cycle receive r as right-hand side and as the
-// residual the first repetition restricts (MG.C:72-77).
+// caller has staged r as the level-0 right-hand side of the cycle (and, in
+// FP64, as the residual its first repetition restricts, MG.C:72-77); z is
+// left in the level-0 x (FP64) or xf (FP32).
 void bmg_ctx::enqueuePrecondition(bool mixed)
 {
     Level& q0 = lv[0];
     const long n = q0.L.ncol;
-    Pair<const double> r{{kr[0], kr[1]}};
     std::vector<int> visits(nLevels(), 0);
     if (!mixed)
     {
-        launch_copy<double>(n, r, pr(q0.b), stream);
-        launch_copy<double>(n, r, pr(q0.r), stream);
         launch_fill<double>(pr(q0.x), n, 0.0, stream);
         cycle<double>(visits, 0);
-        launch_copy<double>(n, cpr(q0.x), pr(kz), stream);
     }
     else
     {
-        launch_to_float(n, r, pr(q0.bf), stream);
         launch_fill<float>(pr(q0.xf), n, 0.0f, stream);
         cycle<float>(visits, 0);
-        launch_to_double(n, Pair<const float>{{q0.xf[0], q0.xf[1]}}, pr(kz),
-                         stream);
     }
+}
+
+// One flexible-CG iteration with its scalars on the device (sc = dScalar):
+// q = A p with p.q, p.r; x += alpha p, r -= alpha q with r.r and r staged
+// for the preconditioner; z = M r; z.q; p = z + beta p; the four scalars
+// copied to the pinned host buffer.
+void bmg_ctx::enqueueFCGIteration(int mode)
+{
+    Level& q0 = lv[0];
+    const OpView<double> o = op<double>(0);
+    Pair<double> X{{kx[0], kx[1]}}, Rv{{kr[0], kr[1]}}, Pv{{kp[0], kp[1]}},
+        Qv{{kq[0], kq[1]}};
+    if (anyPeriodic())
+        launch_bc_fill(q0.L, o, Pv, 0, 1, bcval, stream);
+    const int nb = launch_apply_dots(q0.L, o, cst(Pv), Qv, cst(Rv), partials,
+                                     stream);
+    launch_reduce_rows(partials, nb, 2, dScalar, stream);
+    const Pair<float> rf = mode == 1 ? pr(q0.bf)
+                                     : Pair<float>{{nullptr, nullptr}};
+    const Pair<double> r0 = mode == 0 ? pr(q0.r)
+                                      : Pair<double>{{nullptr, nullptr}};
+    launch_fcg_update(q0.L, dScalar, X, Rv, cst(Pv), cst(Qv), rf, r0,
+                      pr(q0.b), partials, stream);
+    launch_reduce_rows(partials, nb, 1, dScalar + 2, stream);
+    if (mode == 2)
+        enqueueLearned(dScalar + 2, 0.0, pr(q0.x));
+    else
+        enqueuePrecondition(mode == 1);
+    if (mode == 1)
+    {
+        launch_dot_zq<float>(q0.L, cpr(q0.xf), cst(Qv), partials, stream);
+        launch_reduce_rows(partials, nb, 1, dScalar + 3, stream);
+        launch_update_p<float>(q0.L, dScalar, cpr(q0.xf), Pv, stream);
+    }
+    else
+    {
+        launch_dot_zq<double>(q0.L, cpr(q0.x), cst(Qv), partials, stream);
+        launch_reduce_rows(partials, nb, 1, dScalar + 3, stream);
+        launch_update_p<double>(q0.L, dScalar, cpr(q0.x), Pv, stream);
+    }
+    HIPCK(hipGetLastError());
+    HIPCK(hipMemcpyAsync(hScalar, dScalar, 4*sizeof(double),
+                         hipMemcpyDeviceToHost, stream));
+}
+
+// z = U(r / rms(r)) * rms(r) / h on the level-0 interior (the learned
+// preconditioner, bmg_unet.h); rms from *ssDev or ssHost (sum of squares).
+void bmg_ctx::enqueueLearned(const double* ssDev, double ssHost,
+                             Pair<double> z)
+{
+    Level& q0 = lv[0];
+    // z = U * rms / (h s), h the cell width along the first axis and s the
+    // geometric-mean coefficient
+    const double len = lv[0].ax[0].v.back() - lv[0].ax[0].v.front();
+    const double invh = double(q0.L.N[0])/(len*coefScale);
+    launch_split_to_dense(q0.L, Pair<const double>{{kr[0], kr[1]}}, ssDev,
+                          ssHost, uIn, stream);
+    bmg::unet::net_forward(unet, q0.L.N[0], q0.L.N[1], q0.L.N[2], uIn, uCoef,
+                           uOut, stream);
+    launch_dense_to_split(q0.L, uOut, ssDev, ssHost, invh, z, stream);
+}
+
+// Buffers and coefficient channel of the learned preconditioner (outside
+// any graph capture: this allocates).
+void bmg_ctx::prepareLearned()
+{
+    if (!unet)
+        throw Error(BMG_ERR_STATE, "learned Krylov mode without a network "
+                                   "(bmg_set_learned)");
+    Level& q0 = lv[0];
+    for (int d = 0; d < 3; d++)
+        if (q0.L.N[d] % 8)
+            throw Error(BMG_ERR_ARG, "the learned preconditioner needs level-0 "
+                                     "extents that are multiples of 8");
+    const size_t M = size_t(q0.cells());
+    if (!uIn)
+    {
+        uIn = devAlloc<float>(M);
+        uCoef = devAlloc<float>(M);
+        uOut = devAlloc<float>(M);
+        coefUploaded = false;
+        gen++;
+    }
+    if (!coefUploaded)
+    {
+        if (coefChannel.size() != M) coefChannel.assign(M, 1.0f);
+        HIPCK(hipMemcpy(uCoef, coefChannel.data(), M*sizeof(float),
+                        hipMemcpyHostToDevice));
+        coefUploaded = true;
+    }
+    try
+    {
+        bmg::unet::net_prepare(unet, q0.L.N[0], q0.L.N[1], q0.L.N[2]);
+    }
+    catch (const std::exception& e)
+    {
+        throw Error(BMG_ERR_HIP, e.what());
+    }
+}
+
+void bmg_ctx::fcgIteration(int mode)
+{
+    if (!useGraphs)
+    {
+        enqueueFCGIteration(mode);
+        HIPCK(hipStreamSynchronize(stream));
+        return;
+    }
+    if (opts.smoother == BMG_JAC && mode != 2)
+        for (int l = 0; l < nLevels(); l++)
+            mode == 1 ? (void)Tmp<float>(l) : (void)Tmp<double>(l);
+    if (!fgraphExec || fgraphGen != gen || fgraphMode != mode)
+    {
+        if (fgraphExec) { HIPCK(hipGraphExecDestroy(fgraphExec)); fgraphExec = nullptr; }
+        if (fgraph) { HIPCK(hipGraphDestroy(fgraph)); fgraph = nullptr; }
+        HIPCK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        try
+        {
+            enqueueFCGIteration(mode);
+        }
+        catch (...)
+        {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        HIPCK(hipStreamEndCapture(stream, &fgraph));
+        HIPCK(hipGraphInstantiate(&fgraphExec, fgraph, nullptr, nullptr, 0));
+        fgraphGen = gen;
+        fgraphMode = mode;
+    }
+    HIPCK(hipGraphLaunch(fgraphExec, stream));
+    HIPCK(hipStreamSynchronize(stream));
 }
 
 void bmg_ctx::precondition(bool mixed)
@@ -993,29 +1147,18 @@ void bmg_ctx::precondition(bool mixed)
     HIPCK(hipGraphLaunch(pgraphExec, stream));
 }
 
-// Up to three dot products over the level-0 interior; returns the first.
-double bmg_ctx::dots(int nd, const Pair<const double>* ab, double* out)
-{
-    const int nb = launch_dots(lv[0].L, nd, ab, partials, stream);
-    for (int d = 0; d < nd; d++)
-        launch_reduce_final(partials + size_t(d)*nb, nb, dScalar + d, stream);
-    HIPCK(hipGetLastError());
-    HIPCK(hipMemcpyAsync(hScalar, dScalar, nd*sizeof(double),
-                         hipMemcpyDeviceToHost, stream));
-    HIPCK(hipStreamSynchronize(stream));
-    for (int d = 0; d < nd; d++) out[d] = hScalar[d];
-    return out[0];
-}
-
 // Flexible conjugate gradients (Notay 2000, one stored direction) with one
 // MG cycle as the variable preconditioner; the stopping test is the
 // reference's (normFactors/checkConvergence, solver.C:13-95) on the updated
-// residual, confirmed on the true residual b - A x before returning.
+// residual, confirmed on the true residual b - A x before returning.  Each
+// iteration is one graph launch and one host synchronisation.
 void bmg_ctx::runFCG(bool constMatrix, bmg_stats* st)
 {
     prepare(constMatrix);
     const bool mixed = opts.precision == BMG_MIXED;
     if (mixed) ensureMixed();
+    const bool learned = opts.krylov == BMG_KRYLOV_FCG_LEARNED;
+    if (learned) prepareLearned();
     Level& q0 = lv[0];
     const long n = q0.L.ncol;
     if (!kx[0])
@@ -1024,15 +1167,15 @@ void bmg_ctx::runFCG(bool constMatrix, bmg_stats* st)
         {
             kx[c] = devAlloc<double>(n); kb[c] = devAlloc<double>(n);
             kr[c] = devAlloc<double>(n); kp[c] = devAlloc<double>(n);
-            kq[c] = devAlloc<double>(n); kz[c] = devAlloc<double>(n);
+            kq[c] = devAlloc<double>(n);
         }
         gen++;
-        for (double** f : {kr, kp, kq, kz})
+        for (double** f : {kr, kp, kq})
             launch_fill<double>(Pair<double>{{f[0], f[1]}}, n, 0.0, stream);
     }
     const OpView<double> o = op<double>(0);
     Pair<double> X{{kx[0], kx[1]}}, Bv{{kb[0], kb[1]}}, Rv{{kr[0], kr[1]}},
-        Pv{{kp[0], kp[1]}}, Qv{{kq[0], kq[1]}}, Zv{{kz[0], kz[1]}};
+        Pv{{kp[0], kp[1]}};
     launch_copy<double>(n, cpr(q0.x), X, stream);
     launch_copy<double>(n, cpr(q0.b), Bv, stream);
     int nb = 0;
@@ -1052,32 +1195,89 @@ void bmg_ctx::runFCG(bool constMatrix, bmg_stats* st)
     int conv = check(cur);
     int iter = 0;
     bool restart = true;
-    double pq = 0.0;
+    // learned mode: the network's initial guess, kept only if it lowers the
+    // residual (x is saved in q meanwhile)
+    bool learnedActive = learned;
+    int learnedIters = 0, fallbackAt = -1;
+    double guessResidual = 0.0;
+    if (learned && conv == 0)
+    {
+        Pair<double> Qv{{kq[0], kq[1]}};
+        launch_copy<double>(n, cst(X), Qv, stream);
+        enqueueLearned(nullptr, ss, pr(q0.x));
+        launch_add_interior(q0.L, X, cpr(q0.x), stream);
+        if (anyPeriodic())
+            launch_bc_fill(q0.L, o, X, 0, 1, bcval, stream);
+        launch_residual(q0.L, o, cst(X), cst(Bv), Rv, partials, &nb, stream);
+        const double ssg = reduceFinal(nb);
+        guessResidual = std::sqrt(ssg)/nf;
+        if (std::isfinite(ssg) && ssg < ss)
+        {
+            ss = ssg;
+            cur = guessResidual;
+            conv = check(cur);
+        }
+        else
+        {
+            launch_copy<double>(n, cst(Qv), X, stream);
+            launch_residual(q0.L, o, cst(X), cst(Bv), Rv, partials, &nb,
+                            stream);
+            ss = reduceFinal(nb);
+            learnedActive = false;
+            fallbackAt = 0;
+        }
+    }
     while ((iter < opts.max_iter && conv == 0) || iter < opts.min_iter)
     {
+        const int mode = learnedActive ? 2 : (mixed ? 1 : 0);
         if (restart)
         {
-            precondition(mixed);
-            launch_copy<double>(n, cst(Zv), Pv, stream);
+            // p = z = M r for the current residual
+            if (mode == 2)
+            {
+                enqueueLearned(nullptr, ss, pr(q0.x));
+                launch_copy<double>(n, cpr(q0.x), Pv, stream);
+            }
+            else
+            {
+                if (mixed)
+                    launch_to_float(n, cst(Rv), pr(q0.bf), stream);
+                else
+                {
+                    launch_copy<double>(n, cst(Rv), pr(q0.b), stream);
+                    launch_copy<double>(n, cst(Rv), pr(q0.r), stream);
+                }
+                precondition(mixed);
+                if (mixed)
+                    launch_to_double(n, Pair<const float>{{q0.xf[0], q0.xf[1]}},
+                                     Pv, stream);
+                else
+                    launch_copy<double>(n, cpr(q0.x), Pv, stream);
+            }
             restart = false;
         }
-        if (anyPeriodic())
-            launch_bc_fill(q0.L, o, Pv, 0, 1, bcval, stream);
-        launch_apply(q0.L, o, cst(Pv), Qv, stream);
-        const Pair<const double> ab[4] = {cst(Pv), cst(Qv), cst(Pv), cst(Rv)};
-        double d2[2];
-        dots(2, ab, d2);
-        pq = d2[0];
+        const double prev = cur;
+        fcgIteration(mode);
+        const double pq = hScalar[0];
         if (!(pq != 0.0) || !std::isfinite(pq))
             throw Error(BMG_ERR_NUMERIC, "flexible CG breakdown (p.Ap = 0)");
-        const double alpha = d2[1]/pq;
-        nb = launch_cg_update(q0.L, alpha, X, Rv, cst(Pv), cst(Qv), partials,
-                              stream);
-        ss = reduceFinal(nb);
+        ss = hScalar[2];
         cur = std::sqrt(ss)/nf;
         if (!std::isfinite(cur))
             throw Error(BMG_ERR_NUMERIC, "residual norm is not finite");
         iter++;
+        if (mode == 2)
+        {
+            learnedIters++;
+            // fall back to multigrid cycles once an iteration's residual
+            // drop is worse than 0.3
+            if (cur > 0.3*prev)
+            {
+                learnedActive = false;
+                fallbackAt = iter;
+                restart = true;
+            }
+        }
         conv = check(cur);
         if (conv && iter >= opts.min_iter)
         {
@@ -1091,14 +1291,7 @@ void bmg_ctx::runFCG(bool constMatrix, bmg_stats* st)
             conv = check(cur);
             if (conv) break;
             restart = true;
-            continue;
         }
-        precondition(mixed);
-        const Pair<const double> zq[2] = {cst(Zv), cst(Qv)};
-        double d1[1];
-        dots(1, zq, d1);
-        const double beta = -d1[0]/pq;
-        launch_xpby(n, cst(Zv), beta, Pv, stream);
     }
     launch_copy<double>(n, cst(X), pr(q0.x), stream);
     launch_copy<double>(n, cst(Bv), pr(q0.b), stream);
@@ -1109,6 +1302,9 @@ void bmg_ctx::runFCG(bool constMatrix, bmg_stats* st)
         st->iters = iter;
         st->initial_residual = init;
         st->final_residual = cur;
+        st->learned_iters = learnedIters;
+        st->learned_fallback = fallbackAt;
+        st->learned_guess_residual = guessResidual;
     }
 }
 
@@ -1147,7 +1343,7 @@ double bmg_ctx::normFactor()
 // MG::Solve (MG.C:132-254) around the cycles; FP64 or mixed precision.
 void bmg_ctx::runSolve(bool constMatrix, bmg_stats* st)
 {
-    if (opts.krylov == BMG_KRYLOV_FCG)
+    if (opts.krylov == BMG_KRYLOV_FCG || opts.krylov == BMG_KRYLOV_FCG_LEARNED)
     {
         runFCG(constMatrix, st);
         return;
@@ -1298,7 +1494,7 @@ static void validateOpts(const bmg_opts* o)
     if (o->cycle < 0 || o->cycle > 2 || o->coarse_mode < 0 ||
         o->coarse_mode > 1 || o->smoother < 0 || o->smoother > 1 ||
         o->precision < 0 || o->precision > 1 || o->krylov < 0 ||
-        o->krylov > 1)
+        o->krylov > 2)
         throw Error(BMG_ERR_ARG, "unknown enum value in options");
 }
 
@@ -1547,6 +1743,31 @@ int bmg_set_operator_laplacian(bmg_ctx* c, const double* lx, const double* ly,
         const bool withLambda = lx != nullptr;
         if (withLambda && (!ly || !lz))
             throw Error(BMG_ERR_ARG, "need all three lambda arrays");
+        // input channel of the learned preconditioner: (log10(mean of the
+        // six face coefficients) + 1.5) / 1.5 per cell, reference cell order
+        {
+            const LevelDims& L0 = c->lv[0].L;
+            const int N0 = L0.N[0], N1 = L0.N[1], N2 = L0.N[2];
+            const long P1 = L0.P[1], P2 = L0.P[2];
+            c->coefChannel.assign(size_t(N0)*N1*N2, 1.0f);
+            double logSum = 0.0;
+            if (withLambda)
+                for (int i = 1; i <= N0; i++)
+                    for (int j = 1; j <= N1; j++)
+                        for (int k = 1; k <= N2; k++)
+                        {
+                            const long p = (i*P1 + j)*P2 + k;
+                            const double m = (lx[p] + lx[p + P1*P2] + ly[p] +
+                                              ly[p + P2] + lz[p] + lz[p + 1])/6.0;
+                            const double v = m > 0.0 ? std::log10(m) : -6.0;
+                            const float ch = float((v + 1.5)/1.5);
+                            c->coefChannel[(size_t(i - 1)*N1 + (j - 1))*N2 +
+                                           (k - 1)] = ch;
+                            logSum += 1.5*double(ch) - 1.5;
+                        }
+            c->coefScale = std::pow(10.0, logSum/double(c->coefChannel.size()));
+            c->coefUploaded = false;
+        }
         std::vector<double*> lam(3*c->nLevels(), nullptr);
         if (withLambda)
         {
@@ -1977,6 +2198,15 @@ int bmg_time_smoother(bmg_ctx* c, int level, int precision, int reps,
         if (bytes)
             *bytes = cells*per*(precision == BMG_MIXED ? sizeof(float)
                                                        : sizeof(double));
+    });
+}
+
+int bmg_set_learned(bmg_ctx* c, bmg_unet* net)
+{
+    return run([&] {
+        if (!c) throw Error(BMG_ERR_ARG, "null context");
+        c->unet = net;
+        c->gen++;   // the captured learned iteration refers to the network
     });
 }
 

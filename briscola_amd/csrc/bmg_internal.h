@@ -165,17 +165,40 @@ void launch_reduce_final(const double* partials, int n, double* out,
                          hipStream_t s);
 void launch_diag(const LevelDims& L, Pair<const double> a7, Pair<double> x,
                  Pair<const double> b, hipStream_t s);
-// flexible CG building blocks (FP64, level 0)
-void launch_apply(const LevelDims& L, const OpView<double>& op,
-                  Pair<const double> p, Pair<double> q, hipStream_t s);
-// dot products of ab[0]*ab[1], ab[2]*ab[3], ab[4]*ab[5]; returns blocks
-int launch_dots(const LevelDims& L, int nd, const Pair<const double>* ab,
-                double* partials, hipStream_t s);
-int launch_cg_update(const LevelDims& L, double alpha, Pair<double> x,
-                     Pair<double> r, Pair<const double> p,
-                     Pair<const double> q, double* partials, hipStream_t s);
-void launch_xpby(long n, Pair<const double> z, double beta, Pair<double> p,
-                 hipStream_t s);
+// Flexible CG with device-resident scalars sc[0..3] = p.q, p.r, r.r, z.q.
+// q = A p with the partials of p.q (row 0) and p.r (row 1); returns tiles.
+int launch_apply_dots(const LevelDims& L, const OpView<double>& op,
+                      Pair<const double> p, Pair<double> q,
+                      Pair<const double> r, double* partials, hipStream_t s);
+// x += alpha p, r -= alpha q (alpha = sc[1]/sc[0]), partials of r.r, and r
+// copied for the preconditioner: to FP32 rf when rf.c[0] is set, else to the
+// FP64 r0 and b0 when r0.c[0] is set, else nowhere.  Returns tiles.
+int launch_fcg_update(const LevelDims& L, const double* sc, Pair<double> x,
+                      Pair<double> r, Pair<const double> p,
+                      Pair<const double> q, Pair<float> rf, Pair<double> r0,
+                      Pair<double> b0, double* partials, hipStream_t s);
+template <typename Z>
+int launch_dot_zq(const LevelDims& L, Pair<const Z> z, Pair<const double> q,
+                  double* partials, hipStream_t s);
+// p = z + beta p on interior cells, beta = -sc[3]/sc[0]
+template <typename Z>
+void launch_update_p(const LevelDims& L, const double* sc, Pair<const Z> z,
+                     Pair<double> p, hipStream_t s);
+// level-0 interior <-> dense FP32 (reference cell order) for the learned
+// preconditioner: out = r / rms(r), z = in * rms / h, rms = sqrt(ss/cells)
+// with ss = *ssDev when ssDev is set, else ssHost
+void launch_split_to_dense(const LevelDims& L, Pair<const double> r,
+                           const double* ssDev, double ssHost, float* out,
+                           hipStream_t s);
+void launch_dense_to_split(const LevelDims& L, const float* in,
+                           const double* ssDev, double ssHost, double invh,
+                           Pair<double> z, hipStream_t s);
+// x += z on interior cells
+void launch_add_interior(const LevelDims& L, Pair<double> x,
+                         Pair<const double> z, hipStream_t s);
+// out[row] = sum of partials[row*n .. row*n + n) for row < nrows
+void launch_reduce_rows(const double* partials, int n, int nrows, double* out,
+                        hipStream_t s);
 template <typename T>
 void launch_copy(long n, Pair<const T> a, Pair<T> b, hipStream_t s);
 void launch_to_double(long n, Pair<const float> a, Pair<double> b,

@@ -625,7 +625,7 @@ __global__ __launch_bounds__(BX*BY) void k_restrict(LevelDims Lf,
     for (int cx = 0; cx < 2; cx++)
     {
         const int fk = (Kh[h] - 1)*R2 + cz*oz + 1;
-        rv[h][cz][cy][cx] = ld(rf.c[(fi[cx] + fj[cy] + fk) & 1],
+        rv[h][cz][cy][cx] = ld(col(rf, (fi[cx] + fj[cy] + fk) & 1),
                                unsigned(cidx(Lf, fi[cx], fj[cy], fk)));
     }
     const T ux = tv.ru[0][I], uy = tv.ru[1][J];
@@ -712,7 +712,8 @@ __global__ __launch_bounds__(BX*BY) void k_prolong(LevelDims Lf, LevelDims Lc,
         {
             // for nz = 3 the fourth value repeats the third and is unused
             const int zz = zbase + (z < nz ? z : nz - 1);
-            cv[z][bb][a] = ld(xc.c[(par + zz) & 1], unsigned(base + (zz >> 1)));
+            cv[z][bb][a] = ld(col(xc, (par + zz) & 1),
+                               unsigned(base + (zz >> 1)));
         }
     }
     const T old[2] = {ld(col(xf, c.cA), c.idx), ld(col(xf, c.cB), c.idx + 1)};
@@ -793,12 +794,12 @@ __device__ __forceinline__ void ghost_cell(const LevelDims& L,
     if (allPer)
     {
         for (int d = 0; d < 3; d++) s[d] = g[d] - bo[d]*L.N[d];
-        val = x.c[(s[0] + s[1] + s[2]) & 1][cidx(L, s[0], s[1], s[2])];
+        val = col(x, (s[0] + s[1] + s[2]) & 1)[cidx(L, s[0], s[1], s[2])];
     }
     else
     {
         for (int d = 0; d < 3; d++) s[d] = g[d] - bo[d];
-        const T xi = x.c[(s[0] + s[1] + s[2]) & 1][cidx(L, s[0], s[1], s[2])];
+        const T xi = col(x, (s[0] + s[1] + s[2]) & 1)[cidx(L, s[0], s[1], s[2])];
         if (nout > 1)
             val = xi;
         else
@@ -815,7 +816,7 @@ __device__ __forceinline__ void ghost_cell(const LevelDims& L,
                 val = H*T(2.0)*T(bv.v[f]) - xi;
         }
     }
-    x.c[(I + J + K) & 1][cidx(L, I, J, K)] = val;
+    col(x, (I + J + K) & 1)[cidx(L, I, J, K)] = val;
 }
 
 // The ghost shell as the coarse solve walks it: flat 32-bit index, K
@@ -1427,7 +1428,7 @@ __global__ __launch_bounds__(256) void k_coarse_solve(LevelDims L,
     for (int q = tid; q < n; q += nt)
     {
         const int I = 1 + q/(N1*N2), J = 1 + (q/N2) % N1, K = 1 + q % N2;
-        x.c[(I + J + K) & 1][cidx(L, I, J, K)] = T(y[q]);
+        col(x, (I + J + K) & 1)[cidx(L, I, J, K)] = T(y[q]);
     }
     __syncthreads();
     const Shell sh = shell_all(L);
@@ -1520,74 +1521,78 @@ __global__ __launch_bounds__(1024) void k_reduce_final(const double* partials,
     if (threadIdx.x == 0) out[0] = s;
 }
 
-// q = A p (row product in the C,L,R,B,T,A,F order), interior cells.
-template <int OP>
-__global__ __launch_bounds__(BX*BY) void k_apply(LevelDims L,
-                                                 OpView<double> op,
-                                                 Pair<const double> p,
-                                                 Pair<double> q, Tiles tl)
-{
-    int bx, by, bz;
-    if (!tile_of(tl, bx, by, bz)) return;
-    PairPos c;
-    if (!pair_pos(L, bx, by, bz, c)) return;
-    double Ax[2];
-    pair_Ax<double, OP>(L, op, p, c, Ax);
-    st(col(q, c.cA), c.idx, Ax[0]);
-    if (c.vB) st(col(q, c.cB), c.idx + 1, Ax[1]);
-}
 
-// Up to three dot products over interior cells in one pass; per-tile
-// partials at partials[d*ntiles + tile].
-__global__ __launch_bounds__(BX*BY) void k_dots(LevelDims L, int nd,
-                                                Pair<const double> a0,
-                                                Pair<const double> b0,
-                                                Pair<const double> a1,
-                                                Pair<const double> b1,
-                                                Pair<const double> a2,
-                                                Pair<const double> b2,
-                                                Tiles tl, double* partials)
+
+
+// ---------------------------------------------------------------------------
+// Flexible CG iteration kernels with the scalars kept on the device.
+// sc[0] = p.q, sc[1] = p.r, sc[2] = r.r, sc[3] = z.q (each the sum of one
+// row of per-tile partials, reduced by k_reduce_rows).
+
+// q = A p and the per-tile partials of p.q (row 0) and p.r (row 1).
+template <int OP>
+__global__ __launch_bounds__(BX*BY) void k_apply_dots(LevelDims L,
+                                                      OpView<double> op,
+                                                      Pair<const double> p,
+                                                      Pair<double> q,
+                                                      Pair<const double> r,
+                                                      Tiles tl,
+                                                      double* partials)
 {
     __shared__ double sm[BX*BY/64];
     int bx, by, bz;
     if (!tile_of(tl, bx, by, bz)) return;
     PairPos c;
-    double acc[3] = {0.0, 0.0, 0.0};
+    double pq = 0.0, pr = 0.0;
     if (pair_pos(L, bx, by, bz, c))
     {
-        for (int h = 0; h < 2; h++)
+        const double rA = ld(col(r, c.cA), c.idx);
+        const double rB = ld(col(r, c.cB), c.idx + 1);
+        double Ax[2];
+        pair_Ax<double, OP>(L, op, p, c, Ax);
+        const double pA = ld(col(p, c.cA), c.idx);
+        const double pB = ld(col(p, c.cB), c.idx + 1);
+        st(col(q, c.cA), c.idx, Ax[0]);
+        pq += pA*Ax[0];
+        pr += pA*rA;
+        if (c.vB)
         {
-            if (h == 1 && !c.vB) continue;
-            const int k = h ? c.cB : c.cA;
-            const unsigned i = c.idx + h;
-            acc[0] += ld(col(a0, k), i)*ld(col(b0, k), i);
-            if (nd > 1) acc[1] += ld(col(a1, k), i)*ld(col(b1, k), i);
-            if (nd > 2) acc[2] += ld(col(a2, k), i)*ld(col(b2, k), i);
+            st(col(q, c.cB), c.idx + 1, Ax[1]);
+            pq += pB*Ax[1];
+            pr += pB*rB;
         }
     }
     const int nb = tl.gx*tl.gy*tl.gz;
     const int blk = tile_index(tl, bx, by, bz);
-    for (int d = 0; d < nd; d++)
-    {
-        const double s = block_sum(acc[d], sm);
-        if (threadIdx.x == 0 && threadIdx.y == 0) partials[d*nb + blk] = s;
-        __syncthreads();
-    }
+    double s = block_sum(pq, sm);
+    if (threadIdx.x == 0 && threadIdx.y == 0) partials[blk] = s;
+    __syncthreads();
+    s = block_sum(pr, sm);
+    if (threadIdx.x == 0 && threadIdx.y == 0) partials[nb + blk] = s;
 }
 
-// x += alpha p; r -= alpha q (interior) and per-tile partial sums of r^2.
-__global__ __launch_bounds__(BX*BY) void k_cg_update(LevelDims L,
-                                                     double alpha,
-                                                     Pair<double> x,
-                                                     Pair<double> r,
-                                                     Pair<const double> p,
-                                                     Pair<const double> q,
-                                                     Tiles tl,
-                                                     double* partials)
+// x += alpha p, r -= alpha q with alpha = sc[1]/sc[0], per-tile partials of
+// r.r, and r handed to the preconditioner: STAGE 1 as FP32 into rf (mixed
+// cycle), STAGE 0 as FP64 into both r0 and b0 (the level-0 right-hand side
+// and restricted residual of an FP64 cycle), STAGE 2 nowhere (the learned
+// preconditioner reads r itself).
+template <int STAGE>
+__global__ __launch_bounds__(BX*BY) void k_fcg_update(LevelDims L,
+                                                      const double* sc,
+                                                      Pair<double> x,
+                                                      Pair<double> r,
+                                                      Pair<const double> p,
+                                                      Pair<const double> q,
+                                                      Pair<float> rf,
+                                                      Pair<double> r0,
+                                                      Pair<double> b0,
+                                                      Tiles tl,
+                                                      double* partials)
 {
     __shared__ double sm[BX*BY/64];
     int bx, by, bz;
     if (!tile_of(tl, bx, by, bz)) return;
+    const double alpha = sc[1]/sc[0];
     PairPos c;
     double acc = 0.0;
     if (pair_pos(L, bx, by, bz, c))
@@ -1610,6 +1615,13 @@ __global__ __launch_bounds__(BX*BY) void k_cg_update(LevelDims L,
             st(col(x, k), i, xv[h] + alpha*pv[h]);
             const double rn = rv[h] - alpha*qv[h];
             st(col(r, k), i, rn);
+            if (STAGE == 1)
+                st(col(rf, k), i, float(rn));
+            else if (STAGE == 0)
+            {
+                st(col(r0, k), i, rn);
+                st(col(b0, k), i, rn);
+            }
             acc += rn*rn;
         }
     }
@@ -1618,17 +1630,146 @@ __global__ __launch_bounds__(BX*BY) void k_cg_update(LevelDims L,
         partials[tile_index(tl, bx, by, bz)] = s;
 }
 
-// p = z + beta p over the whole padded arrays (ghosts refreshed later).
-__global__ void k_xpby(long n, Pair<const double> z, double beta,
-                       Pair<double> p)
+// per-tile partials of z.q, z in FP64 or FP32
+template <typename Z>
+__global__ __launch_bounds__(BX*BY) void k_dot_zq(LevelDims L, Pair<const Z> z,
+                                                  Pair<const double> q,
+                                                  Tiles tl, double* partials)
 {
-    const long t = (long)blockIdx.x*blockDim.x + threadIdx.x;
-    if (t < n)
+    __shared__ double sm[BX*BY/64];
+    int bx, by, bz;
+    if (!tile_of(tl, bx, by, bz)) return;
+    PairPos c;
+    double acc = 0.0;
+    if (pair_pos(L, bx, by, bz, c))
     {
-        p.c[0][t] = z.c[0][t] + beta*p.c[0][t];
-        p.c[1][t] = z.c[1][t] + beta*p.c[1][t];
+        for (int h = 0; h < 2; h++)
+        {
+            if (h == 1 && !c.vB) continue;
+            const int k = h ? c.cB : c.cA;
+            const unsigned i = c.idx + h;
+            acc += double(ld(col(z, k), i))*ld(col(q, k), i);
+        }
+    }
+    const double s = block_sum(acc, sm);
+    if (threadIdx.x == 0 && threadIdx.y == 0)
+        partials[tile_index(tl, bx, by, bz)] = s;
+}
+
+// p = z + beta p on interior cells, beta = -sc[3]/sc[0]
+template <typename Z>
+__global__ __launch_bounds__(BX*BY) void k_update_p(LevelDims L,
+                                                    const double* sc,
+                                                    Pair<const Z> z,
+                                                    Pair<double> p, Tiles tl)
+{
+    int bx, by, bz;
+    if (!tile_of(tl, bx, by, bz)) return;
+    const double beta = -sc[3]/sc[0];
+    PairPos c;
+    if (!pair_pos(L, bx, by, bz, c)) return;
+    for (int h = 0; h < 2; h++)
+    {
+        if (h == 1 && !c.vB) continue;
+        const int k = h ? c.cB : c.cA;
+        const unsigned i = c.idx + h;
+        st(col(p, k), i, double(ld(col(z, k), i)) + beta*ld(col(p, k), i));
     }
 }
+
+// Interior of a level-0 FP64 colour-split field to a dense FP32 array in the
+// reference cell order (m = ((I-1)*N1 + J-1)*N2 + K-1), scaled by
+// 1/rms: rms = sqrt(ss/ncells) with ss from *ssDev when given, else ssHost.
+__global__ __launch_bounds__(BX*BY) void k_split_to_dense(LevelDims L,
+                                                          Pair<const double> r,
+                                                          const double* ssDev,
+                                                          double ssHost,
+                                                          float* out, Tiles tl)
+{
+    int bx, by, bz;
+    if (!tile_of(tl, bx, by, bz)) return;
+    PairPos c;
+    if (!pair_pos(L, bx, by, bz, c)) return;
+    const double ncell = double(L.N[0])*L.N[1]*L.N[2];
+    const double ss = ssDev ? *ssDev : ssHost;
+    const double inv = ss > 0.0 ? 1.0/sqrt(ss/ncell) : 0.0;
+    const long base = (long(c.I - 1)*L.N[1] + (c.J - 1))*L.N[2];
+    for (int h = 0; h < 2; h++)
+    {
+        if (h == 1 && !c.vB) continue;
+        const int k = h ? c.cB : c.cA;
+        const int K = 2*c.p + 1 + h;
+        out[base + K - 1] = float(ld(col(r, k), c.idx + h)*inv);
+    }
+}
+
+// The inverse map: z (interior) = double(in[m]) * rms / h, rms from the
+// same sum of squares.
+__global__ __launch_bounds__(BX*BY) void k_dense_to_split(LevelDims L,
+                                                          const float* in,
+                                                          const double* ssDev,
+                                                          double ssHost,
+                                                          double invh,
+                                                          Pair<double> z,
+                                                          Tiles tl)
+{
+    int bx, by, bz;
+    if (!tile_of(tl, bx, by, bz)) return;
+    PairPos c;
+    if (!pair_pos(L, bx, by, bz, c)) return;
+    const double ncell = double(L.N[0])*L.N[1]*L.N[2];
+    const double ss = ssDev ? *ssDev : ssHost;
+    const double f = sqrt(ss/ncell)*invh;
+    const long base = (long(c.I - 1)*L.N[1] + (c.J - 1))*L.N[2];
+    for (int h = 0; h < 2; h++)
+    {
+        if (h == 1 && !c.vB) continue;
+        const int k = h ? c.cB : c.cA;
+        const int K = 2*c.p + 1 + h;
+        st(col(z, k), c.idx + h, double(in[base + K - 1])*f);
+    }
+}
+
+// x += z on interior cells
+__global__ __launch_bounds__(BX*BY) void k_add_interior(LevelDims L,
+                                                        Pair<double> x,
+                                                        Pair<const double> z,
+                                                        Tiles tl)
+{
+    int bx, by, bz;
+    if (!tile_of(tl, bx, by, bz)) return;
+    PairPos c;
+    if (!pair_pos(L, bx, by, bz, c)) return;
+    for (int h = 0; h < 2; h++)
+    {
+        if (h == 1 && !c.vB) continue;
+        const int k = h ? c.cB : c.cA;
+        st(col(x, k), c.idx + h, ld(col(x, k), c.idx + h) +
+                                  ld(col(z, k), c.idx + h));
+    }
+}
+
+// Sums of nrows rows of n partials each, one workgroup per row, into
+// out[row] (fixed order, as k_reduce_final).
+__global__ __launch_bounds__(1024) void k_reduce_rows(const double* partials,
+                                                      int n, double* out)
+{
+    __shared__ double sm[16];
+    const double* row = partials + size_t(blockIdx.x)*n;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int i = threadIdx.x;
+    for (; i + 3*1024 < n; i += 4*1024)
+    {
+        a0 += row[i];
+        a1 += row[i + 1024];
+        a2 += row[i + 2*1024];
+        a3 += row[i + 3*1024];
+    }
+    for (; i < n; i += 1024) a0 += row[i];
+    const double s = block_sum((a0 + a1) + (a2 + a3), sm);
+    if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
 
 template <typename T>
 __global__ void k_copy(long n, Pair<const T> a, Pair<T> b)
@@ -2109,49 +2250,106 @@ void launch_diag(const LevelDims& L, Pair<const double> a7, Pair<double> x,
     hipLaunchKernelGGL(k_diag, grid_interior(L), BLK, 0, s, L, a7, x, b);
 }
 
-void launch_apply(const LevelDims& L, const OpView<double>& op,
-                  Pair<const double> p, Pair<double> q, hipStream_t s)
+
+
+
+int launch_apply_dots(const LevelDims& L, const OpView<double>& op,
+                      Pair<const double> p, Pair<double> q,
+                      Pair<const double> r, double* partials, hipStream_t s)
 {
     const Tiles tl = pair_tiles(L);
     const dim3 g = tiles_grid(tl);
     if (op.kind == OP_STENCIL)
-        hipLaunchKernelGGL((k_apply<OP_STENCIL>), g, BLK, 0, s, L, op, p,
-                           q, tl);
+        hipLaunchKernelGGL((k_apply_dots<OP_STENCIL>), g, BLK, 0, s, L, op, p,
+                           q, r, tl, partials);
     else if (op.kind == OP_CONST)
-        hipLaunchKernelGGL((k_apply<OP_CONST>), g, BLK, 0, s, L, op, p, q,
-                           tl);
+        hipLaunchKernelGGL((k_apply_dots<OP_CONST>), g, BLK, 0, s, L, op, p, q,
+                           r, tl, partials);
     else
-        hipLaunchKernelGGL((k_apply<OP_FACES>), g, BLK, 0, s, L, op, p, q,
-                           tl);
-}
-
-int launch_dots(const LevelDims& L, int nd, const Pair<const double>* ab,
-                double* partials, hipStream_t s)
-{
-    const Tiles tl = pair_tiles(L);
-    Pair<const double> z{{nullptr, nullptr}};
-    hipLaunchKernelGGL(k_dots, tiles_grid(tl), BLK, 0, s, L, nd, ab[0],
-                       ab[1], nd > 1 ? ab[2] : z, nd > 1 ? ab[3] : z,
-                       nd > 2 ? ab[4] : z, nd > 2 ? ab[5] : z, tl, partials);
+        hipLaunchKernelGGL((k_apply_dots<OP_FACES>), g, BLK, 0, s, L, op, p, q,
+                           r, tl, partials);
     return tiles_count(tl);
 }
 
-int launch_cg_update(const LevelDims& L, double alpha, Pair<double> x,
-                     Pair<double> r, Pair<const double> p,
-                     Pair<const double> q, double* partials, hipStream_t s)
+int launch_fcg_update(const LevelDims& L, const double* sc, Pair<double> x,
+                      Pair<double> r, Pair<const double> p,
+                      Pair<const double> q, Pair<float> rf, Pair<double> r0,
+                      Pair<double> b0, double* partials, hipStream_t s)
 {
     const Tiles tl = pair_tiles(L);
-    hipLaunchKernelGGL(k_cg_update, tiles_grid(tl), BLK, 0, s, L, alpha,
-                       x, r, p, q, tl, partials);
+    if (rf.c[0])
+        hipLaunchKernelGGL((k_fcg_update<1>), tiles_grid(tl), BLK, 0, s, L, sc,
+                           x, r, p, q, rf, r0, b0, tl, partials);
+    else if (r0.c[0])
+        hipLaunchKernelGGL((k_fcg_update<0>), tiles_grid(tl), BLK, 0, s, L, sc,
+                           x, r, p, q, rf, r0, b0, tl, partials);
+    else
+        hipLaunchKernelGGL((k_fcg_update<2>), tiles_grid(tl), BLK, 0, s, L, sc,
+                           x, r, p, q, rf, r0, b0, tl, partials);
     return tiles_count(tl);
 }
 
-void launch_xpby(long n, Pair<const double> z, double beta, Pair<double> p,
-                 hipStream_t s)
+void launch_split_to_dense(const LevelDims& L, Pair<const double> r,
+                           const double* ssDev, double ssHost, float* out,
+                           hipStream_t s)
 {
-    hipLaunchKernelGGL(k_xpby, dim3((n + 255)/256), dim3(256), 0, s, n, z,
-                       beta, p);
+    const Tiles tl = pair_tiles(L);
+    hipLaunchKernelGGL(k_split_to_dense, tiles_grid(tl), BLK, 0, s, L, r,
+                       ssDev, ssHost, out, tl);
 }
+
+void launch_dense_to_split(const LevelDims& L, const float* in,
+                           const double* ssDev, double ssHost, double invh,
+                           Pair<double> z, hipStream_t s)
+{
+    const Tiles tl = pair_tiles(L);
+    hipLaunchKernelGGL(k_dense_to_split, tiles_grid(tl), BLK, 0, s, L, in,
+                       ssDev, ssHost, invh, z, tl);
+}
+
+void launch_add_interior(const LevelDims& L, Pair<double> x,
+                         Pair<const double> z, hipStream_t s)
+{
+    const Tiles tl = pair_tiles(L);
+    hipLaunchKernelGGL(k_add_interior, tiles_grid(tl), BLK, 0, s, L, x, z, tl);
+}
+
+template <typename Z>
+int launch_dot_zq(const LevelDims& L, Pair<const Z> z, Pair<const double> q,
+                  double* partials, hipStream_t s)
+{
+    const Tiles tl = pair_tiles(L);
+    hipLaunchKernelGGL((k_dot_zq<Z>), tiles_grid(tl), BLK, 0, s, L, z, q, tl,
+                       partials);
+    return tiles_count(tl);
+}
+template int launch_dot_zq<double>(const LevelDims&, Pair<const double>,
+                                   Pair<const double>, double*, hipStream_t);
+template int launch_dot_zq<float>(const LevelDims&, Pair<const float>,
+                                  Pair<const double>, double*, hipStream_t);
+
+template <typename Z>
+void launch_update_p(const LevelDims& L, const double* sc, Pair<const Z> z,
+                     Pair<double> p, hipStream_t s)
+{
+    const Tiles tl = pair_tiles(L);
+    hipLaunchKernelGGL((k_update_p<Z>), tiles_grid(tl), BLK, 0, s, L, sc, z, p,
+                       tl);
+}
+template void launch_update_p<double>(const LevelDims&, const double*,
+                                      Pair<const double>, Pair<double>,
+                                      hipStream_t);
+template void launch_update_p<float>(const LevelDims&, const double*,
+                                     Pair<const float>, Pair<double>,
+                                     hipStream_t);
+
+void launch_reduce_rows(const double* partials, int n, int nrows, double* out,
+                        hipStream_t s)
+{
+    hipLaunchKernelGGL(k_reduce_rows, dim3(nrows), dim3(1024), 0, s, partials,
+                       n, out);
+}
+
 
 template <typename T>
 void launch_copy(long n, Pair<const T> a, Pair<T> b, hipStream_t s)

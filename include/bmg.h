@@ -60,8 +60,12 @@ enum bmg_precision { BMG_FP64 = 0, BMG_MIXED = 1 };
 /* Outer iteration: BMG_KRYLOV_NONE repeats MG cycles as MG::Solve does
  * (MG.C:191-242); BMG_KRYLOV_FCG runs flexible conjugate gradients (Notay,
  * FCG with one stored direction) preconditioned by one MG cycle of the
- * chosen precision, with the same stopping test. */
-enum bmg_krylov { BMG_KRYLOV_NONE = 0, BMG_KRYLOV_FCG = 1 };
+ * chosen precision, with the same stopping test; BMG_KRYLOV_FCG_LEARNED
+ * starts FCG from the guess of the attached learned network and
+ * preconditions with it until an iteration reduces the residual by less
+ * than a factor 0.3, then with MG cycles (bmg_unet.h). */
+enum bmg_krylov { BMG_KRYLOV_NONE = 0, BMG_KRYLOV_FCG = 1,
+                  BMG_KRYLOV_FCG_LEARNED = 2 };
 
 /* Mesh + boundary conditions: what MG reads from fvMesh and the BCs of
  * sys.x() (mesh.C:19-53 hierarchy, fvMeshMetrics.C metrics,
@@ -97,7 +101,12 @@ typedef struct bmg_stats {
     double initial_residual;
     double final_residual;
     double solve_ms;        /* device time of the whole solve             */
-    double reserved[4];
+    int learned_iters;      /* FCG iterations preconditioned by the network */
+    int learned_fallback;   /* iteration after which MG cycles took over
+                               (-1: never)                                 */
+    double learned_guess_residual;  /* residual after the network's initial
+                                       guess (normalised; 0: no guess)     */
+    double reserved[2];
 } bmg_stats;
 
 /* Fill opts with the reference defaults (rel_tol/tolerance set to 0). */

@@ -10,12 +10,14 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "bmg.h")
+UNET_HEADER = os.path.join(ROOT, "include", "bmg_unet.h")
 
 
-def declared_functions():
-    text = open(HEADER).read()
+def declared_functions(header=HEADER):
+    text = open(header).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    names = re.findall(r"^\s*(?:int|void|const char\*)\s+(bmg_\w+)\s*\(", text, flags=re.M)
+    names = re.findall(r"^\s*(?:int|void|long|const char\*)\s+(bmg_\w+)\s*\(",
+                       text, flags=re.M)
     return sorted(set(names))
 
 
@@ -39,6 +41,18 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, n)
     # the Python binding names exactly the declared set
     assert sorted(bmg.EXPORTS) == declared_functions()
+
+
+def test_library_exports_learned_preconditioner_symbols():
+    from briscola_amd import bmg, unet
+    out = subprocess.run(["nm", "-D", "--defined-only", bmg.LIB_PATH],
+                         capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r" T (bmg_\w+)", out))
+    names = declared_functions(UNET_HEADER)
+    assert "bmg_unet_train_step" in names and "bmg_set_learned" in names
+    assert not [n for n in names if n not in exported]
+    assert sorted(unet.EXPORTS) == names
+    unet._lib()   # every signature binds
 
 
 def test_library_is_gfx950_code_object(tmp_path):
@@ -81,9 +95,11 @@ def test_struct_layouts_match_header(tmp_path):
 #include <stddef.h>
 #include "bmg.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu %zu\\n", sizeof(bmg_grid), offsetof(bmg_grid, bc),
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(bmg_grid), offsetof(bmg_grid, bc),
          offsetof(bmg_grid, bc_value), sizeof(bmg_opts), offsetof(bmg_opts, precision),
-         sizeof(bmg_stats), offsetof(bmg_stats, solve_ms));
+         sizeof(bmg_stats), offsetof(bmg_stats, solve_ms),
+         offsetof(bmg_stats, learned_fallback),
+         offsetof(bmg_stats, learned_guess_residual));
   return 0;
 }""")
     exe = tmp_path / "layout"
@@ -93,7 +109,8 @@ int main(void) {
                                         check=True).stdout.split()]
     py = [ctypes.sizeof(bmg.Grid), bmg.Grid.bc.offset, bmg.Grid.bc_value.offset,
           ctypes.sizeof(bmg.Opts), bmg.Opts.precision.offset,
-          ctypes.sizeof(bmg.Stats), bmg.Stats.solve_ms.offset]
+          ctypes.sizeof(bmg.Stats), bmg.Stats.solve_ms.offset,
+          bmg.Stats.learned_fallback.offset, bmg.Stats.learned_guess_residual.offset]
     assert py == c
 
 

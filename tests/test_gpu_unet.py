@@ -1,0 +1,130 @@
+"""Learned preconditioner on the device: the BF16 matrix-core convolution
+kernels against the CPU reference (oracle/unet_ref.py) -- forward, data
+gradient and weight gradient --, a training step that lowers the loss, and
+the learned flexible-CG solve, which must reach the same tolerance and the
+same solution as the plain FP64 solve (the fall-back to MG cycles keeps it
+convergent whatever the network predicts)."""
+import numpy as np
+import pytest
+
+from oracle import unet_ref as U
+from briscola_amd import bmg, unet, workloads as W
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def dev_bf16(a):
+    """float32 array of BF16 values -> bfloat16 cuda tensor (same bits)."""
+    t = torch.from_numpy(U.bf16_bits(a).view(np.int16)).to("cuda:0")
+    return t.view(torch.bfloat16)
+
+
+def host(t):
+    return t.float().cpu().numpy()
+
+
+def close(gpu, ref, rel=2e-2):
+    scale = np.abs(ref).max() + 1e-30
+    err = np.abs(gpu - ref).max() / scale
+    assert err < rel, f"max error {err:.3e} of the largest value"
+
+
+@pytest.mark.parametrize("shape,cin,cout,relu", [
+    ((8, 8, 8), 16, 16, True),
+    ((8, 16, 8), 8, 32, False),
+    ((16, 8, 8), 48, 48, True),
+    ((8, 8, 16), 192, 64, True),
+])
+def test_conv_forward(gpu, shape, cin, cout, relu):
+    r = np.random.default_rng(cin + cout)
+    x = U.to_bf16(r.standard_normal(shape + (cin,)).astype(np.float32))
+    kp = unet.round32(27 * cin)
+    w = np.zeros((cout, kp), np.float32)
+    w[:, :27 * cin] = U.to_bf16(
+        (r.standard_normal((cout, 27 * cin)) / np.sqrt(27 * cin)).astype(np.float32))
+    bias = r.standard_normal(cout).astype(np.float32)
+    ref = U.conv(x, w[:, :27 * cin], bias, relu)
+    y = unet.op_conv(dev_bf16(x), dev_bf16(w),
+                     torch.from_numpy(bias).to("cuda:0"), cout, relu)
+    close(host(y), ref)
+
+
+@pytest.mark.parametrize("cin,cout", [(16, 32), (48, 16), (96, 32)])
+def test_conv_data_gradient(gpu, cin, cout):
+    r = np.random.default_rng(7 + cin)
+    shape = (8, 8, 16)
+    dz = U.to_bf16(r.standard_normal(shape + (cout,)).astype(np.float32))
+    kp = unet.round32(27 * cin)
+    w = np.zeros((cout, kp), np.float32)
+    w[:, :27 * cin] = U.to_bf16(
+        (r.standard_normal((cout, 27 * cin)) / np.sqrt(27 * cout)).astype(np.float32))
+    ref = U.conv_dgrad(dz, w[:, :27 * cin], cin)
+    dx = unet.op_conv_dgrad(dev_bf16(dz), dev_bf16(w), cin)
+    close(host(dx), ref)
+
+
+@pytest.mark.parametrize("cin,cout,chunk", [(8, 16, 256), (32, 64, 160),
+                                            (48, 16, 1024)])
+def test_conv_weight_gradient(gpu, cin, cout, chunk):
+    r = np.random.default_rng(11 + cin)
+    shape = (8, 16, 8)
+    x = U.to_bf16(r.standard_normal(shape + (cin,)).astype(np.float32))
+    dz = U.to_bf16(r.standard_normal(shape + (cout,)).astype(np.float32))
+    kp = unet.round32(27 * cin)
+    ref = U.conv_wgrad(x, dz, kp)
+    dw = unet.op_conv_wgrad(dev_bf16(x), dev_bf16(dz), kp, chunk)
+    got = dw.cpu().numpy()
+    # FP32 accumulation of exact BF16 products
+    assert np.abs(got - ref).max() < 1e-4 * np.abs(ref).max()
+    assert np.all(got[:, 27 * cin:] == 0.0)
+
+
+def small_problem(n=16, seed=3):
+    return W.spheres(n, seed=seed, nspheres=4)
+
+
+def test_training_lowers_the_loss(gpu):
+    prob = small_problem()
+    ctx = bmg.Context(prob.vertices, prob.bc, prob.bc_value, rel_tol=1e-10,
+                      tolerance=0.0, max_iter=500, n_sweeps_pre=2,
+                      n_sweeps_post=2, cycle=bmg.VCYCLE, coarse_level=1,
+                      krylov=bmg.KRYLOV_FCG)
+    ctx.set_laplacian(prob.lam)
+    x, _ = ctx.solve(prob.b.copy())
+    f, c, t, _, _ = unet.scaled_pair(prob, x)
+    net = unet.Net(seed=1)
+    d = lambda a: torch.from_numpy(a).to("cuda:0")
+    F, C, T = d(f), d(c), d(t)
+    losses = [net.train_step(F, C, T, 1e-3) for _ in range(40)]
+    assert np.isfinite(losses).all()
+    assert losses[-1] < 0.5 * losses[0], losses[::8]
+    # the forward entry point gives the trained prediction
+    p = net.forward(F, C).cpu().numpy()
+    e = p - p.mean() - t
+    assert np.mean(e * e) < losses[0]
+
+
+@pytest.mark.parametrize("precision", [bmg.FP64, bmg.MIXED], ids=["fp64", "mixed"])
+def test_learned_fcg_reaches_tolerance(gpu, precision):
+    prob = W.spheres(32, seed=2, nspheres=6)
+    base = dict(rel_tol=1e-8, tolerance=0.0, max_iter=500, n_sweeps_pre=2,
+                n_sweeps_post=2, cycle=bmg.VCYCLE, coarse_level=1,
+                precision=precision)
+    ref = bmg.Context(prob.vertices, prob.bc, prob.bc_value,
+                      **dict(base, krylov=bmg.KRYLOV_FCG, rel_tol=1e-12,
+                             precision=bmg.FP64))
+    ref.set_laplacian(prob.lam)
+    xr, _ = ref.solve(prob.b.copy())
+    ctx = bmg.Context(prob.vertices, prob.bc, prob.bc_value,
+                      **dict(base, krylov=bmg.KRYLOV_FCG_LEARNED))
+    ctx.set_laplacian(prob.lam)
+    net = unet.Net(seed=5)
+    net.attach(ctx)
+    x, st = ctx.solve(prob.b.copy())
+    assert st.final_residual / st.initial_residual <= 1e-8
+    xi, ri = x[1:-1, 1:-1, 1:-1], xr[1:-1, 1:-1, 1:-1]
+    err = np.linalg.norm((xi - xi.mean()) - (ri - ri.mean())) / np.linalg.norm(ri - ri.mean())
+    assert err < 1e-5
+    assert st.learned_fallback != -1 or st.learned_iters == st.iters
