@@ -10,6 +10,12 @@ Fourier-mode RHS).  Both use Briscola MG settings V(2,2) red-black
 Gauss-Seidel, linear transfers, coarse level 1 (4^3) solved directly,
 relTol 1e-8.  See briscola_amd/workloads.py for the generators.
 
+On the headline workload the learned preconditioner (a 3-D U-Net trained on
+the device before the timed solves, on pairs the multigrid generates at 64^3;
+briscola_amd/train_unet.py) is run as initial guess and preconditioner of
+flexible CG with the mixed-precision MG cycle as fall-back; its training
+time is reported separately and not part of a step.
+
 One step = one complete solve from b resident in HBM (x0 = 0) to x with
 ||r|| / ||r0|| <= 1e-8.  Methods: plain FP64 MG (the reference algorithm,
 bitwise equal to the CPU restatement), mixed-precision MG (FP32 cycles inside
@@ -95,12 +101,14 @@ def traffic_from_profiles(fname):
 
 
 def run_gpu(prob, precision, krylov, steps, warmup, profile, reps=50,
-            rel_tol=1e-8, max_iter=None):
+            rel_tol=1e-8, max_iter=None, net=None):
     import torch
     from briscola_amd import bmg
     ctx = bmg.Context(prob.vertices, prob.bc, prob.bc_value,
                       **solver_opts(precision, krylov, rel_tol, max_iter))
     ctx.set_laplacian(prob.lam)
+    if net is not None:
+        net.attach(ctx)
     b_dev = torch.from_numpy(prob.b).to("cuda:0")
     x_dev = torch.zeros_like(b_dev)
     stats = None
@@ -125,7 +133,9 @@ def run_gpu(prob, precision, krylov, steps, warmup, profile, reps=50,
     ctx.close()
     return dict(ms=1e3 * wall / steps, device_ms=dev_ms / steps,
                 iters=stats.iters, final_residual=stats.final_residual, x=x,
-                prof=prof)
+                prof=prof, learned_iters=stats.learned_iters,
+                learned_fallback=stats.learned_fallback,
+                learned_guess_residual=stats.learned_guess_residual)
 
 
 def reference_solution(name, prob):
@@ -202,13 +212,35 @@ def run_workload(name, args):
             ("mixed", bmg.MIXED, bmg.KRYLOV_NONE),
             ("fcg_fp64", bmg.FP64, bmg.KRYLOV_FCG),
             ("fcg_mixed", bmg.MIXED, bmg.KRYLOV_FCG)]
+    learned = None
+    if name == "spheres" and not args.no_learned:
+        plan.append(("fcg_learned_mixed", bmg.MIXED, bmg.KRYLOV_FCG_LEARNED))
     if args.only:
         plan = [p for p in plan if p[0] == args.only]
     results = {}
+    net = None
+    if any(p[2] == bmg.KRYLOV_FCG_LEARNED for p in plan):
+        from briscola_amd import train_unet
+        log(f"[bench] training the learned preconditioner "
+            f"({args.train_n}^3, {args.train_samples} pairs, "
+            f"{args.train_epochs} epochs)")
+        net, hist, t_data, t_train = train_unet.train(
+            args.train_n, args.train_samples, args.train_epochs, lr=2e-3,
+            verbose=False)
+        learned = {"train_grid": f"{args.train_n}^3",
+                   "train_pairs": args.train_samples,
+                   "epochs": args.train_epochs,
+                   "data_s": round(t_data, 2), "train_s": round(t_train, 2),
+                   "loss_first_epoch": hist[0], "loss_last_epoch": hist[-1],
+                   "network": ("4-level 3-D U-Net 16/32/64/128 channels, 3^3 "
+                               "convolutions, BF16 weights/activations, FP32 "
+                               "accumulate")}
     for mname, prec, kry in plan:
         log(f"[bench] {name} {mname}: warmup {args.warmup}, steps {args.steps}")
         results[mname] = run_gpu(prob, prec, kry, args.steps, args.warmup,
-                                 profile=(kry == bmg.KRYLOV_NONE))
+                                 profile=(kry == bmg.KRYLOV_NONE),
+                                 net=net if kry == bmg.KRYLOV_FCG_LEARNED
+                                 else None)
         r = results[mname]
         log(f"[bench] {name} {mname}: {r['ms']:.2f} ms/solve, {r['iters']} "
             f"iterations, final rel. residual {r['final_residual']:.3e}")
@@ -243,6 +275,16 @@ def run_workload(name, args):
         out["speedup_vs_plain_fp64_mg"] = {
             k: round(results["fp64"]["ms"] / v["ms"], 3)
             for k, v in results.items()}
+    if learned is not None and "fcg_learned_mixed" in results:
+        r = results["fcg_learned_mixed"]
+        learned.update({
+            "learned_iterations": r["learned_iters"],
+            "fallback_after_iteration": r["learned_fallback"],
+            "residual_after_guess": r["learned_guess_residual"]})
+        if "fcg_mixed" in results:
+            learned["time_vs_fcg_mixed"] = round(
+                r["ms"] / results["fcg_mixed"]["ms"], 3)
+        out["learned"] = learned
     return out, results
 
 
@@ -260,8 +302,13 @@ def main():
                     help="skip the CPU baseline")
     ap.add_argument("--no-check", action="store_true",
                     help="skip the FP64 reference solution")
-    ap.add_argument("--only", choices=["fp64", "mixed", "fcg_fp64", "fcg_mixed"],
-                    default=None)
+    ap.add_argument("--only", choices=["fp64", "mixed", "fcg_fp64", "fcg_mixed",
+                                       "fcg_learned_mixed"], default=None)
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned preconditioner")
+    ap.add_argument("--train-n", type=int, default=64)
+    ap.add_argument("--train-samples", type=int, default=32)
+    ap.add_argument("--train-epochs", type=int, default=12)
     ap.add_argument("--max-iter", type=int, default=999,
                     help="cap on MG iterations (profiling runs only)")
     args = ap.parse_args()
@@ -293,7 +340,8 @@ def main():
         "higher_is_better": False,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "f64" if best in ("fp64", "fcg_fp64") else "f32+f64",
+        "dtype": ("f64" if best in ("fp64", "fcg_fp64") else
+                  "bf16+f32+f64" if best == "fcg_learned_mixed" else "f32+f64"),
         "data": "synthetic",
         "config": {"workload": main_out["workload"], "method": best},
         "roofline": main_out["roofline"],
@@ -303,6 +351,8 @@ def main():
     }
     if "speedup_vs_plain_fp64_mg" in main_out:
         line["speedup_vs_plain_fp64_mg"] = main_out["speedup_vs_plain_fp64_mg"]
+    if "learned" in main_out:
+        line["learned"] = main_out["learned"]
     if extra is not None:
         line["channel256"] = extra
     print(json.dumps(line), flush=True)
