@@ -30,7 +30,9 @@ EXPORTS = [
     "bmg_op_prepare", "bmg_op_smooth", "bmg_op_residual", "bmg_op_restrict",
     "bmg_op_prolong", "bmg_op_correct_bcs", "bmg_op_coarse_solve",
     "bmg_op_cycle", "bmg_op_sum_sq_residual", "bmg_time_smoother",
-    "bmg_set_graphs", "bmg_solve_diagonal",
+    "bmg_set_graphs", "bmg_solve_diagonal", "bmg_batch_create",
+    "bmg_batch_destroy", "bmg_solve_batch", "bmg_batch_info",
+    "bmg_batch_set_autotune",
 ]
 
 
@@ -112,6 +114,12 @@ def load():
         "bmg_time_smoother": (I, [P, I, I, I, DP, DP]),
         "bmg_set_graphs": (I, [P, I]),
         "bmg_solve_diagonal": (I, [P, DP, DP, DP]),
+        "bmg_batch_create": (I, [ctypes.POINTER(P), ctypes.POINTER(P), I]),
+        "bmg_batch_destroy": (I, [P]),
+        "bmg_solve_batch": (I, [P, ctypes.POINTER(P), ctypes.POINTER(P), I,
+                                ctypes.POINTER(Stats)]),
+        "bmg_batch_info": (I, [P, IP, ctypes.POINTER(ctypes.c_long)]),
+        "bmg_batch_set_autotune": (I, [P, I]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -285,3 +293,53 @@ class Context:
         out = np.zeros(1)
         _check(self.lib.bmg_op_sum_sq_residual(self.ptr, _dp(out)))
         return float(out[0])
+
+
+class Batch:
+    """Independent systems solved together (``bmg_solve_batch``): every
+    kernel of an iteration is one launch over all of them.  ``contexts``
+    are :class:`Context` objects with their operators set; they must outlive
+    the batch."""
+
+    def __init__(self, contexts, autotune=True):
+        self.lib = load()
+        self.contexts = list(contexts)
+        arr = (ctypes.c_void_p * len(self.contexts))(
+            *[c.ptr.value for c in self.contexts])
+        p = ctypes.c_void_p()
+        _check(self.lib.bmg_batch_create(ctypes.byref(p), arr,
+                                         len(self.contexts)))
+        self.ptr = p
+        if not autotune:
+            _check(self.lib.bmg_batch_set_autotune(self.ptr, 0))
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            self.lib.bmg_batch_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve_device(self, b_ptrs, x_ptrs, const_matrix=False):
+        """Device pointers (reference layout) per system; returns the list
+        of per-system :class:`Stats`."""
+        n = len(self.contexts)
+        assert len(b_ptrs) == n and len(x_ptrs) == n
+        bp = (ctypes.c_void_p * n)(*b_ptrs)
+        xp = (ctypes.c_void_p * n)(*x_ptrs)
+        st = (Stats * n)()
+        _check(self.lib.bmg_solve_batch(self.ptr, bp, xp, int(const_matrix),
+                                        st))
+        return list(st)
+
+    def info(self):
+        """(launches, virtual workgroups) of one merged iteration."""
+        n = ctypes.c_int(0)
+        w = ctypes.c_long(0)
+        _check(self.lib.bmg_batch_info(self.ptr, ctypes.byref(n),
+                                       ctypes.byref(w)))
+        return n.value, w.value

@@ -10,8 +10,8 @@ import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SOURCES = ["csrc/bmg_kernels.hip", "csrc/bmg_capi.hip", "csrc/unet_kernels.hip",
-           "csrc/unet.hip"]
+SOURCES = ["csrc/bmg_kernels.hip", "csrc/bmg_capi.hip", "csrc/bmg_batch.hip",
+           "csrc/unet_kernels.hip", "csrc/unet.hip"]
 LIB = os.path.join(HERE, "libbmg.so")
 ARCH = os.environ.get("BMG_OFFLOAD_ARCH", "gfx950")
 

@@ -6,6 +6,7 @@
 #include "bmg_unet.h"
 #include "unet_internal.h"
 #include "bmg_internal.h"
+#include "bmg_context.h"
 
 #include <hip/hip_runtime.h>
 
@@ -26,20 +27,6 @@ namespace {
 
 thread_local std::string g_error = "";
 
-struct Error : std::runtime_error
-{
-    int code;
-    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-
-void hipCheck(hipError_t e, const char* what, int line)
-{
-    if (e != hipSuccess)
-        throw Error(BMG_ERR_HIP, std::string(what) + " (line " +
-                                     std::to_string(line) + "): " +
-                                     hipGetErrorString(e));
-}
-#define HIPCK(x) hipCheck((x), #x, __LINE__)
 
 // Levels with more cells than this keep a matrix-free / face-coefficient
 // operator; smaller ones (and the pinned coarsest level) store the stencil.
@@ -70,295 +57,8 @@ double hexaCentre(double lo, double hi, int axis)
     return c;
 }
 
-struct HostAxis
-{
-    int N = 0, R = 0;
-    std::vector<double> v, cc, w, dl;
-};
-
-template <typename T>
-T* devAlloc(size_t n)
-{
-    if (n == 0) return nullptr;
-    void* p = nullptr;
-    HIPCK(hipMalloc(&p, n*sizeof(T)));
-    return static_cast<T*>(p);
-}
-
-template <typename T>
-void devFree(T*& p)
-{
-    if (p) (void)hipFree(p);
-    p = nullptr;
-}
-
-template <typename T>
-T* devUpload(const std::vector<T>& h)
-{
-    T* d = devAlloc<T>(h.size());
-    HIPCK(hipMemcpy(d, h.data(), h.size()*sizeof(T), hipMemcpyHostToDevice));
-    return d;
-}
-
-struct Level
-{
-    LevelDims L{};
-    HostAxis ax[3];
-    int opkind = OP_STENCIL;
-    bool haveOp = false;
-    // FP64 fields
-    double* x[2] = {nullptr, nullptr};
-    double* b[2] = {nullptr, nullptr};
-    double* r[2] = {nullptr, nullptr};
-    double* tmp[2] = {nullptr, nullptr};
-    // FP32 fields (mixed precision)
-    float* xf[2] = {nullptr, nullptr};
-    float* bf[2] = {nullptr, nullptr};
-    float* rf[2] = {nullptr, nullptr};
-    float* tmpf[2] = {nullptr, nullptr};
-    // operator
-    double* a7[2] = {nullptr, nullptr};
-    float* a7f[2] = {nullptr, nullptr};
-    double* wf[2] = {nullptr, nullptr};
-    float* wff[2] = {nullptr, nullptr};
-    // metrics and transfer data
-    double* dw[3] = {nullptr, nullptr, nullptr};
-    double* ddl[3] = {nullptr, nullptr, nullptr};
-    float* fw[3] = {nullptr, nullptr, nullptr};
-    float* fdl[3] = {nullptr, nullptr, nullptr};
-    double* ru[3] = {nullptr, nullptr, nullptr};
-    double* pu[3] = {nullptr, nullptr, nullptr};
-    float* ruf[3] = {nullptr, nullptr, nullptr};
-    float* puf[3] = {nullptr, nullptr, nullptr};
-    // snapshots of the slab planes 1 and N_d of every periodic axis d
-    // (P_a1 x P_a2 values each), read by the slab smoother
-    double* snap[3][2] = {{nullptr, nullptr}, {nullptr, nullptr},
-                          {nullptr, nullptr}};
-    float* snapf[3][2] = {{nullptr, nullptr}, {nullptr, nullptr},
-                          {nullptr, nullptr}};
-
-    size_t planeCells(int d) const
-    {
-        return size_t(d == 0 ? L.P[1] : L.P[0])*size_t(d == 2 ? L.P[1] : L.P[2]);
-    }
-
-    long cells() const { return long(L.N[0])*L.N[1]*L.N[2]; }
-
-    void freeAll()
-    {
-        for (int c = 0; c < 2; c++)
-        {
-            if (rf[c] == bf[c]) rf[c] = nullptr;   // level-0 alias
-            devFree(x[c]); devFree(b[c]); devFree(r[c]); devFree(tmp[c]);
-            devFree(xf[c]); devFree(bf[c]); devFree(rf[c]); devFree(tmpf[c]);
-            devFree(a7[c]); devFree(a7f[c]); devFree(wf[c]); devFree(wff[c]);
-        }
-        for (int d = 0; d < 3; d++)
-        {
-            devFree(dw[d]); devFree(ddl[d]); devFree(fw[d]); devFree(fdl[d]);
-            devFree(ru[d]); devFree(pu[d]); devFree(ruf[d]); devFree(puf[d]);
-            for (int k = 0; k < 2; k++) { devFree(snap[d][k]); devFree(snapf[d][k]); }
-        }
-    }
-};
-
-template <typename T>
-Pair<T> pr(T* const (&a)[2])
-{
-    return Pair<T>{{a[0], a[1]}};
-}
-
-template <typename T>
-Pair<const T> cpr(T* const (&a)[2])
-{
-    return Pair<const T>{{a[0], a[1]}};
-}
-
 }  // namespace
 
-struct bmg_ctx
-{
-    int device = 0;
-    hipStream_t stream = nullptr;
-    int bc[6];
-    double bcval[6];
-    bmg_opts opts;
-    std::vector<Level> lv;
-    bool symmetric = true;
-    bool eliminated = false;
-    bool prepared = false;       // coarse factorisation valid
-    int preparedCoarse = -1;
-    // coarse LU
-    double* dLU = nullptr;
-    int* dPiv = nullptr;
-    int nc = 0;
-    // reductions
-    double* partials = nullptr;
-    size_t npartials = 0;
-    double* dScalar = nullptr;
-    double* hScalar = nullptr;   // pinned
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    bool mixedReady = false;
-    // flexible CG state (level-0 FP64 fields) and the captured preconditioner
-    double* kx[2] = {nullptr, nullptr};
-    double* kb[2] = {nullptr, nullptr};
-    double* kr[2] = {nullptr, nullptr};
-    double* kp[2] = {nullptr, nullptr};
-    double* kq[2] = {nullptr, nullptr};
-    hipGraph_t pgraph = nullptr;
-    hipGraphExec_t pgraphExec = nullptr;
-    int pgraphMixed = -1;
-    long pgraphGen = -1;
-    // one flexible-CG iteration (operator, updates, preconditioner, next
-    // direction) captured as a HIP graph
-    hipGraph_t fgraph = nullptr;
-    hipGraphExec_t fgraphExec = nullptr;
-    int fgraphMode = -1;
-    long fgraphGen = -1;
-    // learned preconditioner: network (not owned), dense FP32 buffers of the
-    // level-0 interior (input, coefficient channel, output) and the host
-    // copy of the coefficient channel of the current operator
-    bmg_unet* unet = nullptr;
-    float* uIn = nullptr;
-    float* uCoef = nullptr;
-    float* uOut = nullptr;
-    std::vector<float> coefChannel;
-    double coefScale = 1.0;   // 10^mean(log10 lambda) of the cells
-    bool coefUploaded = false;
-    // one MG iteration captured as a HIP graph and replayed
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graphExec = nullptr;
-    int graphMixed = -1;
-    long gen = 0;          // bumped whenever device buffers are (re)allocated
-    long graphGen = -1;
-    bool useGraphs = true;
-
-    int nLevels() const { return int(lv.size()); }
-    int coarseLevel() const { return nLevels() - opts.coarse_level - 1; }
-    bool periodic(int f) const { return bc[f] == BMG_PERIODIC; }
-    bool singular() const
-    {
-        if (!symmetric) return false;
-        for (int f = 0; f < 6; f++)
-            if (bc[f] == BMG_DIRICHLET) return false;
-        return true;
-    }
-    bool anyPeriodic() const
-    {
-        for (int f = 0; f < 6; f++)
-            if (periodic(f)) return true;
-        return false;
-    }
-
-    ~bmg_ctx()
-    {
-        for (auto& l : lv) l.freeAll();
-        devFree(dLU);
-        devFree(dPiv);
-        devFree(partials);
-        devFree(dScalar);
-        if (hScalar) (void)hipHostFree(hScalar);
-        if (ev0) (void)hipEventDestroy(ev0);
-        if (ev1) (void)hipEventDestroy(ev1);
-        if (graphExec) (void)hipGraphExecDestroy(graphExec);
-        if (graph) (void)hipGraphDestroy(graph);
-        if (pgraphExec) (void)hipGraphExecDestroy(pgraphExec);
-        if (pgraph) (void)hipGraphDestroy(pgraph);
-        if (fgraphExec) (void)hipGraphExecDestroy(fgraphExec);
-        if (fgraph) (void)hipGraphDestroy(fgraph);
-        devFree(uIn);
-        devFree(uCoef);
-        devFree(uOut);
-        for (int c = 0; c < 2; c++)
-        {
-            devFree(kx[c]); devFree(kb[c]); devFree(kr[c]);
-            devFree(kp[c]); devFree(kq[c]);
-        }
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-
-    // ---------------------------------------------------------------- views
-    template <typename T>
-    OpView<T> op(int l) const;
-
-    template <typename T>
-    TransferView<T> transfer(int lf) const;
-
-    template <typename T>
-    Pair<T> X(int l);
-    template <typename T>
-    Pair<T> B(int l);
-    template <typename T>
-    Pair<T> Rr(int l);
-    template <typename T>
-    Pair<T> Tmp(int l);
-    template <typename T>
-    T* const (*Snap(int l))[2];
-
-    // -------------------------------------------------------------- helpers
-    double reduceFinal(int nblocks)
-    {
-        launch_reduce_final(partials, nblocks, dScalar, stream);
-        HIPCK(hipGetLastError());
-        HIPCK(hipMemcpyAsync(hScalar, dScalar, sizeof(double),
-                             hipMemcpyDeviceToHost, stream));
-        HIPCK(hipStreamSynchronize(stream));
-        return *hScalar;
-    }
-
-    Region interior(int l, int padding) const
-    {
-        Region rg;
-        for (int d = 0; d < 3; d++)
-        {
-            const int padLo = periodic(2*d), padHi = periodic(2*d + 1);
-            rg.S[d] = 1 + padding*padLo;
-            rg.E[d] = lv[l].L.N[d] + 1 - padding*padHi;
-        }
-        return rg;
-    }
-
-    Region slab(int l, int f) const
-    {
-        Region rg;
-        for (int d = 0; d < 3; d++) { rg.S[d] = 1; rg.E[d] = lv[l].L.N[d] + 1; }
-        const int fd = f/2;
-        if (f % 2 == 0) rg.E[fd] = rg.S[fd] + 1;
-        else rg.S[fd] = rg.E[fd] - 1;
-        return rg;
-    }
-
-    template <typename T>
-    void smooth(int l, int sweeps);
-    template <typename T>
-    void residual(int l, bool corrBcs, bool norm, int* nblocks);
-    template <typename T>
-    void restrictTo(int l);
-    template <typename T>
-    void prolongFrom(int l);
-    template <typename T>
-    void coarseSolve();
-    template <typename T>
-    void cycle(std::vector<int>& visits, int l);
-
-    void enqueueIteration(bool mixed);
-    double iterate(bool mixed);
-    void enqueuePrecondition(bool mixed);
-    // FCG preconditioner modes: 0 FP64 cycle, 1 mixed cycle, 2 learned
-    void enqueueFCGIteration(int mode);
-    void fcgIteration(int mode);
-    void enqueueLearned(const double* ssDev, double ssHost, Pair<double> z);
-    void prepareLearned();
-    void precondition(bool mixed);
-    double normFactorOf(Pair<const double> x, Pair<const double> r,
-                        Pair<const double> b);
-    void runFCG(bool constMatrix, bmg_stats* st);
-    void prepare(bool constMatrix);
-    void prepareCoarse();
-    void ensureMixed();
-    double normFactor();
-    void runSolve(bool constMatrix, bmg_stats* st);
-};
 
 template <>
 OpView<double> bmg_ctx::op<double>(int l) const
@@ -756,6 +456,23 @@ void bmg_ctx::prepareCoarse()
             for (int i = j + 1; i < n; i++) D[size_t(i)*n + j] *= rr;
         }
     }
+    // a coarse level that fits one wavefront is solved from padded
+    // column-major factors and the final row permutation (the pivot swaps
+    // applied in order to the identity)
+    const int w = coarse_wave_width(n);
+    if (w)
+    {
+        std::vector<double> F(size_t(w)*w, 0.0);
+        for (int i = 0; i < w; i++)
+            for (int j = 0; j < w; j++)
+                F[size_t(j)*w + i] = i < n && j < n ? D[size_t(i)*n + j]
+                                                    : double(i == j);
+        std::vector<int> perm(n);
+        for (int i = 0; i < n; i++) perm[i] = i;
+        for (int i = 0; i < n; i++) std::swap(perm[piv[i]], perm[i]);
+        D.swap(F);
+        piv.swap(perm);
+    }
     if (dLU && nc == n)
     {
         HIPCK(hipMemcpy(dLU, D.data(), D.size()*sizeof(double),
@@ -916,9 +633,7 @@ void bmg_ctx::enqueueIteration(bool mixed)
     }
     int nb = 0;
     residual<double>(0, true, true, &nb);
-    launch_reduce_final(partials, nb, dScalar, stream);
-    HIPCK(hipMemcpyAsync(hScalar, dScalar, sizeof(double),
-                         hipMemcpyDeviceToHost, stream));
+    launch_reduce_final(partials, nb, dScalar, stream, hScalarDev);
 }
 
 // Runs one iteration and returns the sum of squared level-0 residuals.
@@ -993,14 +708,14 @@ void bmg_ctx::enqueueFCGIteration(int mode)
         launch_bc_fill(q0.L, o, Pv, 0, 1, bcval, stream);
     const int nb = launch_apply_dots(q0.L, o, cst(Pv), Qv, cst(Rv), partials,
                                      stream);
-    launch_reduce_rows(partials, nb, 2, dScalar, stream);
+    launch_reduce_rows(partials, nb, 2, dScalar, stream, hScalarDev);
     const Pair<float> rf = mode == 1 ? pr(q0.bf)
                                      : Pair<float>{{nullptr, nullptr}};
     const Pair<double> r0 = mode == 0 ? pr(q0.r)
                                       : Pair<double>{{nullptr, nullptr}};
     launch_fcg_update(q0.L, dScalar, X, Rv, cst(Pv), cst(Qv), rf, r0,
                       pr(q0.b), partials, stream);
-    launch_reduce_rows(partials, nb, 1, dScalar + 2, stream);
+    launch_reduce_rows(partials, nb, 1, dScalar + 2, stream, hScalarDev + 2);
     if (mode == 2)
         enqueueLearned(dScalar + 2, 0.0, pr(q0.x));
     else
@@ -1008,18 +723,18 @@ void bmg_ctx::enqueueFCGIteration(int mode)
     if (mode == 1)
     {
         launch_dot_zq<float>(q0.L, cpr(q0.xf), cst(Qv), partials, stream);
-        launch_reduce_rows(partials, nb, 1, dScalar + 3, stream);
+        launch_reduce_rows(partials, nb, 1, dScalar + 3, stream,
+                           hScalarDev + 3);
         launch_update_p<float>(q0.L, dScalar, cpr(q0.xf), Pv, stream);
     }
     else
     {
         launch_dot_zq<double>(q0.L, cpr(q0.x), cst(Qv), partials, stream);
-        launch_reduce_rows(partials, nb, 1, dScalar + 3, stream);
+        launch_reduce_rows(partials, nb, 1, dScalar + 3, stream,
+                           hScalarDev + 3);
         launch_update_p<double>(q0.L, dScalar, cpr(q0.x), Pv, stream);
     }
     HIPCK(hipGetLastError());
-    HIPCK(hipMemcpyAsync(hScalar, dScalar, 4*sizeof(double),
-                         hipMemcpyDeviceToHost, stream));
 }
 
 // z = U(r / rms(r)) * rms(r) / h on the level-0 interior (the learned
@@ -1324,20 +1039,47 @@ double bmg_ctx::normFactorOf(Pair<const double> x, Pair<const double> r,
     return std::max(f, n*1e-12);
 }
 
-// solver::normFactors (solver.C:13-61)
-double bmg_ctx::normFactor()
+// Whether prepare() does more than device work this solve: the first
+// solve (materialising, eliminating) or a new coarse factorisation.
+bool bmg_ctx::prepareNeedsHost(bool constMatrix) const
 {
-    Level& q = lv[0];
-    const double n = double(q.cells());
+    for (auto& q : lv)
+        if (!q.haveOp) return true;
+    const int lc = coarseLevel();
+    if (lc < 1 && nLevels() > 1) return true;
+    if (lv[lc].opkind != OP_STENCIL || !eliminated) return true;
+    return opts.coarse_mode == BMG_COARSE_DIRECT &&
+           (!constMatrix || !prepared || preparedCoarse != lc);
+}
+
+// MG.C:166-172: r = b - A x without BC correction and its squared norm
+// (host slot 0), and the sum of x for gAverage (slot 1, solver.C:24).
+void bmg_ctx::enqueueSolveStart()
+{
+    Level& q0 = lv[0];
+    launch_fill<double>(pr(q0.r), q0.L.ncol, 0.0, stream);
     int nb = 0;
-    launch_sum<double>(q.L, cpr(q.x), partials, &nb, stream);
-    const double sum = reduceFinal(nb);
-    const double y = sum/n;
-    launch_norm_factor_terms<double>(q.L, op<double>(0), cpr(q.r), cpr(q.b), y,
-                                     partials, &nb, stream);
-    double f = reduceFinal(nb);
-    f = std::sqrt(f);
-    return std::max(f, n*1e-12);
+    launch_residual(q0.L, op<double>(0), cpr(q0.x), cpr(q0.b), pr(q0.r),
+                    partials, &nb, stream);
+    launch_reduce_final(partials, nb, dScalar, stream, hScalarDev);
+    launch_sum<double>(q0.L, cpr(q0.x), partials, &nb, stream);
+    launch_reduce_final(partials, nb, dScalar + 1, stream, hScalarDev + 1);
+}
+
+// normFactors terms with y = gAverage(x) (solver.C:35-46), host slot 2.
+void bmg_ctx::enqueueNormTerms(double y)
+{
+    Level& q0 = lv[0];
+    int nb = 0;
+    launch_norm_factor_terms<double>(q0.L, op<double>(0), cpr(q0.r),
+                                     cpr(q0.b), y, partials, &nb, stream);
+    launch_reduce_final(partials, nb, dScalar + 2, stream, hScalarDev + 2);
+}
+
+// x[0].correct<eliminatedBcs>() (MG.C:475)
+void bmg_ctx::enqueueSolveEnd()
+{
+    launch_bc_fill(lv[0].L, op<double>(0), pr(lv[0].x), 2, 1, bcval, stream);
 }
 
 // MG::Solve (MG.C:132-254) around the cycles; FP64 or mixed precision.
@@ -1349,16 +1091,18 @@ void bmg_ctx::runSolve(bool constMatrix, bmg_stats* st)
         return;
     }
     prepare(constMatrix);
-    Level& q0 = lv[0];
     const bool mixed = opts.precision == BMG_MIXED;
     if (mixed) ensureMixed();
 
-    launch_fill<double>(pr(q0.r), q0.L.ncol, 0.0, stream);
-    int nb = 0;
-    launch_residual(q0.L, op<double>(0), cpr(q0.x), cpr(q0.b), pr(q0.r),
-                    partials, &nb, stream);
-    const double ss0 = reduceFinal(nb);
-    const double nf = normFactor();
+    const double cells = double(lv[0].cells());
+    enqueueSolveStart();
+    HIPCK(hipGetLastError());
+    HIPCK(hipStreamSynchronize(stream));
+    const double ss0 = hScalar[0];
+    enqueueNormTerms(hScalar[1]/cells);
+    HIPCK(hipGetLastError());
+    HIPCK(hipStreamSynchronize(stream));
+    const double nf = std::max(std::sqrt(hScalar[2]), cells*1e-12);
     const double init = std::sqrt(ss0)/nf;
     if (!std::isfinite(init))
         throw Error(BMG_ERR_NUMERIC, "initial residual is not finite");
@@ -1380,8 +1124,7 @@ void bmg_ctx::runSolve(bool constMatrix, bmg_stats* st)
         conv = check(cur);
         iter++;
     }
-    // x[0].correct<eliminatedBcs>() (MG.C:475)
-    launch_bc_fill(q0.L, op<double>(0), pr(q0.x), 2, 1, bcval, stream);
+    enqueueSolveEnd();
     HIPCK(hipGetLastError());
     if (st)
     {
@@ -1644,7 +1387,9 @@ int bmg_create(bmg_ctx** out, const bmg_grid* g, const bmg_opts* opts)
         c->npartials = 3*maxBlocks;
         c->partials = devAlloc<double>(3*maxBlocks);
         c->dScalar = devAlloc<double>(4);
-        HIPCK(hipHostMalloc(&c->hScalar, 4*sizeof(double)));
+        HIPCK(hipHostMalloc(&c->hScalar, 4*sizeof(double),
+                            hipHostMallocMapped));
+        HIPCK(hipHostGetDevicePointer((void**)&c->hScalarDev, c->hScalar, 0));
         *out = c.release();
     });
 }
@@ -1953,6 +1698,32 @@ static void solveImpl(bmg_ctx* c, double* b, double* x, bool device,
 int bmg_solve(bmg_ctx* c, double* b, double* x, int constMatrix, bmg_stats* st)
 {
     return run([&] { solveImpl(c, b, x, false, constMatrix, st); });
+}
+
+int bmg_batch_create(bmg_batch** out, bmg_ctx* const* ctxs, int n)
+{
+    return run([&] { batch_create(out, ctxs, n); });
+}
+
+int bmg_batch_destroy(bmg_batch* b)
+{
+    return run([&] { batch_destroy(b); });
+}
+
+int bmg_solve_batch(bmg_batch* b, double* const* db, double* const* dx,
+                    int constMatrix, bmg_stats* st)
+{
+    return run([&] { batch_solve(b, db, dx, constMatrix != 0, st); });
+}
+
+int bmg_batch_info(const bmg_batch* b, int* launches, long* workgroups)
+{
+    return run([&] { batch_info(b, launches, workgroups); });
+}
+
+int bmg_batch_set_autotune(bmg_batch* b, int enable)
+{
+    return run([&] { batch_set_autotune(b, enable); });
 }
 
 int bmg_solve_device(bmg_ctx* c, double* db, double* dx, int constMatrix,

@@ -1,0 +1,367 @@
+// Host-side solver context shared by the C ABI (bmg_capi.hip) and the
+// batched solver (bmg_batch.hip): the level hierarchy with its device
+// fields, the coarse factorisation, reduction buffers, captured graphs and
+// the MG control flow that enqueues kernels on the context's stream.
+#pragma once
+
+#include "bmg.h"
+#include "bmg_unet.h"
+#include "bmg_internal.h"
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace bmg {
+
+struct Error : std::runtime_error
+{
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+inline void hipCheck(hipError_t e, const char* what, int line)
+{
+    if (e != hipSuccess)
+        throw Error(BMG_ERR_HIP, std::string(what) + " (line " +
+                                     std::to_string(line) + "): " +
+                                     hipGetErrorString(e));
+}
+#define HIPCK(x) ::bmg::hipCheck((x), #x, __LINE__)
+
+struct HostAxis
+{
+    int N = 0, R = 0;
+    std::vector<double> v, cc, w, dl;
+};
+
+template <typename T>
+T* devAlloc(size_t n)
+{
+    if (n == 0) return nullptr;
+    void* p = nullptr;
+    HIPCK(hipMalloc(&p, n*sizeof(T)));
+    return static_cast<T*>(p);
+}
+
+template <typename T>
+void devFree(T*& p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+template <typename T>
+T* devUpload(const std::vector<T>& h)
+{
+    T* d = devAlloc<T>(h.size());
+    HIPCK(hipMemcpy(d, h.data(), h.size()*sizeof(T), hipMemcpyHostToDevice));
+    return d;
+}
+
+struct Level
+{
+    LevelDims L{};
+    HostAxis ax[3];
+    int opkind = OP_STENCIL;
+    bool haveOp = false;
+    // FP64 fields
+    double* x[2] = {nullptr, nullptr};
+    double* b[2] = {nullptr, nullptr};
+    double* r[2] = {nullptr, nullptr};
+    double* tmp[2] = {nullptr, nullptr};
+    // FP32 fields (mixed precision)
+    float* xf[2] = {nullptr, nullptr};
+    float* bf[2] = {nullptr, nullptr};
+    float* rf[2] = {nullptr, nullptr};
+    float* tmpf[2] = {nullptr, nullptr};
+    // operator
+    double* a7[2] = {nullptr, nullptr};
+    float* a7f[2] = {nullptr, nullptr};
+    double* wf[2] = {nullptr, nullptr};
+    float* wff[2] = {nullptr, nullptr};
+    // metrics and transfer data
+    double* dw[3] = {nullptr, nullptr, nullptr};
+    double* ddl[3] = {nullptr, nullptr, nullptr};
+    float* fw[3] = {nullptr, nullptr, nullptr};
+    float* fdl[3] = {nullptr, nullptr, nullptr};
+    double* ru[3] = {nullptr, nullptr, nullptr};
+    double* pu[3] = {nullptr, nullptr, nullptr};
+    float* ruf[3] = {nullptr, nullptr, nullptr};
+    float* puf[3] = {nullptr, nullptr, nullptr};
+    // snapshots of the slab planes 1 and N_d of every periodic axis d
+    // (P_a1 x P_a2 values each), read by the slab smoother
+    double* snap[3][2] = {{nullptr, nullptr}, {nullptr, nullptr},
+                          {nullptr, nullptr}};
+    float* snapf[3][2] = {{nullptr, nullptr}, {nullptr, nullptr},
+                          {nullptr, nullptr}};
+
+    size_t planeCells(int d) const
+    {
+        return size_t(d == 0 ? L.P[1] : L.P[0])*size_t(d == 2 ? L.P[1] : L.P[2]);
+    }
+
+    long cells() const { return long(L.N[0])*L.N[1]*L.N[2]; }
+
+    void freeAll()
+    {
+        for (int c = 0; c < 2; c++)
+        {
+            if (rf[c] == bf[c]) rf[c] = nullptr;   // level-0 alias
+            devFree(x[c]); devFree(b[c]); devFree(r[c]); devFree(tmp[c]);
+            devFree(xf[c]); devFree(bf[c]); devFree(rf[c]); devFree(tmpf[c]);
+            devFree(a7[c]); devFree(a7f[c]); devFree(wf[c]); devFree(wff[c]);
+        }
+        for (int d = 0; d < 3; d++)
+        {
+            devFree(dw[d]); devFree(ddl[d]); devFree(fw[d]); devFree(fdl[d]);
+            devFree(ru[d]); devFree(pu[d]); devFree(ruf[d]); devFree(puf[d]);
+            for (int k = 0; k < 2; k++) { devFree(snap[d][k]); devFree(snapf[d][k]); }
+        }
+    }
+};
+
+template <typename T>
+inline Pair<T> pr(T* const (&a)[2])
+{
+    return Pair<T>{{a[0], a[1]}};
+}
+
+template <typename T>
+inline Pair<const T> cpr(T* const (&a)[2])
+{
+    return Pair<const T>{{a[0], a[1]}};
+}
+
+
+}  // namespace bmg
+
+// The context is the opaque bmg_ctx of the C ABI, so it lives in the global
+// namespace; its members use the solver's types.
+using bmg::devAlloc;
+using bmg::devFree;
+using bmg::Level;
+using bmg::LevelDims;
+using bmg::OpView;
+using bmg::Pair;
+using bmg::Region;
+using bmg::TransferView;
+
+struct bmg_ctx
+{
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int bc[6];
+    double bcval[6];
+    bmg_opts opts;
+    std::vector<Level> lv;
+    bool symmetric = true;
+    bool eliminated = false;
+    bool prepared = false;       // coarse factorisation valid
+    int preparedCoarse = -1;
+    // coarse LU
+    double* dLU = nullptr;
+    int* dPiv = nullptr;
+    int nc = 0;
+    // reductions
+    double* partials = nullptr;
+    size_t npartials = 0;
+    double* dScalar = nullptr;
+    double* hScalar = nullptr;   // pinned, mapped into the device
+    double* hScalarDev = nullptr;   // device address of hScalar
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool mixedReady = false;
+    // flexible CG state (level-0 FP64 fields) and the captured preconditioner
+    double* kx[2] = {nullptr, nullptr};
+    double* kb[2] = {nullptr, nullptr};
+    double* kr[2] = {nullptr, nullptr};
+    double* kp[2] = {nullptr, nullptr};
+    double* kq[2] = {nullptr, nullptr};
+    hipGraph_t pgraph = nullptr;
+    hipGraphExec_t pgraphExec = nullptr;
+    int pgraphMixed = -1;
+    long pgraphGen = -1;
+    // one flexible-CG iteration (operator, updates, preconditioner, next
+    // direction) captured as a HIP graph
+    hipGraph_t fgraph = nullptr;
+    hipGraphExec_t fgraphExec = nullptr;
+    int fgraphMode = -1;
+    long fgraphGen = -1;
+    // learned preconditioner: network (not owned), dense FP32 buffers of the
+    // level-0 interior (input, coefficient channel, output) and the host
+    // copy of the coefficient channel of the current operator
+    bmg_unet* unet = nullptr;
+    float* uIn = nullptr;
+    float* uCoef = nullptr;
+    float* uOut = nullptr;
+    std::vector<float> coefChannel;
+    double coefScale = 1.0;   // 10^mean(log10 lambda) of the cells
+    bool coefUploaded = false;
+    // one MG iteration captured as a HIP graph and replayed
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graphExec = nullptr;
+    int graphMixed = -1;
+    long gen = 0;          // bumped whenever device buffers are (re)allocated
+    long graphGen = -1;
+    bool useGraphs = true;
+
+    int nLevels() const { return int(lv.size()); }
+    int coarseLevel() const { return nLevels() - opts.coarse_level - 1; }
+    bool periodic(int f) const { return bc[f] == BMG_PERIODIC; }
+    bool singular() const
+    {
+        if (!symmetric) return false;
+        for (int f = 0; f < 6; f++)
+            if (bc[f] == BMG_DIRICHLET) return false;
+        return true;
+    }
+    bool anyPeriodic() const
+    {
+        for (int f = 0; f < 6; f++)
+            if (periodic(f)) return true;
+        return false;
+    }
+
+    ~bmg_ctx()
+    {
+        for (auto& l : lv) l.freeAll();
+        devFree(dLU);
+        devFree(dPiv);
+        devFree(partials);
+        devFree(dScalar);
+        if (hScalar) (void)hipHostFree(hScalar);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (graphExec) (void)hipGraphExecDestroy(graphExec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (pgraphExec) (void)hipGraphExecDestroy(pgraphExec);
+        if (pgraph) (void)hipGraphDestroy(pgraph);
+        if (fgraphExec) (void)hipGraphExecDestroy(fgraphExec);
+        if (fgraph) (void)hipGraphDestroy(fgraph);
+        devFree(uIn);
+        devFree(uCoef);
+        devFree(uOut);
+        for (int c = 0; c < 2; c++)
+        {
+            devFree(kx[c]); devFree(kb[c]); devFree(kr[c]);
+            devFree(kp[c]); devFree(kq[c]);
+        }
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    // ---------------------------------------------------------------- views
+    template <typename T>
+    OpView<T> op(int l) const;
+
+    template <typename T>
+    TransferView<T> transfer(int lf) const;
+
+    template <typename T>
+    Pair<T> X(int l);
+    template <typename T>
+    Pair<T> B(int l);
+    template <typename T>
+    Pair<T> Rr(int l);
+    template <typename T>
+    Pair<T> Tmp(int l);
+    template <typename T>
+    T* const (*Snap(int l))[2];
+
+    // -------------------------------------------------------------- helpers
+    double reduceFinal(int nblocks)
+    {
+        bmg::launch_reduce_final(partials, nblocks, dScalar, stream,
+                                 hScalarDev);
+        HIPCK(hipGetLastError());
+        HIPCK(hipStreamSynchronize(stream));
+        return *hScalar;
+    }
+
+    Region interior(int l, int padding) const
+    {
+        Region rg;
+        for (int d = 0; d < 3; d++)
+        {
+            const int padLo = periodic(2*d), padHi = periodic(2*d + 1);
+            rg.S[d] = 1 + padding*padLo;
+            rg.E[d] = lv[l].L.N[d] + 1 - padding*padHi;
+        }
+        return rg;
+    }
+
+    Region slab(int l, int f) const
+    {
+        Region rg;
+        for (int d = 0; d < 3; d++) { rg.S[d] = 1; rg.E[d] = lv[l].L.N[d] + 1; }
+        const int fd = f/2;
+        if (f % 2 == 0) rg.E[fd] = rg.S[fd] + 1;
+        else rg.S[fd] = rg.E[fd] - 1;
+        return rg;
+    }
+
+    template <typename T>
+    void smooth(int l, int sweeps);
+    template <typename T>
+    void residual(int l, bool corrBcs, bool norm, int* nblocks);
+    template <typename T>
+    void restrictTo(int l);
+    template <typename T>
+    void prolongFrom(int l);
+    template <typename T>
+    void coarseSolve();
+    template <typename T>
+    void cycle(std::vector<int>& visits, int l);
+
+    void enqueueIteration(bool mixed);
+    double iterate(bool mixed);
+    void enqueuePrecondition(bool mixed);
+    // FCG preconditioner modes: 0 FP64 cycle, 1 mixed cycle, 2 learned
+    void enqueueFCGIteration(int mode);
+    void fcgIteration(int mode);
+    void enqueueLearned(const double* ssDev, double ssHost, Pair<double> z);
+    void prepareLearned();
+    void precondition(bool mixed);
+    double normFactorOf(Pair<const double> x, Pair<const double> r,
+                        Pair<const double> b);
+    void runFCG(bool constMatrix, bmg_stats* st);
+    void prepare(bool constMatrix);
+    void prepareCoarse();
+    void ensureMixed();
+    void runSolve(bool constMatrix, bmg_stats* st);
+    // runSolve in phases, each enqueued without synchronisation (and so
+    // recordable for batched solves)
+    bool prepareNeedsHost(bool constMatrix) const;
+    void enqueueSolveStart();
+    void enqueueNormTerms(double y);
+    void enqueueSolveEnd();
+};
+
+template <> OpView<double> bmg_ctx::op<double>(int l) const;
+template <> OpView<float> bmg_ctx::op<float>(int l) const;
+template <> TransferView<double> bmg_ctx::transfer<double>(int lf) const;
+template <> TransferView<float> bmg_ctx::transfer<float>(int lf) const;
+template <> Pair<double> bmg_ctx::X<double>(int l);
+template <> Pair<double> bmg_ctx::B<double>(int l);
+template <> Pair<double> bmg_ctx::Rr<double>(int l);
+template <> Pair<double> bmg_ctx::Tmp<double>(int l);
+template <> Pair<float> bmg_ctx::X<float>(int l);
+template <> Pair<float> bmg_ctx::B<float>(int l);
+template <> Pair<float> bmg_ctx::Rr<float>(int l);
+template <> Pair<float> bmg_ctx::Tmp<float>(int l);
+
+struct bmg_batch;
+
+namespace bmg {
+
+// batched solves (bmg_batch.hip); throw Error on failure
+void batch_create(bmg_batch** out, bmg_ctx* const* ctxs, int n);
+void batch_solve(bmg_batch* b, double* const* db, double* const* dx,
+                 bool constMatrix, bmg_stats* st);
+void batch_info(const bmg_batch* b, int* launches, long* workgroups);
+void batch_set_autotune(bmg_batch* b, int on);
+void batch_destroy(bmg_batch* b);
+
+}  // namespace bmg

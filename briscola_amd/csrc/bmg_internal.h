@@ -11,6 +11,7 @@
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <vector>
 
 namespace bmg {
 
@@ -61,6 +62,41 @@ struct Region {
     int S[3];
     int E[3];
 };
+
+// ---- batched execution (bmg_kernels.hip, bmg_batch.hip) ----
+// One system's share of a batched launch: virtual workgroups
+// [start, start + n) run the system's launch with grid (gx, gy, n/(gx*gy)).
+struct BatchItem {
+    unsigned start;
+    unsigned gx, gy, n;
+    int sys;          // index into the active flags
+};
+// A batched launch of one kernel over many systems.
+struct BatchLaunch {
+    const void* args;         // device array of nitems argument structs
+    const BatchItem* items;   // device array, ascending starts
+    int nitems;
+    unsigned total;           // virtual workgroups
+    const int* active;        // per-system flags (0: skip), or null
+    unsigned grid;            // workgroups launched (grid-stride loop)
+    size_t shm;
+};
+// A launch recorded instead of run: the kernel (kind), its grid, dynamic
+// LDS, argument struct, and the function that launches many of them.
+struct LaunchRecord {
+    const void* kind;
+    unsigned grid[3];
+    size_t shm;
+    size_t align;
+    std::vector<unsigned char> args;
+    void (*many)(const BatchLaunch&, hipStream_t);
+};
+struct LaunchRecorder {
+    std::vector<LaunchRecord> recs;
+};
+// The calling thread's recorder: while it is set, the batchable launchers
+// append records instead of launching and the others throw.
+LaunchRecorder*& launch_recorder();
 
 // ---- launchers (bmg_kernels.hip) ----
 // number of per-tile partial sums a cell-pair reduction over level L writes
@@ -146,6 +182,19 @@ void launch_face_coeffs(const LevelDims& L, const double* w0,
                         const double* w1, const double* w2, const double* dl,
                         const double* lam, int fd, Pair<double> wf,
                         hipStream_t s);
+// Width of the one-wavefront coarse solve for an n-cell coarse level (a
+// power of two >= n, at least 8), or 0 when n exceeds one wavefront.  With a
+// width N the factors are stored column-major in an N x N array padded with
+// unit rows (lu[j*N + i] = LU(i, j)) and piv holds the row permutation
+// (row i of the permuted right-hand side is row piv[i] of the original);
+// otherwise lu is the n x n row-major LU and piv the pivot sequence.
+inline int coarse_wave_width(int n)
+{
+    if (n > 64) return 0;
+    int w = 8;
+    while (w < n) w *= 2;
+    return w;
+}
 template <typename T>
 void launch_coarse_solve(const LevelDims& L, const OpView<T>& op, Pair<T> x,
                          Pair<const T> b, const double* lu, const int* piv,
@@ -161,8 +210,10 @@ template <typename T>
 void launch_norm_factor_terms(const LevelDims& L, const OpView<T>& op,
                               Pair<const T> r, Pair<const T> b, double y,
                               double* partials, int* nblocks, hipStream_t s);
+// out[0] = sum of n partials; also written to hostOut (host-mapped pinned
+// memory, read by the host after the stream synchronises) when given
 void launch_reduce_final(const double* partials, int n, double* out,
-                         hipStream_t s);
+                         hipStream_t s, double* hostOut = nullptr);
 void launch_diag(const LevelDims& L, Pair<const double> a7, Pair<double> x,
                  Pair<const double> b, hipStream_t s);
 // Flexible CG with device-resident scalars sc[0..3] = p.q, p.r, r.r, z.q.
@@ -198,7 +249,7 @@ void launch_add_interior(const LevelDims& L, Pair<double> x,
                          Pair<const double> z, hipStream_t s);
 // out[row] = sum of partials[row*n .. row*n + n) for row < nrows
 void launch_reduce_rows(const double* partials, int n, int nrows, double* out,
-                        hipStream_t s);
+                        hipStream_t s, double* hostOut = nullptr);
 template <typename T>
 void launch_copy(long n, Pair<const T> a, Pair<T> b, hipStream_t s);
 void launch_to_double(long n, Pair<const float> a, Pair<double> b,

@@ -8,6 +8,9 @@
 #include "bmg_internal.h"
 
 #include <algorithm>
+#include <cstring>
+#include <stdexcept>
+#include <string>
 
 namespace bmg {
 
@@ -157,6 +160,12 @@ __device__ __forceinline__ double block_sum(double v, double* sm)
     return s;
 }
 
+// Workgroup index of a kernel body: blockIdx for a single launch, the
+// system-local index in a batched launch.
+struct Bid {
+    unsigned x, y, z;
+};
+
 // A logical 3-D grid of tiles launched as a 1-D grid.  The device hands
 // workgroups to its eight XCDs in turn (workgroup w to XCD w mod 8); tile
 // (w mod 8)*per + w/8 gives every XCD one contiguous run of tiles, so the
@@ -167,10 +176,9 @@ struct Tiles {
     int per;   // tiles per XCD
 };
 
-__device__ __forceinline__ bool tile_of(const Tiles& t, int& bx, int& by,
-                                        int& bz)
+__device__ __forceinline__ bool tile_of(const Tiles& t, unsigned w, int& bx,
+                                        int& by, int& bz)
 {
-    const unsigned w = blockIdx.x;
     const unsigned lin = (w & 7u)*unsigned(t.per) + (w >> 3);
     if (lin >= unsigned(t.gx*t.gy*t.gz)) return false;
     const unsigned r = lin/unsigned(t.gx);
@@ -262,15 +270,16 @@ __device__ __forceinline__ void pair_Ax(const LevelDims& L,
 // tiles of BX positions x BY rows, one tile row per plane, dealt to the
 // XCDs in contiguous runs of planes.
 template <typename T, int OP>
-__global__ __launch_bounds__(BX*BY) void k_rbgs(LevelDims L, OpView<T> op,
-                                                T* __restrict__ xc,
-                                                const T* __restrict__ xo,
-                                                const T* __restrict__ bcl,
-                                                int colour, Region rg,
-                                                Tiles tl)
+__device__ __forceinline__ void rbgs_body(const Bid bid, const LevelDims& L,
+                                          const OpView<T>& op,
+                                          T* __restrict__ xc,
+                                          const T* __restrict__ xo,
+                                          const T* __restrict__ bcl,
+                                          int colour, const Region& rg,
+                                          const Tiles& tl)
 {
     int bx, by, bz;
-    if (!tile_of(tl, bx, by, bz)) return;
+    if (!tile_of(tl, bid.x, bx, by, bz)) return;
     const int I = rg.S[0] + bz;
     // a wavefront is one row (BX = 64), so J and everything derived from it
     // is uniform and lives in scalar registers
@@ -300,6 +309,47 @@ __global__ __launch_bounds__(BX*BY) void k_rbgs(LevelDims L, OpView<T> op,
       + a[6]*nb[5];
     st(xc, idx, (bv - P)/a[0]);
 }
+
+template <typename T, int OP>
+__global__ __launch_bounds__(BX*BY) void k_rbgs(LevelDims L, OpView<T> op,
+                                                T* __restrict__ xc,
+                                                const T* __restrict__ xo,
+                                                const T* __restrict__ bcl,
+                                                int colour, Region rg,
+                                                Tiles tl)
+{
+    const Bid bid{blockIdx.x, blockIdx.y, blockIdx.z};
+    rbgs_body<T, OP>(bid, L, op, xc, xo, bcl, colour, rg, tl);
+}
+
+template <typename T, int OP>
+struct KRbgs
+{
+    static constexpr int bx = BX, by = BY;
+    static constexpr bool lds = false;
+    struct Args
+    {
+        LevelDims L;
+        OpView<T> op;
+        T* xc;
+        const T* xo;
+        const T* bcl;
+        int colour;
+        Region rg;
+        Tiles tl;
+    };
+    __device__ static void run(const Bid bid, const Args& a)
+    {
+        rbgs_body<T, OP>(bid, a.L, a.op, a.xc, a.xo, a.bcl, a.colour, a.rg,
+                         a.tl);
+    }
+    static void launch(dim3 g, size_t shm, hipStream_t s, const Args& a)
+    {
+        hipLaunchKernelGGL((k_rbgs<T, OP>), g, dim3(bx, by), shm, s, a.L, a.op,
+                           a.xc, a.xo, a.bcl, a.colour, a.rg, a.tl);
+    }
+};
+
 
 // All red-black phases of the width-1 boundary slabs of one face pair
 // (rbgsSmoother::SweepBoundary, rbgsSmoother.C:186-374) in one launch.  A
@@ -337,21 +387,24 @@ struct SlabArgs {
 constexpr int SLAB_W = 32;   // LDS tile edge
 
 template <typename T, int OP, int D>
-__global__ __launch_bounds__(256) void k_rbgs_slabs(LevelDims L, OpView<T> op,
-                                                    Pair<T> x, Pair<const T> b,
-                                                    SlabArgs<T> sa)
+__device__ __forceinline__ void rbgs_slabs_body(const Bid bid,
+                                                const LevelDims& L,
+                                                const OpView<T>& op,
+                                                const Pair<T>& x,
+                                                const Pair<const T>& b,
+                                                const SlabArgs<T>& sa)
 {
     constexpr int A1 = D == 0 ? 1 : 0;
     constexpr int A2 = D == 2 ? 1 : 2;
     __shared__ T xs[SLAB_W][SLAB_W + 1];
-    const int slab = blockIdx.z;
+    const int slab = bid.z;
     const int H = sa.nphase;
     const int n1 = L.N[A1], n2 = L.N[A2];
     const bool whole = sa.tt == 0;
     const int W1 = whole ? n1 + 2 : sa.tt + 2*H;
     const int W2 = whole ? n2 + 2 : sa.tt + 2*H;
-    const int o1 = whole ? 0 : 1 + int(blockIdx.y)*sa.tt - H;
-    const int o2 = whole ? 0 : 1 + int(blockIdx.x)*sa.tt - H;
+    const int o1 = whole ? 0 : 1 + int(bid.y)*sa.tt - H;
+    const int o2 = whole ? 0 : 1 + int(bid.x)*sa.tt - H;
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
     const T* src = sa.src[slab];
     const int Ps = L.P[A2];
@@ -489,6 +542,40 @@ __global__ __launch_bounds__(256) void k_rbgs_slabs(LevelDims L, OpView<T> op,
     }
 }
 
+template <typename T, int OP, int D>
+__global__ __launch_bounds__(256) void k_rbgs_slabs(LevelDims L, OpView<T> op,
+                                                    Pair<T> x, Pair<const T> b,
+                                                    SlabArgs<T> sa)
+{
+    const Bid bid{blockIdx.x, blockIdx.y, blockIdx.z};
+    rbgs_slabs_body<T, OP, D>(bid, L, op, x, b, sa);
+}
+
+template <typename T, int OP, int D>
+struct KSlabs
+{
+    static constexpr int bx = 256, by = 1;
+    static constexpr bool lds = true;
+    struct Args
+    {
+        LevelDims L;
+        OpView<T> op;
+        Pair<T> x;
+        Pair<const T> b;
+        SlabArgs<T> sa;
+    };
+    __device__ static void run(const Bid bid, const Args& a)
+    {
+        rbgs_slabs_body<T, OP, D>(bid, a.L, a.op, a.x, a.b, a.sa);
+    }
+    static void launch(dim3 g, size_t shm, hipStream_t s, const Args& a)
+    {
+        hipLaunchKernelGGL((k_rbgs_slabs<T, OP, D>), g, dim3(bx, by), shm, s,
+                           a.L, a.op, a.x, a.b, a.sa);
+    }
+};
+
+
 // Damped Jacobi (jacSmoother.C:18-127, omega 0.8): y from x, then copy back.
 template <typename T, int OP>
 __global__ __launch_bounds__(BX*BY) void k_jacobi(LevelDims L, OpView<T> op,
@@ -550,15 +637,18 @@ __global__ __launch_bounds__(BX*BY) void k_copy_region(LevelDims L,
 // C,L,R,B,T,A,F order, linearSystemFunctions.H:42-49) over the interior.
 // Optionally writes per-tile partial sums of r^2 (deterministic order).
 template <typename T, int OP, bool NORM>
-__global__ __launch_bounds__(BX*BY) void k_residual(LevelDims L, OpView<T> op,
-                                                    Pair<const T> x,
-                                                    Pair<const T> b,
-                                                    Pair<T> r, Tiles tl,
-                                                    double* partials)
+__device__ __forceinline__ void residual_body(const Bid bid,
+                                              const LevelDims& L,
+                                              const OpView<T>& op,
+                                              const Pair<const T>& x,
+                                              const Pair<const T>& b,
+                                              const Pair<T>& r,
+                                              const Tiles& tl,
+                                              double* partials)
 {
     __shared__ double sm[BX*BY/64];
     int bx, by, bz;
-    if (!tile_of(tl, bx, by, bz)) return;
+    if (!tile_of(tl, bid.x, bx, by, bz)) return;
     PairPos c;
     double acc = 0.0;
     if (pair_pos(L, bx, by, bz, c))
@@ -583,6 +673,44 @@ __global__ __launch_bounds__(BX*BY) void k_residual(LevelDims L, OpView<T> op,
     }
 }
 
+template <typename T, int OP, bool NORM>
+__global__ __launch_bounds__(BX*BY) void k_residual(LevelDims L, OpView<T> op,
+                                                    Pair<const T> x,
+                                                    Pair<const T> b, Pair<T> r,
+                                                    Tiles tl, double* partials)
+{
+    const Bid bid{blockIdx.x, blockIdx.y, blockIdx.z};
+    residual_body<T, OP, NORM>(bid, L, op, x, b, r, tl, partials);
+}
+
+template <typename T, int OP, bool NORM>
+struct KResidual
+{
+    static constexpr int bx = BX, by = BY;
+    static constexpr bool lds = true;
+    struct Args
+    {
+        LevelDims L;
+        OpView<T> op;
+        Pair<const T> x;
+        Pair<const T> b;
+        Pair<T> r;
+        Tiles tl;
+        double* partials;
+    };
+    __device__ static void run(const Bid bid, const Args& a)
+    {
+        residual_body<T, OP, NORM>(bid, a.L, a.op, a.x, a.b, a.r, a.tl,
+                                   a.partials);
+    }
+    static void launch(dim3 g, size_t shm, hipStream_t s, const Args& a)
+    {
+        hipLaunchKernelGGL((k_residual<T, OP, NORM>), g, dim3(bx, by), shm, s,
+                           a.L, a.op, a.x, a.b, a.r, a.tl, a.partials);
+    }
+};
+
+
 // ---------------------------------------------------------------------------
 // Scaled linear restriction (linearRestrictionScheme.C:103-160, weights
 // :15-80): b_c = cvc * sum_q (w_q * r_q) * (1/cvf_q), q over the 8 children
@@ -591,14 +719,16 @@ __global__ __launch_bounds__(BX*BY) void k_residual(LevelDims L, OpView<T> op,
 // One thread per pair of coarse cells K = 2p+1, 2p+2 (cell-pair tiles of
 // the coarse level); the 16 fine values are loaded before any arithmetic.
 template <typename T>
-__global__ __launch_bounds__(BX*BY) void k_restrict(LevelDims Lf,
-                                                    LevelDims Lc,
-                                                    TransferView<T> tv,
-                                                    Pair<const T> rf,
-                                                    Pair<T> bc, Tiles tl)
+__device__ __forceinline__ void restrict_body(const Bid bid,
+                                              const LevelDims& Lf,
+                                              const LevelDims& Lc,
+                                              const TransferView<T>& tv,
+                                              const Pair<const T>& rf,
+                                              const Pair<T>& bc,
+                                              const Tiles& tl)
 {
     int bx, by, bz;
-    if (!tile_of(tl, bx, by, bz)) return;
+    if (!tile_of(tl, bid.x, bx, by, bz)) return;
     PairPos c;
     if (!pair_pos(Lc, bx, by, bz, c)) return;
     const int I = c.I, J = c.J;
@@ -669,6 +799,42 @@ __global__ __launch_bounds__(BX*BY) void k_restrict(LevelDims Lf,
     if (c.vB) st(col(bc, c.cB), c.idx + 1, out[1]);
 }
 
+template <typename T>
+__global__ __launch_bounds__(BX*BY) void k_restrict(LevelDims Lf, LevelDims Lc,
+                                                    TransferView<T> tv,
+                                                    Pair<const T> rf,
+                                                    Pair<T> bc, Tiles tl)
+{
+    const Bid bid{blockIdx.x, blockIdx.y, blockIdx.z};
+    restrict_body<T>(bid, Lf, Lc, tv, rf, bc, tl);
+}
+
+template <typename T>
+struct KRestrict
+{
+    static constexpr int bx = BX, by = BY;
+    static constexpr bool lds = false;
+    struct Args
+    {
+        LevelDims Lf;
+        LevelDims Lc;
+        TransferView<T> tv;
+        Pair<const T> rf;
+        Pair<T> bc;
+        Tiles tl;
+    };
+    __device__ static void run(const Bid bid, const Args& a)
+    {
+        restrict_body<T>(bid, a.Lf, a.Lc, a.tv, a.rf, a.bc, a.tl);
+    }
+    static void launch(dim3 g, size_t shm, hipStream_t s, const Args& a)
+    {
+        hipLaunchKernelGGL((k_restrict<T>), g, dim3(bx, by), shm, s, a.Lf,
+                           a.Lc, a.tv, a.rf, a.bc, a.tl);
+    }
+};
+
+
 // Linear prolongation with plusEqOp (linearProlongationScheme.C:112-147,
 // weights :12-110): x_f += sum_v w_v x_c(idx_v), v = (cx,cy,cz) with x
 // fastest, idx = i/R + abc*off, off = i odd ? +1 : -1, w_v = (gx*gy)*gz with
@@ -681,13 +847,16 @@ __global__ __launch_bounds__(BX*BY) void k_restrict(LevelDims Lf,
 // p+2; R = 1: cells 2p+1, 2p+2 with neighbours 2p and 2p+3), loaded once
 // for the 4 coarse columns.
 template <typename T>
-__global__ __launch_bounds__(BX*BY) void k_prolong(LevelDims Lf, LevelDims Lc,
-                                                   TransferView<T> tv,
-                                                   Pair<const T> xc,
-                                                   Pair<T> xf, Tiles tl)
+__device__ __forceinline__ void prolong_body(const Bid bid,
+                                             const LevelDims& Lf,
+                                             const LevelDims& Lc,
+                                             const TransferView<T>& tv,
+                                             const Pair<const T>& xc,
+                                             const Pair<T>& xf,
+                                             const Tiles& tl)
 {
     int bx, by, bz;
-    if (!tile_of(tl, bx, by, bz)) return;
+    if (!tile_of(tl, bid.x, bx, by, bz)) return;
     PairPos c;
     if (!pair_pos(Lf, bx, by, bz, c)) return;
     const int I = c.I, J = c.J, p = c.p;
@@ -737,6 +906,9 @@ __global__ __launch_bounds__(BX*BY) void k_prolong(LevelDims Lf, LevelDims Lc,
         const T gy[2] = {T(1.0) - u1, u1};
         const T gz[2] = {T(1.0) - u2, u2};
         const int zsel[2] = {zpar, znb};
+        // the reference skips zero weights (linearProlongationScheme.C:
+        // 139-143); adding w*c = +-0 for w = 0 leaves the sum unchanged for
+        // finite c, so every term is added
         T value = T(0);
 #pragma unroll
         for (int cz = 0; cz < 2; cz++)
@@ -747,17 +919,50 @@ __global__ __launch_bounds__(BX*BY) void k_prolong(LevelDims Lf, LevelDims Lc,
         {
             T w = gx[cx]*gy[cy];
             w = w*gz[cz];
-            if (w != T(0))
-            {
-                const T t = w*cv[zsel[cz]][cy][cx];
-                value = value + t;
-            }
+            const T t = w*cv[zsel[cz]][cy][cx];
+            value = value + t;
         }
         add[h] = value;
     }
     st(col(xf, c.cA), c.idx, old[0] + add[0]);
     if (c.vB) st(col(xf, c.cB), c.idx + 1, old[1] + add[1]);
 }
+
+template <typename T>
+__global__ __launch_bounds__(BX*BY) void k_prolong(LevelDims Lf, LevelDims Lc,
+                                                   TransferView<T> tv,
+                                                   Pair<const T> xc,
+                                                   Pair<T> xf, Tiles tl)
+{
+    const Bid bid{blockIdx.x, blockIdx.y, blockIdx.z};
+    prolong_body<T>(bid, Lf, Lc, tv, xc, xf, tl);
+}
+
+template <typename T>
+struct KProlong
+{
+    static constexpr int bx = BX, by = BY;
+    static constexpr bool lds = false;
+    struct Args
+    {
+        LevelDims Lf;
+        LevelDims Lc;
+        TransferView<T> tv;
+        Pair<const T> xc;
+        Pair<T> xf;
+        Tiles tl;
+    };
+    __device__ static void run(const Bid bid, const Args& a)
+    {
+        prolong_body<T>(bid, a.Lf, a.Lc, a.tv, a.xc, a.xf, a.tl);
+    }
+    static void launch(dim3 g, size_t shm, hipStream_t s, const Args& a)
+    {
+        hipLaunchKernelGGL((k_prolong<T>), g, dim3(bx, by), shm, s, a.Lf, a.Lc,
+                           a.tv, a.xc, a.xf, a.tl);
+    }
+};
+
 
 // ---------------------------------------------------------------------------
 // Ghost values (meshLevel.C:231-326).  mode 0: periodic face ghosts only
@@ -993,12 +1198,14 @@ struct SnapPtrs {
 };
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_bc_fill(LevelDims L, OpView<T> op,
-                                                 Pair<T> x, int mode,
-                                                 int level0, BcVals bv,
-                                                 BcGroups gr, SnapPtrs<T> sn)
+__device__ __forceinline__ void bc_fill_body(const Bid bid, const LevelDims& L,
+                                             const OpView<T>& op,
+                                             const Pair<T>& x, int mode,
+                                             int level0, const BcVals& bv,
+                                             const BcGroups& gr,
+                                             const SnapPtrs<T>& sn)
 {
-    const int b0 = blockIdx.x*256;
+    const int b0 = bid.x*256;
     int g = 0;
     while (g < 6 && b0 >= gr.start[g + 1]) g++;
     const int t = b0 + int(threadIdx.x) - gr.start[g];
@@ -1071,22 +1278,94 @@ __global__ __launch_bounds__(256) void k_bc_fill(LevelDims L, OpView<T> op,
 }
 
 template <typename T>
-__global__ void k_fill(T* a, T* b, long n, T v)
+__global__ __launch_bounds__(256) void k_bc_fill(LevelDims L, OpView<T> op,
+                                                 Pair<T> x, int mode,
+                                                 int level0, BcVals bv,
+                                                 BcGroups gr, SnapPtrs<T> sn)
 {
-    const long t = (long)blockIdx.x*blockDim.x + threadIdx.x;
+    const Bid bid{blockIdx.x, blockIdx.y, blockIdx.z};
+    bc_fill_body<T>(bid, L, op, x, mode, level0, bv, gr, sn);
+}
+
+template <typename T>
+struct KBcFill
+{
+    static constexpr int bx = 256, by = 1;
+    static constexpr bool lds = false;
+    struct Args
+    {
+        LevelDims L;
+        OpView<T> op;
+        Pair<T> x;
+        int mode;
+        int level0;
+        BcVals bv;
+        BcGroups gr;
+        SnapPtrs<T> sn;
+    };
+    __device__ static void run(const Bid bid, const Args& a)
+    {
+        bc_fill_body<T>(bid, a.L, a.op, a.x, a.mode, a.level0, a.bv, a.gr,
+                        a.sn);
+    }
+    static void launch(dim3 g, size_t shm, hipStream_t s, const Args& a)
+    {
+        hipLaunchKernelGGL((k_bc_fill<T>), g, dim3(bx, by), shm, s, a.L, a.op,
+                           a.x, a.mode, a.level0, a.bv, a.gr, a.sn);
+    }
+};
+
+
+template <typename T>
+__device__ __forceinline__ void fill_body(const Bid bid, T* a, T* b, long n,
+                                          T v)
+{
+    const long t = (long)bid.x*blockDim.x + threadIdx.x;
     if (t < n) { a[t] = v; b[t] = v; }
 }
+
+template <typename T>
+__global__ void k_fill(T* a, T* b, long n, T v)
+{
+    const Bid bid{blockIdx.x, blockIdx.y, blockIdx.z};
+    fill_body<T>(bid, a, b, n, v);
+}
+
+template <typename T>
+struct KFill
+{
+    static constexpr int bx = 256, by = 1;
+    static constexpr bool lds = false;
+    struct Args
+    {
+        T* a;
+        T* b;
+        long n;
+        T v;
+    };
+    __device__ static void run(const Bid bid, const Args& a)
+    {
+        fill_body<T>(bid, a.a, a.b, a.n, a.v);
+    }
+    static void launch(dim3 g, size_t shm, hipStream_t s, const Args& a)
+    {
+        hipLaunchKernelGGL((k_fill<T>), g, dim3(bx, by), shm, s, a.a, a.b, a.n,
+                           a.v);
+    }
+};
+
 
 // ---------------------------------------------------------------------------
 // Layout conversions between the reference block layout and colour-split.
 template <typename T>
-__global__ __launch_bounds__(BX*BY) void k_ref_to_split(LevelDims L,
-                                                        const double* ref,
-                                                        Pair<T> out)
+__device__ __forceinline__ void ref_to_split_body(const Bid bid,
+                                                  const LevelDims& L,
+                                                  const double* ref,
+                                                  const Pair<T>& out)
 {
-    const int I = blockIdx.z;
-    const int J = blockIdx.y*BY + threadIdx.y;
-    const int p = blockIdx.x*BX + threadIdx.x;
+    const int I = bid.z;
+    const int J = bid.y*BY + threadIdx.y;
+    const int p = bid.x*BX + threadIdx.x;
     if (J >= L.P[1] || p >= L.Hz) return;
     const int idx = (I*L.P[1] + J)*L.Hz + p;
     for (int h = 0; h < 2; h++)
@@ -1099,13 +1378,46 @@ __global__ __launch_bounds__(BX*BY) void k_ref_to_split(LevelDims L,
 }
 
 template <typename T>
-__global__ __launch_bounds__(BX*BY) void k_split_to_ref(LevelDims L,
-                                                        Pair<const T> in,
-                                                        double* ref)
+__global__ __launch_bounds__(BX*BY) void k_ref_to_split(LevelDims L,
+                                                        const double* ref,
+                                                        Pair<T> out)
 {
-    const int I = blockIdx.z;
-    const int J = blockIdx.y*BY + threadIdx.y;
-    const int p = blockIdx.x*BX + threadIdx.x;
+    const Bid bid{blockIdx.x, blockIdx.y, blockIdx.z};
+    ref_to_split_body<T>(bid, L, ref, out);
+}
+
+template <typename T>
+struct KRefToSplit
+{
+    static constexpr int bx = BX, by = BY;
+    static constexpr bool lds = false;
+    struct Args
+    {
+        LevelDims L;
+        const double* ref;
+        Pair<T> out;
+    };
+    __device__ static void run(const Bid bid, const Args& a)
+    {
+        ref_to_split_body<T>(bid, a.L, a.ref, a.out);
+    }
+    static void launch(dim3 g, size_t shm, hipStream_t s, const Args& a)
+    {
+        hipLaunchKernelGGL((k_ref_to_split<T>), g, dim3(bx, by), shm, s, a.L,
+                           a.ref, a.out);
+    }
+};
+
+
+template <typename T>
+__device__ __forceinline__ void split_to_ref_body(const Bid bid,
+                                                  const LevelDims& L,
+                                                  const Pair<const T>& in,
+                                                  double* ref)
+{
+    const int I = bid.z;
+    const int J = bid.y*BY + threadIdx.y;
+    const int p = bid.x*BX + threadIdx.x;
     if (J >= L.P[1] || p >= L.Hz) return;
     const int idx = (I*L.P[1] + J)*L.Hz + p;
     for (int h = 0; h < 2; h++)
@@ -1116,6 +1428,38 @@ __global__ __launch_bounds__(BX*BY) void k_split_to_ref(LevelDims L,
             double(in.c[(I + J + K) & 1][idx]);
     }
 }
+
+template <typename T>
+__global__ __launch_bounds__(BX*BY) void k_split_to_ref(LevelDims L,
+                                                        Pair<const T> in,
+                                                        double* ref)
+{
+    const Bid bid{blockIdx.x, blockIdx.y, blockIdx.z};
+    split_to_ref_body<T>(bid, L, in, ref);
+}
+
+template <typename T>
+struct KSplitToRef
+{
+    static constexpr int bx = BX, by = BY;
+    static constexpr bool lds = false;
+    struct Args
+    {
+        LevelDims L;
+        Pair<const T> in;
+        double* ref;
+    };
+    __device__ static void run(const Bid bid, const Args& a)
+    {
+        split_to_ref_body<T>(bid, a.L, a.in, a.ref);
+    }
+    static void launch(dim3 g, size_t shm, hipStream_t s, const Args& a)
+    {
+        hipLaunchKernelGGL((k_split_to_ref<T>), g, dim3(bx, by), shm, s, a.L,
+                           a.in, a.ref);
+    }
+};
+
 
 __global__ __launch_bounds__(BX*BY) void k_stencil_ref_to_split(
     LevelDims L, const double* ref7, Pair<double> a7)
@@ -1198,11 +1542,12 @@ __device__ __forceinline__ bool is_cell_zero(const LevelDims& L, int I, int J,
 
 // setSingularityConstraint (solver.C:200-231): A(0,0,0) = diag(1) and every
 // coupling to global cell 0 removed.
-__global__ __launch_bounds__(BX*BY) void k_pin(LevelDims L, Pair<double> a7)
+__device__ __forceinline__ void pin_body(const Bid bid, const LevelDims& L,
+                                         const Pair<double>& a7)
 {
-    const int I = 1 + blockIdx.z;
-    const int J = 1 + blockIdx.y*BY + threadIdx.y;
-    const int p = blockIdx.x*BX + threadIdx.x;
+    const int I = 1 + bid.z;
+    const int J = 1 + bid.y*BY + threadIdx.y;
+    const int p = bid.x*BX + threadIdx.x;
     if (J > L.N[1]) return;
     const int idx = (I*L.P[1] + J)*L.Hz + p;
     const int off[7][3] = {{0,0,0},{-1,0,0},{1,0,0},{0,-1,0},{0,1,0},
@@ -1224,6 +1569,32 @@ __global__ __launch_bounds__(BX*BY) void k_pin(LevelDims L, Pair<double> a7)
                 a[s*L.ncol + idx] = 0.0;
     }
 }
+
+__global__ __launch_bounds__(BX*BY) void k_pin(LevelDims L, Pair<double> a7)
+{
+    const Bid bid{blockIdx.x, blockIdx.y, blockIdx.z};
+    pin_body(bid, L, a7);
+}
+
+struct KPin
+{
+    static constexpr int bx = BX, by = BY;
+    static constexpr bool lds = false;
+    struct Args
+    {
+        LevelDims L;
+        Pair<double> a7;
+    };
+    __device__ static void run(const Bid bid, const Args& a)
+    {
+        pin_body(bid, a.L, a.a7);
+    }
+    static void launch(dim3 g, size_t shm, hipStream_t s, const Args& a)
+    {
+        hipLaunchKernelGGL((k_pin), g, dim3(bx, by), shm, s, a.L, a.a7);
+    }
+};
+
 
 __global__ __launch_bounds__(BX*BY) void k_eliminate(LevelDims L,
                                                      Pair<double> a7)
@@ -1249,13 +1620,15 @@ __global__ __launch_bounds__(BX*BY) void k_eliminate(LevelDims L,
 // Neumann b -= g*grad/delta, Dirichlet b -= g*2*value, g the assembled
 // ghost coefficient.
 template <int OP>
-__global__ __launch_bounds__(BX*BY) void k_bc_source(LevelDims L,
-                                                     OpView<double> op,
-                                                     Pair<double> b, BcVals bv)
+__device__ __forceinline__ void bc_source_body(const Bid bid,
+                                               const LevelDims& L,
+                                               const OpView<double>& op,
+                                               const Pair<double>& b,
+                                               const BcVals& bv)
 {
-    const int I = 1 + blockIdx.z;
-    const int J = 1 + blockIdx.y*BY + threadIdx.y;
-    const int p = blockIdx.x*BX + threadIdx.x;
+    const int I = 1 + bid.z;
+    const int J = 1 + bid.y*BY + threadIdx.y;
+    const int p = bid.x*BX + threadIdx.x;
     if (J > L.N[1]) return;
     const int row = (I*L.P[1] + J)*L.Hz;
     const int idx = row + p;
@@ -1285,6 +1658,39 @@ __global__ __launch_bounds__(BX*BY) void k_bc_source(LevelDims L,
         b.c[colour][idx] = v;
     }
 }
+
+template <int OP>
+__global__ __launch_bounds__(BX*BY) void k_bc_source(LevelDims L,
+                                                     OpView<double> op,
+                                                     Pair<double> b, BcVals bv)
+{
+    const Bid bid{blockIdx.x, blockIdx.y, blockIdx.z};
+    bc_source_body<OP>(bid, L, op, b, bv);
+}
+
+template <int OP>
+struct KBcSource
+{
+    static constexpr int bx = BX, by = BY;
+    static constexpr bool lds = false;
+    struct Args
+    {
+        LevelDims L;
+        OpView<double> op;
+        Pair<double> b;
+        BcVals bv;
+    };
+    __device__ static void run(const Bid bid, const Args& a)
+    {
+        bc_source_body<OP>(bid, a.L, a.op, a.b, a.bv);
+    }
+    static void launch(dim3 g, size_t shm, hipStream_t s, const Args& a)
+    {
+        hipLaunchKernelGGL((k_bc_source<OP>), g, dim3(bx, by), shm, s, a.L,
+                           a.op, a.b, a.bv);
+    }
+};
+
 
 // areaWeightedFaceRestrictionScheme.C:19-56 for face direction fd: coarse
 // face value = sum(a*lam)/sum(a) over its fine faces.  Reference layout.
@@ -1349,12 +1755,14 @@ __global__ void k_face_coeffs(LevelDims L, const double* w0, const double* w1,
 // The factors are staged in LDS when they fit.  Then the full ghost update
 // of the level (APLU.C:126).
 template <typename T, bool LDS>
-__global__ __launch_bounds__(256) void k_coarse_solve(LevelDims L,
-                                                      OpView<T> op, Pair<T> x,
-                                                      Pair<const T> b,
-                                                      const double* lug,
-                                                      const int* piv, int n,
-                                                      int level0, BcVals bv)
+__device__ __forceinline__ void coarse_solve_body(const Bid bid,
+                                                  const LevelDims& L,
+                                                  const OpView<T>& op,
+                                                  const Pair<T>& x,
+                                                  const Pair<const T>& b,
+                                                  const double* lug,
+                                                  const int* piv, int n,
+                                                  int level0, const BcVals& bv)
 {
     extern __shared__ double smem[];
     double* y = smem;
@@ -1441,17 +1849,206 @@ __global__ __launch_bounds__(256) void k_coarse_solve(LevelDims L,
     }
 }
 
+template <typename T, bool LDS>
+__global__ __launch_bounds__(256) void k_coarse_solve(LevelDims L,
+                                                      OpView<T> op, Pair<T> x,
+                                                      Pair<const T> b,
+                                                      const double* lug,
+                                                      const int* piv, int n,
+                                                      int level0, BcVals bv)
+{
+    const Bid bid{blockIdx.x, blockIdx.y, blockIdx.z};
+    coarse_solve_body<T, LDS>(bid, L, op, x, b, lug, piv, n, level0, bv);
+}
+
+template <typename T, bool LDS>
+struct KCoarse
+{
+    static constexpr int bx = 256, by = 1;
+    static constexpr bool lds = true;
+    struct Args
+    {
+        LevelDims L;
+        OpView<T> op;
+        Pair<T> x;
+        Pair<const T> b;
+        const double* lug;
+        const int* piv;
+        int n;
+        int level0;
+        BcVals bv;
+    };
+    __device__ static void run(const Bid bid, const Args& a)
+    {
+        coarse_solve_body<T, LDS>(bid, a.L, a.op, a.x, a.b, a.lug, a.piv, a.n,
+                                  a.level0, a.bv);
+    }
+    static void launch(dim3 g, size_t shm, hipStream_t s, const Args& a)
+    {
+        hipLaunchKernelGGL((k_coarse_solve<T, LDS>), g, dim3(bx, by), shm, s,
+                           a.L, a.op, a.x, a.b, a.lug, a.piv, a.n, a.level0,
+                           a.bv);
+    }
+};
+
+
+// Value of lane `lane` (a compile-time lane) in every lane.
+__device__ __forceinline__ double lane_value(double v, int lane)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane(int(b), lane);
+    const int hi = __builtin_amdgcn_readlane(int(b >> 32), lane);
+    return __longlong_as_double((long long)(unsigned)lo |
+                                ((long long)hi << 32));
+}
+
+// Coarsest-level direct solve of at most N <= 64 cells in one wavefront,
+// with the same operations in the same order as k_coarse_solve (APLU.C:
+// 91-128; LUBacksubstitute row by row, subtractions in increasing column
+// order).  Lane i holds row i of the factors in registers (loaded from the
+// column-major copy, one coalesced load per column).  The system is padded
+// to N with unit rows and zero right-hand sides: every padded term is an
+// exact zero subtracted after all real terms of its row, so the results do
+// not change.  Forward substitution broadcasts y_j and updates the rows
+// below it; the backward pass keeps, per lane, the products U(i,j)*y_j of
+// the columns already solved, so row i's subtraction chain waits on no
+// memory.  The row permutation is applied once (precomputed from the pivot
+// sequence).  Ghost update as in k_coarse_solve.
+template <typename T, int N>
+__device__ __forceinline__ void coarse_solve_wave_body(const Bid bid,
+                                                       const LevelDims& L,
+                                                       const OpView<T>& op,
+                                                       const Pair<T>& x,
+                                                       const Pair<const T>& b,
+                                                       const double* luc,
+                                                       const int* perm, int n,
+                                                       int level0,
+                                                       const BcVals& bv)
+{
+    const int lane = threadIdx.x;
+    double row[N];
+#pragma unroll
+    for (int j = 0; j < N; j++) row[j] = lane < N ? luc[j*N + lane] : 0.0;
+    const int N1 = L.N[1], N2 = L.N[2];
+    const int off[7][3] = {{0,0,0},{-1,0,0},{1,0,0},{0,-1,0},{0,1,0},
+                           {0,0,-1},{0,0,1}};
+    double v = 0.0;
+    if (lane < n)
+    {
+        const int q = lane;
+        const int I = 1 + q/(N1*N2), J = 1 + (q/N2) % N1, K = 1 + q % N2;
+        const int colour = (I + J + K) & 1;
+        const int idx = cidx(L, I, J, K);
+        v = double(b.c[colour][idx]);
+        for (int s = 1; s < 7; s++)
+        {
+            const int g[3] = {I + off[s][0], J + off[s][1], K + off[s][2]};
+            bool out = false;
+            for (int d = 0; d < 3; d++)
+                if ((g[d] < 1 || g[d] > L.N[d]) &&
+                    L.bc[2*d + (g[d] > L.N[d])] != PER)
+                    out = true;
+            if (out)
+            {
+                const double a = double(op.a7[colour][s*nci(L) + idx]);
+                const double t = a*double(x.c[colour ^ 1][cidx(L, g[0], g[1], g[2])]);
+                v = v - t;
+            }
+        }
+    }
+    double y = __shfl(v, lane < n ? perm[lane] : lane);
+#pragma unroll
+    for (int j = 0; j < N - 1; j++)
+    {
+        const double yj = lane_value(y, j);
+        const double t = row[j]*yj;
+        const double yn = y - t;
+        y = lane > j ? yn : y;
+    }
+#pragma unroll
+    for (int i = N - 1; i >= 0; i--)
+    {
+        double sum = y;
+#pragma unroll
+        for (int j = i + 1; j < N; j++) sum = sum - row[j];
+        const double yi = lane_value(sum/row[i], i);
+        y = lane == i ? yi : y;
+        row[i] = row[i]*yi;
+    }
+    if (lane < n)
+    {
+        const int q = lane;
+        const int I = 1 + q/(N1*N2), J = 1 + (q/N2) % N1, K = 1 + q % N2;
+        col(x, (I + J + K) & 1)[cidx(L, I, J, K)] = T(y);
+    }
+    __syncthreads();
+    const Shell sh = shell_all(L);
+    const int ns = sh.nA + sh.nB + sh.nC;
+    for (int t = lane; t < ns; t += 64)
+    {
+        int I, J, K;
+        if (shell_cell(L, sh, t, I, J, K))
+            ghost_cell(L, op, x, 1, level0, bv, I, J, K);
+    }
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__(64) void k_coarse_solve_wave(LevelDims L,
+                                                          OpView<T> op,
+                                                          Pair<T> x,
+                                                          Pair<const T> b,
+                                                          const double* luc,
+                                                          const int* perm,
+                                                          int n, int level0,
+                                                          BcVals bv)
+{
+    const Bid bid{blockIdx.x, blockIdx.y, blockIdx.z};
+    coarse_solve_wave_body<T, N>(bid, L, op, x, b, luc, perm, n, level0, bv);
+}
+
+template <typename T, int N>
+struct KCoarseWave
+{
+    static constexpr int bx = 64, by = 1;
+    static constexpr bool lds = false;
+    struct Args
+    {
+        LevelDims L;
+        OpView<T> op;
+        Pair<T> x;
+        Pair<const T> b;
+        const double* luc;
+        const int* perm;
+        int n;
+        int level0;
+        BcVals bv;
+    };
+    __device__ static void run(const Bid bid, const Args& a)
+    {
+        coarse_solve_wave_body<T, N>(bid, a.L, a.op, a.x, a.b, a.luc, a.perm,
+                                     a.n, a.level0, a.bv);
+    }
+    static void launch(dim3 g, size_t shm, hipStream_t s, const Args& a)
+    {
+        hipLaunchKernelGGL((k_coarse_solve_wave<T, N>), g, dim3(bx, by), shm,
+                           s, a.L, a.op, a.x, a.b, a.luc, a.perm, a.n,
+                           a.level0, a.bv);
+    }
+};
+
+
 // ---------------------------------------------------------------------------
 // Reductions over interior cells (per-block partials, fixed order).
 template <typename T, int MODE>
-__global__ __launch_bounds__(BX*BY) void k_reduce_field(LevelDims L,
-                                                        Pair<const T> f,
-                                                        Tiles tl,
-                                                        double* partials)
+__device__ __forceinline__ void reduce_field_body(const Bid bid,
+                                                  const LevelDims& L,
+                                                  const Pair<const T>& f,
+                                                  const Tiles& tl,
+                                                  double* partials)
 {
     __shared__ double sm[BX*BY/64];
     int bx, by, bz;
-    if (!tile_of(tl, bx, by, bz)) return;
+    if (!tile_of(tl, bid.x, bx, by, bz)) return;
     PairPos c;
     double acc = 0.0;
     if (pair_pos(L, bx, by, bz, c))
@@ -1466,17 +2063,54 @@ __global__ __launch_bounds__(BX*BY) void k_reduce_field(LevelDims L,
         partials[tile_index(tl, bx, by, bz)] = s;
 }
 
+template <typename T, int MODE>
+__global__ __launch_bounds__(BX*BY) void k_reduce_field(LevelDims L,
+                                                        Pair<const T> f,
+                                                        Tiles tl,
+                                                        double* partials)
+{
+    const Bid bid{blockIdx.x, blockIdx.y, blockIdx.z};
+    reduce_field_body<T, MODE>(bid, L, f, tl, partials);
+}
+
+template <typename T, int MODE>
+struct KReduceField
+{
+    static constexpr int bx = BX, by = BY;
+    static constexpr bool lds = true;
+    struct Args
+    {
+        LevelDims L;
+        Pair<const T> f;
+        Tiles tl;
+        double* partials;
+    };
+    __device__ static void run(const Bid bid, const Args& a)
+    {
+        reduce_field_body<T, MODE>(bid, a.L, a.f, a.tl, a.partials);
+    }
+    static void launch(dim3 g, size_t shm, hipStream_t s, const Args& a)
+    {
+        hipLaunchKernelGGL((k_reduce_field<T, MODE>), g, dim3(bx, by), shm, s,
+                           a.L, a.f, a.tl, a.partials);
+    }
+};
+
+
 // Terms of normFactors (solver.C:35-46): (|r - b + Ay| + |b - Ay|)^2 with
 // Ay = rowSum(A)*y, rowSum = C + L + R + B + T + A + F in that order.
 template <typename T, int OP>
-__global__ __launch_bounds__(BX*BY) void k_nf_terms(LevelDims L, OpView<T> op,
-                                                    Pair<const T> r,
-                                                    Pair<const T> b, double y,
-                                                    Tiles tl, double* partials)
+__device__ __forceinline__ void nf_terms_body(const Bid bid,
+                                              const LevelDims& L,
+                                              const OpView<T>& op,
+                                              const Pair<const T>& r,
+                                              const Pair<const T>& b, double y,
+                                              const Tiles& tl,
+                                              double* partials)
 {
     __shared__ double sm[BX*BY/64];
     int bx, by, bz;
-    if (!tile_of(tl, bx, by, bz)) return;
+    if (!tile_of(tl, bid.x, bx, by, bz)) return;
     PairPos c;
     double acc = 0.0;
     if (pair_pos(L, bx, by, bz, c))
@@ -1503,8 +2137,47 @@ __global__ __launch_bounds__(BX*BY) void k_nf_terms(LevelDims L, OpView<T> op,
         partials[tile_index(tl, bx, by, bz)] = s;
 }
 
-__global__ __launch_bounds__(1024) void k_reduce_final(const double* partials,
-                                                       int n, double* out)
+template <typename T, int OP>
+__global__ __launch_bounds__(BX*BY) void k_nf_terms(LevelDims L, OpView<T> op,
+                                                    Pair<const T> r,
+                                                    Pair<const T> b, double y,
+                                                    Tiles tl, double* partials)
+{
+    const Bid bid{blockIdx.x, blockIdx.y, blockIdx.z};
+    nf_terms_body<T, OP>(bid, L, op, r, b, y, tl, partials);
+}
+
+template <typename T, int OP>
+struct KNfTerms
+{
+    static constexpr int bx = BX, by = BY;
+    static constexpr bool lds = true;
+    struct Args
+    {
+        LevelDims L;
+        OpView<T> op;
+        Pair<const T> r;
+        Pair<const T> b;
+        double y;
+        Tiles tl;
+        double* partials;
+    };
+    __device__ static void run(const Bid bid, const Args& a)
+    {
+        nf_terms_body<T, OP>(bid, a.L, a.op, a.r, a.b, a.y, a.tl, a.partials);
+    }
+    static void launch(dim3 g, size_t shm, hipStream_t s, const Args& a)
+    {
+        hipLaunchKernelGGL((k_nf_terms<T, OP>), g, dim3(bx, by), shm, s, a.L,
+                           a.op, a.r, a.b, a.y, a.tl, a.partials);
+    }
+};
+
+
+__device__ __forceinline__ void reduce_final_body(const Bid bid,
+                                                  const double* partials,
+                                                  int n, double* out,
+                                                  double* hostOut)
 {
     __shared__ double sm[16];
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
@@ -1518,8 +2191,43 @@ __global__ __launch_bounds__(1024) void k_reduce_final(const double* partials,
     }
     for (; i < n; i += 1024) a0 += partials[i];
     const double s = block_sum((a0 + a1) + (a2 + a3), sm);
-    if (threadIdx.x == 0) out[0] = s;
+    if (threadIdx.x == 0)
+    {
+        out[0] = s;
+        if (hostOut) hostOut[0] = s;
+    }
 }
+
+__global__ __launch_bounds__(1024) void k_reduce_final(const double* partials,
+                                                       int n, double* out,
+                                                       double* hostOut)
+{
+    const Bid bid{blockIdx.x, blockIdx.y, blockIdx.z};
+    reduce_final_body(bid, partials, n, out, hostOut);
+}
+
+struct KReduceFinal
+{
+    static constexpr int bx = 1024, by = 1;
+    static constexpr bool lds = true;
+    struct Args
+    {
+        const double* partials;
+        int n;
+        double* out;
+        double* hostOut;
+    };
+    __device__ static void run(const Bid bid, const Args& a)
+    {
+        reduce_final_body(bid, a.partials, a.n, a.out, a.hostOut);
+    }
+    static void launch(dim3 g, size_t shm, hipStream_t s, const Args& a)
+    {
+        hipLaunchKernelGGL((k_reduce_final), g, dim3(bx, by), shm, s,
+                           a.partials, a.n, a.out, a.hostOut);
+    }
+};
+
 
 
 
@@ -1541,7 +2249,7 @@ __global__ __launch_bounds__(BX*BY) void k_apply_dots(LevelDims L,
 {
     __shared__ double sm[BX*BY/64];
     int bx, by, bz;
-    if (!tile_of(tl, bx, by, bz)) return;
+    if (!tile_of(tl, blockIdx.x, bx, by, bz)) return;
     PairPos c;
     double pq = 0.0, pr = 0.0;
     if (pair_pos(L, bx, by, bz, c))
@@ -1591,7 +2299,7 @@ __global__ __launch_bounds__(BX*BY) void k_fcg_update(LevelDims L,
 {
     __shared__ double sm[BX*BY/64];
     int bx, by, bz;
-    if (!tile_of(tl, bx, by, bz)) return;
+    if (!tile_of(tl, blockIdx.x, bx, by, bz)) return;
     const double alpha = sc[1]/sc[0];
     PairPos c;
     double acc = 0.0;
@@ -1638,7 +2346,7 @@ __global__ __launch_bounds__(BX*BY) void k_dot_zq(LevelDims L, Pair<const Z> z,
 {
     __shared__ double sm[BX*BY/64];
     int bx, by, bz;
-    if (!tile_of(tl, bx, by, bz)) return;
+    if (!tile_of(tl, blockIdx.x, bx, by, bz)) return;
     PairPos c;
     double acc = 0.0;
     if (pair_pos(L, bx, by, bz, c))
@@ -1664,7 +2372,7 @@ __global__ __launch_bounds__(BX*BY) void k_update_p(LevelDims L,
                                                     Pair<double> p, Tiles tl)
 {
     int bx, by, bz;
-    if (!tile_of(tl, bx, by, bz)) return;
+    if (!tile_of(tl, blockIdx.x, bx, by, bz)) return;
     const double beta = -sc[3]/sc[0];
     PairPos c;
     if (!pair_pos(L, bx, by, bz, c)) return;
@@ -1687,7 +2395,7 @@ __global__ __launch_bounds__(BX*BY) void k_split_to_dense(LevelDims L,
                                                           float* out, Tiles tl)
 {
     int bx, by, bz;
-    if (!tile_of(tl, bx, by, bz)) return;
+    if (!tile_of(tl, blockIdx.x, bx, by, bz)) return;
     PairPos c;
     if (!pair_pos(L, bx, by, bz, c)) return;
     const double ncell = double(L.N[0])*L.N[1]*L.N[2];
@@ -1714,7 +2422,7 @@ __global__ __launch_bounds__(BX*BY) void k_dense_to_split(LevelDims L,
                                                           Tiles tl)
 {
     int bx, by, bz;
-    if (!tile_of(tl, bx, by, bz)) return;
+    if (!tile_of(tl, blockIdx.x, bx, by, bz)) return;
     PairPos c;
     if (!pair_pos(L, bx, by, bz, c)) return;
     const double ncell = double(L.N[0])*L.N[1]*L.N[2];
@@ -1737,7 +2445,7 @@ __global__ __launch_bounds__(BX*BY) void k_add_interior(LevelDims L,
                                                         Tiles tl)
 {
     int bx, by, bz;
-    if (!tile_of(tl, bx, by, bz)) return;
+    if (!tile_of(tl, blockIdx.x, bx, by, bz)) return;
     PairPos c;
     if (!pair_pos(L, bx, by, bz, c)) return;
     for (int h = 0; h < 2; h++)
@@ -1752,7 +2460,8 @@ __global__ __launch_bounds__(BX*BY) void k_add_interior(LevelDims L,
 // Sums of nrows rows of n partials each, one workgroup per row, into
 // out[row] (fixed order, as k_reduce_final).
 __global__ __launch_bounds__(1024) void k_reduce_rows(const double* partials,
-                                                      int n, double* out)
+                                                      int n, double* out,
+                                                      double* hostOut)
 {
     __shared__ double sm[16];
     const double* row = partials + size_t(blockIdx.x)*n;
@@ -1767,7 +2476,11 @@ __global__ __launch_bounds__(1024) void k_reduce_rows(const double* partials,
     }
     for (; i < n; i += 1024) a0 += row[i];
     const double s = block_sum((a0 + a1) + (a2 + a3), sm);
-    if (threadIdx.x == 0) out[blockIdx.x] = s;
+    if (threadIdx.x == 0)
+    {
+        out[blockIdx.x] = s;
+        if (hostOut) hostOut[blockIdx.x] = s;
+    }
 }
 
 
@@ -1803,13 +2516,13 @@ __global__ __launch_bounds__(BX*BY) void k_diag(LevelDims L, Pair<const double> 
     }
 }
 
-__global__ __launch_bounds__(BX*BY) void k_add_correction(LevelDims L,
-                                                          Pair<double> x,
-                                                          Pair<const float> e,
-                                                          Tiles tl)
+__device__ __forceinline__ void add_correction_body(const Bid bid,
+                                                    const LevelDims& L,
+                                                    const Pair<double>& x,
+    const Pair<const float>& e, const Tiles& tl)
 {
     int bx, by, bz;
-    if (!tile_of(tl, bx, by, bz)) return;
+    if (!tile_of(tl, bid.x, bx, by, bz)) return;
     PairPos c;
     if (!pair_pos(L, bx, by, bz, c)) return;
     const double xA = ld(col(x, c.cA), c.idx), xB = ld(col(x, c.cB), c.idx + 1);
@@ -1819,9 +2532,43 @@ __global__ __launch_bounds__(BX*BY) void k_add_correction(LevelDims L,
     if (c.vB) st(col(x, c.cB), c.idx + 1, xB + eB);
 }
 
-__global__ void k_to_float(long n, Pair<const double> in, Pair<float> out)
+__global__ __launch_bounds__(BX*BY) void k_add_correction(LevelDims L,
+                                                          Pair<double> x,
+                                                          Pair<const float> e,
+                                                          Tiles tl)
 {
-    const long t = (long)blockIdx.x*blockDim.x + threadIdx.x;
+    const Bid bid{blockIdx.x, blockIdx.y, blockIdx.z};
+    add_correction_body(bid, L, x, e, tl);
+}
+
+struct KAddCorrection
+{
+    static constexpr int bx = BX, by = BY;
+    static constexpr bool lds = false;
+    struct Args
+    {
+        LevelDims L;
+        Pair<double> x;
+        Pair<const float> e;
+        Tiles tl;
+    };
+    __device__ static void run(const Bid bid, const Args& a)
+    {
+        add_correction_body(bid, a.L, a.x, a.e, a.tl);
+    }
+    static void launch(dim3 g, size_t shm, hipStream_t s, const Args& a)
+    {
+        hipLaunchKernelGGL((k_add_correction), g, dim3(bx, by), shm, s, a.L,
+                           a.x, a.e, a.tl);
+    }
+};
+
+
+__device__ __forceinline__ void to_float_body(const Bid bid, long n,
+                                              const Pair<const double>& in,
+                                              const Pair<float>& out)
+{
+    const long t = (long)bid.x*blockDim.x + threadIdx.x;
     if (t < n)
     {
         out.c[0][t] = float(in.c[0][t]);
@@ -1829,11 +2576,71 @@ __global__ void k_to_float(long n, Pair<const double> in, Pair<float> out)
     }
 }
 
+__global__ void k_to_float(long n, Pair<const double> in, Pair<float> out)
+{
+    const Bid bid{blockIdx.x, blockIdx.y, blockIdx.z};
+    to_float_body(bid, n, in, out);
+}
+
+struct KToFloat
+{
+    static constexpr int bx = 256, by = 1;
+    static constexpr bool lds = false;
+    struct Args
+    {
+        long n;
+        Pair<const double> in;
+        Pair<float> out;
+    };
+    __device__ static void run(const Bid bid, const Args& a)
+    {
+        to_float_body(bid, a.n, a.in, a.out);
+    }
+    static void launch(dim3 g, size_t shm, hipStream_t s, const Args& a)
+    {
+        hipLaunchKernelGGL((k_to_float), g, dim3(bx, by), shm, s, a.n, a.in,
+                           a.out);
+    }
+};
+
+
 template <typename T>
-__global__ void k_set_cell(Pair<T> f, long idx, int colour, T v)
+__device__ __forceinline__ void set_cell_body(const Bid bid, const Pair<T>& f,
+                                              long idx, int colour, T v)
 {
     f.c[colour][idx] = v;
 }
+
+template <typename T>
+__global__ void k_set_cell(Pair<T> f, long idx, int colour, T v)
+{
+    const Bid bid{blockIdx.x, blockIdx.y, blockIdx.z};
+    set_cell_body<T>(bid, f, idx, colour, v);
+}
+
+template <typename T>
+struct KSetCell
+{
+    static constexpr int bx = 1, by = 1;
+    static constexpr bool lds = false;
+    struct Args
+    {
+        Pair<T> f;
+        long idx;
+        int colour;
+        T v;
+    };
+    __device__ static void run(const Bid bid, const Args& a)
+    {
+        set_cell_body<T>(bid, a.f, a.idx, a.colour, a.v);
+    }
+    static void launch(dim3 g, size_t shm, hipStream_t s, const Args& a)
+    {
+        hipLaunchKernelGGL((k_set_cell<T>), g, dim3(bx, by), shm, s, a.f,
+                           a.idx, a.colour, a.v);
+    }
+};
+
 
 inline dim3 grid3(const Region& rg)
 {
@@ -1884,7 +2691,88 @@ BcVals bcvals(const double* v)
     return b;
 }
 
+// ---------------------------------------------------------------------------
+// Batched launches.  Every batchable kernel k_x has a struct K with its
+// argument struct K::Args, its workgroup shape (K::bx, K::by), its body
+// K::run(workgroup, args) and K::launch (one launch of k_x).  k_many runs
+// one launch for many independent systems: the systems' workgroups are
+// concatenated into virtual indices (each system's share starting at a
+// multiple of 8, so a workgroup keeps the XCD it would have on its own), a
+// fixed grid walks them with a grid-stride loop, and each workgroup finds
+// its system by binary search over the shares and reads that system's
+// argument struct.  Systems whose active flag is 0 are skipped.
+template <class K>
+__global__ __launch_bounds__(K::bx*K::by) void k_many(
+    const typename K::Args* __restrict__ args,
+    const BatchItem* __restrict__ items, int nitems, unsigned total,
+    const int* __restrict__ active)
+{
+    for (unsigned W = blockIdx.x; W < total; W += gridDim.x)
+    {
+        int lo = 0, hi = nitems - 1;
+        while (lo < hi)
+        {
+            const int mid = (lo + hi + 1) >> 1;
+            if (items[mid].start <= W) lo = mid;
+            else hi = mid - 1;
+        }
+        const unsigned w = W - items[lo].start;
+        const unsigned gx = items[lo].gx, gy = items[lo].gy;
+        if (w < items[lo].n && (!active || active[items[lo].sys]))
+            K::run(Bid{w % gx, (w/gx) % gy, w/(gx*gy)}, args[lo]);
+        // the next system's body reuses the workgroup's LDS
+        if constexpr (K::lds) __syncthreads();
+    }
+}
+
+template <class K>
+char kind_tag;
+
+template <class K>
+void launch_many(const BatchLaunch& b, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_many<K>), dim3(b.grid), dim3(K::bx, K::by), b.shm,
+                       s, static_cast<const typename K::Args*>(b.args),
+                       b.items, b.nitems, b.total, b.active);
+}
+
+// Launches K, or appends the launch to the calling thread's recorder.
+template <class K>
+void dispatch(dim3 g, size_t shm, hipStream_t s, const typename K::Args& a)
+{
+    if (LaunchRecorder* r = launch_recorder())
+    {
+        LaunchRecord rec;
+        rec.kind = &kind_tag<K>;
+        rec.grid[0] = g.x;
+        rec.grid[1] = g.y;
+        rec.grid[2] = g.z;
+        rec.shm = shm;
+        rec.align = alignof(typename K::Args);
+        rec.args.resize(sizeof(a));
+        std::memcpy(rec.args.data(), &a, sizeof(a));
+        rec.many = &launch_many<K>;
+        r->recs.push_back(std::move(rec));
+        return;
+    }
+    K::launch(g, shm, s, a);
+}
+
+// Launchers of kernels without a batched form refuse to record.
+void direct_only(const char* what)
+{
+    if (launch_recorder())
+        throw std::runtime_error(std::string(what) +
+                                 " has no batched form");
+}
+
 }  // namespace
+
+LaunchRecorder*& launch_recorder()
+{
+    thread_local LaunchRecorder* r = nullptr;
+    return r;
+}
 
 // ---------------------------------------------------------------------------
 // launchers
@@ -1906,16 +2794,16 @@ void launch_rbgs(const LevelDims& L, const OpView<T>& op, Pair<T> x,
     switch (op.kind)
     {
     case OP_STENCIL:
-        hipLaunchKernelGGL((k_rbgs<T, OP_STENCIL>), g, BLK, 0, s, L, op, xc,
-                           xo, bc, colour, rg, tl);
+        dispatch<KRbgs<T, OP_STENCIL>>(g, 0, s,
+                                       {L, op, xc, xo, bc, colour, rg, tl});
         break;
     case OP_CONST:
-        hipLaunchKernelGGL((k_rbgs<T, OP_CONST>), g, BLK, 0, s, L, op, xc, xo,
-                           bc, colour, rg, tl);
+        dispatch<KRbgs<T, OP_CONST>>(g, 0, s,
+                                     {L, op, xc, xo, bc, colour, rg, tl});
         break;
     default:
-        hipLaunchKernelGGL((k_rbgs<T, OP_FACES>), g, BLK, 0, s, L, op, xc, xo,
-                           bc, colour, rg, tl);
+        dispatch<KRbgs<T, OP_FACES>>(g, 0, s,
+                                     {L, op, xc, xo, bc, colour, rg, tl});
         break;
     }
 }
@@ -1954,8 +2842,7 @@ void launch_rbgs_slabs(const LevelDims& L, const OpView<T>& op, Pair<T> x,
         g = dim3((n2 + sa.tt - 1)/sa.tt, (n1 + sa.tt - 1)/sa.tt, nslab);
     }
 #define BMG_SLAB(OPK, DD)                                                     \
-    hipLaunchKernelGGL((k_rbgs_slabs<T, OPK, DD>), g, dim3(256), 0, s, L, op, \
-                       x, b, sa)
+    dispatch<KSlabs<T, OPK, DD>>(g, 0, s, {L, op, x, b, sa})
 #define BMG_SLAB_D(OPK)                                                       \
     do {                                                                      \
         if (d == 0) BMG_SLAB(OPK, 0);                                         \
@@ -1974,6 +2861,7 @@ void launch_jacobi(const LevelDims& L, const OpView<T>& op, Pair<T> x,
                    Pair<const T> b, Pair<T> tmp, const Region& rg,
                    hipStream_t s)
 {
+    direct_only("jacobi smoothing");
     if (rg.E[0] <= rg.S[0] || rg.E[1] <= rg.S[1] || rg.E[2] <= rg.S[2]) return;
     const dim3 g = grid3(rg);
     switch (op.kind)
@@ -2004,8 +2892,7 @@ void launch_residual(const LevelDims& L, const OpView<T>& op, Pair<const T> x,
     const dim3 g = tiles_grid(tl);
     if (nblocks) *nblocks = tiles_count(tl);
 #define BMG_RES(OPK, NRM)                                                     \
-    hipLaunchKernelGGL((k_residual<T, OPK, NRM>), g, BLK, 0, s, L, op,   \
-                       x, b, r, tl, partials)
+    dispatch<KResidual<T, OPK, NRM>>(g, 0, s, {L, op, x, b, r, tl, partials})
     if (partials)
     {
         if (op.kind == OP_STENCIL) BMG_RES(OP_STENCIL, true);
@@ -2027,8 +2914,7 @@ void launch_restrict(const LevelDims& Lf, const LevelDims& Lc,
                      hipStream_t s)
 {
     const Tiles tl = pair_tiles(Lc);
-    hipLaunchKernelGGL((k_restrict<T>), tiles_grid(tl), BLK, 0, s, Lf,
-                       Lc, tv, rf, bc, tl);
+    dispatch<KRestrict<T>>(tiles_grid(tl), 0, s, {Lf, Lc, tv, rf, bc, tl});
 }
 
 template <typename T>
@@ -2037,8 +2923,7 @@ void launch_prolong(const LevelDims& Lf, const LevelDims& Lc,
                     hipStream_t s)
 {
     const Tiles tl = pair_tiles(Lf);
-    hipLaunchKernelGGL((k_prolong<T>), tiles_grid(tl), BLK, 0, s, Lf, Lc,
-                       tv, xc, xf, tl);
+    dispatch<KProlong<T>>(tiles_grid(tl), 0, s, {Lf, Lc, tv, xc, xf, tl});
 }
 
 template <typename T>
@@ -2077,8 +2962,8 @@ void launch_bc_fill_snap(const LevelDims& L, const OpView<T>& op, Pair<T> x,
     }
     gr.start[7] = total;
     if (total == 0) return;
-    hipLaunchKernelGGL((k_bc_fill<T>), dim3(total/256), dim3(256), 0, s, L,
-                       op, x, mode, level0, bcvals(bcval), gr, sn);
+    dispatch<KBcFill<T>>(dim3(total/256), 0, s,
+                         {L, op, x, mode, level0, bcvals(bcval), gr, sn});
 }
 
 template <typename T>
@@ -2091,29 +2976,27 @@ void launch_bc_fill(const LevelDims& L, const OpView<T>& op, Pair<T> x,
 template <typename T>
 void launch_fill(Pair<T> x, long ncol, T v, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_fill<T>), dim3((ncol + 255)/256), dim3(256), 0, s,
-                       x.c[0], x.c[1], ncol, v);
+    dispatch<KFill<T>>(dim3((ncol + 255)/256), 0, s, {x.c[0], x.c[1], ncol, v});
 }
 
 template <typename T>
 void launch_ref_to_split(const LevelDims& L, const double* ref, Pair<T> out,
                          hipStream_t s)
 {
-    hipLaunchKernelGGL((k_ref_to_split<T>), grid_padded(L), BLK, 0, s, L, ref,
-                       out);
+    dispatch<KRefToSplit<T>>(grid_padded(L), 0, s, {L, ref, out});
 }
 
 template <typename T>
 void launch_split_to_ref(const LevelDims& L, Pair<const T> in, double* ref,
                          hipStream_t s)
 {
-    hipLaunchKernelGGL((k_split_to_ref<T>), grid_padded(L), BLK, 0, s, L, in,
-                       ref);
+    dispatch<KSplitToRef<T>>(grid_padded(L), 0, s, {L, in, ref});
 }
 
 void launch_stencil_ref_to_split(const LevelDims& L, const double* ref7,
                                  Pair<double> a7, hipStream_t s)
 {
+    direct_only("stencil upload");
     hipLaunchKernelGGL(k_stencil_ref_to_split, grid_padded(L), BLK, 0, s, L,
                        ref7, a7);
 }
@@ -2121,6 +3004,7 @@ void launch_stencil_ref_to_split(const LevelDims& L, const double* ref7,
 void launch_stencil_split_to_ref(const LevelDims& L, Pair<const double> a7,
                                  double* ref7, hipStream_t s)
 {
+    direct_only("stencil download");
     hipLaunchKernelGGL(k_stencil_split_to_ref, grid_padded(L), BLK, 0, s, L,
                        a7, ref7);
 }
@@ -2129,6 +3013,7 @@ template <typename T>
 void launch_materialize_stencil(const LevelDims& L, const OpView<T>& op,
                                 Pair<T> a7, hipStream_t s)
 {
+    direct_only("stencil materialisation");
     if (op.kind == OP_CONST)
         hipLaunchKernelGGL((k_materialize<T, OP_CONST>), grid_interior(L), BLK,
                            0, s, L, op, a7);
@@ -2139,11 +3024,12 @@ void launch_materialize_stencil(const LevelDims& L, const OpView<T>& op,
 
 void launch_pin(const LevelDims& L, Pair<double> a7, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_pin, grid_interior(L), BLK, 0, s, L, a7);
+    dispatch<KPin>(grid_interior(L), 0, s, {L, a7});
 }
 
 void launch_eliminate(const LevelDims& L, Pair<double> a7, hipStream_t s)
 {
+    direct_only("ghost elimination");
     hipLaunchKernelGGL(k_eliminate, grid_interior(L), BLK, 0, s, L, a7);
 }
 
@@ -2151,14 +3037,14 @@ void launch_bc_source(const LevelDims& L, const OpView<double>& op,
                       Pair<double> b, const double* bcval, hipStream_t s)
 {
     if (op.kind == OP_STENCIL)
-        hipLaunchKernelGGL((k_bc_source<OP_STENCIL>), grid_interior(L), BLK, 0,
-                           s, L, op, b, bcvals(bcval));
+        dispatch<KBcSource<OP_STENCIL>>(grid_interior(L), 0, s,
+                                        {L, op, b, bcvals(bcval)});
     else if (op.kind == OP_CONST)
-        hipLaunchKernelGGL((k_bc_source<OP_CONST>), grid_interior(L), BLK, 0,
-                           s, L, op, b, bcvals(bcval));
+        dispatch<KBcSource<OP_CONST>>(grid_interior(L), 0, s,
+                                      {L, op, b, bcvals(bcval)});
     else
-        hipLaunchKernelGGL((k_bc_source<OP_FACES>), grid_interior(L), BLK, 0,
-                           s, L, op, b, bcvals(bcval));
+        dispatch<KBcSource<OP_FACES>>(grid_interior(L), 0, s,
+                                      {L, op, b, bcvals(bcval)});
 }
 
 void launch_lambda_restrict(const LevelDims& Lf, const LevelDims& Lc,
@@ -2166,6 +3052,7 @@ void launch_lambda_restrict(const LevelDims& Lf, const LevelDims& Lc,
                             const double* fw2, const double* lamf,
                             double* lamc, int fd, hipStream_t s)
 {
+    direct_only("coefficient restriction");
     const long n = long(Lc.N[0] + (fd == 0))*(Lc.N[1] + (fd == 1))*
                    (Lc.N[2] + (fd == 2));
     hipLaunchKernelGGL(k_lambda_restrict, dim3((n + 255)/256), dim3(256), 0, s,
@@ -2176,6 +3063,7 @@ void launch_face_coeffs(const LevelDims& L, const double* w0, const double* w1,
                         const double* w2, const double* dl, const double* lam,
                         int fd, Pair<double> wf, hipStream_t s)
 {
+    direct_only("face coefficients");
     const long n = long(L.N[0] + (fd == 0))*(L.N[1] + (fd == 1))*
                    (L.N[2] + (fd == 2));
     hipLaunchKernelGGL(k_face_coeffs, dim3((n + 255)/256), dim3(256), 0, s, L,
@@ -2188,14 +3076,39 @@ void launch_coarse_solve(const LevelDims& L, const OpView<T>& op, Pair<T> x,
                          int n, int level0, const double* bcval,
                          hipStream_t s)
 {
+    switch (coarse_wave_width(n))
+    {
+    case 8:
+        dispatch<KCoarseWave<T, 8>>(dim3(1), 0, s,
+                                    {L, op, x, b, lu, piv, n, level0,
+                                     bcvals(bcval)});
+        return;
+    case 16:
+        dispatch<KCoarseWave<T, 16>>(dim3(1), 0, s,
+                                     {L, op, x, b, lu, piv, n, level0,
+                                      bcvals(bcval)});
+        return;
+    case 32:
+        dispatch<KCoarseWave<T, 32>>(dim3(1), 0, s,
+                                     {L, op, x, b, lu, piv, n, level0,
+                                      bcvals(bcval)});
+        return;
+    case 64:
+        dispatch<KCoarseWave<T, 64>>(dim3(1), 0, s,
+                                     {L, op, x, b, lu, piv, n, level0,
+                                      bcvals(bcval)});
+        return;
+    default:
+        break;
+    }
     if (size_t(n)*(n + 1)*sizeof(double) <= 56*1024)
-        hipLaunchKernelGGL((k_coarse_solve<T, true>), dim3(1), dim3(256),
-                           size_t(n)*(n + 1)*sizeof(double), s, L, op, x, b,
-                           lu, piv, n, level0, bcvals(bcval));
+        dispatch<KCoarse<T, true>>(dim3(1), size_t(n)*(n + 1)*sizeof(double), s,
+                                   {L, op, x, b, lu, piv, n, level0,
+                                    bcvals(bcval)});
     else
-        hipLaunchKernelGGL((k_coarse_solve<T, false>), dim3(1), dim3(256),
-                           size_t(n)*sizeof(double), s, L, op, x, b, lu, piv,
-                           n, level0, bcvals(bcval));
+        dispatch<KCoarse<T, false>>(dim3(1), size_t(n)*sizeof(double), s,
+                                    {L, op, x, b, lu, piv, n, level0,
+                                     bcvals(bcval)});
 }
 
 template <typename T>
@@ -2204,8 +3117,7 @@ void launch_sum_sq(const LevelDims& L, Pair<const T> f, double* partials,
 {
     const Tiles tl = pair_tiles(L);
     *nblocks = tiles_count(tl);
-    hipLaunchKernelGGL((k_reduce_field<T, 0>), tiles_grid(tl), BLK, 0, s,
-                       L, f, tl, partials);
+    dispatch<KReduceField<T, 0>>(tiles_grid(tl), 0, s, {L, f, tl, partials});
 }
 
 template <typename T>
@@ -2214,8 +3126,7 @@ void launch_sum(const LevelDims& L, Pair<const T> f, double* partials,
 {
     const Tiles tl = pair_tiles(L);
     *nblocks = tiles_count(tl);
-    hipLaunchKernelGGL((k_reduce_field<T, 1>), tiles_grid(tl), BLK, 0, s,
-                       L, f, tl, partials);
+    dispatch<KReduceField<T, 1>>(tiles_grid(tl), 0, s, {L, f, tl, partials});
 }
 
 template <typename T>
@@ -2227,26 +3138,26 @@ void launch_norm_factor_terms(const LevelDims& L, const OpView<T>& op,
     const dim3 g = tiles_grid(tl);
     *nblocks = tiles_count(tl);
     if (op.kind == OP_STENCIL)
-        hipLaunchKernelGGL((k_nf_terms<T, OP_STENCIL>), g, BLK, 0, s, L,
-                           op, r, b, y, tl, partials);
+        dispatch<KNfTerms<T, OP_STENCIL>>(g, 0, s,
+                                          {L, op, r, b, y, tl, partials});
     else if (op.kind == OP_CONST)
-        hipLaunchKernelGGL((k_nf_terms<T, OP_CONST>), g, BLK, 0, s, L, op,
-                           r, b, y, tl, partials);
+        dispatch<KNfTerms<T, OP_CONST>>(g, 0, s,
+                                        {L, op, r, b, y, tl, partials});
     else
-        hipLaunchKernelGGL((k_nf_terms<T, OP_FACES>), g, BLK, 0, s, L, op,
-                           r, b, y, tl, partials);
+        dispatch<KNfTerms<T, OP_FACES>>(g, 0, s,
+                                        {L, op, r, b, y, tl, partials});
 }
 
 void launch_reduce_final(const double* partials, int n, double* out,
-                         hipStream_t s)
+                         hipStream_t s, double* hostOut)
 {
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(1024), 0, s, partials, n,
-                       out);
+    dispatch<KReduceFinal>(dim3(1), 0, s, {partials, n, out, hostOut});
 }
 
 void launch_diag(const LevelDims& L, Pair<const double> a7, Pair<double> x,
                  Pair<const double> b, hipStream_t s)
 {
+    direct_only("diagonal solve");
     hipLaunchKernelGGL(k_diag, grid_interior(L), BLK, 0, s, L, a7, x, b);
 }
 
@@ -2257,6 +3168,7 @@ int launch_apply_dots(const LevelDims& L, const OpView<double>& op,
                       Pair<const double> p, Pair<double> q,
                       Pair<const double> r, double* partials, hipStream_t s)
 {
+    direct_only("flexible CG");
     const Tiles tl = pair_tiles(L);
     const dim3 g = tiles_grid(tl);
     if (op.kind == OP_STENCIL)
@@ -2276,6 +3188,7 @@ int launch_fcg_update(const LevelDims& L, const double* sc, Pair<double> x,
                       Pair<const double> q, Pair<float> rf, Pair<double> r0,
                       Pair<double> b0, double* partials, hipStream_t s)
 {
+    direct_only("flexible CG");
     const Tiles tl = pair_tiles(L);
     if (rf.c[0])
         hipLaunchKernelGGL((k_fcg_update<1>), tiles_grid(tl), BLK, 0, s, L, sc,
@@ -2293,6 +3206,7 @@ void launch_split_to_dense(const LevelDims& L, Pair<const double> r,
                            const double* ssDev, double ssHost, float* out,
                            hipStream_t s)
 {
+    direct_only("the learned preconditioner");
     const Tiles tl = pair_tiles(L);
     hipLaunchKernelGGL(k_split_to_dense, tiles_grid(tl), BLK, 0, s, L, r,
                        ssDev, ssHost, out, tl);
@@ -2302,6 +3216,7 @@ void launch_dense_to_split(const LevelDims& L, const float* in,
                            const double* ssDev, double ssHost, double invh,
                            Pair<double> z, hipStream_t s)
 {
+    direct_only("the learned preconditioner");
     const Tiles tl = pair_tiles(L);
     hipLaunchKernelGGL(k_dense_to_split, tiles_grid(tl), BLK, 0, s, L, in,
                        ssDev, ssHost, invh, z, tl);
@@ -2310,6 +3225,7 @@ void launch_dense_to_split(const LevelDims& L, const float* in,
 void launch_add_interior(const LevelDims& L, Pair<double> x,
                          Pair<const double> z, hipStream_t s)
 {
+    direct_only("the learned preconditioner");
     const Tiles tl = pair_tiles(L);
     hipLaunchKernelGGL(k_add_interior, tiles_grid(tl), BLK, 0, s, L, x, z, tl);
 }
@@ -2318,6 +3234,7 @@ template <typename Z>
 int launch_dot_zq(const LevelDims& L, Pair<const Z> z, Pair<const double> q,
                   double* partials, hipStream_t s)
 {
+    direct_only("flexible CG");
     const Tiles tl = pair_tiles(L);
     hipLaunchKernelGGL((k_dot_zq<Z>), tiles_grid(tl), BLK, 0, s, L, z, q, tl,
                        partials);
@@ -2332,6 +3249,7 @@ template <typename Z>
 void launch_update_p(const LevelDims& L, const double* sc, Pair<const Z> z,
                      Pair<double> p, hipStream_t s)
 {
+    direct_only("flexible CG");
     const Tiles tl = pair_tiles(L);
     hipLaunchKernelGGL((k_update_p<Z>), tiles_grid(tl), BLK, 0, s, L, sc, z, p,
                        tl);
@@ -2344,16 +3262,18 @@ template void launch_update_p<float>(const LevelDims&, const double*,
                                      hipStream_t);
 
 void launch_reduce_rows(const double* partials, int n, int nrows, double* out,
-                        hipStream_t s)
+                        hipStream_t s, double* hostOut)
 {
+    direct_only("flexible CG");
     hipLaunchKernelGGL(k_reduce_rows, dim3(nrows), dim3(1024), 0, s, partials,
-                       n, out);
+                       n, out, hostOut);
 }
 
 
 template <typename T>
 void launch_copy(long n, Pair<const T> a, Pair<T> b, hipStream_t s)
 {
+    direct_only("field copy");
     hipLaunchKernelGGL((k_copy<T>), dim3((n + 255)/256), dim3(256), 0, s, n,
                        a, b);
 }
@@ -2363,6 +3283,7 @@ template void launch_copy<double>(long, Pair<const double>, Pair<double>,
 void launch_to_double(long n, Pair<const float> a, Pair<double> b,
                       hipStream_t s)
 {
+    direct_only("field conversion");
     hipLaunchKernelGGL(k_to_double, dim3((n + 255)/256), dim3(256), 0, s, n,
                        a, b);
 }
@@ -2371,22 +3292,19 @@ void launch_add_correction(const LevelDims& L, Pair<double> x,
                            Pair<const float> e, hipStream_t s)
 {
     const Tiles tl = pair_tiles(L);
-    hipLaunchKernelGGL(k_add_correction, tiles_grid(tl), BLK, 0, s, L, x,
-                       e, tl);
+    dispatch<KAddCorrection>(tiles_grid(tl), 0, s, {L, x, e, tl});
 }
 
 void launch_to_float(long ncol, Pair<const double> in, Pair<float> out,
                      hipStream_t s)
 {
-    hipLaunchKernelGGL(k_to_float, dim3((ncol + 255)/256), dim3(256), 0, s,
-                       ncol, in, out);
+    dispatch<KToFloat>(dim3((ncol + 255)/256), 0, s, {ncol, in, out});
 }
 
 template <typename T>
 void launch_set_cell(Pair<T> f, long idx, int colour, T v, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_set_cell<T>), dim3(1), dim3(1), 0, s, f, idx, colour,
-                       v);
+    dispatch<KSetCell<T>>(dim3(1), 0, s, {f, idx, colour, v});
 }
 
 #define BMG_INST(T)                                                           \

@@ -16,6 +16,13 @@ Cases (sizes are parameters; the defaults are the benchmark shapes):
   integer wave numbers ``0 < |m| <= 8`` and amplitude ``|m|^(-5/3)`` (random
   phases in x and z, Neumann-compatible ``cos(pi m_y (y+1)/2)`` in y) plus
   ``1e-3 U[-1,1)`` noise, volume-weighted mean removed, ``b = V f``.
+* ``batch_system``: system ``i`` of a batch of independent Poisson problems
+  on the unit box: shape uniform from 32^3, 48^3, 64^3, 96^3, 128^3 and
+  64x32x128; per axis a boundary pair uniform from Neumann(0),
+  Dirichlet(0) and periodic; ``f`` as in ``channel`` (cosine modes
+  ``0 < |m| <= 8`` with amplitude ``|m|^(-5/3)``, random phases along the
+  first and third axes, ``cos(pi m_y y)`` along the second, plus ``1e-3``
+  noise), the mean removed when no face is Dirichlet; ``b = V f``.
 * ``spheres``: unit cube, all Neumann, density 1000 outside and 1 inside
   ``nspheres`` seeded spheres (radius U[0.03,0.1]) with the interface smoothed
   over 1.5 cells; face coefficient ``lambda`` = arithmetic face average of
@@ -154,6 +161,53 @@ def channel(n=256, seed=0, beta=2.0):
     f -= (vol * f).sum() / vol.sum()
     bc = (PERIODIC, PERIODIC, NEUMANN, NEUMANN, PERIODIC, PERIODIC)
     return Problem("channel", verts, bc, _pad(vol * f))
+
+
+BATCH_SHAPES = ((32, 32, 32), (48, 48, 48), (64, 64, 64), (96, 96, 96),
+                (128, 128, 128), (64, 32, 128))
+BATCH_BCS = (NEUMANN, DIRICHLET, PERIODIC)
+
+
+def batch_layout(i, seed=0):
+    """(shape, six face kinds) of system i of batch seed."""
+    z = raw64(CASE_BATCH, seed, 0, 4, start=4 * i)
+    shape = BATCH_SHAPES[int(z[0] % np.uint64(len(BATCH_SHAPES)))]
+    bc = []
+    for d in range(3):
+        k = BATCH_BCS[int(z[1 + d] % np.uint64(3))]
+        bc += [k, k]
+    return shape, tuple(bc)
+
+
+def batch_system(i, seed=0, shape=None):
+    """System i of a batch of independent unit-box Poisson problems
+    (``shape`` overrides the drawn shape, e.g. for small test sizes)."""
+    drawn, bc = batch_layout(i, seed)
+    nx, ny, nz = shape or drawn
+    verts = [np.arange(q + 1, dtype=np.float64) / q for q in (nx, ny, nz)]
+    xc, yc, zc = ((np.arange(q, dtype=np.float64) + 0.5) / q
+                  for q in (nx, ny, nz))
+    modes = _modes(8)
+    key = 1 + 4 * i
+    ph = uniform01(CASE_BATCH, seed, key, 2 * len(modes)) * (2.0 * math.pi)
+    amp = np.array([(mx * mx + my * my + mz * mz) ** (-5.0 / 6.0)
+                    for (mx, my, mz) in modes])
+    G = np.zeros((9, nx, nz))
+    for my in range(9):
+        sel = [q for q, m in enumerate(modes) if m[1] == my]
+        CX = np.stack([np.cos(2.0 * math.pi * modes[q][0] * xc + ph[2 * q])
+                       for q in sel])
+        CZ = np.stack([np.cos(2.0 * math.pi * modes[q][2] * zc + ph[2 * q + 1])
+                       for q in sel])
+        G[my] = CX.T @ (amp[sel][:, None] * CZ)
+    CY = np.stack([np.cos(math.pi * my * yc) for my in range(9)])
+    f = np.einsum("mxz,my->xyz", G, CY, optimize=True)
+    f += 1e-3 * uniform_pm1(CASE_BATCH, seed, key + 1,
+                            nx * ny * nz).reshape(nx, ny, nz)
+    if DIRICHLET not in bc:
+        f -= f.mean()
+    vol = (1.0 / nx) * (1.0 / ny) * (1.0 / nz)
+    return Problem("batch_system", verts, bc, _pad(vol * f))
 
 
 def _smooth_heaviside(phi, eps):

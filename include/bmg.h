@@ -161,6 +161,33 @@ int bmg_solve(bmg_ctx* ctx, double* b, double* x, int const_matrix,
 int bmg_solve_device(bmg_ctx* ctx, double* d_b, double* d_x,
                      int const_matrix, bmg_stats* stats);
 
+/* ---- Batched solves of independent systems.  Each system is a context
+ * created and given its operator as for bmg_solve_device; the batch runs
+ * MG::solve (MG.C:402-483; krylov none, rbgs smoother) of all of them
+ * together: every kernel of an iteration is one launch over all systems
+ * (k_many), the merged iteration is replayed as a HIP graph, and converged
+ * systems drop out.  Each system's result is bitwise the one bmg_solve_device
+ * returns for it alone.  The contexts stay owned by the caller, must be on
+ * one device and must not be used elsewhere while the batch solves; they
+ * may differ in shape, boundary conditions, precision and cycle options.
+ * No reference counterpart: the reference solves one system per call
+ * (solver.H:280-284). ---- */
+typedef struct bmg_batch bmg_batch;
+int bmg_batch_create(bmg_batch** batch, bmg_ctx* const* ctxs, int n);
+int bmg_batch_destroy(bmg_batch* batch);
+/* d_b[i], d_x[i]: device pointers of system i as in bmg_solve_device;
+ * stats: n entries (solve_ms = the whole batch). */
+int bmg_solve_batch(bmg_batch* batch, double* const* d_b, double* const* d_x,
+                    int const_matrix, bmg_stats* stats);
+/* Launches and virtual workgroups of one merged iteration (0 before the
+ * first solve). */
+int bmg_batch_info(const bmg_batch* batch, int* launches,
+                   long* workgroups);
+/* Launch-grid autotuning of the merged iteration (default on: every
+ * launch takes the fastest of all-workgroups or a persistent grid of 4, 8
+ * or 16 workgroups per CU; off: all workgroups).  Rebuilds the plan. */
+int bmg_batch_set_autotune(bmg_batch* batch, int enable);
+
 /* MG::solve of a diagonal system (sys.diagonal(), MG.C:404-428): level-0
  * stencil A7 (reference layout, only the centre is used after ghost
  * elimination), x = b / A_C on every interior cell, then the level-0

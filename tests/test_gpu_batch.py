@@ -1,0 +1,153 @@
+"""Batched solves (bmg_solve_batch): independent systems of different
+shapes, boundary conditions and precisions solved together.  Every kernel of
+an iteration is one launch over all systems, so each system must still get
+exactly what its own solve gives it: the solution, the returned source and
+the iteration count and residuals of the batched solve are compared bitwise
+with bmg_solve_device of the same system alone (whose FP64 path is itself
+bitwise the oracle's, tests/test_gpu_parity.py).  Also: the launch grids the
+autotuner picks do not change any result, a second solve replays the
+captured iteration, and a system with a zero source drops out at once."""
+import numpy as np
+import pytest
+
+from briscola_amd import bmg, workloads as W
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+N, D, P = bmg.NEUMANN, bmg.DIRICHLET, bmg.PERIODIC
+
+# (batch index for the source and faces, shape, faces or None, precision)
+SYSTEMS = [
+    (0, (16, 16, 16), None, bmg.FP64),
+    (1, (24, 24, 24), None, bmg.MIXED),
+    (2, (12, 20, 10), None, bmg.FP64),
+    (3, (16, 8, 32), None, bmg.MIXED),
+    (4, (20, 16, 12), (P, P, N, N, D, D), bmg.FP64),
+    (5, (16, 16, 16), (N, N, P, P, N, N), bmg.MIXED),
+    (6, (8, 12, 16), (D, D, N, N, P, P), bmg.FP64),
+    (7, (32, 32, 32), (P, P, P, P, P, P), bmg.MIXED),
+    (8, (10, 10, 10), (N, N, N, N, N, N), bmg.FP64),
+    (9, (24, 16, 8), (D, D, D, D, D, D), bmg.MIXED),
+]
+
+
+def problem(i, shape, faces):
+    p = W.batch_system(i, shape=shape)
+    if faces is not None:
+        p.bc = tuple(faces)
+        b = p.b.copy()
+        if D not in faces:
+            inner = b[1:-1, 1:-1, 1:-1]
+            inner -= inner.mean()
+        p.b = b
+    return p
+
+
+def context(p, precision):
+    ctx = bmg.Context(p.vertices, p.bc, p.bc_value, rel_tol=1e-8,
+                      tolerance=0.0, max_iter=200, n_sweeps_pre=2,
+                      n_sweeps_post=2, cycle=bmg.VCYCLE,
+                      coarse_mode=bmg.COARSE_DIRECT, coarse_level=0,
+                      precision=precision)
+    ctx.set_laplacian(p.lam)
+    return ctx
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")
+
+
+def solve_alone(systems):
+    out = []
+    for i, shape, faces, prec in systems:
+        p = problem(i, shape, faces)
+        ctx = context(p, prec)
+        b, x = dev(p.b), dev(np.zeros_like(p.b))
+        st = ctx.solve_device(b.data_ptr(), x.data_ptr())
+        out.append((x.cpu().numpy(), b.cpu().numpy(), st.iters,
+                    st.initial_residual, st.final_residual))
+        ctx.close()
+    return out
+
+
+def solve_batched(systems, autotune=True, repeat=1):
+    probs = [problem(i, shape, faces) for i, shape, faces, _ in systems]
+    ctxs = [context(p, s[3]) for p, s in zip(probs, systems)]
+    batch = bmg.Batch(ctxs, autotune=autotune)
+    runs = []
+    for r in range(repeat):
+        bs = [dev(p.b) for p in probs]
+        xs = [dev(np.zeros_like(p.b)) for p in probs]
+        stats = batch.solve_device([b.data_ptr() for b in bs],
+                                   [x.data_ptr() for x in xs],
+                                   const_matrix=r > 0)
+        runs.append([(x.cpu().numpy(), b.cpu().numpy(), st.iters,
+                      st.initial_residual, st.final_residual)
+                     for x, b, st in zip(xs, bs, stats)])
+    info = batch.info()
+    batch.close()
+    for c in ctxs:
+        c.close()
+    return runs, info
+
+
+def same(a, b):
+    return a.shape == b.shape and np.array_equal(a.view(np.int64),
+                                                 b.view(np.int64))
+
+
+@pytest.fixture(scope="module")
+def alone(gpu):
+    return solve_alone(SYSTEMS)
+
+
+def test_batch_equals_individual_solves(gpu, alone):
+    runs, (launches, wg) = solve_batched(SYSTEMS, repeat=2)
+    # the merged iteration is far fewer launches than the systems' own
+    assert 0 < launches and wg > 0
+    for run in runs:
+        for k, (got, ref) in enumerate(zip(run, alone)):
+            assert got[2] == ref[2], (k, got[2], ref[2])
+            assert got[3] == ref[3] and got[4] == ref[4], k
+            assert same(got[0], ref[0]), f"system {k}: x differs"
+            assert same(got[1], ref[1]), f"system {k}: b differs"
+            assert ref[4] / ref[3] <= 1e-8
+
+
+def test_launch_grids_do_not_change_results(gpu, alone):
+    runs, _ = solve_batched(SYSTEMS[:6], autotune=False)
+    for k, (got, ref) in enumerate(zip(runs[0], alone[:6])):
+        assert got[2] == ref[2] and same(got[0], ref[0]), k
+
+
+def test_zero_source_system_drops_out(gpu):
+    p0 = problem(0, (16, 16, 16), None)
+    p1 = problem(1, (12, 12, 12), (D, D, N, N, N, N))
+    p1.b[...] = 0.0
+    ctxs = [context(p0, bmg.FP64), context(p1, bmg.FP64)]
+    batch = bmg.Batch(ctxs)
+    bs = [dev(p0.b), dev(p1.b)]
+    xs = [dev(np.zeros_like(p0.b)), dev(np.zeros_like(p1.b))]
+    st = batch.solve_device([b.data_ptr() for b in bs],
+                            [x.data_ptr() for x in xs])
+    assert st[0].iters > 0 and st[1].iters == 0
+    assert st[1].initial_residual == 0.0
+    assert not xs[1].abs().max().item()
+    batch.close()
+    for c in ctxs:
+        c.close()
+
+
+def test_batch_rejects_flexible_cg(gpu):
+    p = problem(0, (8, 8, 8), None)
+    ctx = bmg.Context(p.vertices, p.bc, p.bc_value, rel_tol=1e-8,
+                      tolerance=0.0, krylov=bmg.KRYLOV_FCG)
+    ctx.set_laplacian(None)
+    batch = bmg.Batch([ctx])
+    b, x = dev(p.b), dev(np.zeros_like(p.b))
+    with pytest.raises(bmg.BmgError):
+        batch.solve_device([b.data_ptr()], [x.data_ptr()])
+    batch.close()
+    ctx.close()
