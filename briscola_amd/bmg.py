@@ -32,7 +32,7 @@ EXPORTS = [
     "bmg_op_cycle", "bmg_op_sum_sq_residual", "bmg_time_smoother",
     "bmg_set_graphs", "bmg_solve_diagonal", "bmg_batch_create",
     "bmg_batch_destroy", "bmg_solve_batch", "bmg_batch_info",
-    "bmg_batch_set_autotune",
+    "bmg_batch_set_autotune", "bmg_face_flux", "bmg_face_flux_device",
 ]
 
 
@@ -120,6 +120,8 @@ def load():
                                 ctypes.POINTER(Stats)]),
         "bmg_batch_info": (I, [P, IP, ctypes.POINTER(ctypes.c_long)]),
         "bmg_batch_set_autotune": (I, [P, I]),
+        "bmg_face_flux": (I, [P, DP, DP, DP, DP, DP, DP, DP]),
+        "bmg_face_flux_device": (I, [P, P, P, P, P, P, P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -285,6 +287,20 @@ class Context:
         _check(self.lib.bmg_time_smoother(self.ptr, level, precision, reps,
                                           ctypes.byref(ms), ctypes.byref(by)))
         return ms.value, by.value
+
+    def face_flux(self, x, lam=None):
+        """PoissonSolver flux of a level-0 field x (host array, reference
+        layout): three padded lower-face arrays."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = [np.zeros(self.padded(0)) for _ in range(3)]
+        if lam is None:
+            ls = [None, None, None]
+        else:
+            lam = [np.ascontiguousarray(a, dtype=np.float64) for a in lam]
+            ls = [_dp(a) for a in lam]
+        _check(self.lib.bmg_face_flux(self.ptr, _dp(x), *ls,
+                                      *[_dp(a) for a in out]))
+        return out
 
     def set_graphs(self, enable):
         _check(self.lib.bmg_set_graphs(self.ptr, int(enable)))

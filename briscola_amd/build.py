@@ -22,7 +22,8 @@ FLAGS = [
 ]
 
 
-CPP_TEST = os.path.join(ROOT, "tests", "cpp", "test_hipmg")
+CPP_TESTS = [os.path.join(ROOT, "tests", "cpp", t)
+             for t in ("test_hipmg", "test_poisson")]
 
 
 def build(verbose=False):
@@ -35,13 +36,15 @@ def build(verbose=False):
     os.replace(LIB + ".tmp", LIB)
     if verbose:
         print(" ".join(cmd))
-    # C++ program driving the adapter of host/hipMG.H (a GPU test runs it)
-    cmd = ["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"),
-           "-I" + os.path.join(HERE, "host"), CPP_TEST + ".cpp", "-o", CPP_TEST,
-           "-L" + HERE, "-lbmg", "-Wl,-rpath,$ORIGIN/../../briscola_amd"]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError("g++ failed:\n" + r.stderr)
+    # C++ programs driving the adapters of host/ (GPU tests run them)
+    for exe in CPP_TESTS:
+        cmd = ["g++", "-O2", "-std=c++17",
+               "-I" + os.path.join(ROOT, "include"),
+               "-I" + os.path.join(HERE, "host"), exe + ".cpp", "-o", exe,
+               "-L" + HERE, "-lbmg", "-Wl,-rpath,$ORIGIN/../../briscola_amd"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("g++ failed:\n" + r.stderr)
     return LIB
 
 

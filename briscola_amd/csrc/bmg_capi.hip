@@ -1771,6 +1771,75 @@ int bmg_solve_diagonal(bmg_ctx* c, const double* A7, double* b, double* x)
     });
 }
 
+// PoissonSolver flux (defaultPoissonSolver.C:108-121) of a level-0 field.
+static void faceFlux(bmg_ctx* c, const double* x, const double* const lam[3],
+                     double* const flux[3])
+{
+    Level& q = c->lv[0];
+    const double* w[3] = {q.dw[0], q.dw[1], q.dw[2]};
+    for (int d = 0; d < 3; d++)
+        launch_face_flux(q.L, x, lam[d], w, q.ddl[d], d, flux[d], c->stream);
+    HIPCK(hipGetLastError());
+    HIPCK(hipStreamSynchronize(c->stream));
+}
+
+int bmg_face_flux_device(bmg_ctx* c, const double* dx, const double* dlx,
+                         const double* dly, const double* dlz, double* dfx,
+                         double* dfy, double* dfz)
+{
+    return run([&] {
+        if (!c || !dx || !dfx || !dfy || !dfz)
+            throw Error(BMG_ERR_ARG, "null pointer");
+        if ((dlx || dly || dlz) && !(dlx && dly && dlz))
+            throw Error(BMG_ERR_ARG, "give all three lambda arrays or none");
+        const double* lam[3] = {dlx, dly, dlz};
+        double* const flux[3] = {dfx, dfy, dfz};
+        faceFlux(c, dx, lam, flux);
+    });
+}
+
+int bmg_face_flux(bmg_ctx* c, const double* x, const double* lx,
+                  const double* ly, const double* lz, double* fx, double* fy,
+                  double* fz)
+{
+    return run([&] {
+        if (!c || !x || !fx || !fy || !fz)
+            throw Error(BMG_ERR_ARG, "null pointer");
+        if ((lx || ly || lz) && !(lx && ly && lz))
+            throw Error(BMG_ERR_ARG, "give all three lambda arrays or none");
+        Level& q = c->lv[0];
+        const size_t n = size_t(q.L.P[0])*q.L.P[1]*q.L.P[2];
+        std::vector<double*> bufs;
+        auto up = [&](const double* h) -> double* {
+            double* d = devAlloc<double>(n);
+            bufs.push_back(d);
+            if (h)
+                HIPCK(hipMemcpy(d, h, n*sizeof(double), hipMemcpyHostToDevice));
+            else
+                HIPCK(hipMemset(d, 0, n*sizeof(double)));
+            return d;
+        };
+        try
+        {
+            const double* dx = up(x);
+            const double* lam[3] = {lx ? up(lx) : nullptr, ly ? up(ly) : nullptr,
+                                    lz ? up(lz) : nullptr};
+            double* const flux[3] = {up(nullptr), up(nullptr), up(nullptr)};
+            faceFlux(c, dx, lam, flux);
+            double* const out[3] = {fx, fy, fz};
+            for (int d = 0; d < 3; d++)
+                HIPCK(hipMemcpy(out[d], flux[d], n*sizeof(double),
+                                hipMemcpyDeviceToHost));
+        }
+        catch (...)
+        {
+            for (auto& p : bufs) devFree(p);
+            throw;
+        }
+        for (auto& p : bufs) devFree(p);
+    });
+}
+
 static double* fieldPtr(bmg_ctx* c, int which, int level, int colour)
 {
     Level& q = c->lv[level];

@@ -182,6 +182,11 @@ void launch_face_coeffs(const LevelDims& L, const double* w0,
                         const double* w1, const double* w2, const double* dl,
                         const double* lam, int fd, Pair<double> wf,
                         hipStream_t s);
+// Flux through the lower faces normal to fd on level-0 arrays in the
+// reference layout: flux = (lam*((x_lower - x)*dl))*A (lam may be null).
+void launch_face_flux(const LevelDims& L, const double* x, const double* lam,
+                      const double* const w[3], const double* dl, int fd,
+                      double* flux, hipStream_t s);
 // Width of the one-wavefront coarse solve for an n-cell coarse level (a
 // power of two >= n, at least 8), or 0 when n exceeds one wavefront.  With a
 // width N the factors are stored column-major in an N x N array padded with

@@ -390,8 +390,8 @@ template <typename T, int OP, int D>
 __device__ __forceinline__ void rbgs_slabs_body(const Bid bid,
                                                 const LevelDims& L,
                                                 const OpView<T>& op,
-                                                const Pair<T>& x,
-                                                const Pair<const T>& b,
+                                                Pair<T> x,
+                                                Pair<const T> b,
                                                 const SlabArgs<T>& sa)
 {
     constexpr int A1 = D == 0 ? 1 : 0;
@@ -640,9 +640,9 @@ template <typename T, int OP, bool NORM>
 __device__ __forceinline__ void residual_body(const Bid bid,
                                               const LevelDims& L,
                                               const OpView<T>& op,
-                                              const Pair<const T>& x,
-                                              const Pair<const T>& b,
-                                              const Pair<T>& r,
+                                              Pair<const T> x,
+                                              Pair<const T> b,
+                                              Pair<T> r,
                                               const Tiles& tl,
                                               double* partials)
 {
@@ -723,8 +723,8 @@ __device__ __forceinline__ void restrict_body(const Bid bid,
                                               const LevelDims& Lf,
                                               const LevelDims& Lc,
                                               const TransferView<T>& tv,
-                                              const Pair<const T>& rf,
-                                              const Pair<T>& bc,
+                                              Pair<const T> rf,
+                                              Pair<T> bc,
                                               const Tiles& tl)
 {
     int bx, by, bz;
@@ -851,8 +851,8 @@ __device__ __forceinline__ void prolong_body(const Bid bid,
                                              const LevelDims& Lf,
                                              const LevelDims& Lc,
                                              const TransferView<T>& tv,
-                                             const Pair<const T>& xc,
-                                             const Pair<T>& xf,
+                                             Pair<const T> xc,
+                                             Pair<T> xf,
                                              const Tiles& tl)
 {
     int bx, by, bz;
@@ -1200,7 +1200,7 @@ struct SnapPtrs {
 template <typename T>
 __device__ __forceinline__ void bc_fill_body(const Bid bid, const LevelDims& L,
                                              const OpView<T>& op,
-                                             const Pair<T>& x, int mode,
+                                             Pair<T> x, int mode,
                                              int level0, const BcVals& bv,
                                              const BcGroups& gr,
                                              const SnapPtrs<T>& sn)
@@ -1361,7 +1361,7 @@ template <typename T>
 __device__ __forceinline__ void ref_to_split_body(const Bid bid,
                                                   const LevelDims& L,
                                                   const double* ref,
-                                                  const Pair<T>& out)
+                                                  Pair<T> out)
 {
     const int I = bid.z;
     const int J = bid.y*BY + threadIdx.y;
@@ -1372,7 +1372,7 @@ __device__ __forceinline__ void ref_to_split_body(const Bid bid,
     {
         const int K = 2*p + h;
         if (K >= L.P[2]) continue;
-        out.c[(I + J + K) & 1][idx] =
+        col(out, (I + J + K) & 1)[idx] =
             T(ref[((long)I*L.P[1] + J)*L.P[2] + K]);
     }
 }
@@ -1412,7 +1412,7 @@ struct KRefToSplit
 template <typename T>
 __device__ __forceinline__ void split_to_ref_body(const Bid bid,
                                                   const LevelDims& L,
-                                                  const Pair<const T>& in,
+                                                  Pair<const T> in,
                                                   double* ref)
 {
     const int I = bid.z;
@@ -1425,7 +1425,7 @@ __device__ __forceinline__ void split_to_ref_body(const Bid bid,
         const int K = 2*p + h;
         if (K >= L.P[2]) continue;
         ref[((long)I*L.P[1] + J)*L.P[2] + K] =
-            double(in.c[(I + J + K) & 1][idx]);
+            double(col(in, (I + J + K) & 1)[idx]);
     }
 }
 
@@ -1543,7 +1543,7 @@ __device__ __forceinline__ bool is_cell_zero(const LevelDims& L, int I, int J,
 // setSingularityConstraint (solver.C:200-231): A(0,0,0) = diag(1) and every
 // coupling to global cell 0 removed.
 __device__ __forceinline__ void pin_body(const Bid bid, const LevelDims& L,
-                                         const Pair<double>& a7)
+                                         Pair<double> a7)
 {
     const int I = 1 + bid.z;
     const int J = 1 + bid.y*BY + threadIdx.y;
@@ -1556,7 +1556,7 @@ __device__ __forceinline__ void pin_body(const Bid bid, const LevelDims& L,
     {
         const int K = 2*p + h;
         if (K < 1 || K > L.N[2]) continue;
-        double* a = a7.c[(I + J + K) & 1];
+        double* a = col(a7, (I + J + K) & 1);
         if (I == 1 && J == 1 && K == 1)
         {
             a[idx] = 1.0;
@@ -1623,7 +1623,7 @@ template <int OP>
 __device__ __forceinline__ void bc_source_body(const Bid bid,
                                                const LevelDims& L,
                                                const OpView<double>& op,
-                                               const Pair<double>& b,
+                                               Pair<double> b,
                                                const BcVals& bv)
 {
     const int I = 1 + bid.z;
@@ -1642,7 +1642,7 @@ __device__ __forceinline__ void bc_source_body(const Bid bid,
         const bool on[6] = {I == 1, I == L.N[0], J == 1, J == L.N[1],
                             K == 1, K == L.N[2]};
         const int pc[3] = {I, J, K};
-        double v = b.c[colour][idx];
+        double v = col(b, colour)[idx];
         for (int f = 0; f < 6; f++)
         {
             if (!on[f] || L.bc[f] == PER) continue;
@@ -1655,7 +1655,7 @@ __device__ __forceinline__ void bc_source_body(const Bid bid,
             else
                 v = v - g*2.0*bv.v[f];
         }
-        b.c[colour][idx] = v;
+        col(b, colour)[idx] = v;
     }
 }
 
@@ -1746,6 +1746,36 @@ __global__ void k_face_coeffs(LevelDims L, const double* w0, const double* w1,
     wf.c[(I + J + K) & 1][fd*L.ncol + cidx(L, I, J, K)] = v;
 }
 
+// Flux of x through the lower faces normal to axis fd, reference layout
+// (PoissonSolver flux, defaultPoissonSolver.C:108-121): the face gradient
+// along the outward normal of the lower face, g = (x_lower - x_cell)*delta
+// (linearFaceGradientScheme.C:44-53), times lambda when given, times the
+// face area (the product of the two in-face widths, the magnitude of the
+// hexa face-area normal of a rectilinear cell).  Faces 1..N+1 along fd,
+// interior cells along the other axes.
+__global__ void k_face_flux(LevelDims L, const double* x, const double* lam,
+                            const double* w0, const double* w1,
+                            const double* w2, const double* dl, int fd,
+                            double* flux)
+{
+    const int lim0 = L.N[0] + (fd == 0), lim1 = L.N[1] + (fd == 1),
+              lim2 = L.N[2] + (fd == 2);
+    const long t = (long)blockIdx.x*blockDim.x + threadIdx.x;
+    if (t >= (long)lim0*lim1*lim2) return;
+    const int I = 1 + int(t/((long)lim1*lim2));
+    const int J = 1 + int((t/lim2) % lim1);
+    const int K = 1 + int(t % lim2);
+    const long c = ((long)I*L.P[1] + J)*L.P[2] + K;
+    const long o = fd == 0 ? (long)L.P[1]*L.P[2] : (fd == 1 ? L.P[2] : 1);
+    const int pc = fd == 0 ? I : (fd == 1 ? J : K);
+    const double d = x[c - o] - x[c];
+    const double g = d*dl[pc];
+    const double fa = fd == 0 ? w1[J]*w2[K] : (fd == 1 ? w0[I]*w2[K]
+                                                        : w0[I]*w1[J]);
+    const double v = lam ? lam[c]*g : g;
+    flux[c] = v*fa;
+}
+
 // ---------------------------------------------------------------------------
 // Coarsest-level direct solve, one workgroup (APLU.C:91-128 with rhsSource,
 // linearSystemAggregation.C:336-430).  The permutation is applied first and
@@ -1758,8 +1788,8 @@ template <typename T, bool LDS>
 __device__ __forceinline__ void coarse_solve_body(const Bid bid,
                                                   const LevelDims& L,
                                                   const OpView<T>& op,
-                                                  const Pair<T>& x,
-                                                  const Pair<const T>& b,
+                                                  Pair<T> x,
+                                                  Pair<const T> b,
                                                   const double* lug,
                                                   const int* piv, int n,
                                                   int level0, const BcVals& bv)
@@ -1782,7 +1812,7 @@ __device__ __forceinline__ void coarse_solve_body(const Bid bid,
         const int I = 1 + q/(N1*N2), J = 1 + (q/N2) % N1, K = 1 + q % N2;
         const int colour = (I + J + K) & 1;
         const int idx = cidx(L, I, J, K);
-        double v = double(b.c[colour][idx]);
+        double v = double(col(b, colour)[idx]);
         for (int s = 1; s < 7; s++)
         {
             const int g[3] = {I + off[s][0], J + off[s][1], K + off[s][2]};
@@ -1794,7 +1824,7 @@ __device__ __forceinline__ void coarse_solve_body(const Bid bid,
             if (out)
             {
                 const double a = double(op.a7[colour][s*nci(L) + idx]);
-                const double t = a*double(x.c[colour ^ 1][cidx(L, g[0], g[1], g[2])]);
+                const double t = a*double(col(x, colour ^ 1)[cidx(L, g[0], g[1], g[2])]);
                 v = v - t;
             }
         }
@@ -1918,8 +1948,8 @@ template <typename T, int N>
 __device__ __forceinline__ void coarse_solve_wave_body(const Bid bid,
                                                        const LevelDims& L,
                                                        const OpView<T>& op,
-                                                       const Pair<T>& x,
-                                                       const Pair<const T>& b,
+                                                       Pair<T> x,
+                                                       Pair<const T> b,
                                                        const double* luc,
                                                        const int* perm, int n,
                                                        int level0,
@@ -1939,7 +1969,7 @@ __device__ __forceinline__ void coarse_solve_wave_body(const Bid bid,
         const int I = 1 + q/(N1*N2), J = 1 + (q/N2) % N1, K = 1 + q % N2;
         const int colour = (I + J + K) & 1;
         const int idx = cidx(L, I, J, K);
-        v = double(b.c[colour][idx]);
+        v = double(col(b, colour)[idx]);
         for (int s = 1; s < 7; s++)
         {
             const int g[3] = {I + off[s][0], J + off[s][1], K + off[s][2]};
@@ -1951,7 +1981,7 @@ __device__ __forceinline__ void coarse_solve_wave_body(const Bid bid,
             if (out)
             {
                 const double a = double(op.a7[colour][s*nci(L) + idx]);
-                const double t = a*double(x.c[colour ^ 1][cidx(L, g[0], g[1], g[2])]);
+                const double t = a*double(col(x, colour ^ 1)[cidx(L, g[0], g[1], g[2])]);
                 v = v - t;
             }
         }
@@ -2042,7 +2072,7 @@ struct KCoarseWave
 template <typename T, int MODE>
 __device__ __forceinline__ void reduce_field_body(const Bid bid,
                                                   const LevelDims& L,
-                                                  const Pair<const T>& f,
+                                                  Pair<const T> f,
                                                   const Tiles& tl,
                                                   double* partials)
 {
@@ -2103,8 +2133,8 @@ template <typename T, int OP>
 __device__ __forceinline__ void nf_terms_body(const Bid bid,
                                               const LevelDims& L,
                                               const OpView<T>& op,
-                                              const Pair<const T>& r,
-                                              const Pair<const T>& b, double y,
+                                              Pair<const T> r,
+                                              Pair<const T> b, double y,
                                               const Tiles& tl,
                                               double* partials)
 {
@@ -2518,8 +2548,8 @@ __global__ __launch_bounds__(BX*BY) void k_diag(LevelDims L, Pair<const double> 
 
 __device__ __forceinline__ void add_correction_body(const Bid bid,
                                                     const LevelDims& L,
-                                                    const Pair<double>& x,
-    const Pair<const float>& e, const Tiles& tl)
+                                                    Pair<double> x,
+    Pair<const float> e, const Tiles& tl)
 {
     int bx, by, bz;
     if (!tile_of(tl, bid.x, bx, by, bz)) return;
@@ -2565,8 +2595,8 @@ struct KAddCorrection
 
 
 __device__ __forceinline__ void to_float_body(const Bid bid, long n,
-                                              const Pair<const double>& in,
-                                              const Pair<float>& out)
+                                              Pair<const double> in,
+                                              Pair<float> out)
 {
     const long t = (long)bid.x*blockDim.x + threadIdx.x;
     if (t < n)
@@ -2605,10 +2635,10 @@ struct KToFloat
 
 
 template <typename T>
-__device__ __forceinline__ void set_cell_body(const Bid bid, const Pair<T>& f,
+__device__ __forceinline__ void set_cell_body(const Bid bid, Pair<T> f,
                                               long idx, int colour, T v)
 {
-    f.c[colour][idx] = v;
+    col(f, colour)[idx] = v;
 }
 
 template <typename T>
@@ -3068,6 +3098,17 @@ void launch_face_coeffs(const LevelDims& L, const double* w0, const double* w1,
                    (L.N[2] + (fd == 2));
     hipLaunchKernelGGL(k_face_coeffs, dim3((n + 255)/256), dim3(256), 0, s, L,
                        w0, w1, w2, dl, lam, fd, wf);
+}
+
+void launch_face_flux(const LevelDims& L, const double* x, const double* lam,
+                      const double* const w[3], const double* dl, int fd,
+                      double* flux, hipStream_t s)
+{
+    direct_only("face flux");
+    const long n = long(L.N[0] + (fd == 0))*(L.N[1] + (fd == 1))*
+                   (L.N[2] + (fd == 2));
+    hipLaunchKernelGGL(k_face_flux, dim3((n + 255)/256), dim3(256), 0, s, L,
+                       x, lam, w[0], w[1], w[2], dl, fd, flux);
 }
 
 template <typename T>

@@ -74,13 +74,16 @@ public:
 };
 
 // What MG reads from fvMesh: level-0 shape, rectilinear vertex coordinates
-// and the boundary condition of every face of x (-x,+x,-y,+y,-z,+z).
+// and the boundary condition of every face of x (-x,+x,-y,+y,-z,+z); the
+// Poisson solver also reads the cell volumes (level 0, ghost-padded, as the
+// mesh metrics hold them, fvMeshMetrics.C:62-100).
 struct fvMesh
 {
     int N[3];
     std::vector<double> vertices[3];
     int bc[6];                 // BMG_NEUMANN / BMG_DIRICHLET / BMG_PERIODIC
     double bcValue[6] = {0, 0, 0, 0, 0, 0};
+    std::vector<double> cellVolumes;
 };
 
 // linearSystem<stencil,scalar,colocated>: unknown x, deep stencil field A
@@ -109,6 +112,66 @@ public:
     virtual void solve(linearSystem& sys, bool constMatrix = false) = 0;
 };
 
+inline void check(int code)
+{
+    if (code != BMG_OK) throw FatalError(bmg_last_error());
+}
+
+// Solver options from the MG dictionary keys and defaults (MG.C:257-304,
+// solver.C:114-123), plus this solver's "precision" (FP64 | mixed).
+inline bmg_opts optsFromDict(const dictionary& dict)
+{
+    bmg_opts o;
+    bmg_default_opts(&o);
+    o.rel_tol = dict.lookupScalar("relTol");
+    o.tolerance = dict.lookupScalar("tolerance");
+    o.min_iter = dict.lookupOrDefault("minIter", 0);
+    o.max_iter = dict.lookupOrDefault("maxIter", 999);
+    o.n_sweeps_pre = dict.lookupOrDefault("nSweepsPre", 0);
+    o.n_sweeps_post = dict.lookupOrDefault("nSweepsPost", 2);
+    const std::string cyc = dict.lookupOrDefault("cycleType", "F");
+    if (cyc == "V") o.cycle = BMG_VCYCLE;
+    else if (cyc == "W") o.cycle = BMG_WCYCLE;
+    else if (cyc == "F") o.cycle = BMG_FCYCLE;
+    else throw FatalError("Unknown cycleType " + cyc);
+    const std::string cm = dict.lookupOrDefault("coarseMode", "direct");
+    if (cm == "direct") o.coarse_mode = BMG_COARSE_DIRECT;
+    else if (cm == "smooth") o.coarse_mode = BMG_COARSE_SMOOTH;
+    else throw FatalError("Unknown coarseMode " + cm);
+    o.coarse_level = dict.lookupOrDefault("coarseLevel", 0);
+    const std::string sm = dict.lookupOrDefault("smoother", "rbgs");
+    if (sm == "rbgs") o.smoother = BMG_RBGS;
+    else if (sm == "jac") o.smoother = BMG_JAC;
+    else throw FatalError("Unsupported smoother type " + sm);
+    if (dict.lookupOrDefault("prolong", "linear") != "linear" ||
+        dict.lookupOrDefault("restrict", "linear") != "linear")
+        throw FatalError("only the linear transfer schemes are provided");
+    const std::string prec = dict.lookupOrDefault("precision", "FP64");
+    o.precision = prec == "mixed" ? BMG_MIXED : BMG_FP64;
+    return o;
+}
+
+// A solver context on the mesh's level 0 (vertices copied into verts).
+inline bmg_ctx* createContext(const fvMesh& mesh, const bmg_opts& opts,
+                              std::vector<double> (&verts)[3])
+{
+    bmg_grid g;
+    for (int d = 0; d < 3; d++)
+    {
+        g.n[d] = mesh.N[d];
+        verts[d] = mesh.vertices[d];
+        g.vertices[d] = verts[d].data();
+    }
+    for (int f = 0; f < 6; f++)
+    {
+        g.bc[f] = mesh.bc[f];
+        g.bc_value[f] = mesh.bcValue[f];
+    }
+    bmg_ctx* ctx = nullptr;
+    check(bmg_create(&ctx, &g, &opts));
+    return ctx;
+}
+
 class MG : public solver
 {
     bmg_ctx* ctx_ = nullptr;
@@ -118,58 +181,14 @@ class MG : public solver
     bool verbose_;
     solverStats last_;
 
-    static void check(int code)
-    {
-        if (code != BMG_OK) throw FatalError(bmg_last_error());
-    }
-
 public:
     static constexpr const char* typeName = "MG";
 
     // MG(const dictionary&, const fvMesh&) -- MG.C:257-399
     MG(const dictionary& dict, const fvMesh& mesh, bool verbose = true)
-    : verbose_(verbose)
+    : opts_(optsFromDict(dict)), verbose_(verbose)
     {
-        bmg_default_opts(&opts_);
-        opts_.rel_tol = dict.lookupScalar("relTol");
-        opts_.tolerance = dict.lookupScalar("tolerance");
-        opts_.min_iter = dict.lookupOrDefault("minIter", 0);
-        opts_.max_iter = dict.lookupOrDefault("maxIter", 999);
-        opts_.n_sweeps_pre = dict.lookupOrDefault("nSweepsPre", 0);
-        opts_.n_sweeps_post = dict.lookupOrDefault("nSweepsPost", 2);
-        const std::string cyc = dict.lookupOrDefault("cycleType", "F");
-        if (cyc == "V") opts_.cycle = BMG_VCYCLE;
-        else if (cyc == "W") opts_.cycle = BMG_WCYCLE;
-        else if (cyc == "F") opts_.cycle = BMG_FCYCLE;
-        else throw FatalError("Unknown cycleType " + cyc);
-        const std::string cm = dict.lookupOrDefault("coarseMode", "direct");
-        if (cm == "direct") opts_.coarse_mode = BMG_COARSE_DIRECT;
-        else if (cm == "smooth") opts_.coarse_mode = BMG_COARSE_SMOOTH;
-        else throw FatalError("Unknown coarseMode " + cm);
-        opts_.coarse_level = dict.lookupOrDefault("coarseLevel", 0);
-        const std::string sm = dict.lookupOrDefault("smoother", "rbgs");
-        if (sm == "rbgs") opts_.smoother = BMG_RBGS;
-        else if (sm == "jac") opts_.smoother = BMG_JAC;
-        else throw FatalError("Unsupported smoother type " + sm);
-        if (dict.lookupOrDefault("prolong", "linear") != "linear" ||
-            dict.lookupOrDefault("restrict", "linear") != "linear")
-            throw FatalError("only the linear transfer schemes are provided");
-        const std::string prec = dict.lookupOrDefault("precision", "FP64");
-        opts_.precision = prec == "mixed" ? BMG_MIXED : BMG_FP64;
-
-        bmg_grid g;
-        for (int d = 0; d < 3; d++)
-        {
-            g.n[d] = mesh.N[d];
-            verts_[d] = mesh.vertices[d];
-            g.vertices[d] = verts_[d].data();
-        }
-        for (int f = 0; f < 6; f++)
-        {
-            g.bc[f] = mesh.bc[f];
-            g.bc_value[f] = mesh.bcValue[f];
-        }
-        check(bmg_create(&ctx_, &g, &opts_));
+        ctx_ = createContext(mesh, opts_, verts_);
     }
 
     ~MG() override { if (ctx_) bmg_destroy(ctx_); }

@@ -188,7 +188,27 @@ int bmg_batch_info(const bmg_batch* batch, int* launches,
  * or 16 workgroups per CU; off: all workgroups).  Rebuilds the plan. */
 int bmg_batch_set_autotune(bmg_batch* batch, int enable);
 
-/* MG::solve of a diagonal system (sys.diagonal(), MG.C:404-428): level-0
+/* Flux of a level-0 field through every face, as PoissonSolver computes it
+ * after a solve (defaultPoissonSolver.C:108-121): for axis d and padded cell
+ * c whose lower face lies in the face range (1 <= c_d <= N_d + 1, interior
+ * along the other axes), flux_d[c] = (lambda_d[c] * ((x[c - e_d] - x[c]) *
+ * delta_d)) * A_d -- the face gradient along the outward normal of the lower
+ * face (linearFaceGradientScheme.C:44-53) with delta_d = 1/|cc_c -
+ * cc_{c-e_d}|, times the face coefficient, times the face area.  lambda
+ * arrays (the padded lower-face layout of bmg_set_operator_laplacian) may
+ * all be null (lambda = 1, no product).  Reference layout: bmg_face_flux
+ * takes host arrays and sets every other flux entry to 0;
+ * bmg_face_flux_device takes device pointers and leaves other entries as
+ * they are.  No counterpart in the MG solver itself. */
+int bmg_face_flux(bmg_ctx* ctx, const double* x, const double* lambda_x,
+                  const double* lambda_y, const double* lambda_z,
+                  double* flux_x, double* flux_y, double* flux_z);
+int bmg_face_flux_device(bmg_ctx* ctx, const double* d_x,
+                         const double* d_lambda_x, const double* d_lambda_y,
+                         const double* d_lambda_z, double* d_flux_x,
+                         double* d_flux_y, double* d_flux_z);
+
+/* MG::solve of a diagonal system (sys.diagonal(), MG.C:404-428): level-0/* MG::solve of a diagonal system (sys.diagonal(), MG.C:404-428): level-0
  * stencil A7 (reference layout, only the centre is used after ghost
  * elimination), x = b / A_C on every interior cell, then the level-0
  * boundary update.  b and x as in bmg_solve. */

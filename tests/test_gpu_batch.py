@@ -18,17 +18,18 @@ torch = pytest.importorskip("torch")
 
 N, D, P = bmg.NEUMANN, bmg.DIRICHLET, bmg.PERIODIC
 
-# (batch index for the source and faces, shape, faces or None, precision)
+# (batch index for the source and faces, shape, faces or None, precision);
+# every size halves down to at most 3 through even sizes
 SYSTEMS = [
     (0, (16, 16, 16), None, bmg.FP64),
     (1, (24, 24, 24), None, bmg.MIXED),
-    (2, (12, 20, 10), None, bmg.FP64),
+    (2, (12, 24, 6), None, bmg.FP64),
     (3, (16, 8, 32), None, bmg.MIXED),
-    (4, (20, 16, 12), (P, P, N, N, D, D), bmg.FP64),
+    (4, (24, 16, 12), (P, P, N, N, D, D), bmg.FP64),
     (5, (16, 16, 16), (N, N, P, P, N, N), bmg.MIXED),
     (6, (8, 12, 16), (D, D, N, N, P, P), bmg.FP64),
     (7, (32, 32, 32), (P, P, P, P, P, P), bmg.MIXED),
-    (8, (10, 10, 10), (N, N, N, N, N, N), bmg.FP64),
+    (8, (12, 12, 12), (N, N, N, N, N, N), bmg.FP64),
     (9, (24, 16, 8), (D, D, D, D, D, D), bmg.MIXED),
 ]
 
