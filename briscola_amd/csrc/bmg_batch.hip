@@ -10,10 +10,10 @@
 // launch of k_many over those systems.  Every context keeps its own order
 // and the systems share no memory, so each system computes exactly what its
 // own solve computes.  The merged iteration is captured in a HIP graph and
-// replayed; systems that have converged are switched off by a flag the
-// kernels read, so the graph never changes during a solve.  The launch
-// grid of every merged launch (all its workgroups, or a fixed persistent
-// grid walking them) is chosen by timing the candidates once per plan.
+// replayed; systems that have converged are dropped from the shares the
+// kernels read from device memory, so the graph never changes during a
+// solve.  Every merged launch is a persistent grid walking its shares; its
+// size is chosen by timing the candidates once per plan.
 #include "bmg_context.h"
 
 #include <algorithm>
@@ -45,20 +45,25 @@ struct Merged
     size_t shm = 0;
     size_t argSize = 0, align = 8;
     std::vector<unsigned char> args;   // argument structs, argSize apart
-    std::vector<BatchItem> items;
-    unsigned total = 0;                // virtual workgroups
-    size_t argsOff = 0, itemsOff = 0;  // offsets in the plan's buffer
+    std::vector<BatchItem> items;      // every system's share
+    std::vector<int> sys;              // the system of each share
+    unsigned total = 0;                // virtual workgroups, all systems
+    size_t argsOff = 0, itemsOff = 0;  // offsets in the plan's regions
     unsigned grid = 0;                 // workgroups launched
 };
 
 }  // namespace
 
 // A merged launch sequence with its device copy of argument structs and
-// system shares.
+// system shares.  The device buffer holds the argument structs, then the
+// shares region: every launch's shares followed by one BatchState per
+// launch.  Dropping systems rewrites the shares region only.
 struct BatchPlan
 {
     std::vector<Merged> launches;
     unsigned char* dev = nullptr;
+    unsigned char* staging = nullptr;   // pinned copy of the shares region
+    size_t sharesOff = 0, sharesBytes = 0, statesOff = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 
@@ -67,14 +72,51 @@ struct BatchPlan
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         if (dev) (void)hipFree(dev);
+        if (staging) (void)hipHostFree(staging);
     }
 
-    // Lockstep merge of per-system sequences; the active-flag index of a
-    // system is the index of its sequence.
-    void build(std::vector<std::vector<LaunchRecord>>& seqs)
+    // Merge of per-system sequences; a system's index is the index of its
+    // sequence.  When every sequence's tags are non-decreasing, the
+    // sequences are cut into segments of equal tag and the segments of each
+    // tag are merged with each other, tags in increasing order; otherwise
+    // the whole sequences are merged at once.  Within a merge, repeatedly
+    // the kernel at the head of the most sequences (the longest remainder
+    // breaking ties) is taken from all of them.  Launch grids start
+    // persistent at `grid` workgroups.
+    void build(std::vector<std::vector<LaunchRecord>>& seqs, unsigned grid)
     {
         const size_t n = seqs.size();
-        std::vector<size_t> head(n, 0);
+        bool tagged = true;
+        std::vector<long> tags;
+        for (const auto& q : seqs)
+            for (size_t i = 0; i < q.size(); i++)
+            {
+                if (i > 0 && q[i].tag < q[i - 1].tag) tagged = false;
+                tags.push_back(q[i].tag);
+            }
+        std::sort(tags.begin(), tags.end());
+        tags.erase(std::unique(tags.begin(), tags.end()), tags.end());
+        if (!tagged) tags.assign(1, 0);
+        std::vector<size_t> head(n, 0), end(n, 0);
+        for (long tag : tags)
+        {
+            for (size_t s = 0; s < n; s++)
+            {
+                end[s] = head[s];
+                while (end[s] < seqs[s].size() &&
+                       (!tagged || seqs[s][end[s]].tag == tag))
+                    end[s]++;
+            }
+            mergeSegment(seqs, head, end);
+        }
+        layout(n, grid);
+    }
+
+    void mergeSegment(std::vector<std::vector<LaunchRecord>>& seqs,
+                      std::vector<size_t>& head,
+                      const std::vector<size_t>& end)
+    {
+        const size_t n = seqs.size();
         for (;;)
         {
             // kernels at the heads: number of sequences, longest remainder
@@ -82,11 +124,11 @@ struct BatchPlan
                      std::pair<size_t, size_t>> heads;
             for (size_t s = 0; s < n; s++)
             {
-                if (head[s] >= seqs[s].size()) continue;
+                if (head[s] >= end[s]) continue;
                 const LaunchRecord& r = seqs[s][head[s]];
                 auto& h = heads[{r.kind, r.shm}];
                 h.first++;
-                h.second = std::max(h.second, seqs[s].size() - head[s]);
+                h.second = std::max(h.second, end[s] - head[s]);
             }
             if (heads.empty()) break;
             auto best = heads.begin();
@@ -97,7 +139,7 @@ struct BatchPlan
             m.shm = best->first.second;
             for (size_t s = 0; s < n; s++)
             {
-                if (head[s] >= seqs[s].size()) continue;
+                if (head[s] >= end[s]) continue;
                 const LaunchRecord& r = seqs[s][head[s]];
                 if (r.kind != m.kind || r.shm != m.shm) continue;
                 head[s]++;
@@ -114,69 +156,118 @@ struct BatchPlan
                 it.gx = r.grid[0];
                 it.gy = r.grid[1];
                 it.n = wg;
-                it.sys = int(s);
+                it.arg = int(m.items.size());
                 m.items.push_back(it);
+                m.sys.push_back(int(s));
                 m.args.insert(m.args.end(), r.args.begin(), r.args.end());
                 // each share starts at a multiple of 8 (its XCD order)
                 m.total += unsigned(alignUp(wg, 8));
             }
             if (!m.items.empty()) launches.push_back(std::move(m));
         }
+    }
+
+    // Device copies of the argument structs and shares (all systems on).
+    void layout(size_t n, unsigned grid)
+    {
         size_t bytes = 0;
         for (auto& m : launches)
         {
             bytes = alignUp(bytes, 256);
             m.argsOff = bytes;
             bytes += m.args.size();
-            bytes = alignUp(bytes, 256);
-            m.itemsOff = bytes;
-            bytes += m.items.size()*sizeof(BatchItem);
-            m.grid = m.total;
+            m.grid = std::min(grid, m.total);
         }
-        std::vector<unsigned char> host(std::max<size_t>(bytes, 1), 0);
+        sharesOff = alignUp(bytes, 256);
+        size_t sh = 0;
         for (auto& m : launches)
         {
-            std::memcpy(host.data() + m.argsOff, m.args.data(), m.args.size());
-            std::memcpy(host.data() + m.itemsOff, m.items.data(),
-                        m.items.size()*sizeof(BatchItem));
+            sh = alignUp(sh, 16);
+            m.itemsOff = sh;
+            sh += m.items.size()*sizeof(BatchItem);
         }
-        HIPCK(hipMalloc(&dev, host.size()));
+        statesOff = alignUp(sh, 16);
+        sharesBytes = statesOff + launches.size()*sizeof(BatchState);
+        HIPCK(hipHostMalloc(&staging, std::max<size_t>(sharesBytes, 1)));
+        std::vector<unsigned char> host(sharesOff, 0);
+        for (auto& m : launches)
+            std::memcpy(host.data() + m.argsOff, m.args.data(), m.args.size());
+        HIPCK(hipMalloc(&dev, sharesOff + std::max<size_t>(sharesBytes, 1)));
         HIPCK(hipMemcpy(dev, host.data(), host.size(), hipMemcpyHostToDevice));
+        std::vector<char> all(n, 1);
+        writeShares(all);
+        HIPCK(hipMemcpy(dev + sharesOff, staging, sharesBytes,
+                        hipMemcpyHostToDevice));
     }
 
-    void launch(const Merged& m, const int* active, hipStream_t s) const
+    // Shares of the systems that are on, renumbered from 0, into staging.
+    void writeShares(const std::vector<char>& on)
     {
+        BatchState* st = reinterpret_cast<BatchState*>(staging + statesOff);
+        for (size_t i = 0; i < launches.size(); i++)
+        {
+            const Merged& m = launches[i];
+            BatchItem* out = reinterpret_cast<BatchItem*>(staging + m.itemsOff);
+            int k = 0;
+            unsigned total = 0;
+            for (size_t j = 0; j < m.items.size(); j++)
+            {
+                if (!on[m.sys[j]]) continue;
+                BatchItem it = m.items[j];
+                it.start = total;
+                total += unsigned(alignUp(it.n, 8));
+                out[k++] = it;
+            }
+            st[i].nitems = k;
+            st[i].total = total;
+        }
+    }
+
+    // Drops the systems that are off from every launch; the copy runs on
+    // stream s before whatever is enqueued next.  The caller synchronises
+    // s before the next call (the staging buffer is reused).
+    void setActive(const std::vector<char>& on, hipStream_t s)
+    {
+        writeShares(on);
+        HIPCK(hipMemcpyAsync(dev + sharesOff, staging, sharesBytes,
+                             hipMemcpyHostToDevice, s));
+    }
+
+    void launch(size_t i, hipStream_t s) const
+    {
+        const Merged& m = launches[i];
         BatchLaunch b;
         b.args = dev + m.argsOff;
-        b.items = reinterpret_cast<const BatchItem*>(dev + m.itemsOff);
-        b.nitems = int(m.items.size());
-        b.total = m.total;
-        b.active = active;
-        b.grid = std::max(1u, std::min(m.grid, m.total));
+        b.items = reinterpret_cast<const BatchItem*>(dev + sharesOff +
+                                                     m.itemsOff);
+        b.state = reinterpret_cast<const BatchState*>(dev + sharesOff +
+                                                      statesOff) + i;
+        // a multiple of 8 workgroups (one group per XCD)
+        b.grid = std::max(8u, m.grid/8*8);
         b.shm = m.shm;
         m.many(b, s);
     }
 
-    void enqueue(const int* active, hipStream_t s) const
+    void enqueue(hipStream_t s) const
     {
-        for (const auto& m : launches) launch(m, active, s);
+        for (size_t i = 0; i < launches.size(); i++) launch(i, s);
         HIPCK(hipGetLastError());
     }
 
     // Runs the sequence once, synchronously.
     void run(hipStream_t s) const
     {
-        enqueue(nullptr, s);
+        enqueue(s);
         HIPCK(hipStreamSynchronize(s));
     }
 
-    // Picks the grid of every launch: all of its workgroups, or a
-    // persistent grid of 4, 8 or 16 workgroups per CU walking them, by the
-    // fastest of a few timed runs of the whole sequence.
+    // Picks the persistent grid of every launch, 4, 8, 16 or 32 workgroups
+    // per CU (never more than its workgroups), by the fastest of timed runs
+    // of the whole sequence.
     void autotune(hipStream_t s, int cus)
     {
-        const unsigned cand[4] = {0u, unsigned(4*cus), unsigned(8*cus),
-                                  unsigned(16*cus)};
+        const unsigned cand[4] = {unsigned(4*cus), unsigned(8*cus),
+                                  unsigned(16*cus), unsigned(32*cus)};
         const size_t n = launches.size();
         std::vector<hipEvent_t> ev(n + 1);
         for (auto& e : ev) HIPCK(hipEventCreate(&e));
@@ -190,9 +281,8 @@ struct BatchPlan
                     HIPCK(hipEventRecord(ev[0], s));
                     for (size_t i = 0; i < n; i++)
                     {
-                        Merged& m = launches[i];
-                        m.grid = c ? std::min(c, m.total) : m.total;
-                        launch(m, nullptr, s);
+                        launches[i].grid = std::min(c, launches[i].total);
+                        launch(i, s);
                         HIPCK(hipEventRecord(ev[i + 1], s));
                     }
                     HIPCK(hipGetLastError());
@@ -218,12 +308,12 @@ struct BatchPlan
         for (size_t i = 0; i < n; i++) launches[i].grid = pick[i];
     }
 
-    void capture(const int* active, hipStream_t s)
+    void capture(hipStream_t s)
     {
         HIPCK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         try
         {
-            enqueue(active, s);
+            enqueue(s);
         }
         catch (...)
         {
@@ -252,8 +342,6 @@ struct bmg_batch
 {
     std::vector<bmg_ctx*> ctx;
     hipStream_t stream = nullptr;
-    int* hActive = nullptr;   // pinned
-    int* dActive = nullptr;
     std::unique_ptr<BatchPlan> iter;
     std::vector<long> iterGen;   // context generations the plan was built on
     std::vector<int> iterMixed;
@@ -263,8 +351,6 @@ struct bmg_batch
     ~bmg_batch()
     {
         iter.reset();
-        if (hActive) (void)hipHostFree(hActive);
-        if (dActive) (void)hipFree(dActive);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -282,7 +368,7 @@ std::unique_ptr<BatchPlan> recordPlan(bmg_batch* B, F&& phase)
         seqs[s] = std::move(r.rec.recs);
     }
     std::unique_ptr<BatchPlan> p(new BatchPlan);
-    p->build(seqs);
+    p->build(seqs, unsigned(16*B->cus));
     return p;
 }
 
@@ -310,8 +396,6 @@ void batch_create(bmg_batch** out, bmg_ctx* const* ctxs, int n)
     }
     HIPCK(hipSetDevice(dev));
     HIPCK(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
-    HIPCK(hipHostMalloc(&B->hActive, n*sizeof(int)));
-    HIPCK(hipMalloc(&B->dActive, n*sizeof(int)));
     hipDeviceProp_t prop;
     HIPCK(hipGetDeviceProperties(&prop, dev));
     B->cus = std::max(1, prop.multiProcessorCount);
@@ -380,7 +464,7 @@ void batch_solve(bmg_batch* B, double* const* db, double* const* dx,
             }
             // timing runs before the solve's fields are initialised below
             if (B->tune) B->iter->autotune(B->stream, B->cus);
-            B->iter->capture(B->dActive, B->stream);
+            B->iter->capture(B->stream);
         }
         // initial residual and the sum of x
         recordPlan(B, [&](bmg_ctx* c, size_t s) {
@@ -413,6 +497,9 @@ void batch_solve(bmg_batch* B, double* const* db, double* const* dx,
                     v/(init[s] + 1e-20) > o.rel_tol) ? 0 : 1;
         };
         for (size_t s = 0; s < n; s++) conv[s] = check(s, cur[s]);
+        // the captured iteration walks the shares of the systems still
+        // iterating; they are rewritten whenever that set changes
+        std::vector<char> active(n, 1), last;
         for (;;)
         {
             size_t nactive = 0;
@@ -421,17 +508,20 @@ void batch_solve(bmg_batch* B, double* const* db, double* const* dx,
                 const bmg_opts& o = B->ctx[s]->opts;
                 const bool go = (iter[s] < o.max_iter && conv[s] == 0) ||
                                 iter[s] < o.min_iter;
-                B->hActive[s] = go;
+                active[s] = go;
                 nactive += go;
             }
             if (nactive == 0) break;
-            HIPCK(hipMemcpyAsync(B->dActive, B->hActive, n*sizeof(int),
-                                 hipMemcpyHostToDevice, B->stream));
+            if (active != last)
+            {
+                B->iter->setActive(active, B->stream);
+                last = active;
+            }
             HIPCK(hipGraphLaunch(B->iter->exec, B->stream));
             HIPCK(hipStreamSynchronize(B->stream));
             for (size_t s = 0; s < n; s++)
             {
-                if (!B->hActive[s]) continue;
+                if (!active[s]) continue;
                 bmg_ctx* c = B->ctx[s];
                 cur[s] = std::sqrt(c->hScalar[0])/nf[s];
                 if (!std::isfinite(cur[s]))

@@ -313,11 +313,20 @@ void bmg_ctx::coarseSolve()
 // pre-smoothing is skipped: APLU::solve overwrites every interior value and
 // reads ghosts only through eliminated (zero) couplings, so the result is the
 // same.
+// Structural position of a level's launches for batched merges: the part
+// before the coarser levels at height h above the coarse level, the coarse
+// level, and the part after them; increasing along a V-cycle.
+static long cycleTag(int height, bool after)
+{
+    return after ? 1000 + height : 1000 - height;
+}
+
 template <typename T>
 void bmg_ctx::cycle(std::vector<int>& visits, int l)
 {
     Level& q = lv[l];
     const int lc = coarseLevel();
+    record_tag(cycleTag(lc - l, false));
     if (singular() && l == lc)
         launch_set_cell<T>(B<T>(l), ((long)1*q.L.P[1] + 1)*q.L.Hz + 0,
                            (1 + 1 + 1) & 1, T(0), stream);
@@ -337,6 +346,7 @@ void bmg_ctx::cycle(std::vector<int>& visits, int l)
                 residual<T>(l, true, false, nullptr);
             restrictTo<T>(l);
             cycle<T>(visits, l + 1);
+            record_tag(cycleTag(lc - l, true));
             prolongFrom<T>(l);
             if (opts.n_sweeps_post > 0)
                 smooth<T>(l, opts.n_sweeps_post);
@@ -620,6 +630,7 @@ void bmg_ctx::enqueueIteration(bool mixed)
 {
     Level& q0 = lv[0];
     std::vector<int> visits(nLevels(), 0);
+    record_tag(cycleTag(coarseLevel(), false));
     if (!mixed)
         cycle<double>(visits, 0);
     else

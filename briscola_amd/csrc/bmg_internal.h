@@ -66,25 +66,29 @@ struct Region {
 // ---- batched execution (bmg_kernels.hip, bmg_batch.hip) ----
 // One system's share of a batched launch: virtual workgroups
 // [start, start + n) run the system's launch with grid (gx, gy, n/(gx*gy)).
+// The argument struct of the share is args[arg].
 struct BatchItem {
     unsigned start;
     unsigned gx, gy, n;
-    int sys;          // index into the active flags
+    int arg;
+};
+// The shares a batched launch walks, in device memory so that the host can
+// drop systems between replays of a captured launch sequence.
+struct BatchState {
+    int nitems;
+    unsigned total;           // virtual workgroups
 };
 // A batched launch of one kernel over many systems.
 struct BatchLaunch {
-    const void* args;         // device array of nitems argument structs
+    const void* args;         // device array of argument structs
     const BatchItem* items;   // device array, ascending starts
-    int nitems;
-    unsigned total;           // virtual workgroups
-    const int* active;        // per-system flags (0: skip), or null
-    unsigned grid;            // workgroups launched (grid-stride loop)
+    const BatchState* state;
+    unsigned grid;            // workgroups launched, a multiple of 8
     size_t shm;
 };
-// A launch recorded instead of run: the kernel (kind), its grid, dynamic
-// LDS, argument struct, and the function that launches many of them.
 struct LaunchRecord {
     const void* kind;
+    long tag;          // position in the solve's structure (record_tag)
     unsigned grid[3];
     size_t shm;
     size_t align;
@@ -93,10 +97,16 @@ struct LaunchRecord {
 };
 struct LaunchRecorder {
     std::vector<LaunchRecord> recs;
+    long tag = 0;
 };
 // The calling thread's recorder: while it is set, the batchable launchers
 // append records instead of launching and the others throw.
 LaunchRecorder*& launch_recorder();
+// Tags the launches recorded from now on with a position in the solve's
+// structure (no effect without a recorder).  Launches of different systems
+// with equal tags are merged with each other; tags must not decrease along
+// one system's sequence for the tags to be used.
+void record_tag(long tag);
 
 // ---- launchers (bmg_kernels.hip) ----
 // number of per-tile partial sums a cell-pair reduction over level L writes

@@ -382,6 +382,7 @@ struct SlabArgs {
     T* tptr[6];
     int tdim[6];
     int tplane[6];
+    int axis;          // D, the axis normal to the slabs
 };
 
 constexpr int SLAB_W = 32;   // LDS tile edge
@@ -392,11 +393,11 @@ __device__ __forceinline__ void rbgs_slabs_body(const Bid bid,
                                                 const OpView<T>& op,
                                                 Pair<T> x,
                                                 Pair<const T> b,
-                                                const SlabArgs<T>& sa)
+                                                const SlabArgs<T>& sa,
+                                                T (*xs)[SLAB_W + 1])
 {
     constexpr int A1 = D == 0 ? 1 : 0;
     constexpr int A2 = D == 2 ? 1 : 2;
-    __shared__ T xs[SLAB_W][SLAB_W + 1];
     const int slab = bid.z;
     const int H = sa.nphase;
     const int n1 = L.N[A1], n2 = L.N[A2];
@@ -547,8 +548,9 @@ __global__ __launch_bounds__(256) void k_rbgs_slabs(LevelDims L, OpView<T> op,
                                                     Pair<T> x, Pair<const T> b,
                                                     SlabArgs<T> sa)
 {
+    __shared__ T xs[SLAB_W][SLAB_W + 1];
     const Bid bid{blockIdx.x, blockIdx.y, blockIdx.z};
-    rbgs_slabs_body<T, OP, D>(bid, L, op, x, b, sa);
+    rbgs_slabs_body<T, OP, D>(bid, L, op, x, b, sa, xs);
 }
 
 template <typename T, int OP, int D>
@@ -566,7 +568,8 @@ struct KSlabs
     };
     __device__ static void run(const Bid bid, const Args& a)
     {
-        rbgs_slabs_body<T, OP, D>(bid, a.L, a.op, a.x, a.b, a.sa);
+        __shared__ T xs[SLAB_W][SLAB_W + 1];
+        rbgs_slabs_body<T, OP, D>(bid, a.L, a.op, a.x, a.b, a.sa, xs);
     }
     static void launch(dim3 g, size_t shm, hipStream_t s, const Args& a)
     {
@@ -2722,35 +2725,176 @@ BcVals bcvals(const double* v)
 }
 
 // ---------------------------------------------------------------------------
+// Batched forms of the kernels whose specialisations differ only in the
+// operator representation (and, for the slabs, the slab axis): they take
+// both from the argument struct, whose layout every specialisation shares,
+// so that systems whose levels store different operator kinds share one
+// launch.  BatchKind maps a kernel to the form its batched launches use.
+template <typename T>
+struct KRbgsAny
+{
+    static constexpr int bx = BX, by = BY;
+    static constexpr bool lds = false;
+    using Args = typename KRbgs<T, OP_CONST>::Args;
+    __device__ static void run(const Bid bid, const Args& a)
+    {
+        if (a.op.kind == OP_STENCIL)
+            rbgs_body<T, OP_STENCIL>(bid, a.L, a.op, a.xc, a.xo, a.bcl,
+                                     a.colour, a.rg, a.tl);
+        else if (a.op.kind == OP_CONST)
+            rbgs_body<T, OP_CONST>(bid, a.L, a.op, a.xc, a.xo, a.bcl,
+                                   a.colour, a.rg, a.tl);
+        else
+            rbgs_body<T, OP_FACES>(bid, a.L, a.op, a.xc, a.xo, a.bcl,
+                                   a.colour, a.rg, a.tl);
+    }
+};
+
+template <typename T, bool NORM>
+struct KResidualAny
+{
+    static constexpr int bx = BX, by = BY;
+    static constexpr bool lds = true;
+    using Args = typename KResidual<T, OP_CONST, NORM>::Args;
+    __device__ static void run(const Bid bid, const Args& a)
+    {
+        if (a.op.kind == OP_STENCIL)
+            residual_body<T, OP_STENCIL, NORM>(bid, a.L, a.op, a.x, a.b, a.r,
+                                               a.tl, a.partials);
+        else if (a.op.kind == OP_CONST)
+            residual_body<T, OP_CONST, NORM>(bid, a.L, a.op, a.x, a.b, a.r,
+                                             a.tl, a.partials);
+        else
+            residual_body<T, OP_FACES, NORM>(bid, a.L, a.op, a.x, a.b, a.r,
+                                             a.tl, a.partials);
+    }
+};
+
+template <typename T>
+struct KSlabsAny
+{
+    static constexpr int bx = 256, by = 1;
+    static constexpr bool lds = true;
+    using Args = typename KSlabs<T, OP_CONST, 0>::Args;
+    template <int OP>
+    __device__ static void axis(const Bid bid, const Args& a,
+                                T (*xs)[SLAB_W + 1])
+    {
+        if (a.sa.axis == 0)
+            rbgs_slabs_body<T, OP, 0>(bid, a.L, a.op, a.x, a.b, a.sa, xs);
+        else if (a.sa.axis == 1)
+            rbgs_slabs_body<T, OP, 1>(bid, a.L, a.op, a.x, a.b, a.sa, xs);
+        else
+            rbgs_slabs_body<T, OP, 2>(bid, a.L, a.op, a.x, a.b, a.sa, xs);
+    }
+    __device__ static void run(const Bid bid, const Args& a)
+    {
+        __shared__ T xs[SLAB_W][SLAB_W + 1];
+        if (a.op.kind == OP_STENCIL) axis<OP_STENCIL>(bid, a, xs);
+        else if (a.op.kind == OP_CONST) axis<OP_CONST>(bid, a, xs);
+        else axis<OP_FACES>(bid, a, xs);
+    }
+};
+
+template <typename T>
+struct KNfTermsAny
+{
+    static constexpr int bx = BX, by = BY;
+    static constexpr bool lds = true;
+    using Args = typename KNfTerms<T, OP_CONST>::Args;
+    __device__ static void run(const Bid bid, const Args& a)
+    {
+        if (a.op.kind == OP_STENCIL)
+            nf_terms_body<T, OP_STENCIL>(bid, a.L, a.op, a.r, a.b, a.y, a.tl,
+                                         a.partials);
+        else if (a.op.kind == OP_CONST)
+            nf_terms_body<T, OP_CONST>(bid, a.L, a.op, a.r, a.b, a.y, a.tl,
+                                       a.partials);
+        else
+            nf_terms_body<T, OP_FACES>(bid, a.L, a.op, a.r, a.b, a.y, a.tl,
+                                       a.partials);
+    }
+};
+
+struct KBcSourceAny
+{
+    static constexpr int bx = BX, by = BY;
+    static constexpr bool lds = false;
+    using Args = KBcSource<OP_CONST>::Args;
+    __device__ static void run(const Bid bid, const Args& a)
+    {
+        if (a.op.kind == OP_STENCIL)
+            bc_source_body<OP_STENCIL>(bid, a.L, a.op, a.b, a.bv);
+        else if (a.op.kind == OP_CONST)
+            bc_source_body<OP_CONST>(bid, a.L, a.op, a.b, a.bv);
+        else
+            bc_source_body<OP_FACES>(bid, a.L, a.op, a.b, a.bv);
+    }
+};
+
+template <class K>
+struct BatchKind { using type = K; };
+template <typename T, int OP>
+struct BatchKind<KRbgs<T, OP>> { using type = KRbgsAny<T>; };
+template <typename T, int OP, bool NORM>
+struct BatchKind<KResidual<T, OP, NORM>> { using type = KResidualAny<T, NORM>; };
+template <typename T, int OP, int D>
+struct BatchKind<KSlabs<T, OP, D>> { using type = KSlabsAny<T>; };
+template <typename T, int OP>
+struct BatchKind<KNfTerms<T, OP>> { using type = KNfTermsAny<T>; };
+template <int OP>
+struct BatchKind<KBcSource<OP>> { using type = KBcSourceAny; };
+
+// ---------------------------------------------------------------------------
 // Batched launches.  Every batchable kernel k_x has a struct K with its
 // argument struct K::Args, its workgroup shape (K::bx, K::by), its body
 // K::run(workgroup, args) and K::launch (one launch of k_x).  k_many runs
 // one launch for many independent systems: the systems' workgroups are
 // concatenated into virtual indices (each system's share starting at a
 // multiple of 8, so a workgroup keeps the XCD it would have on its own), a
-// fixed grid walks them with a grid-stride loop, and each workgroup finds
-// its system by binary search over the shares and reads that system's
-// argument struct.  Systems whose active flag is 0 are skipped.
+// fixed grid (a multiple of 8 workgroups) walks them, and each workgroup
+// runs the body on its share's argument struct.  The shares and their
+// number are read from device memory, so the host can drop converged
+// systems between replays of a captured graph.
 template <class K>
 __global__ __launch_bounds__(K::bx*K::by) void k_many(
     const typename K::Args* __restrict__ args,
-    const BatchItem* __restrict__ items, int nitems, unsigned total,
-    const int* __restrict__ active)
+    const BatchItem* __restrict__ items,
+    const BatchState* __restrict__ state)
 {
-    for (unsigned W = blockIdx.x; W < total; W += gridDim.x)
+    const int nitems = state->nitems;
+    const unsigned total = state->total;
+    if (total == 0) return;
+    // Virtual index W = 8p + g: the workgroups of XCD g (blockIdx & 7) split
+    // the positions p into contiguous chunks, so one workgroup walks its
+    // shares in order and finds each next share by a short forward step.
+    const unsigned g = blockIdx.x & 7u, k = blockIdx.x >> 3;
+    const unsigned per = gridDim.x >> 3;
+    const unsigned P = total >> 3;
+    const unsigned chunk = (P + per - 1)/per;
+    const unsigned p0 = k*chunk;
+    const unsigned p1 = p0 + chunk < P ? p0 + chunk : P;
+    if (p0 >= p1) return;
+    int lo = 0;
     {
-        int lo = 0, hi = nitems - 1;
+        const unsigned W = 8u*p0 + g;
+        int hi = nitems - 1;
         while (lo < hi)
         {
             const int mid = (lo + hi + 1) >> 1;
             if (items[mid].start <= W) lo = mid;
             else hi = mid - 1;
         }
+    }
+    for (unsigned p = p0; p < p1; p++)
+    {
+        const unsigned W = 8u*p + g;
+        while (lo + 1 < nitems && items[lo + 1].start <= W) lo++;
         const unsigned w = W - items[lo].start;
         const unsigned gx = items[lo].gx, gy = items[lo].gy;
-        if (w < items[lo].n && (!active || active[items[lo].sys]))
-            K::run(Bid{w % gx, (w/gx) % gy, w/(gx*gy)}, args[lo]);
-        // the next system's body reuses the workgroup's LDS
+        if (w < items[lo].n)
+            K::run(Bid{w % gx, (w/gx) % gy, w/(gx*gy)}, args[items[lo].arg]);
+        // the next share's body reuses the workgroup's LDS
         if constexpr (K::lds) __syncthreads();
     }
 }
@@ -2763,7 +2907,7 @@ void launch_many(const BatchLaunch& b, hipStream_t s)
 {
     hipLaunchKernelGGL((k_many<K>), dim3(b.grid), dim3(K::bx, K::by), b.shm,
                        s, static_cast<const typename K::Args*>(b.args),
-                       b.items, b.nitems, b.total, b.active);
+                       b.items, b.state);
 }
 
 // Launches K, or appends the launch to the calling thread's recorder.
@@ -2772,8 +2916,12 @@ void dispatch(dim3 g, size_t shm, hipStream_t s, const typename K::Args& a)
 {
     if (LaunchRecorder* r = launch_recorder())
     {
+        using B = typename BatchKind<K>::type;
+        static_assert(sizeof(typename B::Args) == sizeof(typename K::Args),
+                      "a batched form shares the argument layout");
         LaunchRecord rec;
-        rec.kind = &kind_tag<K>;
+        rec.kind = &kind_tag<B>;
+        rec.tag = r->tag;
         rec.grid[0] = g.x;
         rec.grid[1] = g.y;
         rec.grid[2] = g.z;
@@ -2781,7 +2929,7 @@ void dispatch(dim3 g, size_t shm, hipStream_t s, const typename K::Args& a)
         rec.align = alignof(typename K::Args);
         rec.args.resize(sizeof(a));
         std::memcpy(rec.args.data(), &a, sizeof(a));
-        rec.many = &launch_many<K>;
+        rec.many = &launch_many<B>;
         r->recs.push_back(std::move(rec));
         return;
     }
@@ -2802,6 +2950,11 @@ LaunchRecorder*& launch_recorder()
 {
     thread_local LaunchRecorder* r = nullptr;
     return r;
+}
+
+void record_tag(long tag)
+{
+    if (LaunchRecorder* r = launch_recorder()) r->tag = tag;
 }
 
 // ---------------------------------------------------------------------------
@@ -2860,6 +3013,7 @@ void launch_rbgs_slabs(const LevelDims& L, const OpView<T>& op, Pair<T> x,
         sa.tdim[t] = t < io.ntarget ? io.tdim[t] : 0;
         sa.tplane[t] = t < io.ntarget ? io.tplane[t] : -1;
     }
+    sa.axis = d;
     const int a1 = d == 0 ? 1 : 0;
     const int a2 = d == 2 ? 1 : 2;
     const int n1 = L.N[a1], n2 = L.N[a2];

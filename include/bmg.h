@@ -164,9 +164,9 @@ int bmg_solve_device(bmg_ctx* ctx, double* d_b, double* d_x,
 /* ---- Batched solves of independent systems.  Each system is a context
  * created and given its operator as for bmg_solve_device; the batch runs
  * MG::solve (MG.C:402-483; krylov none, rbgs smoother) of all of them
- * together: every kernel of an iteration is one launch over all systems
- * (k_many), the merged iteration is replayed as a HIP graph, and converged
- * systems drop out.  Each system's result is bitwise the one bmg_solve_device
+ * together: every kernel of an iteration is one persistent launch over all
+ * systems, the merged iteration is replayed as a HIP graph, and converged
+ * systems drop out of it.  Each system's result is bitwise the one bmg_solve_device
  * returns for it alone.  The contexts stay owned by the caller, must be on
  * one device and must not be used elsewhere while the batch solves; they
  * may differ in shape, boundary conditions, precision and cycle options.
@@ -184,8 +184,8 @@ int bmg_solve_batch(bmg_batch* batch, double* const* d_b, double* const* d_x,
 int bmg_batch_info(const bmg_batch* batch, int* launches,
                    long* workgroups);
 /* Launch-grid autotuning of the merged iteration (default on: every
- * launch takes the fastest of all-workgroups or a persistent grid of 4, 8
- * or 16 workgroups per CU; off: all workgroups).  Rebuilds the plan. */
+ * launch takes the fastest persistent grid of 4, 8, 16 or 32 workgroups per
+ * CU; off: 16 per CU).  Rebuilds the plan. */
 int bmg_batch_set_autotune(bmg_batch* batch, int enable);
 
 /* Flux of a level-0 field through every face, as PoissonSolver computes it
