@@ -33,6 +33,8 @@ EXPORTS = [
     "bmg_set_graphs", "bmg_solve_diagonal", "bmg_batch_create",
     "bmg_batch_destroy", "bmg_solve_batch", "bmg_batch_info",
     "bmg_batch_set_autotune", "bmg_face_flux", "bmg_face_flux_device",
+    "bmg_fft_create", "bmg_fft_destroy", "bmg_fft_solve",
+    "bmg_fft_solve_device",
 ]
 
 
@@ -122,6 +124,10 @@ def load():
         "bmg_batch_set_autotune": (I, [P, I]),
         "bmg_face_flux": (I, [P, DP, DP, DP, DP, DP, DP, DP]),
         "bmg_face_flux_device": (I, [P, P, P, P, P, P, P, P]),
+        "bmg_fft_create": (I, [ctypes.POINTER(P), ctypes.POINTER(Grid)]),
+        "bmg_fft_destroy": (I, [P]),
+        "bmg_fft_solve": (I, [P, DP, DP]),
+        "bmg_fft_solve_device": (I, [P, P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -359,3 +365,51 @@ class Batch:
         _check(self.lib.bmg_batch_info(self.ptr, ctypes.byref(n),
                                        ctypes.byref(w)))
         return n.value, w.value
+
+
+class FFT:
+    """The FFT Poisson solver (``bmg_fft_*``): div(grad x) = -b on a grid
+    with at least two uniform directions."""
+
+    def __init__(self, vertices, bc, bc_value=(0.0,) * 6):
+        self.lib = load()
+        self._verts = [np.ascontiguousarray(v, dtype=np.float64) for v in vertices]
+        g = Grid()
+        for d in range(3):
+            g.n[d] = len(self._verts[d]) - 1
+            g.vertices[d] = _dp(self._verts[d])
+        for f in range(6):
+            g.bc[f] = int(bc[f])
+            g.bc_value[f] = float(bc_value[f])
+        self.shape = tuple(len(v) + 1 for v in self._verts)
+        p = ctypes.c_void_p()
+        _check(self.lib.bmg_fft_create(ctypes.byref(p), ctypes.byref(g)))
+        self.ptr = p
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            self.lib.bmg_fft_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve(self, b, x=None):
+        """Host arrays (reference layout); x (initial ghosts) is updated in
+        place and returned."""
+        if x is None:
+            x = np.zeros(self.shape)
+        for a in (b, x):
+            if a is not None and (a.dtype != np.float64 or not a.flags.c_contiguous
+                                  or a.shape != self.shape):
+                raise ValueError("b and x must be C-contiguous float64 padded arrays")
+        _check(self.lib.bmg_fft_solve(self.ptr, _dp(b) if b is not None else None,
+                                      _dp(x)))
+        return x
+
+    def solve_device(self, b_ptr, x_ptr):
+        _check(self.lib.bmg_fft_solve_device(self.ptr, ctypes.c_void_p(b_ptr),
+                                             ctypes.c_void_p(x_ptr)))

@@ -11,7 +11,10 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 SOURCES = ["csrc/bmg_kernels.hip", "csrc/bmg_capi.hip", "csrc/bmg_batch.hip",
-           "csrc/unet_kernels.hip", "csrc/unet.hip"]
+           "csrc/bmg_fft.hip", "csrc/unet_kernels.hip", "csrc/unet.hip"]
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+LIBS = ["-L" + os.path.join(ROCM, "lib"), "-lrocfft",
+        "-Wl,-rpath," + os.path.join(ROCM, "lib")]
 LIB = os.path.join(HERE, "libbmg.so")
 ARCH = os.environ.get("BMG_OFFLOAD_ARCH", "gfx950")
 
@@ -29,7 +32,7 @@ CPP_TESTS = [os.path.join(ROOT, "tests", "cpp", t)
 def build(verbose=False):
     cmd = (["hipcc"] + FLAGS + ["-I" + os.path.join(ROOT, "include"),
                                 "-o", LIB + ".tmp"]
-           + [os.path.join(HERE, s) for s in SOURCES])
+           + [os.path.join(HERE, s) for s in SOURCES] + LIBS)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + r.stderr)

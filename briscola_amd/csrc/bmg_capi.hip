@@ -1711,6 +1711,26 @@ int bmg_solve(bmg_ctx* c, double* b, double* x, int constMatrix, bmg_stats* st)
     return run([&] { solveImpl(c, b, x, false, constMatrix, st); });
 }
 
+int bmg_fft_create(bmg_fft** out, const bmg_grid* grid)
+{
+    return run([&] { fft_create(out, grid); });
+}
+
+int bmg_fft_destroy(bmg_fft* f)
+{
+    return run([&] { fft_destroy(f); });
+}
+
+int bmg_fft_solve(bmg_fft* f, const double* b, double* x)
+{
+    return run([&] { fft_solve(f, b, x); });
+}
+
+int bmg_fft_solve_device(bmg_fft* f, const double* db, double* dx)
+{
+    return run([&] { fft_solve_device(f, db, dx); });
+}
+
 int bmg_batch_create(bmg_batch** out, bmg_ctx* const* ctxs, int n)
 {
     return run([&] { batch_create(out, ctxs, n); });

@@ -365,3 +365,15 @@ void batch_set_autotune(bmg_batch* b, int on);
 void batch_destroy(bmg_batch* b);
 
 }  // namespace bmg
+
+struct bmg_fft;
+
+namespace bmg {
+
+// FFT Poisson solver (bmg_fft.hip); throw Error on failure
+void fft_create(bmg_fft** out, const bmg_grid* grid);
+void fft_destroy(bmg_fft* f);
+void fft_solve(bmg_fft* f, const double* b, double* x);
+void fft_solve_device(bmg_fft* f, const double* db, double* dx);
+
+}  // namespace bmg

@@ -188,6 +188,24 @@ int bmg_batch_info(const bmg_batch* batch, int* launches,
  * CU; off: 16 per CU).  Rebuilds the plan. */
 int bmg_batch_set_autotune(bmg_batch* batch, int enable);
 
+/* ---- FFT Poisson solver (the reference's FFTPoissonSolver,
+ * FFTPoissonSolver.C:97-465): div(grad x) = -b in finite-difference form on
+ * a rectilinear grid with at least two uniform directions, by FFTs over
+ * the two transformed directions and a tridiagonal solve per wave-number
+ * pair along the other one (the first non-uniform direction, else x;
+ * planner.C:76-117).  Transformed directions may be periodic,
+ * Neumann-Neumann or Dirichlet-Dirichlet; the tridiagonal direction takes
+ * any pair.  b: the source (null: zero); x: in, its ghosts give the
+ * boundary terms (FFTPoissonSolver.C:123-180); out, the solution with every
+ * ghost refreshed.  Reference layout.  With only Neumann and periodic faces
+ * the zero mode is pinned as the reference pins it (the solution is defined
+ * up to a constant).  Runs on the current device and its own stream. ---- */
+typedef struct bmg_fft bmg_fft;
+int bmg_fft_create(bmg_fft** fft, const bmg_grid* grid);
+int bmg_fft_destroy(bmg_fft* fft);
+int bmg_fft_solve(bmg_fft* fft, const double* b, double* x);
+int bmg_fft_solve_device(bmg_fft* fft, const double* d_b, double* d_x);
+
 /* Flux of a level-0 field through every face, as PoissonSolver computes it
  * after a solve (defaultPoissonSolver.C:108-121): for axis d and padded cell
  * c whose lower face lies in the face range (1 <= c_d <= N_d + 1, interior
