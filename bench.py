@@ -142,6 +142,32 @@ def run_gpu(prob, precision, krylov, steps, warmup, profile, reps=50,
                 learned_guess_residual=stats.learned_guess_residual)
 
 
+def run_fft(prob, steps, warmup):
+    """The FFT Poisson solver (bmg_fft, the reference's FFT solver) on the
+    same problem in its finite-difference form (source -b/V)."""
+    import numpy as np
+    import torch
+    from briscola_amd import bmg
+    dx, dy, dz = (np.diff(v) for v in prob.vertices)
+    V = dx[:, None, None] * dy[None, :, None] * dz[None, None, :]
+    b = np.zeros_like(prob.b)
+    b[1:-1, 1:-1, 1:-1] = -prob.b[1:-1, 1:-1, 1:-1] / V
+    fft = bmg.FFT(prob.vertices, prob.bc, prob.bc_value)
+    b_dev = torch.from_numpy(b).to("cuda:0")
+    x_dev = torch.zeros_like(b_dev)
+    for _ in range(max(1, warmup)):
+        fft.solve_device(b_dev.data_ptr(), x_dev.data_ptr())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fft.solve_device(b_dev.data_ptr(), x_dev.data_ptr())
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    x = x_dev.cpu().numpy()
+    fft.close()
+    return dict(ms=ms, x=x)
+
+
 def reference_solution(name, prob):
     """FP64 reference solution and how it was obtained: the channel by the
     direct FFT (x,z) + Thomas (y) solve on the host; the variable-coefficient
@@ -255,6 +281,14 @@ def run_workload(name, args):
         for r in results.values():
             r["err_vs_fp64_reference"] = direct.rel_error(
                 r["x"][1:-1, 1:-1, 1:-1], xe, True)
+    fft = None
+    if name == "channel":
+        log(f"[bench] {name}: FFT Poisson solver")
+        fft = run_fft(prob, args.steps, args.warmup)
+        log(f"[bench] {name} fft: {fft['ms']:.2f} ms/solve")
+        if not args.no_check:
+            fft["err_vs_fp64_reference"] = direct.rel_error(
+                fft["x"][1:-1, 1:-1, 1:-1], xe, True)
     cpu = None
     if not args.no_cpu:
         log(f"[bench] {name}: CPU baseline sample")
@@ -279,6 +313,13 @@ def run_workload(name, args):
         out["speedup_vs_plain_fp64_mg"] = {
             k: round(results["fp64"]["ms"] / v["ms"], 3)
             for k, v in results.items()}
+    if fft is not None:
+        out["fft_solver"] = {
+            "method": ("FFT(x,z) + tridiagonal(y) direct solve of the "
+                       "finite-difference system (the reference's FFT "
+                       "solver; rocFFT + batched Thomas)"),
+            "ms": round(fft["ms"], 3),
+            "err_vs_fp64_reference": fft.get("err_vs_fp64_reference")}
     if learned is not None and "fcg_learned_mixed" in results:
         r = results["fcg_learned_mixed"]
         learned.update({
