@@ -22,7 +22,8 @@ bitwise equal to the CPU restatement), mixed-precision MG (FP32 cycles inside
 FP64 defect correction), and flexible CG preconditioned by one FP64 or mixed
 MG cycle.  ``value`` is the fastest method on the headline workload; every
 method is reported with iterations, final residual and its error against an
-FP64 reference solution.
+FP64 reference solution; ``equal_error`` times plain FP64 MG and FCG+mixed
+again to the tolerance at which each one's error falls below 1e-8.
 
 The roofline object covers the dominant kernel of the plain FP64 solve, the
 level-0 red-black smoother phase, timed live with HIP events on the library's
@@ -231,6 +232,44 @@ def roofline(result, kernel, traffic):
             "bytes_per_launch": per_launch}
 
 
+TARGET_ERROR = 1e-8
+
+
+def equal_error_solves(prob, results, xe, args):
+    """Plain FP64 MG and FCG+mixed solved to the relative residual that
+    brings each one's error against the FP64 reference below 1e-8: the
+    error at relTol 1e-8 is set by the operator's conditioning, and falls in
+    proportion to the tolerance, so the tolerance is the largest power of ten
+    below relTol * (1e-8 / error at relTol 1e-8), lowered once more if the
+    error still misses.  One untimed and one timed solve each."""
+    import math
+    from briscola_amd import bmg
+    from oracle import direct
+    out = {"target_rel_error": TARGET_ERROR, "methods": {}}
+    for mname, prec, kry in (("fp64", bmg.FP64, bmg.KRYLOV_NONE),
+                             ("fcg_mixed", bmg.MIXED, bmg.KRYLOV_FCG)):
+        err0 = results[mname]["err_vs_fp64_reference"]
+        tol = 10.0 ** math.floor(math.log10(1e-8 * TARGET_ERROR / err0))
+        for _ in range(2):
+            log(f"[bench] spheres {mname} to relTol {tol:.0e} (equal error)")
+            r = run_gpu(prob, prec, kry, 1, 1, False, rel_tol=tol,
+                        max_iter=4000)
+            err = direct.rel_error(r["x"][1:-1, 1:-1, 1:-1], xe, True)
+            if err <= TARGET_ERROR:
+                break
+            tol /= 10.0
+        out["methods"][mname] = {"rel_tol": tol, "ms": round(r["ms"], 3),
+                                 "iters": r["iters"],
+                                 "final_rel_residual": r["final_residual"],
+                                 "err_vs_fp64_reference": err}
+        log(f"[bench] spheres {mname} relTol {tol:.0e}: {r['ms']:.1f} ms, "
+            f"{r['iters']} iterations, error {err:.2e}")
+    m = out["methods"]
+    out["speedup_fcg_mixed_vs_fp64"] = round(m["fp64"]["ms"] /
+                                             m["fcg_mixed"]["ms"], 3)
+    return out
+
+
 def run_workload(name, args):
     import numpy as np
     from briscola_amd import bmg
@@ -281,6 +320,10 @@ def run_workload(name, args):
         for r in results.values():
             r["err_vs_fp64_reference"] = direct.rel_error(
                 r["x"][1:-1, 1:-1, 1:-1], xe, True)
+    equal = None
+    if (name == "spheres" and not args.no_check and not args.no_equal_error
+            and "fp64" in results and "fcg_mixed" in results):
+        equal = equal_error_solves(prob, results, xe, args)
     fft = None
     if name == "channel":
         log(f"[bench] {name}: FFT Poisson solver")
@@ -313,6 +356,8 @@ def run_workload(name, args):
         out["speedup_vs_plain_fp64_mg"] = {
             k: round(results["fp64"]["ms"] / v["ms"], 3)
             for k, v in results.items()}
+    if equal is not None:
+        out["equal_error"] = equal
     if fft is not None:
         out["fft_solver"] = {
             "method": ("FFT(x,z) + tridiagonal(y) direct solve of the "
@@ -474,6 +519,8 @@ def main():
                     help="skip the FP64 reference solution")
     ap.add_argument("--only", choices=["fp64", "mixed", "fcg_fp64", "fcg_mixed",
                                        "fcg_learned_mixed"], default=None)
+    ap.add_argument("--no-equal-error", action="store_true",
+                    help="skip the solves to a 1e-8 error on the headline")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned preconditioner")
     ap.add_argument("--train-n", type=int, default=64)
@@ -527,6 +574,8 @@ def main():
     }
     if "speedup_vs_plain_fp64_mg" in main_out:
         line["speedup_vs_plain_fp64_mg"] = main_out["speedup_vs_plain_fp64_mg"]
+    if "equal_error" in main_out:
+        line["equal_error"] = main_out["equal_error"]
     if "learned" in main_out:
         line["learned"] = main_out["learned"]
     if extra is not None:

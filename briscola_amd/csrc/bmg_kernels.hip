@@ -2966,8 +2966,10 @@ void launch_rbgs(const LevelDims& L, const OpView<T>& op, Pair<T> x,
                  Pair<const T> b, int colour, const Region& rg, hipStream_t s)
 {
     if (rg.E[0] <= rg.S[0] || rg.E[1] <= rg.S[1] || rg.E[2] <= rg.S[2]) return;
+    // a row of the region holds at most (nK + 1)/2 cells of one colour,
+    // counted from the first of them (pmin in rbgs_body)
     const int nK = rg.E[2] - rg.S[2];
-    const Tiles tl = make_tiles(((nK + 1)/2 + 1 + BX - 1)/BX,
+    const Tiles tl = make_tiles(((nK + 1)/2 + BX - 1)/BX,
                                 (rg.E[1] - rg.S[1] + BY - 1)/BY,
                                 rg.E[0] - rg.S[0]);
     const dim3 g = tiles_grid(tl);
