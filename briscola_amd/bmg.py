@@ -30,7 +30,8 @@ EXPORTS = [
     "bmg_op_prepare", "bmg_op_smooth", "bmg_op_residual", "bmg_op_restrict",
     "bmg_op_prolong", "bmg_op_correct_bcs", "bmg_op_coarse_solve",
     "bmg_op_cycle", "bmg_op_sum_sq_residual", "bmg_time_smoother",
-    "bmg_set_graphs", "bmg_solve_diagonal", "bmg_batch_create",
+    "bmg_set_graphs", "bmg_set_fused_smoothing", "bmg_solve_diagonal",
+    "bmg_batch_create",
     "bmg_batch_destroy", "bmg_solve_batch", "bmg_batch_info",
     "bmg_batch_set_autotune", "bmg_face_flux", "bmg_face_flux_device",
     "bmg_fft_create", "bmg_fft_destroy", "bmg_fft_solve",
@@ -115,6 +116,7 @@ def load():
         "bmg_op_sum_sq_residual": (I, [P, DP]),
         "bmg_time_smoother": (I, [P, I, I, I, DP, DP]),
         "bmg_set_graphs": (I, [P, I]),
+        "bmg_set_fused_smoothing": (I, [P, I]),
         "bmg_solve_diagonal": (I, [P, DP, DP, DP]),
         "bmg_batch_create": (I, [ctypes.POINTER(P), ctypes.POINTER(P), I]),
         "bmg_batch_destroy": (I, [P]),
@@ -310,6 +312,12 @@ class Context:
 
     def set_graphs(self, enable):
         _check(self.lib.bmg_set_graphs(self.ptr, int(enable)))
+
+    def set_fused_smoothing(self, mode):
+        """0 (default): one launch per red-black phase; 1: phase pairs in one
+        LDS-staged launch on large levels without periodic faces; 2: on every
+        level.  Same bits in every mode."""
+        _check(self.lib.bmg_set_fused_smoothing(self.ptr, int(mode)))
 
     def sum_sq_residual(self):
         out = np.zeros(1)

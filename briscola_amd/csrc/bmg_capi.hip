@@ -151,6 +151,93 @@ static Pair<const T> cst(Pair<T> p)
     return Pair<const T>{{p.c[0], p.c[1]}};
 }
 
+// The colours of the four counter-relative phases of a region
+// (rbgsSmoother.C:65-183): forward even = colour of S, forward odd = the
+// other, backward even/odd anchored at E-1; a phase equal to the one before
+// is dropped.  Returns the number of phases.
+static int phase_colours(const Region& rg, int out[4])
+{
+    const int Ss = rg.S[0] + rg.S[1] + rg.S[2];
+    const int Es = rg.E[0] + rg.E[1] + rg.E[2];
+    const int c4[4] = {Ss & 1, (Ss + 1) & 1, (Es + 1) & 1, Es & 1};
+    int n = 0;
+    for (int i = 0; i < 4; i++)
+        if (n == 0 || out[n - 1] != c4[i]) out[n++] = c4[i];
+    return n;
+}
+
+template <typename T>
+bool bmg_ctx::fusedLevel(int l) const
+{
+    if (fuseSmoothing == 0 || opts.smoother != BMG_RBGS || anyPeriodic() ||
+        launch_recorder())
+        return false;
+    const int tiles = fused_rbgs_tiles<T>(lv[l].L);
+    return tiles > 0 && (fuseSmoothing == 2 || tiles >= 256);
+}
+
+// Smoothing of a level without periodic faces, whose sweeps are interior
+// phases only: the phase sequence of all sweeps (a phase equal to the one
+// before skipped, as in smooth) is run as fused pairs (c1, c2) plus a last
+// single c1 phase when the count is odd.  Colour c2 moves between x and the
+// level's spare buffer with every pair and is copied back at the end when
+// it finishes in the spare buffer.
+template <typename T>
+void bmg_ctx::smoothFused(int l, int sweeps)
+{
+    Level& q = lv[l];
+    const OpView<T> o = op<T>(l);
+    Pair<T> x = X<T>(l);
+    Pair<T> bb = B<T>(l);
+    const Region rg = interior(l, 1);
+    std::vector<int> ph;
+    for (int s = 0; s < sweeps; s++)
+    {
+        int cols[4];
+        const int nc = phase_colours(rg, cols);
+        for (int i = 0; i < nc; i++)
+            if (ph.empty() || ph.back() != cols[i]) ph.push_back(cols[i]);
+    }
+    if (ph.empty()) return;
+    T* spare = Tmp<T>(l).c[0];
+    const int c1 = ph[0], c2 = c1 ^ 1;
+    bool inSpare = false;
+    size_t i = 0;
+    while (i < ph.size())
+    {
+        T* cur2 = inSpare ? spare : x.c[c2];
+        if (i + 1 < ph.size())
+        {
+            T* next2 = inSpare ? x.c[c2] : spare;
+            launch_rbgs2(q.L, o, x.c[c1], cur2, next2, bb.c[c1], bb.c[c2], c1,
+                         rg, stream);
+            inSpare = !inSpare;
+            i += 2;
+        }
+        else
+        {
+            Pair<T> xs;
+            xs.c[c1] = x.c[c1];
+            xs.c[c2] = cur2;
+            launch_rbgs(q.L, o, xs, cst(bb), c1, rg, stream);
+            i += 1;
+        }
+    }
+    if (inSpare)
+        HIPCK(hipMemcpyAsync(x.c[c2], spare, size_t(q.L.ncol)*sizeof(T),
+                             hipMemcpyDeviceToDevice, stream));
+}
+
+void bmg_ctx::ensureSmootherBuffers(bool mixed)
+{
+    for (int l = 0; l < nLevels(); l++)
+    {
+        const bool need = opts.smoother == BMG_JAC ||
+            (mixed ? fusedLevel<float>(l) : fusedLevel<double>(l));
+        if (need) mixed ? (void)Tmp<float>(l) : (void)Tmp<double>(l);
+    }
+}
+
 // rbgsSmoother::Smooth (rbgsSmoother.C:376-437) / jacSmoother::Smooth.  The
 // four counter-relative phases of every region (rbgsSmoother.C:65-183) are
 // turned into colours of the padded index: forward even = colour of S,
@@ -182,16 +269,12 @@ void bmg_ctx::smooth(int l, int sweeps)
         }
         return;
     }
-    auto colours = [](const Region& rg, int out[4]) -> int
+    if (fusedLevel<T>(l))
     {
-        const int Ss = rg.S[0] + rg.S[1] + rg.S[2];
-        const int Es = rg.E[0] + rg.E[1] + rg.E[2];
-        const int c4[4] = {Ss & 1, (Ss + 1) & 1, (Es + 1) & 1, Es & 1};
-        int n = 0;
-        for (int i = 0; i < 4; i++)
-            if (n == 0 || out[n - 1] != c4[i]) out[n++] = c4[i];
-        return n;
-    };
+        smoothFused<T>(l, sweeps);
+        return;
+    }
+    auto colours = phase_colours;
     // the previous phase launched on this level: a phase that repeats it on
     // the same region would rewrite the same values and is skipped
     int prevColour = -1;
@@ -652,9 +735,7 @@ double bmg_ctx::iterate(bool mixed)
 {
     if (useGraphs)
     {
-        if (opts.smoother == BMG_JAC)
-            for (int l = 0; l < nLevels(); l++)
-                mixed ? (void)Tmp<float>(l) : (void)Tmp<double>(l);
+        ensureSmootherBuffers(mixed);
         if (!graphExec || graphGen != gen || graphMixed != int(mixed))
         {
             if (graphExec) { HIPCK(hipGraphExecDestroy(graphExec)); graphExec = nullptr; }
@@ -811,9 +892,7 @@ void bmg_ctx::fcgIteration(int mode)
         HIPCK(hipStreamSynchronize(stream));
         return;
     }
-    if (opts.smoother == BMG_JAC && mode != 2)
-        for (int l = 0; l < nLevels(); l++)
-            mode == 1 ? (void)Tmp<float>(l) : (void)Tmp<double>(l);
+    if (mode != 2) ensureSmootherBuffers(mode == 1);
     if (!fgraphExec || fgraphGen != gen || fgraphMode != mode)
     {
         if (fgraphExec) { HIPCK(hipGraphExecDestroy(fgraphExec)); fgraphExec = nullptr; }
@@ -846,9 +925,7 @@ void bmg_ctx::precondition(bool mixed)
         enqueuePrecondition(mixed);
         return;
     }
-    if (opts.smoother == BMG_JAC)
-        for (int l = 0; l < nLevels(); l++)
-            mixed ? (void)Tmp<float>(l) : (void)Tmp<double>(l);
+    ensureSmootherBuffers(mixed);
     if (!pgraphExec || pgraphGen != gen || pgraphMixed != int(mixed))
     {
         if (pgraphExec) { HIPCK(hipGraphExecDestroy(pgraphExec)); pgraphExec = nullptr; }
@@ -2086,6 +2163,17 @@ int bmg_set_graphs(bmg_ctx* c, int enable)
     return run([&] {
         if (!c) throw Error(BMG_ERR_ARG, "null context");
         c->useGraphs = enable != 0;
+    });
+}
+
+int bmg_set_fused_smoothing(bmg_ctx* c, int mode)
+{
+    return run([&] {
+        if (!c) throw Error(BMG_ERR_ARG, "null context");
+        if (mode < 0 || mode > 2)
+            throw Error(BMG_ERR_ARG, "fused smoothing mode must be 0, 1 or 2");
+        if (c->fuseSmoothing != mode) c->gen++;
+        c->fuseSmoothing = mode;
     });
 }
 

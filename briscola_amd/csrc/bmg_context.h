@@ -207,6 +207,11 @@ struct bmg_ctx
     long gen = 0;          // bumped whenever device buffers are (re)allocated
     long graphGen = -1;
     bool useGraphs = true;
+    // fused two-phase smoothing (launch_rbgs2) on levels without periodic
+    // faces: 0 never (default: one launch per phase is faster on MI355X),
+    // 1 on levels whose launch has at least 256 workgroups, 2 on every level
+    // whose rows fit the LDS ring
+    int fuseSmoothing = 0;
 
     int nLevels() const { return int(lv.size()); }
     int coarseLevel() const { return nLevels() - opts.coarse_level - 1; }
@@ -304,6 +309,12 @@ struct bmg_ctx
 
     template <typename T>
     void smooth(int l, int sweeps);
+    template <typename T>
+    bool fusedLevel(int l) const;
+    template <typename T>
+    void smoothFused(int l, int sweeps);
+    // buffers the smoothers use, allocated before any graph capture
+    void ensureSmootherBuffers(bool mixed);
     template <typename T>
     void residual(int l, bool corrBcs, bool norm, int* nblocks);
     template <typename T>

@@ -115,6 +115,17 @@ template <typename T>
 void launch_rbgs(const LevelDims& L, const OpView<T>& op, Pair<T> x,
                  Pair<const T> b, int colour, const Region& rg,
                  hipStream_t s);
+// Two consecutive red-black phases (colour c1, then 1 - c1) over rg in one
+// launch: c1 updated in place in xR, the other colour read from xBin and
+// written to xBout (a second buffer; cells outside rg copied).  Not
+// batchable.  fused_rbgs_tiles: the launch's workgroup count, 0 when the
+// level's rows are too long for its LDS ring.
+template <typename T>
+void launch_rbgs2(const LevelDims& L, const OpView<T>& op, T* xR,
+                  const T* xBin, T* xBout, const T* bR, const T* bB, int c1,
+                  const Region& rg, hipStream_t s);
+template <typename T>
+int fused_rbgs_tiles(const LevelDims& L);
 // Slab-plane snapshots and the snapshots a slab launch writes on to: src
 // holds the snapshot of each of its planes; every target is the snapshot of
 // a later slab launch, of plane tplane normal to axis tdim.

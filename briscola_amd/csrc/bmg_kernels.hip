@@ -351,6 +351,165 @@ struct KRbgs
 };
 
 
+// Two consecutive red-black phases in one launch: colour c1 over region rg,
+// then colour c2 = 1 - c1 over rg (rbgsSmoother.C:65-183), with the per-cell
+// arithmetic of rbgs_body.  Workgroup (jt, it) owns padded rows
+// [j0, j0 + TJ) of padded planes [i0, i0 + CI) and streams through them:
+// at step s it updates colour c1 of plane s on its rows plus one halo row on
+// each side, keeping the new values in an LDS ring of three planes, then
+// updates colour c2 of plane s - 1 from the ring.  Colour c1 is updated in
+// place (no cell reads c1's old value); colour c2 is read from xBin and
+// written to xBout, a second buffer, so a halo update of c1 always sees the
+// old c2 whatever the neighbouring workgroups have written.  Outside rg,
+// c1 keeps its values and c2 is copied to xBout, so xBout is a complete
+// colour array.  Between the first colour's HBM reads (its b, coefficients
+// and the old c2) and the second's, every coefficient and right-hand-side
+// value is read once per launch instead of once per phase.
+struct FusedTiles {
+    int TJ, CI;        // rows and planes owned by a workgroup
+    int nJt, nIt;      // tiles along J and I
+    int per;           // tiles per XCD
+    int bs;            // threads per workgroup (256, 512 or 1024)
+    unsigned magic;    // ceil(2^32 / Hz): e / Hz = umulhi(e, magic)
+};
+
+
+template <typename T, int OP>
+__device__ __forceinline__ T fused_update(const LevelDims& L,
+                                          const OpView<T>& op, int colour,
+                                          unsigned idx, int row, int p,
+                                          int q, int I, int J, int K,
+                                          const T nb[6], T bv)
+{
+    T a[7];
+    coeffs<T, OP>(L, op, colour, int(idx), row, p, q, I, J, K, a);
+    const T P =
+        a[1]*nb[0]
+      + a[2]*nb[1]
+      + a[3]*nb[2]
+      + a[4]*nb[3]
+      + a[5]*nb[4]
+      + a[6]*nb[5];
+    return (bv - P)/a[0];
+}
+
+__device__ __forceinline__ bool in_region(const Region& rg, int I, int J,
+                                          int K)
+{
+    return I >= rg.S[0] && I < rg.E[0] && J >= rg.S[1] && J < rg.E[1] &&
+           K >= rg.S[2] && K < rg.E[2];
+}
+
+template <typename T, int OP, int BS>
+__device__ __forceinline__ void rbgs2_body(const Bid bid, const LevelDims& L,
+                                           const OpView<T>& op,
+                                           T* __restrict__ xR,
+                                           const T* __restrict__ xBin,
+                                           T* __restrict__ xBout,
+                                           const T* __restrict__ bR,
+                                           const T* __restrict__ bB, int c1,
+                                           const Region& rg,
+                                           const FusedTiles& ft, T* ring)
+{
+    const unsigned lin = (bid.x & 7u)*unsigned(ft.per) + (bid.x >> 3);
+    if (lin >= unsigned(ft.nJt*ft.nIt)) return;
+    const int jt = int(lin % unsigned(ft.nJt));
+    const int it = int(lin/unsigned(ft.nJt));
+    const int P0 = L.P[0], P1 = L.P[1], P2 = L.P[2], Hz = L.Hz;
+    const unsigned sI = unsigned(L.sI), uHz = unsigned(Hz);
+    const int j0 = jt*ft.TJ, j1 = min(j0 + ft.TJ, P1);
+    const int i0 = it*ft.CI, i1 = min(i0 + ft.CI, P0);
+    // ring rows j0-1 .. j1 (row j0-1 at ring row 0), four plane slots
+    const int rowsR = j1 - j0 + 2;
+    const int slotSize = rowsR*Hz;
+    const int c2 = c1 ^ 1;
+    const int J0 = max(j0 - 1, 0), J1 = min(j1 + 1, P1);
+    // step s: colour c1 of plane s into slot s % 4 and, in the same pass,
+    // colour c2 of plane s - 2 from the slots of planes s-3, s-2, s-1
+    for (int s = i0 - 1; s <= i1 + 1; s++)
+    {
+        const bool doR = s >= 0 && s < P0 && s <= i1;
+        const int sB = s - 2;
+        const bool doB = sB >= i0 && sB < i1;
+        const int nR = doR ? (J1 - J0)*Hz : 0;
+        const int nB = doB ? (j1 - j0)*Hz : 0;
+        for (int e0 = int(threadIdx.x); e0 < nR + nB; e0 += BS)
+        {
+            if (e0 < nR)
+            {
+                const int e = e0;
+                const int jr = int(__umulhi(unsigned(e), ft.magic));
+                const int p = e - jr*Hz;
+                const int J = J0 + jr;
+                const int q = (c1 - s - J) & 1;
+                const int K = 2*p + q;
+                if (K >= P2) continue;
+                const int row = (s*P1 + J)*Hz;
+                const unsigned idx = unsigned(row + p);
+                T v;
+                if (in_region(rg, s, J, K))
+                {
+                    const T nb[6] = {ld(xBin, idx - sI), ld(xBin, idx + sI),
+                                     ld(xBin, idx - uHz), ld(xBin, idx + uHz),
+                                     ld(xBin, idx - 1 + q), ld(xBin, idx + q)};
+                    const T bv = ld(bR, idx);
+                    v = fused_update<T, OP>(L, op, c1, idx, row, p, q, s, J,
+                                            K, nb, bv);
+                    if (s >= i0 && s < i1 && J >= j0 && J < j1)
+                        st(xR, idx, v);
+                }
+                else
+                    v = ld(xR, idx);
+                ring[(s & 3)*slotSize + (J - j0 + 1)*Hz + p] = v;
+            }
+            else
+            {
+                const int e = e0 - nR;
+                const int jr = int(__umulhi(unsigned(e), ft.magic));
+                const int p = e - jr*Hz;
+                const int J = j0 + jr;
+                const int q = (c2 - sB - J) & 1;
+                const int K = 2*p + q;
+                if (K >= P2) continue;
+                const int row = (sB*P1 + J)*Hz;
+                const unsigned idx = unsigned(row + p);
+                if (in_region(rg, sB, J, K))
+                {
+                    const T* rm = ring + ((sB - 1) & 3)*slotSize;
+                    const T* r0 = ring + (sB & 3)*slotSize;
+                    const T* rp = ring + ((sB + 1) & 3)*slotSize;
+                    const int lr = (jr + 1)*Hz + p;
+                    const T nb[6] = {rm[lr], rp[lr], r0[lr - Hz], r0[lr + Hz],
+                                     r0[lr - 1 + q], r0[lr + q]};
+                    const T bv = ld(bB, idx);
+                    st(xBout, idx, fused_update<T, OP>(L, op, c2, idx, row, p,
+                                                       q, sB, J, K, nb, bv));
+                }
+                else
+                    st(xBout, idx, ld(xBin, idx));
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T, int OP, int BS>
+__global__ __launch_bounds__(BS) void k_rbgs2(LevelDims L, OpView<T> op,
+                                                    T* __restrict__ xR,
+                                                    const T* __restrict__ xBin,
+                                                    T* __restrict__ xBout,
+                                                    const T* __restrict__ bR,
+                                                    const T* __restrict__ bB,
+                                                    int c1, Region rg,
+                                                    FusedTiles ft)
+{
+    extern __shared__ unsigned char fused_lds[];
+    const Bid bid{blockIdx.x, blockIdx.y, blockIdx.z};
+    rbgs2_body<T, OP, BS>(bid, L, op, xR, xBin, xBout, bR, bB, c1, rg, ft,
+                          reinterpret_cast<T*>(fused_lds));
+}
+
+
 // All red-black phases of the width-1 boundary slabs of one face pair
 // (rbgsSmoother::SweepBoundary, rbgsSmoother.C:186-374) in one launch.  A
 // slab is a plane of cells whose out-of-plane neighbours (the interior layer
@@ -2993,6 +3152,79 @@ void launch_rbgs(const LevelDims& L, const OpView<T>& op, Pair<T> x,
     }
 }
 
+// Tiles of the fused two-phase launch: TJ rows as large as an LDS ring of
+// four planes within 64 KB allows (at most 16), CI = 9 planes, 512 threads.
+FusedTiles fused_tiles(const LevelDims& L, size_t elem)
+{
+    FusedTiles ft{};
+    const size_t perRow = 4*size_t(L.Hz)*elem;
+    ft.TJ = int(std::min<size_t>(16, 65536/perRow - 2));
+    ft.CI = 9;
+    ft.bs = 512;
+    ft.nJt = (L.P[1] + ft.TJ - 1)/ft.TJ;
+    ft.nIt = (L.P[0] + ft.CI - 1)/ft.CI;
+    ft.per = (ft.nJt*ft.nIt + 7)/8;
+    ft.magic = unsigned(((1ull << 32) + unsigned(L.Hz) - 1)/unsigned(L.Hz));
+    return ft;
+}
+
+template <typename T>
+int fused_rbgs_tiles(const LevelDims& L)
+{
+    if (65536/(4*size_t(L.Hz)*sizeof(T)) < 4) return 0;
+    const FusedTiles ft = fused_tiles(L, sizeof(T));
+    return ft.nJt*ft.nIt;
+}
+
+template <typename T, int BS>
+void launch_rbgs2_bs(dim3 g, size_t shm, hipStream_t s, const LevelDims& L,
+                     const OpView<T>& op, T* xR, const T* xBin, T* xBout,
+                     const T* bR, const T* bB, int c1, const Region& rg,
+                     const FusedTiles& ft)
+{
+    switch (op.kind)
+    {
+    case OP_STENCIL:
+        hipLaunchKernelGGL((k_rbgs2<T, OP_STENCIL, BS>), g, dim3(BS), shm, s,
+                           L, op, xR, xBin, xBout, bR, bB, c1, rg, ft);
+        break;
+    case OP_CONST:
+        hipLaunchKernelGGL((k_rbgs2<T, OP_CONST, BS>), g, dim3(BS), shm, s, L,
+                           op, xR, xBin, xBout, bR, bB, c1, rg, ft);
+        break;
+    default:
+        hipLaunchKernelGGL((k_rbgs2<T, OP_FACES, BS>), g, dim3(BS), shm, s, L,
+                           op, xR, xBin, xBout, bR, bB, c1, rg, ft);
+        break;
+    }
+}
+
+template <typename T>
+void launch_rbgs2(const LevelDims& L, const OpView<T>& op, T* xR,
+                  const T* xBin, T* xBout, const T* bR, const T* bB, int c1,
+                  const Region& rg, hipStream_t s)
+{
+    direct_only("fused red-black phases");
+    if (fused_rbgs_tiles<T>(L) == 0)
+        throw std::runtime_error("fused red-black phases: rows too long for "
+                                 "the LDS ring");
+    const FusedTiles ft = fused_tiles(L, sizeof(T));
+    // e / Hz by the multiplier is exact for e * Hz < 2^32
+    if (uint64_t(ft.TJ + 2)*uint64_t(L.Hz)*uint64_t(L.Hz) >= (1ull << 32))
+        throw std::runtime_error("fused red-black phases: level too large");
+    const size_t shm = 4*size_t(ft.TJ + 2)*size_t(L.Hz)*sizeof(T);
+    const dim3 g(8*ft.per);
+    if (ft.bs == 1024)
+        launch_rbgs2_bs<T, 1024>(g, shm, s, L, op, xR, xBin, xBout, bR, bB,
+                                 c1, rg, ft);
+    else if (ft.bs == 512)
+        launch_rbgs2_bs<T, 512>(g, shm, s, L, op, xR, xBin, xBout, bR, bB, c1,
+                                rg, ft);
+    else
+        launch_rbgs2_bs<T, 256>(g, shm, s, L, op, xR, xBin, xBout, bR, bB, c1,
+                                rg, ft);
+}
+
 template <typename T>
 void launch_rbgs_slabs(const LevelDims& L, const OpView<T>& op, Pair<T> x,
                        Pair<const T> b, int d, const int* planes, int nslab,
@@ -3508,6 +3740,10 @@ void launch_set_cell(Pair<T> f, long idx, int colour, T v, hipStream_t s)
     template void launch_rbgs<T>(const LevelDims&, const OpView<T>&, Pair<T>, \
                                  Pair<const T>, int, const Region&,           \
                                  hipStream_t);                                \
+    template void launch_rbgs2<T>(const LevelDims&, const OpView<T>&, T*,     \
+                                  const T*, T*, const T*, const T*, int,      \
+                                  const Region&, hipStream_t);                \
+    template int fused_rbgs_tiles<T>(const LevelDims&);                        \
     template void launch_rbgs_slabs<T>(const LevelDims&, const OpView<T>&,    \
                                        Pair<T>, Pair<const T>, int,           \
                                        const int*, int, const int*, int,      \

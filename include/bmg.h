@@ -226,7 +226,7 @@ int bmg_face_flux_device(bmg_ctx* ctx, const double* d_x,
                          const double* d_lambda_z, double* d_flux_x,
                          double* d_flux_y, double* d_flux_z);
 
-/* MG::solve of a diagonal system (sys.diagonal(), MG.C:404-428): level-0/* MG::solve of a diagonal system (sys.diagonal(), MG.C:404-428): level-0
+/* MG::solve of a diagonal system (sys.diagonal(), MG.C:404-428): level-0
  * stencil A7 (reference layout, only the centre is used after ghost
  * elimination), x = b / A_C on every interior cell, then the level-0
  * boundary update.  b and x as in bmg_solve. */
@@ -270,6 +270,16 @@ int bmg_time_smoother(bmg_ctx* ctx, int level, int precision, int reps,
 /* Replay each MG iteration as a captured HIP graph (default on; the
  * environment variable BMG_NO_GRAPHS=1 turns it off at creation). */
 int bmg_set_graphs(bmg_ctx* ctx, int enable);
+/* Smoothing on levels without periodic faces can run consecutive red-black
+ * phases in pairs, one launch per pair (an LDS-staged two-phase kernel that
+ * reads each coefficient and right-hand-side value once per pair; the same
+ * per-cell arithmetic, so the same bits as one launch per phase).
+ * mode 0 (default): one launch per phase everywhere, the faster choice at
+ * 256^3 on MI355X; 1: pairs on levels large enough to fill the device (at
+ * least 256 workgroups); 2: pairs on every level.  No reference
+ * counterpart (the order of the phases is rbgsSmoother.C:376-437's either
+ * way). */
+int bmg_set_fused_smoothing(bmg_ctx* ctx, int mode);
 
 #ifdef __cplusplus
 }
