@@ -174,6 +174,8 @@ struct Bid {
 struct Tiles {
     int gx, gy, gz;
     int per;   // tiles per XCD
+    int ws;    // log2 of the cells of a row a wavefront covers (6: one row
+               // segment of 64 per wavefront; below 6, 64 >> ws rows)
 };
 
 __device__ __forceinline__ bool tile_of(const Tiles& t, unsigned w, int& bx,
@@ -261,34 +263,21 @@ __device__ __forceinline__ void pair_Ax(const LevelDims& L,
     }
 }
 
-// ---------------------------------------------------------------------------
-// Red-black Gauss-Seidel phase: update every cell of one colour inside
-// region [S,E) (rbgsSmoother.C:65-183):
-//   x_c = (b_c - (A1 x_-i + A2 x_+i + A3 x_-j + A4 x_+j + A5 x_-k + A6 x_+k))
-//         / A0
-// One thread per cell of the colour; lanes run along the contiguous axis;
-// tiles of BX positions x BY rows, one tile row per plane, dealt to the
-// XCDs in contiguous runs of planes.
+// One cell of a red-black phase: row J of plane I, the off-th cell of the
+// colour in the row.
 template <typename T, int OP>
-__device__ __forceinline__ void rbgs_body(const Bid bid, const LevelDims& L,
+__device__ __forceinline__ void rbgs_cell(const LevelDims& L,
                                           const OpView<T>& op,
                                           T* __restrict__ xc,
                                           const T* __restrict__ xo,
                                           const T* __restrict__ bcl,
-                                          int colour, const Region& rg,
-                                          const Tiles& tl)
+                                          int colour, const Region& rg, int I,
+                                          int J, int off)
 {
-    int bx, by, bz;
-    if (!tile_of(tl, bid.x, bx, by, bz)) return;
-    const int I = rg.S[0] + bz;
-    // a wavefront is one row (BX = 64), so J and everything derived from it
-    // is uniform and lives in scalar registers
-    const int J = __builtin_amdgcn_readfirstlane(rg.S[1] + by*BY +
-                                                 int(threadIdx.y));
     if (J >= rg.E[1]) return;
     const int q = (colour - I - J) & 1;
     const int pmin = (rg.S[2] - q + 1) >> 1;
-    const int p = pmin + bx*BX + int(threadIdx.x);
+    const int p = pmin + off;
     const int K = 2*p + q;
     if (K >= rg.E[2]) return;
     const unsigned sI = L.sI, Hz = L.Hz;
@@ -308,6 +297,48 @@ __device__ __forceinline__ void rbgs_body(const Bid bid, const LevelDims& L,
       + a[5]*nb[4]
       + a[6]*nb[5];
     st(xc, idx, (bv - P)/a[0]);
+}
+
+// ---------------------------------------------------------------------------
+// Red-black Gauss-Seidel phase: update every cell of one colour inside
+// region [S,E) (rbgsSmoother.C:65-183):
+//   x_c = (b_c - (A1 x_-i + A2 x_+i + A3 x_-j + A4 x_+j + A5 x_-k + A6 x_+k))
+//         / A0
+// One thread per cell of the colour; lanes run along the contiguous axis;
+// tiles of BX positions x BY rows, one tile row per plane, dealt to the
+// XCDs in contiguous runs of planes.  Rows of at most 32 cells of a colour
+// are packed: a wavefront covers 64 >> ws rows of 2^ws cells (tl.ws < 6),
+// so short rows do not leave most lanes of a wavefront idle.
+template <typename T, int OP>
+__device__ __forceinline__ void rbgs_body(const Bid bid, const LevelDims& L,
+                                          const OpView<T>& op,
+                                          T* __restrict__ xc,
+                                          const T* __restrict__ xo,
+                                          const T* __restrict__ bcl,
+                                          int colour, const Region& rg,
+                                          const Tiles& tl)
+{
+    int bx, by, bz;
+    if (!tile_of(tl, bid.x, bx, by, bz)) return;
+    const int I = rg.S[0] + bz;
+    if (tl.ws == 6)
+    {
+        // a wavefront is one row (BX = 64), so J and everything derived
+        // from it is uniform and lives in scalar registers
+        const int J = __builtin_amdgcn_readfirstlane(rg.S[1] + by*BY +
+                                                     int(threadIdx.y));
+        rbgs_cell<T, OP>(L, op, xc, xo, bcl, colour, rg, I, J,
+                         bx*BX + int(threadIdx.x));
+    }
+    else
+    {
+        const int ws = tl.ws;
+        const int lane = int(threadIdx.x);
+        const int J = rg.S[1] + (by*BY + int(threadIdx.y))*(64 >> ws) +
+                      (lane >> ws);
+        rbgs_cell<T, OP>(L, op, xc, xo, bcl, colour, rg, I, J,
+                         (bx << ws) + (lane & ((1 << ws) - 1)));
+    }
 }
 
 template <typename T, int OP>
@@ -2862,7 +2893,23 @@ Tiles make_tiles(int gx, int gy, int gz)
     t.gy = gy;
     t.gz = gz;
     t.per = (gx*gy*gz + 7)/8;
+    t.ws = 6;
     return t;
+}
+
+// Cells of a row per wavefront row for rows of n cells: the width among
+// 64, 32, 16 and 8 that leaves the fewest lanes idle (ties: the wider), as
+// its log2.
+int row_pack_shift(int n)
+{
+    int best = 6, waste = ((n + 63)/64)*64 - n;
+    for (int ws = 5; ws >= 3; ws--)
+    {
+        const int w = 1 << ws;
+        const int x = ((n + w - 1)/w)*w - n;
+        if (x < waste) { best = ws; waste = x; }
+    }
+    return best;
 }
 
 inline dim3 tiles_grid(const Tiles& t) { return dim3(8*t.per); }
@@ -3126,11 +3173,14 @@ void launch_rbgs(const LevelDims& L, const OpView<T>& op, Pair<T> x,
 {
     if (rg.E[0] <= rg.S[0] || rg.E[1] <= rg.S[1] || rg.E[2] <= rg.S[2]) return;
     // a row of the region holds at most (nK + 1)/2 cells of one colour,
-    // counted from the first of them (pmin in rbgs_body)
+    // counted from the first of them (pmin in rbgs_cell)
     const int nK = rg.E[2] - rg.S[2];
-    const Tiles tl = make_tiles(((nK + 1)/2 + BX - 1)/BX,
-                                (rg.E[1] - rg.S[1] + BY - 1)/BY,
-                                rg.E[0] - rg.S[0]);
+    const int ws = row_pack_shift((nK + 1)/2);
+    const int rows = BY*(64 >> ws);
+    Tiles tl = make_tiles(((nK + 1)/2 + (1 << ws) - 1) >> ws,
+                          (rg.E[1] - rg.S[1] + rows - 1)/rows,
+                          rg.E[0] - rg.S[0]);
+    tl.ws = ws;
     const dim3 g = tiles_grid(tl);
     T* xc = x.c[colour];
     const T* xo = x.c[colour ^ 1];
