@@ -211,12 +211,24 @@ struct PairPos {
     bool vB;          // K = 2p+2 is an interior cell
 };
 
+// PACK: a wavefront covers 64 >> ws rows of 2^ws pairs (tiles of
+// pair_tiles_packed); else one row segment of 64 pairs, J uniform.
+template <bool PACK = false>
 __device__ __forceinline__ bool pair_pos(const LevelDims& L, int bx, int by,
-                                         int bz, PairPos& c)
+                                         int bz, PairPos& c, int ws = 6)
 {
     c.I = 1 + bz;
-    c.J = __builtin_amdgcn_readfirstlane(1 + by*BY + int(threadIdx.y));
-    c.p = bx*BX + int(threadIdx.x);
+    if (PACK)
+    {
+        const int lane = int(threadIdx.x);
+        c.J = 1 + (by*BY + int(threadIdx.y))*(64 >> ws) + (lane >> ws);
+        c.p = (bx << ws) + (lane & ((1 << ws) - 1));
+    }
+    else
+    {
+        c.J = __builtin_amdgcn_readfirstlane(1 + by*BY + int(threadIdx.y));
+        c.p = bx*BX + int(threadIdx.x);
+    }
     if (c.J > L.N[1] || c.p >= (L.N[2] + 1)/2) return false;
     c.cB = (c.I + c.J) & 1;
     c.cA = c.cB ^ 1;
@@ -829,8 +841,8 @@ __global__ __launch_bounds__(BX*BY) void k_copy_region(LevelDims L,
 // Residual r = b - A x (linearSystemResidual.C:75-95; row product in
 // C,L,R,B,T,A,F order, linearSystemFunctions.H:42-49) over the interior.
 // Optionally writes per-tile partial sums of r^2 (deterministic order).
-template <typename T, int OP, bool NORM>
-__device__ __forceinline__ void residual_body(const Bid bid,
+template <typename T, int OP, bool NORM, bool PACK>
+__device__ __forceinline__ void residual_impl(const Bid bid,
                                               const LevelDims& L,
                                               const OpView<T>& op,
                                               Pair<const T> x,
@@ -844,7 +856,7 @@ __device__ __forceinline__ void residual_body(const Bid bid,
     if (!tile_of(tl, bid.x, bx, by, bz)) return;
     PairPos c;
     double acc = 0.0;
-    if (pair_pos(L, bx, by, bz, c))
+    if (pair_pos<PACK>(L, bx, by, bz, c, tl.ws))
     {
         const T bA = ld(col(b, c.cA), c.idx), bB = ld(col(b, c.cB), c.idx + 1);
         T Ax[2];
@@ -864,6 +876,25 @@ __device__ __forceinline__ void residual_body(const Bid bid,
         if (threadIdx.x == 0 && threadIdx.y == 0)
             partials[tile_index(tl, bx, by, bz)] = s;
     }
+}
+
+// Rows of short levels are packed (tl.ws < 6) when no norm is taken; the
+// norm's tiles keep one row segment per wavefront so that its partial sums
+// keep their order.
+template <typename T, int OP, bool NORM>
+__device__ __forceinline__ void residual_body(const Bid bid,
+                                              const LevelDims& L,
+                                              const OpView<T>& op,
+                                              Pair<const T> x,
+                                              Pair<const T> b,
+                                              Pair<T> r,
+                                              const Tiles& tl,
+                                              double* partials)
+{
+    if (!NORM && tl.ws != 6)
+        residual_impl<T, OP, NORM, true>(bid, L, op, x, b, r, tl, partials);
+    else
+        residual_impl<T, OP, NORM, false>(bid, L, op, x, b, r, tl, partials);
 }
 
 template <typename T, int OP, bool NORM>
@@ -911,8 +942,8 @@ struct KResidual
 //
 // One thread per pair of coarse cells K = 2p+1, 2p+2 (cell-pair tiles of
 // the coarse level); the 16 fine values are loaded before any arithmetic.
-template <typename T>
-__device__ __forceinline__ void restrict_body(const Bid bid,
+template <typename T, bool PACK>
+__device__ __forceinline__ void restrict_impl(const Bid bid,
                                               const LevelDims& Lf,
                                               const LevelDims& Lc,
                                               const TransferView<T>& tv,
@@ -923,7 +954,7 @@ __device__ __forceinline__ void restrict_body(const Bid bid,
     int bx, by, bz;
     if (!tile_of(tl, bid.x, bx, by, bz)) return;
     PairPos c;
-    if (!pair_pos(Lc, bx, by, bz, c)) return;
+    if (!pair_pos<PACK>(Lc, bx, by, bz, c, tl.ws)) return;
     const int I = c.I, J = c.J;
     const int R0 = Lc.R[0], R1 = Lc.R[1], R2 = Lc.R[2];
     const int ox = R0 > 1, oy = R1 > 1, oz = R2 > 1;
@@ -993,6 +1024,21 @@ __device__ __forceinline__ void restrict_body(const Bid bid,
 }
 
 template <typename T>
+__device__ __forceinline__ void restrict_body(const Bid bid,
+                                              const LevelDims& Lf,
+                                              const LevelDims& Lc,
+                                              const TransferView<T>& tv,
+                                              Pair<const T> rf,
+                                              Pair<T> bc,
+                                              const Tiles& tl)
+{
+    if (tl.ws != 6)
+        restrict_impl<T, true>(bid, Lf, Lc, tv, rf, bc, tl);
+    else
+        restrict_impl<T, false>(bid, Lf, Lc, tv, rf, bc, tl);
+}
+
+template <typename T>
 __global__ __launch_bounds__(BX*BY) void k_restrict(LevelDims Lf, LevelDims Lc,
                                                     TransferView<T> tv,
                                                     Pair<const T> rf,
@@ -1039,8 +1085,8 @@ struct KRestrict
 // indices lie in zbase..zbase+nz-1 (R = 2: parents p+1, neighbours p and
 // p+2; R = 1: cells 2p+1, 2p+2 with neighbours 2p and 2p+3), loaded once
 // for the 4 coarse columns.
-template <typename T>
-__device__ __forceinline__ void prolong_body(const Bid bid,
+template <typename T, bool PACK>
+__device__ __forceinline__ void prolong_impl(const Bid bid,
                                              const LevelDims& Lf,
                                              const LevelDims& Lc,
                                              const TransferView<T>& tv,
@@ -1051,7 +1097,7 @@ __device__ __forceinline__ void prolong_body(const Bid bid,
     int bx, by, bz;
     if (!tile_of(tl, bid.x, bx, by, bz)) return;
     PairPos c;
-    if (!pair_pos(Lf, bx, by, bz, c)) return;
+    if (!pair_pos<PACK>(Lf, bx, by, bz, c, tl.ws)) return;
     const int I = c.I, J = c.J, p = c.p;
     const int i = I - 1, j = J - 1;
     const int offx = (i & 1) ? 1 : -1, offy = (j & 1) ? 1 : -1;
@@ -1119,6 +1165,21 @@ __device__ __forceinline__ void prolong_body(const Bid bid,
     }
     st(col(xf, c.cA), c.idx, old[0] + add[0]);
     if (c.vB) st(col(xf, c.cB), c.idx + 1, old[1] + add[1]);
+}
+
+template <typename T>
+__device__ __forceinline__ void prolong_body(const Bid bid,
+                                             const LevelDims& Lf,
+                                             const LevelDims& Lc,
+                                             const TransferView<T>& tv,
+                                             Pair<const T> xc,
+                                             Pair<T> xf,
+                                             const Tiles& tl)
+{
+    if (tl.ws != 6)
+        prolong_impl<T, true>(bid, Lf, Lc, tv, xc, xf, tl);
+    else
+        prolong_impl<T, false>(bid, Lf, Lc, tv, xc, xf, tl);
 }
 
 template <typename T>
@@ -2923,6 +2984,18 @@ Tiles pair_tiles(const LevelDims& L)
                       L.N[0]);
 }
 
+// the same with short rows packed (the bodies that read tl.ws)
+Tiles pair_tiles_packed(const LevelDims& L)
+{
+    const int n = (L.N[2] + 1)/2;
+    const int ws = row_pack_shift(n);
+    const int rows = BY*(64 >> ws);
+    Tiles t = make_tiles((n + (1 << ws) - 1) >> ws,
+                         (L.N[1] + rows - 1)/rows, L.N[0]);
+    t.ws = ws;
+    return t;
+}
+
 BcVals bcvals(const double* v)
 {
     BcVals b;
@@ -3356,7 +3429,7 @@ void launch_residual(const LevelDims& L, const OpView<T>& op, Pair<const T> x,
                      Pair<const T> b, Pair<T> r, double* partials,
                      int* nblocks, hipStream_t s)
 {
-    const Tiles tl = pair_tiles(L);
+    const Tiles tl = partials ? pair_tiles(L) : pair_tiles_packed(L);
     const dim3 g = tiles_grid(tl);
     if (nblocks) *nblocks = tiles_count(tl);
 #define BMG_RES(OPK, NRM)                                                     \
@@ -3381,7 +3454,7 @@ void launch_restrict(const LevelDims& Lf, const LevelDims& Lc,
                      const TransferView<T>& tv, Pair<const T> rf, Pair<T> bc,
                      hipStream_t s)
 {
-    const Tiles tl = pair_tiles(Lc);
+    const Tiles tl = pair_tiles_packed(Lc);
     dispatch<KRestrict<T>>(tiles_grid(tl), 0, s, {Lf, Lc, tv, rf, bc, tl});
 }
 
@@ -3390,7 +3463,7 @@ void launch_prolong(const LevelDims& Lf, const LevelDims& Lc,
                     const TransferView<T>& tv, Pair<const T> xc, Pair<T> xf,
                     hipStream_t s)
 {
-    const Tiles tl = pair_tiles(Lf);
+    const Tiles tl = pair_tiles_packed(Lf);
     dispatch<KProlong<T>>(tiles_grid(tl), 0, s, {Lf, Lc, tv, xc, xf, tl});
 }
 
