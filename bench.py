@@ -23,7 +23,9 @@ FP64 defect correction), and flexible CG preconditioned by one FP64 or mixed
 MG cycle.  ``value`` is the fastest method on the headline workload; every
 method is reported with iterations, final residual and its error against an
 FP64 reference solution; ``equal_error`` times plain FP64 MG and FCG+mixed
-again to the tolerance at which each one's error falls below 1e-8.
+again to the tolerance at which each one's error falls below 1e-8, and
+``seeds`` times both on seeds 1-4 of the generator and reports the medians
+over seeds 0-4.
 
 The roofline object covers the dominant kernel of the plain FP64 solve, the
 level-0 red-black smoother phase, timed live with HIP events on the library's
@@ -186,9 +188,22 @@ def reference_solution(name, prob):
             f"relative residual ({r['iters']} iterations) on the device")
 
 
+def cpu_cores():
+    """Host cores this process may use (the GPU box's share when
+    OMP_NUM_THREADS states it)."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
 def cpu_baseline(prob, iters_fp64, sample_iters, n):
-    """The CPU restatement of the reference MG (one core) on a bounded
-    sample of the same FP64 solve."""
+    """The CPU restatement of the reference MG on a bounded sample of the
+    same FP64 solve, on one core and on the host's P cores (OpenMP over the
+    planes of every per-cell loop; the same bits on any core count)."""
     import oracle
     o = oracle.Oracle(prob.vertices, prob.bc, prob.bc_value)
     opts = solver_opts(0)
@@ -200,23 +215,33 @@ def cpu_baseline(prob, iters_fp64, sample_iters, n):
     o.pin_and_eliminate()
     o.prepare_coarse()
     o.residual(0, False)
-    t0 = time.perf_counter()
-    for _ in range(sample_iters):
-        o.cycle()
-        o.residual(0, True)
-        o.sum_sq_residual()
-    per_iter = (time.perf_counter() - t0) / sample_iters
+    cores = cpu_cores()
+    per = {}
+    prev = oracle.get_threads()
+    try:
+        for nt in sorted({1, cores}):
+            oracle.set_threads(nt)
+            t0 = time.perf_counter()
+            for _ in range(sample_iters):
+                o.cycle()
+                o.residual(0, True)
+                o.sum_sq_residual()
+            per[nt] = (time.perf_counter() - t0) / sample_iters
+    finally:
+        oracle.set_threads(prev)
     del o
+    host = platform.processor() or platform.machine()
     return {
-        "value": round(per_iter * iters_fp64 * 1e3, 1),
+        "value": round(per[cores] * iters_fp64 * 1e3, 1),
         "unit": "ms",
-        "cores": 1,
+        "cores": cores,
         "kind": "port",
-        "sample": (f"{sample_iters} V(2,2) iterations (cycle + residual + norm) "
-                   f"of the same {n}^3 FP64 solve on one host core "
-                   f"({per_iter * 1e3:.0f} ms each), scaled by the "
-                   f"{iters_fp64} iterations the solve takes; "
-                   f"host {platform.processor() or platform.machine()}"),
+        "sample": (f"{sample_iters} V(2,2) iterations (cycle + residual + "
+                   f"norm) of the same {n}^3 FP64 solve, timed on {cores} "
+                   f"host cores ({per[cores] * 1e3:.0f} ms each) and on one "
+                   f"core ({per[1] * 1e3:.0f} ms each), scaled by the "
+                   f"{iters_fp64} iterations the solve takes; host {host}"),
+        "one_core_ms": round(per[1] * iters_fp64 * 1e3, 1),
     }
 
 
@@ -233,6 +258,60 @@ def roofline(result, kernel, traffic):
 
 
 TARGET_ERROR = 1e-8
+
+
+def _spheres_problem(args):
+    n, seed = args
+    from briscola_amd import workloads
+    return workloads.spheres(n, seed=seed)
+
+
+# Problems of the extra headline seeds, generated by worker processes that
+# are started before this process touches the GPU (see main).
+SEED_JOBS = {}
+
+
+def start_seed_jobs(args):
+    import multiprocessing
+    from concurrent.futures import ProcessPoolExecutor
+    seeds = list(range(1, args.seeds))
+    if not seeds:
+        return None
+    pool = ProcessPoolExecutor(max_workers=min(4, len(seeds)),
+                               mp_context=multiprocessing.get_context("fork"))
+    for s in seeds:
+        SEED_JOBS[s] = pool.submit(_spheres_problem, (args.n, s))
+    return pool
+
+
+def seed_solves(results, args):
+    """Plain FP64 MG and FCG+mixed on the headline generator with seeds
+    1..S-1 (seed 0 is the main measurement): one timed solve each (FCG+mixed
+    after one untimed solve), and the medians over seeds 0..S-1."""
+    import statistics
+    from briscola_amd import bmg
+    rows = {0: {m: (round(results[m]["ms"], 3), results[m]["iters"])
+                for m in ("fp64", "fcg_mixed")}}
+    for seed in sorted(SEED_JOBS):
+        prob = SEED_JOBS.pop(seed).result()
+        row = {}
+        for mname, prec, kry, warm in (("fp64", bmg.FP64, bmg.KRYLOV_NONE, 0),
+                                       ("fcg_mixed", bmg.MIXED,
+                                        bmg.KRYLOV_FCG, 1)):
+            r = run_gpu(prob, prec, kry, 1, warm, False)
+            row[mname] = (round(r["ms"], 3), r["iters"])
+        rows[seed] = row
+        del prob
+        log(f"[bench] spheres seed {seed}: fp64 {row['fp64'][0]:.1f} ms "
+            f"({row['fp64'][1]} it), fcg_mixed {row['fcg_mixed'][0]:.1f} ms "
+            f"({row['fcg_mixed'][1]} it)")
+    med = {m: statistics.median(rows[s][m][0] for s in rows)
+           for m in ("fp64", "fcg_mixed")}
+    return {"seeds": sorted(rows),
+            "ms_iters": {str(s): rows[s] for s in sorted(rows)},
+            "median_ms": med,
+            "median_speedup_fcg_mixed_vs_fp64": round(
+                med["fp64"] / med["fcg_mixed"], 3)}
 
 
 def equal_error_solves(prob, results, xe, args):
@@ -320,6 +399,10 @@ def run_workload(name, args):
         for r in results.values():
             r["err_vs_fp64_reference"] = direct.rel_error(
                 r["x"][1:-1, 1:-1, 1:-1], xe, True)
+    seeds = None
+    if (name == "spheres" and SEED_JOBS and "fp64" in results
+            and "fcg_mixed" in results):
+        seeds = seed_solves(results, args)
     equal = None
     if (name == "spheres" and not args.no_check and not args.no_equal_error
             and "fp64" in results and "fcg_mixed" in results):
@@ -358,6 +441,8 @@ def run_workload(name, args):
             for k, v in results.items()}
     if equal is not None:
         out["equal_error"] = equal
+    if seeds is not None:
+        out["seeds"] = seeds
     if fft is not None:
         out["fft_solver"] = {
             "method": ("FFT(x,z) + tridiagonal(y) direct solve of the "
@@ -519,6 +604,8 @@ def main():
                     help="skip the FP64 reference solution")
     ap.add_argument("--only", choices=["fp64", "mixed", "fcg_fp64", "fcg_mixed",
                                        "fcg_learned_mixed"], default=None)
+    ap.add_argument("--seeds", type=int, default=5,
+                    help="headline seeds 0..S-1 for the median (1: seed 0 only)")
     ap.add_argument("--no-equal-error", action="store_true",
                     help="skip the solves to a 1e-8 error on the headline")
     ap.add_argument("--no-learned", action="store_true",
@@ -537,6 +624,12 @@ def main():
               "(each MI355X runs one brick; see the sharding design).",
               file=sys.stderr)
         return 2
+
+    # workers for the extra seeds fork now, before any GPU work
+    pool = None
+    if (args.workload in ("spheres", "both", "all") and args.seeds > 1
+            and args.only is None):
+        pool = start_seed_jobs(args)
 
     if args.workload == "batch":
         out = run_batch(args)
@@ -576,12 +669,16 @@ def main():
         line["speedup_vs_plain_fp64_mg"] = main_out["speedup_vs_plain_fp64_mg"]
     if "equal_error" in main_out:
         line["equal_error"] = main_out["equal_error"]
+    if "seeds" in main_out:
+        line["seeds"] = main_out["seeds"]
     if "learned" in main_out:
         line["learned"] = main_out["learned"]
     if extra is not None:
         line["channel256"] = extra
     if batch is not None:
         line["batch512"] = batch
+    if pool is not None:
+        pool.shutdown(wait=True, cancel_futures=True)
     print(json.dumps(line), flush=True)
     return 0
 

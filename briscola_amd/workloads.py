@@ -227,12 +227,22 @@ def spheres(n=192, seed=0, nspheres=32, rho_out=1000.0, rho_in=1.0):
     # padded cell centres (ghosts mirrored) so that face averages exist
     xp = [np.concatenate(([2 * a[0] - a[1]], a, [2 * a[-1] - a[-2]])) for a in xc]
     phi = np.full(tuple(len(a) for a in xp), np.inf)
-    X = xp[0][:, None, None]
-    Y = xp[1][None, :, None]
-    Z = xp[2][None, None, :]
+    # each sphere's signed distance only inside its bounding box grown by
+    # 4h: outside it the distance exceeds the smoothing width 1.5h, where
+    # the smoothed Heaviside is saturated, so the result is the same as
+    # with the distance everywhere
     for s in range(nspheres):
+        sl = []
+        for d in range(3):
+            lo = np.searchsorted(xp[d], c[s, d] - (r[s] + 4 * h), "left")
+            hi = np.searchsorted(xp[d], c[s, d] + (r[s] + 4 * h), "right")
+            sl.append(slice(lo, hi))
+        X = xp[0][sl[0], None, None]
+        Y = xp[1][None, sl[1], None]
+        Z = xp[2][None, None, sl[2]]
         d = np.sqrt((X - c[s, 0]) ** 2 + (Y - c[s, 1]) ** 2 + (Z - c[s, 2]) ** 2) - r[s]
-        np.minimum(phi, d, out=phi)
+        sub = phi[sl[0], sl[1], sl[2]]
+        np.minimum(sub, d, out=sub)
     Hin = _smooth_heaviside(-phi, 1.5 * h)  # 1 inside spheres
     rho = rho_out + (rho_in - rho_out) * Hin
     inv = 1.0 / rho

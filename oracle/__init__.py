@@ -32,7 +32,7 @@ def build(quiet=True):
     rounding sequence)."""
     import subprocess
     cmd = ["g++", "-O3", "-march=x86-64-v3", "-ffp-contract=off", "-fPIC",
-           "-shared", "-std=c++17", "-o", LIB_PATH,
+           "-fopenmp", "-shared", "-std=c++17", "-o", LIB_PATH,
            os.path.join(_HERE, "bmg_oracle.cpp")]
     subprocess.run(cmd, check=True, capture_output=quiet)
 
@@ -51,6 +51,8 @@ def _load():
     sig = {
         "oracle_create": (P, [I, D, D, D, I, D]),
         "oracle_destroy": (None, [P]),
+        "oracle_set_threads": (None, [c_int]),
+        "oracle_get_threads": (c_int, []),
         "oracle_set_opts": (None, [P, c_double, c_double, c_int, c_int, c_int,
                                    c_int, c_int, c_int, c_int, c_int]),
         "oracle_num_levels": (c_int, [P]),
@@ -87,6 +89,16 @@ def _load():
         f.argtypes = args
     _lib = lib
     return lib
+
+
+def set_threads(n):
+    """Threads of the oracle's per-cell loops (OpenMP over planes; every
+    result is the same for any count)."""
+    _load().oracle_set_threads(int(n))
+
+
+def get_threads():
+    return int(_load().oracle_get_threads())
 
 
 def _dp(a):

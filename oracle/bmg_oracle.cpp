@@ -38,6 +38,8 @@
 #include <cstring>
 #include <vector>
 
+#include <omp.h>
+
 namespace {
 
 constexpr int NEUMANN = ORACLE_NEUMANN;
@@ -371,14 +373,19 @@ void rbgsRegion(Level& lv, const int S[3], const int E[3])
         x[c] = (b[c] - P)/a[0];
     };
 
+    (void)Jj;
+    // the cells of one phase read only cells of the other parity, so the
+    // planes of a phase may be updated in any order (same bits)
+    const bool par = long(M[0])*M[1]*M[2] > (1L << 15);
     for (int phase = 0; phase < 4; phase++)
     {
         const bool fwd = phase < 2;
         const int want = phase % 2;          // 0 even, 1 odd
-        long c = fwd ? lv.lin(S[0], S[1], S[2])
-                     : lv.lin(E[0]-1, E[1]-1, E[2]-1);
+        #pragma omp parallel for schedule(static) if (par)
         for (int i = 0; i < M[0]; i++)
         {
+            long c = fwd ? lv.lin(S[0] + i, S[1], S[2])
+                         : lv.lin(E[0] - 1 - i, E[1] - 1, E[2] - 1);
             for (int j = 0; j < M[1]; j++)
             {
                 // first k with (i+j+k)%2 == want
@@ -387,7 +394,6 @@ void rbgsRegion(Level& lv, const int S[3], const int E[3])
                     upd(fwd ? c + k : c - k);
                 c += fwd ? (M[2] + Jk) : -(M[2] + Jk);
             }
-            c += fwd ? Jj : -Jj;
         }
     }
 }
@@ -499,6 +505,8 @@ void residualRegion(Level& lv, const int S[3], const int E[3])
 {
     const long Si = long(lv.P[1])*lv.P[2];
     const long Sj = lv.P[2];
+    const bool par = long(E[0] - S[0])*(E[1] - S[1])*(E[2] - S[2]) > (1L << 15);
+    #pragma omp parallel for schedule(static) if (par)
     for (int i = S[0]; i < E[0]; i++)
     for (int j = S[1]; j < E[1]; j++)
     {
@@ -595,6 +603,8 @@ void restrictScaled(oracle_ctx& c, int l)
     Level& coarse = c.L[l+1];
     const int* R = coarse.R;
     const int ox = R[0] > 1, oy = R[1] > 1, oz = R[2] > 1;
+    const bool par = long(coarse.N[0])*coarse.N[1]*coarse.N[2] > (1L << 12);
+    #pragma omp parallel for schedule(static) if (par)
     for (int i = 1; i <= coarse.N[0]; i++)
     for (int j = 1; j <= coarse.N[1]; j++)
     for (int k = 1; k <= coarse.N[2]; k++)
@@ -678,18 +688,19 @@ void prolongAdd(oracle_ctx& c, int l)
     Level& fine = c.L[l];
     Level& coarse = c.L[l+1];
     correctBoundaryConditions(c, l + 1, coarse, coarse.x);
-    size_t q = 0;
+    const bool par = long(fine.N[0])*fine.N[1]*fine.N[2] > (1L << 15);
+    #pragma omp parallel for schedule(static) if (par)
     for (int i = 1; i <= fine.N[0]; i++)
     for (int j = 1; j <= fine.N[1]; j++)
     for (int k = 1; k <= fine.N[2]; k++)
     {
+        const size_t q = (size_t(i - 1)*fine.N[1] + (j - 1))*fine.N[2] + (k - 1);
         double value = 0.0;
         for (int v = 0; v < 8; v++)
             if (fine.pw[q*8 + v] != 0.0)
                 value += fine.pw[q*8 + v]*coarse.x[fine.pidx[q*8 + v]];
         double& xf = fine.x[fine.lin(i, j, k)];
         xf = xf + value;
-        q++;
     }
 }
 
@@ -1150,6 +1161,10 @@ oracle_ctx* oracle_create(const int n[3], const double* vx, const double* vy,
 }
 
 void oracle_destroy(oracle_ctx* c) { delete c; }
+
+/* Threads of the per-cell loops (OpenMP; results do not depend on it). */
+void oracle_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int oracle_get_threads(void) { return omp_get_max_threads(); }
 
 void oracle_set_opts(oracle_ctx* c, double rel_tol, double tolerance,
                      int min_iter, int max_iter, int n_sweeps_pre,

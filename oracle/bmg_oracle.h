@@ -38,6 +38,8 @@ oracle_ctx* oracle_create(const int n[3], const double* vx, const double* vy,
                           const double* vz, const int bc[6],
                           const double bc_value[6]);
 void oracle_destroy(oracle_ctx*);
+void oracle_set_threads(int n);
+int oracle_get_threads(void);
 
 /* Solver dictionary (MG.C:257-304, solver.C:114-123). */
 void oracle_set_opts(oracle_ctx*, double rel_tol, double tolerance,

@@ -577,3 +577,34 @@ def test_iteration_count_reported_w_f_cycles():
     # W and F visit coarse levels more often than V: never slower to converge
     assert its[oracle.WCYCLE] <= its[oracle.VCYCLE]
     assert its[oracle.FCYCLE] <= its[oracle.VCYCLE]
+
+
+@pytest.mark.parametrize("case", ["spheres", "channel"])
+def test_oracle_threads_do_not_change_results(case):
+    """The oracle's per-cell loops run on OpenMP threads (the CPU baseline
+    on many cores); a cycle plus residual must give the same bits for any
+    thread count, periodic slabs included."""
+    prob = (W.spheres(48, seed=3, nspheres=6) if case == "spheres"
+            else W.channel(32, seed=2))
+    out = []
+    prev = oracle.get_threads()
+    try:
+        for nt in (1, 4):
+            oracle.set_threads(nt)
+            o = oracle.Oracle(prob.vertices, prob.bc, prob.bc_value)
+            o.set_opts(rel_tol=1e-10, tolerance=0.0, n_sweeps_pre=2,
+                       n_sweeps_post=2, cycle=oracle.VCYCLE,
+                       coarse_mode=oracle.COARSE_DIRECT, coarse_level=1)
+            o.assemble(prob.lam)
+            o.set_field(oracle.B, 0, prob.b)
+            o.pin_and_eliminate()
+            o.prepare_coarse()
+            for _ in range(2):
+                o.cycle()
+                o.residual(0, True)
+            out.append((o.field(oracle.X, 0).copy(),
+                        o.field(oracle.R, 0).copy()))
+    finally:
+        oracle.set_threads(prev)
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
