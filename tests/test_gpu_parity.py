@@ -57,6 +57,12 @@ CASES = {
     "thin1": lambda: thin_periodic(
         (8, 12, 1), (W.PERIODIC, W.PERIODIC, W.DIRICHLET, W.DIRICHLET,
                      W.PERIODIC, W.PERIODIC), seed=7),
+    # rows of 64 and 128 cells of a colour: one row segment per wavefront
+    # (the large-level form; shorter rows are packed several to a wavefront)
+    "neumann_longz": lambda: W.uniform_neumann((8, 12, 128), seed=8),
+    "spheres_longz": lambda: W.spheres((8, 12, 128), seed=8, nspheres=4),
+    "channel_longz": lambda: W.channel((8, 16, 128), seed=8),
+    "neumann_z256": lambda: W.uniform_neumann((6, 8, 256), seed=9),
 }
 
 
@@ -138,7 +144,8 @@ def test_level_ops_bitwise(gpu, case, native):
 @pytest.mark.parametrize("native", [False, True], ids=["stencil", "native"])
 @pytest.mark.parametrize("case", ["neumann16", "channel16", "spheres16",
                                   "mixed16", "aniso", "neumann48",
-                                  "channel48", "spheres48"])
+                                  "channel48", "spheres48", "neumann_longz",
+                                  "spheres_longz", "channel_longz"])
 def test_solve_bitwise(gpu, case, native):
     prob = CASES[case]()
     print(f"[solve] case={case} native={native}", flush=True)
