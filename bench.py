@@ -1,7 +1,7 @@
 #!/usr/bin/env python
 """Benchmark: 256^3 pressure Poisson solve to 1e-8 relative residual.
 
-Headline workload (the north-star case): the variable-coefficient pressure
+Headline workload: the variable-coefficient pressure
 equation div(rho^-1 grad p) = f on a 256^3 Neumann unit box with density
 ratio 1000 (rho = 1000 outside, 1 inside 32 seeded spheres, interface smoothed
 over 1.5 cells; f = divergence of a seeded random face velocity).  Secondary

@@ -82,7 +82,7 @@ def load():
     if _lib is not None:
         return _lib
     if not os.path.exists(LIB_PATH):
-        raise ImportError(f"{LIB_PATH} is missing: run __graft_entry__.build()")
+        raise ImportError(f"{LIB_PATH} is missing: run python briscola_amd/build.py")
     lib = ctypes.CDLL(LIB_PATH)
     P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
     DP = ctypes.POINTER(ctypes.c_double)
