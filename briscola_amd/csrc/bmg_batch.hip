@@ -345,12 +345,20 @@ struct bmg_batch
     std::unique_ptr<BatchPlan> iter;
     std::vector<long> iterGen;   // context generations the plan was built on
     std::vector<int> iterMixed;
+    // flexible CG: the iteration (captured), the restart p = M r and the
+    // confirmation on the true residual
+    std::unique_ptr<BatchPlan> fcgIter, fcgRestart, fcgConfirm;
+    std::vector<long> fcgGen;
+    std::vector<int> fcgMixed;
     bool tune = true;
     int cus = 256;
 
     ~bmg_batch()
     {
         iter.reset();
+        fcgIter.reset();
+        fcgRestart.reset();
+        fcgConfirm.reset();
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -373,6 +381,146 @@ std::unique_ptr<BatchPlan> recordPlan(bmg_batch* B, F&& phase)
 }
 
 bool mixedOf(const bmg_ctx* c) { return c->opts.precision == BMG_MIXED; }
+
+// Flexible CG of every context as bmg_ctx::runFCG runs it (no learned
+// preconditioner): every phase between two host decisions is recorded and
+// run as merged launches over the systems it concerns -- the restart
+// p = M r for the systems that need one, one captured iteration for the
+// systems still iterating, the confirmation on the true residual for the
+// systems whose updated residual met the tolerance.
+void batchFCG(bmg_batch* B, double* const* db, double* const* dx,
+              bool constMatrix, const std::vector<char>& hostPrepared,
+              std::vector<int>& iter, std::vector<double>& init,
+              std::vector<double>& cur)
+{
+    const size_t n = B->ctx.size();
+    bool rebuild = !B->fcgIter;
+    for (size_t s = 0; s < n && !rebuild; s++)
+        rebuild = B->fcgGen[s] != B->ctx[s]->gen ||
+                  B->fcgMixed[s] != int(mixedOf(B->ctx[s]));
+    if (rebuild)
+    {
+        B->fcgIter = recordPlan(B, [](bmg_ctx* c, size_t) {
+            c->enqueueFCGIteration(mixedOf(c) ? 1 : 0);
+        });
+        B->fcgRestart = recordPlan(B, [](bmg_ctx* c, size_t) {
+            c->enqueueFCGRestart(mixedOf(c));
+        });
+        B->fcgConfirm = recordPlan(B, [](bmg_ctx* c, size_t) {
+            c->enqueueFCGConfirm();
+        });
+        B->fcgGen.resize(n);
+        B->fcgMixed.resize(n);
+        for (size_t s = 0; s < n; s++)
+        {
+            B->fcgGen[s] = B->ctx[s]->gen;
+            B->fcgMixed[s] = int(mixedOf(B->ctx[s]));
+        }
+        // timing runs before the solve's fields are initialised below
+        if (B->tune) B->fcgIter->autotune(B->stream, B->cus);
+        B->fcgIter->capture(B->stream);
+    }
+    // copies of x and b, initial residual, sum of x
+    recordPlan(B, [&](bmg_ctx* c, size_t s) {
+        Level& q = c->lv[0];
+        launch_ref_to_split<double>(q.L, db[s], pr(q.b), c->stream);
+        launch_ref_to_split<double>(q.L, dx[s], pr(q.x), c->stream);
+        if (!hostPrepared[s]) c->prepare(constMatrix);
+        c->enqueueFCGStart();
+    })->run(B->stream);
+    std::vector<double> nf(n), ss(n);
+    for (size_t s = 0; s < n; s++) ss[s] = B->ctx[s]->hScalar[0];
+    recordPlan(B, [&](bmg_ctx* c, size_t) {
+        c->enqueueFCGNormTerms(c->hScalar[1]/double(c->lv[0].cells()));
+    })->run(B->stream);
+    auto check = [&](size_t s, double v) -> int
+    {
+        const bmg_opts& o = B->ctx[s]->opts;
+        return (init[s] > 0.0 && v > o.tolerance &&
+                v/(init[s] + 1e-20) > o.rel_tol) ? 0 : 1;
+    };
+    std::vector<int> conv(n, 0);
+    std::vector<char> restart(n, 1), done(n, 0);
+    for (size_t s = 0; s < n; s++)
+    {
+        bmg_ctx* c = B->ctx[s];
+        const double cells = double(c->lv[0].cells());
+        nf[s] = std::max(std::sqrt(c->hScalar[2]), cells*1e-12);
+        init[s] = std::sqrt(ss[s])/nf[s];
+        if (!std::isfinite(init[s]))
+            throw Error(BMG_ERR_NUMERIC, "initial residual is not finite");
+        cur[s] = init[s];
+        conv[s] = check(s, cur[s]);
+    }
+    std::vector<char> go(n), rs(n), cf(n), lastGo;
+    for (;;)
+    {
+        size_t ngo = 0, nrs = 0;
+        for (size_t s = 0; s < n; s++)
+        {
+            const bmg_opts& o = B->ctx[s]->opts;
+            go[s] = !done[s] && ((iter[s] < o.max_iter && conv[s] == 0) ||
+                                 iter[s] < o.min_iter);
+            rs[s] = go[s] && restart[s];
+            ngo += go[s];
+            nrs += rs[s];
+        }
+        if (ngo == 0) break;
+        if (nrs > 0)
+        {
+            B->fcgRestart->setActive(rs, B->stream);
+            B->fcgRestart->run(B->stream);
+            for (size_t s = 0; s < n; s++)
+                if (rs[s]) restart[s] = 0;
+        }
+        if (go != lastGo)
+        {
+            B->fcgIter->setActive(go, B->stream);
+            lastGo = go;
+        }
+        HIPCK(hipGraphLaunch(B->fcgIter->exec, B->stream));
+        HIPCK(hipStreamSynchronize(B->stream));
+        size_t ncf = 0;
+        for (size_t s = 0; s < n; s++)
+        {
+            cf[s] = 0;
+            if (!go[s]) continue;
+            bmg_ctx* c = B->ctx[s];
+            const double pq = c->hScalar[0];
+            if (!(pq != 0.0) || !std::isfinite(pq))
+                throw Error(BMG_ERR_NUMERIC, "flexible CG breakdown (p.Ap = 0)");
+            cur[s] = std::sqrt(c->hScalar[2])/nf[s];
+            if (!std::isfinite(cur[s]))
+                throw Error(BMG_ERR_NUMERIC, "residual norm is not finite");
+            iter[s]++;
+            conv[s] = check(s, cur[s]);
+            if (conv[s] && iter[s] >= c->opts.min_iter)
+            {
+                cf[s] = 1;
+                ncf++;
+            }
+        }
+        if (ncf == 0) continue;
+        // confirm on the true residual (with the boundary update)
+        B->fcgConfirm->setActive(cf, B->stream);
+        B->fcgConfirm->run(B->stream);
+        for (size_t s = 0; s < n; s++)
+        {
+            if (!cf[s]) continue;
+            cur[s] = std::sqrt(B->ctx[s]->hScalar[0])/nf[s];
+            conv[s] = check(s, cur[s]);
+            if (conv[s]) done[s] = 1;
+            else restart[s] = 1;
+        }
+    }
+    // x and b back, eliminated ghosts, results in the reference layout
+    recordPlan(B, [&](bmg_ctx* c, size_t s) {
+        Level& q = c->lv[0];
+        c->enqueueFCGEnd();
+        launch_split_to_ref<double>(q.L, cpr(q.x), dx[s], c->stream);
+        launch_split_to_ref<double>(q.L, cpr(q.b), db[s], c->stream);
+    })->run(B->stream);
+}
 
 }  // namespace
 
@@ -414,9 +562,12 @@ void batch_solve(bmg_batch* B, double* const* db, double* const* dx,
     {
         bmg_ctx* c = B->ctx[s];
         if (!db[s] || !dx[s]) throw Error(BMG_ERR_ARG, "null field pointer");
-        if (c->opts.krylov != BMG_KRYLOV_NONE)
-            throw Error(BMG_ERR_ARG, "batched solves run MG iterations "
-                                     "(krylov none)");
+        if (c->opts.krylov == BMG_KRYLOV_FCG_LEARNED)
+            throw Error(BMG_ERR_ARG, "the learned preconditioner has no "
+                                     "batched form");
+        if (c->opts.krylov != B->ctx[0]->opts.krylov)
+            throw Error(BMG_ERR_ARG, "every system of a batch uses the same "
+                                     "outer iteration (MG or flexible CG)");
         if (c->opts.smoother != BMG_RBGS)
             throw Error(BMG_ERR_ARG, "batched solves use the rbgs smoother");
         HIPCK(hipStreamSynchronize(c->stream));
@@ -444,99 +595,111 @@ void batch_solve(bmg_batch* B, double* const* db, double* const* dx,
                 c->ensureMixed();
                 HIPCK(hipStreamSynchronize(c->stream));
             }
+            if (c->opts.krylov == BMG_KRYLOV_FCG)
+            {
+                c->prepareFCG();
+                HIPCK(hipStreamSynchronize(c->stream));
+            }
         }
-        // the merged iteration, rebuilt when a context's buffers changed
-        bool rebuild = !B->iter;
-        for (size_t s = 0; s < n && !rebuild; s++)
-            rebuild = B->iterGen[s] != B->ctx[s]->gen ||
-                      B->iterMixed[s] != int(mixedOf(B->ctx[s]));
-        if (rebuild)
+        std::vector<double> init(n), cur(n);
+        std::vector<int> iter(n, 0);
+        if (B->ctx[0]->opts.krylov == BMG_KRYLOV_FCG)
+            batchFCG(B, db, dx, constMatrix, hostPrepared, iter, init, cur);
+        else
         {
-            B->iter = recordPlan(B, [](bmg_ctx* c, size_t) {
-                c->enqueueIteration(mixedOf(c));
-            });
-            B->iterGen.resize(n);
-            B->iterMixed.resize(n);
+            // the merged iteration, rebuilt when a context's buffers changed
+            bool rebuild = !B->iter;
+            for (size_t s = 0; s < n && !rebuild; s++)
+                rebuild = B->iterGen[s] != B->ctx[s]->gen ||
+                          B->iterMixed[s] != int(mixedOf(B->ctx[s]));
+            if (rebuild)
+            {
+                B->iter = recordPlan(B, [](bmg_ctx* c, size_t) {
+                    c->enqueueIteration(mixedOf(c));
+                });
+                B->iterGen.resize(n);
+                B->iterMixed.resize(n);
+                for (size_t s = 0; s < n; s++)
+                {
+                    B->iterGen[s] = B->ctx[s]->gen;
+                    B->iterMixed[s] = int(mixedOf(B->ctx[s]));
+                }
+                // timing runs before the solve's fields are initialised below
+                if (B->tune) B->iter->autotune(B->stream, B->cus);
+                B->iter->capture(B->stream);
+            }
+            // initial residual and the sum of x
+            recordPlan(B, [&](bmg_ctx* c, size_t s) {
+                Level& q = c->lv[0];
+                launch_ref_to_split<double>(q.L, db[s], pr(q.b), c->stream);
+                launch_ref_to_split<double>(q.L, dx[s], pr(q.x), c->stream);
+                if (!hostPrepared[s]) c->prepare(constMatrix);
+                c->enqueueSolveStart();
+            })->run(B->stream);
+            // normFactors
+            std::vector<double> nf(n);
+            std::vector<int> conv(n, 0);
+            recordPlan(B, [&](bmg_ctx* c, size_t) {
+                c->enqueueNormTerms(c->hScalar[1]/double(c->lv[0].cells()));
+            })->run(B->stream);
             for (size_t s = 0; s < n; s++)
             {
-                B->iterGen[s] = B->ctx[s]->gen;
-                B->iterMixed[s] = int(mixedOf(B->ctx[s]));
+                bmg_ctx* c = B->ctx[s];
+                const double cells = double(c->lv[0].cells());
+                nf[s] = std::max(std::sqrt(c->hScalar[2]), cells*1e-12);
+                init[s] = std::sqrt(c->hScalar[0])/nf[s];
+                if (!std::isfinite(init[s]))
+                    throw Error(BMG_ERR_NUMERIC, "initial residual is not finite");
+                cur[s] = init[s];
             }
-            // timing runs before the solve's fields are initialised below
-            if (B->tune) B->iter->autotune(B->stream, B->cus);
-            B->iter->capture(B->stream);
-        }
-        // initial residual and the sum of x
-        recordPlan(B, [&](bmg_ctx* c, size_t s) {
-            Level& q = c->lv[0];
-            launch_ref_to_split<double>(q.L, db[s], pr(q.b), c->stream);
-            launch_ref_to_split<double>(q.L, dx[s], pr(q.x), c->stream);
-            if (!hostPrepared[s]) c->prepare(constMatrix);
-            c->enqueueSolveStart();
-        })->run(B->stream);
-        // normFactors
-        std::vector<double> init(n), nf(n), cur(n);
-        std::vector<int> iter(n, 0), conv(n, 0);
-        recordPlan(B, [&](bmg_ctx* c, size_t) {
-            c->enqueueNormTerms(c->hScalar[1]/double(c->lv[0].cells()));
-        })->run(B->stream);
-        for (size_t s = 0; s < n; s++)
-        {
-            bmg_ctx* c = B->ctx[s];
-            const double cells = double(c->lv[0].cells());
-            nf[s] = std::max(std::sqrt(c->hScalar[2]), cells*1e-12);
-            init[s] = std::sqrt(c->hScalar[0])/nf[s];
-            if (!std::isfinite(init[s]))
-                throw Error(BMG_ERR_NUMERIC, "initial residual is not finite");
-            cur[s] = init[s];
-        }
-        auto check = [&](size_t s, double v) -> int
-        {
-            const bmg_opts& o = B->ctx[s]->opts;
-            return (init[s] > 0.0 && v > o.tolerance &&
-                    v/(init[s] + 1e-20) > o.rel_tol) ? 0 : 1;
-        };
-        for (size_t s = 0; s < n; s++) conv[s] = check(s, cur[s]);
-        // the captured iteration walks the shares of the systems still
-        // iterating; they are rewritten whenever that set changes
-        std::vector<char> active(n, 1), last;
-        for (;;)
-        {
-            size_t nactive = 0;
-            for (size_t s = 0; s < n; s++)
+            auto check = [&](size_t s, double v) -> int
             {
                 const bmg_opts& o = B->ctx[s]->opts;
-                const bool go = (iter[s] < o.max_iter && conv[s] == 0) ||
-                                iter[s] < o.min_iter;
-                active[s] = go;
-                nactive += go;
-            }
-            if (nactive == 0) break;
-            if (active != last)
+                return (init[s] > 0.0 && v > o.tolerance &&
+                        v/(init[s] + 1e-20) > o.rel_tol) ? 0 : 1;
+            };
+            for (size_t s = 0; s < n; s++) conv[s] = check(s, cur[s]);
+            // the captured iteration walks the shares of the systems still
+            // iterating; they are rewritten whenever that set changes
+            std::vector<char> active(n, 1), last;
+            for (;;)
             {
-                B->iter->setActive(active, B->stream);
-                last = active;
+                size_t nactive = 0;
+                for (size_t s = 0; s < n; s++)
+                {
+                    const bmg_opts& o = B->ctx[s]->opts;
+                    const bool go = (iter[s] < o.max_iter && conv[s] == 0) ||
+                                    iter[s] < o.min_iter;
+                    active[s] = go;
+                    nactive += go;
+                }
+                if (nactive == 0) break;
+                if (active != last)
+                {
+                    B->iter->setActive(active, B->stream);
+                    last = active;
+                }
+                HIPCK(hipGraphLaunch(B->iter->exec, B->stream));
+                HIPCK(hipStreamSynchronize(B->stream));
+                for (size_t s = 0; s < n; s++)
+                {
+                    if (!active[s]) continue;
+                    bmg_ctx* c = B->ctx[s];
+                    cur[s] = std::sqrt(c->hScalar[0])/nf[s];
+                    if (!std::isfinite(cur[s]))
+                        throw Error(BMG_ERR_NUMERIC, "residual norm is not finite");
+                    conv[s] = check(s, cur[s]);
+                    iter[s]++;
+                }
             }
-            HIPCK(hipGraphLaunch(B->iter->exec, B->stream));
-            HIPCK(hipStreamSynchronize(B->stream));
-            for (size_t s = 0; s < n; s++)
-            {
-                if (!active[s]) continue;
-                bmg_ctx* c = B->ctx[s];
-                cur[s] = std::sqrt(c->hScalar[0])/nf[s];
-                if (!std::isfinite(cur[s]))
-                    throw Error(BMG_ERR_NUMERIC, "residual norm is not finite");
-                conv[s] = check(s, cur[s]);
-                iter[s]++;
-            }
+            // boundary values of x, results back in the reference layout
+            recordPlan(B, [&](bmg_ctx* c, size_t s) {
+                Level& q = c->lv[0];
+                c->enqueueSolveEnd();
+                launch_split_to_ref<double>(q.L, cpr(q.x), dx[s], c->stream);
+                launch_split_to_ref<double>(q.L, cpr(q.b), db[s], c->stream);
+            })->run(B->stream);
         }
-        // boundary values of x, results back in the reference layout
-        recordPlan(B, [&](bmg_ctx* c, size_t s) {
-            Level& q = c->lv[0];
-            c->enqueueSolveEnd();
-            launch_split_to_ref<double>(q.L, cpr(q.x), dx[s], c->stream);
-            launch_split_to_ref<double>(q.L, cpr(q.b), db[s], c->stream);
-        })->run(B->stream);
         HIPCK(hipEventRecord(e1, B->stream));
         HIPCK(hipEventSynchronize(e1));
         float ms = 0.0f;
@@ -566,8 +729,9 @@ void batch_solve(bmg_batch* B, double* const* db, double* const* dx,
 void batch_info(const bmg_batch* B, int* launches, long* workgroups)
 {
     if (!B) throw Error(BMG_ERR_ARG, "null batch");
-    if (launches) *launches = B->iter ? int(B->iter->launches.size()) : 0;
-    if (workgroups) *workgroups = B->iter ? long(B->iter->workgroups()) : 0;
+    const BatchPlan* p = B->iter ? B->iter.get() : B->fcgIter.get();
+    if (launches) *launches = p ? int(p->launches.size()) : 0;
+    if (workgroups) *workgroups = p ? long(p->workgroups()) : 0;
 }
 
 void batch_destroy(bmg_batch* B) { delete B; }
@@ -577,6 +741,7 @@ void batch_set_autotune(bmg_batch* B, int on)
     if (!B) throw Error(BMG_ERR_ARG, "null batch");
     B->tune = on != 0;
     B->iter.reset();
+    B->fcgIter.reset();
 }
 
 }  // namespace bmg

@@ -796,6 +796,9 @@ void bmg_ctx::enqueueFCGIteration(int mode)
     const OpView<double> o = op<double>(0);
     Pair<double> X{{kx[0], kx[1]}}, Rv{{kr[0], kr[1]}}, Pv{{kp[0], kp[1]}},
         Qv{{kq[0], kq[1]}};
+    // batched merges: the part before the cycle, the cycle's own positions,
+    // the part after it
+    record_tag(0);
     if (anyPeriodic())
         launch_bc_fill(q0.L, o, Pv, 0, 1, bcval, stream);
     const int nb = launch_apply_dots(q0.L, o, cst(Pv), Qv, cst(Rv), partials,
@@ -812,6 +815,7 @@ void bmg_ctx::enqueueFCGIteration(int mode)
         enqueueLearned(dScalar + 2, 0.0, pr(q0.x));
     else
         enqueuePrecondition(mode == 1);
+    record_tag(3000);
     if (mode == 1)
     {
         launch_dot_zq<float>(q0.L, cpr(q0.xf), cst(Qv), partials, stream);
@@ -964,28 +968,19 @@ void bmg_ctx::runFCG(bool constMatrix, bmg_stats* st)
     if (learned) prepareLearned();
     Level& q0 = lv[0];
     const long n = q0.L.ncol;
-    if (!kx[0])
-    {
-        for (int c = 0; c < 2; c++)
-        {
-            kx[c] = devAlloc<double>(n); kb[c] = devAlloc<double>(n);
-            kr[c] = devAlloc<double>(n); kp[c] = devAlloc<double>(n);
-            kq[c] = devAlloc<double>(n);
-        }
-        gen++;
-        for (double** f : {kr, kp, kq})
-            launch_fill<double>(Pair<double>{{f[0], f[1]}}, n, 0.0, stream);
-    }
+    prepareFCG();
     const OpView<double> o = op<double>(0);
     Pair<double> X{{kx[0], kx[1]}}, Bv{{kb[0], kb[1]}}, Rv{{kr[0], kr[1]}},
         Pv{{kp[0], kp[1]}};
-    launch_copy<double>(n, cpr(q0.x), X, stream);
-    launch_copy<double>(n, cpr(q0.b), Bv, stream);
     int nb = 0;
-    // initial residual without boundary update (MG.C:163)
-    launch_residual(q0.L, o, cst(X), cst(Bv), Rv, partials, &nb, stream);
-    double ss = reduceFinal(nb);
-    const double nf = normFactorOf(cst(X), cst(Rv), cst(Bv));
+    // initial residual without boundary update (MG.C:163) and normFactors
+    enqueueFCGStart();
+    HIPCK(hipStreamSynchronize(stream));
+    double ss = hScalar[0];
+    const double cells = double(q0.cells());
+    enqueueFCGNormTerms(hScalar[1]/cells);
+    HIPCK(hipStreamSynchronize(stream));
+    const double nf = std::max(std::sqrt(hScalar[2]), cells*1e-12);
     const double init = std::sqrt(ss)/nf;
     if (!std::isfinite(init))
         throw Error(BMG_ERR_NUMERIC, "initial residual is not finite");
@@ -1043,19 +1038,15 @@ void bmg_ctx::runFCG(bool constMatrix, bmg_stats* st)
             }
             else
             {
-                if (mixed)
-                    launch_to_float(n, cst(Rv), pr(q0.bf), stream);
-                else
+                if (useGraphs)
                 {
-                    launch_copy<double>(n, cst(Rv), pr(q0.b), stream);
-                    launch_copy<double>(n, cst(Rv), pr(q0.r), stream);
+                    // the cycle from its own captured graph
+                    stageRestart(mixed);
+                    precondition(mixed);
+                    finishRestart(mixed);
                 }
-                precondition(mixed);
-                if (mixed)
-                    launch_to_double(n, Pair<const float>{{q0.xf[0], q0.xf[1]}},
-                                     Pv, stream);
                 else
-                    launch_copy<double>(n, cpr(q0.x), Pv, stream);
+                    enqueueFCGRestart(mixed);
             }
             restart = false;
         }
@@ -1085,20 +1076,16 @@ void bmg_ctx::runFCG(bool constMatrix, bmg_stats* st)
         if (conv && iter >= opts.min_iter)
         {
             // confirm on the true residual (with the boundary update)
-            if (anyPeriodic())
-                launch_bc_fill(q0.L, o, X, 0, 1, bcval, stream);
-            launch_residual(q0.L, o, cst(X), cst(Bv), Rv, partials, &nb,
-                            stream);
-            ss = reduceFinal(nb);
+            enqueueFCGConfirm();
+            HIPCK(hipStreamSynchronize(stream));
+            ss = hScalar[0];
             cur = std::sqrt(ss)/nf;
             conv = check(cur);
             if (conv) break;
             restart = true;
         }
     }
-    launch_copy<double>(n, cst(X), pr(q0.x), stream);
-    launch_copy<double>(n, cst(Bv), pr(q0.b), stream);
-    launch_bc_fill(q0.L, op<double>(0), pr(q0.x), 2, 1, bcval, stream);
+    enqueueFCGEnd();
     HIPCK(hipGetLastError());
     if (st)
     {
@@ -1109,6 +1096,113 @@ void bmg_ctx::runFCG(bool constMatrix, bmg_stats* st)
         st->learned_fallback = fallbackAt;
         st->learned_guess_residual = guessResidual;
     }
+}
+
+// Flexible-CG buffers (FP64 x, b, r, p, q on level 0), allocated once.
+void bmg_ctx::prepareFCG()
+{
+    if (kx[0]) return;
+    const long n = lv[0].L.ncol;
+    for (int c = 0; c < 2; c++)
+    {
+        kx[c] = devAlloc<double>(n); kb[c] = devAlloc<double>(n);
+        kr[c] = devAlloc<double>(n); kp[c] = devAlloc<double>(n);
+        kq[c] = devAlloc<double>(n);
+    }
+    gen++;
+    for (double** f : {kr, kp, kq})
+        launch_fill<double>(Pair<double>{{f[0], f[1]}}, n, 0.0, stream);
+}
+
+// FCG start: copies of x and b, r = b - A x without boundary update
+// (MG.C:163) with its squared norm (host slot 0) and the sum of x (slot 1).
+void bmg_ctx::enqueueFCGStart()
+{
+    Level& q0 = lv[0];
+    const long n = q0.L.ncol;
+    Pair<double> X{{kx[0], kx[1]}}, Bv{{kb[0], kb[1]}}, Rv{{kr[0], kr[1]}};
+    launch_copy<double>(n, cpr(q0.x), X, stream);
+    launch_copy<double>(n, cpr(q0.b), Bv, stream);
+    int nb = 0;
+    launch_residual(q0.L, op<double>(0), cst(X), cst(Bv), Rv, partials, &nb,
+                    stream);
+    launch_reduce_final(partials, nb, dScalar, stream, hScalarDev);
+    launch_sum<double>(q0.L, cst(X), partials, &nb, stream);
+    launch_reduce_final(partials, nb, dScalar + 1, stream, hScalarDev + 1);
+}
+
+// normFactors terms of the FCG residual with y = gAverage(x), slot 2.
+void bmg_ctx::enqueueFCGNormTerms(double y)
+{
+    Level& q0 = lv[0];
+    Pair<double> Bv{{kb[0], kb[1]}}, Rv{{kr[0], kr[1]}};
+    int nb = 0;
+    launch_norm_factor_terms<double>(q0.L, op<double>(0), cst(Rv), cst(Bv), y,
+                                     partials, &nb, stream);
+    launch_reduce_final(partials, nb, dScalar + 2, stream, hScalarDev + 2);
+}
+
+// r staged as the cycle's right-hand side (FP32, or FP64 as both the
+// right-hand side and the residual the first repetition restricts).
+void bmg_ctx::stageRestart(bool mixed)
+{
+    Level& q0 = lv[0];
+    const long n = q0.L.ncol;
+    Pair<const double> Rv{{kr[0], kr[1]}};
+    if (mixed)
+        launch_to_float(n, Rv, pr(q0.bf), stream);
+    else
+    {
+        launch_copy<double>(n, Rv, pr(q0.b), stream);
+        launch_copy<double>(n, Rv, pr(q0.r), stream);
+    }
+}
+
+// p = z, the cycle's result.
+void bmg_ctx::finishRestart(bool mixed)
+{
+    Level& q0 = lv[0];
+    const long n = q0.L.ncol;
+    Pair<double> Pv{{kp[0], kp[1]}};
+    if (mixed)
+        launch_to_double(n, Pair<const float>{{q0.xf[0], q0.xf[1]}}, Pv,
+                         stream);
+    else
+        launch_copy<double>(n, cpr(q0.x), Pv, stream);
+}
+
+// FCG restart: p = z = M r for the current residual.
+void bmg_ctx::enqueueFCGRestart(bool mixed)
+{
+    record_tag(0);
+    stageRestart(mixed);
+    enqueuePrecondition(mixed);
+    record_tag(3000);
+    finishRestart(mixed);
+}
+
+// The true residual b - A x with the boundary update, slot 0.
+void bmg_ctx::enqueueFCGConfirm()
+{
+    Level& q0 = lv[0];
+    const OpView<double> o = op<double>(0);
+    Pair<double> X{{kx[0], kx[1]}}, Bv{{kb[0], kb[1]}}, Rv{{kr[0], kr[1]}};
+    if (anyPeriodic())
+        launch_bc_fill(q0.L, o, X, 0, 1, bcval, stream);
+    int nb = 0;
+    launch_residual(q0.L, o, cst(X), cst(Bv), Rv, partials, &nb, stream);
+    launch_reduce_final(partials, nb, dScalar, stream, hScalarDev);
+}
+
+// x and b back to the solver's level 0, eliminated ghosts (MG.C:475).
+void bmg_ctx::enqueueFCGEnd()
+{
+    Level& q0 = lv[0];
+    const long n = q0.L.ncol;
+    Pair<const double> X{{kx[0], kx[1]}}, Bv{{kb[0], kb[1]}};
+    launch_copy<double>(n, X, pr(q0.x), stream);
+    launch_copy<double>(n, Bv, pr(q0.b), stream);
+    launch_bc_fill(q0.L, op<double>(0), pr(q0.x), 2, 1, bcval, stream);
 }
 
 double bmg_ctx::normFactorOf(Pair<const double> x, Pair<const double> r,

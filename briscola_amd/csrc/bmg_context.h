@@ -348,6 +348,15 @@ struct bmg_ctx
     void enqueueSolveStart();
     void enqueueNormTerms(double y);
     void enqueueSolveEnd();
+    // runFCG in phases (no learned preconditioner), likewise recordable
+    void prepareFCG();
+    void enqueueFCGStart();
+    void enqueueFCGNormTerms(double y);
+    void enqueueFCGRestart(bool mixed);
+    void stageRestart(bool mixed);
+    void finishRestart(bool mixed);
+    void enqueueFCGConfirm();
+    void enqueueFCGEnd();
 };
 
 template <> OpView<double> bmg_ctx::op<double>(int l) const;

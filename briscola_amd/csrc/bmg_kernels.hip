@@ -2523,17 +2523,18 @@ struct KReduceFinal
 
 // q = A p and the per-tile partials of p.q (row 0) and p.r (row 1).
 template <int OP>
-__global__ __launch_bounds__(BX*BY) void k_apply_dots(LevelDims L,
-                                                      OpView<double> op,
-                                                      Pair<const double> p,
-                                                      Pair<double> q,
-                                                      Pair<const double> r,
-                                                      Tiles tl,
-                                                      double* partials)
+__device__ __forceinline__ void apply_dots_body(const Bid bid,
+                                                const LevelDims& L,
+                                                const OpView<double>& op,
+                                                Pair<const double> p,
+                                                Pair<double> q,
+                                                Pair<const double> r,
+                                                const Tiles& tl,
+                                                double* partials)
 {
     __shared__ double sm[BX*BY/64];
     int bx, by, bz;
-    if (!tile_of(tl, blockIdx.x, bx, by, bz)) return;
+    if (!tile_of(tl, bid.x, bx, by, bz)) return;
     PairPos c;
     double pq = 0.0, pr = 0.0;
     if (pair_pos(L, bx, by, bz, c))
@@ -2563,27 +2564,67 @@ __global__ __launch_bounds__(BX*BY) void k_apply_dots(LevelDims L,
     if (threadIdx.x == 0 && threadIdx.y == 0) partials[nb + blk] = s;
 }
 
+template <int OP>
+__global__ __launch_bounds__(BX*BY) void k_apply_dots(LevelDims L,
+                                                      OpView<double> op,
+                                                      Pair<const double> p,
+                                                      Pair<double> q,
+                                                      Pair<const double> r,
+                                                      Tiles tl,
+                                                      double* partials)
+{
+    const Bid bid{blockIdx.x, blockIdx.y, blockIdx.z};
+    apply_dots_body<OP>(bid, L, op, p, q, r, tl, partials);
+}
+
+template <int OP>
+struct KApplyDots
+{
+    static constexpr int bx = BX, by = BY;
+    static constexpr bool lds = true;
+    struct Args
+    {
+        LevelDims L;
+        OpView<double> op;
+        Pair<const double> p;
+        Pair<double> q;
+        Pair<const double> r;
+        Tiles tl;
+        double* partials;
+    };
+    __device__ static void run(const Bid bid, const Args& a)
+    {
+        apply_dots_body<OP>(bid, a.L, a.op, a.p, a.q, a.r, a.tl, a.partials);
+    }
+    static void launch(dim3 g, size_t shm, hipStream_t s, const Args& a)
+    {
+        hipLaunchKernelGGL((k_apply_dots<OP>), g, dim3(bx, by), shm, s, a.L,
+                           a.op, a.p, a.q, a.r, a.tl, a.partials);
+    }
+};
+
 // x += alpha p, r -= alpha q with alpha = sc[1]/sc[0], per-tile partials of
 // r.r, and r handed to the preconditioner: STAGE 1 as FP32 into rf (mixed
 // cycle), STAGE 0 as FP64 into both r0 and b0 (the level-0 right-hand side
 // and restricted residual of an FP64 cycle), STAGE 2 nowhere (the learned
 // preconditioner reads r itself).
 template <int STAGE>
-__global__ __launch_bounds__(BX*BY) void k_fcg_update(LevelDims L,
-                                                      const double* sc,
-                                                      Pair<double> x,
-                                                      Pair<double> r,
-                                                      Pair<const double> p,
-                                                      Pair<const double> q,
-                                                      Pair<float> rf,
-                                                      Pair<double> r0,
-                                                      Pair<double> b0,
-                                                      Tiles tl,
-                                                      double* partials)
+__device__ __forceinline__ void fcg_update_body(const Bid bid,
+                                                const LevelDims& L,
+                                                const double* sc,
+                                                Pair<double> x,
+                                                Pair<double> r,
+                                                Pair<const double> p,
+                                                Pair<const double> q,
+                                                Pair<float> rf,
+                                                Pair<double> r0,
+                                                Pair<double> b0,
+                                                const Tiles& tl,
+                                                double* partials)
 {
     __shared__ double sm[BX*BY/64];
     int bx, by, bz;
-    if (!tile_of(tl, blockIdx.x, bx, by, bz)) return;
+    if (!tile_of(tl, bid.x, bx, by, bz)) return;
     const double alpha = sc[1]/sc[0];
     PairPos c;
     double acc = 0.0;
@@ -2622,15 +2663,65 @@ __global__ __launch_bounds__(BX*BY) void k_fcg_update(LevelDims L,
         partials[tile_index(tl, bx, by, bz)] = s;
 }
 
+template <int STAGE>
+__global__ __launch_bounds__(BX*BY) void k_fcg_update(LevelDims L,
+                                                      const double* sc,
+                                                      Pair<double> x,
+                                                      Pair<double> r,
+                                                      Pair<const double> p,
+                                                      Pair<const double> q,
+                                                      Pair<float> rf,
+                                                      Pair<double> r0,
+                                                      Pair<double> b0,
+                                                      Tiles tl,
+                                                      double* partials)
+{
+    const Bid bid{blockIdx.x, blockIdx.y, blockIdx.z};
+    fcg_update_body<STAGE>(bid, L, sc, x, r, p, q, rf, r0, b0, tl, partials);
+}
+
+template <int STAGE>
+struct KFcgUpdate
+{
+    static constexpr int bx = BX, by = BY;
+    static constexpr bool lds = true;
+    struct Args
+    {
+        LevelDims L;
+        const double* sc;
+        Pair<double> x;
+        Pair<double> r;
+        Pair<const double> p;
+        Pair<const double> q;
+        Pair<float> rf;
+        Pair<double> r0;
+        Pair<double> b0;
+        Tiles tl;
+        double* partials;
+    };
+    __device__ static void run(const Bid bid, const Args& a)
+    {
+        fcg_update_body<STAGE>(bid, a.L, a.sc, a.x, a.r, a.p, a.q, a.rf, a.r0,
+                               a.b0, a.tl, a.partials);
+    }
+    static void launch(dim3 g, size_t shm, hipStream_t s, const Args& a)
+    {
+        hipLaunchKernelGGL((k_fcg_update<STAGE>), g, dim3(bx, by), shm, s, a.L,
+                           a.sc, a.x, a.r, a.p, a.q, a.rf, a.r0, a.b0, a.tl,
+                           a.partials);
+    }
+};
+
 // per-tile partials of z.q, z in FP64 or FP32
 template <typename Z>
-__global__ __launch_bounds__(BX*BY) void k_dot_zq(LevelDims L, Pair<const Z> z,
-                                                  Pair<const double> q,
-                                                  Tiles tl, double* partials)
+__device__ __forceinline__ void dot_zq_body(const Bid bid, const LevelDims& L,
+                                            Pair<const Z> z,
+                                            Pair<const double> q,
+                                            const Tiles& tl, double* partials)
 {
     __shared__ double sm[BX*BY/64];
     int bx, by, bz;
-    if (!tile_of(tl, blockIdx.x, bx, by, bz)) return;
+    if (!tile_of(tl, bid.x, bx, by, bz)) return;
     PairPos c;
     double acc = 0.0;
     if (pair_pos(L, bx, by, bz, c))
@@ -2648,15 +2739,49 @@ __global__ __launch_bounds__(BX*BY) void k_dot_zq(LevelDims L, Pair<const Z> z,
         partials[tile_index(tl, bx, by, bz)] = s;
 }
 
+template <typename Z>
+__global__ __launch_bounds__(BX*BY) void k_dot_zq(LevelDims L, Pair<const Z> z,
+                                                  Pair<const double> q,
+                                                  Tiles tl, double* partials)
+{
+    const Bid bid{blockIdx.x, blockIdx.y, blockIdx.z};
+    dot_zq_body<Z>(bid, L, z, q, tl, partials);
+}
+
+template <typename Z>
+struct KDotZq
+{
+    static constexpr int bx = BX, by = BY;
+    static constexpr bool lds = true;
+    struct Args
+    {
+        LevelDims L;
+        Pair<const Z> z;
+        Pair<const double> q;
+        Tiles tl;
+        double* partials;
+    };
+    __device__ static void run(const Bid bid, const Args& a)
+    {
+        dot_zq_body<Z>(bid, a.L, a.z, a.q, a.tl, a.partials);
+    }
+    static void launch(dim3 g, size_t shm, hipStream_t s, const Args& a)
+    {
+        hipLaunchKernelGGL((k_dot_zq<Z>), g, dim3(bx, by), shm, s, a.L, a.z,
+                           a.q, a.tl, a.partials);
+    }
+};
+
 // p = z + beta p on interior cells, beta = -sc[3]/sc[0]
 template <typename Z>
-__global__ __launch_bounds__(BX*BY) void k_update_p(LevelDims L,
-                                                    const double* sc,
-                                                    Pair<const Z> z,
-                                                    Pair<double> p, Tiles tl)
+__device__ __forceinline__ void update_p_body(const Bid bid,
+                                              const LevelDims& L,
+                                              const double* sc,
+                                              Pair<const Z> z,
+                                              Pair<double> p, const Tiles& tl)
 {
     int bx, by, bz;
-    if (!tile_of(tl, blockIdx.x, bx, by, bz)) return;
+    if (!tile_of(tl, bid.x, bx, by, bz)) return;
     const double beta = -sc[3]/sc[0];
     PairPos c;
     if (!pair_pos(L, bx, by, bz, c)) return;
@@ -2668,6 +2793,40 @@ __global__ __launch_bounds__(BX*BY) void k_update_p(LevelDims L,
         st(col(p, k), i, double(ld(col(z, k), i)) + beta*ld(col(p, k), i));
     }
 }
+
+template <typename Z>
+__global__ __launch_bounds__(BX*BY) void k_update_p(LevelDims L,
+                                                    const double* sc,
+                                                    Pair<const Z> z,
+                                                    Pair<double> p, Tiles tl)
+{
+    const Bid bid{blockIdx.x, blockIdx.y, blockIdx.z};
+    update_p_body<Z>(bid, L, sc, z, p, tl);
+}
+
+template <typename Z>
+struct KUpdateP
+{
+    static constexpr int bx = BX, by = BY;
+    static constexpr bool lds = false;
+    struct Args
+    {
+        LevelDims L;
+        const double* sc;
+        Pair<const Z> z;
+        Pair<double> p;
+        Tiles tl;
+    };
+    __device__ static void run(const Bid bid, const Args& a)
+    {
+        update_p_body<Z>(bid, a.L, a.sc, a.z, a.p, a.tl);
+    }
+    static void launch(dim3 g, size_t shm, hipStream_t s, const Args& a)
+    {
+        hipLaunchKernelGGL((k_update_p<Z>), g, dim3(bx, by), shm, s, a.L, a.sc,
+                           a.z, a.p, a.tl);
+    }
+};
 
 // Interior of a level-0 FP64 colour-split field to a dense FP32 array in the
 // reference cell order (m = ((I-1)*N1 + J-1)*N2 + K-1), scaled by
@@ -2743,12 +2902,13 @@ __global__ __launch_bounds__(BX*BY) void k_add_interior(LevelDims L,
 
 // Sums of nrows rows of n partials each, one workgroup per row, into
 // out[row] (fixed order, as k_reduce_final).
-__global__ __launch_bounds__(1024) void k_reduce_rows(const double* partials,
-                                                      int n, double* out,
-                                                      double* hostOut)
+__device__ __forceinline__ void reduce_rows_body(const Bid bid,
+                                                 const double* partials,
+                                                 int n, double* out,
+                                                 double* hostOut)
 {
     __shared__ double sm[16];
-    const double* row = partials + size_t(blockIdx.x)*n;
+    const double* row = partials + size_t(bid.x)*n;
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
     int i = threadIdx.x;
     for (; i + 3*1024 < n; i += 4*1024)
@@ -2762,24 +2922,115 @@ __global__ __launch_bounds__(1024) void k_reduce_rows(const double* partials,
     const double s = block_sum((a0 + a1) + (a2 + a3), sm);
     if (threadIdx.x == 0)
     {
-        out[blockIdx.x] = s;
-        if (hostOut) hostOut[blockIdx.x] = s;
+        out[bid.x] = s;
+        if (hostOut) hostOut[bid.x] = s;
     }
 }
 
+__global__ __launch_bounds__(1024) void k_reduce_rows(const double* partials,
+                                                      int n, double* out,
+                                                      double* hostOut)
+{
+    const Bid bid{blockIdx.x, blockIdx.y, blockIdx.z};
+    reduce_rows_body(bid, partials, n, out, hostOut);
+}
+
+struct KReduceRows
+{
+    static constexpr int bx = 1024, by = 1;
+    static constexpr bool lds = true;
+    struct Args
+    {
+        const double* partials;
+        int n;
+        double* out;
+        double* hostOut;
+    };
+    __device__ static void run(const Bid bid, const Args& a)
+    {
+        reduce_rows_body(bid, a.partials, a.n, a.out, a.hostOut);
+    }
+    static void launch(dim3 g, size_t shm, hipStream_t s, const Args& a)
+    {
+        hipLaunchKernelGGL((k_reduce_rows), g, dim3(bx, by), shm, s,
+                           a.partials, a.n, a.out, a.hostOut);
+    }
+};
+
 
 template <typename T>
-__global__ void k_copy(long n, Pair<const T> a, Pair<T> b)
+__device__ __forceinline__ void copy_body(const Bid bid, long n,
+                                          Pair<const T> a, Pair<T> b)
 {
-    const long t = (long)blockIdx.x*blockDim.x + threadIdx.x;
+    const long t = (long)bid.x*256 + threadIdx.x;
     if (t < n) { b.c[0][t] = a.c[0][t]; b.c[1][t] = a.c[1][t]; }
 }
 
-__global__ void k_to_double(long n, Pair<const float> a, Pair<double> b)
+template <typename T>
+__global__ __launch_bounds__(256) void k_copy(long n, Pair<const T> a,
+                                              Pair<T> b)
 {
-    const long t = (long)blockIdx.x*blockDim.x + threadIdx.x;
+    const Bid bid{blockIdx.x, blockIdx.y, blockIdx.z};
+    copy_body<T>(bid, n, a, b);
+}
+
+template <typename T>
+struct KCopy
+{
+    static constexpr int bx = 256, by = 1;
+    static constexpr bool lds = false;
+    struct Args
+    {
+        long n;
+        Pair<const T> a;
+        Pair<T> b;
+    };
+    __device__ static void run(const Bid bid, const Args& a)
+    {
+        copy_body<T>(bid, a.n, a.a, a.b);
+    }
+    static void launch(dim3 g, size_t shm, hipStream_t s, const Args& a)
+    {
+        hipLaunchKernelGGL((k_copy<T>), g, dim3(bx, by), shm, s, a.n, a.a,
+                           a.b);
+    }
+};
+
+__device__ __forceinline__ void to_double_body(const Bid bid, long n,
+                                               Pair<const float> a,
+                                               Pair<double> b)
+{
+    const long t = (long)bid.x*256 + threadIdx.x;
     if (t < n) { b.c[0][t] = double(a.c[0][t]); b.c[1][t] = double(a.c[1][t]); }
 }
+
+__global__ __launch_bounds__(256) void k_to_double(long n, Pair<const float> a,
+                                                   Pair<double> b)
+{
+    const Bid bid{blockIdx.x, blockIdx.y, blockIdx.z};
+    to_double_body(bid, n, a, b);
+}
+
+struct KToDouble
+{
+    static constexpr int bx = 256, by = 1;
+    static constexpr bool lds = false;
+    struct Args
+    {
+        long n;
+        Pair<const float> a;
+        Pair<double> b;
+    };
+    __device__ static void run(const Bid bid, const Args& a)
+    {
+        to_double_body(bid, a.n, a.a, a.b);
+    }
+    static void launch(dim3 g, size_t shm, hipStream_t s, const Args& a)
+    {
+        hipLaunchKernelGGL((k_to_double), g, dim3(bx, by), shm, s, a.n, a.a,
+                           a.b);
+    }
+};
 
 // diagonalSmoother::Sweep (diagonalSmoother.C:13-69): x = b / A_C.
 __global__ __launch_bounds__(BX*BY) void k_diag(LevelDims L, Pair<const double> a7,
@@ -3111,8 +3362,29 @@ struct KBcSourceAny
     }
 };
 
+struct KApplyDotsAny
+{
+    static constexpr int bx = BX, by = BY;
+    static constexpr bool lds = true;
+    using Args = KApplyDots<OP_CONST>::Args;
+    __device__ static void run(const Bid bid, const Args& a)
+    {
+        if (a.op.kind == OP_STENCIL)
+            apply_dots_body<OP_STENCIL>(bid, a.L, a.op, a.p, a.q, a.r, a.tl,
+                                        a.partials);
+        else if (a.op.kind == OP_CONST)
+            apply_dots_body<OP_CONST>(bid, a.L, a.op, a.p, a.q, a.r, a.tl,
+                                      a.partials);
+        else
+            apply_dots_body<OP_FACES>(bid, a.L, a.op, a.p, a.q, a.r, a.tl,
+                                      a.partials);
+    }
+};
+
 template <class K>
 struct BatchKind { using type = K; };
+template <int OP>
+struct BatchKind<KApplyDots<OP>> { using type = KApplyDotsAny; };
 template <typename T, int OP>
 struct BatchKind<KRbgs<T, OP>> { using type = KRbgsAny<T>; };
 template <typename T, int OP, bool NORM>
@@ -3720,18 +3992,14 @@ int launch_apply_dots(const LevelDims& L, const OpView<double>& op,
                       Pair<const double> p, Pair<double> q,
                       Pair<const double> r, double* partials, hipStream_t s)
 {
-    direct_only("flexible CG");
     const Tiles tl = pair_tiles(L);
     const dim3 g = tiles_grid(tl);
     if (op.kind == OP_STENCIL)
-        hipLaunchKernelGGL((k_apply_dots<OP_STENCIL>), g, BLK, 0, s, L, op, p,
-                           q, r, tl, partials);
+        dispatch<KApplyDots<OP_STENCIL>>(g, 0, s, {L, op, p, q, r, tl, partials});
     else if (op.kind == OP_CONST)
-        hipLaunchKernelGGL((k_apply_dots<OP_CONST>), g, BLK, 0, s, L, op, p, q,
-                           r, tl, partials);
+        dispatch<KApplyDots<OP_CONST>>(g, 0, s, {L, op, p, q, r, tl, partials});
     else
-        hipLaunchKernelGGL((k_apply_dots<OP_FACES>), g, BLK, 0, s, L, op, p, q,
-                           r, tl, partials);
+        dispatch<KApplyDots<OP_FACES>>(g, 0, s, {L, op, p, q, r, tl, partials});
     return tiles_count(tl);
 }
 
@@ -3740,17 +4008,17 @@ int launch_fcg_update(const LevelDims& L, const double* sc, Pair<double> x,
                       Pair<const double> q, Pair<float> rf, Pair<double> r0,
                       Pair<double> b0, double* partials, hipStream_t s)
 {
-    direct_only("flexible CG");
     const Tiles tl = pair_tiles(L);
+    const dim3 g = tiles_grid(tl);
     if (rf.c[0])
-        hipLaunchKernelGGL((k_fcg_update<1>), tiles_grid(tl), BLK, 0, s, L, sc,
-                           x, r, p, q, rf, r0, b0, tl, partials);
+        dispatch<KFcgUpdate<1>>(g, 0, s,
+                                {L, sc, x, r, p, q, rf, r0, b0, tl, partials});
     else if (r0.c[0])
-        hipLaunchKernelGGL((k_fcg_update<0>), tiles_grid(tl), BLK, 0, s, L, sc,
-                           x, r, p, q, rf, r0, b0, tl, partials);
+        dispatch<KFcgUpdate<0>>(g, 0, s,
+                                {L, sc, x, r, p, q, rf, r0, b0, tl, partials});
     else
-        hipLaunchKernelGGL((k_fcg_update<2>), tiles_grid(tl), BLK, 0, s, L, sc,
-                           x, r, p, q, rf, r0, b0, tl, partials);
+        dispatch<KFcgUpdate<2>>(g, 0, s,
+                                {L, sc, x, r, p, q, rf, r0, b0, tl, partials});
     return tiles_count(tl);
 }
 
@@ -3786,10 +4054,8 @@ template <typename Z>
 int launch_dot_zq(const LevelDims& L, Pair<const Z> z, Pair<const double> q,
                   double* partials, hipStream_t s)
 {
-    direct_only("flexible CG");
     const Tiles tl = pair_tiles(L);
-    hipLaunchKernelGGL((k_dot_zq<Z>), tiles_grid(tl), BLK, 0, s, L, z, q, tl,
-                       partials);
+    dispatch<KDotZq<Z>>(tiles_grid(tl), 0, s, {L, z, q, tl, partials});
     return tiles_count(tl);
 }
 template int launch_dot_zq<double>(const LevelDims&, Pair<const double>,
@@ -3801,10 +4067,8 @@ template <typename Z>
 void launch_update_p(const LevelDims& L, const double* sc, Pair<const Z> z,
                      Pair<double> p, hipStream_t s)
 {
-    direct_only("flexible CG");
     const Tiles tl = pair_tiles(L);
-    hipLaunchKernelGGL((k_update_p<Z>), tiles_grid(tl), BLK, 0, s, L, sc, z, p,
-                       tl);
+    dispatch<KUpdateP<Z>>(tiles_grid(tl), 0, s, {L, sc, z, p, tl});
 }
 template void launch_update_p<double>(const LevelDims&, const double*,
                                       Pair<const double>, Pair<double>,
@@ -3816,18 +4080,14 @@ template void launch_update_p<float>(const LevelDims&, const double*,
 void launch_reduce_rows(const double* partials, int n, int nrows, double* out,
                         hipStream_t s, double* hostOut)
 {
-    direct_only("flexible CG");
-    hipLaunchKernelGGL(k_reduce_rows, dim3(nrows), dim3(1024), 0, s, partials,
-                       n, out, hostOut);
+    dispatch<KReduceRows>(dim3(nrows), 0, s, {partials, n, out, hostOut});
 }
 
 
 template <typename T>
 void launch_copy(long n, Pair<const T> a, Pair<T> b, hipStream_t s)
 {
-    direct_only("field copy");
-    hipLaunchKernelGGL((k_copy<T>), dim3((n + 255)/256), dim3(256), 0, s, n,
-                       a, b);
+    dispatch<KCopy<T>>(dim3(unsigned((n + 255)/256)), 0, s, {n, a, b});
 }
 template void launch_copy<double>(long, Pair<const double>, Pair<double>,
                                   hipStream_t);
@@ -3835,9 +4095,7 @@ template void launch_copy<double>(long, Pair<const double>, Pair<double>,
 void launch_to_double(long n, Pair<const float> a, Pair<double> b,
                       hipStream_t s)
 {
-    direct_only("field conversion");
-    hipLaunchKernelGGL(k_to_double, dim3((n + 255)/256), dim3(256), 0, s, n,
-                       a, b);
+    dispatch<KToDouble>(dim3(unsigned((n + 255)/256)), 0, s, {n, a, b});
 }
 
 void launch_add_correction(const LevelDims& L, Pair<double> x,

@@ -6,7 +6,9 @@ the iteration count and residuals of the batched solve are compared bitwise
 with bmg_solve_device of the same system alone (whose FP64 path is itself
 bitwise the oracle's, tests/test_gpu_parity.py).  Also: the launch grids the
 autotuner picks do not change any result, a second solve replays the
-captured iteration, and a system with a zero source drops out at once."""
+captured iteration, and a system with a zero source drops out at once.
+Batched flexible CG is held to the same bar against each system's own FCG
+solve."""
 import numpy as np
 import pytest
 
@@ -46,12 +48,12 @@ def problem(i, shape, faces):
     return p
 
 
-def context(p, precision):
+def context(p, precision, krylov=bmg.KRYLOV_NONE):
     ctx = bmg.Context(p.vertices, p.bc, p.bc_value, rel_tol=1e-8,
                       tolerance=0.0, max_iter=200, n_sweeps_pre=2,
                       n_sweeps_post=2, cycle=bmg.VCYCLE,
                       coarse_mode=bmg.COARSE_DIRECT, coarse_level=0,
-                      precision=precision)
+                      precision=precision, krylov=krylov)
     ctx.set_laplacian(p.lam)
     return ctx
 
@@ -60,11 +62,11 @@ def dev(a):
     return torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")
 
 
-def solve_alone(systems):
+def solve_alone(systems, krylov=bmg.KRYLOV_NONE):
     out = []
     for i, shape, faces, prec in systems:
         p = problem(i, shape, faces)
-        ctx = context(p, prec)
+        ctx = context(p, prec, krylov)
         b, x = dev(p.b), dev(np.zeros_like(p.b))
         st = ctx.solve_device(b.data_ptr(), x.data_ptr())
         out.append((x.cpu().numpy(), b.cpu().numpy(), st.iters,
@@ -73,9 +75,9 @@ def solve_alone(systems):
     return out
 
 
-def solve_batched(systems, autotune=True, repeat=1):
+def solve_batched(systems, autotune=True, repeat=1, krylov=bmg.KRYLOV_NONE):
     probs = [problem(i, shape, faces) for i, shape, faces, _ in systems]
-    ctxs = [context(p, s[3]) for p, s in zip(probs, systems)]
+    ctxs = [context(p, s[3], krylov) for p, s in zip(probs, systems)]
     batch = bmg.Batch(ctxs, autotune=autotune)
     runs = []
     for r in range(repeat):
@@ -141,14 +143,47 @@ def test_zero_source_system_drops_out(gpu):
         c.close()
 
 
-def test_batch_rejects_flexible_cg(gpu):
+# systems for flexible CG: the uneven shapes of SYSTEMS plus anisotropic
+# cells (the slow case of plain MG), FP64 and mixed cycles
+FCG_SYSTEMS = SYSTEMS[:6] + [
+    (10, (16, 8, 32), (N, N, D, D, P, P), bmg.MIXED),
+    (11, (32, 16, 64), (D, D, N, N, N, N), bmg.FP64),
+]
+
+
+def test_batch_fcg_equals_individual_solves(gpu):
+    """Batched flexible CG (restart p = M r, captured iterations, the
+    confirmation on the true residual, all merged over the systems that
+    need each) gives every system exactly its own FCG solve."""
+    alone = solve_alone(FCG_SYSTEMS, bmg.KRYLOV_FCG)
+    runs, (launches, wg) = solve_batched(FCG_SYSTEMS, repeat=2,
+                                         krylov=bmg.KRYLOV_FCG)
+    assert launches > 0 and wg > 0
+    for run in runs:
+        for k, (got, ref) in enumerate(zip(run, alone)):
+            assert got[2] == ref[2], (k, got[2], ref[2])
+            assert got[3] == ref[3] and got[4] == ref[4], k
+            assert same(got[0], ref[0]), f"system {k}: x differs"
+            assert same(got[1], ref[1]), f"system {k}: b differs"
+            assert ref[4] / ref[3] <= 1e-8
+
+
+def test_batch_rejects_learned_and_mixed_outer_iterations(gpu):
     p = problem(0, (8, 8, 8), None)
-    ctx = bmg.Context(p.vertices, p.bc, p.bc_value, rel_tol=1e-8,
-                      tolerance=0.0, krylov=bmg.KRYLOV_FCG)
-    ctx.set_laplacian(None)
-    batch = bmg.Batch([ctx])
-    b, x = dev(p.b), dev(np.zeros_like(p.b))
-    with pytest.raises(bmg.BmgError):
-        batch.solve_device([b.data_ptr()], [x.data_ptr()])
-    batch.close()
-    ctx.close()
+    for kinds in ((bmg.KRYLOV_FCG_LEARNED,),
+                  (bmg.KRYLOV_NONE, bmg.KRYLOV_FCG)):
+        ctxs = []
+        for k in kinds:
+            ctx = bmg.Context(p.vertices, p.bc, p.bc_value, rel_tol=1e-8,
+                              tolerance=0.0, krylov=k)
+            ctx.set_laplacian(None)
+            ctxs.append(ctx)
+        batch = bmg.Batch(ctxs)
+        bs = [dev(p.b) for _ in ctxs]
+        xs = [dev(np.zeros_like(p.b)) for _ in ctxs]
+        with pytest.raises(bmg.BmgError):
+            batch.solve_device([b.data_ptr() for b in bs],
+                               [x.data_ptr() for x in xs])
+        batch.close()
+        for c in ctxs:
+            c.close()
