@@ -350,6 +350,7 @@ struct bmg_batch
     std::unique_ptr<BatchPlan> fcgIter, fcgRestart, fcgConfirm;
     std::vector<long> fcgGen;
     std::vector<int> fcgMixed;
+    bool lastFcg = false;   // the last solve ran flexible CG
     bool tune = true;
     int cus = 256;
 
@@ -603,7 +604,8 @@ void batch_solve(bmg_batch* B, double* const* db, double* const* dx,
         }
         std::vector<double> init(n), cur(n);
         std::vector<int> iter(n, 0);
-        if (B->ctx[0]->opts.krylov == BMG_KRYLOV_FCG)
+        B->lastFcg = B->ctx[0]->opts.krylov == BMG_KRYLOV_FCG;
+        if (B->lastFcg)
             batchFCG(B, db, dx, constMatrix, hostPrepared, iter, init, cur);
         else
         {
@@ -729,7 +731,7 @@ void batch_solve(bmg_batch* B, double* const* db, double* const* dx,
 void batch_info(const bmg_batch* B, int* launches, long* workgroups)
 {
     if (!B) throw Error(BMG_ERR_ARG, "null batch");
-    const BatchPlan* p = B->iter ? B->iter.get() : B->fcgIter.get();
+    const BatchPlan* p = B->lastFcg ? B->fcgIter.get() : B->iter.get();
     if (launches) *launches = p ? int(p->launches.size()) : 0;
     if (workgroups) *workgroups = p ? long(p->workgroups()) : 0;
 }

@@ -163,10 +163,12 @@ int bmg_solve_device(bmg_ctx* ctx, double* d_b, double* d_x,
 
 /* ---- Batched solves of independent systems.  Each system is a context
  * created and given its operator as for bmg_solve_device; the batch runs
- * MG::solve (MG.C:402-483; krylov none, rbgs smoother) of all of them
- * together: every kernel of an iteration is one persistent launch over all
- * systems, the merged iteration is replayed as a HIP graph, and converged
- * systems drop out of it.  Each system's result is bitwise the one bmg_solve_device
+ * MG::solve (MG.C:402-483; rbgs smoother) of all of them together, with MG
+ * iterations (krylov none) or flexible CG (BMG_KRYLOV_FCG, the same for
+ * every system; the learned preconditioner has no batched form): every
+ * kernel of an iteration is one persistent launch over all systems, the
+ * merged iteration is replayed as a HIP graph, and converged systems drop
+ * out of it.  Each system's result is bitwise the one bmg_solve_device
  * returns for it alone.  The contexts stay owned by the caller, must be on
  * one device and must not be used elsewhere while the batch solves; they
  * may differ in shape, boundary conditions, precision and cycle options.
@@ -179,8 +181,8 @@ int bmg_batch_destroy(bmg_batch* batch);
  * stats: n entries (solve_ms = the whole batch). */
 int bmg_solve_batch(bmg_batch* batch, double* const* d_b, double* const* d_x,
                     int const_matrix, bmg_stats* stats);
-/* Launches and virtual workgroups of one merged iteration (0 before the
- * first solve). */
+/* Launches and virtual workgroups of one merged iteration of the last
+ * solve's outer iteration (0 before the first solve). */
 int bmg_batch_info(const bmg_batch* batch, int* launches,
                    long* workgroups);
 /* Launch-grid autotuning of the merged iteration (default on: every
