@@ -9,7 +9,10 @@
 //      (s the tridiagonal axis, a and c the transformed axes); a Neumann or
 //      Dirichlet transformed axis is extended to 2N by even or odd
 //      reflection, so that one complex FFT of the extension gives the
-//      DCT-II / DST-II (FFTW REDFT10 / RODFT10) after a twiddle;
+//      DCT-II / DST-II (FFTW REDFT10 / RODFT10) after a twiddle, and a
+//      Dirichlet-Neumann or Neumann-Dirichlet axis to 4N by the quarter-wave
+//      reflections, whose FFT gives the DST-IV / DCT-IV (RODFT11 / REDFT11)
+//      at the odd positions;
 //   2. one batched 2-D complex FFT over (a, c) (rocFFT);
 //   3. per wave-number pair, the twiddle and the tridiagonal solve along s
 //      with the reference's coefficients, boundary rows, pinned zero mode
@@ -20,9 +23,7 @@
 //   5. the real part scaled by 1/(L_a L_c) (the reference's N or 2N per
 //      transformed axis) into x, then every ghost of x refreshed
 //      (x.correctBoundaryConditions(), FFTPoissonSolver.C:437-440).
-// Transformed axes may be periodic, Neumann-Neumann or Dirichlet-Dirichlet;
-// a mixed Dirichlet/Neumann pair on a transformed axis (the reference's
-// DCT-IV/DST-IV) is refused.  The tridiagonal axis takes any pair.
+// Every boundary pair is taken on every axis.
 #include "bmg_context.h"
 
 #include <rocfft/rocfft.h>
@@ -48,7 +49,7 @@ struct FftGeom {
     int N[3], P[3];
     int s, d1, d2;          // tridiagonal axis, transformed axes
     int n1, n2, ns;         // sizes along d1, d2, s
-    int L1, L2;             // transform lengths (N or 2N)
+    int L1, L2;             // transform lengths (N, 2N or 4N)
     int code1, code2, codes;
     int bc[6];
     double bcval[6];
@@ -79,6 +80,33 @@ __device__ __forceinline__ long refIndex(const FftGeom& g, int i, int j,
     return ((long)(i + 1)*g.P[1] + (j + 1))*g.P[2] + (k + 1);
 }
 
+// The positions of value a of a transformed axis in its extension and their
+// signs: periodic a; Neumann-Neumann (even about -1/2 and N-1/2, period 2N)
+// a, 2N-1-a; Dirichlet-Dirichlet (odd about both) a, -(2N-1-a);
+// Neumann-Dirichlet (DCT-IV: even about -1/2, odd about N-1/2, period 4N)
+// a, -(2N-1-a), -(2N+a), 4N-1-a; Dirichlet-Neumann (DST-IV: odd about -1/2,
+// even about N-1/2) a, 2N-1-a, -(2N+a), -(4N-1-a).
+__device__ __forceinline__ int extension(int code, int n, int a, int p[4],
+                                         double w[4])
+{
+    p[0] = a;
+    w[0] = 1.0;
+    if (code == C_PER) return 1;
+    p[1] = 2*n - 1 - a;
+    if (code == C_NN || code == C_DD)
+    {
+        w[1] = code == C_NN ? 1.0 : -1.0;
+        return 2;
+    }
+    p[2] = 2*n + a;
+    p[3] = 4*n - 1 - a;
+    const bool dct = code == C_ND;
+    w[1] = dct ? -1.0 : 1.0;
+    w[2] = -1.0;
+    w[3] = dct ? 1.0 : -1.0;
+    return 4;
+}
+
 // Step 1: interior cell t of the reference layout.
 __global__ void k_fft_rhs(FftGeom g, const double* b, const double* x,
                           C2* E)
@@ -105,23 +133,22 @@ __global__ void k_fft_rhs(FftGeom g, const double* b, const double* x,
         f -= term;
     }
     const int s = q[g.s], a = q[g.d1], cc = q[g.d2];
-    const int e1 = g.code1 == C_PER ? 0 : (g.code1 == C_NN ? 1 : -1);
-    const int e2 = g.code2 == C_PER ? 0 : (g.code2 == C_NN ? 1 : -1);
+    int p1[4], p2[4];
+    double w1[4], w2[4];
+    const int c1 = extension(g.code1, g.n1, a, p1, w1);
+    const int c2 = extension(g.code2, g.n2, cc, p2, w2);
     const long base = (long)s*g.L1*g.L2;
-    E[base + (long)a*g.L2 + cc] = C2{f, 0.0};
-    if (e1)
-        E[base + (long)(2*g.n1 - 1 - a)*g.L2 + cc] = C2{e1*f, 0.0};
-    if (e2)
-        E[base + (long)a*g.L2 + (2*g.n2 - 1 - cc)] = C2{e2*f, 0.0};
-    if (e1 && e2)
-        E[base + (long)(2*g.n1 - 1 - a)*g.L2 + (2*g.n2 - 1 - cc)] =
-            C2{e1*e2*f, 0.0};
+    for (int u = 0; u < c1; u++)
+        for (int v = 0; v < c2; v++)
+            E[base + (long)p1[u]*g.L2 + p2[v]] = C2{w1[u]*w2[v]*f, 0.0};
 }
 
 // Mode k of a transformed axis: its position m in the spectrum of the
 // (extended) FFT and the factor turning that FFT into the transform:
 // periodic X_k = Y_k; DCT-II X_k = e^{-i pi k/2N} Y_k; DST-II X_k =
-// i e^{-i pi (k+1)/2N} Y_{k+1}.
+// i e^{-i pi (k+1)/2N} Y_{k+1}; DCT-IV X_k = 1/2 e^{-i phi} Y_{2k+1} and
+// DST-IV X_k = i/2 e^{-i phi} Y_{2k+1} with phi = pi (2k+1)/4N (the sum of
+// the four reflected copies is twice the transform).
 __device__ __forceinline__ void fwdMode(int code, int n, int k, int& m,
                                         C2& w)
 {
@@ -131,6 +158,13 @@ __device__ __forceinline__ void fwdMode(int code, int n, int k, int& m,
     {
         m = k;
         w = cexpi(-pi*k/(2.0*n));
+        return;
+    }
+    if (code == C_ND || code == C_DN)
+    {
+        m = 2*k + 1;
+        const C2 e = cexpi(-pi*m/(4.0*n));
+        w = code == C_ND ? C2{0.5*e.x, 0.5*e.y} : C2{-0.5*e.y, 0.5*e.x};
         return;
     }
     m = k + 1;
@@ -266,10 +300,27 @@ __global__ void k_fft_tridiag(FftGeom g, const C2* E, C2* X, double* Dh,
 // Spectrum position m of a transformed axis from the modes: the mode index
 // (or -1 for a zero) and the factor (inverse of fwdMode, with the
 // reflection symmetries X_{2N-m} = -X_m (DCT-II), S_{2N-m} = S_m (DST-II)).
+// Quarter-wave axes (L = 4N): x_a = 1/2N * 2 sum_k X_k cos or sin(phi_k
+// (2a+1)) is the real inverse FFT, scaled by 1/4N, of the spectrum with
+// 2 e^{i phi_k} X_k (DCT-IV; -2i e^{i phi_k} X_k for DST-IV) at 2k+1 and
+// its conjugate factor (2 e^{-i phi_k}; +2i e^{-i phi_k}) at 4N-1-2k.
 __device__ __forceinline__ void bwdMode(int code, int n, int m, int& k, C2& w)
 {
     if (code == C_PER) { k = m; w = C2{1.0, 0.0}; return; }
     const double pi = 3.14159265358979323846;
+    if (code == C_ND || code == C_DN)
+    {
+        if ((m & 1) == 0) { k = -1; return; }
+        const bool lower = m < 2*n;
+        k = lower ? (m - 1)/2 : (4*n - 1 - m)/2;
+        const double phi = pi*(2*k + 1)/(4.0*n);
+        const C2 e = cexpi(lower ? phi : -phi);
+        if (code == C_ND)
+            w = C2{2.0*e.x, 2.0*e.y};
+        else
+            w = lower ? C2{2.0*e.y, -2.0*e.x} : C2{-2.0*e.y, 2.0*e.x};
+        return;
+    }
     const C2 e = cexpi(pi*m/(2.0*n));
     if (code == C_NN)
     {
@@ -454,14 +505,13 @@ void fft_create(bmg_fft** out, const bmg_grid* grid)
     g.code1 = axisCode(g.bc[2*g.d1], g.bc[2*g.d1 + 1]);
     g.code2 = axisCode(g.bc[2*g.d2], g.bc[2*g.d2 + 1]);
     g.codes = axisCode(g.bc[2*g.s], g.bc[2*g.s + 1]);
-    for (int c : {g.code1, g.code2})
-        if (c == C_DN || c == C_ND)
-            throw Error(BMG_ERR_ARG, "a Dirichlet-Neumann pair on a "
-                                     "transformed axis is not provided");
     if (g.ns < 2 || (g.codes == C_PER && g.ns < 3))
         throw Error(BMG_ERR_ARG, "too few cells along the tridiagonal axis");
-    g.L1 = g.code1 == C_PER ? g.n1 : 2*g.n1;
-    g.L2 = g.code2 == C_PER ? g.n2 : 2*g.n2;
+    auto extLen = [](int code, int n) {
+        return code == C_PER ? n : (code == C_ND || code == C_DN ? 4*n : 2*n);
+    };
+    g.L1 = extLen(g.code1, g.n1);
+    g.L2 = extLen(g.code2, g.n2);
     g.d1sqr = w[g.d1][0]*w[g.d1][0];
     g.d2sqr = w[g.d2][0]*w[g.d2][0];
     // widths along s with the ghost widths (mirrored, or the periodic

@@ -4,9 +4,10 @@ which is pinned by manufactured solutions (tests/test_fft_oracle.py): every
 supported boundary pair on the transformed and tridiagonal axes, uniform and
 stretched meshes, inhomogeneous boundary values through the ghosts of x,
 and the ghost refresh after the solve.  FFT rounding differs from the
-oracle's, so the bar is 1e-11 of the solution's magnitude.  Also: on the
-channel the FFT solution equals the multigrid's to its tolerance, and the
-unsupported configurations are refused."""
+oracle's, so the bar is 1e-11 of the solution's magnitude.  Also: mixed
+Dirichlet/Neumann pairs on the transformed axes (quarter-wave transforms),
+on the channel the FFT solution equals the multigrid's to its tolerance, and
+a mesh with two non-uniform directions is refused."""
 import itertools
 
 import numpy as np
@@ -93,11 +94,52 @@ def test_fft_channel_equals_multigrid(gpu):
 def test_fft_refuses_unsupported(gpu):
     u = np.linspace(0, 1, 9)
     s = np.tanh(np.linspace(-1, 1, 9))
+    # two non-uniform directions
     with pytest.raises(bmg.BmgError):
         bmg.FFT([s, s, u], [F.NEU] * 6)
-    # a Dirichlet-Neumann pair on a transformed axis
-    with pytest.raises(bmg.BmgError):
-        bmg.FFT([u, u, u], [F.NEU, F.NEU, F.DIR, F.NEU, F.PER, F.PER])
+
+
+PAIRS_T = [(F.DIR, F.NEU), (F.NEU, F.DIR), (F.NEU, F.NEU), (F.PER, F.PER)]
+
+
+@pytest.mark.parametrize("solve_pair", [(F.NEU, F.NEU), (F.DIR, F.NEU),
+                                        (F.PER, F.PER)],
+                         ids=["NN", "DN", "PP"])
+@pytest.mark.parametrize("pairs", [q for q in itertools.product(PAIRS_T,
+                                                                repeat=2)
+                                   if q[0][0] != q[0][1] and q[0][0] != F.PER
+                                   or q[1][0] != q[1][1] and q[1][0] != F.PER],
+                         ids=lambda q: "-".join("NDP"[a[0]] + "NDP"[a[1]]
+                                                for a in q))
+@pytest.mark.parametrize("stretch", [None, 1])
+def test_fft_quarter_wave_transformed_axes(gpu, stretch, pairs, solve_pair):
+    """Dirichlet-Neumann and Neumann-Dirichlet pairs on the transformed
+    axes (the reference's DST-IV / DCT-IV, RODFT11 / REDFT11), computed from
+    the quarter-wave extension to 4N."""
+    n = (12, 16, 10)
+    verts = [np.linspace(0.0, 1.0 + d, q + 1) for d, q in enumerate(n)]
+    if stretch is not None:
+        t = np.linspace(-1.0, 1.0, n[stretch] + 1)
+        verts[stretch] = np.tanh(1.5 * t) / np.tanh(1.5)
+    s_axis = 0 if stretch is None else stretch
+    trans = [d for d in (2, 1, 0) if d != s_axis][:2]
+    bc = [None] * 6
+    bc[2 * s_axis], bc[2 * s_axis + 1] = solve_pair
+    for d, pr in zip(trans, pairs):
+        bc[2 * d], bc[2 * d + 1] = pr
+    rng = np.random.default_rng(5)
+    vals = [0.0 if k == F.PER else v for k, v in zip(bc, rng.standard_normal(6))]
+    from test_fft_oracle import set_ghosts
+    x, b = manufactured(n, bc, verts, seed=13)
+    x = set_ghosts(x, bc, verts, vals)
+    b[1:-1, 1:-1, 1:-1] = -F.operator(x, verts, bc)
+    ref = F.solve(b, x, bc, verts)
+    print(f"[fft quarter-wave] bc={bc} stretch={stretch}", flush=True)
+    fft = bmg.FFT(verts, bc, vals)
+    xg = fft.solve(b.copy(), x.copy())
+    singular = F.DIR not in bc
+    assert rel(xg[1:-1, 1:-1, 1:-1], ref, singular) < TOL
+    assert rel(xg[1:-1, 1:-1, 1:-1], x[1:-1, 1:-1, 1:-1], singular) < 1e-10
 
 
 @pytest.mark.parametrize("pair", [(F.DIR, F.NEU), (F.NEU, F.DIR)])
