@@ -25,7 +25,8 @@ EXPORTS = [
     "bmg_default_opts", "bmg_last_error", "bmg_device_count", "bmg_create",
     "bmg_destroy", "bmg_set_opts", "bmg_num_levels", "bmg_level_shape",
     "bmg_set_operator_stencil", "bmg_set_operator_stencil_device",
-    "bmg_set_operator_laplacian", "bmg_get_operator_stencil", "bmg_solve",
+    "bmg_set_operator_laplacian", "bmg_set_operator_laplacian_ddt",
+    "bmg_get_operator_stencil", "bmg_solve",
     "bmg_solve_device", "bmg_upload_field", "bmg_download_field",
     "bmg_op_prepare", "bmg_op_smooth", "bmg_op_residual", "bmg_op_restrict",
     "bmg_op_prolong", "bmg_op_correct_bcs", "bmg_op_coarse_solve",
@@ -100,6 +101,8 @@ def load():
         "bmg_set_operator_stencil": (I, [P, I, DP, I]),
         "bmg_set_operator_stencil_device": (I, [P, I, P, I]),
         "bmg_set_operator_laplacian": (I, [P, DP, DP, DP]),
+        "bmg_set_operator_laplacian_ddt": (I, [P, DP, DP, DP, ctypes.c_double,
+                                               ctypes.c_double]),
         "bmg_get_operator_stencil": (I, [P, I, DP]),
         "bmg_solve": (I, [P, DP, DP, I, ctypes.POINTER(Stats)]),
         "bmg_solve_device": (I, [P, P, P, I, ctypes.POINTER(Stats)]),
@@ -208,17 +211,24 @@ class Context:
             raise ValueError(f"stencil shape {A.shape} != {self.padded(l) + (7,)}")
         _check(self.lib.bmg_set_operator_stencil(self.ptr, l, _dp(A), int(symmetric)))
 
-    def set_laplacian(self, lam=None):
+    def set_laplacian(self, lam=None, ddt=None):
+        """laplacian(lam, x) on every level; ddt = (delta_t, dt_frac) also
+        subtracts the Euler time term ddt(x)/dt_frac from the matrix."""
         if lam is None:
             null = ctypes.POINTER(ctypes.c_double)()
-            _check(self.lib.bmg_set_operator_laplacian(self.ptr, null, null, null))
+            ptrs = [null, null, null]
         else:
             self._lam = [np.ascontiguousarray(a, dtype=np.float64) for a in lam]
             for a in self._lam:
                 if a.shape != self.padded(0):
                     raise ValueError("lambda arrays must have the padded level-0 shape")
-            _check(self.lib.bmg_set_operator_laplacian(
-                self.ptr, *[_dp(a) for a in self._lam]))
+            ptrs = [_dp(a) for a in self._lam]
+        if ddt is None:
+            _check(self.lib.bmg_set_operator_laplacian(self.ptr, *ptrs))
+        else:
+            dt, frac = ddt
+            _check(self.lib.bmg_set_operator_laplacian_ddt(
+                self.ptr, *ptrs, float(dt), float(frac)))
 
     def stencil(self, l):
         A = np.zeros(self.padded(l) + (7,))

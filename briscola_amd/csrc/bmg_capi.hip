@@ -66,7 +66,10 @@ OpView<double> bmg_ctx::op<double>(int l) const
     const Level& q = lv[l];
     OpView<double> o{};
     o.kind = q.opkind;
-    for (int c = 0; c < 2; c++) { o.a7[c] = q.a7[c]; o.wf[c] = q.wf[c]; }
+    for (int c = 0; c < 2; c++)
+    {
+        o.a7[c] = q.a7[c]; o.wf[c] = q.wf[c]; o.dg[c] = q.dg[c];
+    }
     for (int d = 0; d < 3; d++) { o.w[d] = q.dw[d]; o.dl[d] = q.ddl[d]; }
     return o;
 }
@@ -77,7 +80,10 @@ OpView<float> bmg_ctx::op<float>(int l) const
     const Level& q = lv[l];
     OpView<float> o{};
     o.kind = q.opkind;
-    for (int c = 0; c < 2; c++) { o.a7[c] = q.a7f[c]; o.wf[c] = q.wff[c]; }
+    for (int c = 0; c < 2; c++)
+    {
+        o.a7[c] = q.a7f[c]; o.wf[c] = q.wff[c]; o.dg[c] = q.dgf[c];
+    }
     for (int d = 0; d < 3; d++) { o.w[d] = q.fw[d]; o.dl[d] = q.fdl[d]; }
     return o;
 }
@@ -669,6 +675,13 @@ void bmg_ctx::ensureMixed()
                 if (!q.wff[c]) q.wff[c] = devAlloc<float>(n*3);
             Pair<const double> in{{q.wf[0], q.wf[1]}};
             launch_to_float(n*3, in, pr(q.wff), stream);
+        }
+        if (q.opkind != OP_STENCIL && q.dg[0])
+        {
+            for (int c = 0; c < 2; c++)
+                if (!q.dgf[c]) q.dgf[c] = devAlloc<float>(n);
+            Pair<const double> in{{q.dg[0], q.dg[1]}};
+            launch_to_float(n, in, pr(q.dgf), stream);
         }
         for (int d = 0; d < 3; d++)
         {
@@ -1630,6 +1643,8 @@ static void setStencil(bmg_ctx* c, int level, const double* A7, bool device,
     {
         devFree(q.wf[cc]);
         devFree(q.wff[cc]);
+        devFree(q.dg[cc]);
+        devFree(q.dgf[cc]);
         if (!q.a7[cc]) q.a7[cc] = devAlloc<double>(q.L.ncol*7);
         HIPCK(hipMemsetAsync(q.a7[cc], 0, q.L.ncol*7*sizeof(double),
                              c->stream));
@@ -1661,15 +1676,23 @@ int bmg_set_operator_stencil_device(bmg_ctx* c, int level, const double* A7,
 
 // linearGaussLaplacianScheme::imLaplacian (linearGaussLaplacianScheme.C:
 // 22-102) on every level: face coefficients area*delta*lambda (lambda
-// restricted area-weighted to coarse levels) or the lambda-free operator.
-int bmg_set_operator_laplacian(bmg_ctx* c, const double* lx, const double* ly,
-                               const double* lz)
+// restricted area-weighted to coarse levels) or the lambda-free operator;
+// with dt > 0 minus im::ddt(x)/dtFrac of the Euler scheme, whose matrix is
+// the diagonal V/dt on every level (EulerDdtScheme.C:50-54) divided by
+// dtFrac (defaultPoissonSolver.C:91-92).
+static void setLaplacian(bmg_ctx* c, const double* lx, const double* ly,
+                         const double* lz, double dt, double dtFrac)
 {
-    return run([&] {
+    {
         if (!c) throw Error(BMG_ERR_ARG, "null context");
         const bool withLambda = lx != nullptr;
         if (withLambda && (!ly || !lz))
             throw Error(BMG_ERR_ARG, "need all three lambda arrays");
+        const bool withDdt = dt != 0.0;
+        if (withDdt && !(std::isfinite(dt) && dt > 0.0 &&
+                         std::isfinite(dtFrac) && dtFrac != 0.0))
+            throw Error(BMG_ERR_ARG, "time term needs a finite delta_t > 0 "
+                                     "and a finite, nonzero dt_frac");
         // input channel of the learned preconditioner: (log10(mean of the
         // six face coefficients) + 1.5) / 1.5 per cell, reference cell order
         {
@@ -1730,6 +1753,19 @@ int bmg_set_operator_laplacian(bmg_ctx* c, const double* lx, const double* ly,
             {
                 devFree(q.a7[cc]); devFree(q.a7f[cc]);
                 devFree(q.wf[cc]); devFree(q.wff[cc]);
+                devFree(q.dg[cc]); devFree(q.dgf[cc]);
+            }
+            if (withDdt)
+            {
+                // ghost entries stay zero; only interior cells are read
+                for (int cc = 0; cc < 2; cc++)
+                {
+                    q.dg[cc] = devAlloc<double>(q.L.ncol);
+                    HIPCK(hipMemsetAsync(q.dg[cc], 0,
+                                         q.L.ncol*sizeof(double), c->stream));
+                }
+                launch_ddt_diagonal(q.L, q.dw[0], q.dw[1], q.dw[2], dt, dtFrac,
+                                    pr(q.dg), c->stream);
             }
             if (withLambda)
             {
@@ -1758,7 +1794,12 @@ int bmg_set_operator_laplacian(bmg_ctx* c, const double* lx, const double* ly,
                 launch_materialize_stencil(q.L, c->op<double>(l), pr(q.a7),
                                            c->stream);
                 q.opkind = OP_STENCIL;
-                for (int cc = 0; cc < 2; cc++) devFree(q.wf[cc]);
+                HIPCK(hipStreamSynchronize(c->stream));
+                for (int cc = 0; cc < 2; cc++)
+                {
+                    devFree(q.wf[cc]);
+                    devFree(q.dg[cc]);
+                }
             }
             q.haveOp = true;
         }
@@ -1770,6 +1811,23 @@ int bmg_set_operator_laplacian(bmg_ctx* c, const double* lx, const double* ly,
         c->eliminated = false;
         c->prepared = false;
         c->mixedReady = false;
+    }
+}
+
+int bmg_set_operator_laplacian(bmg_ctx* c, const double* lx, const double* ly,
+                               const double* lz)
+{
+    return run([&] { setLaplacian(c, lx, ly, lz, 0.0, 1.0); });
+}
+
+int bmg_set_operator_laplacian_ddt(bmg_ctx* c, const double* lx,
+                                   const double* ly, const double* lz,
+                                   double delta_t, double dt_frac)
+{
+    return run([&] {
+        if (!(delta_t > 0.0))
+            throw Error(BMG_ERR_ARG, "delta_t must be > 0");
+        setLaplacian(c, lx, ly, lz, delta_t, dt_frac);
     });
 }
 
@@ -2235,7 +2293,8 @@ int bmg_time_smoother(bmg_ctx* c, int level, int precision, int reps,
         for (int d = 0; d < 3; d++) m[d] = std::max(0, rg.E[d] - rg.S[d]);
         const double cells = 0.5*double(m[0]*m[1]*m[2]);
         const int kind = q.opkind;
-        const double per = kind == OP_CONST ? 3.0 : kind == OP_FACES ? 9.0 : 10.0;
+        double per = kind == OP_CONST ? 3.0 : kind == OP_FACES ? 9.0 : 10.0;
+        if (kind != OP_STENCIL && q.dg[0]) per += 1.0;   // time-term diagonal
         if (avg_ms) *avg_ms = double(ms)/reps;
         if (bytes)
             *bytes = cells*per*(precision == BMG_MIXED ? sizeof(float)

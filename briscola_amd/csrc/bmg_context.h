@@ -83,6 +83,8 @@ struct Level
     float* a7f[2] = {nullptr, nullptr};
     double* wf[2] = {nullptr, nullptr};
     float* wff[2] = {nullptr, nullptr};
+    double* dg[2] = {nullptr, nullptr};    // Euler time-term diagonal
+    float* dgf[2] = {nullptr, nullptr};
     // metrics and transfer data
     double* dw[3] = {nullptr, nullptr, nullptr};
     double* ddl[3] = {nullptr, nullptr, nullptr};
@@ -114,6 +116,7 @@ struct Level
             devFree(x[c]); devFree(b[c]); devFree(r[c]); devFree(tmp[c]);
             devFree(xf[c]); devFree(bf[c]); devFree(rf[c]); devFree(tmpf[c]);
             devFree(a7[c]); devFree(a7f[c]); devFree(wf[c]); devFree(wff[c]);
+            devFree(dg[c]); devFree(dgf[c]);
         }
         for (int d = 0; d < 3; d++)
         {

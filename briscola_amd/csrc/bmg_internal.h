@@ -42,6 +42,9 @@ struct OpView {
     const T* wf[2];   // OP_FACES: 3*ncol per colour, component-major
     const T* w[3];    // cell widths per padded index
     const T* dl[3];   // 1/|dcc| of the lower face of each padded index
+    const T* dg[2];   // OP_CONST/OP_FACES: implicit Euler diagonal
+                      // (V/dt)/dtFrac per colour, subtracted from the
+                      // centre; null without a time term
 };
 
 template <typename T>
@@ -199,6 +202,9 @@ void launch_lambda_restrict(const LevelDims& Lf, const LevelDims& Lc,
                             const double* fw0, const double* fw1,
                             const double* fw2, const double* lamf,
                             double* lamc, int fd, hipStream_t s);
+void launch_ddt_diagonal(const LevelDims& L, const double* w0,
+                         const double* w1, const double* w2, double delta_t,
+                         double dt_frac, Pair<double> dg, hipStream_t s);
 void launch_face_coeffs(const LevelDims& L, const double* w0,
                         const double* w1, const double* w2, const double* dl,
                         const double* lam, int fd, Pair<double> wf,

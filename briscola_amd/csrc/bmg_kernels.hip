@@ -106,6 +106,9 @@ __device__ __forceinline__ void raw_coeffs(const LevelDims& L,
         c0 = c0 - a[4];
         c0 = c0 - a[5];
         c0 = c0 - a[6];
+        // sys -= ddt(x)/dtFrac lowers the centre by the Euler diagonal
+        // (defaultPoissonSolver.C:91-92, EulerDdtScheme.C:53)
+        if (op.dg[0]) c0 = c0 - ld(op.dg[colour], idx);
         a[0] = c0;
     }
 }
@@ -2000,6 +2003,22 @@ __global__ void k_face_coeffs(LevelDims L, const double* w0, const double* w1,
     wf.c[(I + J + K) & 1][fd*L.ncol + cidx(L, I, J, K)] = v;
 }
 
+// Diagonal of im::ddt(x)/dtFrac with the Euler scheme on one level:
+// ((V/dt)/dtFrac) per interior cell, V = (w0*w1)*w2 (EulerDdtScheme.C:53,
+// linearSystemI.H:392-410), stored colour-split.
+__global__ void k_ddt_diagonal(LevelDims L, const double* w0, const double* w1,
+                               const double* w2, double dt, double frac,
+                               Pair<double> dg)
+{
+    const long t = (long)blockIdx.x*blockDim.x + threadIdx.x;
+    if (t >= (long)L.N[0]*L.N[1]*L.N[2]) return;
+    const int I = 1 + int(t/((long)L.N[1]*L.N[2]));
+    const int J = 1 + int((t/L.N[2]) % L.N[1]);
+    const int K = 1 + int(t % L.N[2]);
+    const double v = (w0[I]*w1[J])*w2[K];
+    dg.c[(I + J + K) & 1][cidx(L, I, J, K)] = (v/dt)/frac;
+}
+
 // Flux of x through the lower faces normal to axis fd, reference layout
 // (PoissonSolver flux, defaultPoissonSolver.C:108-121): the face gradient
 // along the outward normal of the lower face, g = (x_lower - x_cell)*delta
@@ -3881,6 +3900,16 @@ void launch_face_coeffs(const LevelDims& L, const double* w0, const double* w1,
                    (L.N[2] + (fd == 2));
     hipLaunchKernelGGL(k_face_coeffs, dim3((n + 255)/256), dim3(256), 0, s, L,
                        w0, w1, w2, dl, lam, fd, wf);
+}
+
+void launch_ddt_diagonal(const LevelDims& L, const double* w0,
+                         const double* w1, const double* w2, double delta_t,
+                         double dt_frac, Pair<double> dg, hipStream_t s)
+{
+    direct_only("time-term diagonal");
+    const long n = long(L.N[0])*L.N[1]*L.N[2];
+    hipLaunchKernelGGL(k_ddt_diagonal, dim3((n + 255)/256), dim3(256), 0, s, L,
+                       w0, w1, w2, delta_t, dt_frac, dg);
 }
 
 void launch_face_flux(const LevelDims& L, const double* x, const double* lam,

@@ -84,6 +84,7 @@ struct fvMesh
     int bc[6];                 // BMG_NEUMANN / BMG_DIRICHLET / BMG_PERIODIC
     double bcValue[6] = {0, 0, 0, 0, 0, 0};
     std::vector<double> cellVolumes;
+    double deltaT = 0.0;       // time().deltaTValue() (ddtScheme.H:59)
 };
 
 // linearSystem<stencil,scalar,colocated>: unknown x, deep stencil field A

@@ -14,7 +14,12 @@
 //   * the system is laplacian(lambda, x) (assembled on the device on every
 //     level, lambda restricted by face area) plus the source: sys += b means
 //     b_sys = 0 - V*b per cell (linearSystem.C:546-556);
-//   * constMatrix = !ddt && !lambda: without lambda the operator is
+//   * with ddt the Euler time term is subtracted, sys -= ddt(x)/dtFrac
+//     (defaultPoissonSolver.C:91-92, EulerDdtScheme.C:50-54): the matrix
+//     centre by (V/deltaT)/dtFrac on every level (assembled on the device)
+//     and the source by ((V*xOld)/deltaT)/dtFrac, xOld being x.oldTime()
+//     and deltaT the mesh's time step;
+//   * constMatrix = !ddt && !lambda: without lambda or ddt the operator is
 //     assembled once and the coarse factorisation kept (MG.C:449-459);
 //   * with a Runge-Kutta scheme the previous solution of the same stage is
 //     the initial guess, and the new solution is stored for the next time
@@ -23,8 +28,6 @@
 //     every face after the solve (defaultPoissonSolver.C:108-121);
 //   * the solver is the one the dictionary names (its keys are MG's; see
 //     hipMG.H), and its statistics line is printed as MG prints it.
-// Not provided: the implicit time term (ddt = true), which needs a diagonal
-// V/(dt*dtFrac) on every level; asking for it is a FatalError.
 #ifndef BRISCOLA_AMD_HIPPOISSONSOLVER_H
 #define BRISCOLA_AMD_HIPPOISSONSOLVER_H
 
@@ -100,35 +103,43 @@ public:
     const solverStats& lastStats() const { return last_; }
     const std::vector<meshField>& stages() const { return stages_; }
 
-    // defaultPoissonSolver::solve (defaultPoissonSolver.C:39-124)
+    // defaultPoissonSolver::solve (defaultPoissonSolver.C:39-124).  xOld is
+    // x.oldTime(), read when ddt is set.
     void solve(meshField& x, const meshField* bPtr,
                const faceField* lambdaPtr, bool ddt = false,
-               double dtFrac = 1.0, const RungeKuttaScheme* rk = nullptr)
+               double dtFrac = 1.0, const RungeKuttaScheme* rk = nullptr,
+               const meshField* xOld = nullptr)
     {
-        (void)dtFrac;
         const size_t n = cells();
         if (x.size() != n) throw FatalError("x must be the padded level 0");
         if (ddt)
-            throw FatalError("the implicit time term is not provided by the "
-                             "GPU Poisson solver");
+        {
+            if (!xOld || xOld->size() != n)
+                throw FatalError("ddt needs the old-time field of x, padded "
+                                 "level 0");
+            if (mesh_.cellVolumes.size() != n)
+                throw FatalError("ddt needs the padded level-0 cell volumes");
+            if (!(mesh_.deltaT > 0.0))
+                throw FatalError("ddt needs deltaT > 0");
+        }
         if (rk && stages_.empty()) stages_.assign(rk->nStages, x);
 
-        // sys = im::laplacian(lambda, x)
+        // sys = im::laplacian(lambda, x), minus im::ddt(x)/dtFrac's matrix
+        const double* l[3] = {nullptr, nullptr, nullptr};
         if (lambdaPtr)
-        {
-            for (const auto& a : lambdaPtr->f)
-                if (a.size() != n) throw FatalError("lambda size");
-            check(bmg_set_operator_laplacian(ctx_, lambdaPtr->f[0].data(),
-                                             lambdaPtr->f[1].data(),
-                                             lambdaPtr->f[2].data()));
-        }
-        else if (!assembled_)
-        {
-            check(bmg_set_operator_laplacian(ctx_, nullptr, nullptr,
-                                             nullptr));
-        }
-        // a lambda operator replaces the lambda-free one
-        assembled_ = !lambdaPtr;
+            for (int d = 0; d < 3; d++)
+            {
+                if (lambdaPtr->f[d].size() != n)
+                    throw FatalError("lambda size");
+                l[d] = lambdaPtr->f[d].data();
+            }
+        if (ddt)
+            check(bmg_set_operator_laplacian_ddt(ctx_, l[0], l[1], l[2],
+                                                 mesh_.deltaT, dtFrac));
+        else if (lambdaPtr || !assembled_)
+            check(bmg_set_operator_laplacian(ctx_, l[0], l[1], l[2]));
+        // a lambda or time-term operator replaces the plain one
+        assembled_ = !lambdaPtr && !ddt;
 
         // sys += b: b_sys = 0 - V*b (linearSystem.C:546-556)
         std::vector<double> bsys(n, 0.0);
@@ -143,6 +154,15 @@ public:
                 bsys[c] = 0.0 - t;
             }
         }
+        // sys -= ddt(x)/dtFrac: b_sys -= ((V*xOld)/deltaT)/dtFrac
+        // (EulerDdtScheme.C:54, linearSystemI.H:392-410, linearSystem.C:
+        // 694-706)
+        if (ddt)
+            for (size_t c = 0; c < n; c++)
+            {
+                const double t = mesh_.cellVolumes[c]*(*xOld)[c];
+                bsys[c] = bsys[c] - (t/mesh_.deltaT)/dtFrac;
+            }
         const bool constMatrix = !ddt && !lambdaPtr;
 
         if (rk) x = stages_.at(rk->stage - 1);

@@ -145,6 +145,19 @@ int bmg_set_operator_stencil_device(bmg_ctx* ctx, int level,
 int bmg_set_operator_laplacian(bmg_ctx* ctx, const double* lambda_x,
                                const double* lambda_y,
                                const double* lambda_z);
+/* The same minus the implicit Euler time term, the system defaultPoissonSolver
+ * builds with ddt = true (defaultPoissonSolver.C:86-94): sys -= im::ddt(x) /
+ * dt_frac, whose matrix is the diagonal V/delta_t of every level
+ * (EulerDdtScheme.C:50-54; V = (dx*dy)*dz of the level's cell), so every
+ * centre coefficient is lowered by (V/delta_t)/dt_frac before ghost
+ * elimination.  The matching source term, b_sys -= ((V*x_old)/delta_t)/
+ * dt_frac on level 0, is the caller's part of b.  delta_t > 0 and a finite
+ * nonzero dt_frac; BMG_ERR_ARG otherwise.  Replaces
+ * defaultPoissonSolver.C:91-92 with Euler as the ddt scheme. */
+int bmg_set_operator_laplacian_ddt(bmg_ctx* ctx, const double* lambda_x,
+                                   const double* lambda_y,
+                                   const double* lambda_z, double delta_t,
+                                   double dt_frac);
 
 /* Read back sys.A()[level] after the solver's ghost elimination and
  * singularity pin (the reference mutates the caller's A, MG.C:433-447). */

@@ -61,6 +61,8 @@ def _load():
         "oracle_level_centres": (None, [P, c_int, c_int, D]),
         "oracle_level_widths": (None, [P, c_int, c_int, D]),
         "oracle_assemble_laplacian": (None, [P, D, D, D]),
+        "oracle_assemble_laplacian_ddt": (None, [P, D, D, D, ctypes.c_double,
+                                                 ctypes.c_double]),
         "oracle_set_stencil": (None, [P, c_int, D]),
         "oracle_get_stencil": (None, [P, c_int, D]),
         "oracle_set_symmetric": (None, [P, c_int]),
@@ -158,16 +160,20 @@ class Oracle:
         self.lib.oracle_level_widths(self.ptr, l, axis, _dp(out))
         return out
 
-    def assemble(self, lam=None):
+    def assemble(self, lam=None, ddt=None):
+        """laplacian(lam, x); with ddt = (delta_t, dt_frac) minus the
+        Euler time term ddt(x)/dt_frac."""
+        dt, frac = ddt if ddt is not None else (0.0, 1.0)
         if lam is None:
             null = ctypes.POINTER(ctypes.c_double)()
-            self.lib.oracle_assemble_laplacian(self.ptr, null, null, null)
+            self.lib.oracle_assemble_laplacian_ddt(self.ptr, null, null, null,
+                                                   dt, frac)
         else:
             self._lam = [np.ascontiguousarray(a, dtype=np.float64) for a in lam]
             for a in self._lam:
                 assert a.shape == self.padded(0)
-            self.lib.oracle_assemble_laplacian(self.ptr,
-                                               *[_dp(a) for a in self._lam])
+            self.lib.oracle_assemble_laplacian_ddt(
+                self.ptr, *[_dp(a) for a in self._lam], dt, frac)
 
     def set_stencil(self, l, A):
         A = np.ascontiguousarray(A, dtype=np.float64)

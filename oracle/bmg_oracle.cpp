@@ -1211,6 +1211,19 @@ void oracle_level_widths(const oracle_ctx* c, int l, int axis, double* out)
 void oracle_assemble_laplacian(oracle_ctx* c, const double* lam_x,
                                const double* lam_y, const double* lam_z)
 {
+    oracle_assemble_laplacian_ddt(c, lam_x, lam_y, lam_z, 0.0, 1.0);
+}
+
+/* sys -= im::ddt(x)/dtFrac (defaultPoissonSolver.C:91-92) after the
+ * Laplacian: the Euler matrix is cv/deltaT on every level
+ * (EulerDdtScheme.C:50-54), divided by dtFrac (linearSystemI.H:392-410),
+ * and subtracted from the stencil centre (linearSystem.C:694-706).  Volumes
+ * are (dx*dy)*dz; only interior rows are solved, so ghost rows keep the
+ * Laplacian alone. */
+void oracle_assemble_laplacian_ddt(oracle_ctx* c, const double* lam_x,
+                                   const double* lam_y, const double* lam_z,
+                                   double delta_t, double dt_frac)
+{
     const bool withLambda = lam_x != nullptr;
     if (withLambda)
     {
@@ -1224,6 +1237,15 @@ void oracle_assemble_laplacian(oracle_ctx* c, const double* lam_x,
     {
         assembleLevel(lv, withLambda);
         std::fill(lv.b.begin(), lv.b.end(), 0.0);
+        if (delta_t == 0.0) continue;
+        for (int i = 1; i <= lv.N[0]; i++)
+        for (int j = 1; j <= lv.N[1]; j++)
+        for (int k = 1; k <= lv.N[2]; k++)
+        {
+            const double cv = (lv.ax[0].w[i]*lv.ax[1].w[j])*lv.ax[2].w[k];
+            const double a = cv/delta_t;
+            lv.A[7*lv.lin(i, j, k)] -= a/dt_frac;
+        }
     }
     c->symmetric = true;
     c->eliminated = false;

@@ -59,6 +59,13 @@ void oracle_level_widths(const oracle_ctx*, int l, int axis, double* out);
  * lower-face coefficients, restricted to coarse levels area-weighted. */
 void oracle_assemble_laplacian(oracle_ctx*, const double* lam_x,
                                const double* lam_y, const double* lam_z);
+/* The same minus im::ddt(x)/dt_frac of the Euler scheme: every interior
+ * centre lowered by (((dx*dy)*dz)/delta_t)/dt_frac, every level
+ * (defaultPoissonSolver.C:91-92, EulerDdtScheme.C:50-54).  delta_t = 0: no
+ * time term. */
+void oracle_assemble_laplacian_ddt(oracle_ctx*, const double* lam_x,
+                                   const double* lam_y, const double* lam_z,
+                                   double delta_t, double dt_frac);
 void oracle_set_stencil(oracle_ctx*, int l, const double* A7);
 void oracle_get_stencil(const oracle_ctx*, int l, double* A7);
 void oracle_set_symmetric(oracle_ctx*, int symmetric);

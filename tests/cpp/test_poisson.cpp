@@ -1,12 +1,16 @@
 // Drives the PoissonSolver adapter (briscola_amd/host/hipPoissonSolver.H)
 // the way a Briscola flow solver drives defaultPoissonSolver: a mesh with
 // cell volumes, an optional face coefficient lambda, the flux switched on
-// and a Runge-Kutta scheme whose stages run 1..S and then 1 again.
+// and a Runge-Kutta scheme whose stages run 1..S and then 1 again.  With
+// ddt set every solve carries the implicit Euler term with x.oldTime() the
+// previous solve's result (zero before the first).
 //
 //   test_poisson <dir>
-// reads  dir/shape.txt  "nx ny nz bc0..bc5 relTol coarseLevel stages lambda"
+// reads  dir/shape.txt  "nx ny nz bc0..bc5 relTol coarseLevel stages lambda
+//                        ddt deltaT dtFrac"
 //        dir/vertices.bin, dir/volumes.bin, dir/lambda.bin (3 face arrays,
-//        when lambda is 1), dir/src.bin (stages+1 padded sources)
+//        when lambda is 1), dir/src.bin (stages+1 padded sources),
+//        dir/xold.bin (optional: x.oldTime() of the first solve)
 // writes dir/x<k>.bin, dir/flux<k>.bin (3 arrays), dir/stats.txt (one line
 // per solve) for the solves k = 0..stages.
 #include "hipPoissonSolver.H"
@@ -40,10 +44,13 @@ int main(int argc, char** argv)
     std::ifstream sh(dir + "/shape.txt");
     fvMesh mesh;
     std::string relTol;
-    int coarseLevel = 0, stages = 1, useLambda = 0;
+    int coarseLevel = 0, stages = 1, useLambda = 0, ddt = 0;
+    double dtFrac = 1.0;
     sh >> mesh.N[0] >> mesh.N[1] >> mesh.N[2];
     for (int f = 0; f < 6; f++) sh >> mesh.bc[f];
-    sh >> relTol >> coarseLevel >> stages >> useLambda;
+    sh >> relTol >> coarseLevel >> stages >> useLambda >> ddt >> mesh.deltaT
+       >> dtFrac;
+    if (!sh) { std::fprintf(stderr, "bad shape.txt\n"); return 2; }
     const size_t nv = size_t(mesh.N[0] + 1) + mesh.N[1] + 1 + mesh.N[2] + 1;
     std::vector<double> v = readBin(dir + "/vertices.bin", nv);
     size_t o = 0;
@@ -72,13 +79,17 @@ int main(int argc, char** argv)
         ps.enableFlux();
         RungeKuttaScheme rk;
         rk.nStages = stages;
-        meshField x(n, 0.0);
+        meshField x(n, 0.0), xOld(n, 0.0);
+        if (std::ifstream(dir + "/xold.bin").good())
+            xOld = readBin(dir + "/xold.bin", n);
         std::ofstream st(dir + "/stats.txt");
         for (int k = 0; k <= stages; k++)
         {
             rk.stage = k < stages ? k + 1 : 1;
             meshField b(src.begin() + k*n, src.begin() + (k + 1)*n);
-            ps.solve(x, &b, useLambda ? &lambda : nullptr, false, 1.0, &rk);
+            ps.solve(x, &b, useLambda ? &lambda : nullptr, ddt != 0, dtFrac,
+                     &rk, &xOld);
+            xOld = x;
             writeBin(dir + "/x" + std::to_string(k) + ".bin", x);
             std::vector<double> f;
             for (int d = 0; d < 3; d++)
@@ -87,11 +98,11 @@ int main(int argc, char** argv)
             st << ps.lastStats().nIter << " " << ps.lastStats().initialResidual
                << " " << ps.lastStats().finalResidual << "\n";
         }
-        // the implicit time term is refused, as documented
+        // the time term needs x.oldTime()
         bool threw = false;
         try { ps.solve(x, nullptr, nullptr, true, 1.0); }
         catch (const FatalError&) { threw = true; }
-        if (!threw) throw FatalError("ddt was accepted");
+        if (!threw) throw FatalError("ddt without the old-time field");
         ps.disableFlux();
         threw = false;
         try { ps.flux(); }

@@ -608,3 +608,96 @@ def test_oracle_threads_do_not_change_results(case):
         oracle.set_threads(prev)
     assert np.array_equal(out[0][0], out[1][0])
     assert np.array_equal(out[0][1], out[1][1])
+
+
+# ------------------------------------------------------- implicit time term
+
+def _dirichlet_sines(shape, modes):
+    """sin(pi m (i - 1/2)/N) per axis on the padded grid: with homogeneous
+    Dirichlet faces (ghost = -interior after elimination) it is an exact
+    eigenvector of the 7-point operator, eigenvalue sum_d h_d^-1 A_d
+    (2 cos(pi m_d/N_d) - 2) for face area A_d and spacing h_d."""
+    axes = []
+    for q, m in zip(shape, modes):
+        i = np.arange(q + 2)
+        axes.append(np.sin(np.pi * m * (i - 0.5) / q))
+    return axes[0][:, None, None] * axes[1][None, :, None] * axes[2][None, None, :]
+
+
+def _apply_stencil(A, x):
+    """y = A x on interior cells, A the eliminated 7-point stencil
+    [C,L,R,B,T,A,F] (ghost couplings zeroed by the elimination)."""
+    y = A[1:-1, 1:-1, 1:-1, 0] * inner(x)
+    nb = [(-1, 0, 0), (1, 0, 0), (0, -1, 0), (0, 1, 0), (0, 0, -1), (0, 0, 1)]
+    for s, (di, dj, dk) in enumerate(nb, start=1):
+        sh = x[1 + di:x.shape[0] - 1 + di, 1 + dj:x.shape[1] - 1 + dj,
+               1 + dk:x.shape[2] - 1 + dk]
+        y = y + A[1:-1, 1:-1, 1:-1, s] * sh
+    return y
+
+
+@pytest.mark.parametrize("dt,frac", [(2e-3, 1.0), (5e-4, 0.5), (3e-3, 1.0 / 3)])
+def test_ddt_operator_is_laplacian_minus_euler_diagonal(dt, frac):
+    """Every level of laplacian(x) - ddt(x)/dtFrac (Euler) maps a Dirichlet
+    sine mode to lambda_L - V/(dt*dtFrac) times itself, V = h^3 of that
+    level (EulerDdtScheme.C:50-54 on every level, defaultPoissonSolver.C:
+    91-92).  A wrong sign, a dropped dtFrac or a fine-level volume on a
+    coarse level moves the eigenvalue."""
+    n = (16, 8, 32)
+    L = [1.0, 0.5, 2.0]
+    verts = [np.linspace(0, Ld, q + 1) for Ld, q in zip(L, n)]
+    o = oracle.Oracle(verts, (D,) * 6)
+    o.assemble(ddt=(dt, frac))
+    o.pin_and_eliminate()
+    assert not o.singular()
+    for l in range(o.nlevels):
+        shp = o.shapes[l]
+        h = [Ld / q for Ld, q in zip(L, shp)]
+        vol = h[0] * h[1] * h[2]
+        for modes in [(1, 1, 1), (2, 1, 3), (shp[0], shp[1], max(shp[2] - 1, 1))]:
+            x = _dirichlet_sines(shp, modes)
+            lam = sum(vol / h[d] ** 2 * (2 * math.cos(math.pi * modes[d] / shp[d]) - 2)
+                      for d in range(3))
+            lam -= vol / (dt * frac)
+            y = _apply_stencil(o.stencil(l), x)
+            scale = np.abs(inner(x)).max() * abs(lam)
+            assert np.abs(y - lam * inner(x)).max() <= 1e-12 * scale, \
+                f"level {l} modes {modes}"
+
+
+def test_ddt_solve_is_one_implicit_euler_step():
+    """With b = -((V x_old)/dt)/dtFrac (the ddt source, no other term) the
+    solve is one backward Euler step of V dx/dt = laplacian(x) with step
+    dt*dtFrac; per sine mode x = x_old / (1 - dt dtFrac lambda_L / V), the
+    backward Euler amplification factor."""
+    n = 16
+    dt, frac = 1e-2, 0.5
+    verts = [np.linspace(0, 1, n + 1)] * 3
+    o = oracle.Oracle(verts, (D,) * 6)
+    o.set_opts(rel_tol=1e-13, n_sweeps_pre=2, n_sweeps_post=2,
+               cycle=oracle.VCYCLE, coarse_level=1)
+    o.assemble(ddt=(dt, frac))
+    h = 1.0 / n
+    V = h ** 3
+    modes = (3, 1, 2)
+    xold = _dirichlet_sines((n, n, n), modes)
+    b = 0.0 - ((V * xold) / dt) / frac
+    o.set_field(oracle.B, 0, b)
+    it, r0, r1 = o.solve()
+    assert r1 <= 1e-12
+    lamL = sum(h * (2 * math.cos(math.pi * m / n) - 2) for m in modes)
+    g = 1.0 / (1.0 - dt * frac * lamL / V)
+    x = inner(o.field(oracle.X, 0))
+    assert np.abs(x - g * inner(xold)).max() <= 1e-10 * np.abs(inner(xold)).max()
+    # without the time term's matrix the same b gives a different field
+    assert abs(g - 1.0) > 0.1
+
+
+def test_ddt_zero_dt_is_plain_laplacian():
+    p = W.spheres(8, seed=2, nspheres=2)
+    a = oracle.Oracle(p.vertices, p.bc, p.bc_value)
+    a.assemble(p.lam)
+    b = oracle.Oracle(p.vertices, p.bc, p.bc_value)
+    b.assemble(p.lam, ddt=(0.0, 1.0))
+    for l in range(a.nlevels):
+        assert np.array_equal(a.stencil(l), b.stencil(l))
