@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(_HERE, "libbmg.so")
 NEUMANN, DIRICHLET, PERIODIC = 0, 1, 2
 VCYCLE, WCYCLE, FCYCLE = 0, 1, 2
 COARSE_SMOOTH, COARSE_DIRECT = 0, 1
-RBGS, JAC = 0, 1
+RBGS, JAC, LEXGS = 0, 1, 2
 FP64, MIXED = 0, 1
 KRYLOV_NONE, KRYLOV_FCG, KRYLOV_FCG_LEARNED = 0, 1, 2
 X, B, R = 0, 1, 2

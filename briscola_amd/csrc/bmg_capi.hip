@@ -259,6 +259,23 @@ void bmg_ctx::smooth(int l, int sweeps)
     const OpView<T> o = op<T>(l);
     Pair<T> x = X<T>(l);
     Pair<const T> b = cst(B<T>(l));
+    if (opts.smoother == BMG_LEXGS)
+    {
+        // lexgsSmoother::Smooth (lexgsSmoother.C:256-317): interior region,
+        // periodic ghosts, then the width-1 slab of every periodic face
+        for (int s = 0; s < sweeps; s++)
+        {
+            launch_lexgs(q.L, o, x, b, interior(l, 1), stream);
+            if (anyPeriodic())
+            {
+                launch_bc_fill(q.L, o, x, 0, l == 0, bcval, stream);
+                for (int f = 0; f < 6; f++)
+                    if (periodic(f))
+                        launch_lexgs(q.L, o, x, b, slab(l, f), stream);
+            }
+        }
+        return;
+    }
     if (opts.smoother == BMG_JAC)
     {
         Pair<T> tmp = Tmp<T>(l);
@@ -1430,7 +1447,7 @@ static void validateOpts(const bmg_opts* o)
         o->n_sweeps_post < 0 || o->coarse_level < 0)
         throw Error(BMG_ERR_ARG, "negative iteration/sweep/level option");
     if (o->cycle < 0 || o->cycle > 2 || o->coarse_mode < 0 ||
-        o->coarse_mode > 1 || o->smoother < 0 || o->smoother > 1 ||
+        o->coarse_mode > 1 || o->smoother < 0 || o->smoother > 2 ||
         o->precision < 0 || o->precision > 1 || o->krylov < 0 ||
         o->krylov > 2)
         throw Error(BMG_ERR_ARG, "unknown enum value in options");

@@ -148,6 +148,9 @@ void launch_rbgs_slabs(const LevelDims& L, const OpView<T>& op, Pair<T> x,
                        const int* colours, int nphase, const SlabIO<T>& io,
                        hipStream_t s);
 template <typename T>
+void launch_lexgs(const LevelDims& L, const OpView<T>& op, Pair<T> x,
+                  Pair<const T> b, const Region& rg, hipStream_t s);
+template <typename T>
 void launch_jacobi(const LevelDims& L, const OpView<T>& op, Pair<T> x,
                    Pair<const T> b, Pair<T> tmp, const Region& rg,
                    hipStream_t s);

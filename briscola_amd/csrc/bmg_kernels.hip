@@ -784,6 +784,54 @@ struct KSlabs
 };
 
 
+// Lexicographic Gauss-Seidel over region [S,E) (lexgsSmoother.C:14-124,
+// SweepBoundary 126-254): a forward pass in (i,j,k) order, then a backward
+// pass in the reverse order, each cell
+//   x_c = (b_c - (A1 x_-i + A2 x_+i + A3 x_-j + A4 x_+j + A5 x_-k + A6 x_+k))
+//         / A0.
+// In the forward pass a cell reads its three lower neighbours after their
+// update and its three upper neighbours before theirs (the reverse in the
+// backward pass), and no two cells of one hyperplane (i-S0)+(j-S1)+(k-S2) =
+// s are neighbours.  One launch per hyperplane, in pass order, therefore
+// hands every cell the very values the sequential sweep hands it.  Block
+// row y is i = S0 + ilo + y; its threads take the j of the hyperplane's
+// (i, j) segment, k follows.
+template <typename T, int OP>
+__global__ __launch_bounds__(256) void k_lexgs_plane(LevelDims L,
+                                                     OpView<T> op, Pair<T> x,
+                                                     Pair<const T> b,
+                                                     Region rg, int s, int ilo)
+{
+    const int M1 = rg.E[1] - rg.S[1], M2 = rg.E[2] - rg.S[2];
+    const int ii = ilo + int(blockIdx.y);
+    const int jlo = max(0, s - ii - (M2 - 1));
+    const int jhi = min(M1 - 1, s - ii);
+    const int jj = jlo + int(blockIdx.x*blockDim.x + threadIdx.x);
+    if (jj > jhi) return;
+    const int kk = s - ii - jj;
+    if (kk < 0 || kk >= M2) return;
+    const int I = rg.S[0] + ii, J = rg.S[1] + jj, K = rg.S[2] + kk;
+    const int colour = (I + J + K) & 1;
+    const int q = K & 1, p = K >> 1;
+    const int row = (I*L.P[1] + J)*L.Hz;
+    const unsigned idx = unsigned(row + p);
+    const unsigned sI = L.sI, Hz = L.Hz;
+    const T* xo = col(x, colour ^ 1);
+    const T nb[6] = {ld(xo, idx - sI), ld(xo, idx + sI), ld(xo, idx - Hz),
+                     ld(xo, idx + Hz), ld(xo, idx - 1 + q), ld(xo, idx + q)};
+    const T bv = ld(col(b, colour), idx);
+    T a[7];
+    coeffs<T, OP>(L, op, colour, int(idx), row, p, q, I, J, K, a);
+    const T P =
+        a[1]*nb[0]
+      + a[2]*nb[1]
+      + a[3]*nb[2]
+      + a[4]*nb[3]
+      + a[5]*nb[4]
+      + a[6]*nb[5];
+    st(col(x, colour), idx, (bv - P)/a[0]);
+}
+
 // Damped Jacobi (jacSmoother.C:18-127, omega 0.8): y from x, then copy back.
 template <typename T, int OP>
 __global__ __launch_bounds__(BX*BY) void k_jacobi(LevelDims L, OpView<T> op,
@@ -3716,6 +3764,45 @@ void launch_jacobi(const LevelDims& L, const OpView<T>& op, Pair<T> x,
 }
 
 template <typename T>
+void launch_lexgs(const LevelDims& L, const OpView<T>& op, Pair<T> x,
+                  Pair<const T> b, const Region& rg, hipStream_t s)
+{
+    direct_only("lexicographic smoothing");
+    const int M[3] = {rg.E[0] - rg.S[0], rg.E[1] - rg.S[1], rg.E[2] - rg.S[2]};
+    if (M[0] <= 0 || M[1] <= 0 || M[2] <= 0) return;
+    for (int d = 0; d < 3; d++)
+        if (rg.S[d] < 1 || rg.E[d] > L.N[d] + 1)
+            throw std::runtime_error("lexicographic sweep outside the level");
+    if (M[0] > 65535)
+        throw std::runtime_error("lexicographic sweep: level too long along x");
+    const int ns = M[0] + M[1] + M[2] - 2;
+    const int w = std::min(M[1], M[2]);
+    for (int pass = 0; pass < 2; pass++)
+        for (int t = 0; t < ns; t++)
+        {
+            const int sp = pass == 0 ? t : ns - 1 - t;
+            const int ilo = std::max(0, sp - (M[1] - 1) - (M[2] - 1));
+            const int ihi = std::min(M[0] - 1, sp);
+            const dim3 g((w + 255)/256, ihi - ilo + 1);
+            switch (op.kind)
+            {
+            case OP_STENCIL:
+                hipLaunchKernelGGL((k_lexgs_plane<T, OP_STENCIL>), g,
+                                   dim3(256), 0, s, L, op, x, b, rg, sp, ilo);
+                break;
+            case OP_CONST:
+                hipLaunchKernelGGL((k_lexgs_plane<T, OP_CONST>), g, dim3(256),
+                                   0, s, L, op, x, b, rg, sp, ilo);
+                break;
+            default:
+                hipLaunchKernelGGL((k_lexgs_plane<T, OP_FACES>), g, dim3(256),
+                                   0, s, L, op, x, b, rg, sp, ilo);
+                break;
+            }
+        }
+}
+
+template <typename T>
 void launch_residual(const LevelDims& L, const OpView<T>& op, Pair<const T> x,
                      Pair<const T> b, Pair<T> r, double* partials,
                      int* nblocks, hipStream_t s)
@@ -4161,6 +4248,9 @@ void launch_set_cell(Pair<T> f, long idx, int colour, T v, hipStream_t s)
     template void launch_jacobi<T>(const LevelDims&, const OpView<T>&,        \
                                    Pair<T>, Pair<const T>, Pair<T>,           \
                                    const Region&, hipStream_t);               \
+    template void launch_lexgs<T>(const LevelDims&, const OpView<T>&,         \
+                                  Pair<T>, Pair<const T>, const Region&,      \
+                                  hipStream_t);                               \
     template void launch_residual<T>(const LevelDims&, const OpView<T>&,      \
                                      Pair<const T>, Pair<const T>, Pair<T>,   \
                                      double*, int*, hipStream_t);             \

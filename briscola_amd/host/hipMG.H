@@ -143,6 +143,7 @@ inline bmg_opts optsFromDict(const dictionary& dict)
     const std::string sm = dict.lookupOrDefault("smoother", "rbgs");
     if (sm == "rbgs") o.smoother = BMG_RBGS;
     else if (sm == "jac") o.smoother = BMG_JAC;
+    else if (sm == "lexgs") o.smoother = BMG_LEXGS;
     else throw FatalError("Unsupported smoother type " + sm);
     if (dict.lookupOrDefault("prolong", "linear") != "linear" ||
         dict.lookupOrDefault("restrict", "linear") != "linear")

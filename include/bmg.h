@@ -51,8 +51,12 @@ enum bmg_bc { BMG_NEUMANN = 0, BMG_DIRICHLET = 1, BMG_PERIODIC = 2 };
 enum bmg_cycle { BMG_VCYCLE = 0, BMG_WCYCLE = 1, BMG_FCYCLE = 2 };
 /* MGCoarseMode, MG.H:26-30 */
 enum bmg_coarse_mode { BMG_COARSE_SMOOTH = 0, BMG_COARSE_DIRECT = 1 };
-/* smoother TypeNames rbgs / jac (rbgsSmoother.H:56, jacSmoother.H) */
-enum bmg_smoother { BMG_RBGS = 0, BMG_JAC = 1 };
+/* smoother TypeNames rbgs / jac / lexgs (rbgsSmoother.H:56, jacSmoother.H,
+ * lexgsSmoother.H).  BMG_LEXGS keeps the sequential lexicographic order
+ * exactly (one launch per hyperplane i+j+k = const of the swept region), so
+ * it is far slower than BMG_RBGS; it is there for parity with setups that
+ * name it. */
+enum bmg_smoother { BMG_RBGS = 0, BMG_JAC = 1, BMG_LEXGS = 2 };
 /* BMG_FP64: every pass in FP64 (the reference's arithmetic).
  * BMG_MIXED: FP32 cycles on the defect equation inside FP64 defect
  * correction; FP64 residual, update and coarsest direct solve. */

@@ -190,3 +190,57 @@ def test_coarse_smooth_mode(gpu):
     x, st = ctx.solve(prob.b.copy())
     assert st.iters == it
     assert_same(x, o.field(oracle.X, 0), "solution")
+
+
+@pytest.mark.parametrize("native", [False, True], ids=["stencil", "native"])
+@pytest.mark.parametrize("case", ["neumann16", "channel16", "spheres16",
+                                  "mixed16", "thin1", "thin2", "channel48",
+                                  "spheres48"])
+def test_lexgs_bitwise(gpu, case, native):
+    """Lexicographic Gauss-Seidel (one launch per hyperplane of the swept
+    region) against the oracle's sequential sweep: two sweeps on every
+    level, then a whole solve with its iteration count."""
+    prob = CASES[case]()
+    print(f"[lexgs] case={case} native={native}", flush=True)
+    opts = dict(OPTS, smoother=oracle.LEXGS, rel_tol=1e-8, max_iter=200)
+    o, ctx = build_pair(gpu, prob, native, **opts)
+    o.pin_and_eliminate()
+    o.prepare_coarse()
+    ctx.prepare()
+    rng = np.random.default_rng(13)
+    for l in range(o.nlevels):
+        shp = o.padded(l)
+        x = rng.uniform(-1, 1, shp)
+        b = rng.uniform(-1, 1, shp)
+        for which, v in ((oracle.X, x), (oracle.B, b)):
+            o.set_field(which, l, v)
+            ctx.upload(which, l, v)
+        o.smooth(l, 2)
+        ctx.smooth(l, 2)
+        assert_same(ctx.download(0, l), o.field(oracle.X, l), f"lexgs l={l}")
+    o2, ctx2 = build_pair(gpu, prob, native, **opts)
+    o2.set_field(oracle.B, 0, prob.b)
+    it, r0, r1 = o2.solve()
+    x, st = ctx2.solve(prob.b.copy())
+    assert st.iters == it
+    assert_same(x, o2.field(oracle.X, 0), "solution")
+    ctx.close()
+    ctx2.close()
+
+
+@pytest.mark.parametrize("krylov", [0, 1], ids=["mg", "fcg"])
+def test_lexgs_mixed_precision_converges(gpu, krylov):
+    prob = CASES["spheres48"]()
+    base = dict(OPTS, smoother=oracle.LEXGS, rel_tol=1e-9, max_iter=200)
+    ref = gpu.Context(prob.vertices, prob.bc, prob.bc_value, **base)
+    ref.set_laplacian(prob.lam)
+    xr, _ = ref.solve(prob.b.copy())
+    ctx = gpu.Context(prob.vertices, prob.bc, prob.bc_value,
+                      precision=gpu.MIXED, krylov=krylov, **base)
+    ctx.set_laplacian(prob.lam)
+    x, st = ctx.solve(prob.b.copy())
+    assert st.final_residual <= 1e-9 * st.initial_residual
+    err = np.abs(inner(x - xr)).max() / np.abs(inner(xr)).max()
+    assert err < 1e-6, err
+    ctx.close()
+    ref.close()

@@ -701,3 +701,47 @@ def test_ddt_zero_dt_is_plain_laplacian():
     b.assemble(p.lam, ddt=(0.0, 1.0))
     for l in range(a.nlevels):
         assert np.array_equal(a.stencil(l), b.stencil(l))
+
+
+def test_lexgs_is_textbook_symmetric_gauss_seidel():
+    """lexgsSmoother.C:14-124 on a mesh without periodic faces (the whole
+    interior is one region): one sweep is a forward Gauss-Seidel pass in
+    (i, j, k) order, k fastest, then a backward pass in the reverse order,
+    on the eliminated system written as a dense matrix."""
+    v = [np.linspace(0, 1, 4), stretched(3), np.linspace(0, 1, 5)]
+    bc = (D, N, N, D, N, D)
+    o = make(v, bc, smoother=oracle.LEXGS)
+    o.pin_and_eliminate()
+    A7 = o.stencil(0)
+    n = o.shapes[0]
+    cells = [(i, j, k) for i in range(1, n[0] + 1) for j in range(1, n[1] + 1)
+             for k in range(1, n[2] + 1)]
+    pos = {c: e for e, c in enumerate(cells)}
+    M = np.zeros((len(cells), len(cells)))
+    offs = [(0, 0, 0), (-1, 0, 0), (1, 0, 0), (0, -1, 0), (0, 1, 0), (0, 0, -1), (0, 0, 1)]
+    for c, e in pos.items():
+        for s, (di, dj, dk) in enumerate(offs):
+            nb = (c[0] + di, c[1] + dj, c[2] + dk)
+            if A7[c + (s,)] != 0.0:
+                assert nb in pos
+                M[e, pos[nb]] += A7[c + (s,)]
+    rng = np.random.default_rng(12)
+    x = rng.uniform(-1, 1, o.padded(0))
+    b = rng.uniform(-1, 1, o.padded(0))
+    o.set_field(oracle.X, 0, x)
+    o.set_field(oracle.B, 0, b)
+    o.smooth(0, 1)
+    got = np.array([o.field(oracle.X, 0)[c] for c in cells])
+    y = np.array([x[c] for c in cells])
+    bb = np.array([b[c] for c in cells])
+    order = list(range(len(cells)))
+    for seq in (order, order[::-1]):
+        for e in seq:
+            y[e] = (bb[e] - (M[e] @ y - M[e, e] * y[e])) / M[e, e]
+    assert np.allclose(got, y, rtol=1e-12, atol=1e-14)
+    # the order matters: red-black ordering gives a different result
+    o.set_opts(smoother=oracle.RBGS)
+    o.set_field(oracle.X, 0, x)
+    o.smooth(0, 1)
+    rb = np.array([o.field(oracle.X, 0)[c] for c in cells])
+    assert np.abs(rb - y).max() > 1e-6
