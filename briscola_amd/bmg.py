@@ -36,7 +36,7 @@ EXPORTS = [
     "bmg_batch_destroy", "bmg_solve_batch", "bmg_batch_info",
     "bmg_batch_set_autotune", "bmg_face_flux", "bmg_face_flux_device",
     "bmg_fft_create", "bmg_fft_destroy", "bmg_fft_solve",
-    "bmg_fft_solve_device",
+    "bmg_fft_solve_device", "bmg_fft_solve_ddt", "bmg_fft_solve_ddt_device",
 ]
 
 
@@ -133,6 +133,8 @@ def load():
         "bmg_fft_destroy": (I, [P]),
         "bmg_fft_solve": (I, [P, DP, DP]),
         "bmg_fft_solve_device": (I, [P, P, P]),
+        "bmg_fft_solve_ddt": (I, [P, DP, DP, DP, D, D]),
+        "bmg_fft_solve_ddt_device": (I, [P, P, P, P, D, D]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -415,19 +417,31 @@ class FFT:
         except Exception:
             pass
 
-    def solve(self, b, x=None):
+    def solve(self, b, x=None, ddt=None, x_old=None):
         """Host arrays (reference layout); x (initial ghosts) is updated in
-        place and returned."""
+        place and returned.  ddt = (delta_t, dt_frac) with x_old adds the
+        implicit time term."""
         if x is None:
             x = np.zeros(self.shape)
-        for a in (b, x):
+        for a in (b, x, x_old):
             if a is not None and (a.dtype != np.float64 or not a.flags.c_contiguous
                                   or a.shape != self.shape):
-                raise ValueError("b and x must be C-contiguous float64 padded arrays")
-        _check(self.lib.bmg_fft_solve(self.ptr, _dp(b) if b is not None else None,
-                                      _dp(x)))
+                raise ValueError("b, x and x_old must be C-contiguous float64 "
+                                 "padded arrays")
+        bp = _dp(b) if b is not None else None
+        if ddt is None:
+            _check(self.lib.bmg_fft_solve(self.ptr, bp, _dp(x)))
+        else:
+            _check(self.lib.bmg_fft_solve_ddt(
+                self.ptr, bp, _dp(x), _dp(x_old) if x_old is not None else None,
+                float(ddt[0]), float(ddt[1])))
         return x
 
-    def solve_device(self, b_ptr, x_ptr):
-        _check(self.lib.bmg_fft_solve_device(self.ptr, ctypes.c_void_p(b_ptr),
-                                             ctypes.c_void_p(x_ptr)))
+    def solve_device(self, b_ptr, x_ptr, ddt=None, x_old_ptr=None):
+        if ddt is None:
+            _check(self.lib.bmg_fft_solve_device(self.ptr, ctypes.c_void_p(b_ptr),
+                                                 ctypes.c_void_p(x_ptr)))
+        else:
+            _check(self.lib.bmg_fft_solve_ddt_device(
+                self.ptr, ctypes.c_void_p(b_ptr), ctypes.c_void_p(x_ptr),
+                ctypes.c_void_p(x_old_ptr), float(ddt[0]), float(ddt[1])))

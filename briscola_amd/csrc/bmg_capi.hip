@@ -1977,6 +1977,33 @@ int bmg_fft_solve_device(bmg_fft* f, const double* db, double* dx)
     return run([&] { fft_solve_device(f, db, dx); });
 }
 
+static double fftTimeStep(const double* xold, double delta_t, double dt_frac)
+{
+    if (!xold) throw Error(BMG_ERR_ARG, "the time term needs x_old");
+    const double dT = delta_t*dt_frac;
+    if (!(std::isfinite(delta_t) && std::isfinite(dt_frac) && dT > 0.0))
+        throw Error(BMG_ERR_ARG, "the time term needs delta_t*dt_frac > 0");
+    return dT;
+}
+
+int bmg_fft_solve_ddt(bmg_fft* f, const double* b, double* x,
+                      const double* x_old, double delta_t, double dt_frac)
+{
+    return run([&] {
+        fft_solve(f, b, x, x_old, fftTimeStep(x_old, delta_t, dt_frac));
+    });
+}
+
+int bmg_fft_solve_ddt_device(bmg_fft* f, const double* db, double* dx,
+                             const double* dx_old, double delta_t,
+                             double dt_frac)
+{
+    return run([&] {
+        fft_solve_device(f, db, dx, dx_old,
+                         fftTimeStep(dx_old, delta_t, dt_frac));
+    });
+}
+
 int bmg_batch_create(bmg_batch** out, bmg_ctx* const* ctxs, int n)
 {
     return run([&] { batch_create(out, ctxs, n); });

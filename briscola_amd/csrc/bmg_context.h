@@ -396,7 +396,9 @@ namespace bmg {
 // FFT Poisson solver (bmg_fft.hip); throw Error on failure
 void fft_create(bmg_fft** out, const bmg_grid* grid);
 void fft_destroy(bmg_fft* f);
-void fft_solve(bmg_fft* f, const double* b, double* x);
-void fft_solve_device(bmg_fft* f, const double* db, double* dx);
+void fft_solve(bmg_fft* f, const double* b, double* x,
+               const double* xold = nullptr, double dT = 0.0);
+void fft_solve_device(bmg_fft* f, const double* db, double* dx,
+                      const double* dxold = nullptr, double dT = 0.0);
 
 }  // namespace bmg

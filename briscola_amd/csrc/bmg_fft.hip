@@ -55,6 +55,7 @@ struct FftGeom {
     double bcval[6];
     double hwall[6];        // width of the boundary cell of each face
     double d1sqr, d2sqr;
+    double dT;              // deltaT*dtFrac of the time term; 0: none
 };
 
 struct C2 {
@@ -109,7 +110,7 @@ __device__ __forceinline__ int extension(int code, int n, int a, int p[4],
 
 // Step 1: interior cell t of the reference layout.
 __global__ void k_fft_rhs(FftGeom g, const double* b, const double* x,
-                          C2* E)
+                          const double* xold, C2* E)
 {
     const long t = (long)blockIdx.x*blockDim.x + threadIdx.x;
     if (t >= (long)g.N[0]*g.N[1]*g.N[2]) return;
@@ -119,6 +120,8 @@ __global__ void k_fft_rhs(FftGeom g, const double* b, const double* x,
     q[2] = int(t % g.N[2]);
     const long c = refIndex(g, q[0], q[1], q[2]);
     double f = b ? -b[c] : 0.0;
+    // ddt: minus x.oldTime()/(deltaT*dtFrac) (FFTPoissonSolver.C:116-131)
+    if (xold) f = f - xold[c]/g.dT;
     const long stride[3] = {(long)g.P[1]*g.P[2], (long)g.P[2], 1};
     for (int face = 0; face < 6; face++)
     {
@@ -229,6 +232,12 @@ __global__ void k_fft_tridiag(FftGeom g, const C2* E, C2* X, double* Dh,
             return Lm - east - west;
         }
     };
+    // ddt: every diagonal minus 1/(deltaT*dtFrac) (tridiagonalSolver.C:352,
+    // :430), the pinned zero-mode row included
+    auto diagT = [&](int k) -> double {
+        const double d = diag(k);
+        return g.dT != 0.0 ? d - 1.0/g.dT : d;
+    };
     auto DU = [&](int k) { return 1.0/(h[k]*de[k]); };
     auto DL = [&](int k) { return 1.0/(h[k]*dw[k]); };
     const long plane = (long)g.L1*g.L2;
@@ -240,7 +249,7 @@ __global__ void k_fft_tridiag(FftGeom g, const C2* E, C2* X, double* Dh,
     {
         const C2 y = E[k*plane + off];
         X[k*nm + t] = cmul(w, y);
-        Dh[k*nm + t] = diag(k);
+        Dh[k*nm + t] = diagT(k);
         if (cyc) V[k*nm + t] = k == 1 ? -DL(1) : (k == last ? -DU(last) : 0.0);
     }
     // forward elimination (tridiagonalSolver.C:340-346)
@@ -281,7 +290,7 @@ __global__ void k_fft_tridiag(FftGeom g, const C2* E, C2* X, double* Dh,
         const C2 f0 = cmul(w, E[off]);
         const C2 uN = X[last*nm + t], u1 = X[nm + t];
         const double vN = V[last*nm + t], v1 = V[nm + t];
-        const double den = (diag(0) + DL(0)*vN) + DU(0)*v1;
+        const double den = (diagT(0) + DL(0)*vN) + DU(0)*v1;
         C2 p0;
         p0.x = ((f0.x - DL(0)*uN.x) - DU(0)*u1.x)/den;
         p0.y = ((f0.y - DL(0)*uN.y) - DU(0)*u1.y)/den;
@@ -569,11 +578,17 @@ void fft_create(bmg_fft** out, const bmg_grid* grid)
 void fft_destroy(bmg_fft* F) { delete F; }
 
 // FFTPoissonSolver::solve (FFTPoissonSolver.C:97-465) on device arrays in
-// the reference layout; b may be null (b = 0).
-void fft_solve_device(bmg_fft* F, const double* db, double* dx)
+// the reference layout; b may be null (b = 0).  With dxold the time term:
+// the source minus x_old/dT and every diagonal minus 1/dT, dT =
+// deltaT*dtFrac (FFTPoissonSolver.C:116-131, tridiagonalSolver.C:334-352).
+void fft_solve_device(bmg_fft* F, const double* db, double* dx,
+                      const double* dxold, double dT)
 {
     if (!F || !dx) throw Error(BMG_ERR_ARG, "null pointer");
-    const FftGeom& g = F->g;
+    if (dxold && !(std::isfinite(dT) && dT > 0.0))
+        throw Error(BMG_ERR_ARG, "time term needs deltaT*dtFrac > 0");
+    FftGeom g = F->g;
+    g.dT = dxold ? dT : 0.0;
     const long cells = long(g.N[0])*g.N[1]*g.N[2];
     const long padded = long(g.P[0])*g.P[1]*g.P[2];
     const long nE = long(g.ns)*g.L1*g.L2;
@@ -581,7 +596,7 @@ void fft_solve_device(bmg_fft* F, const double* db, double* dx)
     hipStream_t s = F->stream;
     auto blocks = [](long n) { return dim3(unsigned((n + 255)/256)); };
     hipLaunchKernelGGL(k_fft_rhs, blocks(cells), dim3(256), 0, s, g, db, dx,
-                       F->E);
+                       dxold, F->E);
     HIPCK(hipGetLastError());
     void* buf[1] = {F->E};
     rocfftCheck(rocfft_execute(F->fwd, buf, nullptr, F->info), "forward");
@@ -599,12 +614,14 @@ void fft_solve_device(bmg_fft* F, const double* db, double* dx)
     HIPCK(hipStreamSynchronize(s));
 }
 
-void fft_solve(bmg_fft* F, const double* b, double* x)
+void fft_solve(bmg_fft* F, const double* b, double* x, const double* xold,
+               double dT)
 {
     if (!F || !x) throw Error(BMG_ERR_ARG, "null pointer");
     const FftGeom& g = F->g;
     const size_t n = size_t(g.P[0])*g.P[1]*g.P[2];
     double* db = nullptr;
+    double* dold = nullptr;
     double* dx = devAlloc<double>(n);
     try
     {
@@ -613,17 +630,25 @@ void fft_solve(bmg_fft* F, const double* b, double* x)
             db = devAlloc<double>(n);
             HIPCK(hipMemcpy(db, b, n*sizeof(double), hipMemcpyHostToDevice));
         }
+        if (xold)
+        {
+            dold = devAlloc<double>(n);
+            HIPCK(hipMemcpy(dold, xold, n*sizeof(double),
+                            hipMemcpyHostToDevice));
+        }
         HIPCK(hipMemcpy(dx, x, n*sizeof(double), hipMemcpyHostToDevice));
-        fft_solve_device(F, db, dx);
+        fft_solve_device(F, db, dx, dold, dT);
         HIPCK(hipMemcpy(x, dx, n*sizeof(double), hipMemcpyDeviceToHost));
     }
     catch (...)
     {
         devFree(db);
+        devFree(dold);
         devFree(dx);
         throw;
     }
     devFree(db);
+    devFree(dold);
     devFree(dx);
 }
 

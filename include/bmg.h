@@ -224,6 +224,18 @@ int bmg_fft_create(bmg_fft** fft, const bmg_grid* grid);
 int bmg_fft_destroy(bmg_fft* fft);
 int bmg_fft_solve(bmg_fft* fft, const double* b, double* x);
 int bmg_fft_solve_device(bmg_fft* fft, const double* d_b, double* d_x);
+/* The same with the implicit time term, as the reference's solve(x, b,
+ * nullptr, ddt = true, dtFrac) (FFTPoissonSolver.C:116-131,
+ * tridiagonalSolver.C:334-352, :430): with dT = delta_t*dt_frac the source
+ * becomes -b - x_old/dT and every diagonal of the tridiagonal systems is
+ * lowered by 1/dT (the pinned zero-mode row of an all Neumann/periodic
+ * mesh included, as the reference does).  x_old: the old-time field,
+ * reference layout.  BMG_ERR_ARG without x_old or with dT <= 0. */
+int bmg_fft_solve_ddt(bmg_fft* fft, const double* b, double* x,
+                      const double* x_old, double delta_t, double dt_frac);
+int bmg_fft_solve_ddt_device(bmg_fft* fft, const double* d_b, double* d_x,
+                             const double* d_x_old, double delta_t,
+                             double dt_frac);
 
 /* Flux of a level-0 field through every face, as PoissonSolver computes it
  * after a solve (defaultPoissonSolver.C:108-121): for axis d and padded cell

@@ -95,10 +95,13 @@ def _widths_with_ghosts(v, periodic):
     return np.concatenate([[lo], h, [hi]])   # index k+1 holds h_k
 
 
-def rhs(b, x, bc, vertices):
-    """-b minus the boundary terms, interior cells (FFTPoissonSolver.C:
-    123-180), faces in order -x,+x,-y,+y,-z,+z."""
+def rhs(b, x, bc, vertices, x_old=None, dT=None):
+    """-b (minus x_old/dT with the time term, FFTPoissonSolver.C:116-131)
+    minus the boundary terms, interior cells (FFTPoissonSolver.C:133-180),
+    faces in order -x,+x,-y,+y,-z,+z."""
     f = -np.asarray(b, np.float64)[1:-1, 1:-1, 1:-1].copy()
+    if x_old is not None:
+        f = f - np.asarray(x_old, np.float64)[1:-1, 1:-1, 1:-1] / dT
     n = f.shape
     for face in range(6):
         d, side = divmod(face, 2)
@@ -121,13 +124,18 @@ def rhs(b, x, bc, vertices):
     return f
 
 
-def solve(b, x, bc, vertices):
+def solve(b, x, bc, vertices, ddt=None, x_old=None):
     """Interior solution of the reference FFT solver; b and x ghost-padded
-    (x supplies the boundary ghost values)."""
+    (x supplies the boundary ghost values).  ddt = (delta_t, dt_frac) with
+    x_old adds the implicit time term with dT = delta_t*dt_frac: the source
+    minus x_old/dT and every diagonal, the pinned zero-mode row included,
+    minus 1/dT (tridiagonalSolver.C:334-352, :430)."""
     s, d1, d2 = plan(vertices)
     n = [len(v) - 1 for v in vertices]
     codes = [bc_code(bc[2 * d], bc[2 * d + 1]) for d in range(3)]
-    f = rhs(b, x, bc, vertices).astype(np.complex128)
+    dT = ddt[0] * ddt[1] if ddt is not None else None
+    f = rhs(b, x, bc, vertices, x_old if ddt is not None else None,
+            dT).astype(np.complex128)
     f = _fwd(f, codes[d1], d1)
     f = _fwd(f, codes[d2], d2)
     lam1 = eigenvalues(codes[d1], n[d1])
@@ -167,6 +175,8 @@ def solve(b, x, bc, vertices):
     else:
         D[:, :, 0] = np.where(zero, pin, L - 1.0 / (d3e[0] * d3[0]) - 1.0 / (d3[0] * d3w[0]))
         D[:, :, last] = L - 1.0 / (d3e[last] * d3[last]) - 1.0 / (d3[last] * d3w[last])
+    if ddt is not None:
+        D = D - 1.0 / dT
 
     def thomas(fr, start):
         Dh = D.copy()

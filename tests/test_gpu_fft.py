@@ -7,7 +7,9 @@ and the ghost refresh after the solve.  FFT rounding differs from the
 oracle's, so the bar is 1e-11 of the solution's magnitude.  Also: mixed
 Dirichlet/Neumann pairs on the transformed axes (quarter-wave transforms),
 on the channel the FFT solution equals the multigrid's to its tolerance, and
-a mesh with two non-uniform directions is refused."""
+a mesh with two non-uniform directions is refused.  With the implicit time
+term every pair matches the oracle (pinned zero modes included) and
+recovers the manufactured field where no zero mode is pinned."""
 import itertools
 
 import numpy as np
@@ -156,3 +158,39 @@ def test_fft_mixed_pair_on_the_tridiagonal_axis(gpu, pair):
     fft = bmg.FFT(verts, bc, vals)
     xg = fft.solve(b.copy(), x.copy())
     assert rel(xg[1:-1, 1:-1, 1:-1], ref, False) < TOL
+
+
+@pytest.mark.parametrize("stretch", [None, 2])
+@pytest.mark.parametrize("kinds", list(itertools.product(KINDS, repeat=3)),
+                         ids=lambda k: "".join("NDP"[q] for q in k))
+def test_fft_ddt_matches_oracle(gpu, stretch, kinds):
+    """(laplacian - 1/dT) x = -b - x_old/dT, dT = deltaT*dtFrac, host and
+    device entry points."""
+    from test_fft_oracle import manufactured_ddt, _pinned
+    torch = pytest.importorskip("torch")
+    n = (16, 12, 10)
+    verts = [np.linspace(0.0, 1.0 + d, q + 1) for d, q in enumerate(n)]
+    if stretch is not None:
+        t = np.linspace(-1.0, 1.0, n[stretch] + 1)
+        verts[stretch] = np.tanh(1.5 * t) / np.tanh(1.5)
+    bc = [k for k in kinds for _ in (0, 1)]
+    dt, frac = 4e-3, 0.5
+    print(f"[fft ddt] kinds={kinds} stretch={stretch}", flush=True)
+    x, b, x_old = manufactured_ddt(n, bc, verts, dt * frac, seed=sum(kinds) + 5)
+    ref = F.solve(b, x, bc, verts, ddt=(dt, frac), x_old=x_old)
+    fft = bmg.FFT(verts, bc)
+    xg = x.copy()
+    fft.solve(b.copy(), xg, ddt=(dt, frac), x_old=x_old)
+    assert rel(xg[1:-1, 1:-1, 1:-1], ref, False) < TOL
+    if not _pinned(bc, verts):
+        assert rel(xg[1:-1, 1:-1, 1:-1], x[1:-1, 1:-1, 1:-1], False) < 1e-10
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")
+    db, dx, dold = dev(b), dev(x), dev(x_old)
+    fft.solve_device(db.data_ptr(), dx.data_ptr(), ddt=(dt, frac),
+                     x_old_ptr=dold.data_ptr())
+    assert np.array_equal(dx.cpu().numpy(), xg)
+    with pytest.raises(bmg.BmgError):
+        fft.solve(b.copy(), x.copy(), ddt=(dt, frac), x_old=None)
+    with pytest.raises(bmg.BmgError):
+        fft.solve(b.copy(), x.copy(), ddt=(0.0, frac), x_old=x_old)
+    fft.close()
