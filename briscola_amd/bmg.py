@@ -37,6 +37,7 @@ EXPORTS = [
     "bmg_batch_set_autotune", "bmg_face_flux", "bmg_face_flux_device",
     "bmg_fft_create", "bmg_fft_destroy", "bmg_fft_solve",
     "bmg_fft_solve_device", "bmg_fft_solve_ddt", "bmg_fft_solve_ddt_device",
+    "bmg_split_source", "bmg_split_source_device",
 ]
 
 
@@ -134,6 +135,8 @@ def load():
         "bmg_fft_solve": (I, [P, DP, DP]),
         "bmg_fft_solve_device": (I, [P, P, P]),
         "bmg_fft_solve_ddt": (I, [P, DP, DP, DP, D, D]),
+        "bmg_split_source": (I, [P, DP, DP, DP, DP, D, DP, DP, DP, DP, DP]),
+        "bmg_split_source_device": (I, [P, P, P, P, P, D, P, P, P, P, P]),
         "bmg_fft_solve_ddt_device": (I, [P, P, P, P, D, D]),
     }
     for name, (res, args) in sig.items():
@@ -307,6 +310,22 @@ class Context:
         _check(self.lib.bmg_time_smoother(self.ptr, level, precision, reps,
                                           ctypes.byref(ms), ctypes.byref(by)))
         return ms.value, by.value
+
+    def split_source(self, x, lam, lambda0, b=None):
+        """splitPoissonSolver's correction source on host arrays: returns
+        (bHat, [phi_x, phi_y, phi_z])."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in [x] + list(lam)]
+        bb = np.ascontiguousarray(b, dtype=np.float64) if b is not None else None
+        for a in arrs + ([bb] if bb is not None else []):
+            if a.shape != self.padded(0):
+                raise ValueError("fields must have the padded level-0 shape")
+        bhat = np.zeros(self.padded(0))
+        phi = [np.zeros(self.padded(0)) for _ in range(3)]
+        _check(self.lib.bmg_split_source(
+            self.ptr, *[_dp(a) for a in arrs], float(lambda0),
+            _dp(bb) if bb is not None else None, _dp(bhat),
+            *[_dp(p) for p in phi]))
+        return bhat, phi
 
     def face_flux(self, x, lam=None):
         """PoissonSolver flux of a level-0 field x (host array, reference

@@ -26,7 +26,7 @@ FLAGS = [
 
 
 CPP_TESTS = [os.path.join(ROOT, "tests", "cpp", t)
-             for t in ("test_hipmg", "test_poisson")]
+             for t in ("test_hipmg", "test_poisson", "test_split")]
 
 
 def build(verbose=False):

@@ -2087,6 +2087,81 @@ static void faceFlux(bmg_ctx* c, const double* x, const double* const lam[3],
     HIPCK(hipStreamSynchronize(c->stream));
 }
 
+// splitPoissonSolver's correction source (splitPoissonSolver.C:124-137) on
+// level 0: phi per face direction, then bHat.
+static void splitSource(bmg_ctx* c, const double* x, const double* const lam[3],
+                        double lam0, const double* b, double* const phi[3],
+                        double* bhat)
+{
+    if (!(std::isfinite(lam0) && lam0 > 0.0))
+        throw Error(BMG_ERR_ARG, "lambda0 must be finite and > 0");
+    Level& q = c->lv[0];
+    const double* w[3] = {q.dw[0], q.dw[1], q.dw[2]};
+    const double* dl[3] = {q.ddl[0], q.ddl[1], q.ddl[2]};
+    launch_split_source(q.L, x, lam, lam0, b, w, dl, phi, bhat, c->stream);
+    HIPCK(hipGetLastError());
+    HIPCK(hipStreamSynchronize(c->stream));
+}
+
+int bmg_split_source_device(bmg_ctx* c, const double* dx, const double* dlx,
+                            const double* dly, const double* dlz,
+                            double lambda0, const double* db, double* dbhat,
+                            double* dphix, double* dphiy, double* dphiz)
+{
+    return run([&] {
+        if (!c || !dx || !dlx || !dly || !dlz || !dbhat || !dphix || !dphiy ||
+            !dphiz)
+            throw Error(BMG_ERR_ARG, "null pointer");
+        const double* lam[3] = {dlx, dly, dlz};
+        double* const phi[3] = {dphix, dphiy, dphiz};
+        splitSource(c, dx, lam, lambda0, db, phi, dbhat);
+    });
+}
+
+int bmg_split_source(bmg_ctx* c, const double* x, const double* lx,
+                     const double* ly, const double* lz, double lambda0,
+                     const double* b, double* bhat, double* phix, double* phiy,
+                     double* phiz)
+{
+    return run([&] {
+        if (!c || !x || !lx || !ly || !lz || !bhat || !phix || !phiy || !phiz)
+            throw Error(BMG_ERR_ARG, "null pointer");
+        Level& q = c->lv[0];
+        const size_t n = size_t(q.L.P[0])*q.L.P[1]*q.L.P[2];
+        std::vector<double*> bufs;
+        auto up = [&](const double* h) -> double* {
+            double* d = devAlloc<double>(n);
+            bufs.push_back(d);
+            if (h)
+                HIPCK(hipMemcpy(d, h, n*sizeof(double), hipMemcpyHostToDevice));
+            else
+                HIPCK(hipMemset(d, 0, n*sizeof(double)));
+            return d;
+        };
+        try
+        {
+            const double* dx = up(x);
+            const double* lam[3] = {up(lx), up(ly), up(lz)};
+            const double* db = b ? up(b) : nullptr;
+            double* dbh = up(bhat);
+            double* const phi[3] = {up(phix), up(phiy), up(phiz)};
+            splitSource(c, dx, lam, lambda0, db, phi, dbh);
+            HIPCK(hipMemcpy(bhat, dbh, n*sizeof(double),
+                            hipMemcpyDeviceToHost));
+            double* const out[3] = {phix, phiy, phiz};
+            for (int d = 0; d < 3; d++)
+                HIPCK(hipMemcpy(out[d], phi[d], n*sizeof(double),
+                                hipMemcpyDeviceToHost));
+        }
+        catch (...)
+        {
+            for (auto& p : bufs) devFree(p);
+            throw;
+        }
+        for (auto& p : bufs) devFree(p);
+    });
+}
+
 int bmg_face_flux_device(bmg_ctx* c, const double* dx, const double* dlx,
                          const double* dly, const double* dlz, double* dfx,
                          double* dfy, double* dfz)

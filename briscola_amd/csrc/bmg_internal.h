@@ -205,6 +205,11 @@ void launch_lambda_restrict(const LevelDims& Lf, const LevelDims& Lc,
                             const double* fw0, const double* fw1,
                             const double* fw2, const double* lamf,
                             double* lamc, int fd, hipStream_t s);
+void launch_split_source(const LevelDims& L, const double* x,
+                         const double* const lam[3], double lam0,
+                         const double* b, const double* const w[3],
+                         const double* const dl[3], double* const phi[3],
+                         double* bhat, hipStream_t s);
 void launch_ddt_diagonal(const LevelDims& L, const double* w0,
                          const double* w1, const double* w2, double delta_t,
                          double dt_frac, Pair<double> dg, hipStream_t s);

@@ -2097,6 +2097,62 @@ __global__ void k_face_flux(LevelDims L, const double* x, const double* lam,
     flux[c] = v*fa;
 }
 
+// Split correction flux of splitPoissonSolver (splitPoissonSolver.C:
+// 132-135): phi = ((lambda/lambda0 - 1)*g)*A on the lower faces normal to
+// fd, faces 1..N+1 along fd, with the face gradient g = (x_lower - x)*delta
+// and area A of k_face_flux.
+__global__ void k_split_phi(LevelDims L, const double* x, const double* lam,
+                            double lam0, const double* w0, const double* w1,
+                            const double* w2, const double* dl, int fd,
+                            double* phi)
+{
+    const int lim0 = L.N[0] + (fd == 0), lim1 = L.N[1] + (fd == 1),
+              lim2 = L.N[2] + (fd == 2);
+    const long t = (long)blockIdx.x*blockDim.x + threadIdx.x;
+    if (t >= (long)lim0*lim1*lim2) return;
+    const int I = 1 + int(t/((long)lim1*lim2));
+    const int J = 1 + int((t/lim2) % lim1);
+    const int K = 1 + int(t % lim2);
+    const long c = ((long)I*L.P[1] + J)*L.P[2] + K;
+    const long o = fd == 0 ? (long)L.P[1]*L.P[2] : (fd == 1 ? L.P[2] : 1);
+    const int pc = fd == 0 ? I : (fd == 1 ? J : K);
+    const double d = x[c - o] - x[c];
+    const double g = d*dl[pc];
+    const double fa = fd == 0 ? w1[J]*w2[K] : (fd == 1 ? w0[I]*w2[K]
+                                                        : w0[I]*w1[J]);
+    const double r = lam[c]/lam0 - 1.0;
+    phi[c] = (r*g)*fa;
+}
+
+// Modified source of splitPoissonSolver (splitPoissonSolver.C:124-137):
+// bHat = b/lambda0 + div(phi), div as divergenceScheme.C:66-95 accumulates
+// it -- faces by direction, each cell's lower face added and its upper face
+// subtracted -- times the inverse volume 1/((dx*dy)*dz).  Interior cells.
+__global__ void k_split_div(LevelDims L, const double* b, double lam0,
+                            const double* phi0, const double* phi1,
+                            const double* phi2, const double* w0,
+                            const double* w1, const double* w2, double* bhat)
+{
+    const long t = (long)blockIdx.x*blockDim.x + threadIdx.x;
+    if (t >= (long)L.N[0]*L.N[1]*L.N[2]) return;
+    const int I = 1 + int(t/((long)L.N[1]*L.N[2]));
+    const int J = 1 + int((t/L.N[2]) % L.N[1]);
+    const int K = 1 + int(t % L.N[2]);
+    const long c = ((long)I*L.P[1] + J)*L.P[2] + K;
+    const long oI = (long)L.P[1]*L.P[2], oJ = L.P[2];
+    double acc = 0.0;
+    acc = acc + phi0[c];
+    acc = acc - phi0[c + oI];
+    acc = acc + phi1[c];
+    acc = acc - phi1[c + oJ];
+    acc = acc + phi2[c];
+    acc = acc - phi2[c + 1];
+    const double cv = (w0[I]*w1[J])*w2[K];
+    const double icv = 1.0/cv;
+    const double base = b ? b[c]/lam0 : 0.0;
+    bhat[c] = base + acc*icv;
+}
+
 // ---------------------------------------------------------------------------
 // Coarsest-level direct solve, one workgroup (APLU.C:91-128 with rhsSource,
 // linearSystemAggregation.C:336-430).  The permutation is applied first and
@@ -3997,6 +4053,26 @@ void launch_ddt_diagonal(const LevelDims& L, const double* w0,
     const long n = long(L.N[0])*L.N[1]*L.N[2];
     hipLaunchKernelGGL(k_ddt_diagonal, dim3((n + 255)/256), dim3(256), 0, s, L,
                        w0, w1, w2, delta_t, dt_frac, dg);
+}
+
+void launch_split_source(const LevelDims& L, const double* x,
+                         const double* const lam[3], double lam0,
+                         const double* b, const double* const w[3],
+                         const double* const dl[3], double* const phi[3],
+                         double* bhat, hipStream_t s)
+{
+    direct_only("split source");
+    for (int fd = 0; fd < 3; fd++)
+    {
+        const long n = long(L.N[0] + (fd == 0))*(L.N[1] + (fd == 1))*
+                       (L.N[2] + (fd == 2));
+        hipLaunchKernelGGL(k_split_phi, dim3((n + 255)/256), dim3(256), 0, s,
+                           L, x, lam[fd], lam0, w[0], w[1], w[2], dl[fd], fd,
+                           phi[fd]);
+    }
+    const long n = long(L.N[0])*L.N[1]*L.N[2];
+    hipLaunchKernelGGL(k_split_div, dim3((n + 255)/256), dim3(256), 0, s, L, b,
+                       lam0, phi[0], phi[1], phi[2], w[0], w[1], w[2], bhat);
 }
 
 void launch_face_flux(const LevelDims& L, const double* x, const double* lam,

@@ -85,6 +85,7 @@ struct fvMesh
     double bcValue[6] = {0, 0, 0, 0, 0, 0};
     std::vector<double> cellVolumes;
     double deltaT = 0.0;       // time().deltaTValue() (ddtScheme.H:59)
+    double deltaT0 = 0.0;      // time().deltaT0Value(), the previous step
 };
 
 // linearSystem<stencil,scalar,colocated>: unknown x, deep stencil field A

@@ -237,6 +237,27 @@ int bmg_fft_solve_ddt_device(bmg_fft* fft, const double* d_b, double* d_x,
                              const double* d_x_old, double delta_t,
                              double dt_frac);
 
+/* Correction source of the split Poisson solver (splitPoissonSolver.C:
+ * 124-137) on level 0 of ctx's mesh: per face direction d, on the faces of
+ * bmg_face_flux's layout, phi_d = (((lambda_d/lambda0) - 1) * g) * A_d with
+ * the face gradient g = (x[c - e_d] - x[c]) * delta_d and face area A_d;
+ * then per interior cell bHat = b/lambda0 + div(phi), the divergence
+ * accumulated as divergenceScheme.C:66-95 does (lower face added, upper
+ * face subtracted, directions in order) times 1/((dx*dy)*dz).  b may be
+ * null (zero).  lambda0 > 0 (the reference takes the largest face
+ * coefficient).  phi and bHat are written on those faces and interior
+ * cells; their other entries keep what they held.  Replaces the source assembly of
+ * splitPoissonSolver::solve; the caller runs the inner solve on bHat. */
+int bmg_split_source(bmg_ctx* ctx, const double* x, const double* lambda_x,
+                     const double* lambda_y, const double* lambda_z,
+                     double lambda0, const double* b, double* bhat,
+                     double* phi_x, double* phi_y, double* phi_z);
+int bmg_split_source_device(bmg_ctx* ctx, const double* d_x,
+                            const double* d_lambda_x, const double* d_lambda_y,
+                            const double* d_lambda_z, double lambda0,
+                            const double* d_b, double* d_bhat, double* d_phi_x,
+                            double* d_phi_y, double* d_phi_z);
+
 /* Flux of a level-0 field through every face, as PoissonSolver computes it
  * after a solve (defaultPoissonSolver.C:108-121): for axis d and padded cell
  * c whose lower face lies in the face range (1 <= c_d <= N_d + 1, interior

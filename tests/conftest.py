@@ -15,6 +15,13 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def gpu():
+    # torch (device buffers of the device-pointer tests) ships its own HIP
+    # runtime under the same soname as ROCm's: load it before the solver
+    # library so that one runtime, torch's, serves both
+    try:
+        import torch
+    except ImportError:
+        torch = None
     from briscola_amd import bmg
     bmg.load()
     import ctypes
@@ -22,4 +29,6 @@ def gpu():
     code = bmg.load().bmg_device_count(ctypes.byref(n))
     if code != 0 or n.value < 1:
         pytest.fail("no HIP device visible")
+    if torch is not None and torch.cuda.is_available():
+        torch.zeros(1, device="cuda:0")
     return bmg
