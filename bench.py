@@ -189,7 +189,7 @@ def reference_solution(name, prob):
 
 
 def cpu_cores():
-    """Host cores this process may use (the GPU box's share when
+    """Host cores this process may use (the machine's share when
     OMP_NUM_THREADS states it)."""
     env = os.environ.get("OMP_NUM_THREADS")
     if env and env.isdigit() and int(env) > 0:
