@@ -30,8 +30,9 @@ over seeds 0-4.
 The roofline object covers the dominant kernel of the plain FP64 solve, the
 level-0 red-black smoother phase, timed live with HIP events on the library's
 stream right after the timed steps.  The CPU baseline times the CPU
-restatement of the reference multigrid (one core) on a bounded sample of the
-same solve and scales it by the iteration count.
+restatement of the reference multigrid on a bounded sample of the same solve,
+on the host cores this process may use and on one core, and scales it by the
+iteration count.
 
 The default run also reports the batched workload (``batch512``): 512
 independent systems of uneven shapes and boundary conditions solved
