@@ -190,3 +190,28 @@ def test_ddt_arguments_refused(gpu):
         with pytest.raises(bmg.BmgError):
             ctx.set_laplacian(prob.lam, ddt=bad)
     ctx.close()
+
+
+@pytest.mark.parametrize("cycle", [oracle.VCYCLE, oracle.WCYCLE, oracle.FCYCLE],
+                         ids=["V", "W", "F"])
+@pytest.mark.parametrize("smoother", [oracle.RBGS, oracle.JAC, oracle.LEXGS],
+                         ids=["rbgs", "jac", "lexgs"])
+@pytest.mark.parametrize("coarse_mode", [oracle.COARSE_DIRECT,
+                                         oracle.COARSE_SMOOTH],
+                         ids=["direct", "smooth"])
+def test_ddt_cycles_smoothers_coarse_modes(gpu, cycle, smoother, coarse_mode):
+    """The time-term operator through every cycle type, smoother and coarse
+    mode, whole solves bitwise against the oracle."""
+    prob = CASES["mixed16"]() if smoother != oracle.JAC else CASES["channel16"]()
+    print(f"[ddt variants] cycle={cycle} smoother={smoother} "
+          f"coarse={coarse_mode}", flush=True)
+    opts = dict(rel_tol=1e-8, tolerance=0.0, max_iter=150, n_sweeps_pre=1,
+                n_sweeps_post=2, cycle=cycle, coarse_mode=coarse_mode,
+                coarse_level=0, smoother=smoother)
+    o, ctx = build_pair(gpu, prob, STEPS["dt1e-2_rk3"], **opts)
+    o.set_field(oracle.B, 0, prob.b)
+    it, r0, r1 = o.solve()
+    x, st = ctx.solve(prob.b.copy())
+    assert st.iters == it
+    assert_same(x, o.field(oracle.X, 0), "solution")
+    ctx.close()
