@@ -5,8 +5,8 @@
 //
 //   test_split <dir>
 // reads  dir/shape.txt  "nx ny nz bc0..bc5 inner nCorr prediction ddt
-//                        deltaT deltaT0 dtFrac relTol"
-//        (inner: FFT or MG), dir/vertices.bin, dir/volumes.bin,
+//                        deltaT deltaT0 dtFrac relTol precond"
+//        (inner: FFT or MG; precond 1: a multigrid preconditioner solve), dir/vertices.bin, dir/volumes.bin,
 //        dir/lambda.bin (3 face arrays), dir/src.bin, dir/x0.bin,
 //        dir/xold.bin, dir/xoldold.bin (padded level-0 fields)
 // writes dir/x.bin, dir/flux.bin (3 arrays), dir/bhat.bin
@@ -41,12 +41,12 @@ int main(int argc, char** argv)
     std::ifstream sh(dir + "/shape.txt");
     fvMesh mesh;
     std::string inner, prediction, relTol;
-    int nCorr = 1, ddt = 0;
+    int nCorr = 1, ddt = 0, precond = 0;
     double dtFrac = 1.0;
     sh >> mesh.N[0] >> mesh.N[1] >> mesh.N[2];
     for (int f = 0; f < 6; f++) sh >> mesh.bc[f];
     sh >> inner >> nCorr >> prediction >> ddt >> mesh.deltaT >> mesh.deltaT0
-       >> dtFrac >> relTol;
+       >> dtFrac >> relTol >> precond;
     if (!sh) { std::fprintf(stderr, "bad shape.txt\n"); return 2; }
     const size_t nv = size_t(mesh.N[0] + 1) + mesh.N[1] + 1 + mesh.N[2] + 1;
     std::vector<double> v = readBin(dir + "/vertices.bin", nv);
@@ -74,7 +74,12 @@ int main(int argc, char** argv)
                           {"coarseLevel", "1"}};
     try
     {
-        splitPoissonSolver ps(dict, solverDict, mesh);
+        dictionary preDict{{"type", "MG"}, {"relTol", relTol},
+                           {"tolerance", "0"}, {"cycleType", "V"},
+                           {"nSweepsPre", "2"}, {"nSweepsPost", "2"},
+                           {"coarseLevel", "1"}};
+        splitPoissonSolver ps(dict, solverDict, mesh,
+                              precond ? &preDict : nullptr);
         ps.enableFlux();
         ps.solve(x, &b, &lambda, ddt != 0, dtFrac, &xOld, &xOldOld);
         writeBin(dir + "/x.bin", x);

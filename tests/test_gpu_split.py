@@ -82,13 +82,14 @@ def test_split_source_matches_restatement(gpu, case):
 
 
 def run_adapter(tmp_path, prob, lam, inner, ncorr, prediction, ddt, x0, xold,
-                xoldold, rel_tol="1e-10"):
+                xoldold, rel_tol="1e-10", precond=False):
     dt, dt0, frac = ddt if ddt is not None else (1e-3, 1e-3, 1.0)
     with open(tmp_path / "shape.txt", "w") as f:
         f.write(" ".join(str(q) for q in prob.n) + " ")
         f.write(" ".join(str(q) for q in prob.bc) +
                 f" {inner} {ncorr} {'true' if prediction else 'false'} "
-                f"{int(ddt is not None)} {dt!r} {dt0!r} {frac!r} {rel_tol}\n")
+                f"{int(ddt is not None)} {dt!r} {dt0!r} {frac!r} {rel_tol} "
+                f"{int(precond)}\n")
     w, _ = metrics(prob)
     V = w[0][:, None, None] * w[1][None, :, None] * w[2][None, None, :]
     np.concatenate(prob.vertices).tofile(tmp_path / "vertices.bin")
@@ -108,7 +109,7 @@ def run_adapter(tmp_path, prob, lam, inner, ncorr, prediction, ddt, x0, xold,
 
 
 @pytest.mark.parametrize("inner", ["FFT", "MG"])
-@pytest.mark.parametrize("mode", ["plain", "prediction_ddt"])
+@pytest.mark.parametrize("mode", ["plain", "prediction_ddt", "precond_ddt"])
 def test_split_adapter_equals_composed_steps(gpu, tmp_path, inner, mode):
     prob, lam = problem("spheres")
     print(f"[split adapter] inner={inner} mode={mode}", flush=True)
@@ -117,16 +118,26 @@ def test_split_adapter_equals_composed_steps(gpu, tmp_path, inner, mode):
     xold = rng.standard_normal(prob.padded) * 1e-3
     xoldold = rng.standard_normal(prob.padded) * 1e-3
     prediction = mode != "plain"
+    precond = mode == "precond_ddt"
     ddt = (2e-3, 4e-3, 0.5) if mode != "plain" else None
     ncorr = 3
     x, bh, fl, V = run_adapter(tmp_path, prob, lam, inner, ncorr, prediction,
-                               ddt, x0, xold, xoldold)
+                               ddt, x0, xold, xoldold, precond=precond)
     # the same steps composed from the library's calls
     l0 = face_max(lam, prob.n)
     xr = x0.copy()
     if prediction:
         r = (ddt[0] * ddt[2]) / ddt[1]
         xr = (1.0 + r) * xold - r * xoldold
+    if precond:
+        pre = gpu.Context(prob.vertices, prob.bc, prob.bc_value,
+                          rel_tol=1e-10, tolerance=0.0, n_sweeps_pre=2,
+                          n_sweeps_post=2, cycle=bmg.VCYCLE, coarse_level=1)
+        pre.set_laplacian(lam, ddt=(ddt[0], ddt[2]))
+        bs = 0.0 - V * prob.b
+        bs = bs - ((V * xold) / ddt[0]) / ddt[2]
+        xr, _ = pre.solve(bs, xr)
+        pre.close()
     geo = gpu.Context(prob.vertices, prob.bc, prob.bc_value)
     if inner == "FFT":
         solver = gpu.FFT(prob.vertices, prob.bc, prob.bc_value)
