@@ -38,6 +38,8 @@ EXPORTS = [
     "bmg_fft_create", "bmg_fft_destroy", "bmg_fft_solve",
     "bmg_fft_solve_device", "bmg_fft_solve_ddt", "bmg_fft_solve_ddt_device",
     "bmg_split_source", "bmg_split_source_device",
+    "bmg_set_boundary_values", "bmg_set_boundary_values_device",
+    "bmg_renew_source_terms",
 ]
 
 
@@ -138,6 +140,9 @@ def load():
         "bmg_split_source": (I, [P, DP, DP, DP, DP, D, DP, DP, DP, DP, DP]),
         "bmg_split_source_device": (I, [P, P, P, P, P, D, P, P, P, P, P]),
         "bmg_fft_solve_ddt_device": (I, [P, P, P, P, D, D]),
+        "bmg_set_boundary_values": (I, [P, I, DP]),
+        "bmg_set_boundary_values_device": (I, [P, I, P]),
+        "bmg_renew_source_terms": (I, [P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -209,6 +214,32 @@ class Context:
                 raise KeyError(k)
             setattr(self.opts, k, v)
         _check(self.lib.bmg_set_opts(self.ptr, ctypes.byref(self.opts)))
+
+    def boundary_shape(self, face):
+        """Shape of a face's per-cell boundary data: the level-0 boundary
+        layer with the normal axis of size 1."""
+        s = list(self.shapes[0])
+        s[face // 2] = 1
+        return tuple(s)
+
+    def set_boundary_values(self, face, values):
+        """Per-cell Dirichlet value / Neumann gradient of a face (the
+        reference's patch block, boundary_shape(face), last axis fastest);
+        None: the uniform value of the constructor."""
+        if values is None:
+            null = ctypes.POINTER(ctypes.c_double)()
+            _check(self.lib.bmg_set_boundary_values(self.ptr, face, null))
+            return
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        if v.shape != self.boundary_shape(face):
+            raise ValueError(f"boundary data shape {v.shape} != "
+                             f"{self.boundary_shape(face)}")
+        _check(self.lib.bmg_set_boundary_values(self.ptr, face, _dp(v)))
+
+    def renew_source_terms(self):
+        """The next solve folds the boundary terms into its b again, as for
+        a newly assembled system with the same operator."""
+        _check(self.lib.bmg_renew_source_terms(self.ptr))
 
     def set_stencil(self, l, A, symmetric=True):
         A = np.ascontiguousarray(A, dtype=np.float64)

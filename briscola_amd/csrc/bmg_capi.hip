@@ -268,7 +268,7 @@ void bmg_ctx::smooth(int l, int sweeps)
             launch_lexgs(q.L, o, x, b, interior(l, 1), stream);
             if (anyPeriodic())
             {
-                launch_bc_fill(q.L, o, x, 0, l == 0, bcval, stream);
+                launch_bc_fill(q.L, o, x, 0, l == 0, &bcval, stream);
                 for (int f = 0; f < 6; f++)
                     if (periodic(f))
                         launch_lexgs(q.L, o, x, b, slab(l, f), stream);
@@ -284,7 +284,7 @@ void bmg_ctx::smooth(int l, int sweeps)
             launch_jacobi(q.L, o, x, b, tmp, interior(l, 1), stream);
             if (anyPeriodic())
             {
-                launch_bc_fill(q.L, o, x, 0, l == 0, bcval, stream);
+                launch_bc_fill(q.L, o, x, 0, l == 0, &bcval, stream);
                 for (int f = 0; f < 6; f++)
                     if (periodic(f))
                         launch_jacobi(q.L, o, x, b, tmp, slab(l, f), stream);
@@ -316,7 +316,7 @@ void bmg_ctx::smooth(int l, int sweeps)
         if (!anyPeriodic()) continue;
         // periodic ghosts, plus snapshots of the slab planes
         T* const (*sn)[2] = Snap<T>(l);
-        launch_bc_fill_snap(q.L, o, x, 0, l == 0, bcval, sn, stream);
+        launch_bc_fill_snap(q.L, o, x, 0, l == 0, &bcval, sn, stream);
         // width-1 slabs of the periodic faces, face pairs in order: one
         // launch per pair when its two planes are independent, else one per
         // plane
@@ -382,7 +382,7 @@ void bmg_ctx::residual(int l, bool corrBcs, bool norm, int* nblocks)
     Level& q = lv[l];
     const OpView<T> o = op<T>(l);
     if (corrBcs && anyPeriodic())
-        launch_bc_fill(q.L, o, X<T>(l), 0, l == 0, bcval, stream);
+        launch_bc_fill(q.L, o, X<T>(l), 0, l == 0, &bcval, stream);
     launch_residual(q.L, o, cst(X<T>(l)), cst(B<T>(l)), Rr<T>(l),
                     norm ? partials : nullptr, nblocks, stream);
 }
@@ -400,7 +400,7 @@ void bmg_ctx::prolongFrom(int l)
     // the coarse ghosts are used, so correct them first
     // (linearProlongationScheme.C:128)
     launch_bc_fill(lv[l + 1].L, op<T>(l + 1), X<T>(l + 1), 1, (l + 1) == 0,
-                   bcval, stream);
+                   &bcval, stream);
     launch_prolong(lv[l].L, lv[l + 1].L, transfer<T>(l), cst(X<T>(l + 1)),
                    X<T>(l), stream);
 }
@@ -412,7 +412,7 @@ void bmg_ctx::coarseSolve()
     if (!prepared || preparedCoarse != l)
         throw Error(BMG_ERR_STATE, "coarse level not factorised");
     launch_coarse_solve(lv[l].L, op<T>(l), X<T>(l), cst(B<T>(l)), dLU, dPiv,
-                        nc, l == 0, bcval, stream);
+                        nc, l == 0, &bcval, stream);
 }
 
 // MG::cycle (MG.C:19-129).  On the coarsest level in direct mode the
@@ -638,11 +638,18 @@ void bmg_ctx::prepare(bool constMatrix)
         launch_pin(qc.L, pr(qc.a7), stream);
     if (!eliminated)
     {
-        bool inhom = false;
+        // the boundary terms of b are folded at solve start (after b is
+        // loaded); a stored level-0 stencil keeps its ghost coefficients for
+        // them here, before elimination zeroes them
         for (int f = 0; f < 6; f++)
-            if (!periodic(f) && bcval[f] != 0.0) inhom = true;
-        if (inhom)
-            launch_bc_source(lv[0].L, op<double>(0), pr(lv[0].b), bcval, stream);
+        {
+            bcval.g[f] = nullptr;
+            if (lv[0].opkind != OP_STENCIL || periodic(f) || !gsave[f])
+                continue;
+            bmg::launch_save_ghost_coeffs(lv[0].L, cpr(lv[0].a7), f, gsave[f],
+                                          stream);
+            bcval.g[f] = gsave[f];
+        }
         for (auto& q : lv)
             if (q.opkind == OP_STENCIL)
                 launch_eliminate(q.L, pr(q.a7), stream);
@@ -830,7 +837,7 @@ void bmg_ctx::enqueueFCGIteration(int mode)
     // the part after it
     record_tag(0);
     if (anyPeriodic())
-        launch_bc_fill(q0.L, o, Pv, 0, 1, bcval, stream);
+        launch_bc_fill(q0.L, o, Pv, 0, 1, &bcval, stream);
     const int nb = launch_apply_dots(q0.L, o, cst(Pv), Qv, cst(Rv), partials,
                                      stream);
     launch_reduce_rows(partials, nb, 2, dScalar, stream, hScalarDev);
@@ -1035,7 +1042,7 @@ void bmg_ctx::runFCG(bool constMatrix, bmg_stats* st)
         enqueueLearned(nullptr, ss, pr(q0.x));
         launch_add_interior(q0.L, X, cpr(q0.x), stream);
         if (anyPeriodic())
-            launch_bc_fill(q0.L, o, X, 0, 1, bcval, stream);
+            launch_bc_fill(q0.L, o, X, 0, 1, &bcval, stream);
         launch_residual(q0.L, o, cst(X), cst(Bv), Rv, partials, &nb, stream);
         const double ssg = reduceFinal(nb);
         guessResidual = std::sqrt(ssg)/nf;
@@ -1148,6 +1155,7 @@ void bmg_ctx::prepareFCG()
 // (MG.C:163) with its squared norm (host slot 0) and the sum of x (slot 1).
 void bmg_ctx::enqueueFCGStart()
 {
+    enqueueBoundaryTerms();
     Level& q0 = lv[0];
     const long n = q0.L.ncol;
     Pair<double> X{{kx[0], kx[1]}}, Bv{{kb[0], kb[1]}}, Rv{{kr[0], kr[1]}};
@@ -1218,7 +1226,7 @@ void bmg_ctx::enqueueFCGConfirm()
     const OpView<double> o = op<double>(0);
     Pair<double> X{{kx[0], kx[1]}}, Bv{{kb[0], kb[1]}}, Rv{{kr[0], kr[1]}};
     if (anyPeriodic())
-        launch_bc_fill(q0.L, o, X, 0, 1, bcval, stream);
+        launch_bc_fill(q0.L, o, X, 0, 1, &bcval, stream);
     int nb = 0;
     launch_residual(q0.L, o, cst(X), cst(Bv), Rv, partials, &nb, stream);
     launch_reduce_final(partials, nb, dScalar, stream, hScalarDev);
@@ -1232,7 +1240,7 @@ void bmg_ctx::enqueueFCGEnd()
     Pair<const double> X{{kx[0], kx[1]}}, Bv{{kb[0], kb[1]}};
     launch_copy<double>(n, X, pr(q0.x), stream);
     launch_copy<double>(n, Bv, pr(q0.b), stream);
-    launch_bc_fill(q0.L, op<double>(0), pr(q0.x), 2, 1, bcval, stream);
+    launch_bc_fill(q0.L, op<double>(0), pr(q0.x), 2, 1, &bcval, stream);
 }
 
 double bmg_ctx::normFactorOf(Pair<const double> x, Pair<const double> r,
@@ -1266,8 +1274,16 @@ bool bmg_ctx::prepareNeedsHost(bool constMatrix) const
 
 // MG.C:166-172: r = b - A x without BC correction and its squared norm
 // (host slot 0), and the sum of x for gAverage (slot 1, solver.C:24).
+void bmg_ctx::enqueueBoundaryTerms()
+{
+    if (foldPending && inhomogeneous())
+        launch_bc_source(lv[0].L, op<double>(0), pr(lv[0].b), &bcval, stream);
+    foldPending = false;
+}
+
 void bmg_ctx::enqueueSolveStart()
 {
+    enqueueBoundaryTerms();
     Level& q0 = lv[0];
     launch_fill<double>(pr(q0.r), q0.L.ncol, 0.0, stream);
     int nb = 0;
@@ -1291,7 +1307,7 @@ void bmg_ctx::enqueueNormTerms(double y)
 // x[0].correct<eliminatedBcs>() (MG.C:475)
 void bmg_ctx::enqueueSolveEnd()
 {
-    launch_bc_fill(lv[0].L, op<double>(0), pr(lv[0].x), 2, 1, bcval, stream);
+    launch_bc_fill(lv[0].L, op<double>(0), pr(lv[0].x), 2, 1, &bcval, stream);
 }
 
 // MG::Solve (MG.C:132-254) around the cycles; FP64 or mixed precision.
@@ -1481,7 +1497,7 @@ int bmg_create(bmg_ctx** out, const bmg_grid* g, const bmg_opts* opts)
         HIPCK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         HIPCK(hipEventCreate(&c->ev0));
         HIPCK(hipEventCreate(&c->ev1));
-        for (int f = 0; f < 6; f++) { c->bc[f] = g->bc[f]; c->bcval[f] = g->bc_value[f]; }
+        for (int f = 0; f < 6; f++) { c->bc[f] = g->bc[f]; c->bcval.v[f] = g->bc_value[f]; }
         bmg_opts o;
         bmg_default_opts(&o);
         if (opts) o = *opts;
@@ -1596,6 +1612,15 @@ int bmg_create(bmg_ctx** out, const bmg_grid* g, const bmg_opts* opts)
                 f.pu[d] = devUpload(pu);
             }
         }
+        // saved level-0 ghost coefficients of a stored stencil, one per
+        // boundary cell of each non-periodic face
+        for (int f = 0; f < 6; f++)
+        {
+            if (c->periodic(f)) continue;
+            const int d = f/2;
+            c->gsave[f] = devAlloc<double>(size_t(c->lv[0].L.N[d == 0 ? 1 : 0])*
+                                           c->lv[0].L.N[d == 2 ? 1 : 2]);
+        }
         c->npartials = 3*maxBlocks;
         c->partials = devAlloc<double>(3*maxBlocks);
         c->dScalar = devAlloc<double>(4);
@@ -1630,6 +1655,66 @@ int bmg_set_opts(bmg_ctx* c, const bmg_opts* o)
         c->opts = *o;
         c->gen++;
     });
+}
+
+// Per-cell level-0 boundary data of one Neumann/Dirichlet face: the patch
+// block boundaryValues_[0] / boundaryGradients_[0] of the face's boundary
+// condition (DirichletBoundaryConditionBase.H:30,
+// NeumannBoundaryConditionBase.H:30), copied to the device on the
+// context's stream.  Captured graphs carry the data pointers, so a change
+// from uniform to per-cell data (or back) invalidates them.
+static void setBoundaryValues(bmg_ctx* c, int face, const double* v,
+                              bool device)
+{
+    if (!c) throw Error(BMG_ERR_ARG, "null pointer");
+    if (face < 0 || face > 5) throw Error(BMG_ERR_ARG, "face out of range");
+    if (c->periodic(face))
+        throw Error(BMG_ERR_ARG, "a periodic face has no boundary values");
+    if (!v)
+    {
+        if (c->bcdev[face])
+        {
+            HIPCK(hipStreamSynchronize(c->stream));
+            devFree(c->bcdev[face]);
+            c->gen++;
+        }
+        c->bcval.p[face] = nullptr;
+        return;
+    }
+    const int fd = face/2;
+    size_t n = 1;
+    for (int d = 0; d < 3; d++)
+        if (d != fd) n *= size_t(c->lv[0].L.N[d]);
+    if (!c->bcdev[face])
+    {
+        HIPCK(hipStreamSynchronize(c->stream));
+        c->bcdev[face] = devAlloc<double>(n);
+        c->gen++;
+    }
+    HIPCK(hipMemcpyAsync(c->bcdev[face], v, n*sizeof(double),
+                         device ? hipMemcpyDeviceToDevice
+                                : hipMemcpyHostToDevice, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+    c->bcval.p[face] = c->bcdev[face];
+}
+
+int bmg_renew_source_terms(bmg_ctx* c)
+{
+    return run([&] {
+        if (!c) throw Error(BMG_ERR_ARG, "null context");
+        c->foldPending = true;
+    });
+}
+
+int bmg_set_boundary_values(bmg_ctx* c, int face, const double* values)
+{
+    return run([&] { setBoundaryValues(c, face, values, false); });
+}
+
+int bmg_set_boundary_values_device(bmg_ctx* c, int face,
+                                   const double* d_values)
+{
+    return run([&] { setBoundaryValues(c, face, d_values, true); });
 }
 
 int bmg_num_levels(const bmg_ctx* c) { return c ? c->nLevels() : BMG_ERR_ARG; }
@@ -1675,6 +1760,7 @@ static void setStencil(bmg_ctx* c, int level, const double* A7, bool device,
     c->gen++;
     c->symmetric = symmetric != 0;
     c->eliminated = false;
+    c->foldPending = true;
     c->prepared = false;
     c->mixedReady = false;
 }
@@ -1826,6 +1912,7 @@ static void setLaplacian(bmg_ctx* c, const double* lx, const double* ly,
         c->gen++;
         c->symmetric = true;
         c->eliminated = false;
+        c->foldPending = true;
         c->prepared = false;
         c->mixedReady = false;
     }
@@ -2053,15 +2140,12 @@ int bmg_solve_diagonal(bmg_ctx* c, const double* A7, double* b, double* x)
                              c->stream));
         launch_ref_to_split<double>(q.L, db, pr(q.b), c->stream);
         launch_ref_to_split<double>(q.L, dx, pr(q.x), c->stream);
-        bool inhom = false;
-        for (int f = 0; f < 6; f++)
-            if (!c->periodic(f) && c->bcval[f] != 0.0) inhom = true;
-        if (inhom)
-            launch_bc_source(q.L, c->op<double>(0), pr(q.b), c->bcval, c->stream);
+        if (c->inhomogeneous())
+            launch_bc_source(q.L, c->op<double>(0), pr(q.b), &c->bcval, c->stream);
         launch_eliminate(q.L, pr(q.a7), c->stream);
         c->eliminated = true;
         launch_diag(q.L, cpr(q.a7), pr(q.x), cpr(q.b), c->stream);
-        launch_bc_fill(q.L, c->op<double>(0), pr(q.x), 1, 1, c->bcval, c->stream);
+        launch_bc_fill(q.L, c->op<double>(0), pr(q.x), 1, 1, &c->bcval, c->stream);
         launch_split_to_ref<double>(q.L, cpr(q.x), dx, c->stream);
         launch_split_to_ref<double>(q.L, cpr(q.b), db, c->stream);
         HIPCK(hipGetLastError());
@@ -2274,6 +2358,7 @@ int bmg_op_prepare(bmg_ctx* c)
     return run([&] {
         if (!c) throw Error(BMG_ERR_ARG, "null context");
         c->prepare(false);
+        c->enqueueBoundaryTerms();
         HIPCK(hipStreamSynchronize(c->stream));
     });
 }
@@ -2342,7 +2427,7 @@ int bmg_op_correct_bcs(bmg_ctx* c, int level)
     return run([&] {
         checkLevel(c, level);
         launch_bc_fill(c->lv[level].L, c->op<double>(level),
-                       pr(c->lv[level].x), 1, level == 0, c->bcval, c->stream);
+                       pr(c->lv[level].x), 1, level == 0, &c->bcval, c->stream);
         HIPCK(hipGetLastError());
         HIPCK(hipStreamSynchronize(c->stream));
     });

@@ -158,7 +158,16 @@ struct bmg_ctx
     int device = 0;
     hipStream_t stream = nullptr;
     int bc[6];
-    double bcval[6];
+    // level-0 boundary data: uniform values, or per-cell device arrays
+    // (bcdev, owned) set by bmg_set_boundary_values
+    bmg::BcData bcval{};
+    double* bcdev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    // level-0 ghost coefficients of a stored stencil saved before its
+    // elimination (one per boundary cell of each non-periodic face)
+    double* gsave[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    // the next solve folds the level-0 boundary terms into its b (set with
+    // the operator, and by bmg_renew_source_terms)
+    bool foldPending = true;
     bmg_opts opts;
     std::vector<Level> lv;
     bool symmetric = true;
@@ -226,6 +235,13 @@ struct bmg_ctx
             if (bc[f] == BMG_DIRICHLET) return false;
         return true;
     }
+    // level-0 boundary terms to fold into b (eliminateGhosts, l == 0)
+    bool inhomogeneous() const
+    {
+        for (int f = 0; f < 6; f++)
+            if (!periodic(f) && (bcval.p[f] || bcval.v[f] != 0.0)) return true;
+        return false;
+    }
     bool anyPeriodic() const
     {
         for (int f = 0; f < 6; f++)
@@ -236,6 +252,7 @@ struct bmg_ctx
     ~bmg_ctx()
     {
         for (auto& l : lv) l.freeAll();
+        for (int f = 0; f < 6; f++) { devFree(bcdev[f]); devFree(gsave[f]); }
         devFree(dLU);
         devFree(dPiv);
         devFree(partials);
@@ -348,6 +365,9 @@ struct bmg_ctx
     // runSolve in phases, each enqueued without synchronisation (and so
     // recordable for batched solves)
     bool prepareNeedsHost(bool constMatrix) const;
+    // b[0] -= the level-0 boundary terms of eliminateGhosts, once per
+    // assembled (or renewed) system
+    void enqueueBoundaryTerms();
     void enqueueSolveStart();
     void enqueueNormTerms(double y);
     void enqueueSolveEnd();

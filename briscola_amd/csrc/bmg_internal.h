@@ -66,6 +66,22 @@ struct Region {
     int E[3];
 };
 
+// Level-0 boundary data of the Neumann (gradient) and Dirichlet (value)
+// faces, order -x,+x,-y,+y,-z,+z: p[f] null -> the uniform value v[f];
+// else per boundary cell, device memory in the reference's patch-block
+// layout (the face's boundary layer with the normal axis of size 1, index
+// (ijk - S), last axis fastest): the value of interior cell (I,J,K) (padded
+// indices) of the face normal to d is p[f][(t1 - 1)*N[a2] + (t2 - 1)] with
+// (t1, t2) its indices along the two other axes a1 < a2.
+// g[f], when set, holds the level-0 ghost coefficients of face f before
+// elimination in the same layout (a stored stencil loses them when it is
+// eliminated in place); otherwise they come from the operator.
+struct BcData {
+    double v[6];
+    const double* p[6];
+    const double* g[6];
+};
+
 // ---- batched execution (bmg_kernels.hip, bmg_batch.hip) ----
 // One system's share of a batched launch: virtual workgroups
 // [start, start + n) run the system's launch with grid (gx, gy, n/(gx*gy)).
@@ -173,13 +189,13 @@ void launch_prolong(const LevelDims& Lf, const LevelDims& Lc,
 // only (correct<eliminatedBcs>); level0 enables the inhomogeneous terms.
 template <typename T>
 void launch_bc_fill(const LevelDims& L, const OpView<T>& op, Pair<T> x,
-                    int mode, int level0, const double* bcval, hipStream_t s);
+                    int mode, int level0, const BcData* bcval, hipStream_t s);
 // The same, also copying the slab planes of every periodic axis d into
 // snap[d][0] (plane 1) and snap[d][1] (plane N_d), P_a1 x P_a2 each
 // (snap == nullptr or a null row: none for that axis).
 template <typename T>
 void launch_bc_fill_snap(const LevelDims& L, const OpView<T>& op, Pair<T> x,
-                         int mode, int level0, const double* bcval,
+                         int mode, int level0, const BcData* bcval,
                          T* const (*snap)[2], hipStream_t s);
 template <typename T>
 void launch_fill(Pair<T> x, long ncol, T v, hipStream_t s);
@@ -198,9 +214,13 @@ void launch_materialize_stencil(const LevelDims& L, const OpView<T>& op,
                                 Pair<T> a7, hipStream_t s);
 void launch_pin(const LevelDims& L, Pair<double> a7, hipStream_t s);
 void launch_eliminate(const LevelDims& L, Pair<double> a7, hipStream_t s);
+// The ghost coefficient (component f+1) of every level-0 boundary cell of
+// face f of a stored stencil, in the boundary-data layout of BcData.
+void launch_save_ghost_coeffs(const LevelDims& L, Pair<const double> a7,
+                              int f, double* out, hipStream_t s);
 // b[0] -= inhomogeneous boundary terms of eliminateGhosts (level 0 only)
 void launch_bc_source(const LevelDims& L, const OpView<double>& op,
-                      Pair<double> b, const double* bcval, hipStream_t s);
+                      Pair<double> b, const BcData* bcval, hipStream_t s);
 void launch_lambda_restrict(const LevelDims& Lf, const LevelDims& Lc,
                             const double* fw0, const double* fw1,
                             const double* fw2, const double* lamf,
@@ -238,7 +258,7 @@ inline int coarse_wave_width(int n)
 template <typename T>
 void launch_coarse_solve(const LevelDims& L, const OpView<T>& op, Pair<T> x,
                          Pair<const T> b, const double* lu, const int* piv,
-                         int n, int level0, const double* bcval,
+                         int n, int level0, const BcData* bcval,
                          hipStream_t s);
 template <typename T>
 void launch_sum_sq(const LevelDims& L, Pair<const T> f, double* partials,

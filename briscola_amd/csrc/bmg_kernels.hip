@@ -19,9 +19,28 @@ namespace {
 constexpr int BX = 64;   // one wavefront along the contiguous axis
 constexpr int BY = 4;
 
-struct BcVals {
-    double v[6];
-};
+using BcVals = BcData;
+
+// Boundary value of face f at interior cell s (padded indices) of the
+// face's boundary layer: the per-cell array when set (DirichletBoundary-
+// Condition.C:45,68, NeumannBoundaryCondition.C:51,83 read
+// boundaryValues_[d](ijk - S)), else the face's uniform value.
+__device__ __forceinline__ double bc_value_at(const LevelDims& L,
+                                              const double* p, int f,
+                                              const int s[3])
+{
+    const int d = f >> 1;
+    const int a1 = d == 0 ? 1 : 0, a2 = d == 2 ? 1 : 2;
+    return p[(s[a1] - 1)*L.N[a2] + (s[a2] - 1)];
+}
+
+__device__ __forceinline__ double bc_value(const LevelDims& L,
+                                           const BcVals& bv, int f,
+                                           const int s[3])
+{
+    return bv.p[f] ? bc_value_at(L, bv.p[f], f, s) : bv.v[f];
+}
+
 
 __device__ __forceinline__ int cidx(const LevelDims& L, int I, int J, int K)
 {
@@ -29,6 +48,25 @@ __device__ __forceinline__ int cidx(const LevelDims& L, int I, int J, int K)
 }
 
 __device__ __forceinline__ int nci(const LevelDims& L) { return int(L.ncol); }
+
+// Ghost coefficient of every boundary cell of face f of a stored level-0
+// stencil (component f+1, before elimination), one thread per cell.
+__global__ void k_save_ghost_coeffs(LevelDims L, Pair<const double> a7,
+                                    int f, double* out)
+{
+    const int d = f >> 1;
+    const int a1 = d == 0 ? 1 : 0, a2 = d == 2 ? 1 : 2;
+    const int n1 = L.N[a1], n2 = L.N[a2];
+    const int t = int(blockIdx.x*blockDim.x + threadIdx.x);
+    if (t >= n1*n2) return;
+    int s[3];
+    s[a1] = 1 + t/n2;
+    s[a2] = 1 + t%n2;
+    s[d] = (f & 1) ? L.N[d] : 1;
+    const int c = (s[0] + s[1] + s[2]) & 1;
+    out[t] = (c ? a7.c[1] : a7.c[0])[size_t(f + 1)*size_t(L.ncol) +
+                                     size_t(cidx(L, s[0], s[1], s[2]))];
+}
 
 // Element i of an array addressed by a 32-bit byte offset from a base held
 // in scalar registers (every array is below 2^31 bytes), so that a load is
@@ -1317,13 +1355,14 @@ __device__ __forceinline__ void ghost_cell(const LevelDims& L,
             int d = bo[0] != 0 ? 0 : (bo[1] != 0 ? 1 : 2);
             const int f = 2*d + (bo[d] > 0);
             const T H = T(level0 ? 1.0 : 0.0);
+            const T bvf = T(level0 ? bc_value(L, bv, f, s) : 0.0);
             if (L.bc[f] == NEU)
             {
                 const T delta = op.dl[d][bo[d] > 0 ? L.N[d] + 1 : 1];
-                val = xi + H*T(bv.v[f])/delta;
+                val = xi + H*bvf/delta;
             }
             else
-                val = H*T(2.0)*T(bv.v[f]) - xi;
+                val = H*T(2.0)*bvf - xi;
         }
     }
     col(x, (I + J + K) & 1)[cidx(L, I, J, K)] = val;
@@ -1419,7 +1458,7 @@ __device__ __forceinline__ void face_ghost(const LevelDims& L,
     else
     {
         const T H = T(level0 ? 1.0 : 0.0);
-        const T bvf = T(side ? bv.v[2*D + 1] : bv.v[2*D]);
+        const T bvf = T(level0 ? bc_value(L, bv, 2*D + side, s) : 0.0);
         if (bcf == NEU)
         {
             const T delta = ld(op.dl[D], unsigned(side ? N + 1 : 1));
@@ -1951,14 +1990,16 @@ __device__ __forceinline__ void bc_source_body(const Bid bid,
         for (int f = 0; f < 6; f++)
         {
             if (!on[f] || L.bc[f] == PER) continue;
-            const double g = a[f + 1];
+            const double g = bv.g[f] ? bc_value_at(L, bv.g[f], f, pc)
+                                     : a[f + 1];
+            const double bvf = bc_value(L, bv, f, pc);
             if (L.bc[f] == NEU)
             {
                 const double delta = op.dl[f/2][pc[f/2] + (f & 1)];
-                v = v - g*bv.v[f]/delta;
+                v = v - g*bvf/delta;
             }
             else
-                v = v - g*2.0*bv.v[f];
+                v = v - g*2.0*bvf;
         }
         col(b, colour)[idx] = v;
     }
@@ -3370,12 +3411,18 @@ Tiles pair_tiles_packed(const LevelDims& L)
     return t;
 }
 
-BcVals bcvals(const double* v)
+BcVals bcvals(const BcData* v)
 {
     BcVals b;
-    for (int f = 0; f < 6; f++) b.v[f] = v ? v[f] : 0.0;
+    for (int f = 0; f < 6; f++)
+    {
+        b.v[f] = v ? v->v[f] : 0.0;
+        b.p[f] = v ? v->p[f] : nullptr;
+        b.g[f] = v ? v->g[f] : nullptr;
+    }
     return b;
 }
+
 
 // ---------------------------------------------------------------------------
 // Batched forms of the kernels whose specialisations differ only in the
@@ -3903,7 +3950,7 @@ void launch_prolong(const LevelDims& Lf, const LevelDims& Lc,
 
 template <typename T>
 void launch_bc_fill_snap(const LevelDims& L, const OpView<T>& op, Pair<T> x,
-                         int mode, int level0, const double* bcval,
+                         int mode, int level0, const BcData* bcval,
                          T* const (*snap)[2], hipStream_t s)
 {
     // groups holding ghosts this mode writes: mode 0 the periodic faces,
@@ -3943,7 +3990,7 @@ void launch_bc_fill_snap(const LevelDims& L, const OpView<T>& op, Pair<T> x,
 
 template <typename T>
 void launch_bc_fill(const LevelDims& L, const OpView<T>& op, Pair<T> x,
-                    int mode, int level0, const double* bcval, hipStream_t s)
+                    int mode, int level0, const BcData* bcval, hipStream_t s)
 {
     launch_bc_fill_snap<T>(L, op, x, mode, level0, bcval, nullptr, s);
 }
@@ -4008,8 +4055,17 @@ void launch_eliminate(const LevelDims& L, Pair<double> a7, hipStream_t s)
     hipLaunchKernelGGL(k_eliminate, grid_interior(L), BLK, 0, s, L, a7);
 }
 
+void launch_save_ghost_coeffs(const LevelDims& L, Pair<const double> a7,
+                              int f, double* out, hipStream_t s)
+{
+    const int d = f/2;
+    const int n = L.N[d == 0 ? 1 : 0]*L.N[d == 2 ? 1 : 2];
+    hipLaunchKernelGGL(k_save_ghost_coeffs, dim3((n + 255)/256), dim3(256), 0,
+                       s, L, a7, f, out);
+}
+
 void launch_bc_source(const LevelDims& L, const OpView<double>& op,
-                      Pair<double> b, const double* bcval, hipStream_t s)
+                      Pair<double> b, const BcData* bcval, hipStream_t s)
 {
     if (op.kind == OP_STENCIL)
         dispatch<KBcSource<OP_STENCIL>>(grid_interior(L), 0, s,
@@ -4089,7 +4145,7 @@ void launch_face_flux(const LevelDims& L, const double* x, const double* lam,
 template <typename T>
 void launch_coarse_solve(const LevelDims& L, const OpView<T>& op, Pair<T> x,
                          Pair<const T> b, const double* lu, const int* piv,
-                         int n, int level0, const double* bcval,
+                         int n, int level0, const BcData* bcval,
                          hipStream_t s)
 {
     switch (coarse_wave_width(n))
@@ -4337,11 +4393,11 @@ void launch_set_cell(Pair<T> f, long idx, int colour, T v, hipStream_t s)
                                     const TransferView<T>&, Pair<const T>,    \
                                     Pair<T>, hipStream_t);                    \
     template void launch_bc_fill<T>(const LevelDims&, const OpView<T>&,       \
-                                    Pair<T>, int, int, const double*,         \
-                                    hipStream_t);                                                                              \
+                                    Pair<T>, int, int, const BcData*,         \
+                                    hipStream_t);                             \
     template void launch_bc_fill_snap<T>(const LevelDims&, const OpView<T>&,  \
-                                         Pair<T>, int, int, const double*,    \
-                                         T* const (*)[2], hipStream_t);                             \
+                                         Pair<T>, int, int, const BcData*,    \
+                                         T* const (*)[2], hipStream_t);       \
     template void launch_fill<T>(Pair<T>, long, T, hipStream_t);              \
     template void launch_ref_to_split<T>(const LevelDims&, const double*,     \
                                          Pair<T>, hipStream_t);               \
@@ -4353,7 +4409,7 @@ void launch_set_cell(Pair<T> f, long idx, int colour, T v, hipStream_t s)
     template void launch_coarse_solve<T>(const LevelDims&, const OpView<T>&,  \
                                          Pair<T>, Pair<const T>,              \
                                          const double*, const int*, int, int, \
-                                         const double*, hipStream_t);         \
+                                         const BcData*, hipStream_t);         \
     template void launch_sum_sq<T>(const LevelDims&, Pair<const T>, double*,  \
                                    int*, hipStream_t);                        \
     template void launch_sum<T>(const LevelDims&, Pair<const T>, double*,     \

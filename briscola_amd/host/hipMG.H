@@ -83,6 +83,12 @@ struct fvMesh
     std::vector<double> vertices[3];
     int bc[6];                 // BMG_NEUMANN / BMG_DIRICHLET / BMG_PERIODIC
     double bcValue[6] = {0, 0, 0, 0, 0, 0};
+    // per-cell data of a Neumann/Dirichlet face, replacing bcValue[f] when
+    // not empty: boundaryValues()[0] / boundaryGradients()[0] of the face's
+    // boundary condition of x (DirichletBoundaryConditionBase.H:82,
+    // NeumannBoundaryConditionBase.H:82), the face's layer of level-0
+    // cells with the normal axis of size 1, last axis fastest
+    std::vector<double> bcValues[6];
     std::vector<double> cellVolumes;
     double deltaT = 0.0;       // time().deltaTValue() (ddtScheme.H:59)
     double deltaT0 = 0.0;      // time().deltaT0Value(), the previous step
@@ -175,6 +181,25 @@ inline bmg_ctx* createContext(const fvMesh& mesh, const bmg_opts& opts,
     return ctx;
 }
 
+// The per-cell boundary data of the mesh's faces into the context (read
+// again before every solve: boundary values may change between solves).
+inline void applyBoundaryData(bmg_ctx* ctx, const fvMesh& mesh)
+{
+    for (int f = 0; f < 6; f++)
+    {
+        if (mesh.bc[f] == BMG_PERIODIC) continue;
+        const int d = f/2;
+        const size_t n = size_t(mesh.N[d == 0 ? 1 : 0])*mesh.N[d == 2 ? 1 : 2];
+        if (mesh.bcValues[f].empty())
+            check(bmg_set_boundary_values(ctx, f, nullptr));
+        else if (mesh.bcValues[f].size() != n)
+            throw FatalError("boundary data of face " + std::to_string(f) +
+                             " must hold one value per boundary cell");
+        else
+            check(bmg_set_boundary_values(ctx, f, mesh.bcValues[f].data()));
+    }
+}
+
 class MG : public solver
 {
     bmg_ctx* ctx_ = nullptr;
@@ -183,15 +208,18 @@ class MG : public solver
     bool operatorSet_ = false;
     bool verbose_;
     solverStats last_;
+    const fvMesh* mesh_;
+    int bc_[6];
 
 public:
     static constexpr const char* typeName = "MG";
 
     // MG(const dictionary&, const fvMesh&) -- MG.C:257-399
     MG(const dictionary& dict, const fvMesh& mesh, bool verbose = true)
-    : opts_(optsFromDict(dict)), verbose_(verbose)
+    : opts_(optsFromDict(dict)), verbose_(verbose), mesh_(&mesh)
     {
         ctx_ = createContext(mesh, opts_, verts_);
+        for (int f = 0; f < 6; f++) bc_[f] = mesh.bc[f];
     }
 
     ~MG() override { if (ctx_) bmg_destroy(ctx_); }
@@ -213,6 +241,7 @@ public:
     void solve(linearSystem& sys, bool constMatrix = false) override
     {
         const int L = nLevels();
+        applyBoundaryData(ctx_, *mesh_);
         if (sys.diagonal)
         {
             check(bmg_solve_diagonal(ctx_, sys.A.at(0).data(), sys.b.data(),
@@ -230,6 +259,11 @@ public:
                                                sys.symmetric));
             operatorSet_ = true;
         }
+        else if (hasGhostCoefficients(sys))
+            // a newly assembled system with the same matrix: its ghost
+            // coefficients are still in place, so MG::solve's eliminateGhosts
+            // folds the boundary data into b again (MG.C:445)
+            check(bmg_renew_source_terms(ctx_));
         bmg_stats st;
         check(bmg_solve(ctx_, sys.b.data(), sys.x.data(), constMatrix, &st));
         // the reference eliminates ghosts and pins the coarse level in place
@@ -240,6 +274,33 @@ public:
     }
 
 private:
+    // Whether level 0 of sys.A still couples a boundary cell to a ghost
+    // across a Neumann or Dirichlet face (elimination zeroes these,
+    // NeumannBoundaryCondition.C:48, DirichletBoundaryCondition.C:42).
+    bool hasGhostCoefficients(const linearSystem& sys) const
+    {
+        const std::vector<int> n = levelShape(0);
+        const long P1 = n[1] + 2, P2 = n[2] + 2;
+        const std::vector<double>& A = sys.A.at(0);
+        for (int f = 0; f < 6; f++)
+        {
+            if (bc_[f] == BMG_PERIODIC) continue;
+            const int d = f/2;
+            const int a1 = d == 0 ? 1 : 0, a2 = d == 2 ? 1 : 2;
+            for (int t1 = 1; t1 <= n[a1]; t1++)
+                for (int t2 = 1; t2 <= n[a2]; t2++)
+                {
+                    int c[3];
+                    c[d] = (f & 1) ? n[d] : 1;
+                    c[a1] = t1;
+                    c[a2] = t2;
+                    const long p = (c[0]*P1 + c[1])*P2 + c[2];
+                    if (A[7*p + f + 1] != 0.0) return true;
+                }
+        }
+        return false;
+    }
+
     void print(const char* solverName, const std::string& field) const
     {
         if (!verbose_) return;

@@ -167,6 +167,11 @@ public:
 
         if (rk) x = stages_.at(rk->stage - 1);
 
+        // the system is assembled anew on every call (sys =
+        // im::laplacian(...), defaultPoissonSolver.C:89), so MG::solve folds
+        // the boundary data into b every time, also when the matrix is kept
+        applyBoundaryData(ctx_, mesh_);
+        check(bmg_renew_source_terms(ctx_));
         bmg_stats st;
         check(bmg_solve(ctx_, bsys.data(), x.data(), constMatrix, &st));
         last_ = {st.iters, st.initial_residual, st.final_residual};

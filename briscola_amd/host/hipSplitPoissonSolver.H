@@ -216,11 +216,19 @@ public:
     const meshField& bHat() const { return bHat_; }
 
     // xOld, xOldOld: x.oldTime() and x.oldTime().oldTime() (the prediction
-    // reads both; ddt reads xOld)
+    // reads both; ddt reads xOld).  rk: the flow solver's Runge-Kutta scheme
+    // (the reference finds it in the object registry as "rkScheme"); the
+    // preconditioner and a multigrid inner solver are defaultPoissonSolvers,
+    // which then start from, and store, their own stage solution
+    // (defaultPoissonSolver.C:67-80,97-104) -- so the prediction and the
+    // preconditioner's result are replaced by the inner solver's stored
+    // stage before its first solve, as in the reference.  The FFT inner
+    // solver has no stages.
     void solve(meshField& x, const meshField* bPtr,
                const faceField* lambdaPtr, bool ddt = false,
                double dtFrac = 1.0, const meshField* xOld = nullptr,
-               const meshField* xOldOld = nullptr)
+               const meshField* xOldOld = nullptr,
+               const RungeKuttaScheme* rk = nullptr)
     {
         const size_t n = cells();
         if (!lambdaPtr)
@@ -250,7 +258,7 @@ public:
         }
 
         // preconditioner (splitPoissonSolver.C:84-87)
-        if (pre_) pre_->solve(x, bPtr, lambdaPtr, ddt, dtFrac, nullptr, xOld);
+        if (pre_) pre_->solve(x, bPtr, lambdaPtr, ddt, dtFrac, rk, xOld);
 
         const double l0 = lambda0(*lambdaPtr);
         bHat_.assign(n, 0.0);
@@ -264,7 +272,7 @@ public:
                                    bHat_.data(), phi_.f[0].data(),
                                    phi_.f[1].data(), phi_.f[2].data()));
             if (fft_) fft_->solve(x, &bHat_, nullptr, ddt, dtFrac, nullptr, xOld);
-            else mg_->solve(x, &bHat_, nullptr, ddt, dtFrac, nullptr, xOld);
+            else mg_->solve(x, &bHat_, nullptr, ddt, dtFrac, rk, xOld);
             // flux = lambda0*(inner flux + phi) (splitPoissonSolver.C:
             // 141-142)
             if (computeFlux_ && corr == nCorr_ - 1)

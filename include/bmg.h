@@ -79,7 +79,8 @@ typedef struct bmg_grid {
     const double* vertices[3];  /* n[d]+1 vertex coordinates per axis       */
     int bc[6];                  /* enum bmg_bc per face                     */
     double bc_value[6];         /* Neumann gradient / Dirichlet value,
-                                   uniform per face, level 0 only           */
+                                   uniform per face, level 0 only (per-cell
+                                   data: bmg_set_boundary_values)           */
 } bmg_grid;
 
 /* The MG solver dictionary (MG.C:257-304, solver.C:114-123). */
@@ -127,6 +128,26 @@ int bmg_create(bmg_ctx** ctx, const bmg_grid* grid, const bmg_opts* opts);
 int bmg_destroy(bmg_ctx* ctx);
 int bmg_set_opts(bmg_ctx* ctx, const bmg_opts* opts);
 
+/* Per-cell boundary data of one Neumann (gradient) or Dirichlet (value)
+ * face on level 0, replacing the face's uniform bc_value: the patch block
+ * of the face's boundary condition, boundaryValues()[0] of a Dirichlet
+ * condition (DirichletBoundaryConditionBase.H:30,82) or
+ * boundaryGradients()[0] of a Neumann one (NeumannBoundaryConditionBase.H:
+ * 30,82), read cell by cell as boundaryValues_[d](ijk - S)
+ * (DirichletBoundaryCondition.C:45,68, NeumannBoundaryCondition.C:51,83).
+ * Layout: the face's boundary layer of level-0 interior cells with the
+ * normal axis of size 1, (i,j,k) order with the last axis fastest -- for
+ * face -x or +x, Ny*Nz values, value (j,k) at j*Nz + k; for -y/+y, Nx*Nz
+ * values at i*Nz + k; for -z/+z, Nx*Ny values at i*Ny + j.  The data is
+ * copied; it enters b through the ghost elimination of the next solve after
+ * the operator is set (as eliminateGhosts folds it, MG.C:445) and the ghost
+ * values written at the end of every solve (MG.C:475).  values NULL: back
+ * to the uniform value.  BMG_ERR_ARG for a periodic face. */
+int bmg_set_boundary_values(bmg_ctx* ctx, int face, const double* values);
+/* Same, device memory. */
+int bmg_set_boundary_values_device(bmg_ctx* ctx, int face,
+                                   const double* d_values);
+
 int bmg_num_levels(const bmg_ctx* ctx);
 int bmg_level_shape(const bmg_ctx* ctx, int level, int n[3]);
 
@@ -162,6 +183,21 @@ int bmg_set_operator_laplacian_ddt(bmg_ctx* ctx, const double* lambda_x,
                                    const double* lambda_y,
                                    const double* lambda_z, double delta_t,
                                    double dt_frac);
+
+/* The operator as last set stands for a newly assembled linearSystem
+ * again: the next solve folds the level-0 boundary terms of the Dirichlet
+ * values and Neumann gradients into its b, as MG::solve's eliminateGhosts
+ * does for a system whose ghost coefficients are still in place
+ * (MG.C:445, DirichletBoundaryCondition.C:36-46,
+ * NeumannBoundaryCondition.C:40-54), from the coefficients before
+ * elimination, without assembling or eliminating the operator again (the
+ * coarse factorisation stays valid for const_matrix).  Setting an operator
+ * implies it.  For callers that build a new system with the same matrix
+ * for every solve (defaultPoissonSolver.C:39-124 with constMatrix); a
+ * solve without it after a previous one treats b as the caller's
+ * already-eliminated system, as the reference does when the same mutated
+ * linearSystem is solved again. */
+int bmg_renew_source_terms(bmg_ctx* ctx);
 
 /* Read back sys.A()[level] after the solver's ghost elimination and
  * singularity pin (the reference mutates the caller's A, MG.C:433-447). */

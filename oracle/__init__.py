@@ -84,6 +84,8 @@ def _load():
         "oracle_solve": (None, [P, c_int, D]),
         "oracle_coarse_dense": (c_int, [P, D, D, I]),
         "oracle_solve_diagonal": (None, [P]),
+        "oracle_cycle_counts": (None, [P, c_int, ctypes.POINTER(ctypes.c_long)]),
+        "oracle_set_boundary_values": (c_int, [P, c_int, D]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -224,6 +226,42 @@ class Oracle:
 
     def cycle(self):
         self.lib.oracle_cycle(self.ptr)
+
+    COUNT_NAMES = ("visits", "smooth", "residual", "restrict", "prolong",
+                   "coarse_solve")
+
+    def cycle_counts(self):
+        """Per-level operation counts of the last cycle (oracle_cycle or
+        the last iteration of solve): dict name -> list over levels."""
+        out = {}
+        for w, name in enumerate(self.COUNT_NAMES):
+            a = np.zeros(self.nlevels, np.int64)
+            self.lib.oracle_cycle_counts(
+                self.ptr, w, a.ctypes.data_as(ctypes.POINTER(ctypes.c_long)))
+            out[name] = [int(q) for q in a]
+        return out
+
+    def boundary_shape(self, face):
+        """Shape of the per-cell boundary data of a face: the level-0
+        boundary layer with the normal axis of size 1."""
+        s = list(self.shapes[0])
+        s[face // 2] = 1
+        return tuple(s)
+
+    def set_boundary_values(self, face, values):
+        """Per-cell Dirichlet value / Neumann gradient of a face, in the
+        reference's patch-block layout (boundary_shape(face), last axis
+        fastest); None: the uniform value given at construction."""
+        if values is None:
+            null = ctypes.POINTER(ctypes.c_double)()
+            self.lib.oracle_set_boundary_values(self.ptr, face, null)
+            return
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        assert v.shape == self.boundary_shape(face), (v.shape, face)
+        self._bcarr = getattr(self, "_bcarr", {})
+        self._bcarr[face] = v
+        rc = self.lib.oracle_set_boundary_values(self.ptr, face, _dp(v))
+        assert rc == 0
 
     def sum_sq_residual(self):
         return self.lib.oracle_sum_sq_residual(self.ptr)

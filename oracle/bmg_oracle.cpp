@@ -104,6 +104,17 @@ struct oracle_ctx
     std::vector<Level> L;
     int bc[6];
     double bcval[6];
+    // per-cell level-0 boundary data of face f (Dirichlet value / Neumann
+    // gradient), the reference's patch block boundaryValues_[d] /
+    // boundaryGradients_[d] (DirichletBoundaryConditionBase.H:30,
+    // NeumannBoundaryConditionBase.H:30): shape E - S of the face's layer,
+    // index (ijk - S), last axis fastest; empty = uniform bcval[f]
+    std::vector<double> bcarr[6];
+
+    // operation counts of the last cycle: per level, MG::cycle calls
+    // (visits), Smooth calls, residuals, restrictions, prolongations and
+    // coarse solves
+    std::vector<long> cnt[6];
 
     double relTol = 0, tolerance = 0;
     int minIter = 0, maxIter = 999, nPre = 0, nPost = 2;
@@ -239,6 +250,18 @@ bool hasBoundary(const oracle_ctx& c, const int bo[3])
     return true;
 }
 
+/* Boundary value of face f at boundary cell (i,j,k) of the layer [S,E):
+ * boundaryValues_[d](ijk - S) (DirichletBoundaryCondition.C:45,68,
+ * NeumannBoundaryCondition.C:51,83), or the uniform value. */
+double bcValueAt(const oracle_ctx& c, int face, const int S[3], const int E[3],
+                 int i, int j, int k)
+{
+    const std::vector<double>& a = c.bcarr[face];
+    if (a.empty()) return c.bcval[face];
+    const long n1 = E[1] - S[1], n2 = E[2] - S[2];
+    return a[((long)(i - S[0])*n1 + (j - S[1]))*n2 + (k - S[2])];
+}
+
 /* Neumann/Dirichlet evaluate (NeumannBoundaryCondition.C:58-85,
  * DirichletBoundaryCondition.C:50-71) for face f of level l. */
 void patchEvaluate(const oracle_ctx& c, int l, Level& lv,
@@ -266,11 +289,11 @@ void patchEvaluate(const oracle_ctx& c, int l, Level& lv,
             const int pc[3] = {i, j, k};
             const int fi = pc[fd] + (face%2);
             const double delta = lv.ax[fd].dl[fi];
-            f[g] = f[p] + H*c.bcval[face]/delta;
+            f[g] = f[p] + H*bcValueAt(c, face, S, E, i, j, k)/delta;
         }
         else if (c.bc[face] == DIRICHLET)
         {
-            f[g] = H*2.0*c.bcval[face] - f[p];
+            f[g] = H*2.0*bcValueAt(c, face, S, E, i, j, k) - f[p];
         }
     }
 }
@@ -851,7 +874,8 @@ void eliminateGhosts(oracle_ctx& c)
                         const int pc[3] = {i, j, k};
                         const double delta =
                             lv.ax[fd].dl[pc[fd] + (face%2)];
-                        lv.b[p] -= g*c.bcval[face]/delta;
+                        lv.b[p] -= g*bcValueAt(c, face, S, E, i, j, k)
+                                 / delta;
                     }
                 }
                 else
@@ -859,7 +883,7 @@ void eliminateGhosts(oracle_ctx& c)
                     a[0] -= g;
                     a[face + 1] = 0.0;
                     if (l == 0)
-                        lv.b[p] -= g*2.0*c.bcval[face];
+                        lv.b[p] -= g*2.0*bcValueAt(c, face, S, E, i, j, k);
                 }
             }
         }
@@ -1027,28 +1051,51 @@ void cycleLevel(oracle_ctx& c, std::vector<int>& visits, int l)
     if (l > 0)
         std::fill(lv.x.begin(), lv.x.end(), 0.0);
     if (c.nPre > 0)
+    {
         smooth(c, l, c.nPre);
+        c.cnt[1][l]++;
+    }
     if (l < lc)
     {
         for (int rep = 0; rep < levelReps(c, l, visits); rep++)
         {
             if (rep > 0 || l > 0)
+            {
                 residual(c, l, true);
+                c.cnt[2][l]++;
+            }
             restrictScaled(c, l);
+            c.cnt[3][l]++;
             cycleLevel(c, visits, l + 1);
             prolongAdd(c, l);
+            c.cnt[4][l]++;
             if (c.nPost > 0)
+            {
                 smooth(c, l, c.nPost);
+                c.cnt[1][l]++;
+            }
         }
     }
     else
     {
         if (c.coarseMode == ORACLE_COARSE_DIRECT)
+        {
             coarseSolve(c);
+            c.cnt[5][l]++;
+        }
         else
+        {
             smooth(c, l, std::max(c.nPost, 2));
+            c.cnt[1][l]++;
+        }
     }
     visits[l]++;
+    c.cnt[0][l]++;
+}
+
+void resetCounts(oracle_ctx& c)
+{
+    for (auto& v : c.cnt) v.assign(c.nLevels(), 0);
 }
 
 double sumSq(const Level& lv, const std::vector<double>& f)
@@ -1309,7 +1356,27 @@ void oracle_coarse_solve(oracle_ctx* c) { coarseSolve(*c); }
 void oracle_cycle(oracle_ctx* c)
 {
     std::vector<int> visits(c->nLevels(), 0);
+    resetCounts(*c);
     cycleLevel(*c, visits, 0);
+}
+
+void oracle_cycle_counts(const oracle_ctx* c, int what, long* out)
+{
+    for (int l = 0; l < c->nLevels(); l++)
+        out[l] = (what >= 0 && what < 6 && !c->cnt[what].empty())
+               ? c->cnt[what][l] : 0;
+}
+
+int oracle_set_boundary_values(oracle_ctx* c, int face, const double* v)
+{
+    if (face < 0 || face > 5) return -1;
+    if (!v) { c->bcarr[face].clear(); return 0; }
+    const int fd = face/2;
+    long n = 1;
+    for (int d = 0; d < 3; d++)
+        if (d != fd) n *= c->L[0].N[d];
+    c->bcarr[face].assign(v, v + n);
+    return 0;
 }
 
 void oracle_correct_eliminated(oracle_ctx* c)
@@ -1350,6 +1417,7 @@ void oracle_solve(oracle_ctx* c, int const_matrix, double stats[3])
     {
         if (iter < c->minIter) conv = 0;
         std::fill(visits.begin(), visits.end(), 0);
+        resetCounts(*c);
         cycleLevel(*c, visits, 0);
         residual(*c, 0, true);
         cur = std::sqrt(sumSq(l0, l0.r))/nf;

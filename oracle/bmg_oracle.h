@@ -33,7 +33,8 @@ enum { ORACLE_X = 0, ORACLE_B = 1, ORACLE_R = 2 };
 
 /* Build the level hierarchy and metrics from level-0 vertex coordinates
  * (n[d]+1 values per axis).  bc_value: Neumann gradient / Dirichlet value,
- * uniform per face, applied on level 0 only. */
+ * uniform per face, applied on level 0 only (per-cell data:
+ * oracle_set_boundary_values). */
 oracle_ctx* oracle_create(const int n[3], const double* vx, const double* vy,
                           const double* vz, const int bc[6],
                           const double bc_value[6]);
@@ -84,6 +85,17 @@ void oracle_prolong(oracle_ctx*, int l);      /* x[l] += P x[l+1] */
 void oracle_correct_bcs(oracle_ctx*, int l);  /* correctBoundaryConditions */
 void oracle_coarse_solve(oracle_ctx*);        /* APLU::solve */
 void oracle_cycle(oracle_ctx*);               /* one MG::cycle from level 0 */
+/* Per-level counts of the last cycle (oracle_cycle or the last iteration of
+ * oracle_solve): what = 0 MG::cycle calls (visits), 1 Smooth calls,
+ * 2 residuals, 3 restrictions, 4 prolongations, 5 coarse solves. */
+void oracle_cycle_counts(const oracle_ctx*, int what, long* out);
+/* Per-cell level-0 boundary data of a Neumann (gradient) or Dirichlet
+ * (value) face: the patch block of the face's boundary layer, shape E - S
+ * with the normal axis of size 1, index (ijk - S), last axis fastest
+ * (boundaryValues_[d], DirichletBoundaryConditionBase.H:30;
+ * boundaryGradients_[d], NeumannBoundaryConditionBase.H:30).  NULL: back to
+ * the uniform value of oracle_create.  Returns 0, or -1 for a bad face. */
+int oracle_set_boundary_values(oracle_ctx*, int face, const double* v);
 void oracle_correct_eliminated(oracle_ctx*);  /* x[0].correct<eliminatedBcs> */
 double oracle_sum_sq_residual(const oracle_ctx*);
 void oracle_norm_factor(const oracle_ctx*, double* nf);

@@ -7,7 +7,7 @@
 //
 //   test_poisson <dir>
 // reads  dir/shape.txt  "nx ny nz bc0..bc5 relTol coarseLevel stages lambda
-//                        ddt deltaT dtFrac"
+//                        ddt deltaT dtFrac [bcValue0..5]"
 //        dir/vertices.bin, dir/volumes.bin, dir/lambda.bin (3 face arrays,
 //        when lambda is 1), dir/src.bin (stages+1 padded sources),
 //        dir/xold.bin (optional: x.oldTime() of the first solve)
@@ -51,6 +51,11 @@ int main(int argc, char** argv)
     sh >> relTol >> coarseLevel >> stages >> useLambda >> ddt >> mesh.deltaT
        >> dtFrac;
     if (!sh) { std::fprintf(stderr, "bad shape.txt\n"); return 2; }
+    // optional: the six uniform boundary values (Neumann gradient /
+    // Dirichlet value per face)
+    double bv[6];
+    if (sh >> bv[0] >> bv[1] >> bv[2] >> bv[3] >> bv[4] >> bv[5])
+        for (int f = 0; f < 6; f++) mesh.bcValue[f] = bv[f];
     const size_t nv = size_t(mesh.N[0] + 1) + mesh.N[1] + 1 + mesh.N[2] + 1;
     std::vector<double> v = readBin(dir + "/vertices.bin", nv);
     size_t o = 0;

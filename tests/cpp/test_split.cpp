@@ -5,7 +5,8 @@
 //
 //   test_split <dir>
 // reads  dir/shape.txt  "nx ny nz bc0..bc5 inner nCorr prediction ddt
-//                        deltaT deltaT0 dtFrac relTol precond"
+//                        deltaT deltaT0 dtFrac relTol precond
+//                        [bcValue0..5]"
 //        (inner: FFT or MG; precond 1: a multigrid preconditioner solve), dir/vertices.bin, dir/volumes.bin,
 //        dir/lambda.bin (3 face arrays), dir/src.bin, dir/x0.bin,
 //        dir/xold.bin, dir/xoldold.bin (padded level-0 fields)
@@ -48,6 +49,11 @@ int main(int argc, char** argv)
     sh >> inner >> nCorr >> prediction >> ddt >> mesh.deltaT >> mesh.deltaT0
        >> dtFrac >> relTol >> precond;
     if (!sh) { std::fprintf(stderr, "bad shape.txt\n"); return 2; }
+    // optional: the six uniform boundary values (Neumann gradient /
+    // Dirichlet value per face)
+    double bv[6];
+    if (sh >> bv[0] >> bv[1] >> bv[2] >> bv[3] >> bv[4] >> bv[5])
+        for (int f = 0; f < 6; f++) mesh.bcValue[f] = bv[f];
     const size_t nv = size_t(mesh.N[0] + 1) + mesh.N[1] + 1 + mesh.N[2] + 1;
     std::vector<double> v = readBin(dir + "/vertices.bin", nv);
     size_t o = 0;

@@ -187,3 +187,60 @@ def test_batch_rejects_learned_and_mixed_outer_iterations(gpu):
         batch.close()
         for c in ctxs:
             c.close()
+
+
+def _oracle(p, rel_tol):
+    import oracle
+    o = oracle.Oracle(p.vertices, p.bc, p.bc_value)
+    o.set_opts(rel_tol=rel_tol, tolerance=0.0, max_iter=400, n_sweeps_pre=2,
+               n_sweeps_post=2, cycle=oracle.VCYCLE,
+               coarse_mode=oracle.COARSE_DIRECT, coarse_level=0)
+    o.assemble(p.lam)
+    o.set_field(oracle.B, 0, p.b)
+    it, r0, r1 = o.solve()
+    return o, it, r0, r1
+
+
+def _fp64_reference(p):
+    """Sparse LU of the eliminated system (oracle/direct.py), or for the
+    fully periodic 32^3 system (whose LU fills in) the oracle's FP64 MG
+    to 1e-13."""
+    import oracle
+    from oracle import direct
+    if p.bc == (P,) * 6 and min(p.n) >= 32:
+        o, _, _, r1 = _oracle(p, 1e-13)
+        assert r1 <= 1e-13
+        return o.field(oracle.X, 0)[1:-1, 1:-1, 1:-1]
+    o = oracle.Oracle(p.vertices, p.bc, p.bc_value)
+    o.assemble(p.lam)
+    o.pin_and_eliminate()
+    return direct.solve_sparse(o.stencil(0), p.b, p.bc)
+
+
+def test_batch_against_the_oracle(gpu):
+    """The batched solve against the CPU oracle, system by system: the FP64
+    systems bitwise equal to the oracle's MG::solve (x, returned b,
+    iteration count); the mixed-precision systems, and every system of the
+    batched flexible CG, within 1e-7 of an FP64 direct solve."""
+    import oracle
+    from oracle import direct
+    runs, _ = solve_batched(SYSTEMS)
+    fruns, _ = solve_batched(FCG_SYSTEMS, krylov=bmg.KRYLOV_FCG)
+    for tag, systems, run in (("mg", SYSTEMS, runs[0]),
+                              ("fcg", FCG_SYSTEMS, fruns[0])):
+        for k, (i, shape, faces, prec) in enumerate(systems):
+            p = problem(i, shape, faces)
+            got = run[k]
+            print(f"[batch vs oracle] {tag} system {k} {shape} {p.bc} "
+                  f"prec={prec}", flush=True)
+            if tag == "mg" and prec == bmg.FP64:
+                o, it, r0, r1 = _oracle(p, 1e-8)
+                assert got[2] == it, (k, got[2], it)
+                assert np.array_equal(got[0], o.field(oracle.X, 0)), k
+                assert np.array_equal(got[1], o.field(oracle.B, 0)), k
+            xe = _fp64_reference(p)
+            err = direct.rel_error(got[0][1:-1, 1:-1, 1:-1], xe,
+                                   D not in p.bc)
+            print(f"[batch vs oracle] {tag} system {k}: {got[2]} iterations,"
+                  f" error vs FP64 direct {err:.2e}", flush=True)
+            assert err < 1e-7, (tag, k, err)

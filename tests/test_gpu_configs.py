@@ -1,0 +1,132 @@
+"""Every workload at its stated size and settings on the GPU.
+
+* C1 -- 64^3 unit cube, all Neumann, V(2,2) red-black Gauss-Seidel, linear
+  transfers, coarsest 4^3 solved directly (coarseLevel 1), relTol 1e-10:
+  bitwise against the oracle on both operator inputs (x, returned b,
+  iteration count), and within 1e-9 of the exact DCT solution.
+* C2 -- 256^3 channel (periodic x,z, Neumann y, tanh beta = 2): plain FP64
+  MG and FCG with the mixed-precision cycle against the host FFT(x,z) +
+  Thomas(y) direct solve of the same finite-volume system (the reference
+  FFT solver's algorithm, oracle/direct.py).
+* C3 -- 192^3 variable-coefficient spheres (rho 1000 / 1): mixed MG and FCG
+  to 1e-8 with their error against an FP64 reference solve (FP64 FCG to
+  1e-13), whose residual the CPU oracle recomputes independently.
+* The 256^3 spheres reference of the headline benchmark, checked the same
+  way.
+Sizes are the workloads' own; each test prints what it runs before it runs.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from oracle import direct
+from briscola_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+C1 = dict(rel_tol=1e-10, tolerance=0.0, n_sweeps_pre=2, n_sweeps_post=2,
+          cycle=oracle.VCYCLE, coarse_mode=oracle.COARSE_DIRECT,
+          coarse_level=1, smoother=oracle.RBGS)
+MG = dict(tolerance=0.0, n_sweeps_pre=2, n_sweeps_post=2,
+          cycle=oracle.VCYCLE, coarse_mode=oracle.COARSE_DIRECT,
+          coarse_level=1)
+
+
+def inner(a):
+    return a[1:-1, 1:-1, 1:-1]
+
+
+@pytest.mark.parametrize("native", [False, True], ids=["stencil", "native"])
+def test_c1_full_size_bitwise(gpu, native):
+    p = W.uniform_neumann(64, seed=0)
+    print(f"[C1] 64^3 all-Neumann V(2,2) 4^3 direct relTol 1e-10 "
+          f"native={native}", flush=True)
+    o = oracle.Oracle(p.vertices, p.bc, p.bc_value)
+    o.set_opts(**C1)
+    o.assemble(None)
+    assert o.shapes[-2] == (4, 4, 4)
+    ctx = gpu.Context(p.vertices, p.bc, p.bc_value, **C1)
+    if native:
+        ctx.set_laplacian(None)
+    else:
+        for l in range(o.nlevels):
+            ctx.set_stencil(l, o.stencil(l))
+    o.set_field(oracle.B, 0, p.b)
+    it, r0, r1 = o.solve()
+    b = p.b.copy()
+    x, st = ctx.solve(b)
+    assert st.iters == it and r1 <= 1e-10
+    assert np.array_equal(x, o.field(oracle.X, 0))
+    assert np.array_equal(b, o.field(oracle.B, 0))
+    err = direct.rel_error(inner(x), direct.solve_dct(p.b, 1.0 / 64), True)
+    print(f"[C1] {it} iterations, error vs DCT {err:.2e}", flush=True)
+    assert err < 1e-9
+    ctx.close()
+
+
+def _solve(gpu, p, **opts):
+    ctx = gpu.Context(p.vertices, p.bc, p.bc_value, **opts)
+    ctx.set_laplacian(p.lam)
+    x, st = ctx.solve(p.b.copy())
+    ctx.close()
+    return x, st
+
+
+def test_c2_channel256_vs_fft_thomas(gpu):
+    p = W.channel(256, seed=0)
+    print("[C2] 256^3 channel: host FFT+Thomas reference", flush=True)
+    xe = direct.solve_fft_thomas(p.b, p.vertices)
+    for name, opts in (
+            ("fp64 MG relTol 1e-8", dict(MG, rel_tol=1e-8, max_iter=2000)),
+            ("FCG+mixed relTol 1e-10",
+             dict(MG, rel_tol=1e-10, max_iter=400, precision=gpu.MIXED,
+                  krylov=gpu.KRYLOV_FCG))):
+        print(f"[C2] {name}", flush=True)
+        x, st = _solve(gpu, p, **opts)
+        err = direct.rel_error(inner(x), xe, True)
+        print(f"[C2] {name}: {st.iters} iterations, final residual "
+              f"{st.final_residual:.2e}, error vs FFT+Thomas {err:.2e}",
+              flush=True)
+        assert st.final_residual <= opts["rel_tol"]
+        assert err < (1e-9 if opts["rel_tol"] < 1e-9 else 1e-7)
+
+
+def _oracle_residual(p, x):
+    """||b - A x|| / ||b|| recomputed by the CPU oracle (its own assembly
+    and ghost rules, FP64)."""
+    o = oracle.Oracle(p.vertices, p.bc, p.bc_value)
+    o.assemble(p.lam)
+    o.set_field(oracle.B, 0, p.b)
+    o.set_field(oracle.X, 0, x)
+    o.pin_and_eliminate()
+    o.residual(0, True)
+    return (np.linalg.norm(inner(o.field(oracle.R, 0)))
+            / np.linalg.norm(inner(p.b)))
+
+
+@pytest.mark.parametrize("n", [192, 256])
+def test_spheres_fp64_reference_and_mixed_error(gpu, n):
+    """C3 at 192^3 (and the headline's 256^3): the FP64 reference solve is
+    confirmed by the oracle's residual; the mixed-precision solves to 1e-8
+    state their error against it."""
+    p = W.spheres(n, seed=0)
+    print(f"[C3] {n}^3 spheres: FP64 FCG reference to 1e-13", flush=True)
+    xr, st = _solve(gpu, p, **dict(MG, rel_tol=1e-13, max_iter=600,
+                                   krylov=gpu.KRYLOV_FCG))
+    rr = _oracle_residual(p, xr)
+    print(f"[C3] reference: {st.iters} FCG iterations, oracle residual "
+          f"{rr:.2e}", flush=True)
+    assert rr < 1e-11
+    for name, opts in (
+            ("mixed MG", dict(MG, rel_tol=1e-8, max_iter=2000,
+                              precision=gpu.MIXED)),
+            ("FCG+mixed", dict(MG, rel_tol=1e-8, max_iter=400,
+                               precision=gpu.MIXED, krylov=gpu.KRYLOV_FCG))):
+        x, st = _solve(gpu, p, **opts)
+        err = direct.rel_error(inner(x), inner(xr), True)
+        print(f"[C3] {n}^3 {name}: {st.iters} iterations, error vs FP64 "
+              f"{err:.2e}", flush=True)
+        assert st.final_residual <= 1e-8
+        # the density ratio 1000 conditions the error (about 1e-5 per 1e-8
+        # of relative residual for MG, 3e-7 for FCG at 256^3)
+        assert err < 1e-4

@@ -68,7 +68,8 @@ def write_inputs(tmp_path, prob, lam, V, src, ddt, rel_tol="1e-9",
         f.write(" ".join(str(q) for q in prob.bc) +
                 f" {rel_tol} {coarse_level} {len(src) - 1} "
                 f"{int(lam is not None)} {int(ddt is not None)} "
-                f"{dt!r} {frac!r}\n")
+                f"{dt!r} {frac!r} " +
+                " ".join(repr(float(v)) for v in prob.bc_value) + "\n")
     np.concatenate(prob.vertices).tofile(tmp_path / "vertices.bin")
     V.tofile(tmp_path / "volumes.bin")
     if lam is not None:
@@ -88,14 +89,33 @@ def run_driver(tmp_path):
 DDT = (2e-3, 0.5)
 
 
+def _with_values(prob, bc, values):
+    prob.bc = tuple(bc)
+    prob.bc_value = tuple(float(v) for v in values)
+    return prob
+
+
 @pytest.mark.parametrize("case", ["spheres", "channel", "spheres_ddt",
-                                  "channel_ddt"])
+                                  "channel_ddt", "channel_bc", "mixed_bc",
+                                  "mixed_bc_ddt"])
 def test_poisson_solver_adapter(gpu, tmp_path, case):
+    """Inhomogeneous cases (_bc): Neumann gradients (antisymmetric on the
+    walls of the singular channel) and Dirichlet values; the adapter builds
+    the system anew for every solve, so every solve, also with the matrix
+    kept, folds them into b -- each solve is also checked bitwise against
+    a freshly assembled oracle system."""
     if case.startswith("spheres"):
         prob = W.spheres(16, seed=3, nspheres=4)
         lam = prob.lam
+    elif case.startswith("mixed"):
+        prob = _with_values(W.channel(16, seed=6),
+                            (W.PERIODIC, W.PERIODIC, W.DIRICHLET, W.NEUMANN,
+                             W.PERIODIC, W.PERIODIC), (0, 0, 1.5, -0.75, 0, 0))
+        lam = None
     else:
         prob = W.channel(16, seed=5)
+        if case.endswith("_bc"):
+            prob = _with_values(prob, prob.bc, (0, 0, -0.8, 0.8, 0, 0))
         lam = None
     ddt = DDT if case.endswith("_ddt") else None
     print(f"[poisson] case={case}", flush=True)
@@ -119,17 +139,29 @@ def test_poisson_solver_adapter(gpu, tmp_path, case):
     stats = [l.split() for l in open(tmp_path / "stats.txt")]
     stages = [np.zeros(prob.padded) for _ in range(STAGES)]
     xold = np.zeros(prob.padded)
+    inhom = any(v != 0.0 for v in prob.bc_value)
     for k in range(STAGES + 1):
         stage = k if k < STAGES else 0
         b = 0.0 - V * src[k]
         if ddt is not None:
             b = b - ((V * xold) / ddt[0]) / ddt[1]
         x0 = stages[stage].copy()
+        if inhom:
+            o = oracle.Oracle(prob.vertices, prob.bc, prob.bc_value)
+            o.set_opts(rel_tol=1e-9, tolerance=0.0, n_sweeps_pre=2,
+                       n_sweeps_post=2, cycle=oracle.VCYCLE, coarse_level=1)
+            o.assemble(lam, ddt=ddt)
+            o.set_field(oracle.B, 0, b)
+            o.set_field(oracle.X, 0, x0)
+            o.solve()
+        ctx.renew_source_terms()
         x, st = ctx.solve(b, x0)
         stages[stage] = x.copy()
         xold = x.copy()
         xg = np.fromfile(tmp_path / f"x{k}.bin").reshape(prob.padded)
         assert np.array_equal(xg, x), f"solve {k}"
+        if inhom:
+            assert np.array_equal(x, o.field(oracle.X, 0)), f"oracle {k}"
         assert int(stats[k][0]) == st.iters
         fl = np.fromfile(tmp_path / f"flux{k}.bin").reshape((3,) + prob.padded)
         ref = flux_ref(xg, lam, w, dl, n)

@@ -44,7 +44,12 @@ def face_max(lam, n):
     return m
 
 
-def problem(case, n=16):
+# nonzero Dirichlet values (y walls) and Neumann gradients (z walls) of the
+# "smooth" problem's faces
+BC_VALUES = (0.0, 0.0, 0.5, -1.0, 0.3, -0.2)
+
+
+def problem(case, n=16, bc_value=(0.0,) * 6):
     rng = np.random.default_rng(21)
     if case == "spheres":
         p = W.spheres(n, seed=2, nspheres=3, rho_out=10.0)
@@ -55,7 +60,7 @@ def problem(case, n=16):
               W.NEUMANN, W.NEUMANN)
         b = np.zeros((n + 2,) * 3)
         b[1:-1, 1:-1, 1:-1] = rng.uniform(-1, 1, (n,) * 3)
-        p = W.Problem("smooth", verts, bc, b)
+        p = W.Problem("smooth", verts, bc, b, bc_value=bc_value)
         c = [np.linspace(0, 1, n + 2)] * 3
         base = 0.75 + 0.25 * np.sin(2 * np.pi * c[0][:, None, None]) * \
             np.cos(np.pi * c[1][None, :, None]) * np.ones((1, 1, n + 2))
@@ -89,7 +94,8 @@ def run_adapter(tmp_path, prob, lam, inner, ncorr, prediction, ddt, x0, xold,
         f.write(" ".join(str(q) for q in prob.bc) +
                 f" {inner} {ncorr} {'true' if prediction else 'false'} "
                 f"{int(ddt is not None)} {dt!r} {dt0!r} {frac!r} {rel_tol} "
-                f"{int(precond)}\n")
+                f"{int(precond)} " +
+                " ".join(repr(float(v)) for v in prob.bc_value) + "\n")
     w, _ = metrics(prob)
     V = w[0][:, None, None] * w[1][None, :, None] * w[2][None, None, :]
     np.concatenate(prob.vertices).tofile(tmp_path / "vertices.bin")
@@ -155,6 +161,8 @@ def test_split_adapter_equals_composed_steps(gpu, tmp_path, inner, mode):
             bs = 0.0 - V * bhr
             if ddt is not None:
                 bs = bs - ((V * xold) / ddt[0]) / ddt[2]
+            # the inner defaultPoissonSolver assembles its system anew
+            solver.renew_source_terms()
             xr, _ = solver.solve(bs, xr)
     assert np.array_equal(bh, bhr)
     assert np.array_equal(x, xr)
@@ -181,4 +189,88 @@ def test_split_iteration_converges_to_variable_coefficient_solution(gpu, tmp_pat
     ctx.close()
     xi, mi = x[1:-1, 1:-1, 1:-1], xm[1:-1, 1:-1, 1:-1]
     err = np.abs(xi - mi).max() / np.abs(mi).max()
+    assert err < 1e-8, err
+
+
+def test_split_source_with_boundary_data(gpu):
+    """Nonzero Dirichlet values and Neumann gradients: the source reads the
+    ghosts of x, here set by the oracle's correctBoundaryConditions from
+    that data, bitwise against the restatement."""
+    prob, lam = problem("smooth", bc_value=BC_VALUES)
+    w, dl = metrics(prob)
+    o = oracle.Oracle(prob.vertices, prob.bc, prob.bc_value)
+    x = np.random.default_rng(4).standard_normal(prob.padded)
+    o.set_field(oracle.X, 0, x)
+    o.correct_bcs(0)
+    x = o.field(oracle.X, 0)
+    assert not np.array_equal(x[1:-1, 0, 1:-1], -x[1:-1, 1, 1:-1])
+    l0 = face_max(lam, prob.n)
+    ctx = gpu.Context(prob.vertices, prob.bc, prob.bc_value, rel_tol=1e-8)
+    bh, ph = ctx.split_source(x, lam, l0, prob.b)
+    rbh, rph = S.source(x, lam, l0, prob.b, w, dl)
+    assert np.array_equal(bh, rbh)
+    for d in range(3):
+        assert np.array_equal(ph[d], rph[d])
+    ctx.close()
+
+
+@pytest.mark.parametrize("inner", ["FFT", "MG"])
+def test_split_adapter_with_boundary_data(gpu, tmp_path, inner):
+    """The adapter with nonzero boundary data equals the composed steps
+    bitwise (every inner multigrid solve a newly assembled system)."""
+    prob, lam = problem("smooth", bc_value=BC_VALUES)
+    print(f"[split adapter bc] inner={inner}", flush=True)
+    z = np.zeros(prob.padded)
+    x, bh, fl, V = run_adapter(tmp_path, prob, lam, inner, 3, False, None, z,
+                               z, z)
+    l0 = face_max(lam, prob.n)
+    geo = gpu.Context(prob.vertices, prob.bc, prob.bc_value)
+    if inner == "FFT":
+        solver = gpu.FFT(prob.vertices, prob.bc, prob.bc_value)
+    else:
+        solver = gpu.Context(prob.vertices, prob.bc, prob.bc_value,
+                             rel_tol=1e-10, tolerance=0.0, n_sweeps_pre=2,
+                             n_sweeps_post=2, cycle=bmg.VCYCLE, coarse_level=1)
+        solver.set_laplacian(None)
+    xr = z.copy()
+    for corr in range(3):
+        bhr, ph = geo.split_source(xr, lam, l0, prob.b)
+        if inner == "FFT":
+            solver.solve(bhr, xr)
+        else:
+            solver.renew_source_terms()
+            xr, _ = solver.solve(0.0 - V * bhr, xr)
+    assert np.array_equal(bh, bhr)
+    assert np.array_equal(x, xr)
+    geo.close()
+    solver.close()
+
+
+@pytest.mark.parametrize("inner", ["FFT", "MG"])
+def test_split_with_boundary_data_converges(gpu, tmp_path, inner):
+    """With nonzero Dirichlet values and Neumann gradients the split
+    iteration (FFT or multigrid inner solves) converges to the multigrid
+    solution of div(lambda grad x) = -b with the same boundary data, which
+    is itself bitwise the oracle's."""
+    prob, lam = problem("smooth", n=16, bc_value=BC_VALUES)
+    z = np.zeros(prob.padded)
+    x, _, _, V = run_adapter(tmp_path, prob, lam, inner, 80, False, None, z,
+                             z, z, rel_tol="1e-12")
+    ctx = gpu.Context(prob.vertices, prob.bc, prob.bc_value, rel_tol=1e-12,
+                      tolerance=0.0, n_sweeps_pre=2, n_sweeps_post=2,
+                      cycle=bmg.VCYCLE, coarse_level=1, max_iter=200)
+    ctx.set_laplacian(lam)
+    xm, st = ctx.solve(0.0 - V * prob.b)
+    ctx.close()
+    o = oracle.Oracle(prob.vertices, prob.bc, prob.bc_value)
+    o.set_opts(rel_tol=1e-12, tolerance=0.0, n_sweeps_pre=2, n_sweeps_post=2,
+               cycle=oracle.VCYCLE, coarse_level=1, max_iter=200)
+    o.assemble(lam)
+    o.set_field(oracle.B, 0, 0.0 - V * prob.b)
+    o.solve()
+    assert np.array_equal(xm, o.field(oracle.X, 0))
+    xi, mi = x[1:-1, 1:-1, 1:-1], xm[1:-1, 1:-1, 1:-1]
+    err = np.abs(xi - mi).max() / np.abs(mi).max()
+    print(f"[split bc] inner={inner}: 80 corrections, error vs MG {err:.2e}",
+          flush=True)
     assert err < 1e-8, err

@@ -128,6 +128,17 @@ def test_learned_fcg_reaches_tolerance(gpu, precision):
     err = np.linalg.norm((xi - xi.mean()) - (ri - ri.mean())) / np.linalg.norm(ri - ri.mean())
     assert err < 1e-5
     assert st.learned_fallback != -1 or st.learned_iters == st.iters
+    # and against an FP64 direct solve of the oracle's eliminated system
+    import oracle
+    from oracle import direct
+    o = oracle.Oracle(prob.vertices, prob.bc, prob.bc_value)
+    o.assemble(prob.lam)
+    o.pin_and_eliminate()
+    xs = direct.solve_sparse(o.stencil(0), prob.b, prob.bc)
+    err_d = direct.rel_error(xi, xs, True)
+    print(f"[learned fcg] precision={precision}: {st.iters} iterations, "
+          f"error vs sparse direct {err_d:.2e}", flush=True)
+    assert err_d < 1e-5
 
 
 def guess_error(net, prob, xr):

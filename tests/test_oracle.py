@@ -745,3 +745,273 @@ def test_lexgs_is_textbook_symmetric_gauss_seidel():
     o.smooth(0, 1)
     rb = np.array([o.field(oracle.X, 0)[c] for c in cells])
     assert np.abs(rb - y).max() > 1e-6
+
+
+# ------------------------------------------- cycle structure (MG.H:106-124)
+
+def _expected_counts(cycle, L, lc, pre, post, direct_coarse):
+    """Closed forms of one MG::cycle from level 0 over levels 0..lc.
+
+    levelReps (MG.H:106-124): V 1; W 2 for l > 0; F 2 on the first visit of
+    l > 0 in the cycle, then 1.  Visits of level l+1 = sum over the visits
+    of l of their repetitions, so: V 1 everywhere; W 1, 1, 2, 4, ...
+    (2^(l-1) for l >= 1); F 1, 1, 2, 3, ... (l for l >= 1)."""
+    if cycle == oracle.VCYCLE:
+        v = [1] * (lc + 1)
+    elif cycle == oracle.WCYCLE:
+        v = [1] + [2 ** (l - 1) for l in range(1, lc + 1)]
+    else:
+        v = [1] + [l for l in range(1, lc + 1)]
+    v = v + [0] * (L - lc - 1)
+    z = [0] * L
+    smooth = list(z)
+    resid = list(z)
+    restr = list(z)
+    prol = list(z)
+    coarse = list(z)
+    for l in range(lc):
+        restr[l] = prol[l] = v[l + 1]
+        # the first repetition of level 0 reuses the outer residual
+        # (MG.C:72-73), and level 0 has one repetition in every cycle type
+        resid[l] = v[l + 1] if l > 0 else 0
+        smooth[l] = v[l] * (pre > 0) + v[l + 1] * (post > 0)
+    if direct_coarse:
+        coarse[lc] = v[lc]
+        smooth[lc] = v[lc] * (pre > 0)
+    else:
+        smooth[lc] = v[lc] * ((pre > 0) + 1)
+    return dict(visits=v, smooth=smooth, residual=resid, restrict=restr,
+                prolong=prol, coarse_solve=coarse)
+
+
+@pytest.mark.parametrize("cycle", [oracle.VCYCLE, oracle.WCYCLE, oracle.FCYCLE])
+@pytest.mark.parametrize("coarse_level,mode,pre", [
+    (0, oracle.COARSE_DIRECT, 2), (1, oracle.COARSE_DIRECT, 0),
+    (0, oracle.COARSE_SMOOTH, 1)])
+def test_cycle_counts_closed_form(cycle, coarse_level, mode, pre):
+    """Per-level visits, smoothings, residuals, transfers and coarse solves
+    of one cycle against the closed forms of levelReps (a doubled level 0 or
+    a flipped first-visit test changes them)."""
+    p = W.uniform_neumann(32, seed=0)           # levels 32,16,8,4,2
+    o = make(p.vertices, p.bc, rel_tol=1e-8, n_sweeps_pre=pre,
+             n_sweeps_post=2, cycle=cycle, coarse_mode=mode,
+             coarse_level=coarse_level)
+    assert o.nlevels == 5
+    o.set_field(oracle.B, 0, p.b)
+    o.pin_and_eliminate()
+    if mode == oracle.COARSE_DIRECT:
+        o.prepare_coarse()
+    o.residual(0, False)
+    o.cycle()
+    lc = o.nlevels - 1 - coarse_level
+    exp = _expected_counts(cycle, o.nlevels, lc, pre, 2,
+                           mode == oracle.COARSE_DIRECT)
+    got = o.cycle_counts()
+    assert got == exp, (got, exp)
+    # a whole solve repeats the same structure in every iteration
+    o2 = make(p.vertices, p.bc, rel_tol=1e-6, n_sweeps_pre=pre,
+              n_sweeps_post=2, cycle=cycle, coarse_mode=mode,
+              coarse_level=coarse_level)
+    o2.set_field(oracle.B, 0, p.b)
+    it, _, _ = o2.solve()
+    assert it >= 2 and o2.cycle_counts() == exp
+
+
+def test_w_f_visits_written_out():
+    """The closed forms above, written out for a 6-level hierarchy."""
+    e = _expected_counts(oracle.WCYCLE, 6, 5, 2, 2, True)["visits"]
+    assert e == [1, 1, 2, 4, 8, 16]
+    e = _expected_counts(oracle.FCYCLE, 6, 5, 2, 2, True)["visits"]
+    assert e == [1, 1, 2, 3, 4, 5]
+
+
+# -------------------------------------------- normFactors (solver.C:13-61)
+
+def _mixed_bc_problem(seed):
+    v = [stretched(10, 1.2), np.linspace(0, 2, 9), np.linspace(-1, 1, 13)]
+    bc = (D, N, P, P, N, D)
+    o = oracle.Oracle(v, bc, (0.5, -0.25, 0, 0, 0.75, 1.5))
+    o.set_opts(rel_tol=1e-8, n_sweeps_pre=2, n_sweeps_post=2,
+               cycle=oracle.VCYCLE, coarse_level=0)
+    o.assemble()
+    rng = np.random.default_rng(seed)
+    b = rng.uniform(-1, 1, o.padded(0))
+    return o, b, rng
+
+
+def test_norm_factor_constant_guess_is_residual_norm():
+    """x0 = y constant (ghosts too): the residual without BC update is
+    r = b - rowSum(A) y = b - Ay, so |r - b + Ay| = 0 and normFactor =
+    ||r||, the initial normalised residual is 1.  A sign slip in either
+    term (r + b - Ay, r - b - Ay, b + Ay) or a dropped Ay breaks both."""
+    o, b, _ = _mixed_bc_problem(3)
+    o.set_field(oracle.B, 0, b)
+    o.set_field(oracle.X, 0, np.full(o.padded(0), 0.75))
+    o.pin_and_eliminate()
+    o.residual(0, False)
+    r = inner(o.field(oracle.R, 0))
+    A = inner(o.stencil(0))
+    assert np.abs(A.sum(axis=-1)).max() > 0.1     # Dirichlet rows: Ay != 0
+    assert o.norm_factor() == pytest.approx(np.linalg.norm(r), rel=1e-13)
+    it, r0, r1 = o.solve()
+    assert r0 == pytest.approx(1.0, rel=1e-13)
+
+
+def test_norm_factor_nonconstant_guess():
+    """Non-constant x0 on a Dirichlet+Neumann+periodic stretched mesh:
+    sqrt(sum (|r - b + rowSum(A) ybar| + |b - rowSum(A) ybar|)^2) with ybar
+    the mean of the interior of x, the operator the eliminated stencil
+    (pinned above) and r = b - A x0 (pinned by the dense product); and the
+    solve's initial normalised residual is ||r|| over it."""
+    o, b, rng = _mixed_bc_problem(5)
+    x0 = rng.uniform(-2, 3, o.padded(0))
+    o.set_field(oracle.B, 0, b)
+    o.set_field(oracle.X, 0, x0)
+    o.pin_and_eliminate()
+    o.residual(0, False)
+    r = inner(o.field(oracle.R, 0))
+    A = inner(o.stencil(0))
+    bi = inner(o.field(oracle.B, 0))
+    ybar = inner(x0).mean()
+    Ay = A.sum(axis=-1) * ybar
+    nf = math.sqrt(float(((np.abs(r - bi + Ay) + np.abs(bi - Ay)) ** 2).sum()))
+    assert o.norm_factor() == pytest.approx(nf, rel=1e-13)
+    # with x0 = 0 the same formula gives ||b|| (the zero-guess test above)
+    # and with the guess it differs from it
+    assert abs(nf - np.linalg.norm(bi)) > 1e-3 * nf
+    o2, _, _ = _mixed_bc_problem(5)
+    o2.set_field(oracle.B, 0, b)
+    o2.set_field(oracle.X, 0, x0)
+    it, r0, r1 = o2.solve()
+    assert r0 == pytest.approx(np.linalg.norm(r) / nf, rel=1e-12)
+
+
+def test_norm_factor_floor():
+    """normFactor >= cells * 1e-12 (solver.C:55-58): a zero system."""
+    v = [np.linspace(0, 1, 5)] * 3
+    o = make(v, (N,) * 6)
+    o.set_field(oracle.B, 0, np.zeros(o.padded(0)))
+    o.pin_and_eliminate()
+    o.residual(0, False)
+    assert o.norm_factor() == 64 * 1e-12
+
+
+# --------------------------- inhomogeneous boundary data (Neumann, Dirichlet)
+
+def test_neumann_gradient_linear_solution():
+    """Neumann gradient g on every face (outward normal derivative): the
+    finite-volume system with the wall flux A_f g (b -= A_g g / delta,
+    NeumannBoundaryCondition.C:40-54) reproduces a linear field exactly on a
+    stretched mesh, and the exit ghost refresh gives x_g = x_P + g/delta
+    with 1/delta the boundary cell width (NeumannBoundaryCondition.C:76-84,
+    mirrored ghost centres)."""
+    a, c, e = 0.7, -1.3, 2.1
+    v = [stretched(12, 1.3), np.linspace(0, 2, 9), stretched(10, 0.8)]
+    o = oracle.Oracle(v, (N,) * 6, (-a, a, -c, c, -e, e))
+    o.set_opts(rel_tol=1e-13, n_sweeps_pre=2, n_sweeps_post=2,
+               cycle=oracle.VCYCLE, coarse_level=0, max_iter=400)
+    o.assemble()
+    o.set_field(oracle.B, 0, np.zeros(o.padded(0)))
+    it, r0, r1 = o.solve()
+    assert r1 <= 1e-13
+    X, Y, Z = [o.centres(0, d)[1:-1] for d in range(3)]
+    xe = a * X[:, None, None] + c * Y[None, :, None] + e * Z[None, None, :]
+    x = o.field(oracle.X, 0)
+    xi = inner(x)
+    assert np.abs((xi - xi.mean()) - (xe - xe.mean())).max() < 1e-9
+    w0 = o.widths(0, 0)
+    assert np.allclose(x[0, 1:-1, 1:-1], x[1, 1:-1, 1:-1] + (-a) * w0[1],
+                       rtol=0, atol=1e-12)
+    assert np.allclose(x[-1, 1:-1, 1:-1], x[-2, 1:-1, 1:-1] + a * w0[-2],
+                       rtol=0, atol=1e-12)
+
+
+def _ghosted(o, u, bc, vals):
+    """Padded field with interior u and ghosts by the boundary rules of
+    the data vals[f] (patch arrays): Dirichlet 2v - u_P, Neumann u_P + g w
+    (w the boundary cell width), periodic wrap."""
+    x = np.zeros(o.padded(0))
+    x[1:-1, 1:-1, 1:-1] = u
+    for f in range(6):
+        d, hi = f // 2, f % 2
+        sl_g = [slice(1, -1)] * 3
+        sl_p = [slice(1, -1)] * 3
+        sl_g[d] = -1 if hi else 0
+        if bc[f] == P:
+            sl_p[d] = 1 if hi else -2
+            x[tuple(sl_g)] = x[tuple(sl_p)]
+            continue
+        sl_p[d] = -2 if hi else 1
+        val = np.squeeze(vals[f], axis=d)
+        if bc[f] == D:
+            x[tuple(sl_g)] = 2.0 * val - x[tuple(sl_p)]
+        else:
+            w = o.widths(0, d)[-2 if hi else 1]
+            x[tuple(sl_g)] = x[tuple(sl_p)] + val * w
+    return x
+
+
+def _apply_full(A, x):
+    c = x[1:-1, 1:-1, 1:-1]
+    terms = [c, x[:-2, 1:-1, 1:-1], x[2:, 1:-1, 1:-1], x[1:-1, :-2, 1:-1],
+             x[1:-1, 2:, 1:-1], x[1:-1, 1:-1, :-2], x[1:-1, 1:-1, 2:]]
+    acc = A[..., 0] * terms[0]
+    for s in range(1, 7):
+        acc = acc + A[..., s] * terms[s]
+    return acc
+
+
+@pytest.mark.parametrize("bc", [(D, N, P, P, N, D), (N, D, D, N, P, P),
+                                (D, D, N, N, D, N)])
+def test_nonuniform_boundary_data_manufactured(bc):
+    """Per-cell boundary data in the patch-block layout (index ijk - S,
+    DirichletBoundaryCondition.C:45, NeumannBoundaryCondition.C:51): a
+    random interior field u with ghosts set by the boundary rules from
+    random per-cell data, and b = A u with the un-eliminated operator, is
+    recovered by the solve with that data (a transposed or shifted patch
+    index changes b's boundary terms and fails).  Non-square patches."""
+    v = [stretched(10, 1.1), np.linspace(0, 1.5, 9), np.linspace(0, 1, 7)]
+    o = oracle.Oracle(v, bc)
+    o.set_opts(rel_tol=1e-13, n_sweeps_pre=2, n_sweeps_post=2,
+               cycle=oracle.VCYCLE, coarse_level=0, max_iter=600)
+    o.assemble()
+    rng = np.random.default_rng(11)
+    u = rng.uniform(-1, 1, o.shapes[0])
+    vals = {f: rng.uniform(-2, 2, o.boundary_shape(f)) for f in range(6)
+            if bc[f] != P}
+    for f, a in vals.items():
+        o.set_boundary_values(f, a)
+    A = inner(o.stencil(0))          # assembled, before elimination
+    xg = _ghosted(o, u, bc, vals)
+    b = np.zeros(o.padded(0))
+    b[1:-1, 1:-1, 1:-1] = _apply_full(A, xg)
+    o.set_field(oracle.B, 0, b)
+    it, r0, r1 = o.solve()
+    assert r1 <= 1e-13
+    x = o.field(oracle.X, 0)
+    assert np.abs(inner(x) - u).max() < 1e-9
+    # the exit refresh of the eliminated ghosts follows the same rules
+    xr = _ghosted(o, inner(x), bc, vals)
+    for f in range(6):
+        if bc[f] == P:
+            continue
+        d, hi = f // 2, f % 2
+        sl = [slice(1, -1)] * 3
+        sl[d] = -1 if hi else 0
+        assert np.allclose(x[tuple(sl)], xr[tuple(sl)], rtol=0, atol=1e-12)
+    # uniform data through the arrays is the uniform-value solve, bitwise
+    o3 = oracle.Oracle(v, bc, tuple(0.0 if bc[f] == P else 0.3 * (f + 1)
+                                    for f in range(6)))
+    o4 = oracle.Oracle(v, bc)
+    for q in (o3, o4):
+        q.set_opts(rel_tol=1e-10, n_sweeps_pre=2, n_sweeps_post=2,
+                   cycle=oracle.VCYCLE, coarse_level=0)
+        q.assemble()
+        q.set_field(oracle.B, 0, b)
+    for f in range(6):
+        if bc[f] != P:
+            o4.set_boundary_values(f, np.full(o4.boundary_shape(f), 0.3 * (f + 1)))
+    s3, s4 = o3.solve(), o4.solve()
+    assert s3 == s4
+    assert np.array_equal(o3.field(oracle.X, 0), o4.field(oracle.X, 0))
+    assert np.array_equal(o3.field(oracle.B, 0), o4.field(oracle.B, 0))
