@@ -87,3 +87,141 @@ def conv_wgrad(x, dz, kp):
         xs = shifted(x.astype(np.float64), a, b, c).reshape(-1, cin)
         dw[:, t * cin:(t + 1) * cin] = dzf.T @ xs
     return dw
+
+
+# ---------------------------------------------------------------------------
+# The whole network (briscola_amd/csrc/unet.hip), float64, for the gradient
+# check of a training step.  Parameter layout of bmg_unet_get_params: for
+# each convolution E0, E1, E2, E3, D2, D1, D0 the weights (cout, kp) then
+# the bias (cout); then the head's 16 weights and its bias.  Channel order
+# of the decoder inputs: [upsampled | skip].
+#
+# bf16=True rounds to BF16 wherever the device stores BF16 (the packed
+# input, every convolution, pooling and upsampling output, the head's input
+# gradient, the data gradients, the pooled and upsampled gradients; the
+# convolution weights are the BF16 copies of the parameters), so that the
+# device result differs from it by FP32 accumulation order only;
+# bf16=False is the exact float64 function of the parameters, which the
+# tests pin by central differences.
+
+# (cin padded, cout, cin real, level)
+CONVS = [(8, 16, 2, 0), (16, 32, 16, 1), (32, 64, 32, 2), (64, 128, 64, 3),
+         (192, 64, 192, 2), (96, 32, 96, 1), (48, 16, 48, 0)]
+
+
+def round32(k):
+    return (k + 31) // 32 * 32
+
+
+def param_layout():
+    """[(weight offset, bias offset, kp)] per convolution, head offset,
+    total count."""
+    out, off = [], 0
+    for cin, cout, _, _ in CONVS:
+        kp = round32(27 * cin)
+        out.append((off, off + cout * kp, kp))
+        off += cout * kp + cout
+    return out, off, off + 17
+
+
+def _split(P):
+    lay, hoff, n = param_layout()
+    assert P.size == n, (P.size, n)
+    W, B = [], []
+    for (wo, bo, kp), (cin, cout, _, _) in zip(lay, CONVS):
+        W.append(np.asarray(P[wo:wo + cout * kp], np.float64).reshape(cout, kp))
+        B.append(np.asarray(P[bo:bo + cout], np.float64))
+    return W, B, np.asarray(P[hoff:hoff + 17], np.float64)
+
+
+def _r(a, bf16):
+    return to_bf16(a.astype(np.float32)).astype(np.float64) if bf16 else a
+
+
+def pool(x):
+    D, H, W, C = x.shape
+    return x.reshape(D // 2, 2, H // 2, 2, W // 2, 2, C).sum(axis=(1, 3, 5)) \
+        * 0.125
+
+
+def up(x):
+    return x.repeat(2, axis=0).repeat(2, axis=1).repeat(2, axis=2)
+
+
+def up_bwd(dy):
+    D, H, W, C = dy.shape
+    return dy.reshape(D // 2, 2, H // 2, 2, W // 2, 2, C).sum(axis=(1, 3, 5))
+
+
+def net_forward(P, f, c, bf16=False):
+    """Output (D, H, W) and the activations the backward pass needs."""
+    W, B, head = _split(P)
+    if bf16:
+        W = [_r(w, True) for w in W]
+    D, H, Wd = f.shape
+    xin = np.zeros((D, H, Wd, 8))
+    xin[..., 0] = f
+    xin[..., 1] = c
+    xin = _r(xin, bf16)
+    a = {"xin": xin}
+    cv = lambda i, x: _r(conv(x, W[i], B[i], relu=True), bf16)
+    e0 = cv(0, xin)
+    p1 = _r(pool(e0), bf16)
+    e1 = cv(1, p1)
+    p2 = _r(pool(e1), bf16)
+    e2 = cv(2, p2)
+    p3 = _r(pool(e2), bf16)
+    e3 = cv(3, p3)
+    cat2 = np.concatenate([up(e3), e2], axis=-1)
+    d2 = cv(4, cat2)
+    cat1 = np.concatenate([up(d2), e1], axis=-1)
+    d1 = cv(5, cat1)
+    cat0 = np.concatenate([up(d1), e0], axis=-1)
+    d0 = cv(6, cat0)
+    out = head[16] + d0 @ head[:16]
+    a.update(e0=e0, p1=p1, e1=e1, p2=p2, e2=e2, p3=p3, e3=e3, cat2=cat2,
+             d2=d2, cat1=cat1, d1=d1, cat0=cat0, d0=d0, W=W)
+    return out, a
+
+
+def net_loss_grad(P, f, c, target, bf16=False):
+    """Mean-removed squared error L = mean((o - t - mean(o - t))^2) of the
+    network output o and its gradient in the parameter layout."""
+    out, a = net_forward(P, f, c, bf16)
+    M = out.size
+    d = out - target
+    d = d - d.mean()
+    loss = float(np.mean(d * d))
+    g = 2.0 * d / M                                   # dL/d out
+    W, B, head = _split(P)
+    Wc = a["W"]
+    lay, hoff, n = param_layout()
+    G = np.zeros(n)
+    # head
+    G[hoff:hoff + 16] = np.tensordot(g, a["d0"], axes=3)
+    G[hoff + 16] = g.sum()
+    dd0 = _r(g[..., None] * head[None, None, None, :16], bf16)
+
+    def conv_back(i, x, dz):
+        """Weight and bias gradients of convolution i; input gradient."""
+        cin, cout, _, _ = CONVS[i]
+        wo, bo, kp = lay[i]
+        G[wo:wo + cout * kp] = conv_wgrad(x, dz, kp).ravel()
+        G[bo:bo + cout] = dz.reshape(-1, cout).sum(axis=0)
+        return _r(conv_dgrad(dz, Wc[i], cin), bf16)
+
+    relu = lambda dy, y: np.where(y > 0, dy, 0.0)
+    dcat0 = conv_back(6, a["cat0"], relu(dd0, a["d0"]))
+    dd1 = _r(up_bwd(dcat0[..., :32]), bf16)
+    dcat1 = conv_back(5, a["cat1"], relu(dd1, a["d1"]))
+    dd2 = _r(up_bwd(dcat1[..., :64]), bf16)
+    dcat2 = conv_back(4, a["cat2"], relu(dd2, a["d2"]))
+    de3 = _r(up_bwd(dcat2[..., :128]), bf16)
+    dp3 = conv_back(3, a["p3"], relu(de3, a["e3"]))
+    de2 = _r(dcat2[..., 128:] + 0.125 * up(dp3), bf16)
+    dp2 = conv_back(2, a["p2"], relu(de2, a["e2"]))
+    de1 = _r(dcat1[..., 64:] + 0.125 * up(dp2), bf16)
+    dp1 = conv_back(1, a["p1"], relu(de1, a["e1"]))
+    de0 = _r(dcat0[..., 32:] + 0.125 * up(dp1), bf16)
+    conv_back(0, a["xin"], relu(de0, a["e0"]))
+    return loss, G

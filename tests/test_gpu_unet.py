@@ -106,6 +106,59 @@ def test_training_lowers_the_loss(gpu):
     assert np.mean(e * e) < losses[0]
 
 
+@pytest.mark.parametrize("seed", [1, 4])
+def test_network_gradient_matches_oracle(gpu, seed):
+    """One training step's gradient (every convolution's weights and bias,
+    the head; bmg_unet_get_grads) and loss against the float64 restatement
+    of the whole network (oracle/unet_ref.py, pinned by central differences
+    in tests/test_unet_oracle.py) with BF16 rounding where the device stores
+    BF16, on a 16^3 input: the backward kernels of the ReLU, pooling,
+    upsampling, concatenation, head, bias sums and loss all enter it."""
+    D = 16
+    r = np.random.default_rng(seed)
+    f = r.standard_normal((D, D, D)).astype(np.float32)
+    c = r.uniform(-1, 1, (D, D, D)).astype(np.float32)
+    t = r.standard_normal((D, D, D)).astype(np.float32)
+    net = unet.Net(seed=seed)
+    P = net.params().astype(np.float64)
+    d = lambda a: torch.from_numpy(a).to("cuda:0")
+    loss = net.train_step(d(f), d(c), d(t), 0.0)
+    G = net.grads().astype(np.float64)
+    L, Gr = U.net_loss_grad(P, f.astype(np.float64), c.astype(np.float64),
+                            t.astype(np.float64), bf16=True)
+    L0, G0 = U.net_loss_grad(P, f.astype(np.float64), c.astype(np.float64),
+                             t.astype(np.float64))
+    lay, hoff, n = U.param_layout()
+    names = ["E0", "E1", "E2", "E3", "D2", "D1", "D0"]
+    rows = [("loss", abs(loss - L) / L, 1.0)]
+    groups = []
+    for name, (wo, bo, kp), (cin, cout, _, _) in zip(names, lay, U.CONVS):
+        groups.append((name + " w", slice(wo, wo + cout * kp)))
+        groups.append((name + " b", slice(bo, bo + cout)))
+    groups.append(("head", slice(hoff, hoff + 17)))
+    worst = 0.0
+    for name, sl in groups:
+        g, gr, g0 = G[sl], Gr[sl], G0[sl]
+        rel = np.linalg.norm(g - gr) / np.linalg.norm(gr)
+        cos = float(g @ gr / (np.linalg.norm(g) * np.linalg.norm(gr)))
+        rel0 = np.linalg.norm(g - g0) / np.linalg.norm(g0)
+        rows.append((name, rel, cos, rel0))
+        worst = max(worst, rel)
+    print(f"\n[unet grad] seed {seed}: loss device {loss:.6g} oracle {L:.6g}")
+    for row in rows[1:]:
+        print(f"[unet grad] {row[0]:7s} rel err vs BF16 oracle {row[1]:.2e} "
+              f"cos {row[2]:.6f}; vs exact float64 {row[3]:.2e}")
+    assert abs(loss - L) <= 1e-3 * L and abs(L - L0) <= 2e-2 * L0
+    for name, rel, cos, rel0 in rows[1:]:
+        assert rel < 0.05 and cos > 0.998, (name, rel, cos)
+        assert rel0 < 0.15, (name, rel0)
+    # the padded channels of the first convolution get no gradient
+    w0 = G[lay[0][0]:lay[0][1]].reshape(16, lay[0][2])
+    assert not w0[:, 27 * 8:].any()
+    assert not w0[:, :27 * 8].reshape(16, 27, 8)[:, :, 2:].any()
+    net.close()
+
+
 @pytest.mark.parametrize("precision", [bmg.FP64, bmg.MIXED], ids=["fp64", "mixed"])
 def test_learned_fcg_reaches_tolerance(gpu, precision):
     prob = W.spheres(32, seed=2, nspheres=6)

@@ -70,3 +70,73 @@ def test_wgrad_is_derivative_in_w():
     fd = (np.sum(dy * U.conv(x, wp[:, :216])) -
           np.sum(dy * U.conv(x, wm[:, :216]))) / (2 * e)
     assert abs(fd - g[i, k]) < 1e-6 * max(1.0, abs(g[i, k]))
+
+
+def _net_params(seed):
+    """Parameters in the device layout: He-normal weights on the real
+    input channels, small random biases, random head."""
+    r = rng(seed)
+    lay, hoff, n = U.param_layout()
+    P = np.zeros(n)
+    for (wo, bo, kp), (cin, cout, creal, _) in zip(lay, U.CONVS):
+        w = np.zeros((cout, kp))
+        for t in range(27):
+            w[:, t * cin:t * cin + creal] = (r.standard_normal((cout, creal))
+                                             * np.sqrt(2.0 / (27 * creal)))
+        P[wo:wo + cout * kp] = w.ravel()
+        P[bo:bo + cout] = 0.05 * r.standard_normal(cout)
+    P[hoff:hoff + 16] = 0.25 * r.standard_normal(16)
+    P[hoff + 16] = 0.1
+    return P
+
+
+def test_network_gradient_central_differences():
+    """The float64 U-Net loss gradient (oracle/unet_ref.py) against central
+    differences of the loss along random directions confined to single
+    layers (first encoder weights, deepest encoder weights, a decoder bias,
+    the head) and along one over all parameters."""
+    P = _net_params(0)
+    r = rng(1)
+    D = 8
+    f = r.standard_normal((D, D, D))
+    c = r.uniform(-1, 1, (D, D, D))
+    t = r.standard_normal((D, D, D))
+    L, G = U.net_loss_grad(P, f, c, t)
+    lay, hoff, n = U.param_layout()
+
+    def loss(Q):
+        o, _ = U.net_forward(Q, f, c)
+        d = o - t
+        d = d - d.mean()
+        return float(np.mean(d * d))
+
+    assert abs(loss(P) - L) <= 1e-14 * L
+    groups = [slice(lay[0][0], lay[0][1]), slice(lay[3][0], lay[3][1]),
+              slice(lay[4][1], lay[4][1] + 64), slice(hoff, hoff + 17),
+              slice(0, n)]
+    eps = 1e-6
+    for sl in groups:
+        v = np.zeros(n)
+        v[sl] = r.standard_normal(sl.stop - sl.start)
+        fd = (loss(P + eps * v) - loss(P - eps * v)) / (2 * eps)
+        assert abs(fd - G @ v) <= 1e-6 * abs(G @ v), (sl, fd, G @ v)
+    # padded input channels and padded K columns get no gradient
+    w0 = G[lay[0][0]:lay[0][1]].reshape(16, lay[0][2])
+    assert not w0[:, 27 * 8:].any()
+    assert not w0[:, :27 * 8].reshape(16, 27, 8)[:, :, 2:].any()
+
+
+def test_network_bf16_rounding_is_a_small_perturbation():
+    """Rounding the stored activations and gradients to BF16 (the device's
+    storage) changes the gradient by a few percent at most: the device is
+    compared with the BF16 form, and this bounds how far that form is from
+    the exact function."""
+    P = U.to_bf16(_net_params(2).astype(np.float32)).astype(np.float64)
+    r = rng(3)
+    D = 8
+    f, c = r.standard_normal((D, D, D)), r.uniform(-1, 1, (D, D, D))
+    t = r.standard_normal((D, D, D))
+    L0, G0 = U.net_loss_grad(P, f, c, t)
+    L1, G1 = U.net_loss_grad(P, f, c, t, bf16=True)
+    assert abs(L1 - L0) <= 1e-2 * L0
+    assert np.linalg.norm(G1 - G0) <= 0.05 * np.linalg.norm(G0)
