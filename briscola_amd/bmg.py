@@ -20,6 +20,7 @@ RBGS, JAC, LEXGS = 0, 1, 2
 FP64, MIXED = 0, 1
 KRYLOV_NONE, KRYLOV_FCG, KRYLOV_FCG_LEARNED = 0, 1, 2
 X, B, R = 0, 1, 2
+TIME_PHASE, TIME_RESIDUAL, TIME_RESTRICT, TIME_PROLONG, TIME_ITERATION = range(5)
 
 EXPORTS = [
     "bmg_default_opts", "bmg_last_error", "bmg_device_count", "bmg_create",
@@ -39,7 +40,7 @@ EXPORTS = [
     "bmg_fft_solve_device", "bmg_fft_solve_ddt", "bmg_fft_solve_ddt_device",
     "bmg_split_source", "bmg_split_source_device",
     "bmg_set_boundary_values", "bmg_set_boundary_values_device",
-    "bmg_renew_source_terms",
+    "bmg_renew_source_terms", "bmg_time_op",
 ]
 
 
@@ -121,6 +122,7 @@ def load():
         "bmg_op_cycle": (I, [P]),
         "bmg_op_sum_sq_residual": (I, [P, DP]),
         "bmg_time_smoother": (I, [P, I, I, I, DP, DP]),
+        "bmg_time_op": (I, [P, I, I, I, I, DP, DP]),
         "bmg_set_graphs": (I, [P, I]),
         "bmg_set_fused_smoothing": (I, [P, I]),
         "bmg_solve_diagonal": (I, [P, DP, DP, DP]),
@@ -340,6 +342,17 @@ class Context:
         by = ctypes.c_double(0.0)
         _check(self.lib.bmg_time_smoother(self.ptr, level, precision, reps,
                                           ctypes.byref(ms), ctypes.byref(by)))
+        return ms.value, by.value
+
+    def time_op(self, op, level=0, precision=FP64, reps=20):
+        """Average duration (ms) and algorithmic bytes of one launch of a
+        level operation (TIME_PHASE, TIME_RESIDUAL, TIME_RESTRICT,
+        TIME_PROLONG) or of one whole MG iteration (TIME_ITERATION), timed
+        with HIP events on the solver's stream."""
+        ms = ctypes.c_double(0.0)
+        by = ctypes.c_double(0.0)
+        _check(self.lib.bmg_time_op(self.ptr, op, level, precision, reps,
+                                    ctypes.byref(ms), ctypes.byref(by)))
         return ms.value, by.value
 
     def split_source(self, x, lam, lambda0, b=None):

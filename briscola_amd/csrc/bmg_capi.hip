@@ -2506,6 +2506,221 @@ int bmg_time_smoother(bmg_ctx* c, int level, int precision, int reps,
     });
 }
 
+// Algorithmic bytes of the launches of one MG iteration (enqueueIteration):
+// every array a launch must touch counted once (n interior cells of a level,
+// w bytes per value):
+//   red-black phase      n/2 (3w + coefficients of the representation)
+//   residual             n (3w + coefficients), the norm fused
+//   restriction          n_f w + n_c w;  prolongation  2 n_f w + n_c w
+//   zeroing x            n w;  ghost fill  2 w per ghost cell written
+//   periodic slabs       2 planes x (3w + coefficients) per face pair
+//   coarse solve         the LU factors (8 n_c^2) + b in and x out
+// plus, for mixed precision, the FP64 -> FP32 residual copy (12 n), the
+// zeroing of e (4 n) and x += e (20 n) on level 0.
+static double phaseBytesPerUpdate(const Level& q, size_t w)
+{
+    double per = q.opkind == OP_CONST ? 3.0 : q.opkind == OP_FACES ? 9.0 : 10.0;
+    if (q.opkind != OP_STENCIL && q.dg[0]) per += 1.0;
+    return per*double(w);
+}
+
+static double cellBytesFull(const Level& q, size_t w)
+{
+    // residual-type pass over every cell: x, b, r + each stored coefficient
+    double per = q.opkind == OP_CONST ? 3.0 : q.opkind == OP_FACES ? 6.0 : 10.0;
+    if (q.opkind != OP_STENCIL && q.dg[0]) per += 1.0;
+    return per*double(w);
+}
+
+static double ghostCells(const Level& q)
+{
+    const double p0 = q.L.P[0], p1 = q.L.P[1], p2 = q.L.P[2];
+    return p0*p1*p2 - double(q.cells());
+}
+
+static double smoothBytes(const bmg_ctx* c, int l, int sweeps, size_t w)
+{
+    const Level& q = c->lv[l];
+    const Region rg = c->interior(l, 1);
+    double cells = 1.0;
+    for (int d = 0; d < 3; d++) cells *= std::max(0, rg.E[d] - rg.S[d]);
+    double bytes = 0.0;
+    int prev = -1;
+    for (int s = 0; s < sweeps; s++)
+    {
+        int cols[4];
+        const int nc = phase_colours(rg, cols);
+        for (int i = 0; i < nc; i++)
+        {
+            if (cols[i] != prev) bytes += 0.5*cells*phaseBytesPerUpdate(q, w);
+            prev = cols[i];
+        }
+        if (!c->anyPeriodic()) continue;
+        bytes += 2.0*ghostCells(q)*double(w);
+        for (int d = 0; d < 3; d++)
+            if (c->periodic(2*d))
+                bytes += 2.0*double(q.planeCells(d))*
+                         (phaseBytesPerUpdate(q, w) + double(w));
+    }
+    return bytes;
+}
+
+static double cycleBytes(const bmg_ctx* c, int l, size_t w)
+{
+    const Level& q = c->lv[l];
+    const int lc = c->coarseLevel();
+    const double n = double(q.cells());
+    const bool direct = c->opts.coarse_mode == BMG_COARSE_DIRECT;
+    double bytes = 0.0;
+    if (l > 0) bytes += n*double(w);
+    if (c->opts.n_sweeps_pre > 0 && !(l == lc && direct))
+        bytes += smoothBytes(c, l, c->opts.n_sweeps_pre, w);
+    if (l < lc)
+    {
+        const Level& qc = c->lv[l + 1];
+        const double nc = double(qc.cells());
+        if (l > 0) bytes += n*cellBytesFull(q, w);
+        bytes += n*double(w) + nc*double(w);
+        bytes += cycleBytes(c, l + 1, w);
+        bytes += 2.0*ghostCells(qc)*double(w);
+        bytes += 2.0*n*double(w) + nc*double(w);
+        if (c->opts.n_sweeps_post > 0)
+            bytes += smoothBytes(c, l, c->opts.n_sweeps_post, w);
+    }
+    else if (direct)
+        bytes += 8.0*n*n + 2.0*n*double(w) + 2.0*ghostCells(q)*double(w);
+    else
+        bytes += smoothBytes(c, l, std::max(c->opts.n_sweeps_post, 2), w);
+    return bytes;
+}
+
+static double iterationBytes(const bmg_ctx* c, bool mixed)
+{
+    const Level& q0 = c->lv[0];
+    const double n = double(q0.cells());
+    double bytes = cycleBytes(c, 0, mixed ? sizeof(float) : sizeof(double));
+    if (mixed) bytes += (12.0 + 4.0 + 20.0)*n;
+    bytes += n*cellBytesFull(q0, sizeof(double));
+    return bytes;
+}
+
+int bmg_time_op(bmg_ctx* c, int op, int level, int precision, int reps,
+                double* avg_ms, double* bytes)
+{
+    return run([&] {
+        if (!c) throw Error(BMG_ERR_ARG, "null context");
+        checkLevel(c, level);
+        needEliminated(c);
+        if (reps < 1) throw Error(BMG_ERR_ARG, "reps must be positive");
+        if (precision != BMG_FP64 && precision != BMG_MIXED)
+            throw Error(BMG_ERR_ARG, "precision must be BMG_FP64 or BMG_MIXED");
+        const bool mixed = precision == BMG_MIXED;
+        if (mixed) c->ensureMixed();
+        const size_t w = mixed ? sizeof(float) : sizeof(double);
+        Level& q = c->lv[level];
+        const double n = double(q.cells());
+        const bool coarser = level + 1 < c->nLevels();
+        if ((op == BMG_TIME_RESTRICT || op == BMG_TIME_PROLONG) && !coarser)
+            throw Error(BMG_ERR_ARG, "no coarser level");
+        if (op == BMG_TIME_ITERATION && level != 0)
+            throw Error(BMG_ERR_ARG, "an iteration starts on level 0");
+        if (op == BMG_TIME_ITERATION && !c->prepared)
+            throw Error(BMG_ERR_STATE, "call bmg_op_prepare first");
+        double by = 0.0;
+        auto launchOnce = [&](int r) {
+            switch (op)
+            {
+            case BMG_TIME_PHASE:
+                if (mixed)
+                    launch_rbgs<float>(q.L, c->op<float>(level), pr(q.xf),
+                                       cpr(q.bf), r & 1, c->interior(level, 1),
+                                       c->stream);
+                else
+                    launch_rbgs<double>(q.L, c->op<double>(level), pr(q.x),
+                                        cpr(q.b), r & 1, c->interior(level, 1),
+                                        c->stream);
+                break;
+            case BMG_TIME_RESIDUAL:
+            {
+                int nb = 0;
+                if (mixed)
+                    launch_residual(q.L, c->op<float>(level), cpr(q.xf),
+                                    cpr(q.bf), pr(q.rf), nullptr, nullptr,
+                                    c->stream);
+                else
+                    launch_residual(q.L, c->op<double>(level), cpr(q.x),
+                                    cpr(q.b), pr(q.r), c->partials, &nb,
+                                    c->stream);
+                break;
+            }
+            case BMG_TIME_RESTRICT:
+                if (mixed) c->restrictTo<float>(level);
+                else c->restrictTo<double>(level);
+                break;
+            case BMG_TIME_PROLONG:
+            {
+                Level& qc = c->lv[level + 1];
+                if (mixed)
+                    launch_prolong(q.L, qc.L, c->transfer<float>(level),
+                                   cpr(qc.xf), pr(q.xf), c->stream);
+                else
+                    launch_prolong(q.L, qc.L, c->transfer<double>(level),
+                                   cpr(qc.x), pr(q.x), c->stream);
+                break;
+            }
+            case BMG_TIME_ITERATION:
+                HIPCK(hipGraphLaunch(c->graphExec, c->stream));
+                break;
+            default:
+                throw Error(BMG_ERR_ARG, "unknown operation to time");
+            }
+        };
+        switch (op)
+        {
+        case BMG_TIME_PHASE:
+        {
+            const Region rg = c->interior(level, 1);
+            double cells = 1.0;
+            for (int d = 0; d < 3; d++) cells *= std::max(0, rg.E[d] - rg.S[d]);
+            by = 0.5*cells*phaseBytesPerUpdate(q, w);
+            break;
+        }
+        case BMG_TIME_RESIDUAL:
+            by = n*cellBytesFull(q, w);
+            break;
+        case BMG_TIME_RESTRICT:
+            by = n*double(w) + double(c->lv[level + 1].cells())*double(w);
+            break;
+        case BMG_TIME_PROLONG:
+            by = 2.0*n*double(w) + double(c->lv[level + 1].cells())*double(w);
+            break;
+        case BMG_TIME_ITERATION:
+        {
+            const bool keep = c->useGraphs;
+            c->useGraphs = true;
+            try { (void)c->iterate(mixed); }
+            catch (...) { c->useGraphs = keep; throw; }
+            c->useGraphs = keep;
+            by = iterationBytes(c, mixed);
+            break;
+        }
+        default:
+            throw Error(BMG_ERR_ARG, "unknown operation to time");
+        }
+        launchOnce(0);   // warm caches and code
+        HIPCK(hipStreamSynchronize(c->stream));
+        HIPCK(hipEventRecord(c->ev0, c->stream));
+        for (int r = 0; r < reps; r++) launchOnce(r);
+        HIPCK(hipEventRecord(c->ev1, c->stream));
+        HIPCK(hipEventSynchronize(c->ev1));
+        HIPCK(hipGetLastError());
+        float ms = 0.0f;
+        HIPCK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        if (avg_ms) *avg_ms = double(ms)/reps;
+        if (bytes) *bytes = by;
+    });
+}
+
 int bmg_set_learned(bmg_ctx* c, bmg_unet* net)
 {
     return run([&] {

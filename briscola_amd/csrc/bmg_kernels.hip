@@ -1306,6 +1306,156 @@ struct KProlong
     }
 };
 
+// The same prolongation organised by coarse cell, for levels refined by 2
+// along every axis (N_f = 2 N_c): thread (ci, cj, ck) adds into its eight
+// children I = 2ci-1+a, J = 2cj-1+b, K = 2ck-1+c.  A child with offset a
+// (fine interior index i = 2(ci-1)+a) has parent ci and neighbour
+// ci-1 (a = 0, i even) or ci+1 (a = 1, i odd), so the 27 coarse values of
+// the 3x3x3 block around (ci, cj, ck) serve all eight children: 27 coarse
+// loads per 8 fine cells instead of 8 per cell.  Lanes run along ck, so the
+// children K = 2ck-1 (colour (I+J+1)&1, position ck-1) and K = 2ck (colour
+// (I+J)&1, position ck) of one fine row are contiguous in one colour array
+// across the wavefront.  Per child the arithmetic is prolong_impl's.
+template <typename T, bool PACK>
+__device__ __forceinline__ void prolong8_impl(const Bid bid,
+                                              const LevelDims& Lf,
+                                              const LevelDims& Lc,
+                                              const TransferView<T>& tv,
+                                              Pair<const T> xc,
+                                              Pair<T> xf,
+                                              const Tiles& tl)
+{
+    int bx, by, bz;
+    if (!tile_of(tl, bid.x, bx, by, bz)) return;
+    const int ci = 1 + bz;
+    int cj, ck;
+    if (PACK)
+    {
+        const int lane = int(threadIdx.x), ws = tl.ws;
+        cj = 1 + (by*BY + int(threadIdx.y))*(64 >> ws) + (lane >> ws);
+        ck = 1 + (bx << ws) + (lane & ((1 << ws) - 1));
+    }
+    else
+    {
+        cj = __builtin_amdgcn_readfirstlane(1 + by*BY + int(threadIdx.y));
+        ck = 1 + bx*BX + int(threadIdx.x);
+    }
+    if (cj > Lc.N[1] || ck > Lc.N[2]) return;
+    // coarse block [x][y][z] = (ci-1+x, cj-1+y, ck-1+z)
+    T cv[3][3][3];
+#pragma unroll
+    for (int x = 0; x < 3; x++)
+#pragma unroll
+    for (int y = 0; y < 3; y++)
+    {
+        const int X = ci - 1 + x, Y = cj - 1 + y;
+        const int base = (X*Lc.P[1] + Y)*Lc.Hz;
+#pragma unroll
+        for (int z = 0; z < 3; z++)
+        {
+            const int Z = ck - 1 + z;
+            cv[x][y][z] = ld(col(xc, (X + Y + Z) & 1), unsigned(base + (Z >> 1)));
+        }
+    }
+    // prolongation fractions of the children along each axis
+    T uX[2], uY[2], uZ[2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+    {
+        uX[a] = tv.pu[0][2*ci - 1 + a];
+        uY[a] = tv.pu[1][2*cj - 1 + a];
+        uZ[a] = tv.pu[2][2*ck - 1 + a];
+    }
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+    {
+        const int I = 2*ci - 1 + a, J = 2*cj - 1 + b;
+        // children K = 2ck-1 (c = 0) and K = 2ck (c = 1) of row (I, J)
+        const int cB = (I + J) & 1;
+        const unsigned row = unsigned((I*Lf.P[1] + J)*Lf.Hz);
+        T* xa = col(xf, cB ^ 1);
+        T* xb = col(xf, cB);
+        const unsigned ia = row + unsigned(ck - 1), ib = row + unsigned(ck);
+        const T old[2] = {ld(xa, ia), ld(xb, ib)};
+        // parent (abc 0) and neighbour (abc 1) positions in the block:
+        // parent 1, neighbour 0 (offset 0) or 2 (offset 1)
+        const int sx[2] = {1, a ? 2 : 0};
+        const int sy[2] = {1, b ? 2 : 0};
+        T add[2];
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+        {
+            const int sz[2] = {1, c ? 2 : 0};
+            T u0 = uX[a], u1 = uY[b], u2 = uZ[c];
+            const bool inside = !(u0 < T(0) || u0 >= T(1) || u1 < T(0) ||
+                                  u1 >= T(1) || u2 < T(0) || u2 >= T(1));
+            if (!inside) { u0 = T(0.5); u1 = T(0.5); u2 = T(0.5); }
+            const T gx[2] = {T(1.0) - u0, u0};
+            const T gy[2] = {T(1.0) - u1, u1};
+            const T gz[2] = {T(1.0) - u2, u2};
+            T value = T(0);
+#pragma unroll
+            for (int cz = 0; cz < 2; cz++)
+#pragma unroll
+            for (int cy = 0; cy < 2; cy++)
+#pragma unroll
+            for (int cx = 0; cx < 2; cx++)
+            {
+                T w = gx[cx]*gy[cy];
+                w = w*gz[cz];
+                const T t = w*cv[sx[cx]][sy[cy]][sz[cz]];
+                value = value + t;
+            }
+            add[c] = value;
+        }
+        st(xa, ia, old[0] + add[0]);
+        st(xb, ib, old[1] + add[1]);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(BX*BY) void k_prolong8(LevelDims Lf, LevelDims Lc,
+                                                    TransferView<T> tv,
+                                                    Pair<const T> xc,
+                                                    Pair<T> xf, Tiles tl)
+{
+    const Bid bid{blockIdx.x, blockIdx.y, blockIdx.z};
+    if (tl.ws != 6)
+        prolong8_impl<T, true>(bid, Lf, Lc, tv, xc, xf, tl);
+    else
+        prolong8_impl<T, false>(bid, Lf, Lc, tv, xc, xf, tl);
+}
+
+template <typename T>
+struct KProlong8
+{
+    static constexpr int bx = BX, by = BY;
+    static constexpr bool lds = false;
+    struct Args
+    {
+        LevelDims Lf;
+        LevelDims Lc;
+        TransferView<T> tv;
+        Pair<const T> xc;
+        Pair<T> xf;
+        Tiles tl;
+    };
+    __device__ static void run(const Bid bid, const Args& a)
+    {
+        if (a.tl.ws != 6)
+            prolong8_impl<T, true>(bid, a.Lf, a.Lc, a.tv, a.xc, a.xf, a.tl);
+        else
+            prolong8_impl<T, false>(bid, a.Lf, a.Lc, a.tv, a.xc, a.xf, a.tl);
+    }
+    static void launch(dim3 g, size_t shm, hipStream_t s, const Args& a)
+    {
+        hipLaunchKernelGGL((k_prolong8<T>), g, dim3(bx, by), shm, s, a.Lf,
+                           a.Lc, a.tv, a.xc, a.xf, a.tl);
+    }
+};
+
 
 // ---------------------------------------------------------------------------
 // Ghost values (meshLevel.C:231-326).  mode 0: periodic face ghosts only
@@ -3944,6 +4094,17 @@ void launch_prolong(const LevelDims& Lf, const LevelDims& Lc,
                     const TransferView<T>& tv, Pair<const T> xc, Pair<T> xf,
                     hipStream_t s)
 {
+    if (Lf.N[0] == 2*Lc.N[0] && Lf.N[1] == 2*Lc.N[1] && Lf.N[2] == 2*Lc.N[2])
+    {
+        // one thread per coarse cell (k_prolong8)
+        const int ws = row_pack_shift(Lc.N[2]);
+        const int rows = BY*(64 >> ws);
+        Tiles tl = make_tiles((Lc.N[2] + (1 << ws) - 1) >> ws,
+                              (Lc.N[1] + rows - 1)/rows, Lc.N[0]);
+        tl.ws = ws;
+        dispatch<KProlong8<T>>(tiles_grid(tl), 0, s, {Lf, Lc, tv, xc, xf, tl});
+        return;
+    }
     const Tiles tl = pair_tiles_packed(Lf);
     dispatch<KProlong<T>>(tiles_grid(tl), 0, s, {Lf, Lc, tv, xc, xf, tl});
 }

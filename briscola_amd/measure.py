@@ -2,18 +2,26 @@
 benchmark reports, for rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE).
 
   rocprofv3 --pmc FETCH_SIZE --output-format csv -d <dir> -o run -- \
-      python3 -m briscola_amd.measure [spheres|channel]
+      python3 -m briscola_amd.measure [spheres|channel] [phase|prolong|
+                                       residual|restrict] [fp64|mixed]
 
-Launches, in order: 5 level-0 red-black phases of the chosen 256^3 operator
-(spheres: k_rbgs<double, 2>, face coefficients, 604 MB algorithmic each;
-channel: k_rbgs<double, 1>, matrix-free, 198 MB algorithmic each), colours
-alternating as in a sweep, and one plain device-to-device copy of 128 MiB
-(the calibration kernel: 128 MiB read, 128 MiB written).
+Launches, in order: 5 level-0 launches of the chosen operation on the
+chosen 256^3 workload, then one plain device-to-device copy of 128 MiB (the
+calibration kernel: 128 MiB read, 128 MiB written).  Operations:
+
+  phase     red-black phase k_rbgs (spheres: face coefficients, 604 MB
+            algorithmic in FP64; channel: matrix-free, 198 MB), colours
+            alternating as in a sweep;
+  prolong   prolongation-add k_prolong from 128^3 into 256^3 (2 n w + n/8 w);
+  residual  level-0 residual k_residual (FP64 with the fused norm partials);
+  restrict  restriction k_restrict of the level-0 residual to 128^3.
 """
 import sys
 
+OPS = {"phase": 0, "residual": 1, "restrict": 2, "prolong": 3}
 
-def main(workload="spheres", n=256, reps=5):
+
+def main(workload="spheres", op="phase", precision="fp64", n=256, reps=5):
     import torch
     from briscola_amd import bmg, workloads
     p = (workloads.spheres(n, seed=0) if workload == "spheres"
@@ -24,9 +32,10 @@ def main(workload="spheres", n=256, reps=5):
     ctx.set_laplacian(p.lam)
     ctx.upload(bmg.B, 0, p.b)
     ctx.prepare()
-    ms, by = ctx.time_smoother(0, bmg.FP64, reps)
-    print(f"{workload} phase: {ms * 1e3:.2f} us, {by / 1e6:.1f} MB algorithmic",
-          flush=True)
+    prec = bmg.MIXED if precision == "mixed" else bmg.FP64
+    ms, by = ctx.time_op(OPS[op], 0, prec, reps)
+    print(f"{workload} {op} ({precision}): {ms * 1e3:.2f} us, "
+          f"{by / 1e6:.1f} MB algorithmic", flush=True)
     a = torch.empty(128 << 20, dtype=torch.uint8, device="cuda:0").fill_(1)
     b = torch.empty_like(a)
     b.copy_(a)
@@ -37,4 +46,4 @@ def main(workload="spheres", n=256, reps=5):
 
 
 if __name__ == "__main__":
-    sys.exit(main(*(sys.argv[1:2] or ["spheres"])))
+    sys.exit(main(*sys.argv[1:4]))

@@ -355,6 +355,33 @@ int bmg_op_sum_sq_residual(bmg_ctx* ctx, double* out);
  * Needs bmg_op_prepare or a solve first; overwrites x of that level. */
 int bmg_time_smoother(bmg_ctx* ctx, int level, int precision, int reps,
                       double* avg_ms, double* bytes);
+/* bmg_time_op: the same for one operation of a level, launched 'reps' times
+ * back to back between two HIP events on the context's stream (after one
+ * untimed launch):
+ *   BMG_TIME_PHASE      one red-black phase over the interior (as above);
+ *   BMG_TIME_RESIDUAL   the residual of the level (FP64: with the fused norm
+ *                       partials, as the iteration's level-0 residual);
+ *   BMG_TIME_RESTRICT   restriction of the level's residual to level+1;
+ *   BMG_TIME_PROLONG    prolongation-add of level+1 into the level (without
+ *                       the coarse ghost refresh);
+ *   BMG_TIME_ITERATION  one whole MG iteration (level must be 0): the
+ *                       captured graph of cycle, level-0 residual and norm.
+ * bytes: algorithmic HBM bytes of one launch (every array touched once:
+ * phase n/2 (3w + coefficients), residual n (3w + coefficients),
+ * restriction n w + n_c w, prolongation 2 n w + n_c w, w = 8 FP64 / 4 FP32,
+ * coefficients 0 matrix-free, 6w face, 7w stored stencil; for an iteration
+ * the sum over its launches).  Needs bmg_op_prepare or a solve first;
+ * overwrites the fields of the level (an iteration: of every level). */
+enum
+{
+    BMG_TIME_PHASE = 0,
+    BMG_TIME_RESIDUAL = 1,
+    BMG_TIME_RESTRICT = 2,
+    BMG_TIME_PROLONG = 3,
+    BMG_TIME_ITERATION = 4
+};
+int bmg_time_op(bmg_ctx* ctx, int op, int level, int precision, int reps,
+                double* avg_ms, double* bytes);
 /* Replay each MG iteration as a captured HIP graph (default on; the
  * environment variable BMG_NO_GRAPHS=1 turns it off at creation). */
 int bmg_set_graphs(bmg_ctx* ctx, int enable);
