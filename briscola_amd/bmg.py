@@ -40,7 +40,7 @@ EXPORTS = [
     "bmg_fft_solve_device", "bmg_fft_solve_ddt", "bmg_fft_solve_ddt_device",
     "bmg_split_source", "bmg_split_source_device",
     "bmg_set_boundary_values", "bmg_set_boundary_values_device",
-    "bmg_renew_source_terms", "bmg_time_op",
+    "bmg_renew_source_terms", "bmg_time_op", "bmg_face_flux_noalloc",
 ]
 
 
@@ -133,6 +133,7 @@ def load():
         "bmg_batch_info": (I, [P, IP, ctypes.POINTER(ctypes.c_long)]),
         "bmg_batch_set_autotune": (I, [P, I]),
         "bmg_face_flux": (I, [P, DP, DP, DP, DP, DP, DP, DP]),
+        "bmg_face_flux_noalloc": (I, [P, DP, DP, DP, DP, DP, DP, DP]),
         "bmg_face_flux_device": (I, [P, P, P, P, P, P, P, P]),
         "bmg_fft_create": (I, [ctypes.POINTER(P), ctypes.POINTER(Grid)]),
         "bmg_fft_destroy": (I, [P]),

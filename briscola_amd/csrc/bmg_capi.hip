@@ -17,6 +17,7 @@
 #include <exception>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -2300,6 +2301,88 @@ int bmg_face_flux(bmg_ctx* c, const double* x, const double* lx,
             throw;
         }
         for (auto& p : bufs) devFree(p);
+    });
+}
+
+// Staging memory of bmg_face_flux_noalloc: a fixed array in the code
+// object, one copy per device, held until the process ends.  Host code never
+// passes the array's name to a kernel or a copy (in host code it names a
+// host-side stand-in): it uses the device address hipGetSymbolAddress
+// returns for the device, taken once per device with that device current.
+static __device__ __align__(256) double scratch[BMG_NOALLOC_SCRATCH_DOUBLES];
+
+static std::mutex g_scratchMutex;
+static double* g_scratchPtr[64] = {};
+
+static double* scratchOn(int device)
+{
+    if (device < 0 || device >= 64)
+        throw Error(BMG_ERR_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratchMutex);
+    if (!g_scratchPtr[device])
+    {
+        int prev = 0;
+        HIPCK(hipGetDevice(&prev));
+        if (prev != device) HIPCK(hipSetDevice(device));
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (prev != device) HIPCK(hipSetDevice(prev));
+        if (e != hipSuccess || !p)
+            throw Error(BMG_ERR_HIP, std::string("hipGetSymbolAddress of the "
+                                                 "staging array: ") +
+                                         hipGetErrorString(e));
+        g_scratchPtr[device] = static_cast<double*>(p);
+    }
+    return g_scratchPtr[device];
+}
+
+int bmg_face_flux_noalloc(bmg_ctx* c, const double* x, const double* lx,
+                          const double* ly, const double* lz, double* fx,
+                          double* fy, double* fz)
+{
+    return run([&] {
+        if (!c || !x || !fx || !fy || !fz)
+            throw Error(BMG_ERR_ARG, "null pointer");
+        const bool lam = lx || ly || lz;
+        if (lam && !(lx && ly && lz))
+            throw Error(BMG_ERR_ARG, "give all three lambda arrays or none");
+        Level& q = c->lv[0];
+        const size_t n = size_t(q.L.P[0])*q.L.P[1]*q.L.P[2];
+        const size_t need = n*(lam ? 7 : 4);
+        if (need > size_t(BMG_NOALLOC_SCRATCH_DOUBLES))
+            throw Error(BMG_ERR_ARG,
+                        "level 0 of " + std::to_string(n) + " padded cells "
+                        "needs " + std::to_string(need) + " doubles of "
+                        "staging, more than the " +
+                        std::to_string(BMG_NOALLOC_SCRATCH_DOUBLES) +
+                        " of bmg_face_flux_noalloc");
+        double* s = scratchOn(c->device);
+        // staging: x, the three fluxes, then the three lambda arrays
+        double* dx = s;
+        double* const flux[3] = {s + n, s + 2*n, s + 3*n};
+        const double* dl[3] = {nullptr, nullptr, nullptr};
+        HIPCK(hipMemcpyAsync(dx, x, n*sizeof(double), hipMemcpyHostToDevice,
+                             c->stream));
+        // entries outside the face ranges are returned as 0; the array keeps
+        // the previous call's values, so clear the fluxes every call
+        HIPCK(hipMemsetAsync(flux[0], 0, 3*n*sizeof(double), c->stream));
+        if (lam)
+        {
+            const double* h[3] = {lx, ly, lz};
+            for (int d = 0; d < 3; d++)
+            {
+                double* p = s + (4 + d)*n;
+                HIPCK(hipMemcpyAsync(p, h[d], n*sizeof(double),
+                                     hipMemcpyHostToDevice, c->stream));
+                dl[d] = p;
+            }
+        }
+        faceFlux(c, dx, dl, flux);
+        double* const out[3] = {fx, fy, fz};
+        for (int d = 0; d < 3; d++)
+            HIPCK(hipMemcpyAsync(out[d], flux[d], n*sizeof(double),
+                                 hipMemcpyDeviceToHost, c->stream));
+        HIPCK(hipStreamSynchronize(c->stream));
     });
 }
 

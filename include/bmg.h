@@ -314,6 +314,23 @@ int bmg_face_flux_device(bmg_ctx* ctx, const double* d_x,
                          const double* d_lambda_z, double* d_flux_x,
                          double* d_flux_y, double* d_flux_z);
 
+/* bmg_face_flux without allocation: the same host arrays, the same results,
+ * bit for bit, for callers that cannot allocate device memory (code run
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  The inputs and fluxes are staged in a
+ * fixed device array of BMG_NOALLOC_SCRATCH_DOUBLES doubles (32 MiB) in the
+ * library's code object, one per device, held until the process ends; the
+ * call makes no hipMalloc/hipFree.  Level 0 must fit: 7 x padded cells
+ * doubles with lambda arrays (up to 599 186 padded cells, e.g. 82^3), 4 x
+ * without; larger levels get BMG_ERR_ARG.  Every call on a device shares
+ * that device's array: calls that overlap on one device (from several
+ * threads or contexts) race and must be serialised by the caller. */
+#define BMG_NOALLOC_SCRATCH_DOUBLES 4194304
+int bmg_face_flux_noalloc(bmg_ctx* ctx, const double* x,
+                          const double* lambda_x, const double* lambda_y,
+                          const double* lambda_z, double* flux_x,
+                          double* flux_y, double* flux_z);
+
 /* MG::solve of a diagonal system (sys.diagonal(), MG.C:404-428): level-0
  * stencil A7 (reference layout, only the centre is used after ghost
  * elimination), x = b / A_C on every interior cell, then the level-0
