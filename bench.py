@@ -138,10 +138,20 @@ def run_gpu(prob, precision, krylov, steps, warmup, profile, reps=50,
     # the solver's stream right after the timed steps (overwrites x on the
     # device, which was copied out above)
     prof = ctx.time_smoother(0, precision, reps) if profile else None
+    # the whole V-cycle iteration (captured graph: cycle, level-0 residual
+    # and norm) and the other level-0 kernels, with their algorithmic bytes
+    vcyc = kern = None
+    if profile:
+        vcyc = ctx.time_op(bmg.TIME_ITERATION, 0, precision, 20)
+        kern = {name: ctx.time_op(op, 0, precision, reps)
+                for name, op in (("residual", bmg.TIME_RESIDUAL),
+                                 ("restrict", bmg.TIME_RESTRICT),
+                                 ("prolong", bmg.TIME_PROLONG))}
     ctx.close()
     return dict(ms=1e3 * wall / steps, device_ms=dev_ms / steps,
                 iters=stats.iters, final_residual=stats.final_residual, x=x,
-                prof=prof, learned_iters=stats.learned_iters,
+                prof=prof, vcycle=vcyc, kernels=kern,
+                learned_iters=stats.learned_iters,
                 learned_fallback=stats.learned_fallback,
                 learned_guess_residual=stats.learned_guess_residual)
 
@@ -201,6 +211,31 @@ def cpu_cores():
         return os.cpu_count() or 1
 
 
+def cpu_full_solve(prob):
+    """The whole FP64 solve by the CPU restatement on the host's cores (once;
+    validates the scaled sample)."""
+    import oracle
+    o = oracle.Oracle(prob.vertices, prob.bc, prob.bc_value)
+    opts = solver_opts(0)
+    opts.pop("precision")
+    opts.pop("krylov")
+    o.set_opts(**opts)
+    o.assemble(prob.lam)
+    o.set_field(oracle.B, 0, prob.b)
+    cores = cpu_cores()
+    prev = oracle.get_threads()
+    try:
+        oracle.set_threads(cores)
+        t0 = time.perf_counter()
+        it, r0, r1 = o.solve()
+        sec = time.perf_counter() - t0
+    finally:
+        oracle.set_threads(prev)
+    del o
+    return {"ms": round(sec * 1e3, 1), "iters": it, "cores": cores,
+            "final_rel_residual": r1 / r0 if r0 else r1}
+
+
 def cpu_baseline(prob, iters_fp64, sample_iters, n):
     """The CPU restatement of the reference MG on a bounded sample of the
     same FP64 solve, on one core and on the host's P cores (OpenMP over the
@@ -256,6 +291,51 @@ def roofline(result, kernel, traffic):
             "traffic": traffic, "kernel": kernel,
             "avg_launch_us": round(avg_ms * 1e3, 2),
             "bytes_per_launch": per_launch}
+
+
+def vcycle_roofline(result, precision_name):
+    """One whole MG iteration (V(2,2) cycle over every level, level-0
+    residual and norm) replayed from its captured graph between HIP events:
+    algorithmic bytes of all its launches (each array a launch touches
+    counted once; red-black phases as executed, i.e. 10 per level for
+    V(2,2), the repeated colour of each reference sweep skipped) / device
+    time.  ``frac_reference_phases`` counts the bytes of the reference's
+    12 phases per level (3 per sweep, SURVEY-style p = 3) instead."""
+    if not result or not result.get("vcycle"):
+        return None
+    ms, by = result["vcycle"]
+    achieved = by / (ms * 1e-3) / 1e9
+    out = {"bound": "hbm", "achieved": round(achieved, 1),
+           "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 4),
+           "bytes_per_iteration": round(by),
+           "iteration_us": round(ms * 1e3, 1),
+           "precision": precision_name,
+           "solve_device_us_per_iteration": round(
+               1e3 * result["device_ms"] / max(1, result["iters"]), 1)}
+    if result.get("prof"):
+        # two extra level-0 phases per V(2,2) cycle (one per smoothing) in
+        # the reference's sweep order; the level-0 phase dominates the bytes
+        _, phase_bytes = result["prof"]
+        extra = 2.0 * phase_bytes * 8.0 / 7.0
+        out["frac_reference_phases"] = round(
+            (by + extra) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    return out
+
+
+def kernel_rooflines(result):
+    """Level-0 residual (with the norm partials in FP64), restriction and
+    prolongation, each launched back to back between HIP events."""
+    if not result or not result.get("kernels"):
+        return None
+    out = {}
+    for name, (ms, by) in result["kernels"].items():
+        a = by / (ms * 1e-3) / 1e9
+        out[name] = {"avg_launch_us": round(ms * 1e3, 2),
+                     "bytes_per_launch": round(by),
+                     "achieved": round(a, 1),
+                     "frac": round(a / HBM_PEAK_GBS, 4)}
+    return out
 
 
 TARGET_ERROR = 1e-8
@@ -315,7 +395,7 @@ def seed_solves(results, args):
                 med["fp64"] / med["fcg_mixed"], 3)}
 
 
-def equal_error_solves(prob, results, xe, args):
+def equal_error_solves(prob, results, xe, args, net=None):
     """Plain FP64 MG and FCG+mixed solved to the relative residual that
     brings each one's error against the FP64 reference below 1e-8: the
     error at relTol 1e-8 is set by the operator's conditioning, and falls in
@@ -326,14 +406,18 @@ def equal_error_solves(prob, results, xe, args):
     from briscola_amd import bmg
     from oracle import direct
     out = {"target_rel_error": TARGET_ERROR, "methods": {}}
-    for mname, prec, kry in (("fp64", bmg.FP64, bmg.KRYLOV_NONE),
-                             ("fcg_mixed", bmg.MIXED, bmg.KRYLOV_FCG)):
+    plan = [("fp64", bmg.FP64, bmg.KRYLOV_NONE),
+            ("fcg_mixed", bmg.MIXED, bmg.KRYLOV_FCG)]
+    if net is not None and "fcg_learned_mixed" in results:
+        plan.append(("fcg_learned_mixed", bmg.MIXED, bmg.KRYLOV_FCG_LEARNED))
+    for mname, prec, kry in plan:
         err0 = results[mname]["err_vs_fp64_reference"]
         tol = 10.0 ** math.floor(math.log10(1e-8 * TARGET_ERROR / err0))
         for _ in range(2):
             log(f"[bench] spheres {mname} to relTol {tol:.0e} (equal error)")
             r = run_gpu(prob, prec, kry, 1, 1, False, rel_tol=tol,
-                        max_iter=4000)
+                        max_iter=4000,
+                        net=net if kry == bmg.KRYLOV_FCG_LEARNED else None)
             err = direct.rel_error(r["x"][1:-1, 1:-1, 1:-1], xe, True)
             if err <= TARGET_ERROR:
                 break
@@ -347,6 +431,9 @@ def equal_error_solves(prob, results, xe, args):
     m = out["methods"]
     out["speedup_fcg_mixed_vs_fp64"] = round(m["fp64"]["ms"] /
                                              m["fcg_mixed"]["ms"], 3)
+    if "fcg_learned_mixed" in m:
+        out["speedup_fcg_learned_mixed_vs_fp64"] = round(
+            m["fp64"]["ms"] / m["fcg_learned_mixed"]["ms"], 3)
     return out
 
 
@@ -407,7 +494,7 @@ def run_workload(name, args):
     equal = None
     if (name == "spheres" and not args.no_check and not args.no_equal_error
             and "fp64" in results and "fcg_mixed" in results):
-        equal = equal_error_solves(prob, results, xe, args)
+        equal = equal_error_solves(prob, results, xe, args, net)
     fft = None
     if name == "channel":
         log(f"[bench] {name}: FFT Poisson solver")
@@ -421,12 +508,24 @@ def run_workload(name, args):
         log(f"[bench] {name}: CPU baseline sample")
         iters = results.get("fp64", next(iter(results.values())))["iters"]
         cpu = cpu_baseline(prob, iters, args.cpu_sample_iters, args.n)
+        if args.cpu_full:
+            log(f"[bench] {name}: whole CPU solve on {cpu['cores']} cores")
+            full = cpu_full_solve(prob)
+            cpu["full_solve"] = full
+            cpu["full_solve_vs_scaled_sample"] = round(
+                full["ms"] / cpu["value"], 3)
+            log(f"[bench] {name}: whole CPU solve {full['ms'] / 1e3:.1f} s, "
+                f"{full['iters']} iterations (scaled sample "
+                f"{cpu['value'] / 1e3:.1f} s)")
     best = min(results, key=lambda k: results[k]["ms"])
     out = {
         "workload": w["desc"].format(n=args.n),
         "best": best,
         "roofline": roofline(results.get("fp64"), w["kernel"],
                              traffic_from_profiles(w["traffic"])),
+        "vcycle": vcycle_roofline(results.get("fp64"), "fp64"),
+        "vcycle_mixed": vcycle_roofline(results.get("mixed"), "mixed"),
+        "level0_kernels": kernel_rooflines(results.get("fp64")),
         "cpu_baseline": cpu,
         "fp64_reference": ref_note,
         "solves": {k: {"ms": round(v["ms"], 3),
@@ -649,6 +748,8 @@ def main():
     ap.add_argument("--cpu-sample-iters", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true",
                     help="skip the CPU baseline")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="also time the whole FP64 solve on the CPU (minutes)")
     ap.add_argument("--no-check", action="store_true",
                     help="skip the FP64 reference solution")
     ap.add_argument("--only", choices=["fp64", "mixed", "fcg_fp64", "fcg_mixed",
@@ -710,6 +811,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": main_out["workload"], "method": best},
         "roofline": main_out["roofline"],
+        "vcycle": main_out["vcycle"],
+        "vcycle_mixed": main_out["vcycle_mixed"],
+        "level0_kernels": main_out["level0_kernels"],
         "cpu_baseline": main_out["cpu_baseline"],
         "fp64_reference": main_out["fp64_reference"],
         "solves": main_out["solves"],
@@ -718,6 +822,23 @@ def main():
         line["speedup_vs_plain_fp64_mg"] = main_out["speedup_vs_plain_fp64_mg"]
     if "equal_error" in main_out:
         line["equal_error"] = main_out["equal_error"]
+        # the metric's target as one figure per method: relative residual
+        # <= 1e-8 AND error against the FP64 reference <= 1e-8
+        tt = {}
+        for m, v in main_out["solves"].items():
+            e = main_out["equal_error"]["methods"].get(m)
+            err = v.get("err_vs_fp64_reference")
+            if err is not None and err <= TARGET_ERROR:
+                tt[m] = {"ms": v["ms"], "rel_tol": 1e-8,
+                         "err_vs_fp64_reference": err}
+            elif e is not None:
+                tt[m] = {"ms": e["ms"], "rel_tol": e["rel_tol"],
+                         "err_vs_fp64_reference": e["err_vs_fp64_reference"]}
+        if "fp64" in tt:
+            for m in tt:
+                tt[m]["speedup_vs_fp64"] = round(tt["fp64"]["ms"] /
+                                                 tt[m]["ms"], 3)
+        line["time_to_residual_and_error_1e-8"] = tt
     if "seeds" in main_out:
         line["seeds"] = main_out["seeds"]
     if "learned" in main_out:
