@@ -612,6 +612,7 @@ def run_batch(args):
         times.append(time.perf_counter() - t0)
     ms = 1e3 * sum(times) / len(times)
     launches, wg = batch.info()
+    _, reshaped = batch.tuning()
     xb = [t.cpu().numpy() for t in xs]
     iters = sorted(s.iters for s in st)
     rel = max(s.final_residual / s.initial_residual for s in st
@@ -634,6 +635,7 @@ def run_batch(args):
         ftimes.append(time.perf_counter() - t0)
     fms = 1e3 * sum(ftimes) / len(ftimes)
     flaunches, _ = batch.info()
+    _, freshaped = batch.tuning()
     xf = [t.cpu().numpy() for t in xs]
     fiters = sorted(s_.iters for s_ in fst)
     frel = max(s_.final_residual / s_.initial_residual for s_ in fst
@@ -708,7 +710,8 @@ def run_batch(args):
                                                    iters[-1]],
                      "max_final_rel_residual": rel,
                      "max_err_vs_fp64_1e-12": err,
-                     "launches_per_iteration": launches},
+                     "launches_per_iteration": launches,
+                     "launches_autotuned_to_other_tile_shape": reshaped},
         "fcg_mixed": {"solves_per_s": round(n / fms * 1e3, 1),
                       "ms_per_batch": round(fms, 2),
                       "iterations_min_median_max": [fiters[0],
@@ -716,7 +719,8 @@ def run_batch(args):
                                                     fiters[-1]],
                       "max_final_rel_residual": frel,
                       "max_err_vs_fp64_1e-12": ferr,
-                      "launches_per_iteration": flaunches},
+                      "launches_per_iteration": flaunches,
+                      "launches_autotuned_to_other_tile_shape": freshaped},
         "subset": {"systems": m,
                    "one_after_another_ms": round(seq_ms, 2),
                    "batched_ms": round(sub_ms, 2),

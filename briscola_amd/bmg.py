@@ -41,6 +41,7 @@ EXPORTS = [
     "bmg_split_source", "bmg_split_source_device",
     "bmg_set_boundary_values", "bmg_set_boundary_values_device",
     "bmg_renew_source_terms", "bmg_time_op", "bmg_face_flux_noalloc",
+    "bmg_batch_tuning", "bmg_batch_set_tile_shape",
 ]
 
 
@@ -132,6 +133,8 @@ def load():
                                 ctypes.POINTER(Stats)]),
         "bmg_batch_info": (I, [P, IP, ctypes.POINTER(ctypes.c_long)]),
         "bmg_batch_set_autotune": (I, [P, I]),
+        "bmg_batch_tuning": (I, [P, IP, IP]),
+        "bmg_batch_set_tile_shape": (I, [P, I]),
         "bmg_face_flux": (I, [P, DP, DP, DP, DP, DP, DP, DP]),
         "bmg_face_flux_noalloc": (I, [P, DP, DP, DP, DP, DP, DP, DP]),
         "bmg_face_flux_device": (I, [P, P, P, P, P, P, P, P]),
@@ -407,7 +410,7 @@ class Batch:
     are :class:`Context` objects with their operators set; they must outlive
     the batch."""
 
-    def __init__(self, contexts, autotune=True):
+    def __init__(self, contexts, autotune=True, tile_shape=0):
         self.lib = load()
         self.contexts = list(contexts)
         arr = (ctypes.c_void_p * len(self.contexts))(
@@ -418,6 +421,8 @@ class Batch:
         self.ptr = p
         if not autotune:
             _check(self.lib.bmg_batch_set_autotune(self.ptr, 0))
+        if tile_shape:
+            _check(self.lib.bmg_batch_set_tile_shape(self.ptr, tile_shape))
 
     def close(self):
         if getattr(self, "ptr", None):
@@ -449,6 +454,15 @@ class Batch:
         _check(self.lib.bmg_batch_info(self.ptr, ctypes.byref(n),
                                        ctypes.byref(w)))
         return n.value, w.value
+
+    def tuning(self):
+        """(launches, launches the autotuner moved to another tile shape)
+        of the last solve's merged iteration."""
+        n = ctypes.c_int(0)
+        k = ctypes.c_int(0)
+        _check(self.lib.bmg_batch_tuning(self.ptr, ctypes.byref(n),
+                                         ctypes.byref(k)))
+        return n.value, k.value
 
 
 class FFT:

@@ -17,6 +17,7 @@
 #include "bmg_context.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -45,7 +46,11 @@ struct Merged
     size_t shm = 0;
     size_t argSize = 0, align = 8;
     std::vector<unsigned char> args;   // argument structs, argSize apart
+    std::vector<unsigned char> args0;  // the same as recorded
+    int (*retile)(unsigned char*, int, unsigned*) = nullptr;
+    int ws = 0;                        // tile shape picked (0: recorded)
     std::vector<BatchItem> items;      // every system's share
+    std::vector<std::array<unsigned, 3>> grid0;   // recorded grids
     std::vector<int> sys;              // the system of each share
     unsigned total = 0;                // virtual workgroups, all systems
     size_t argsOff = 0, itemsOff = 0;  // offsets in the plan's regions
@@ -64,6 +69,7 @@ struct BatchPlan
     unsigned char* dev = nullptr;
     unsigned char* staging = nullptr;   // pinned copy of the shares region
     size_t sharesOff = 0, sharesBytes = 0, statesOff = 0;
+    unsigned defGrid = 0;   // persistent grid before tuning
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 
@@ -146,6 +152,7 @@ struct BatchPlan
                 if (m.items.empty())
                 {
                     m.many = r.many;
+                    m.retile = r.retile;
                     m.argSize = r.args.size();
                     m.align = r.align;
                 }
@@ -158,18 +165,24 @@ struct BatchPlan
                 it.n = wg;
                 it.arg = int(m.items.size());
                 m.items.push_back(it);
+                m.grid0.push_back({r.grid[0], r.grid[1], r.grid[2]});
                 m.sys.push_back(int(s));
                 m.args.insert(m.args.end(), r.args.begin(), r.args.end());
                 // each share starts at a multiple of 8 (its XCD order)
                 m.total += unsigned(alignUp(wg, 8));
             }
-            if (!m.items.empty()) launches.push_back(std::move(m));
+            if (!m.items.empty())
+            {
+                m.args0 = m.args;
+                launches.push_back(std::move(m));
+            }
         }
     }
 
     // Device copies of the argument structs and shares (all systems on).
     void layout(size_t n, unsigned grid)
     {
+        defGrid = grid;
         size_t bytes = 0;
         for (auto& m : launches)
         {
@@ -189,15 +202,51 @@ struct BatchPlan
         statesOff = alignUp(sh, 16);
         sharesBytes = statesOff + launches.size()*sizeof(BatchState);
         HIPCK(hipHostMalloc(&staging, std::max<size_t>(sharesBytes, 1)));
+        HIPCK(hipMalloc(&dev, sharesOff + std::max<size_t>(sharesBytes, 1)));
+        uploadAll(n);
+    }
+
+    // Argument structs and the shares of every system to the device.
+    void uploadAll(size_t n)
+    {
         std::vector<unsigned char> host(sharesOff, 0);
         for (auto& m : launches)
             std::memcpy(host.data() + m.argsOff, m.args.data(), m.args.size());
-        HIPCK(hipMalloc(&dev, sharesOff + std::max<size_t>(sharesBytes, 1)));
         HIPCK(hipMemcpy(dev, host.data(), host.size(), hipMemcpyHostToDevice));
         std::vector<char> all(n, 1);
         writeShares(all);
         HIPCK(hipMemcpy(dev + sharesOff, staging, sharesBytes,
                         hipMemcpyHostToDevice));
+    }
+
+    // Tile shape ws (3..6: rows of 2^ws cells per wavefront row; 0: the
+    // recorded shapes) for every share of launch m whose kernel takes it;
+    // the others keep their recorded tiles.  Host copies only.
+    void applyShape(Merged& m, int ws)
+    {
+        m.args = m.args0;
+        m.total = 0;
+        for (size_t j = 0; j < m.items.size(); j++)
+        {
+            BatchItem& it = m.items[j];
+            unsigned char* a = m.args.data() + size_t(it.arg)*m.argSize;
+            unsigned g[3];
+            if (ws != 0 && m.retile && m.retile(a, ws, g))
+            {
+                it.gx = g[0];
+                it.gy = g[1];
+                it.n = g[0]*g[1]*g[2];
+            }
+            else
+            {
+                it.gx = m.grid0[j][0];
+                it.gy = m.grid0[j][1];
+                it.n = m.grid0[j][0]*m.grid0[j][1]*m.grid0[j][2];
+            }
+            it.start = m.total;
+            m.total += unsigned(alignUp(it.n, 8));
+        }
+        m.ws = ws;
     }
 
     // Shares of the systems that are on, renumbered from 0, into staging.
@@ -261,43 +310,103 @@ struct BatchPlan
         HIPCK(hipStreamSynchronize(s));
     }
 
-    // Picks the persistent grid of every launch, 4, 8, 16 or 32 workgroups
-    // per CU (never more than its workgroups), by the fastest of timed runs
-    // of the whole sequence.
-    void autotune(hipStream_t s, int cus)
+    // Runs the sequence once with an event after every launch; ms[i] is
+    // launch i's time.
+    void timeLaunches(hipStream_t s, std::vector<hipEvent_t>& ev,
+                      std::vector<float>& ms) const
     {
-        const unsigned cand[4] = {unsigned(4*cus), unsigned(8*cus),
-                                  unsigned(16*cus), unsigned(32*cus)};
+        const size_t n = launches.size();
+        HIPCK(hipEventRecord(ev[0], s));
+        for (size_t i = 0; i < n; i++)
+        {
+            launch(i, s);
+            HIPCK(hipEventRecord(ev[i + 1], s));
+        }
+        HIPCK(hipGetLastError());
+        HIPCK(hipStreamSynchronize(s));
+        ms.assign(n, 0.0f);
+        for (size_t i = 0; i < n; i++)
+            HIPCK(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
+    }
+
+    // Picks, for every launch, first the tile shape of its shares (rows of
+    // 64, 32, 16 or 8 cells per wavefront row, or the recorded shapes; for
+    // the kernels that take several: red-black phases, residuals without a
+    // norm, restriction, prolongation), then its persistent grid, 4, 8, 16
+    // or 32 workgroups per CU (never more than its workgroups), each by the
+    // fastest of timed runs of the whole sequence (the second of two).
+    // Every launch that takes several tile shapes to shape ws (3..6).
+    void fixShape(int ws, size_t nsys)
+    {
+        for (auto& m : launches)
+            if (m.retile)
+            {
+                applyShape(m, ws);
+                m.grid = std::min(defGrid, m.total);
+            }
+        uploadAll(nsys);
+    }
+
+    void autotune(hipStream_t s, int cus, size_t nsys, int fixedShape = 0)
+    {
         const size_t n = launches.size();
         std::vector<hipEvent_t> ev(n + 1);
         for (auto& e : ev) HIPCK(hipEventCreate(&e));
-        std::vector<float> best(n, 1e30f);
-        std::vector<unsigned> pick(n, 0);
+        std::vector<float> ms;
         try
         {
+            bool shapes = false;
+            for (const auto& m : launches) shapes = shapes || m.retile;
+            if (fixedShape != 0)
+                fixShape(fixedShape, nsys);
+            else if (shapes)
+            {
+                const int cand[5] = {0, 6, 5, 4, 3};
+                std::vector<float> best(n, 1e30f);
+                std::vector<int> pick(n, 0);
+                for (int rep = 0; rep < 2; rep++)
+                    for (int c : cand)
+                    {
+                        for (auto& m : launches)
+                            if (m.retile)
+                            {
+                                applyShape(m, c);
+                                m.grid = std::min(defGrid, m.total);
+                            }
+                        uploadAll(nsys);
+                        timeLaunches(s, ev, ms);
+                        for (size_t i = 0; i < n; i++)
+                            if (rep > 0 && launches[i].retile && ms[i] < best[i])
+                            {
+                                best[i] = ms[i];
+                                pick[i] = c;
+                            }
+                    }
+                for (size_t i = 0; i < n; i++)
+                    if (launches[i].retile)
+                    {
+                        applyShape(launches[i], pick[i]);
+                        launches[i].grid = std::min(defGrid, launches[i].total);
+                    }
+                uploadAll(nsys);
+            }
+            const unsigned cand[4] = {unsigned(4*cus), unsigned(8*cus),
+                                      unsigned(16*cus), unsigned(32*cus)};
+            std::vector<float> best(n, 1e30f);
+            std::vector<unsigned> pick(n, 0);
             for (int rep = 0; rep < 2; rep++)
                 for (unsigned c : cand)
                 {
-                    HIPCK(hipEventRecord(ev[0], s));
+                    for (auto& m : launches) m.grid = std::min(c, m.total);
+                    timeLaunches(s, ev, ms);
                     for (size_t i = 0; i < n; i++)
-                    {
-                        launches[i].grid = std::min(c, launches[i].total);
-                        launch(i, s);
-                        HIPCK(hipEventRecord(ev[i + 1], s));
-                    }
-                    HIPCK(hipGetLastError());
-                    HIPCK(hipStreamSynchronize(s));
-                    for (size_t i = 0; i < n; i++)
-                    {
-                        float ms = 0.0f;
-                        HIPCK(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
-                        if (rep > 0 && ms < best[i])
+                        if (rep > 0 && ms[i] < best[i])
                         {
-                            best[i] = ms;
+                            best[i] = ms[i];
                             pick[i] = launches[i].grid;
                         }
-                    }
                 }
+            for (size_t i = 0; i < n; i++) launches[i].grid = pick[i];
         }
         catch (...)
         {
@@ -305,7 +414,14 @@ struct BatchPlan
             throw;
         }
         for (auto& e : ev) (void)hipEventDestroy(e);
-        for (size_t i = 0; i < n; i++) launches[i].grid = pick[i];
+    }
+
+    // Launches whose tile shape the tuner changed from the recorded one.
+    int reshaped() const
+    {
+        int k = 0;
+        for (const auto& m : launches) k += m.ws != 0;
+        return k;
     }
 
     void capture(hipStream_t s)
@@ -352,6 +468,7 @@ struct bmg_batch
     std::vector<int> fcgMixed;
     bool lastFcg = false;   // the last solve ran flexible CG
     bool tune = true;
+    int shape = 0;   // tile shape of every launch that takes one (0: tuned)
     int cus = 256;
 
     ~bmg_batch()
@@ -418,7 +535,10 @@ void batchFCG(bmg_batch* B, double* const* db, double* const* dx,
             B->fcgMixed[s] = int(mixedOf(B->ctx[s]));
         }
         // timing runs before the solve's fields are initialised below
-        if (B->tune) B->fcgIter->autotune(B->stream, B->cus);
+        if (B->tune)
+            B->fcgIter->autotune(B->stream, B->cus, n, B->shape);
+        else if (B->shape != 0)
+            B->fcgIter->fixShape(B->shape, n);
         B->fcgIter->capture(B->stream);
     }
     // copies of x and b, initial residual, sum of x
@@ -627,7 +747,10 @@ void batch_solve(bmg_batch* B, double* const* db, double* const* dx,
                     B->iterMixed[s] = int(mixedOf(B->ctx[s]));
                 }
                 // timing runs before the solve's fields are initialised below
-                if (B->tune) B->iter->autotune(B->stream, B->cus);
+                if (B->tune)
+                    B->iter->autotune(B->stream, B->cus, n, B->shape);
+                else if (B->shape != 0)
+                    B->iter->fixShape(B->shape, n);
                 B->iter->capture(B->stream);
             }
             // initial residual and the sum of x
@@ -734,6 +857,24 @@ void batch_info(const bmg_batch* B, int* launches, long* workgroups)
     const BatchPlan* p = B->lastFcg ? B->fcgIter.get() : B->iter.get();
     if (launches) *launches = p ? int(p->launches.size()) : 0;
     if (workgroups) *workgroups = p ? long(p->workgroups()) : 0;
+}
+
+void batch_tuning(const bmg_batch* B, int* launches, int* reshaped)
+{
+    if (!B) throw Error(BMG_ERR_ARG, "null batch");
+    const BatchPlan* p = B->lastFcg ? B->fcgIter.get() : B->iter.get();
+    if (launches) *launches = p ? int(p->launches.size()) : 0;
+    if (reshaped) *reshaped = p ? p->reshaped() : 0;
+}
+
+void batch_set_tile_shape(bmg_batch* B, int ws)
+{
+    if (!B) throw Error(BMG_ERR_ARG, "null batch");
+    if (ws != 0 && (ws < 3 || ws > 6))
+        throw Error(BMG_ERR_ARG, "tile shape must be 0 or 3..6");
+    B->shape = ws;
+    B->iter.reset();
+    B->fcgIter.reset();
 }
 
 void batch_destroy(bmg_batch* B) { delete B; }

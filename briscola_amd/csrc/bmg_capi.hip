@@ -2113,6 +2113,16 @@ int bmg_batch_info(const bmg_batch* b, int* launches, long* workgroups)
     return run([&] { batch_info(b, launches, workgroups); });
 }
 
+int bmg_batch_tuning(const bmg_batch* b, int* launches, int* reshaped)
+{
+    return run([&] { batch_tuning(b, launches, reshaped); });
+}
+
+int bmg_batch_set_tile_shape(bmg_batch* b, int ws)
+{
+    return run([&] { batch_set_tile_shape(b, ws); });
+}
+
 int bmg_batch_set_autotune(bmg_batch* b, int enable)
 {
     return run([&] { batch_set_autotune(b, enable); });

@@ -405,6 +405,8 @@ void batch_solve(bmg_batch* b, double* const* db, double* const* dx,
                  bool constMatrix, bmg_stats* st);
 void batch_info(const bmg_batch* b, int* launches, long* workgroups);
 void batch_set_autotune(bmg_batch* b, int on);
+void batch_tuning(const bmg_batch* b, int* launches, int* reshaped);
+void batch_set_tile_shape(bmg_batch* b, int ws);
 void batch_destroy(bmg_batch* b);
 
 }  // namespace bmg

@@ -113,6 +113,12 @@ struct LaunchRecord {
     size_t align;
     std::vector<unsigned char> args;
     void (*many)(const BatchLaunch&, hipStream_t);
+    // Rewrites the tile shape of the argument struct in place for rows of
+    // 2^ws cells per wavefront row (3 <= ws <= 6) and sets its grid;
+    // returns 0, changing nothing, when the kernel keeps its tiles (null:
+    // the kernel has one tile shape).  The per-cell arithmetic is the same
+    // under every shape.
+    int (*retile)(unsigned char* args, int ws, unsigned grid[3]);
 };
 struct LaunchRecorder {
     std::vector<LaunchRecord> recs;

@@ -11,6 +11,8 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
+#include <utility>
 
 namespace bmg {
 
@@ -237,6 +239,10 @@ __device__ __forceinline__ int tile_index(const Tiles& t, int bx, int by,
     return bx + t.gx*(by + t.gy*bz);
 }
 
+// Tiles over nplanes planes of nrows rows of nper positions, a wavefront
+// covering 64 >> ws rows of 2^ws positions (ws = 6: one row segment of 64).
+Tiles packed_tiles(int nper, int nrows, int nplanes, int ws);
+
 // Cell-pair kernels: thread (p, J) owns the two cells K = 2p+1 and
 // K = 2p+2 of padded row (I,J).  Cell A (K = 2p+1) sits at position p of
 // colour cA = (I+J+1)&1, cell B (K = 2p+2) at position p+1 of colour
@@ -426,6 +432,13 @@ struct KRbgs
     {
         rbgs_body<T, OP>(bid, a.L, a.op, a.xc, a.xo, a.bcl, a.colour, a.rg,
                          a.tl);
+    }
+    static bool retile(Args& a, int ws)
+    {
+        const int nK = a.rg.E[2] - a.rg.S[2];
+        a.tl = packed_tiles((nK + 1)/2, a.rg.E[1] - a.rg.S[1],
+                            a.rg.E[0] - a.rg.S[0], ws);
+        return true;
     }
     static void launch(dim3 g, size_t shm, hipStream_t s, const Args& a)
     {
@@ -1016,6 +1029,13 @@ struct KResidual
         residual_body<T, OP, NORM>(bid, a.L, a.op, a.x, a.b, a.r, a.tl,
                                    a.partials);
     }
+    // the norm's partial sums keep one tile shape (their summation order)
+    static bool retile(Args& a, int ws)
+    {
+        if (NORM) return false;
+        a.tl = packed_tiles((a.L.N[2] + 1)/2, a.L.N[1], a.L.N[0], ws);
+        return true;
+    }
     static void launch(dim3 g, size_t shm, hipStream_t s, const Args& a)
     {
         hipLaunchKernelGGL((k_residual<T, OP, NORM>), g, dim3(bx, by), shm, s,
@@ -1154,6 +1174,11 @@ struct KRestrict
     __device__ static void run(const Bid bid, const Args& a)
     {
         restrict_body<T>(bid, a.Lf, a.Lc, a.tv, a.rf, a.bc, a.tl);
+    }
+    static bool retile(Args& a, int ws)
+    {
+        a.tl = packed_tiles((a.Lc.N[2] + 1)/2, a.Lc.N[1], a.Lc.N[0], ws);
+        return true;
     }
     static void launch(dim3 g, size_t shm, hipStream_t s, const Args& a)
     {
@@ -1298,6 +1323,11 @@ struct KProlong
     __device__ static void run(const Bid bid, const Args& a)
     {
         prolong_body<T>(bid, a.Lf, a.Lc, a.tv, a.xc, a.xf, a.tl);
+    }
+    static bool retile(Args& a, int ws)
+    {
+        a.tl = packed_tiles((a.Lf.N[2] + 1)/2, a.Lf.N[1], a.Lf.N[0], ws);
+        return true;
     }
     static void launch(dim3 g, size_t shm, hipStream_t s, const Args& a)
     {
@@ -1448,6 +1478,11 @@ struct KProlong8
             prolong8_impl<T, true>(bid, a.Lf, a.Lc, a.tv, a.xc, a.xf, a.tl);
         else
             prolong8_impl<T, false>(bid, a.Lf, a.Lc, a.tv, a.xc, a.xf, a.tl);
+    }
+    static bool retile(Args& a, int ws)
+    {
+        a.tl = packed_tiles(a.Lc.N[2], a.Lc.N[1], a.Lc.N[0], ws);
+        return true;
     }
     static void launch(dim3 g, size_t shm, hipStream_t s, const Args& a)
     {
@@ -3538,6 +3573,15 @@ int row_pack_shift(int n)
     return best;
 }
 
+Tiles packed_tiles(int nper, int nrows, int nplanes, int ws)
+{
+    const int rows = BY*(64 >> ws);
+    Tiles t = make_tiles((nper + (1 << ws) - 1) >> ws, (nrows + rows - 1)/rows,
+                         nplanes);
+    t.ws = ws;
+    return t;
+}
+
 inline dim3 tiles_grid(const Tiles& t) { return dim3(8*t.per); }
 
 inline int tiles_count(const Tiles& t) { return t.gx*t.gy*t.gz; }
@@ -3773,6 +3817,27 @@ __global__ __launch_bounds__(K::bx*K::by) void k_many(
 template <class K>
 char kind_tag;
 
+// Kinds whose tile shape the batch autotuner may change (K::retile).
+template <class K, class = void>
+struct HasRetile : std::false_type {};
+template <class K>
+struct HasRetile<K, decltype(void(K::retile(
+                        std::declval<typename K::Args&>(), 0)))>
+    : std::true_type {};
+
+template <class K>
+int retile_args(unsigned char* bytes, int ws, unsigned grid[3])
+{
+    typename K::Args a;
+    std::memcpy(&a, bytes, sizeof(a));
+    if (ws < 3 || ws > 6 || !K::retile(a, ws)) return 0;
+    std::memcpy(bytes, &a, sizeof(a));
+    grid[0] = tiles_grid(a.tl).x;
+    grid[1] = 1;
+    grid[2] = 1;
+    return 1;
+}
+
 template <class K>
 void launch_many(const BatchLaunch& b, hipStream_t s)
 {
@@ -3801,6 +3866,10 @@ void dispatch(dim3 g, size_t shm, hipStream_t s, const typename K::Args& a)
         rec.args.resize(sizeof(a));
         std::memcpy(rec.args.data(), &a, sizeof(a));
         rec.many = &launch_many<B>;
+        if constexpr (HasRetile<K>::value)
+            rec.retile = &retile_args<K>;
+        else
+            rec.retile = nullptr;
         r->recs.push_back(std::move(rec));
         return;
     }

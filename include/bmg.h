@@ -238,10 +238,23 @@ int bmg_solve_batch(bmg_batch* batch, double* const* d_b, double* const* d_x,
  * solve's outer iteration (0 before the first solve). */
 int bmg_batch_info(const bmg_batch* batch, int* launches,
                    long* workgroups);
-/* Launch-grid autotuning of the merged iteration (default on: every
- * launch takes the fastest persistent grid of 4, 8, 16 or 32 workgroups per
- * CU; off: 16 per CU).  Rebuilds the plan. */
+/* Autotuning of the merged iteration (default on): every launch of a
+ * kernel with several tile shapes (red-black phases, residuals without a
+ * norm, restriction, prolongation) first takes the fastest shape for all
+ * its shares -- rows of 64, 32, 16 or 8 cells per wavefront row, or the
+ * shapes recorded (the row width with the fewest idle lanes) -- then every
+ * launch the fastest persistent grid of 4, 8, 16 or 32 workgroups per CU;
+ * each by timing the whole sequence.  Off: recorded shapes, 16 workgroups
+ * per CU.  Rebuilds the plan.  The per-cell arithmetic, and so every
+ * result, is the same under every shape and grid. */
 int bmg_batch_set_autotune(bmg_batch* batch, int enable);
+/* Fixes the tile shape of every launch that takes several (ws 3..6: rows
+ * of 2^ws cells per wavefront row) instead of letting the autotuner pick
+ * it; 0 (default): picked by timing.  Rebuilds the plan. */
+int bmg_batch_set_tile_shape(bmg_batch* batch, int ws);
+/* Launches of the last solve's merged iteration and how many of them run
+ * another tile shape than recorded. */
+int bmg_batch_tuning(const bmg_batch* batch, int* launches, int* reshaped);
 
 /* ---- FFT Poisson solver (the reference's FFTPoissonSolver,
  * FFTPoissonSolver.C:97-465): div(grad x) = -b in finite-difference form on

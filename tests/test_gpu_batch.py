@@ -4,8 +4,8 @@ an iteration is one launch over all systems, so each system must still get
 exactly what its own solve gives it: the solution, the returned source and
 the iteration count and residuals of the batched solve are compared bitwise
 with bmg_solve_device of the same system alone (whose FP64 path is itself
-bitwise the oracle's, tests/test_gpu_parity.py).  Also: the launch grids the
-autotuner picks do not change any result, a second solve replays the
+bitwise the oracle's, tests/test_gpu_parity.py).  Also: the launch grids and
+tile shapes the autotuner picks do not change any result, a second solve replays the
 captured iteration, and a system with a zero source drops out at once.
 Batched flexible CG is held to the same bar against each system's own FCG
 solve."""
@@ -75,10 +75,11 @@ def solve_alone(systems, krylov=bmg.KRYLOV_NONE):
     return out
 
 
-def solve_batched(systems, autotune=True, repeat=1, krylov=bmg.KRYLOV_NONE):
+def solve_batched(systems, autotune=True, repeat=1, krylov=bmg.KRYLOV_NONE,
+                  tile_shape=0):
     probs = [problem(i, shape, faces) for i, shape, faces, _ in systems]
     ctxs = [context(p, s[3], krylov) for p, s in zip(probs, systems)]
-    batch = bmg.Batch(ctxs, autotune=autotune)
+    batch = bmg.Batch(ctxs, autotune=autotune, tile_shape=tile_shape)
     runs = []
     for r in range(repeat):
         bs = [dev(p.b) for p in probs]
@@ -123,6 +124,17 @@ def test_launch_grids_do_not_change_results(gpu, alone):
     runs, _ = solve_batched(SYSTEMS[:6], autotune=False)
     for k, (got, ref) in enumerate(zip(runs[0], alone[:6])):
         assert got[2] == ref[2] and same(got[0], ref[0]), k
+
+
+@pytest.mark.parametrize("ws", [3, 4, 5, 6])
+def test_tile_shapes_do_not_change_results(gpu, alone, ws):
+    """Every shape the autotuner may pick (rows of 2^ws cells per wavefront
+    row for the phases, residuals, restriction and prolongation), forced on
+    every launch, with MG iterations."""
+    runs, _ = solve_batched(SYSTEMS, autotune=False, tile_shape=ws)
+    for k, (got, ref) in enumerate(zip(runs[0], alone)):
+        assert got[2] == ref[2], (ws, k, got[2], ref[2])
+        assert same(got[0], ref[0]) and same(got[1], ref[1]), (ws, k)
 
 
 def test_zero_source_system_drops_out(gpu):
