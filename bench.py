@@ -109,10 +109,11 @@ def traffic_from_profiles(fname):
 
 
 def run_gpu(prob, precision, krylov, steps, warmup, profile, reps=50,
-            rel_tol=1e-8, max_iter=None, net=None):
+            rel_tol=1e-8, max_iter=None, net=None, deflation=0):
     import torch
     from briscola_amd import bmg
     ctx = bmg.Context(prob.vertices, prob.bc, prob.bc_value,
+                      deflation=deflation,
                       **solver_opts(precision, krylov, rel_tol, max_iter))
     ctx.set_laplacian(prob.lam)
     if net is not None:
@@ -147,10 +148,11 @@ def run_gpu(prob, precision, krylov, steps, warmup, profile, reps=50,
                 for name, op in (("residual", bmg.TIME_RESIDUAL),
                                  ("restrict", bmg.TIME_RESTRICT),
                                  ("prolong", bmg.TIME_PROLONG))}
+    defl = ctx.deflation_vectors()
     ctx.close()
     return dict(ms=1e3 * wall / steps, device_ms=dev_ms / steps,
                 iters=stats.iters, final_residual=stats.final_residual, x=x,
-                prof=prof, vcycle=vcyc, kernels=kern,
+                prof=prof, vcycle=vcyc, kernels=kern, deflation_vectors=defl,
                 learned_iters=stats.learned_iters,
                 learned_fallback=stats.learned_fallback,
                 learned_guess_residual=stats.learned_guess_residual)
@@ -408,6 +410,8 @@ def equal_error_solves(prob, results, xe, args, net=None):
     out = {"target_rel_error": TARGET_ERROR, "methods": {}}
     plan = [("fp64", bmg.FP64, bmg.KRYLOV_NONE),
             ("fcg_mixed", bmg.MIXED, bmg.KRYLOV_FCG)]
+    if "fcg_deflated_mixed" in results:
+        plan.append(("fcg_deflated_mixed", bmg.MIXED, bmg.KRYLOV_FCG))
     if net is not None and "fcg_learned_mixed" in results:
         plan.append(("fcg_learned_mixed", bmg.MIXED, bmg.KRYLOV_FCG_LEARNED))
     for mname, prec, kry in plan:
@@ -417,7 +421,8 @@ def equal_error_solves(prob, results, xe, args, net=None):
             log(f"[bench] spheres {mname} to relTol {tol:.0e} (equal error)")
             r = run_gpu(prob, prec, kry, 1, 1, False, rel_tol=tol,
                         max_iter=4000,
-                        net=net if kry == bmg.KRYLOV_FCG_LEARNED else None)
+                        net=net if kry == bmg.KRYLOV_FCG_LEARNED else None,
+                        deflation=int("deflated" in mname))
             err = direct.rel_error(r["x"][1:-1, 1:-1, 1:-1], xe, True)
             if err <= TARGET_ERROR:
                 break
@@ -448,6 +453,8 @@ def run_workload(name, args):
             ("mixed", bmg.MIXED, bmg.KRYLOV_NONE),
             ("fcg_fp64", bmg.FP64, bmg.KRYLOV_FCG),
             ("fcg_mixed", bmg.MIXED, bmg.KRYLOV_FCG)]
+    if name == "spheres":
+        plan.append(("fcg_deflated_mixed", bmg.MIXED, bmg.KRYLOV_FCG))
     learned = None
     if name == "spheres" and not args.no_learned:
         plan.append(("fcg_learned_mixed", bmg.MIXED, bmg.KRYLOV_FCG_LEARNED))
@@ -476,7 +483,8 @@ def run_workload(name, args):
         results[mname] = run_gpu(prob, prec, kry, args.steps, args.warmup,
                                  profile=(kry == bmg.KRYLOV_NONE),
                                  net=net if kry == bmg.KRYLOV_FCG_LEARNED
-                                 else None)
+                                 else None,
+                                 deflation=int("deflated" in mname))
         r = results[mname]
         log(f"[bench] {name} {mname}: {r['ms']:.2f} ms/solve, {r['iters']} "
             f"iterations, final rel. residual {r['final_residual']:.3e}")
@@ -531,6 +539,7 @@ def run_workload(name, args):
         "solves": {k: {"ms": round(v["ms"], 3),
                        "device_ms": round(v["device_ms"], 3),
                        "iters": v["iters"],
+                       "deflation_vectors": v["deflation_vectors"],
                        "final_rel_residual": v["final_residual"],
                        "err_vs_fp64_reference": v.get("err_vs_fp64_reference")}
                    for k, v in results.items()},
@@ -757,6 +766,7 @@ def main():
     ap.add_argument("--no-check", action="store_true",
                     help="skip the FP64 reference solution")
     ap.add_argument("--only", choices=["fp64", "mixed", "fcg_fp64", "fcg_mixed",
+                                       "fcg_deflated_mixed",
                                        "fcg_learned_mixed"], default=None)
     ap.add_argument("--seeds", type=int, default=5,
                     help="headline seeds 0..S-1 for the median (1: seed 0 only)")

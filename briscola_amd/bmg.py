@@ -41,7 +41,7 @@ EXPORTS = [
     "bmg_split_source", "bmg_split_source_device",
     "bmg_set_boundary_values", "bmg_set_boundary_values_device",
     "bmg_renew_source_terms", "bmg_time_op", "bmg_face_flux_noalloc",
-    "bmg_batch_tuning", "bmg_batch_set_tile_shape",
+    "bmg_batch_tuning", "bmg_batch_set_tile_shape", "bmg_deflation_vectors",
 ]
 
 
@@ -65,7 +65,7 @@ class Opts(ctypes.Structure):
                 ("cycle", ctypes.c_int), ("coarse_mode", ctypes.c_int),
                 ("coarse_level", ctypes.c_int), ("smoother", ctypes.c_int),
                 ("precision", ctypes.c_int), ("krylov", ctypes.c_int),
-                ("reserved", ctypes.c_int * 4)]
+                ("deflation", ctypes.c_int), ("reserved", ctypes.c_int * 3)]
 
 
 class Stats(ctypes.Structure):
@@ -135,6 +135,7 @@ def load():
         "bmg_batch_set_autotune": (I, [P, I]),
         "bmg_batch_tuning": (I, [P, IP, IP]),
         "bmg_batch_set_tile_shape": (I, [P, I]),
+        "bmg_deflation_vectors": (I, [P, IP]),
         "bmg_face_flux": (I, [P, DP, DP, DP, DP, DP, DP, DP]),
         "bmg_face_flux_noalloc": (I, [P, DP, DP, DP, DP, DP, DP, DP]),
         "bmg_face_flux_device": (I, [P, P, P, P, P, P, P, P]),
@@ -388,6 +389,12 @@ class Context:
         _check(self.lib.bmg_face_flux(self.ptr, _dp(x), *ls,
                                       *[_dp(a) for a in out]))
         return out
+
+    def deflation_vectors(self):
+        """Number of deflation vectors of the current operator."""
+        n = ctypes.c_int(0)
+        _check(self.lib.bmg_deflation_vectors(self.ptr, ctypes.byref(n)))
+        return n.value
 
     def set_graphs(self, enable):
         _check(self.lib.bmg_set_graphs(self.ptr, int(enable)))

@@ -686,6 +686,9 @@ void batch_solve(bmg_batch* B, double* const* db, double* const* dx,
         if (c->opts.krylov == BMG_KRYLOV_FCG_LEARNED)
             throw Error(BMG_ERR_ARG, "the learned preconditioner has no "
                                      "batched form");
+        if (c->opts.deflation)
+            throw Error(BMG_ERR_ARG, "deflated flexible CG has no batched "
+                                     "form");
         if (c->opts.krylov != B->ctx[0]->opts.krylov)
             throw Error(BMG_ERR_ARG, "every system of a batch uses the same "
                                      "outer iteration (MG or flexible CG)");

@@ -20,6 +20,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 using namespace bmg;
@@ -868,6 +869,7 @@ void bmg_ctx::enqueueFCGIteration(int mode)
                            hScalarDev + 3);
         launch_update_p<double>(q0.L, dScalar, cpr(q0.x), Pv, stream);
     }
+    if (mode != 2 && deflating()) enqueueDeflate();
     HIPCK(hipGetLastError());
 }
 
@@ -992,6 +994,321 @@ void bmg_ctx::precondition(bool mixed)
     HIPCK(hipGraphLaunch(pgraphExec, stream));
 }
 
+// ---------------------------------------------------------------------------
+// Deflation of flexible CG by indicator vectors of high-coefficient cell
+// sets (opts.deflation).  Labels are found on the host from the face lambda
+// arrays when the operator is assembled: a cell's coefficient is the mean of
+// its six face lambdas; with a contrast of at least 100 between the extremes
+// the cells above their geometric mean are grouped into face-connected sets
+// (periodic faces wrap), and the largest sets of at least 8 cells, at most
+// 64, are kept.
+static const int DEFL_MAX = 64;
+static const long DEFL_MIN_CELLS = 8;
+
+void bmg_ctx::deflationLabels(const double* lx, const double* ly,
+                              const double* lz)
+{
+    deflLabel.clear();
+    deflM = 0;
+    const LevelDims& L0 = lv[0].L;
+    const int N0 = L0.N[0], N1 = L0.N[1], N2 = L0.N[2];
+    const long P1 = L0.P[1], P2 = L0.P[2];
+    const long n = long(N0)*N1*N2;
+    std::vector<double> m(n);
+    double lo = 1e300, hi = 0.0;
+    for (int i = 1; i <= N0; i++)
+        for (int j = 1; j <= N1; j++)
+            for (int k = 1; k <= N2; k++)
+            {
+                const long p = (i*P1 + j)*P2 + k;
+                const double v = (lx[p] + lx[p + P1*P2] + ly[p] + ly[p + P2] +
+                                  lz[p] + lz[p + 1])/6.0;
+                m[(long(i - 1)*N1 + (j - 1))*N2 + (k - 1)] = v;
+                lo = std::min(lo, v);
+                hi = std::max(hi, v);
+            }
+    if (!(lo > 0.0) || !(hi >= 100.0*lo)) return;
+    const double theta = std::sqrt(lo*hi);
+    std::vector<int> lab(n, -1);
+    std::vector<long> size, stack;
+    const int N[3] = {N0, N1, N2};
+    for (long c = 0; c < n; c++)
+    {
+        if (lab[c] != -1 || !(m[c] > theta)) continue;
+        const int id = int(size.size());
+        long cnt = 0;
+        lab[c] = id;
+        stack.assign(1, c);
+        while (!stack.empty())
+        {
+            const long u = stack.back();
+            stack.pop_back();
+            cnt++;
+            const int g[3] = {int(u/(long(N1)*N2)), int((u/N2) % N1),
+                              int(u % N2)};
+            for (int d = 0; d < 3; d++)
+                for (int sgn = -1; sgn <= 1; sgn += 2)
+                {
+                    int q[3] = {g[0], g[1], g[2]};
+                    q[d] += sgn;
+                    if (q[d] < 0 || q[d] >= N[d])
+                    {
+                        if (!periodic(2*d + (q[d] >= N[d]))) continue;
+                        q[d] = (q[d] + N[d]) % N[d];
+                    }
+                    const long v = (long(q[0])*N1 + q[1])*N2 + q[2];
+                    if (lab[v] == -1 && m[v] > theta)
+                    {
+                        lab[v] = id;
+                        stack.push_back(v);
+                    }
+                }
+        }
+        size.push_back(cnt);
+    }
+    // keep the largest sets of at least DEFL_MIN_CELLS cells
+    std::vector<int> order(size.size());
+    for (size_t i = 0; i < order.size(); i++) order[i] = int(i);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return size[a] > size[b]; });
+    std::vector<int> remap(size.size(), -1);
+    int kept = 0;
+    for (int id : order)
+    {
+        if (kept >= DEFL_MAX || size[id] < DEFL_MIN_CELLS) break;
+        remap[id] = kept++;
+    }
+    if (kept == 0) return;
+    for (long c = 0; c < n; c++)
+        if (lab[c] >= 0) lab[c] = remap[lab[c]];
+    deflLabel.swap(lab);
+    deflM = kept;
+}
+
+bool bmg_ctx::deflating() const
+{
+    return opts.deflation != 0 && deflM > 0 && deflOpGen == opGen &&
+           dDeflEinv != nullptr && launch_recorder() == nullptr;
+}
+
+// Device data of the current operator's deflation: the entries (A z_k)_i
+// (counted, written, sorted by component and cell on the host), E = Z^T A Z
+// from them, its inverse (Gauss-Jordan with partial pivoting), and the cell
+// lists of the components.
+void bmg_ctx::ensureDeflation()
+{
+    if (opts.deflation == 0 || deflM == 0 || deflOpGen == opGen) return;
+    freeDeflation();
+    Level& q0 = lv[0];
+    const LevelDims& L = q0.L;
+    const long n = long(L.N[0])*L.N[1]*L.N[2];
+    const int m = deflM;
+    int* dLab = devAlloc<int>(n);
+    unsigned* dCount = devAlloc<unsigned>(1);
+    std::vector<unsigned> hc;
+    std::vector<int> hk;
+    std::vector<double> hv;
+    try
+    {
+        HIPCK(hipMemcpy(dLab, deflLabel.data(), n*sizeof(int),
+                        hipMemcpyHostToDevice));
+        HIPCK(hipMemsetAsync(dCount, 0, sizeof(unsigned), stream));
+        launch_defl_entries(L, op<double>(0), dLab, dCount, 0, nullptr,
+                            nullptr, nullptr, stream);
+        unsigned cnt = 0;
+        HIPCK(hipMemcpyAsync(&cnt, dCount, sizeof(unsigned),
+                             hipMemcpyDeviceToHost, stream));
+        HIPCK(hipStreamSynchronize(stream));
+        unsigned* dc = devAlloc<unsigned>(std::max(cnt, 1u));
+        int* dk = devAlloc<int>(std::max(cnt, 1u));
+        double* dv = devAlloc<double>(std::max(cnt, 1u));
+        HIPCK(hipMemsetAsync(dCount, 0, sizeof(unsigned), stream));
+        launch_defl_entries(L, op<double>(0), dLab, dCount, cnt, dc, dk, dv,
+                            stream);
+        hc.resize(cnt);
+        hk.resize(cnt);
+        hv.resize(cnt);
+        HIPCK(hipMemcpyAsync(hc.data(), dc, cnt*sizeof(unsigned),
+                             hipMemcpyDeviceToHost, stream));
+        HIPCK(hipMemcpyAsync(hk.data(), dk, cnt*sizeof(int),
+                             hipMemcpyDeviceToHost, stream));
+        HIPCK(hipMemcpyAsync(hv.data(), dv, cnt*sizeof(double),
+                             hipMemcpyDeviceToHost, stream));
+        HIPCK(hipStreamSynchronize(stream));
+        devFree(dc);
+        devFree(dk);
+        devFree(dv);
+    }
+    catch (...)
+    {
+        devFree(dLab);
+        devFree(dCount);
+        throw;
+    }
+    devFree(dLab);
+    devFree(dCount);
+    // entries by component, then cell
+    std::vector<size_t> ord(hc.size());
+    for (size_t e = 0; e < ord.size(); e++) ord[e] = e;
+    std::sort(ord.begin(), ord.end(), [&](size_t a, size_t b) {
+        return hk[a] != hk[b] ? hk[a] < hk[b] : hc[a] < hc[b];
+    });
+    const long Hz = L.Hz, P1 = L.P[1];
+    auto labelOf = [&](unsigned code) -> int {
+        const long idx = long(code & 0x7fffffffu);
+        const int colour = int(code >> 31);
+        const int I = int(idx/(P1*Hz));
+        const long rem = idx - I*P1*Hz;
+        const int J = int(rem/Hz);
+        const int p = int(rem - J*Hz);
+        const int K = 2*p + ((colour - I - J) & 1);
+        return deflLabel[(long(I - 1)*L.N[1] + (J - 1))*L.N[2] + (K - 1)];
+    };
+    std::vector<unsigned> ec(ord.size());
+    std::vector<double> ev(ord.size());
+    std::vector<int> eoff(m + 1, 0);   // entries of component k
+    std::vector<double> E(size_t(m)*m, 0.0);
+    for (size_t e = 0; e < ord.size(); e++)
+    {
+        const size_t o = ord[e];
+        ec[e] = hc[o];
+        ev[e] = hv[o];
+        eoff[hk[o] + 1]++;
+        const int j = labelOf(hc[o]);
+        if (j >= 0) E[size_t(j)*m + hk[o]] += hv[o];
+    }
+    for (int k = 0; k < m; k++) eoff[k + 1] += eoff[k];
+    // E^-1 by Gauss-Jordan with partial pivoting
+    std::vector<double> A(E), Inv(size_t(m)*m, 0.0);
+    for (int i = 0; i < m; i++) Inv[size_t(i)*m + i] = 1.0;
+    for (int col = 0; col < m; col++)
+    {
+        int piv = col;
+        for (int r = col + 1; r < m; r++)
+            if (std::fabs(A[size_t(r)*m + col]) > std::fabs(A[size_t(piv)*m + col]))
+                piv = r;
+        if (!(std::fabs(A[size_t(piv)*m + col]) > 0.0))
+        {
+            deflM = 0;   // singular: solve without deflation
+            return;
+        }
+        if (piv != col)
+            for (int c2 = 0; c2 < m; c2++)
+            {
+                std::swap(A[size_t(piv)*m + c2], A[size_t(col)*m + c2]);
+                std::swap(Inv[size_t(piv)*m + c2], Inv[size_t(col)*m + c2]);
+            }
+        const double d = A[size_t(col)*m + col];
+        for (int c2 = 0; c2 < m; c2++)
+        {
+            A[size_t(col)*m + c2] /= d;
+            Inv[size_t(col)*m + c2] /= d;
+        }
+        for (int r = 0; r < m; r++)
+        {
+            if (r == col) continue;
+            const double f = A[size_t(r)*m + col];
+            if (f == 0.0) continue;
+            for (int c2 = 0; c2 < m; c2++)
+            {
+                A[size_t(r)*m + c2] -= f*A[size_t(col)*m + c2];
+                Inv[size_t(r)*m + c2] -= f*Inv[size_t(col)*m + c2];
+            }
+        }
+    }
+    // component cells
+    std::vector<std::vector<unsigned>> cells(m);
+    for (int I = 1; I <= L.N[0]; I++)
+        for (int J = 1; J <= L.N[1]; J++)
+            for (int K = 1; K <= L.N[2]; K++)
+            {
+                const int k = deflLabel[(long(I - 1)*L.N[1] + (J - 1))*L.N[2] +
+                                        (K - 1)];
+                if (k < 0) continue;
+                const unsigned idx = unsigned((I*P1 + J)*Hz + (K >> 1));
+                cells[k].push_back((unsigned((I + J + K) & 1) << 31) | idx);
+            }
+    std::vector<unsigned> cc;
+    std::vector<int> ccomp, coff(m + 1, 0);
+    for (int k = 0; k < m; k++)
+    {
+        cc.insert(cc.end(), cells[k].begin(), cells[k].end());
+        ccomp.insert(ccomp.end(), cells[k].size(), k);
+        coff[k + 1] = int(cc.size());
+    }
+    deflNCells = long(cc.size());
+    // chunks of at most DEFL_CHUNK list elements within each component
+    const int DEFL_CHUNK = 2048;
+    auto chunks = [&](const std::vector<int>& off, std::vector<int>& beg,
+                      std::vector<int>& end, std::vector<int>& choff) {
+        choff.assign(m + 1, 0);
+        for (int k = 0; k < m; k++)
+        {
+            for (int e = off[k]; e < off[k + 1]; e += DEFL_CHUNK)
+            {
+                beg.push_back(e);
+                end.push_back(std::min(off[k + 1], e + DEFL_CHUNK));
+            }
+            choff[k + 1] = int(beg.size());
+        }
+    };
+    std::vector<int> cb, ce, cco, eb, ee, eco;
+    chunks(coff, cb, ce, cco);
+    chunks(eoff, eb, ee, eco);
+    deflCellChunks = int(cb.size());
+    deflEntChunks = int(eb.size());
+    auto up = [&](auto*& dst, const auto& v) {
+        using E0 = std::remove_reference_t<decltype(*dst)>;
+        dst = devAlloc<E0>(std::max<size_t>(v.size(), 1));
+        if (!v.empty())
+            HIPCK(hipMemcpy(dst, v.data(), v.size()*sizeof(E0),
+                            hipMemcpyHostToDevice));
+    };
+    up(dDeflCell, cc);
+    up(dDeflCellComp, ccomp);
+    up(dDeflEntCell, ec);
+    up(dDeflEntVal, ev);
+    up(dDeflCellBeg, cb);
+    up(dDeflCellEnd, ce);
+    up(dDeflCellChunkOff, cco);
+    up(dDeflEntBeg, eb);
+    up(dDeflEntEnd, ee);
+    up(dDeflEntChunkOff, eco);
+    up(dDeflEinv, Inv);
+    dDeflPart = devAlloc<double>(size_t(std::max(std::max(deflCellChunks,
+                                                          deflEntChunks), 1)));
+    deflOpGen = opGen;
+    gen++;   // captured graphs must see the new buffers
+}
+
+// p -= Z E^-1 (A Z)^T p on the new search direction, in FP64.  With
+// P^T = I - Z E^-1 (A Z)^T a projection and every earlier direction already
+// A-orthogonal to Z, P^T (z + beta p) = P^T z + beta p and z.q is unchanged,
+// so this is deflating the preconditioned residual z, done on the FP64
+// direction (z of the mixed cycle is FP32, too coarse for the projection).
+void bmg_ctx::enqueueDeflate()
+{
+    Pair<double> Pv{{kp[0], kp[1]}};
+    launch_defl_dot<double>(dDeflEntCell, dDeflEntVal, dDeflEntBeg,
+                            dDeflEntEnd, deflEntChunks, cst(Pv), dDeflPart,
+                            stream);
+    launch_defl_correct<double>(dDeflCell, dDeflCellComp, deflNCells,
+                                dDeflEinv, dDeflPart, dDeflEntChunkOff, deflM,
+                                -1.0, Pv, stream);
+}
+
+// x += Z E^-1 Z^T r on the FCG state: the deflated start.
+void bmg_ctx::deflateStart()
+{
+    Pair<double> X{{kx[0], kx[1]}};
+    Pair<const double> Rc{{kr[0], kr[1]}};
+    launch_defl_dot<double>(dDeflCell, nullptr, dDeflCellBeg, dDeflCellEnd,
+                            deflCellChunks, Rc, dDeflPart, stream);
+    launch_defl_correct<double>(dDeflCell, dDeflCellComp, deflNCells, dDeflEinv,
+                                dDeflPart, dDeflCellChunkOff, deflM, 1.0, X,
+                                stream);
+}
+
 // Flexible conjugate gradients (Notay 2000, one stored direction) with one
 // MG cycle as the variable preconditioner; the stopping test is the
 // reference's (normFactors/checkConvergence, solver.C:13-95) on the updated
@@ -1007,6 +1324,7 @@ void bmg_ctx::runFCG(bool constMatrix, bmg_stats* st)
     Level& q0 = lv[0];
     const long n = q0.L.ncol;
     prepareFCG();
+    if (!learned) ensureDeflation();
     const OpView<double> o = op<double>(0);
     Pair<double> X{{kx[0], kx[1]}}, Bv{{kb[0], kb[1]}}, Rv{{kr[0], kr[1]}},
         Pv{{kp[0], kp[1]}};
@@ -1031,6 +1349,19 @@ void bmg_ctx::runFCG(bool constMatrix, bmg_stats* st)
     int conv = check(cur);
     int iter = 0;
     bool restart = true;
+    // deflated start x + Z E^-1 Z^T r (the stopping test keeps init)
+    if (!learned && deflating() && conv == 0)
+    {
+        deflateStart();
+        if (anyPeriodic())
+            launch_bc_fill(q0.L, o, X, 0, 1, &bcval, stream);
+        launch_residual(q0.L, o, cst(X), cst(Bv), Rv, partials, &nb, stream);
+        ss = reduceFinal(nb);
+        cur = std::sqrt(ss)/nf;
+        if (!std::isfinite(cur))
+            throw Error(BMG_ERR_NUMERIC, "residual norm is not finite");
+        conv = check(cur);
+    }
     // learned mode: the network's initial guess, kept only if it lowers the
     // residual (x is saved in q meanwhile)
     bool learnedActive = learned;
@@ -1082,6 +1413,7 @@ void bmg_ctx::runFCG(bool constMatrix, bmg_stats* st)
                     stageRestart(mixed);
                     precondition(mixed);
                     finishRestart(mixed);
+                    if (deflating()) enqueueDeflate();
                 }
                 else
                     enqueueFCGRestart(mixed);
@@ -1218,6 +1550,7 @@ void bmg_ctx::enqueueFCGRestart(bool mixed)
     enqueuePrecondition(mixed);
     record_tag(3000);
     finishRestart(mixed);
+    if (deflating()) enqueueDeflate();
 }
 
 // The true residual b - A x with the boundary update, slot 0.
@@ -1759,6 +2092,13 @@ static void setStencil(bmg_ctx* c, int level, const double* A7, bool device,
     q.opkind = OP_STENCIL;
     q.haveOp = true;
     c->gen++;
+    if (level == 0)
+    {
+        // a stored stencil carries no coefficient field to deflate by
+        c->opGen++;
+        c->deflLabel.clear();
+        c->deflM = 0;
+    }
     c->symmetric = symmetric != 0;
     c->eliminated = false;
     c->foldPending = true;
@@ -1821,6 +2161,15 @@ static void setLaplacian(bmg_ctx* c, const double* lx, const double* ly,
                         }
             c->coefScale = std::pow(10.0, logSum/double(c->coefChannel.size()));
             c->coefUploaded = false;
+        }
+        // deflation sets of the new operator (flexible CG)
+        c->opGen++;
+        if (withLambda && c->opts.deflation)
+            c->deflationLabels(lx, ly, lz);
+        else
+        {
+            c->deflLabel.clear();
+            c->deflM = 0;
         }
         std::vector<double*> lam(3*c->nLevels(), nullptr);
         if (withLambda)
@@ -2111,6 +2460,14 @@ int bmg_solve_batch(bmg_batch* b, double* const* db, double* const* dx,
 int bmg_batch_info(const bmg_batch* b, int* launches, long* workgroups)
 {
     return run([&] { batch_info(b, launches, workgroups); });
+}
+
+int bmg_deflation_vectors(const bmg_ctx* c, int* count)
+{
+    return run([&] {
+        if (!c || !count) throw Error(BMG_ERR_ARG, "null pointer");
+        *count = c->opts.deflation ? c->deflM : 0;
+    });
 }
 
 int bmg_batch_tuning(const bmg_batch* b, int* launches, int* reshaped)

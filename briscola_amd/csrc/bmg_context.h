@@ -212,6 +212,30 @@ struct bmg_ctx
     std::vector<float> coefChannel;
     double coefScale = 1.0;   // 10^mean(log10 lambda) of the cells
     bool coefUploaded = false;
+    // flexible-CG deflation (opts.deflation): face-connected sets of
+    // high-coefficient level-0 cells found when the operator is assembled
+    // (host labels per interior cell, -1: none), and their device data,
+    // built for operator generation deflOpGen
+    std::vector<int> deflLabel;
+    int deflM = 0;
+    long opGen = 0, deflOpGen = -1;
+    unsigned* dDeflCell = nullptr;   // component cells, by component
+    int* dDeflCellComp = nullptr;
+    long deflNCells = 0;
+    unsigned* dDeflEntCell = nullptr;   // entries (A z_k)_i, by component
+    double* dDeflEntVal = nullptr;
+    // both lists cut into chunks that do not straddle components: chunk
+    // ranges, and the chunks of component k at [off[k], off[k+1])
+    int* dDeflCellBeg = nullptr;
+    int* dDeflCellEnd = nullptr;
+    int* dDeflCellChunkOff = nullptr;
+    int deflCellChunks = 0;
+    int* dDeflEntBeg = nullptr;
+    int* dDeflEntEnd = nullptr;
+    int* dDeflEntChunkOff = nullptr;
+    int deflEntChunks = 0;
+    double* dDeflPart = nullptr;     // chunk partial sums
+    double* dDeflEinv = nullptr;     // (Z^T A Z)^-1, m x m
     // one MG iteration captured as a HIP graph and replayed
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
@@ -269,6 +293,7 @@ struct bmg_ctx
         devFree(uIn);
         devFree(uCoef);
         devFree(uOut);
+        freeDeflation();
         for (int c = 0; c < 2; c++)
         {
             devFree(kx[c]); devFree(kb[c]); devFree(kr[c]);
@@ -335,6 +360,22 @@ struct bmg_ctx
     void smoothFused(int l, int sweeps);
     // buffers the smoothers use, allocated before any graph capture
     void ensureSmootherBuffers(bool mixed);
+    // deflation of flexible CG
+    void deflationLabels(const double* lx, const double* ly,
+                         const double* lz);
+    void ensureDeflation();
+    bool deflating() const;
+    void enqueueDeflate();
+    void deflateStart();
+    void freeDeflation()
+    {
+        devFree(dDeflCell); devFree(dDeflCellComp);
+        devFree(dDeflEntCell); devFree(dDeflEntVal);
+        devFree(dDeflCellBeg); devFree(dDeflCellEnd); devFree(dDeflCellChunkOff);
+        devFree(dDeflEntBeg); devFree(dDeflEntEnd); devFree(dDeflEntChunkOff);
+        devFree(dDeflPart); devFree(dDeflEinv);
+        deflOpGen = -1;
+    }
     template <typename T>
     void residual(int l, bool corrBcs, bool norm, int* nblocks);
     template <typename T>

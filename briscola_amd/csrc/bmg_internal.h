@@ -203,6 +203,25 @@ template <typename T>
 void launch_bc_fill_snap(const LevelDims& L, const OpView<T>& op, Pair<T> x,
                          int mode, int level0, const BcData* bcval,
                          T* const (*snap)[2], hipStream_t s);
+// Deflation by indicator vectors of cell sets (flexible CG): the entries
+// (A z_k)_i of the level-0 operator (count only when cell is null), chunk
+// partial sums part[c] = sum val * f(cell) over [beg[c], end[c]) of a list
+// sorted by component (val null: plain sums of f), and the correction
+// f(cell) += sign * (Einv w)[comp(cell)], w[k] the sum of the partials of
+// chunks coff[k]..coff[k+1] (at most 64 components).
+unsigned launch_defl_entries(const LevelDims& L, const OpView<double>& op,
+                             const int* label, unsigned* count, unsigned cap,
+                             unsigned* cell, int* comp, double* val,
+                             hipStream_t s);
+template <typename T>
+void launch_defl_dot(const unsigned* cell, const double* val, const int* beg,
+                     const int* end, int nchunks, Pair<const T> f,
+                     double* part, hipStream_t s);
+template <typename T>
+void launch_defl_correct(const unsigned* cell, const int* comp, long ncells,
+                         const double* einv, const double* part,
+                         const int* coff, int m, double sign, Pair<T> f,
+                         hipStream_t s);
 template <typename T>
 void launch_fill(Pair<T> x, long ncol, T v, hipStream_t s);
 template <typename T>

@@ -322,6 +322,41 @@ __device__ __forceinline__ void pair_Ax(const LevelDims& L,
     }
 }
 
+// The Gauss-Seidel update x = (b - P)/A0, P = A1 x_-i + A2 x_+i + A3 x_-j
+// + A4 x_+j + A5 x_-k + A6 x_+k (rbgsSmoother.C:65-183).  In FP64 every
+// product and sum is rounded in that order and the division is correctly
+// rounded: the reference's arithmetic, bit for bit.  The FP32 cycles of
+// mixed precision carry no bitwise contract (their result only corrects an
+// FP64 iterate): the row product is contracted into fused multiply-adds
+// and the division is a multiplication by the hardware reciprocal.
+template <typename T>
+__device__ __forceinline__ T gs_update(const T a[7], const T nb[6], T bv)
+{
+    const T P =
+        a[1]*nb[0]
+      + a[2]*nb[1]
+      + a[3]*nb[2]
+      + a[4]*nb[3]
+      + a[5]*nb[4]
+      + a[6]*nb[5];
+    return (bv - P)/a[0];
+}
+
+template <>
+__device__ __forceinline__ float gs_update<float>(const float a[7],
+                                                  const float nb[6], float bv)
+{
+#pragma clang fp contract(fast)
+    const float P =
+        a[1]*nb[0]
+      + a[2]*nb[1]
+      + a[3]*nb[2]
+      + a[4]*nb[3]
+      + a[5]*nb[4]
+      + a[6]*nb[5];
+    return (bv - P)*__builtin_amdgcn_rcpf(a[0]);
+}
+
 // One cell of a red-black phase: row J of plane I, the off-th cell of the
 // colour in the row.
 template <typename T, int OP>
@@ -348,14 +383,7 @@ __device__ __forceinline__ void rbgs_cell(const LevelDims& L,
     const T bv = ld(bcl, idx);
     T a[7];
     coeffs<T, OP>(L, op, colour, int(idx), row, p, q, I, J, K, a);
-    const T P =
-        a[1]*nb[0]
-      + a[2]*nb[1]
-      + a[3]*nb[2]
-      + a[4]*nb[3]
-      + a[5]*nb[4]
-      + a[6]*nb[5];
-    st(xc, idx, (bv - P)/a[0]);
+    st(xc, idx, gs_update<T>(a, nb, bv));
 }
 
 // ---------------------------------------------------------------------------
@@ -480,14 +508,7 @@ __device__ __forceinline__ T fused_update(const LevelDims& L,
 {
     T a[7];
     coeffs<T, OP>(L, op, colour, int(idx), row, p, q, I, J, K, a);
-    const T P =
-        a[1]*nb[0]
-      + a[2]*nb[1]
-      + a[3]*nb[2]
-      + a[4]*nb[3]
-      + a[5]*nb[4]
-      + a[6]*nb[5];
-    return (bv - P)/a[0];
+    return gs_update<T>(a, nb, bv);
 }
 
 __device__ __forceinline__ bool in_region(const Region& rg, int I, int J,
@@ -763,14 +784,7 @@ __device__ __forceinline__ void rbgs_slabs_body(const Bid bid,
                 nb[4] = om[k];
                 nb[5] = opl[k];
             }
-            const T P =
-                a[k][1]*nb[0]
-              + a[k][2]*nb[1]
-              + a[k][3]*nb[2]
-              + a[k][4]*nb[3]
-              + a[k][5]*nb[4]
-              + a[k][6]*nb[5];
-            xs[u1][u2] = (bv[k] - P)/a[k][0];
+            xs[u1][u2] = gs_update<T>(a[k], nb, bv[k]);
         }
         __syncthreads();
     }
@@ -873,14 +887,7 @@ __global__ __launch_bounds__(256) void k_lexgs_plane(LevelDims L,
     const T bv = ld(col(b, colour), idx);
     T a[7];
     coeffs<T, OP>(L, op, colour, int(idx), row, p, q, I, J, K, a);
-    const T P =
-        a[1]*nb[0]
-      + a[2]*nb[1]
-      + a[3]*nb[2]
-      + a[4]*nb[3]
-      + a[5]*nb[4]
-      + a[6]*nb[5];
-    st(col(x, colour), idx, (bv - P)/a[0]);
+    st(col(x, colour), idx, gs_update<T>(a, nb, bv));
 }
 
 // Damped Jacobi (jacSmoother.C:18-127, omega 0.8): y from x, then copy back.
@@ -3690,6 +3697,153 @@ struct KSlabsAny
     }
 };
 
+// ---------------------------------------------------------------------------
+// Deflation of flexible CG by indicator vectors z_k of cell sets (components
+// 0..m-1; label -1: none).  A z_k is non-zero only where a cell or one of
+// its six neighbours is in component k; for every interior cell i and every
+// component k among its own and its neighbours' labels, the entry
+// (A z_k)_i = sum over the stencil positions s whose cell is in k of
+// a_s(i), the solver's own eliminated coefficients in the C,L,R,B,T,A,F
+// order.  Labels are given per interior cell, (i,j,k) order, k fastest;
+// a neighbour across a periodic face is the wrapped cell, across any other
+// face none.  count: entries found (atomic); with write, entry e goes to
+// (cell[e], comp[e], val[e]) at the position its atomic add returned (the
+// host sorts them).
+template <int OP>
+__global__ __launch_bounds__(256) void k_defl_entries(LevelDims L,
+                                                      OpView<double> op,
+                                                      const int* label,
+                                                      unsigned* count,
+                                                      int write, unsigned cap,
+                                                      unsigned* cell,
+                                                      int* comp, double* val)
+{
+    const long n = long(L.N[0])*L.N[1]*L.N[2];
+    const long t = long(blockIdx.x)*256 + threadIdx.x;
+    if (t >= n) return;
+    const int i = int(t/(long(L.N[1])*L.N[2]));
+    const int j = int((t/L.N[2]) % L.N[1]);
+    const int k = int(t % L.N[2]);
+    const int g[3] = {i, j, k};
+    const int off[7][3] = {{0,0,0},{-1,0,0},{1,0,0},{0,-1,0},{0,1,0},
+                           {0,0,-1},{0,0,1}};
+    int lab[7];
+#pragma unroll
+    for (int s = 0; s < 7; s++)
+    {
+        int q[3];
+        bool out = false;
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+        {
+            q[d] = g[d] + off[s][d];
+            if (q[d] < 0 || q[d] >= L.N[d])
+            {
+                if (L.bc[2*d + (q[d] >= L.N[d])] == PER)
+                    q[d] = (q[d] + L.N[d]) % L.N[d];
+                else
+                    out = true;
+            }
+        }
+        lab[s] = out ? -1 : label[(long(q[0])*L.N[1] + q[1])*L.N[2] + q[2]];
+    }
+    bool any = false;
+#pragma unroll
+    for (int s = 0; s < 7; s++) any = any || lab[s] >= 0;
+    if (!any) return;
+    const int I = i + 1, J = j + 1, K = k + 1;
+    const int colour = (I + J + K) & 1;
+    const int p = K >> 1, q = K & 1;
+    const int row = (I*L.P[1] + J)*L.Hz;
+    const int idx = row + p;
+    double a[7];
+    coeffs<double, OP>(L, op, colour, idx, row, p, q, I, J, K, a);
+    for (int s = 0; s < 7; s++)
+    {
+        const int c = lab[s];
+        if (c < 0) continue;
+        bool first = true;
+        for (int u = 0; u < s; u++) first = first && lab[u] != c;
+        if (!first) continue;
+        double v = 0.0;
+        for (int u = 0; u < 7; u++)
+            if (lab[u] == c) v = v + a[u];
+        if (v == 0.0) continue;
+        const unsigned e = atomicAdd(count, 1u);
+        if (write && e < cap)
+        {
+            cell[e] = (unsigned(colour) << 31) | unsigned(idx);
+            comp[e] = c;
+            val[e] = v;
+        }
+    }
+}
+
+// Element of a colour-split field addressed by a packed cell code (colour
+// in bit 31, position below).
+template <typename T>
+__device__ __forceinline__ T& at_code(Pair<T> f, unsigned code)
+{
+    return col(f, int(code >> 31))[code & 0x7fffffffu];
+}
+
+// Chunk c of a sorted list: part[c] = sum over [beg[c], end[c]) of
+// val * f(cell) (val null: the plain sum of f), in a fixed order; the
+// chunks of a component are consecutive and never straddle two.
+template <typename T>
+__global__ __launch_bounds__(256) void k_defl_dot(const unsigned* cell,
+                                                  const double* val,
+                                                  const int* beg,
+                                                  const int* end,
+                                                  Pair<const T> f,
+                                                  double* part)
+{
+    __shared__ double sm[4];
+    const int c = blockIdx.x;
+    double acc = 0.0;
+    for (int e = beg[c] + int(threadIdx.x); e < end[c]; e += 256)
+    {
+        const double v = double(at_code(f, cell[e]));
+        acc += val ? val[e]*v : v;
+    }
+    const double s = block_sum(acc, sm);
+    if (threadIdx.x == 0) part[c] = s;
+}
+
+// f(cell) += sign * (Einv w)[comp(cell)] for every cell of the components,
+// w[k] the sum of the chunk partials coff[k]..coff[k+1] (each workgroup
+// sums them and forms Einv w itself; m <= 64).
+template <typename T>
+__global__ __launch_bounds__(256) void k_defl_correct(const unsigned* cell,
+                                                      const int* comp,
+                                                      long ncells,
+                                                      const double* einv,
+                                                      const double* part,
+                                                      const int* coff, int m,
+                                                      double sign, Pair<T> f)
+{
+    __shared__ double ws[64];
+    __shared__ double cs[64];
+    for (int k = int(threadIdx.x); k < m; k += 256)
+    {
+        double v = 0.0;
+        for (int c = coff[k]; c < coff[k + 1]; c++) v += part[c];
+        ws[k] = v;
+    }
+    __syncthreads();
+    for (int k = int(threadIdx.x); k < m; k += 256)
+    {
+        double v = 0.0;
+        for (int j = 0; j < m; j++) v += einv[k*m + j]*ws[j];
+        cs[k] = sign*v;
+    }
+    __syncthreads();
+    const long t = long(blockIdx.x)*256 + threadIdx.x;
+    if (t >= ncells) return;
+    T& x = at_code(f, cell[t]);
+    x = T(double(x) + cs[comp[t]]);
+}
+
 template <typename T>
 struct KNfTermsAny
 {
@@ -4009,11 +4163,13 @@ void launch_rbgs2(const LevelDims& L, const OpView<T>& op, T* xR,
                                 rg, ft);
 }
 
+// Arguments of one slab launch (one face pair, or one plane when nslab =
+// 1) and its grid: one workgroup per plane when the plane with its ghost
+// ring fits an LDS tile (sa.tt = 0), else tiles of 16 x 16.
 template <typename T>
-void launch_rbgs_slabs(const LevelDims& L, const OpView<T>& op, Pair<T> x,
-                       Pair<const T> b, int d, const int* planes, int nslab,
-                       const int* colours, int nphase, const SlabIO<T>& io,
-                       hipStream_t s)
+SlabArgs<T> slab_args(const LevelDims& L, int d, const int* planes, int nslab,
+                      const int* colours, int nphase, const SlabIO<T>& io,
+                      dim3* grid)
 {
     SlabArgs<T> sa;
     sa.plane[0] = planes[0];
@@ -4043,6 +4199,19 @@ void launch_rbgs_slabs(const LevelDims& L, const OpView<T>& op, Pair<T> x,
         sa.tt = 16;
         g = dim3((n2 + sa.tt - 1)/sa.tt, (n1 + sa.tt - 1)/sa.tt, nslab);
     }
+    *grid = g;
+    return sa;
+}
+
+template <typename T>
+void launch_rbgs_slabs(const LevelDims& L, const OpView<T>& op, Pair<T> x,
+                       Pair<const T> b, int d, const int* planes, int nslab,
+                       const int* colours, int nphase, const SlabIO<T>& io,
+                       hipStream_t s)
+{
+    dim3 g;
+    const SlabArgs<T> sa = slab_args<T>(L, d, planes, nslab, colours, nphase,
+                                        io, &g);
 #define BMG_SLAB(OPK, DD)                                                     \
     dispatch<KSlabs<T, OPK, DD>>(g, 0, s, {L, op, x, b, sa})
 #define BMG_SLAB_D(OPK)                                                       \
@@ -4178,10 +4347,13 @@ void launch_prolong(const LevelDims& Lf, const LevelDims& Lc,
     dispatch<KProlong<T>>(tiles_grid(tl), 0, s, {Lf, Lc, tv, xc, xf, tl});
 }
 
+// Arguments of the ghost fill (groups of threads per face pair, edges and
+// corners, slab-plane snapshots); total threads in *threads.
 template <typename T>
-void launch_bc_fill_snap(const LevelDims& L, const OpView<T>& op, Pair<T> x,
-                         int mode, int level0, const BcData* bcval,
-                         T* const (*snap)[2], hipStream_t s)
+typename KBcFill<T>::Args bc_fill_args(const LevelDims& L, const OpView<T>& op,
+                                       Pair<T> x, int mode, int level0,
+                                       const BcData* bcval,
+                                       T* const (*snap)[2], int* threads)
 {
     // groups holding ghosts this mode writes: mode 0 the periodic faces,
     // mode 2 the other faces, mode 1 every ghost; then the snapshots
@@ -4213,9 +4385,20 @@ void launch_bc_fill_snap(const LevelDims& L, const OpView<T>& op, Pair<T> x,
         total += (n + 255)/256*256;
     }
     gr.start[7] = total;
+    *threads = total;
+    return {L, op, x, mode, level0, bcvals(bcval), gr, sn};
+}
+
+template <typename T>
+void launch_bc_fill_snap(const LevelDims& L, const OpView<T>& op, Pair<T> x,
+                         int mode, int level0, const BcData* bcval,
+                         T* const (*snap)[2], hipStream_t s)
+{
+    int total = 0;
+    const typename KBcFill<T>::Args a =
+        bc_fill_args<T>(L, op, x, mode, level0, bcval, snap, &total);
     if (total == 0) return;
-    dispatch<KBcFill<T>>(dim3(total/256), 0, s,
-                         {L, op, x, mode, level0, bcvals(bcval), gr, sn});
+    dispatch<KBcFill<T>>(dim3(total/256), 0, s, a);
 }
 
 template <typename T>
@@ -4589,6 +4772,57 @@ void launch_to_float(long ncol, Pair<const double> in, Pair<float> out,
     dispatch<KToFloat>(dim3((ncol + 255)/256), 0, s, {ncol, in, out});
 }
 
+unsigned launch_defl_entries(const LevelDims& L, const OpView<double>& op,
+                             const int* label, unsigned* count, unsigned cap,
+                             unsigned* cell, int* comp, double* val,
+                             hipStream_t s)
+{
+    direct_only("deflation set-up");
+    const long n = long(L.N[0])*L.N[1]*L.N[2];
+    const dim3 g(unsigned((n + 255)/256));
+    const int write = cell != nullptr;
+    switch (op.kind)
+    {
+    case OP_STENCIL:
+        hipLaunchKernelGGL((k_defl_entries<OP_STENCIL>), g, dim3(256), 0, s, L,
+                           op, label, count, write, cap, cell, comp, val);
+        break;
+    case OP_CONST:
+        hipLaunchKernelGGL((k_defl_entries<OP_CONST>), g, dim3(256), 0, s, L,
+                           op, label, count, write, cap, cell, comp, val);
+        break;
+    default:
+        hipLaunchKernelGGL((k_defl_entries<OP_FACES>), g, dim3(256), 0, s, L,
+                           op, label, count, write, cap, cell, comp, val);
+        break;
+    }
+    return 0;
+}
+
+template <typename T>
+void launch_defl_dot(const unsigned* cell, const double* val, const int* beg,
+                     const int* end, int nchunks, Pair<const T> f,
+                     double* part, hipStream_t s)
+{
+    direct_only("deflation");
+    if (nchunks <= 0) return;
+    hipLaunchKernelGGL((k_defl_dot<T>), dim3(nchunks), dim3(256), 0, s, cell,
+                       val, beg, end, f, part);
+}
+
+template <typename T>
+void launch_defl_correct(const unsigned* cell, const int* comp, long ncells,
+                         const double* einv, const double* part,
+                         const int* coff, int m, double sign, Pair<T> f,
+                         hipStream_t s)
+{
+    direct_only("deflation");
+    if (ncells <= 0) return;
+    hipLaunchKernelGGL((k_defl_correct<T>), dim3(unsigned((ncells + 255)/256)),
+                       dim3(256), 0, s, cell, comp, ncells, einv, part, coff,
+                       m, sign, f);
+}
+
 template <typename T>
 void launch_set_cell(Pair<T> f, long idx, int colour, T v, hipStream_t s)
 {
@@ -4629,6 +4863,13 @@ void launch_set_cell(Pair<T> f, long idx, int colour, T v, hipStream_t s)
                                          Pair<T>, int, int, const BcData*,    \
                                          T* const (*)[2], hipStream_t);       \
     template void launch_fill<T>(Pair<T>, long, T, hipStream_t);              \
+    template void launch_defl_dot<T>(const unsigned*, const double*,          \
+                                     const int*, const int*, int,             \
+                                     Pair<const T>, double*, hipStream_t);    \
+    template void launch_defl_correct<T>(const unsigned*, const int*, long,   \
+                                         const double*, const double*,        \
+                                         const int*, int, double, Pair<T>,    \
+                                         hipStream_t);                        \
     template void launch_ref_to_split<T>(const LevelDims&, const double*,     \
                                          Pair<T>, hipStream_t);               \
     template void launch_split_to_ref<T>(const LevelDims&, Pair<const T>,     \

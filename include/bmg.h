@@ -97,8 +97,24 @@ typedef struct bmg_opts {
     int smoother;           /* smoother, default rbgs       */
     int precision;          /* BMG_FP64 (default) / BMG_MIXED */
     int krylov;             /* BMG_KRYLOV_NONE (default) / BMG_KRYLOV_FCG */
-    int reserved[4];
+    int deflation;          /* 0 (default) / 1: flexible CG deflated by the
+                               connected sets of high-coefficient cells
+                               (see below)                          */
+    int reserved[3];
 } bmg_opts;
+/* Deflation (opts.deflation = 1, BMG_KRYLOV_FCG only; no reference
+ * counterpart): when bmg_set_operator_laplacian assembles an operator whose
+ * cell coefficients (the mean of a cell's six face lambdas) span a ratio of
+ * at least 100, the cells above the geometric mean of the extremes are
+ * grouped into face-connected sets (across periodic faces too), and the
+ * largest sets of at least 8 cells, at most 64, become indicator vectors
+ * Z.  Flexible CG then starts from x0 + Z E^-1 Z^T r0 (E = Z^T A Z) and
+ * projects every new search direction p to p - Z E^-1 (A Z)^T p (in FP64,
+ * the same as projecting the preconditioned residual): the
+ * near-constant modes of high-coefficient inclusions (bubbles, droplets),
+ * which geometric coarsening cannot represent, are removed exactly.  The
+ * stopping test is unchanged.  Other operators or options solve as without
+ * deflation; bmg_deflation_vectors reports the number of vectors. */
 
 typedef struct bmg_stats {
     int iters;              /* nIter of printSolverStats (solver.C:97-112) */
@@ -113,6 +129,9 @@ typedef struct bmg_stats {
                                        guess (normalised; 0: no guess)     */
     double reserved[2];
 } bmg_stats;
+
+/* Number of deflation vectors of the current operator (0: none). */
+int bmg_deflation_vectors(const bmg_ctx* ctx, int* count);
 
 /* Fill opts with the reference defaults (rel_tol/tolerance set to 0). */
 void bmg_default_opts(bmg_opts* opts);

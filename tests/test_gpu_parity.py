@@ -240,7 +240,10 @@ def test_lexgs_mixed_precision_converges(gpu, krylov):
     ctx.set_laplacian(prob.lam)
     x, st = ctx.solve(prob.b.copy())
     assert st.final_residual <= 1e-9 * st.initial_residual
-    err = np.abs(inner(x - xr)).max() / np.abs(inner(xr)).max()
+    # all-Neumann: solutions are defined up to a constant, so compare them
+    # with their means removed
+    d = inner(x) - inner(x).mean() - (inner(xr) - inner(xr).mean())
+    err = np.abs(d).max() / np.abs(inner(xr) - inner(xr).mean()).max()
     assert err < 1e-6, err
     ctx.close()
     ref.close()
