@@ -72,6 +72,7 @@ OpView<double> bmg_ctx::op<double>(int l) const
     {
         o.a7[c] = q.a7[c]; o.wf[c] = q.wf[c]; o.dg[c] = q.dg[c];
     }
+    o.fs = 1.0;
     for (int d = 0; d < 3; d++) { o.w[d] = q.dw[d]; o.dl[d] = q.ddl[d]; }
     return o;
 }
@@ -86,6 +87,7 @@ OpView<float> bmg_ctx::op<float>(int l) const
     {
         o.a7[c] = q.a7f[c]; o.wf[c] = q.wff[c]; o.dg[c] = q.dgf[c];
     }
+    o.fs = q.wfs;
     for (int d = 0; d < 3; d++) { o.w[d] = q.fw[d]; o.dl[d] = q.fdl[d]; }
     return o;
 }
@@ -697,10 +699,16 @@ void bmg_ctx::ensureMixed()
         }
         else if (q.opkind == OP_FACES)
         {
+            // FP16 faces scaled so that the largest is 2^14 (the FP16
+            // normal range then spans nine decades below it)
             for (int c = 0; c < 2; c++)
-                if (!q.wff[c]) q.wff[c] = devAlloc<float>(n*3);
+                if (!q.wff[c]) q.wff[c] = devAlloc<_Float16>(n*3);
             Pair<const double> in{{q.wf[0], q.wf[1]}};
-            launch_to_float(n*3, in, pr(q.wff), stream);
+            const double mx = device_absmax(n*3, in, partials, npartials,
+                                            stream);
+            const double fs = mx > 0.0 ? mx/16384.0 : 1.0;
+            q.wfs = float(fs);
+            launch_to_half(n*3, in, 1.0/double(q.wfs), pr(q.wff), stream);
         }
         if (q.opkind != OP_STENCIL && q.dg[0])
         {
@@ -2917,6 +2925,8 @@ int bmg_op_sum_sq_residual(bmg_ctx* c, double* out)
     });
 }
 
+static double phaseBytesPerUpdate(const Level& q, size_t w);
+
 int bmg_time_smoother(bmg_ctx* c, int level, int precision, int reps,
                       double* avg_ms, double* bytes)
 {
@@ -2947,12 +2957,12 @@ int bmg_time_smoother(bmg_ctx* c, int level, int precision, int reps,
         for (int d = 0; d < 3; d++) m[d] = std::max(0, rg.E[d] - rg.S[d]);
         const double cells = 0.5*double(m[0]*m[1]*m[2]);
         const int kind = q.opkind;
-        double per = kind == OP_CONST ? 3.0 : kind == OP_FACES ? 9.0 : 10.0;
-        if (kind != OP_STENCIL && q.dg[0]) per += 1.0;   // time-term diagonal
+        (void)kind;
         if (avg_ms) *avg_ms = double(ms)/reps;
         if (bytes)
-            *bytes = cells*per*(precision == BMG_MIXED ? sizeof(float)
-                                                       : sizeof(double));
+            *bytes = cells*phaseBytesPerUpdate(q, precision == BMG_MIXED
+                                                      ? sizeof(float)
+                                                      : sizeof(double));
     });
 }
 
@@ -2967,19 +2977,26 @@ int bmg_time_smoother(bmg_ctx* c, int level, int precision, int reps,
 //   coarse solve         the LU factors (8 n_c^2) + b in and x out
 // plus, for mixed precision, the FP64 -> FP32 residual copy (12 n), the
 // zeroing of e (4 n) and x += e (20 n) on level 0.
+// Face coefficients are stored in FP16 for the FP32 cycles.
+static double faceBytes(size_t w) { return w == sizeof(float) ? 2.0 : double(w); }
+
 static double phaseBytesPerUpdate(const Level& q, size_t w)
 {
-    double per = q.opkind == OP_CONST ? 3.0 : q.opkind == OP_FACES ? 9.0 : 10.0;
-    if (q.opkind != OP_STENCIL && q.dg[0]) per += 1.0;
-    return per*double(w);
+    double per = 3.0*double(w);
+    if (q.opkind == OP_FACES) per += 6.0*faceBytes(w);
+    else if (q.opkind == OP_STENCIL) per += 7.0*double(w);
+    if (q.opkind != OP_STENCIL && q.dg[0]) per += double(w);
+    return per;
 }
 
 static double cellBytesFull(const Level& q, size_t w)
 {
     // residual-type pass over every cell: x, b, r + each stored coefficient
-    double per = q.opkind == OP_CONST ? 3.0 : q.opkind == OP_FACES ? 6.0 : 10.0;
-    if (q.opkind != OP_STENCIL && q.dg[0]) per += 1.0;
-    return per*double(w);
+    double per = 3.0*double(w);
+    if (q.opkind == OP_FACES) per += 3.0*faceBytes(w);
+    else if (q.opkind == OP_STENCIL) per += 7.0*double(w);
+    if (q.opkind != OP_STENCIL && q.dg[0]) per += double(w);
+    return per;
 }
 
 static double ghostCells(const Level& q)

@@ -82,7 +82,8 @@ struct Level
     double* a7[2] = {nullptr, nullptr};
     float* a7f[2] = {nullptr, nullptr};
     double* wf[2] = {nullptr, nullptr};
-    float* wff[2] = {nullptr, nullptr};
+    _Float16* wff[2] = {nullptr, nullptr};   // FP16 faces, times wfs
+    float wfs = 1.0f;
     double* dg[2] = {nullptr, nullptr};    // Euler time-term diagonal
     float* dgf[2] = {nullptr, nullptr};
     // metrics and transfer data

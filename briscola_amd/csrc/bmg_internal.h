@@ -35,11 +35,23 @@ struct LevelDims {
     int R[3];      // refinement ratio of this level vs its parent
 };
 
+// Storage of the stored face coefficients (OP_FACES): FP64 for the FP64
+// path; FP16 scaled by a per-level factor for the FP32 cycles of mixed
+// precision, whose smoothing only preconditions an FP64 iteration (half the
+// bytes of the phase's largest streams).
+template <typename T>
+struct FaceStore { using type = T; };
+template <>
+struct FaceStore<float> { using type = _Float16; };
+
 template <typename T>
 struct OpView {
     int kind;
     const T* a7[2];   // OP_STENCIL: 7*ncol per colour, component-major
-    const T* wf[2];   // OP_FACES: 3*ncol per colour, component-major
+    // OP_FACES: 3*ncol per colour, component-major; FP32 coefficients are
+    // the stored FP16 values times fs
+    const typename FaceStore<T>::type* wf[2];
+    T fs;
     const T* w[3];    // cell widths per padded index
     const T* dl[3];   // 1/|dcc| of the lower face of each padded index
     const T* dg[2];   // OP_CONST/OP_FACES: implicit Euler diagonal
@@ -342,6 +354,12 @@ void launch_to_double(long n, Pair<const float> a, Pair<double> b,
 // x64 += e32 (interior), r32 = (float) r64, etc.
 void launch_add_correction(const LevelDims& L, Pair<double> x,
                            Pair<const float> e, hipStream_t s);
+// Largest |value| of the first n elements of both arrays of a pair.
+double device_absmax(long n, Pair<const double> in, double* partials,
+                     size_t npartials, hipStream_t s);
+// out = (FP16) (in * scale), n elements per colour.
+void launch_to_half(long n, Pair<const double> in, double scale,
+                    Pair<_Float16> out, hipStream_t s);
 void launch_to_float(long ncol, Pair<const double> in, Pair<float> out,
                      hipStream_t s);
 template <typename T>

@@ -13,6 +13,7 @@
 #include <string>
 #include <type_traits>
 #include <utility>
+#include <vector>
 
 namespace bmg {
 
@@ -131,14 +132,20 @@ __device__ __forceinline__ void raw_coeffs(const LevelDims& L,
         else
         {
             const int n = nci(L);
-            const T* wc = op.wf[colour];
-            const T* wo = op.wf[colour ^ 1];
-            a[1] = ld(wc, idx);
-            a[2] = ld(wo, idx + L.sI);
-            a[3] = ld(wc, n + idx);
-            a[4] = ld(wo, n + idx + L.Hz);
-            a[5] = ld(wc, 2*n + idx);
-            a[6] = ld(wo, 2*n + row + p + q);
+            using F = typename FaceStore<T>::type;
+            const F* wc = op.wf[colour];
+            const F* wo = op.wf[colour ^ 1];
+            const F f[6] = {ld(wc, idx), ld(wo, idx + L.sI), ld(wc, n + idx),
+                            ld(wo, n + idx + L.Hz), ld(wc, 2*n + idx),
+                            ld(wo, 2*n + row + p + q)};
+#pragma unroll
+            for (int s = 0; s < 6; s++)
+            {
+                if constexpr (std::is_same<F, T>::value)
+                    a[s + 1] = f[s];
+                else
+                    a[s + 1] = T(f[s])*op.fs;
+            }
         }
         T c0 = T(0) - a[1];
         c0 = c0 - a[2];
@@ -3844,6 +3851,40 @@ __global__ __launch_bounds__(256) void k_defl_correct(const unsigned* cell,
     x = T(double(x) + cs[comp[t]]);
 }
 
+// Per-block maxima of |v| over both arrays of a pair (grid-stride).
+__global__ __launch_bounds__(256) void k_absmax(Pair<const double> in, long n,
+                                               double* partials)
+{
+    __shared__ double sm[256];
+    double m = 0.0;
+    for (long i = long(blockIdx.x)*256 + threadIdx.x; i < 2*n;
+         i += long(gridDim.x)*256)
+    {
+        const double v = i < n ? in.c[0][i] : in.c[1][i - n];
+        m = fmax(m, fabs(v));
+    }
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1)
+    {
+        if (int(threadIdx.x) < s)
+            sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = sm[0];
+}
+
+__global__ __launch_bounds__(256) void k_to_half(Pair<const double> in, long n,
+                                                double scale,
+                                                Pair<_Float16> out)
+{
+    const long i = long(blockIdx.x)*256 + threadIdx.x;
+    if (i >= 2*n) return;
+    const int c = i >= n;
+    const long j = i - c*n;
+    out.c[c][j] = _Float16(in.c[c][j]*scale);
+}
+
 template <typename T>
 struct KNfTermsAny
 {
@@ -4764,6 +4805,31 @@ void launch_add_correction(const LevelDims& L, Pair<double> x,
 {
     const Tiles tl = pair_tiles(L);
     dispatch<KAddCorrection>(tiles_grid(tl), 0, s, {L, x, e, tl});
+}
+
+double device_absmax(long n, Pair<const double> in, double* partials,
+                     size_t npartials, hipStream_t s)
+{
+    const long total = 2*n;
+    int nb = int(std::min<long>((total + 255)/256, long(npartials)));
+    if (nb < 1) nb = 1;
+    hipLaunchKernelGGL(k_absmax, dim3(nb), dim3(256), 0, s, in, n, partials);
+    std::vector<double> h(nb);
+    if (hipMemcpyAsync(h.data(), partials, nb*sizeof(double),
+                       hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        throw std::runtime_error("absmax: copy failed");
+    double m = 0.0;
+    for (double v : h) m = std::max(m, v);
+    return m;
+}
+
+void launch_to_half(long n, Pair<const double> in, double scale,
+                    Pair<_Float16> out, hipStream_t s)
+{
+    direct_only("half conversion");
+    hipLaunchKernelGGL(k_to_half, dim3(unsigned((2*n + 255)/256)), dim3(256), 0,
+                       s, in, n, scale, out);
 }
 
 void launch_to_float(long ncol, Pair<const double> in, Pair<float> out,
