@@ -126,7 +126,9 @@ inline void check(int code)
 }
 
 // Solver options from the MG dictionary keys and defaults (MG.C:257-304,
-// solver.C:114-123), plus this solver's "precision" (FP64 | mixed).
+// solver.C:114-123), plus this solver's "precision" (FP64 | mixed),
+// "krylov" (none | FCG: flexible CG preconditioned by one cycle) and
+// "deflation" (off | on: FCG deflated by the high-coefficient inclusions).
 inline bmg_opts optsFromDict(const dictionary& dict)
 {
     bmg_opts o;
@@ -157,6 +159,13 @@ inline bmg_opts optsFromDict(const dictionary& dict)
         throw FatalError("only the linear transfer schemes are provided");
     const std::string prec = dict.lookupOrDefault("precision", "FP64");
     o.precision = prec == "mixed" ? BMG_MIXED : BMG_FP64;
+    const std::string kry = dict.lookupOrDefault("krylov", "none");
+    if (kry == "FCG") o.krylov = BMG_KRYLOV_FCG;
+    else if (kry != "none") throw FatalError("Unknown krylov " + kry);
+    const std::string defl = dict.lookupOrDefault("deflation", "off");
+    if (defl != "on" && defl != "off")
+        throw FatalError("deflation must be on or off");
+    o.deflation = defl == "on" ? 1 : 0;
     return o;
 }
 
