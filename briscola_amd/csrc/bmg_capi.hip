@@ -257,7 +257,7 @@ void bmg_ctx::ensureSmootherBuffers(bool mixed)
 // values.  The boundary slabs of each periodic face pair run as one
 // temporally blocked launch (k_rbgs_slabs).
 template <typename T>
-void bmg_ctx::smooth(int l, int sweeps)
+void bmg_ctx::smooth(int l, int sweeps, bool firstDone)
 {
     Level& q = lv[l];
     const OpView<T> o = op<T>(l);
@@ -313,6 +313,12 @@ void bmg_ctx::smooth(int l, int sweeps)
         const int nc = colours(rg, cols);
         for (int i = 0; i < nc; i++)
         {
+            // the first phase already run from x = 0 (launch_zero_phase)
+            if (s == 0 && i == 0 && firstDone)
+            {
+                prevColour = cols[0];
+                continue;
+            }
             if (cols[i] != prevColour)
                 launch_rbgs(q.L, o, x, b, cols[i], rg, stream);
             prevColour = cols[i];
@@ -415,8 +421,14 @@ void bmg_ctx::coarseSolve()
     const int l = coarseLevel();
     if (!prepared || preparedCoarse != l)
         throw Error(BMG_ERR_STATE, "coarse level not factorised");
-    launch_coarse_solve(lv[l].L, op<T>(l), X<T>(l), cst(B<T>(l)), dLU, dPiv,
-                        nc, l == 0, &bcval, stream);
+    // the FP32 cycles solve by the inverse; FP64 keeps the reference's
+    // substitutions, bit for bit
+    if (std::is_same<T, float>::value && dCoarseInv)
+        launch_coarse_inv(lv[l].L, op<T>(l), X<T>(l), cst(B<T>(l)), dCoarseInv,
+                          nc, l == 0, &bcval, stream);
+    else
+        launch_coarse_solve(lv[l].L, op<T>(l), X<T>(l), cst(B<T>(l)), dLU,
+                            dPiv, nc, l == 0, &bcval, stream);
 }
 
 // MG::cycle (MG.C:19-129).  On the coarsest level in direct mode the
@@ -440,11 +452,29 @@ void bmg_ctx::cycle(std::vector<int>& visits, int l)
     if (singular() && l == lc)
         launch_set_cell<T>(B<T>(l), ((long)1*q.L.P[1] + 1)*q.L.Hz + 0,
                            (1 + 1 + 1) & 1, T(0), stream);
-    if (l > 0)
-        launch_fill<T>(X<T>(l), q.L.ncol, T(0), stream);
     const bool direct = opts.coarse_mode == BMG_COARSE_DIRECT;
-    if (opts.n_sweeps_pre > 0 && !(l == lc && direct))
-        smooth<T>(l, opts.n_sweeps_pre);
+    const bool pre = opts.n_sweeps_pre > 0 && !(l == lc && direct);
+    bool firstDone = false;
+    if (l > 0)
+    {
+        // x = 0 and, with red-black smoothing, the first phase from it in
+        // one launch
+        if (pre && opts.smoother == BMG_RBGS && !fusedLevel<T>(l))
+        {
+            const Region rg = interior(l, 1);
+            int cols[4];
+            if (phase_colours(rg, cols) > 0)
+            {
+                launch_zero_phase<T>(q.L, op<T>(l), X<T>(l), cst(B<T>(l)),
+                                     cols[0], rg, stream);
+                firstDone = true;
+            }
+        }
+        if (!firstDone)
+            launch_fill<T>(X<T>(l), q.L.ncol, T(0), stream);
+    }
+    if (pre)
+        smooth<T>(l, opts.n_sweeps_pre, firstDone);
     if (l < lc)
     {
         int reps = 1;
@@ -575,6 +605,40 @@ void bmg_ctx::prepareCoarse()
             const double rr = 1.0/D[size_t(j)*n + j];
             for (int i = j + 1; i < n; i++) D[size_t(i)*n + j] *= rr;
         }
+    }
+    // the inverse for the FP32 cycles (at most 64 cells): the factors'
+    // substitutions (LUBacksubstitute's order) on every unit vector
+    devFree(dCoarseInv);
+    if (n <= 64)
+    {
+        std::vector<double> G(64*64, 0.0), e(n);
+        for (int k = 0; k < n; k++)
+        {
+            std::fill(e.begin(), e.end(), 0.0);
+            e[k] = 1.0;
+            int ii = 0;
+            for (int i = 0; i < n; i++)
+            {
+                const int ip = piv[i];
+                double sum = e[ip];
+                e[ip] = e[i];
+                if (ii)
+                    for (int j = ii - 1; j < i; j++) sum -= D[size_t(i)*n + j]*e[j];
+                else if (sum != 0.0)
+                    ii = i + 1;
+                e[i] = sum;
+            }
+            for (int i = n - 1; i >= 0; i--)
+            {
+                double sum = e[i];
+                for (int j = i + 1; j < n; j++) sum -= D[size_t(i)*n + j]*e[j];
+                e[i] = sum/D[size_t(i)*n + i];
+            }
+            // column k of the inverse; lane i reads entry (i, j) at j*64 + i
+            for (int i = 0; i < n; i++) G[size_t(k)*64 + i] = e[i];
+        }
+        dCoarseInv = devUpload(G);
+        gen++;
     }
     // a coarse level that fits one wavefront is solved from padded
     // column-major factors and the final row permutation (the pivot swaps

@@ -177,6 +177,9 @@ struct bmg_ctx
     int preparedCoarse = -1;
     // coarse LU
     double* dLU = nullptr;
+    // inverse of the coarse matrix (64 x 64 column-major, zero-padded) for
+    // the FP32 cycles, when the coarsest level has at most 64 cells
+    double* dCoarseInv = nullptr;
     int* dPiv = nullptr;
     int nc = 0;
     // reductions
@@ -280,6 +283,7 @@ struct bmg_ctx
         for (int f = 0; f < 6; f++) { devFree(bcdev[f]); devFree(gsave[f]); }
         devFree(dLU);
         devFree(dPiv);
+        devFree(dCoarseInv);
         devFree(partials);
         devFree(dScalar);
         if (hScalar) (void)hipHostFree(hScalar);
@@ -354,7 +358,7 @@ struct bmg_ctx
     }
 
     template <typename T>
-    void smooth(int l, int sweeps);
+    void smooth(int l, int sweeps, bool firstDone = false);
     template <typename T>
     bool fusedLevel(int l) const;
     template <typename T>

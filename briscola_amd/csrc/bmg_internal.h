@@ -234,6 +234,18 @@ void launch_defl_correct(const unsigned* cell, const int* comp, long ncells,
                          const double* einv, const double* part,
                          const int* coff, int m, double sign, Pair<T> f,
                          hipStream_t s);
+// Coarsest-level solve by a precomputed inverse (column-major, 64 x 64,
+// at most 64 cells), the ghost update after it.
+template <typename T>
+void launch_coarse_inv(const LevelDims& L, const OpView<T>& op, Pair<T> x,
+                       Pair<const T> b, const double* inv, int n, int level0,
+                       const BcData* bcval, hipStream_t s);
+// x = 0 everywhere, then the red-black phase of `colour` over rg from it,
+// in one pass (the same bits as launch_fill followed by launch_rbgs).
+template <typename T>
+void launch_zero_phase(const LevelDims& L, const OpView<T>& op, Pair<T> x,
+                       Pair<const T> b, int colour, const Region& rg,
+                       hipStream_t s);
 template <typename T>
 void launch_fill(Pair<T> x, long ncol, T v, hipStream_t s);
 template <typename T>

@@ -2639,6 +2639,105 @@ __device__ __forceinline__ void coarse_solve_wave_body(const Bid bid,
     }
 }
 
+// The coarsest-level solve of the FP32 cycles of mixed precision (no
+// bitwise contract), for at most 64 cells: x = A^-1 (b - out-of-system
+// couplings) with the inverse precomputed in FP64 from the LU factors
+// (column-major, padded to 64 with zeros), one wavefront, lane i forming row
+// i from broadcasts of the right-hand side -- 64 independent multiply-adds
+// instead of the substitutions' chain of dependent ones.  Right-hand side
+// and ghost update as in coarse_solve_wave_body.
+template <typename T>
+__device__ __forceinline__ void coarse_inv_body(const Bid, const LevelDims& L,
+                                                const OpView<T>& op, Pair<T> x,
+                                                Pair<const T> b,
+                                                const double* inv, int n,
+                                                int level0, const BcVals& bv)
+{
+    const int lane = threadIdx.x;
+    const int N1 = L.N[1], N2 = L.N[2];
+    const int off[7][3] = {{0,0,0},{-1,0,0},{1,0,0},{0,-1,0},{0,1,0},
+                           {0,0,-1},{0,0,1}};
+    double v = 0.0;
+    if (lane < n)
+    {
+        const int q = lane;
+        const int I = 1 + q/(N1*N2), J = 1 + (q/N2) % N1, K = 1 + q % N2;
+        const int colour = (I + J + K) & 1;
+        const int idx = cidx(L, I, J, K);
+        v = double(col(b, colour)[idx]);
+        for (int s = 1; s < 7; s++)
+        {
+            const int g[3] = {I + off[s][0], J + off[s][1], K + off[s][2]};
+            bool out = false;
+            for (int d = 0; d < 3; d++)
+                if ((g[d] < 1 || g[d] > L.N[d]) &&
+                    L.bc[2*d + (g[d] > L.N[d])] != PER)
+                    out = true;
+            if (out)
+            {
+                const double a = double(op.a7[colour][s*nci(L) + idx]);
+                v = v - a*double(col(x, colour ^ 1)[cidx(L, g[0], g[1], g[2])]);
+            }
+        }
+    }
+    double y = 0.0;
+    for (int j = 0; j < n; j++)
+        y += inv[j*64 + lane]*__shfl(v, j, 64);
+    if (lane < n)
+    {
+        const int q = lane;
+        const int I = 1 + q/(N1*N2), J = 1 + (q/N2) % N1, K = 1 + q % N2;
+        col(x, (I + J + K) & 1)[cidx(L, I, J, K)] = T(y);
+    }
+    __syncthreads();
+    const Shell sh = shell_all(L);
+    const int ns = sh.nA + sh.nB + sh.nC;
+    for (int t = lane; t < ns; t += 64)
+    {
+        int I, J, K;
+        if (shell_cell(L, sh, t, I, J, K))
+            ghost_cell(L, op, x, 1, level0, bv, I, J, K);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void k_coarse_inv(LevelDims L, OpView<T> op,
+                                                   Pair<T> x, Pair<const T> b,
+                                                   const double* inv, int n,
+                                                   int level0, BcVals bv)
+{
+    const Bid bid{blockIdx.x, blockIdx.y, blockIdx.z};
+    coarse_inv_body<T>(bid, L, op, x, b, inv, n, level0, bv);
+}
+
+template <typename T>
+struct KCoarseInv
+{
+    static constexpr int bx = 64, by = 1;
+    static constexpr bool lds = false;
+    struct Args
+    {
+        LevelDims L;
+        OpView<T> op;
+        Pair<T> x;
+        Pair<const T> b;
+        const double* inv;
+        int n;
+        int level0;
+        BcVals bv;
+    };
+    __device__ static void run(const Bid bid, const Args& a)
+    {
+        coarse_inv_body<T>(bid, a.L, a.op, a.x, a.b, a.inv, a.n, a.level0,
+                           a.bv);
+    }
+    static void launch(dim3 g, size_t shm, hipStream_t s, const Args& a)
+    {
+        hipLaunchKernelGGL((k_coarse_inv<T>), g, dim3(bx, by), shm, s, a.L,
+                           a.op, a.x, a.b, a.inv, a.n, a.level0, a.bv);
+    }
+};
+
 template <typename T, int N>
 __global__ __launch_bounds__(64) void k_coarse_solve_wave(LevelDims L,
                                                           OpView<T> op,
@@ -3885,6 +3984,96 @@ __global__ __launch_bounds__(256) void k_to_half(Pair<const double> in, long n,
     out.c[c][j] = _Float16(in.c[c][j]*scale);
 }
 
+// Zeroing x and the first red-black phase from it in one pass: every
+// element of both colour arrays is set to 0, except the cells of colour
+// `colour` inside rg, which get the phase's update with all neighbours 0 --
+// the same operations on the same values as a fill followed by that phase
+// (k_fill, then rbgs_cell reading the zeros), so the same bits.  One thread
+// per element, both colours.
+template <typename T, int OP>
+__device__ __forceinline__ T first_phase_cell(const LevelDims& L,
+                                              const OpView<T>& op,
+                                              Pair<const T> b, int colour,
+                                              unsigned idx, int I, int J,
+                                              int p, int q, int K)
+{
+    const int row = (I*L.P[1] + J)*L.Hz;
+    T a[7];
+    coeffs<T, OP>(L, op, colour, int(idx), row, p, q, I, J, K, a);
+    const T nb[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+    return gs_update<T>(a, nb, ld(col(b, colour), idx));
+}
+
+template <typename T>
+__device__ __forceinline__ void zero_phase_body(const Bid bid,
+                                                const LevelDims& L,
+                                                const OpView<T>& op,
+                                                Pair<T> x, Pair<const T> b,
+                                                int colour, const Region& rg)
+{
+    const long n = L.ncol;
+    const long t = long(bid.x)*256 + threadIdx.x;
+    if (t >= 2*n) return;
+    const int c = t >= n;
+    const unsigned idx = unsigned(t - c*n);
+    T v = T(0);
+    if (c == colour)
+    {
+        const unsigned plane = unsigned(L.sI);
+        const int I = int(idx/plane);
+        const unsigned rem = idx - unsigned(I)*plane;
+        const int J = int(rem/unsigned(L.Hz));
+        const int p = int(rem - unsigned(J)*unsigned(L.Hz));
+        const int q = (c - I - J) & 1;
+        const int K = 2*p + q;
+        if (I >= rg.S[0] && I < rg.E[0] && J >= rg.S[1] && J < rg.E[1] &&
+            K >= rg.S[2] && K < rg.E[2])
+        {
+            if (op.kind == OP_STENCIL)
+                v = first_phase_cell<T, OP_STENCIL>(L, op, b, c, idx, I, J, p, q, K);
+            else if (op.kind == OP_CONST)
+                v = first_phase_cell<T, OP_CONST>(L, op, b, c, idx, I, J, p, q, K);
+            else
+                v = first_phase_cell<T, OP_FACES>(L, op, b, c, idx, I, J, p, q, K);
+        }
+    }
+    st(col(x, c), idx, v);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_zero_phase(LevelDims L, OpView<T> op,
+                                                    Pair<T> x, Pair<const T> b,
+                                                    int colour, Region rg)
+{
+    const Bid bid{blockIdx.x, blockIdx.y, blockIdx.z};
+    zero_phase_body<T>(bid, L, op, x, b, colour, rg);
+}
+
+template <typename T>
+struct KZeroPhase
+{
+    static constexpr int bx = 256, by = 1;
+    static constexpr bool lds = false;
+    struct Args
+    {
+        LevelDims L;
+        OpView<T> op;
+        Pair<T> x;
+        Pair<const T> b;
+        int colour;
+        Region rg;
+    };
+    __device__ static void run(const Bid bid, const Args& a)
+    {
+        zero_phase_body<T>(bid, a.L, a.op, a.x, a.b, a.colour, a.rg);
+    }
+    static void launch(dim3 g, size_t shm, hipStream_t s, const Args& a)
+    {
+        hipLaunchKernelGGL((k_zero_phase<T>), g, dim3(bx, by), shm, s, a.L, a.op,
+                           a.x, a.b, a.colour, a.rg);
+    }
+};
+
 template <typename T>
 struct KNfTermsAny
 {
@@ -4597,6 +4786,25 @@ void launch_face_flux(const LevelDims& L, const double* x, const double* lam,
 }
 
 template <typename T>
+void launch_zero_phase(const LevelDims& L, const OpView<T>& op, Pair<T> x,
+                       Pair<const T> b, int colour, const Region& rg,
+                       hipStream_t s)
+{
+    dispatch<KZeroPhase<T>>(dim3(unsigned((2*L.ncol + 255)/256)), 0, s,
+                            {L, op, x, b, colour, rg});
+}
+
+template <typename T>
+void launch_coarse_inv(const LevelDims& L, const OpView<T>& op, Pair<T> x,
+                       Pair<const T> b, const double* inv, int n, int level0,
+                       const BcData* bcval, hipStream_t s)
+{
+    if (n > 64) throw std::runtime_error("coarse inverse: more than 64 cells");
+    dispatch<KCoarseInv<T>>(dim3(1), 0, s,
+                            {L, op, x, b, inv, n, level0, bcvals(bcval)});
+}
+
+template <typename T>
 void launch_coarse_solve(const LevelDims& L, const OpView<T>& op, Pair<T> x,
                          Pair<const T> b, const double* lu, const int* piv,
                          int n, int level0, const BcData* bcval,
@@ -4929,6 +5137,12 @@ void launch_set_cell(Pair<T> f, long idx, int colour, T v, hipStream_t s)
                                          Pair<T>, int, int, const BcData*,    \
                                          T* const (*)[2], hipStream_t);       \
     template void launch_fill<T>(Pair<T>, long, T, hipStream_t);              \
+    template void launch_zero_phase<T>(const LevelDims&, const OpView<T>&,    \
+                                       Pair<T>, Pair<const T>, int,           \
+                                       const Region&, hipStream_t);           \
+    template void launch_coarse_inv<T>(const LevelDims&, const OpView<T>&,    \
+                                       Pair<T>, Pair<const T>, const double*, \
+                                       int, int, const BcData*, hipStream_t); \
     template void launch_defl_dot<T>(const unsigned*, const double*,          \
                                      const int*, const int*, int,             \
                                      Pair<const T>, double*, hipStream_t);    \
