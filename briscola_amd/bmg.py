@@ -32,7 +32,8 @@ EXPORTS = [
     "bmg_op_prepare", "bmg_op_smooth", "bmg_op_residual", "bmg_op_restrict",
     "bmg_op_prolong", "bmg_op_correct_bcs", "bmg_op_coarse_solve",
     "bmg_op_cycle", "bmg_op_sum_sq_residual", "bmg_time_smoother",
-    "bmg_set_graphs", "bmg_set_fused_smoothing", "bmg_solve_diagonal",
+    "bmg_set_graphs", "bmg_set_fused_smoothing",
+    "bmg_set_periodic_smoothing", "bmg_solve_diagonal",
     "bmg_batch_create",
     "bmg_batch_destroy", "bmg_solve_batch", "bmg_batch_info",
     "bmg_batch_set_autotune", "bmg_face_flux", "bmg_face_flux_device",
@@ -126,6 +127,7 @@ def load():
         "bmg_time_op": (I, [P, I, I, I, I, DP, DP]),
         "bmg_set_graphs": (I, [P, I]),
         "bmg_set_fused_smoothing": (I, [P, I]),
+        "bmg_set_periodic_smoothing": (I, [P, I]),
         "bmg_solve_diagonal": (I, [P, DP, DP, DP]),
         "bmg_batch_create": (I, [ctypes.POINTER(P), ctypes.POINTER(P), I]),
         "bmg_batch_destroy": (I, [P]),
@@ -404,6 +406,15 @@ class Context:
         LDS-staged launch on large levels without periodic faces; 2: on every
         level.  Same bits in every mode."""
         _check(self.lib.bmg_set_fused_smoothing(self.ptr, int(mode)))
+
+    def set_periodic_smoothing(self, mode):
+        """Red-black smoothing on levels with periodic faces.  1 (default):
+        in the FP32 cycles each phase covers the whole interior and writes
+        the periodic ghosts with their cells (even cell counts along the
+        periodic axes); 2: in the FP64 cycles too (not the reference's
+        order); 0: interior phases, ghost fill and boundary slabs (the
+        reference's order) everywhere."""
+        _check(self.lib.bmg_set_periodic_smoothing(self.ptr, int(mode)))
 
     def sum_sq_residual(self):
         out = np.zeros(1)

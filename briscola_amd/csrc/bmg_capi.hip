@@ -302,6 +302,34 @@ void bmg_ctx::smooth(int l, int sweeps, bool firstDone)
         return;
     }
     auto colours = phase_colours;
+    if (const int wrap = wrapAxes<T>(l))
+    {
+        // even periodic axes: phases over the whole interior that write the
+        // periodic ghosts with their cells; the ghosts are made current once
+        // on entry unless launch_zero_phase left them so
+        Region rg = interior(l, 0);
+        rg.wrap = wrap;
+        if (!firstDone)
+            launch_bc_fill(q.L, o, x, 0, l == 0, &bcval, stream);
+        int prev = -1;
+        for (int s = 0; s < sweeps; s++)
+        {
+            int cols[4];
+            const int nc = colours(rg, cols);
+            for (int i = 0; i < nc; i++)
+            {
+                if (s == 0 && i == 0 && firstDone)
+                {
+                    prev = cols[0];
+                    continue;
+                }
+                if (cols[i] != prev)
+                    launch_rbgs(q.L, o, x, b, cols[i], rg, stream);
+                prev = cols[i];
+            }
+        }
+        return;
+    }
     // the previous phase launched on this level: a phase that repeats it on
     // the same region would rewrite the same values and is skipped
     int prevColour = -1;
@@ -444,7 +472,7 @@ static long cycleTag(int height, bool after)
 }
 
 template <typename T>
-void bmg_ctx::cycle(std::vector<int>& visits, int l)
+void bmg_ctx::cycle(std::vector<int>& visits, int l, bool zeroX)
 {
     Level& q = lv[l];
     const int lc = coarseLevel();
@@ -455,13 +483,15 @@ void bmg_ctx::cycle(std::vector<int>& visits, int l)
     const bool direct = opts.coarse_mode == BMG_COARSE_DIRECT;
     const bool pre = opts.n_sweeps_pre > 0 && !(l == lc && direct);
     bool firstDone = false;
-    if (l > 0)
+    if (l > 0 || zeroX)
     {
         // x = 0 and, with red-black smoothing, the first phase from it in
         // one launch
         if (pre && opts.smoother == BMG_RBGS && !fusedLevel<T>(l))
         {
-            const Region rg = interior(l, 1);
+            const int wrap = wrapAxes<T>(l);
+            Region rg = interior(l, wrap ? 0 : 1);
+            rg.wrap = wrap;
             int cols[4];
             if (phase_colours(rg, cols) > 0)
             {
@@ -482,10 +512,11 @@ void bmg_ctx::cycle(std::vector<int>& visits, int l)
         else if (opts.cycle == BMG_FCYCLE) reps = 1 + (l > 0 && visits[l] == 0);
         for (int rep = 0; rep < reps; rep++)
         {
+            // the phases of a wrapped level leave its periodic ghosts current
             if (rep > 0 || l > 0)
-                residual<T>(l, true, false, nullptr);
+                residual<T>(l, wrapAxes<T>(l) == 0, false, nullptr);
             restrictTo<T>(l);
-            cycle<T>(visits, l + 1);
+            cycle<T>(visits, l + 1, true);
             record_tag(cycleTag(lc - l, true));
             prolongFrom<T>(l);
             if (opts.n_sweeps_post > 0)
@@ -832,8 +863,7 @@ void bmg_ctx::enqueueIteration(bool mixed)
         // defect equation A e = r in FP32, e0 = 0; x += e in FP64
         Pair<const double> r64{{q0.r[0], q0.r[1]}};
         launch_to_float(q0.L.ncol, r64, pr(q0.bf), stream);
-        launch_fill<float>(pr(q0.xf), q0.L.ncol, 0.0f, stream);
-        cycle<float>(visits, 0);
+        cycle<float>(visits, 0, true);
         launch_add_correction(q0.L, pr(q0.x), cpr(q0.xf), stream);
     }
     int nb = 0;
@@ -882,19 +912,11 @@ double bmg_ctx::iterate(bool mixed)
 // left in the level-0 x (FP64) or xf (FP32).
 void bmg_ctx::enqueuePrecondition(bool mixed)
 {
-    Level& q0 = lv[0];
-    const long n = q0.L.ncol;
     std::vector<int> visits(nLevels(), 0);
     if (!mixed)
-    {
-        launch_fill<double>(pr(q0.x), n, 0.0, stream);
-        cycle<double>(visits, 0);
-    }
+        cycle<double>(visits, 0, true);
     else
-    {
-        launch_fill<float>(pr(q0.xf), n, 0.0f, stream);
-        cycle<float>(visits, 0);
-    }
+        cycle<float>(visits, 0, true);
 }
 
 // One flexible-CG iteration with its scalars on the device (sc = dScalar):
@@ -3266,6 +3288,18 @@ int bmg_set_graphs(bmg_ctx* c, int enable)
     return run([&] {
         if (!c) throw Error(BMG_ERR_ARG, "null context");
         c->useGraphs = enable != 0;
+    });
+}
+
+int bmg_set_periodic_smoothing(bmg_ctx* c, int mode)
+{
+    return run([&] {
+        if (!c) throw Error(BMG_ERR_ARG, "null context");
+        if (mode < 0 || mode > 2)
+            throw Error(BMG_ERR_ARG,
+                        "periodic smoothing mode must be 0, 1 or 2");
+        if (c->wrapSmoothing != mode) c->gen++;
+        c->wrapSmoothing = mode;
     });
 }
 

@@ -76,6 +76,10 @@ struct TransferView {
 struct Region {
     int S[3];
     int E[3];
+    // bit d set: the periodic face ghosts of axis d are written with their
+    // cells (red-black phases of the FP32 cycles on levels with an even cell
+    // count along every periodic axis); 0 everywhere else
+    int wrap = 0;
 };
 
 // Level-0 boundary data of the Neumann (gradient) and Dirichlet (value)

@@ -364,6 +364,35 @@ __device__ __forceinline__ float gs_update<float>(const float a[7],
     return (bv - P)*__builtin_amdgcn_rcpf(a[0]);
 }
 
+// Copies of a cell's new value into the periodic face ghosts that mirror
+// it, for the axes set in wrap.  With an even cell count N along a periodic
+// axis the ghost at 0 mirrors cell N and the ghost at N+1 cell 1, both of
+// the cell's colour, so a phase of one colour keeps every ghost its
+// neighbours read up to date and no ghost fill or boundary-slab pass is
+// needed between phases.
+template <typename T>
+__device__ __forceinline__ void wrap_ghosts(const LevelDims& L,
+                                            T* __restrict__ xc, unsigned idx,
+                                            int I, int J, int K, int wrap,
+                                            T v)
+{
+    if ((wrap & 1) && (I == 1 || I == L.N[0]))
+    {
+        const unsigned d = unsigned(L.N[0])*L.sI;
+        st(xc, I == 1 ? idx + d : idx - d, v);
+    }
+    if ((wrap & 2) && (J == 1 || J == L.N[1]))
+    {
+        const unsigned d = unsigned(L.N[1])*unsigned(L.Hz);
+        st(xc, J == 1 ? idx + d : idx - d, v);
+    }
+    if ((wrap & 4) && (K == 1 || K == L.N[2]))
+    {
+        const unsigned d = unsigned(L.N[2] >> 1);
+        st(xc, K == 1 ? idx + d : idx - d, v);
+    }
+}
+
 // One cell of a red-black phase: row J of plane I, the off-th cell of the
 // colour in the row.
 template <typename T, int OP>
@@ -390,7 +419,9 @@ __device__ __forceinline__ void rbgs_cell(const LevelDims& L,
     const T bv = ld(bcl, idx);
     T a[7];
     coeffs<T, OP>(L, op, colour, int(idx), row, p, q, I, J, K, a);
-    st(xc, idx, gs_update<T>(a, nb, bv));
+    const T v = gs_update<T>(a, nb, bv);
+    st(xc, idx, v);
+    if (rg.wrap) wrap_ghosts(L, xc, idx, I, J, K, rg.wrap, v);
 }
 
 // ---------------------------------------------------------------------------
@@ -3988,8 +4019,10 @@ __global__ __launch_bounds__(256) void k_to_half(Pair<const double> in, long n,
 // element of both colour arrays is set to 0, except the cells of colour
 // `colour` inside rg, which get the phase's update with all neighbours 0 --
 // the same operations on the same values as a fill followed by that phase
-// (k_fill, then rbgs_cell reading the zeros), so the same bits.  One thread
-// per element, both colours.
+// (k_fill, then rbgs_cell reading the zeros), so the same bits.  With
+// rg.wrap, a periodic face ghost gets the value of the cell it mirrors, as
+// the phase's wrap_ghosts would write it.  One thread per element, both
+// colours.
 template <typename T, int OP>
 __device__ __forceinline__ T first_phase_cell(const LevelDims& L,
                                               const OpView<T>& op,
@@ -4020,21 +4053,48 @@ __device__ __forceinline__ void zero_phase_body(const Bid bid,
     if (c == colour)
     {
         const unsigned plane = unsigned(L.sI);
-        const int I = int(idx/plane);
+        int I = int(idx/plane);
         const unsigned rem = idx - unsigned(I)*plane;
-        const int J = int(rem/unsigned(L.Hz));
-        const int p = int(rem - unsigned(J)*unsigned(L.Hz));
+        int J = int(rem/unsigned(L.Hz));
+        int p = int(rem - unsigned(J)*unsigned(L.Hz));
         const int q = (c - I - J) & 1;
-        const int K = 2*p + q;
+        int K = 2*p + q;
+        unsigned src = idx;
+        if (rg.wrap)
+        {
+            // a periodic face ghost takes the value of the cell it mirrors
+            // (an even count along the axis keeps the colour and q)
+            int out = 0;
+            if ((rg.wrap & 1) && (I == 0 || I == L.N[0] + 1))
+            {
+                I = I == 0 ? L.N[0] : 1;
+                out++;
+            }
+            if ((rg.wrap & 2) && (J == 0 || J == L.N[1] + 1))
+            {
+                J = J == 0 ? L.N[1] : 1;
+                out++;
+            }
+            if ((rg.wrap & 4) && (K == 0 || K == L.N[2] + 1))
+            {
+                K = K == 0 ? L.N[2] : 1;
+                p = K >> 1;
+                out++;
+            }
+            if (out == 1)
+                src = unsigned((I*L.P[1] + J)*L.Hz + p);
+            else if (out > 1)
+                I = -1;   // edge or corner ghost: left 0, as a face fill
+        }
         if (I >= rg.S[0] && I < rg.E[0] && J >= rg.S[1] && J < rg.E[1] &&
             K >= rg.S[2] && K < rg.E[2])
         {
             if (op.kind == OP_STENCIL)
-                v = first_phase_cell<T, OP_STENCIL>(L, op, b, c, idx, I, J, p, q, K);
+                v = first_phase_cell<T, OP_STENCIL>(L, op, b, c, src, I, J, p, q, K);
             else if (op.kind == OP_CONST)
-                v = first_phase_cell<T, OP_CONST>(L, op, b, c, idx, I, J, p, q, K);
+                v = first_phase_cell<T, OP_CONST>(L, op, b, c, src, I, J, p, q, K);
             else
-                v = first_phase_cell<T, OP_FACES>(L, op, b, c, idx, I, J, p, q, K);
+                v = first_phase_cell<T, OP_FACES>(L, op, b, c, src, I, J, p, q, K);
         }
     }
     st(col(x, c), idx, v);

@@ -444,6 +444,18 @@ int bmg_set_graphs(bmg_ctx* ctx, int enable);
  * counterpart (the order of the phases is rbgsSmoother.C:376-437's either
  * way). */
 int bmg_set_fused_smoothing(bmg_ctx* ctx, int mode);
+/* Red-black smoothing on levels with periodic faces.  mode 1 (default): in
+ * the FP32 cycles of mixed precision each phase covers the whole interior
+ * and writes the periodic face ghosts together with the cells they mirror
+ * (levels with an even count of at least 8 cells along every periodic
+ * axis), so a sweep
+ * is its phases only; the FP64 cycles keep the reference's order --
+ * interior phases, ghost fill, then the boundary slabs
+ * (rbgsSmoother.C:186-374) -- bit for bit.  mode 2: the wrapped phases in
+ * the FP64 cycles as well (periodic Gauss-Seidel in another order than the
+ * reference's: not bitwise equal to it).  mode 0: the reference's order in
+ * the FP32 cycles too.  No reference counterpart. */
+int bmg_set_periodic_smoothing(bmg_ctx* ctx, int mode);
 
 #ifdef __cplusplus
 }
