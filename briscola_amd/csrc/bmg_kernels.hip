@@ -226,6 +226,9 @@ struct Tiles {
     int per;   // tiles per XCD
     int ws;    // log2 of the cells of a row a wavefront covers (6: one row
                // segment of 64 per wavefront; below 6, 64 >> ws rows)
+    int rpt = 1;   // red-black phases: rows per thread (1 or 2; with 2 a
+                   // tile covers twice the rows, thread rows J and J + BY
+                   // wavefront rows apart)
 };
 
 __device__ __forceinline__ bool tile_of(const Tiles& t, unsigned w, int& bx,
@@ -424,6 +427,79 @@ __device__ __forceinline__ void rbgs_cell(const LevelDims& L,
     if (rg.wrap) wrap_ghosts(L, xc, idx, I, J, K, rg.wrap, v);
 }
 
+// R cells of a red-black phase, rows J[0..R) of plane I at offsets off[]
+// of the colour: every load of the R cells is issued before the arithmetic
+// of any, R times the bytes in flight per wavefront.  The per-cell
+// arithmetic is rbgs_cell's.
+template <typename T, int OP, int R>
+__device__ __forceinline__ void rbgs_cells(const LevelDims& L,
+                                           const OpView<T>& op,
+                                           T* __restrict__ xc,
+                                           const T* __restrict__ xo,
+                                           const T* __restrict__ bcl,
+                                           int colour, const Region& rg,
+                                           int I, const int Jin[R],
+                                           const int off[R])
+{
+    int J[R], p[R], q[R], K[R], row[R];
+    bool ok[R];
+    int src = -1;
+#pragma unroll
+    for (int c = 0; c < R; c++)
+    {
+        J[c] = Jin[c];
+        q[c] = (colour - I - J[c]) & 1;
+        p[c] = ((rg.S[2] - q[c] + 1) >> 1) + off[c];
+        K[c] = 2*p[c] + q[c];
+        ok[c] = J[c] < rg.E[1] && K[c] < rg.E[2];
+        if (ok[c] && src < 0) src = c;
+    }
+    if (src < 0) return;
+    // a cell outside the region takes a valid one's position: its loads stay
+    // in bounds and its result is not stored
+#pragma unroll
+    for (int c = 0; c < R; c++)
+        if (!ok[c])
+        {
+            J[c] = J[src];
+            q[c] = q[src];
+            p[c] = p[src];
+            K[c] = K[src];
+        }
+    const unsigned sI = L.sI, Hz = L.Hz;
+    unsigned idx[R];
+    T nb[R][6], bv[R];
+#pragma unroll
+    for (int c = 0; c < R; c++)
+    {
+        row[c] = (I*L.P[1] + J[c])*L.Hz;
+        idx[c] = unsigned(row[c] + p[c]);
+        const unsigned i = idx[c];
+        nb[c][0] = ld(xo, i - sI);
+        nb[c][1] = ld(xo, i + sI);
+        nb[c][2] = ld(xo, i - Hz);
+        nb[c][3] = ld(xo, i + Hz);
+        nb[c][4] = ld(xo, i - 1 + q[c]);
+        nb[c][5] = ld(xo, i + q[c]);
+        bv[c] = ld(bcl, i);
+    }
+    T a[R][7];
+#pragma unroll
+    for (int c = 0; c < R; c++)
+        coeffs<T, OP>(L, op, colour, int(idx[c]), row[c], p[c], q[c], I, J[c],
+                      K[c], a[c]);
+#pragma unroll
+    for (int c = 0; c < R; c++)
+    {
+        const T v = gs_update<T>(a[c], nb[c], bv[c]);
+        if (ok[c])
+        {
+            st(xc, idx[c], v);
+            if (rg.wrap) wrap_ghosts(L, xc, idx[c], I, J[c], K[c], rg.wrap, v);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Red-black Gauss-Seidel phase: update every cell of one colour inside
 // region [S,E) (rbgsSmoother.C:65-183):
@@ -446,6 +522,37 @@ __device__ __forceinline__ void rbgs_body(const Bid bid, const LevelDims& L,
     int bx, by, bz;
     if (!tile_of(tl, bid.x, bx, by, bz)) return;
     const int I = rg.S[0] + bz;
+    if (tl.rpt > 1)
+    {
+        // rpt rows per thread, BY wavefront rows apart
+        const int ws = tl.ws;
+        const int lane = int(threadIdx.x);
+        const int wr = 64 >> ws;   // rows per wavefront row
+        int J0, off0;
+        if (ws == 6)
+        {
+            J0 = __builtin_amdgcn_readfirstlane(rg.S[1] + tl.rpt*by*BY +
+                                                int(threadIdx.y));
+            off0 = bx*BX + lane;
+        }
+        else
+        {
+            J0 = rg.S[1] + (tl.rpt*by*BY + int(threadIdx.y))*wr + (lane >> ws);
+            off0 = (bx << ws) + (lane & ((1 << ws) - 1));
+        }
+        if (tl.rpt == 2)
+        {
+            const int J[2] = {J0, J0 + BY*wr}, off[2] = {off0, off0};
+            rbgs_cells<T, OP, 2>(L, op, xc, xo, bcl, colour, rg, I, J, off);
+        }
+        else
+        {
+            const int J[4] = {J0, J0 + BY*wr, J0 + 2*BY*wr, J0 + 3*BY*wr};
+            const int off[4] = {off0, off0, off0, off0};
+            rbgs_cells<T, OP, 4>(L, op, xc, xo, bcl, colour, rg, I, J, off);
+        }
+        return;
+    }
     if (tl.ws == 6)
     {
         // a wavefront is one row (BX = 64), so J and everything derived
@@ -4354,11 +4461,29 @@ void launch_rbgs(const LevelDims& L, const OpView<T>& op, Pair<T> x,
     // counted from the first of them (pmin in rbgs_cell)
     const int nK = rg.E[2] - rg.S[2];
     const int ws = row_pack_shift((nK + 1)/2);
-    const int rows = BY*(64 >> ws);
-    Tiles tl = make_tiles(((nK + 1)/2 + (1 << ws) - 1) >> ws,
-                          (rg.E[1] - rg.S[1] + rows - 1)/rows,
-                          rg.E[0] - rg.S[0]);
-    tl.ws = ws;
+    // FP32 phases on large levels: 4 (or 2) rows per thread, so that a
+    // wavefront has more loads in flight (spheres 256^3 level-0 FP32 phase
+    // 0.66 -> 0.79 of HBM peak, matrix-free channel 0.37 -> 0.52, measured),
+    // while the launch keeps at least 1024 (256) workgroups.  FP64 phases
+    // measured no faster with more rows per thread: one.
+    auto tiles_at = [&](int r)
+    {
+        const int rows = BY*(64 >> ws)*r;
+        Tiles t = make_tiles(((nK + 1)/2 + (1 << ws) - 1) >> ws,
+                             (rg.E[1] - rg.S[1] + rows - 1)/rows,
+                             rg.E[0] - rg.S[0]);
+        t.ws = ws;
+        t.rpt = r;
+        return t;
+    };
+    Tiles tl = tiles_at(1);
+    if (std::is_same<T, float>::value)
+    {
+        if (tiles_count(tiles_at(4)) >= 1024)
+            tl = tiles_at(4);
+        else if (tiles_count(tiles_at(2)) >= 256)
+            tl = tiles_at(2);
+    }
     const dim3 g = tiles_grid(tl);
     T* xc = x.c[colour];
     const T* xo = x.c[colour ^ 1];
