@@ -1196,6 +1196,25 @@ struct KResidual
 };
 
 
+// 1/x: correctly rounded in FP64 (the reference's division); the hardware
+// reciprocal in the FP32 cycles, which carry no bitwise contract.
+__device__ __forceinline__ double recip(double x) { return 1.0/x; }
+__device__ __forceinline__ float recip(float x)
+{
+    return __builtin_amdgcn_rcpf(x);
+}
+
+// c + a*b: product and sum each rounded in FP64 (the reference's order), one
+// fused multiply-add in FP32.
+__device__ __forceinline__ double madd(double a, double b, double c)
+{
+    return c + a*b;
+}
+__device__ __forceinline__ float madd(float a, float b, float c)
+{
+    return __builtin_fmaf(a, b, c);
+}
+
 // ---------------------------------------------------------------------------
 // Scaled linear restriction (linearRestrictionScheme.C:103-160, weights
 // :15-80): b_c = cvc * sum_q (w_q * r_q) * (1/cvf_q), q over the 8 children
@@ -1270,9 +1289,9 @@ __device__ __forceinline__ void restrict_impl(const Bid bid,
                 T cvf = tv.fw[0][fi[cx]]*tv.fw[1][fj[cy]];
                 cvf = cvf*tv.fw[2][fk];
                 cvf = cvf > T(1e-16) ? cvf : T(1e-16);
-                const T icvf = T(1.0)/cvf;
+                const T icvf = recip(cvf);
                 const T t = w*rv[h][cz][cy][cx];
-                sum = sum + t*icvf;
+                sum = madd(t, icvf, sum);
             }
         }
         T cvc = tv.cw[0][I]*tv.cw[1][J];
@@ -1574,21 +1593,44 @@ __device__ __forceinline__ void prolong8_impl(const Bid bid,
             const bool inside = !(u0 < T(0) || u0 >= T(1) || u1 < T(0) ||
                                   u1 >= T(1) || u2 < T(0) || u2 >= T(1));
             if (!inside) { u0 = T(0.5); u1 = T(0.5); u2 = T(0.5); }
-            const T gx[2] = {T(1.0) - u0, u0};
-            const T gy[2] = {T(1.0) - u1, u1};
-            const T gz[2] = {T(1.0) - u2, u2};
             T value = T(0);
-#pragma unroll
-            for (int cz = 0; cz < 2; cz++)
-#pragma unroll
-            for (int cy = 0; cy < 2; cy++)
-#pragma unroll
-            for (int cx = 0; cx < 2; cx++)
+            if constexpr (std::is_same<T, float>::value)
             {
-                T w = gx[cx]*gy[cy];
-                w = w*gz[cz];
-                const T t = w*cv[sx[cx]][sy[cy]][sz[cz]];
-                value = value + t;
+                // FP32 cycles: the same trilinear value as nested linear
+                // interpolations in x, then y, then z (7 per child)
+                float ry[2];
+#pragma unroll
+                for (int cz = 0; cz < 2; cz++)
+                {
+                    float rx[2];
+#pragma unroll
+                    for (int cy = 0; cy < 2; cy++)
+                    {
+                        const float v0 = cv[sx[0]][sy[cy]][sz[cz]];
+                        const float v1 = cv[sx[1]][sy[cy]][sz[cz]];
+                        rx[cy] = __builtin_fmaf(u0, v1 - v0, v0);
+                    }
+                    ry[cz] = __builtin_fmaf(u1, rx[1] - rx[0], rx[0]);
+                }
+                value = __builtin_fmaf(u2, ry[1] - ry[0], ry[0]);
+            }
+            else
+            {
+                const T gx[2] = {T(1.0) - u0, u0};
+                const T gy[2] = {T(1.0) - u1, u1};
+                const T gz[2] = {T(1.0) - u2, u2};
+#pragma unroll
+                for (int cz = 0; cz < 2; cz++)
+#pragma unroll
+                for (int cy = 0; cy < 2; cy++)
+#pragma unroll
+                for (int cx = 0; cx < 2; cx++)
+                {
+                    T w = gx[cx]*gy[cy];
+                    w = w*gz[cz];
+                    const T t = w*cv[sx[cx]][sy[cy]][sz[cz]];
+                    value = value + t;
+                }
             }
             add[c] = value;
         }

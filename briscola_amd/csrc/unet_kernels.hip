@@ -110,6 +110,196 @@ __global__ __launch_bounds__(256) void k_conv_fwd(ConvArgs a)
     }
 }
 
+// LDS-staged forward convolution, 3x3x3.  A workgroup of 4 wavefronts
+// computes an output tile of 16 x TY x TZ voxels (x fastest): tile row
+// r = 0 .. 4*MT-1 is (y, z) = (r % TY, r / TY) with 16 consecutive x, and
+// wavefront w owns rows w*MT .. w*MT+MT-1, one 16-voxel M tile each, times
+// all NT output-channel tiles.  The input tile with its one-voxel halo,
+// 18 x (TY+2) x (TZ+2) voxels of CH channels, is staged in LDS once per
+// channel chunk (CH = Cin when Cin <= 48, else 32-channel chunks, Cin % 32
+// == 0), zero outside the domain; every tap's A fragment is then a 16-byte
+// LDS read, shared by the NT MFMAs of the M tile, and each B fragment (the
+// weights of one k step) is read once per wavefront and shared by its MT
+// tiles.  K order within a chunk of all channels: k = t*Cin + c as the
+// weights are stored; with chunks, tap-major within the chunk (k = t*Cin +
+// chunk*32 + 0..31), the same products summed in another order.
+template <int MT>
+struct ConvTile {
+    static constexpr int TY = MT == 4 ? 8 : (MT == 2 ? 4 : 2);
+    static constexpr int TZ = 4*MT/TY;
+    static constexpr int HX = 18, HY = TY + 2, HZ = TZ + 2;
+    static constexpr int HV = HX*HY*HZ;
+};
+
+// CIN > 0: all CIN channels staged at once, the k loop unrolled with each
+// step's tap and channel known at compile time per lane group; CIN = 0:
+// 32-channel chunks of a runtime channel count (a multiple of 32), the 27
+// taps unrolled.  Consecutive k steps alternate between two accumulator
+// sets (AS = 2 when the tiles leave registers for it) so that dependent
+// MFMAs do not follow each other.
+template <int NT, int MT, int CIN>
+__global__ __launch_bounds__(256) void k_conv_fwd_lds(ConvArgs a, int vs)
+{
+    using Tl = ConvTile<MT>;
+    constexpr int AS = NT*MT <= 8 ? 2 : 1;
+    constexpr int CH = CIN > 0 ? CIN : 32;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    bf16* lds = reinterpret_cast<bf16*>(lds_raw);
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int row = lane & 15;
+    const int g = lane >> 4;
+    const int kq = g*8;
+    const int W = a.g.W, H = a.g.H, D = a.g.D;
+    const int ntx = (W + 15)/16, nty = (H + Tl::TY - 1)/Tl::TY;
+    const int b = blockIdx.x;
+    const int x0 = (b % ntx)*16;
+    const int y0 = ((b/ntx) % nty)*Tl::TY;
+    const int z0 = (b/(ntx*nty))*Tl::TZ;
+    const int nchunk = a.cin/CH;
+    constexpr int parts = CH/8;
+    floatx4 acc[AS][MT][NT];
+#pragma unroll
+    for (int h = 0; h < AS; h++)
+#pragma unroll
+    for (int i = 0; i < MT; i++)
+#pragma unroll
+        for (int j = 0; j < NT; j++)
+            acc[h][i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    // LDS element offset of this lane's voxel in tile row wave*MT + i (tap
+    // offsets 0, the corner of its 3x3x3 stencil in the halo tile)
+    int base[MT];
+#pragma unroll
+    for (int i = 0; i < MT; i++)
+    {
+        const int r = wave*MT + i;
+        base[i] = (((r/Tl::TY)*Tl::HY + r % Tl::TY)*Tl::HX + row)*vs;
+    }
+    const bf16* wrow[NT];
+#pragma unroll
+    for (int j = 0; j < NT; j++) wrow[j] = a.w + size_t(j*16 + row)*a.kp + kq;
+    // halo-tile offset of tap t, in voxels
+    auto tap_off = [](int t)
+    {
+        return (t/9)*Tl::HY*Tl::HX + ((t/3) % 3)*Tl::HX + t % 3;
+    };
+    for (int c0 = 0; c0 < nchunk; c0++)
+    {
+        if (c0 > 0) __syncthreads();
+        for (int u = threadIdx.x; u < Tl::HV*parts; u += 256)
+        {
+            const int v = u/parts, part = u - v*parts;
+            const int hx = v % Tl::HX, hy = (v/Tl::HX) % Tl::HY;
+            const int hz = v/(Tl::HX*Tl::HY);
+            const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
+            bf16x8 val = zero8();
+            if (gx >= 0 && gx < W && gy >= 0 && gy < H && gz >= 0 && gz < D)
+                val = *reinterpret_cast<const bf16x8*>(
+                    a.x + size_t((gz*H + gy)*W + gx)*a.ldx + c0*CH + part*8);
+            *reinterpret_cast<bf16x8*>(lds + v*vs + part*8) = val;
+        }
+        __syncthreads();
+        if constexpr (CIN > 0)
+        {
+            constexpr int K = 27*CIN;
+            constexpr int NS = (K + 31)/32;
+#pragma unroll
+            for (int st = 0; st < NS; st++)
+            {
+                // this lane group's k = 32 st + 8 g: tap and channel
+                int off = -1;
+#pragma unroll
+                for (int gg = 0; gg < 4; gg++)
+                {
+                    const int k = st*32 + gg*8;
+                    if (k < K && g == gg)
+                        off = tap_off(k/CIN)*vs + (k % CIN);
+                }
+                bf16x8 bv[NT];
+#pragma unroll
+                for (int j = 0; j < NT; j++)
+                    bv[j] = *reinterpret_cast<const bf16x8*>(wrow[j] + st*32);
+#pragma unroll
+                for (int i = 0; i < MT; i++)
+                {
+                    bf16x8 av = zero8();
+                    if (off >= 0)
+                        av = *reinterpret_cast<const bf16x8*>(
+                            lds + base[i] + off);
+#pragma unroll
+                    for (int j = 0; j < NT; j++)
+                        acc[st % AS][i][j] =
+                            __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                av, bv[j], acc[st % AS][i][j], 0, 0, 0);
+                }
+            }
+        }
+        else
+        {
+#pragma unroll
+            for (int t = 0; t < 27; t++)
+            {
+                const int off = tap_off(t)*vs + kq;
+                const int k = t*a.cin + c0*CH;
+                bf16x8 bv[NT];
+#pragma unroll
+                for (int j = 0; j < NT; j++)
+                    bv[j] = *reinterpret_cast<const bf16x8*>(wrow[j] + k);
+#pragma unroll
+                for (int i = 0; i < MT; i++)
+                {
+                    const bf16x8 av = *reinterpret_cast<const bf16x8*>(
+                        lds + base[i] + off);
+#pragma unroll
+                    for (int j = 0; j < NT; j++)
+                        acc[t % AS][i][j] =
+                            __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                av, bv[j], acc[t % AS][i][j], 0, 0, 0);
+                }
+            }
+        }
+    }
+    // D of tile (i, j): lane holds channel n = j*16 + row for the voxels
+    // x = x0 + 4*(lane/16) + q, q = 0..3, of tile row wave*MT + i.  Each
+    // 16 x 16 tile goes through the wavefront's own 512 bytes of LDS (the
+    // input tile is no longer read) so that a lane stores 8 channels of one
+    // voxel, 16 bytes; every wavefront passes the same barriers.
+    __syncthreads();
+    bf16* tb = lds + wave*256;
+#pragma unroll
+    for (int i = 0; i < MT; i++)
+    {
+        const int r = wave*MT + i;
+        const int yy = y0 + r % Tl::TY, zz = z0 + r/Tl::TY;
+#pragma unroll
+        for (int j = 0; j < NT; j++)
+        {
+            const int n = j*16 + row;
+            const float bias = a.bias ? a.bias[n] : 0.0f;
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+            {
+                float v = acc[0][i][j][q] + bias;
+                if (AS == 2) v = acc[0][i][j][q] + acc[AS - 1][i][j][q] + bias;
+                if (a.relu && v < 0.0f) v = 0.0f;
+                tb[(4*g + q)*16 + row] = bf16(v);
+            }
+            __syncthreads();
+            // lanes 0..31: voxel x0 + lane/2, channels j*16 + 8*(lane&1)
+            if (lane < 32 && yy < H && zz < D)
+            {
+                const int xx = x0 + (lane >> 1);
+                const int n0 = j*16 + 8*(lane & 1);
+                if (xx < W && n0 < a.cout)
+                    *reinterpret_cast<bf16x8*>(
+                        a.y + size_t((zz*H + yy)*W + xx)*a.ldy + n0) =
+                        *reinterpret_cast<const bf16x8*>(tb + lane*8);
+            }
+            __syncthreads();
+        }
+    }
+}
+
 // Weight gradient dW[n][k] += sum over voxels of dZ[m][n] * X[tap(m)][c] for
 // one 16 x 16 tile of (n, k) and one chunk of voxels per wavefront; the
 // chunk partial sums are added into the FP32 gradient atomically.
@@ -244,6 +434,51 @@ __global__ void k_up(const bf16* x, int ldx, Grid3 g, int C, bf16* y, int ldy)
     const int xx = m % g.W, yy = (m/g.W) % g.H, zz = m/(g.W*g.H);
     const int mc = ((zz/2)*(g.H/2) + yy/2)*(g.W/2) + xx/2;
     y[size_t(m)*ldy + c] = x[size_t(mc)*ldx + c];
+}
+
+// k_up and k_pool on 8 channels per thread (16-byte loads and stores;
+// C, the row strides and the slice bases multiples of 8 channels), with
+// the same per-channel arithmetic.
+__global__ void k_up8(const bf16* x, int ldx, Grid3 g, int C, bf16* y,
+                      int ldy)
+{
+    const int C8 = C/8;
+    const long i = long(blockIdx.x)*blockDim.x + threadIdx.x;
+    if (i >= long(g.D)*g.H*g.W*C8) return;
+    const int c = int(i % C8)*8;
+    const int m = int(i/C8);
+    const int xx = m % g.W, yy = (m/g.W) % g.H, zz = m/(g.W*g.H);
+    const int mc = ((zz/2)*(g.H/2) + yy/2)*(g.W/2) + xx/2;
+    *reinterpret_cast<bf16x8*>(y + size_t(m)*ldy + c) =
+        *reinterpret_cast<const bf16x8*>(x + size_t(mc)*ldx + c);
+}
+
+__global__ void k_pool8(const bf16* x, int ldx, Grid3 g, int C, bf16* y,
+                        int ldy)
+{
+    const int Dc = g.D/2, Hc = g.H/2, Wc = g.W/2;
+    const int C8 = C/8;
+    const long i = long(blockIdx.x)*blockDim.x + threadIdx.x;
+    if (i >= long(Dc)*Hc*Wc*C8) return;
+    const int c = int(i % C8)*8;
+    const int mc = int(i/C8);
+    const int xc = mc % Wc, yc = (mc/Wc) % Hc, zc = mc/(Wc*Hc);
+    float s[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) s[q] = 0.0f;
+    for (int d = 0; d < 8; d++)
+    {
+        const int zf = 2*zc + (d >> 2), yf = 2*yc + ((d >> 1) & 1),
+                  xf = 2*xc + (d & 1);
+        const bf16x8 v = *reinterpret_cast<const bf16x8*>(
+            x + size_t((zf*g.H + yf)*g.W + xf)*ldx + c);
+#pragma unroll
+        for (int q = 0; q < 8; q++) s[q] += float(v[q]);
+    }
+    bf16x8 o;
+#pragma unroll
+    for (int q = 0; q < 8; q++) o[q] = bf16(0.125f*s[q]);
+    *reinterpret_cast<bf16x8*>(y + size_t(mc)*ldy + c) = o;
 }
 
 // Gradient of the upsampling: the coarse voxel sums its 8 children.
@@ -412,12 +647,72 @@ inline unsigned nblk(long n, int b) { return unsigned((n + b - 1)/b); }
 
 }  // namespace
 
+namespace {
+
+template <int NT, int MT, int CIN>
+void launch_conv_lds(const ConvArgs& a, hipStream_t s)
+{
+    using Tl = ConvTile<MT>;
+    const int ch = CIN > 0 ? CIN : 32;
+    const int vs = ch + 8;
+    const size_t shm = size_t(Tl::HV)*vs*sizeof(bf16);
+    const unsigned nb = unsigned((a.g.W + 15)/16)*
+                        unsigned((a.g.H + Tl::TY - 1)/Tl::TY)*
+                        unsigned((a.g.D + Tl::TZ - 1)/Tl::TZ);
+    hipLaunchKernelGGL((k_conv_fwd_lds<NT, MT, CIN>), dim3(nb), dim3(256), shm,
+                       s, a, vs);
+}
+
+template <int NT, int CIN>
+void conv_lds_mt(const ConvArgs& a, hipStream_t s)
+{
+    constexpr int MT0 = NT <= 4 ? 4 : (NT <= 8 ? 2 : 1);
+    constexpr int ch = CIN > 0 ? CIN : 32;
+    constexpr bool fit4 = ConvTile<4>::HV*(ch + 8)*2 <= 65536;
+    constexpr bool fit2 = ConvTile<2>::HV*(ch + 8)*2 <= 65536;
+    constexpr int MT = (MT0 >= 4 && fit4) ? 4 : ((MT0 >= 2 && fit2) ? 2 : 1);
+    launch_conv_lds<NT, MT, CIN>(a, s);
+}
+
+// Output tiles per wavefront: at most 16 accumulator tiles (NT x MT), and
+// a staged input tile within 64 KB of LDS (measured: Cin = 48 with four M
+// tiles per wavefront in 80 KB was slower than two in 48 KB).
+template <int NT>
+void conv_lds(const ConvArgs& a, hipStream_t s)
+{
+    switch (a.cin)
+    {
+    case 8: conv_lds_mt<NT, 8>(a, s); return;
+    case 16: conv_lds_mt<NT, 16>(a, s); return;
+    case 32: conv_lds_mt<NT, 32>(a, s); return;
+    case 48: conv_lds_mt<NT, 48>(a, s); return;
+    default: conv_lds_mt<NT, 0>(a, s); return;
+    }
+}
+
+}  // namespace
+
 void conv_fwd(const ConvArgs& a, hipStream_t s)
 {
     const int M = a.g.D*a.g.H*a.g.W;
     const dim3 g(nblk(M, 64));
     if (a.cout % 16 || a.cin % 8 || a.ldx % 8 || a.kp % 32)
         throw UnetError("convolution shape not supported");
+    if (a.taps == 27 && (a.cin == 8 || a.cin == 16 || a.cin == 48 ||
+                         a.cin % 32 == 0))
+    {
+        switch (a.cout/16)
+        {
+        case 1: conv_lds<1>(a, s); return;
+        case 2: conv_lds<2>(a, s); return;
+        case 3: conv_lds<3>(a, s); return;
+        case 4: conv_lds<4>(a, s); return;
+        case 6: conv_lds<6>(a, s); return;
+        case 8: conv_lds<8>(a, s); return;
+        case 12: conv_lds<12>(a, s); return;
+        default: throw UnetError("unsupported output channel count");
+        }
+    }
     switch (a.cout/16)
     {
     case 1: hipLaunchKernelGGL((k_conv_fwd<1>), g, dim3(256), 0, s, a); break;
@@ -457,8 +752,14 @@ void pool(const bf16* x, int ldx, Grid3 g, int C, bf16* y, int ldy,
           hipStream_t s)
 {
     const long n = long(g.D/2)*(g.H/2)*(g.W/2)*C;
-    hipLaunchKernelGGL(k_pool, dim3(nblk(n, 256)), dim3(256), 0, s, x, ldx, g,
-                       C, y, ldy);
+    if (C % 8 == 0 && ldx % 8 == 0 && ldy % 8 == 0 &&
+        reinterpret_cast<uintptr_t>(x) % 16 == 0 &&
+        reinterpret_cast<uintptr_t>(y) % 16 == 0)
+        hipLaunchKernelGGL(k_pool8, dim3(nblk(n/8, 256)), dim3(256), 0, s, x,
+                           ldx, g, C, y, ldy);
+    else
+        hipLaunchKernelGGL(k_pool, dim3(nblk(n, 256)), dim3(256), 0, s, x,
+                           ldx, g, C, y, ldy);
 }
 
 void pool_bwd(const bf16* dy, int lddy, Grid3 g, int C, bf16* dx, int lddx,
@@ -473,8 +774,14 @@ void up(const bf16* x, int ldx, Grid3 g, int C, bf16* y, int ldy,
         hipStream_t s)
 {
     const long n = long(g.D)*g.H*g.W*C;
-    hipLaunchKernelGGL(k_up, dim3(nblk(n, 256)), dim3(256), 0, s, x, ldx, g, C,
-                       y, ldy);
+    if (C % 8 == 0 && ldx % 8 == 0 && ldy % 8 == 0 &&
+        reinterpret_cast<uintptr_t>(x) % 16 == 0 &&
+        reinterpret_cast<uintptr_t>(y) % 16 == 0)
+        hipLaunchKernelGGL(k_up8, dim3(nblk(n/8, 256)), dim3(256), 0, s, x, ldx,
+                           g, C, y, ldy);
+    else
+        hipLaunchKernelGGL(k_up, dim3(nblk(n, 256)), dim3(256), 0, s, x, ldx, g,
+                           C, y, ldy);
 }
 
 void up_bwd(const bf16* dy, int lddy, Grid3 g, int C, bf16* dx, int lddx,

@@ -36,6 +36,10 @@ def close(gpu, ref, rel=2e-2):
     ((8, 16, 8), 8, 32, False),
     ((16, 8, 8), 48, 48, True),
     ((8, 8, 16), 192, 64, True),
+    # partial output tiles (x, y and z extents not multiples of the tile)
+    ((6, 10, 20), 32, 96, True),
+    ((4, 6, 12), 96, 192, False),
+    ((3, 5, 7), 8, 128, True),
 ])
 def test_conv_forward(gpu, shape, cin, cout, relu):
     r = np.random.default_rng(cin + cout)
