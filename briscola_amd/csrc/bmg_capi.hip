@@ -339,22 +339,28 @@ void bmg_ctx::smooth(int l, int sweeps, bool firstDone)
         const Region rg = interior(l, 1);
         int cols[4];
         const int nc = colours(rg, cols);
+        // the first phase already run from x = 0 (launch_zero_phase)
+        auto done = [&](int i) { return s == 0 && i == 0 && firstDone; };
+        // periodic ghosts, plus snapshots of the slab planes: filled in the
+        // launch of the sweep's last interior phase (the fill only reads
+        // slab cells, which the interior phases do not write)
+        T* const (*sn)[2] = anyPeriodic() ? Snap<T>(l) : nullptr;
+        int last = -1;
+        for (int i = 0, pc = prevColour; i < nc; i++)
+        {
+            if (!done(i) && cols[i] != pc) last = i;
+            pc = cols[i];
+        }
         for (int i = 0; i < nc; i++)
         {
-            // the first phase already run from x = 0 (launch_zero_phase)
-            if (s == 0 && i == 0 && firstDone)
-            {
-                prevColour = cols[0];
-                continue;
-            }
-            if (cols[i] != prevColour)
-                launch_rbgs(q.L, o, x, b, cols[i], rg, stream);
+            if (!done(i) && cols[i] != prevColour)
+                launch_rbgs_fill(q.L, o, x, b, cols[i], rg, l == 0, &bcval,
+                                 i == last ? sn : nullptr, stream);
             prevColour = cols[i];
         }
         if (!anyPeriodic()) continue;
-        // periodic ghosts, plus snapshots of the slab planes
-        T* const (*sn)[2] = Snap<T>(l);
-        launch_bc_fill_snap(q.L, o, x, 0, l == 0, &bcval, sn, stream);
+        if (last < 0)
+            launch_bc_fill_snap(q.L, o, x, 0, l == 0, &bcval, sn, stream);
         // width-1 slabs of the periodic faces, face pairs in order: one
         // launch per pair when its two planes are independent, else one per
         // plane

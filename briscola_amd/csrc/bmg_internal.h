@@ -156,6 +156,14 @@ template <typename T>
 void launch_rbgs(const LevelDims& L, const OpView<T>& op, Pair<T> x,
                  Pair<const T> b, int colour, const Region& rg,
                  hipStream_t s);
+// The phase and, with snap, the periodic ghost fill (mode 0) with its
+// slab-plane snapshots in the same launch (the same bits as launch_rbgs
+// followed by launch_bc_fill_snap).
+template <typename T>
+void launch_rbgs_fill(const LevelDims& L, const OpView<T>& op, Pair<T> x,
+                      Pair<const T> b, int colour, const Region& rg,
+                      int level0, const BcData* bcval, T* const (*snap)[2],
+                      hipStream_t s);
 // Two consecutive red-black phases (colour c1, then 1 - c1) over rg in one
 // launch: c1 updated in place in xR, the other colour read from xBin and
 // written to xBout (a second buffer; cells outside rg copied).  Not

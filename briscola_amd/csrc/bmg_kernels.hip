@@ -1926,12 +1926,15 @@ __device__ __forceinline__ void bc_fill_body(const Bid bid, const LevelDims& L,
                                              Pair<T> x, int mode,
                                              int level0, const BcVals& bv,
                                              const BcGroups& gr,
-                                             const SnapPtrs<T>& sn)
+                                             const SnapPtrs<T>& sn,
+                                             int tid = -1)
 {
+    // tid: the thread's index among the 256 of the workgroup (threadIdx.x
+    // by default; the merged phase-and-fill launch passes its own)
     const int b0 = bid.x*256;
     int g = 0;
     while (g < 6 && b0 >= gr.start[g + 1]) g++;
-    const int t = b0 + int(threadIdx.x) - gr.start[g];
+    const int t = b0 + (tid >= 0 ? tid : int(threadIdx.x)) - gr.start[g];
     if (t >= gr.count[g]) return;
     if (g >= 4)
     {
@@ -2037,6 +2040,26 @@ struct KBcFill
                            a.x, a.mode, a.level0, a.bv, a.gr, a.sn);
     }
 };
+
+// A red-black phase over the interior region and the periodic ghost fill
+// with slab-plane snapshots in one launch (workgroups [0, nph) the phase,
+// the rest the fill).  The fill only reads boundary-slab cells and writes
+// periodic ghosts and snapshots; the interior phase writes none of these
+// and reads no ghost or snapshot, so the two halves are independent and
+// give the bits of the two launches in sequence.
+template <typename T, int OP>
+__global__ __launch_bounds__(BX*BY) void k_rbgs_fill(
+    typename KRbgs<T, OP>::Args ph, typename KBcFill<T>::Args fl, unsigned nph)
+{
+    if (blockIdx.x < nph)
+        rbgs_body<T, OP>(Bid{blockIdx.x, 0, 0}, ph.L, ph.op, ph.xc, ph.xo,
+                         ph.bcl, ph.colour, ph.rg, ph.tl);
+    else
+        bc_fill_body<T>(Bid{blockIdx.x - nph, 0, 0}, fl.L, fl.op, fl.x,
+                        fl.mode, fl.level0, fl.bv, fl.gr, fl.sn,
+                        int(threadIdx.x + BX*threadIdx.y));
+}
+
 
 
 template <typename T>
@@ -4498,53 +4521,7 @@ template <typename T>
 void launch_rbgs(const LevelDims& L, const OpView<T>& op, Pair<T> x,
                  Pair<const T> b, int colour, const Region& rg, hipStream_t s)
 {
-    if (rg.E[0] <= rg.S[0] || rg.E[1] <= rg.S[1] || rg.E[2] <= rg.S[2]) return;
-    // a row of the region holds at most (nK + 1)/2 cells of one colour,
-    // counted from the first of them (pmin in rbgs_cell)
-    const int nK = rg.E[2] - rg.S[2];
-    const int ws = row_pack_shift((nK + 1)/2);
-    // FP32 phases on large levels: 4 (or 2) rows per thread, so that a
-    // wavefront has more loads in flight (spheres 256^3 level-0 FP32 phase
-    // 0.66 -> 0.79 of HBM peak, matrix-free channel 0.37 -> 0.52, measured),
-    // while the launch keeps at least 1024 (256) workgroups.  FP64 phases
-    // measured no faster with more rows per thread: one.
-    auto tiles_at = [&](int r)
-    {
-        const int rows = BY*(64 >> ws)*r;
-        Tiles t = make_tiles(((nK + 1)/2 + (1 << ws) - 1) >> ws,
-                             (rg.E[1] - rg.S[1] + rows - 1)/rows,
-                             rg.E[0] - rg.S[0]);
-        t.ws = ws;
-        t.rpt = r;
-        return t;
-    };
-    Tiles tl = tiles_at(1);
-    if (std::is_same<T, float>::value)
-    {
-        if (tiles_count(tiles_at(4)) >= 1024)
-            tl = tiles_at(4);
-        else if (tiles_count(tiles_at(2)) >= 256)
-            tl = tiles_at(2);
-    }
-    const dim3 g = tiles_grid(tl);
-    T* xc = x.c[colour];
-    const T* xo = x.c[colour ^ 1];
-    const T* bc = b.c[colour];
-    switch (op.kind)
-    {
-    case OP_STENCIL:
-        dispatch<KRbgs<T, OP_STENCIL>>(g, 0, s,
-                                       {L, op, xc, xo, bc, colour, rg, tl});
-        break;
-    case OP_CONST:
-        dispatch<KRbgs<T, OP_CONST>>(g, 0, s,
-                                     {L, op, xc, xo, bc, colour, rg, tl});
-        break;
-    default:
-        dispatch<KRbgs<T, OP_FACES>>(g, 0, s,
-                                     {L, op, xc, xo, bc, colour, rg, tl});
-        break;
-    }
+    launch_rbgs_fill<T>(L, op, x, b, colour, rg, 0, nullptr, nullptr, s);
 }
 
 // Tiles of the fused two-phase launch: TJ rows as large as an LDS ring of
@@ -4857,6 +4834,97 @@ void launch_bc_fill_snap(const LevelDims& L, const OpView<T>& op, Pair<T> x,
     if (total == 0) return;
     dispatch<KBcFill<T>>(dim3(total/256), 0, s, a);
 }
+
+
+// Tiles of a red-black phase over region rg.
+template <typename T>
+Tiles rbgs_tiles(const Region& rg)
+{
+    // a row of the region holds at most (nK + 1)/2 cells of one colour,
+    // counted from the first of them (pmin in rbgs_cell)
+    const int nK = rg.E[2] - rg.S[2];
+    const int ws = row_pack_shift((nK + 1)/2);
+    // FP32 phases on large levels: 4 (or 2) rows per thread, so that a
+    // wavefront has more loads in flight (spheres 256^3 level-0 FP32 phase
+    // 0.66 -> 0.79 of HBM peak, matrix-free channel 0.37 -> 0.52, measured),
+    // while the launch keeps at least 1024 (256) workgroups.  FP64 phases
+    // measured no faster with more rows per thread: one.
+    auto tiles_at = [&](int r)
+    {
+        const int rows = BY*(64 >> ws)*r;
+        Tiles t = make_tiles(((nK + 1)/2 + (1 << ws) - 1) >> ws,
+                             (rg.E[1] - rg.S[1] + rows - 1)/rows,
+                             rg.E[0] - rg.S[0]);
+        t.ws = ws;
+        t.rpt = r;
+        return t;
+    };
+    Tiles tl = tiles_at(1);
+    if (std::is_same<T, float>::value)
+    {
+        if (tiles_count(tiles_at(4)) >= 1024)
+            tl = tiles_at(4);
+        else if (tiles_count(tiles_at(2)) >= 256)
+            tl = tiles_at(2);
+    }
+    return tl;
+}
+
+template <typename T, int OP>
+void launch_rbgs_op(const typename KRbgs<T, OP>::Args& a, Pair<T> x,
+                    int level0, const BcData* bcval, T* const (*snap)[2],
+                    hipStream_t s)
+{
+    const dim3 g = tiles_grid(a.tl);
+    int total = 0;
+    typename KBcFill<T>::Args f{};
+    if (snap)
+        f = bc_fill_args<T>(a.L, a.op, x, 0, level0, bcval, snap, &total);
+    if (!snap || total == 0 || launch_recorder())
+    {
+        // recorded (batched) launches keep the phase and the fill apart
+        dispatch<KRbgs<T, OP>>(g, 0, s, a);
+        if (snap && total > 0) dispatch<KBcFill<T>>(dim3(total/256), 0, s, f);
+        return;
+    }
+    hipLaunchKernelGGL((k_rbgs_fill<T, OP>), dim3(g.x + total/256),
+                       dim3(BX, BY), 0, s, a, f, g.x);
+}
+
+// The phase, and with snap the periodic ghost fill (mode 0) with its
+// slab-plane snapshots in the same launch.
+template <typename T>
+void launch_rbgs_fill(const LevelDims& L, const OpView<T>& op, Pair<T> x,
+                      Pair<const T> b, int colour, const Region& rg,
+                      int level0, const BcData* bcval, T* const (*snap)[2],
+                      hipStream_t s)
+{
+    if (rg.E[0] <= rg.S[0] || rg.E[1] <= rg.S[1] || rg.E[2] <= rg.S[2])
+    {
+        if (snap) launch_bc_fill_snap<T>(L, op, x, 0, level0, bcval, snap, s);
+        return;
+    }
+    const Tiles tl = rbgs_tiles<T>(rg);
+    T* xc = x.c[colour];
+    const T* xo = x.c[colour ^ 1];
+    const T* bc = b.c[colour];
+    switch (op.kind)
+    {
+    case OP_STENCIL:
+        launch_rbgs_op<T, OP_STENCIL>({L, op, xc, xo, bc, colour, rg, tl},
+                                      x, level0, bcval, snap, s);
+        break;
+    case OP_CONST:
+        launch_rbgs_op<T, OP_CONST>({L, op, xc, xo, bc, colour, rg, tl},
+                                    x, level0, bcval, snap, s);
+        break;
+    default:
+        launch_rbgs_op<T, OP_FACES>({L, op, xc, xo, bc, colour, rg, tl},
+                                    x, level0, bcval, snap, s);
+        break;
+    }
+}
+
 
 template <typename T>
 void launch_bc_fill(const LevelDims& L, const OpView<T>& op, Pair<T> x,
@@ -5334,6 +5402,10 @@ void launch_set_cell(Pair<T> f, long idx, int colour, T v, hipStream_t s)
     template void launch_rbgs<T>(const LevelDims&, const OpView<T>&, Pair<T>, \
                                  Pair<const T>, int, const Region&,           \
                                  hipStream_t);                                \
+    template void launch_rbgs_fill<T>(const LevelDims&, const OpView<T>&,     \
+                                      Pair<T>, Pair<const T>, int,            \
+                                      const Region&, int, const BcData*,      \
+                                      T* const (*)[2], hipStream_t);          \
     template void launch_rbgs2<T>(const LevelDims&, const OpView<T>&, T*,     \
                                   const T*, T*, const T*, const T*, int,      \
                                   const Region&, hipStream_t);                \
