@@ -2883,8 +2883,11 @@ __device__ __forceinline__ void coarse_inv_body(const Bid, const LevelDims& L,
             }
         }
     }
+    // inv is 64 x 64 with zeros beyond n, v zero beyond n: a fixed trip
+    // count lets the loads of 16 columns be issued together
     double y = 0.0;
-    for (int j = 0; j < n; j++)
+#pragma unroll 16
+    for (int j = 0; j < 64; j++)
         y += inv[j*64 + lane]*__shfl(v, j, 64);
     if (lane < n)
     {

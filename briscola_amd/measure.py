@@ -15,6 +15,10 @@ calibration kernel: 128 MiB read, 128 MiB written).  Operations:
   prolong   prolongation-add k_prolong from 128^3 into 256^3 (2 n w + n/8 w);
   residual  level-0 residual k_residual (FP64 with the fused norm partials);
   restrict  restriction k_restrict of the level-0 residual to 128^3.
+
+``python3 -m briscola_amd.measure unet [n]`` instead runs 5 forward passes
+of the learned preconditioner's U-Net (random weights) on an n^3 input
+(default 256), for rocprofv3 --kernel-trace --stats.
 """
 import sys
 
@@ -45,5 +49,30 @@ def main(workload="spheres", op="phase", precision="fp64", n=256, reps=5):
     return 0
 
 
+def unet_forward(n=256, reps=5):
+    import torch
+    from briscola_amd import unet
+    n = int(n)
+    net = unet.Net(seed=1)
+    g = torch.Generator(device="cuda:0").manual_seed(0)
+    f = torch.randn((n, n, n), device="cuda:0", generator=g)
+    c = torch.randn((n, n, n), device="cuda:0", generator=g)
+    net.forward(f, c)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        net.forward(f, c)
+    e1.record()
+    torch.cuda.synchronize()
+    print(f"U-Net forward {n}^3: {e0.elapsed_time(e1) / reps:.3f} ms",
+          flush=True)
+    net.close()
+    return 0
+
+
 if __name__ == "__main__":
+    if len(sys.argv) > 1 and sys.argv[1] == "unet":
+        sys.exit(unet_forward(*sys.argv[2:3]))
     sys.exit(main(*sys.argv[1:4]))
