@@ -439,14 +439,31 @@ void bmg_ctx::restrictTo(int l)
 }
 
 template <typename T>
-void bmg_ctx::prolongFrom(int l)
+void bmg_ctx::prolongFrom(int l, int skipColour)
 {
     // the coarse ghosts are used, so correct them first
     // (linearProlongationScheme.C:128)
     launch_bc_fill(lv[l + 1].L, op<T>(l + 1), X<T>(l + 1), 1, (l + 1) == 0,
                    &bcval, stream);
     launch_prolong(lv[l].L, lv[l + 1].L, transfer<T>(l), cst(X<T>(l + 1)),
-                   X<T>(l), stream);
+                   X<T>(l), stream, skipColour);
+}
+
+// Colour of the first red-black phase of level l's post-smoothing when that
+// phase rewrites every interior cell of its colour -- no periodic faces, or
+// the wrapped phases over the whole interior -- so that the prolongation can
+// leave those cells alone (the phase reads only the other colour); -1
+// otherwise.
+template <typename T>
+int bmg_ctx::postPhaseColour(int l) const
+{
+    if (opts.n_sweeps_post <= 0 || opts.smoother != BMG_RBGS ||
+        fusedLevel<T>(l))
+        return -1;
+    const int wrap = wrapAxes<T>(l);
+    if (anyPeriodic() && !wrap) return -1;
+    int cols[4];
+    return phase_colours(interior(l, wrap ? 0 : 1), cols) > 0 ? cols[0] : -1;
 }
 
 template <typename T>
@@ -530,7 +547,7 @@ void bmg_ctx::cycle(std::vector<int>& visits, int l, bool zeroX)
             restrictTo<T>(l);
             cycle<T>(visits, l + 1, true);
             record_tag(cycleTag(lc - l, true));
-            prolongFrom<T>(l);
+            prolongFrom<T>(l, postPhaseColour<T>(l));
             if (opts.n_sweeps_post > 0)
                 smooth<T>(l, opts.n_sweeps_post);
         }
@@ -3148,7 +3165,13 @@ static double cycleBytes(const bmg_ctx* c, int l, size_t w)
         bytes += n*double(w) + nc*double(w);
         bytes += cycleBytes(c, l + 1, w);
         bytes += 2.0*ghostCells(qc)*double(w);
-        bytes += 2.0*n*double(w) + nc*double(w);
+        // prolongation: the fine cells read and written, half of them when
+        // the first post-smoothing phase rewrites the other colour
+        const bool by2 = q.L.N[0] == 2*qc.L.N[0] && q.L.N[1] == 2*qc.L.N[1] &&
+                         q.L.N[2] == 2*qc.L.N[2];
+        const int skip = w == sizeof(float) ? c->postPhaseColour<float>(l)
+                                            : c->postPhaseColour<double>(l);
+        bytes += (by2 && skip >= 0 ? 1.0 : 2.0)*n*double(w) + nc*double(w);
         if (c->opts.n_sweeps_post > 0)
             bytes += smoothBytes(c, l, c->opts.n_sweeps_post, w);
     }

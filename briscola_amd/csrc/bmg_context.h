@@ -416,7 +416,9 @@ struct bmg_ctx
     template <typename T>
     void restrictTo(int l);
     template <typename T>
-    void prolongFrom(int l);
+    void prolongFrom(int l, int skipColour = -1);
+    template <typename T>
+    int postPhaseColour(int l) const;
     template <typename T>
     void coarseSolve();
     // zeroX: x of level l starts at 0 (every level below the finest; the

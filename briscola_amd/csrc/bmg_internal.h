@@ -211,9 +211,12 @@ void launch_restrict(const LevelDims& Lf, const LevelDims& Lc,
                      const TransferView<T>& tv, Pair<const T> rf, Pair<T> bc,
                      hipStream_t s);
 template <typename T>
+// skipColour (levels refined by 2 on every axis): the fine cells of that
+// colour are left unchanged, for a caller whose next red-black phase
+// overwrites every cell of it; -1 updates all
 void launch_prolong(const LevelDims& Lf, const LevelDims& Lc,
                     const TransferView<T>& tv, Pair<const T> xc, Pair<T> xf,
-                    hipStream_t s);
+                    hipStream_t s, int skipColour = -1);
 // mode 0: periodic faces only (evaluate<nonEliminatedBcs,1>);
 // mode 1: full correctBoundaryConditions; mode 2: Neumann/Dirichlet faces
 // only (correct<eliminatedBcs>); level0 enables the inhomogeneous terms.

@@ -1524,7 +1524,7 @@ __device__ __forceinline__ void prolong8_impl(const Bid bid,
                                               const TransferView<T>& tv,
                                               Pair<const T> xc,
                                               Pair<T> xf,
-                                              const Tiles& tl)
+                                              const Tiles& tl, int skip)
 {
     int bx, by, bz;
     if (!tile_of(tl, bid.x, bx, by, bz)) return;
@@ -1579,7 +1579,10 @@ __device__ __forceinline__ void prolong8_impl(const Bid bid,
         T* xa = col(xf, cB ^ 1);
         T* xb = col(xf, cB);
         const unsigned ia = row + unsigned(ck - 1), ib = row + unsigned(ck);
-        const T old[2] = {ld(xa, ia), ld(xb, ib)};
+        // children of colour `skip` are left as they are (the caller's next
+        // phase overwrites every cell of that colour)
+        const bool doA = (cB ^ 1) != skip, doB = cB != skip;
+        const T old[2] = {doA ? ld(xa, ia) : T(0), doB ? ld(xb, ib) : T(0)};
         // parent (abc 0) and neighbour (abc 1) positions in the block:
         // parent 1, neighbour 0 (offset 0) or 2 (offset 1)
         const int sx[2] = {1, a ? 2 : 0};
@@ -1634,8 +1637,8 @@ __device__ __forceinline__ void prolong8_impl(const Bid bid,
             }
             add[c] = value;
         }
-        st(xa, ia, old[0] + add[0]);
-        st(xb, ib, old[1] + add[1]);
+        if (doA) st(xa, ia, old[0] + add[0]);
+        if (doB) st(xb, ib, old[1] + add[1]);
     }
 }
 
@@ -1643,13 +1646,14 @@ template <typename T>
 __global__ __launch_bounds__(BX*BY) void k_prolong8(LevelDims Lf, LevelDims Lc,
                                                     TransferView<T> tv,
                                                     Pair<const T> xc,
-                                                    Pair<T> xf, Tiles tl)
+                                                    Pair<T> xf, Tiles tl,
+                                                    int skip)
 {
     const Bid bid{blockIdx.x, blockIdx.y, blockIdx.z};
     if (tl.ws != 6)
-        prolong8_impl<T, true>(bid, Lf, Lc, tv, xc, xf, tl);
+        prolong8_impl<T, true>(bid, Lf, Lc, tv, xc, xf, tl, skip);
     else
-        prolong8_impl<T, false>(bid, Lf, Lc, tv, xc, xf, tl);
+        prolong8_impl<T, false>(bid, Lf, Lc, tv, xc, xf, tl, skip);
 }
 
 template <typename T>
@@ -1665,13 +1669,16 @@ struct KProlong8
         Pair<const T> xc;
         Pair<T> xf;
         Tiles tl;
+        int skip;
     };
     __device__ static void run(const Bid bid, const Args& a)
     {
         if (a.tl.ws != 6)
-            prolong8_impl<T, true>(bid, a.Lf, a.Lc, a.tv, a.xc, a.xf, a.tl);
+            prolong8_impl<T, true>(bid, a.Lf, a.Lc, a.tv, a.xc, a.xf, a.tl,
+                                   a.skip);
         else
-            prolong8_impl<T, false>(bid, a.Lf, a.Lc, a.tv, a.xc, a.xf, a.tl);
+            prolong8_impl<T, false>(bid, a.Lf, a.Lc, a.tv, a.xc, a.xf, a.tl,
+                                    a.skip);
     }
     static bool retile(Args& a, int ws)
     {
@@ -1681,7 +1688,7 @@ struct KProlong8
     static void launch(dim3 g, size_t shm, hipStream_t s, const Args& a)
     {
         hipLaunchKernelGGL((k_prolong8<T>), g, dim3(bx, by), shm, s, a.Lf,
-                           a.Lc, a.tv, a.xc, a.xf, a.tl);
+                           a.Lc, a.tv, a.xc, a.xf, a.tl, a.skip);
     }
 };
 
@@ -4767,7 +4774,7 @@ void launch_restrict(const LevelDims& Lf, const LevelDims& Lc,
 template <typename T>
 void launch_prolong(const LevelDims& Lf, const LevelDims& Lc,
                     const TransferView<T>& tv, Pair<const T> xc, Pair<T> xf,
-                    hipStream_t s)
+                    hipStream_t s, int skipColour)
 {
     if (Lf.N[0] == 2*Lc.N[0] && Lf.N[1] == 2*Lc.N[1] && Lf.N[2] == 2*Lc.N[2])
     {
@@ -4777,7 +4784,8 @@ void launch_prolong(const LevelDims& Lf, const LevelDims& Lc,
         Tiles tl = make_tiles((Lc.N[2] + (1 << ws) - 1) >> ws,
                               (Lc.N[1] + rows - 1)/rows, Lc.N[0]);
         tl.ws = ws;
-        dispatch<KProlong8<T>>(tiles_grid(tl), 0, s, {Lf, Lc, tv, xc, xf, tl});
+        dispatch<KProlong8<T>>(tiles_grid(tl), 0, s,
+                               {Lf, Lc, tv, xc, xf, tl, skipColour});
         return;
     }
     const Tiles tl = pair_tiles_packed(Lf);
@@ -5431,7 +5439,7 @@ void launch_set_cell(Pair<T> f, long idx, int colour, T v, hipStream_t s)
                                      Pair<T>, hipStream_t);                   \
     template void launch_prolong<T>(const LevelDims&, const LevelDims&,       \
                                     const TransferView<T>&, Pair<const T>,    \
-                                    Pair<T>, hipStream_t);                    \
+                                    Pair<T>, hipStream_t, int);               \
     template void launch_bc_fill<T>(const LevelDims&, const OpView<T>&,       \
                                     Pair<T>, int, int, const BcData*,         \
                                     hipStream_t);                             \
