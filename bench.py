@@ -27,9 +27,11 @@ again to the tolerance at which each one's error falls below 1e-8, and
 ``seeds`` times both on seeds 1-4 of the generator and reports the medians
 over seeds 0-4.
 
-The roofline object covers the dominant kernel of the plain FP64 solve, the
-level-0 red-black smoother phase, timed live with HIP events on the library's
-stream right after the timed steps.  The CPU baseline times the CPU
+The roofline object covers the dominant kernel of the fastest method, the
+level-0 red-black smoother phase of its precision (the FP32 phase of the mixed
+cycles for the mixed methods), timed live with HIP events on the library's
+stream right after the timed steps of the MG solve in that precision;
+``roofline_fp64`` the same for the plain FP64 solve's phase.  The CPU baseline times the CPU
 restatement of the reference multigrid on a bounded sample of the same solve,
 on the host cores this process may use and on one core, and scales it by the
 iteration count.
@@ -77,7 +79,10 @@ WORKLOADS = {
               "coarse, relTol 1e-8"),
         kernel="k_rbgs<double, 2> (level-0 red-black phase, face-coefficient "
                "operator, 72 B per updated cell)",
-        traffic="rbgs_phase_spheres256_traffic.txt"),
+        traffic="rbgs_phase_spheres256_traffic.txt",
+        kernel_mixed="k_rbgs<float, 2> (level-0 red-black phase of the FP32 "
+                     "cycles, FP16 face coefficients, 24 B per updated cell)",
+        traffic_mixed="rbgs_phase_fp32_spheres256_traffic.txt"),
     "channel": dict(
         title="channel 256^3",
         desc=("channel {n}^3: periodic x,z, Neumann y, tanh beta=2, Fourier "
@@ -85,7 +90,11 @@ WORKLOADS = {
               "relTol 1e-8"),
         kernel="k_rbgs<double, 1> (level-0 red-black phase, interior region, "
                "matrix-free operator, 24 B per updated cell)",
-        traffic="rbgs_phase_channel256_traffic.txt"),
+        traffic="rbgs_phase_channel256_traffic.txt",
+        kernel_mixed="k_rbgs<float, 1> (level-0 red-black phase of the FP32 "
+                     "cycles, whole interior, matrix-free operator, 12 B per "
+                     "updated cell)",
+        traffic_mixed="rbgs_phase_fp32_channel256_traffic.txt"),
 }
 
 
@@ -529,8 +538,15 @@ def run_workload(name, args):
     out = {
         "workload": w["desc"].format(n=args.n),
         "best": best,
-        "roofline": roofline(results.get("fp64"), w["kernel"],
-                             traffic_from_profiles(w["traffic"])),
+        # the dominant kernel of the fastest method: the level-0 phase of
+        # its precision (FP32 cycles for the mixed methods)
+        "roofline": (roofline(results.get("mixed"), w["kernel_mixed"],
+                              traffic_from_profiles(w["traffic_mixed"]))
+                     if "mixed" in best and "mixed" in results else
+                     roofline(results.get("fp64"), w["kernel"],
+                              traffic_from_profiles(w["traffic"]))),
+        "roofline_fp64": roofline(results.get("fp64"), w["kernel"],
+                                  traffic_from_profiles(w["traffic"])),
         "vcycle": vcycle_roofline(results.get("fp64"), "fp64"),
         "vcycle_mixed": vcycle_roofline(results.get("mixed"), "mixed"),
         "level0_kernels": kernel_rooflines(results.get("fp64")),
@@ -825,6 +841,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": main_out["workload"], "method": best},
         "roofline": main_out["roofline"],
+        "roofline_fp64": main_out["roofline_fp64"],
         "vcycle": main_out["vcycle"],
         "vcycle_mixed": main_out["vcycle_mixed"],
         "level0_kernels": main_out["level0_kernels"],

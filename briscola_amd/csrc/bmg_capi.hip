@@ -970,9 +970,9 @@ void bmg_ctx::enqueueFCGIteration(int mode)
                                      : Pair<float>{{nullptr, nullptr}};
     const Pair<double> r0 = mode == 0 ? pr(q0.r)
                                       : Pair<double>{{nullptr, nullptr}};
-    launch_fcg_update(q0.L, dScalar, X, Rv, cst(Pv), cst(Qv), rf, r0,
-                      pr(q0.b), partials, stream);
-    launch_reduce_rows(partials, nb, 1, dScalar + 2, stream, hScalarDev + 2);
+    const int nbu = launch_fcg_update(q0.L, dScalar, X, Rv, cst(Pv), cst(Qv),
+                                      rf, r0, pr(q0.b), partials, stream);
+    launch_reduce_rows(partials, nbu, 1, dScalar + 2, stream, hScalarDev + 2);
     if (mode == 2)
         enqueueLearned(dScalar + 2, 0.0, pr(q0.x));
     else
@@ -980,15 +980,17 @@ void bmg_ctx::enqueueFCGIteration(int mode)
     record_tag(3000);
     if (mode == 1)
     {
-        launch_dot_zq<float>(q0.L, cpr(q0.xf), cst(Qv), partials, stream);
-        launch_reduce_rows(partials, nb, 1, dScalar + 3, stream,
+        const int nbz = launch_dot_zq<float>(q0.L, cpr(q0.xf), cst(Qv),
+                                             partials, stream);
+        launch_reduce_rows(partials, nbz, 1, dScalar + 3, stream,
                            hScalarDev + 3);
         launch_update_p<float>(q0.L, dScalar, cpr(q0.xf), Pv, stream);
     }
     else
     {
-        launch_dot_zq<double>(q0.L, cpr(q0.x), cst(Qv), partials, stream);
-        launch_reduce_rows(partials, nb, 1, dScalar + 3, stream,
+        const int nbz = launch_dot_zq<double>(q0.L, cpr(q0.x), cst(Qv),
+                                              partials, stream);
+        launch_reduce_rows(partials, nbz, 1, dScalar + 3, stream,
                            hScalarDev + 3);
         launch_update_p<double>(q0.L, dScalar, cpr(q0.x), Pv, stream);
     }

@@ -226,9 +226,9 @@ struct Tiles {
     int per;   // tiles per XCD
     int ws;    // log2 of the cells of a row a wavefront covers (6: one row
                // segment of 64 per wavefront; below 6, 64 >> ws rows)
-    int rpt = 1;   // red-black phases: rows per thread (1 or 2; with 2 a
-                   // tile covers twice the rows, thread rows J and J + BY
-                   // wavefront rows apart)
+    int rpt = 1;   // red-black phases: rows per thread (1, 2 or 4; a tile
+                   // covers rpt times the rows, thread rows BY wavefront
+                   // rows apart)
 };
 
 __device__ __forceinline__ bool tile_of(const Tiles& t, unsigned w, int& bx,
