@@ -138,11 +138,15 @@ struct ConvTile {
 // sets (AS = 2 when the tiles leave registers for it) so that dependent
 // MFMAs do not follow each other.
 template <int NT, int MT, int CIN>
-__global__ __launch_bounds__(256) void k_conv_fwd_lds(ConvArgs a, int vs)
+__global__ __launch_bounds__(256) void k_conv_fwd_lds(ConvArgs a)
 {
     using Tl = ConvTile<MT>;
     constexpr int AS = NT*MT <= 8 ? 2 : 1;
     constexpr int CH = CIN > 0 ? CIN : 32;
+    // LDS voxel stride in elements (padded by 8: 16-byte reads of 16
+    // consecutive voxels spread over the banks), known at compile time so
+    // that every tap offset is a constant
+    constexpr int vs = CH + 8;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     bf16* lds = reinterpret_cast<bf16*>(lds_raw);
     const int lane = threadIdx.x & 63;
@@ -660,7 +664,7 @@ void launch_conv_lds(const ConvArgs& a, hipStream_t s)
                         unsigned((a.g.H + Tl::TY - 1)/Tl::TY)*
                         unsigned((a.g.D + Tl::TZ - 1)/Tl::TZ);
     hipLaunchKernelGGL((k_conv_fwd_lds<NT, MT, CIN>), dim3(nb), dim3(256), shm,
-                       s, a, vs);
+                       s, a);
 }
 
 template <int NT, int CIN>
