@@ -257,7 +257,7 @@ void bmg_ctx::ensureSmootherBuffers(bool mixed)
 // values.  The boundary slabs of each periodic face pair run as one
 // temporally blocked launch (k_rbgs_slabs).
 template <typename T>
-void bmg_ctx::smooth(int l, int sweeps, bool firstDone)
+void bmg_ctx::smooth(int l, int sweeps, bool firstDone, bool firstZero)
 {
     Level& q = lv[l];
     const OpView<T> o = op<T>(l);
@@ -309,7 +309,9 @@ void bmg_ctx::smooth(int l, int sweeps, bool firstDone)
         // on entry unless launch_zero_phase left them so
         Region rg = interior(l, 0);
         rg.wrap = wrap;
-        if (!firstDone)
+        // with firstZero the first phase reads no neighbour and writes the
+        // ghosts of its own colour, which is all the second one reads
+        if (!firstDone && !firstZero)
             launch_bc_fill(q.L, o, x, 0, l == 0, &bcval, stream);
         int prev = -1;
         for (int s = 0; s < sweeps; s++)
@@ -324,7 +326,11 @@ void bmg_ctx::smooth(int l, int sweeps, bool firstDone)
                     continue;
                 }
                 if (cols[i] != prev)
-                    launch_rbgs(q.L, o, x, b, cols[i], rg, stream);
+                {
+                    Region r = rg;
+                    r.zeroNb = s == 0 && i == 0 && firstZero;
+                    launch_rbgs(q.L, o, x, b, cols[i], r, stream);
+                }
                 prev = cols[i];
             }
         }
@@ -354,8 +360,12 @@ void bmg_ctx::smooth(int l, int sweeps, bool firstDone)
         for (int i = 0; i < nc; i++)
         {
             if (!done(i) && cols[i] != prevColour)
-                launch_rbgs_fill(q.L, o, x, b, cols[i], rg, l == 0, &bcval,
+            {
+                Region r = rg;
+                r.zeroNb = s == 0 && i == 0 && firstZero;
+                launch_rbgs_fill(q.L, o, x, b, cols[i], r, l == 0, &bcval,
                                  i == last ? sn : nullptr, stream);
+            }
             prevColour = cols[i];
         }
         if (!anyPeriodic()) continue;
@@ -454,6 +464,19 @@ void bmg_ctx::prolongFrom(int l, int skipColour)
 // the wrapped phases over the whole interior -- so that the prolongation can
 // leave those cells alone (the phase reads only the other colour); -1
 // otherwise.
+// The first sweep of red-black smoothing on level l covers every interior
+// cell with at least two phases -- no periodic faces, or the wrapped phases --
+// so the second phase rewrites every cell of the colour the first one reads.
+template <typename T>
+bool bmg_ctx::firstPhaseRewrites(int l) const
+{
+    if (opts.smoother != BMG_RBGS || fusedLevel<T>(l)) return false;
+    const int wrap = wrapAxes<T>(l);
+    if (anyPeriodic() && !wrap) return false;
+    int cols[4];
+    return phase_colours(interior(l, wrap ? 0 : 1), cols) >= 2;
+}
+
 template <typename T>
 int bmg_ctx::postPhaseColour(int l) const
 {
@@ -505,7 +528,7 @@ void bmg_ctx::cycle(std::vector<int>& visits, int l, bool zeroX)
                            (1 + 1 + 1) & 1, T(0), stream);
     const bool direct = opts.coarse_mode == BMG_COARSE_DIRECT;
     const bool pre = opts.n_sweeps_pre > 0 && !(l == lc && direct);
-    bool firstDone = false;
+    bool firstDone = false, firstZero = false;
     if (l > 0 || zeroX)
     {
         // x = 0 and, with red-black smoothing, the first phase from it in
@@ -515,8 +538,19 @@ void bmg_ctx::cycle(std::vector<int>& visits, int l, bool zeroX)
         // 62.6 us in one launch, whose linear element order reuses the
         // face coefficients less)
         const long cells = long(q.L.N[0])*q.L.N[1]*q.L.N[2];
-        if (pre && opts.smoother == BMG_RBGS && !fusedLevel<T>(l) &&
-            cells <= (1L << 18))
+        if (pre && std::is_same<T, float>::value && !launch_recorder() &&
+            firstPhaseRewrites<T>(l))
+        {
+            // FP32 cycles: no fill; the first phase takes its neighbours as
+            // 0 and the second rewrites every cell of the other colour
+            // before anything reads it (no bitwise contract here: ghost
+            // values behind eliminated faces, multiplied by 0, may differ
+            // in the sign of a zero); recorded (batched) cycles keep the
+            // fill
+            firstZero = true;
+        }
+        else if (pre && opts.smoother == BMG_RBGS && !fusedLevel<T>(l) &&
+                 cells <= (1L << 18))
         {
             const int wrap = wrapAxes<T>(l);
             Region rg = interior(l, wrap ? 0 : 1);
@@ -529,11 +563,11 @@ void bmg_ctx::cycle(std::vector<int>& visits, int l, bool zeroX)
                 firstDone = true;
             }
         }
-        if (!firstDone)
+        if (!firstDone && !firstZero)
             launch_fill<T>(X<T>(l), q.L.ncol, T(0), stream);
     }
     if (pre)
-        smooth<T>(l, opts.n_sweeps_pre, firstDone);
+        smooth<T>(l, opts.n_sweeps_pre, firstDone, firstZero);
     if (l < lc)
     {
         int reps = 1;

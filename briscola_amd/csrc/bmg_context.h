@@ -388,7 +388,10 @@ struct bmg_ctx
     }
 
     template <typename T>
-    void smooth(int l, int sweeps, bool firstDone = false);
+    void smooth(int l, int sweeps, bool firstDone = false,
+                bool firstZero = false);
+    template <typename T>
+    bool firstPhaseRewrites(int l) const;
     template <typename T>
     bool fusedLevel(int l) const;
     template <typename T>

@@ -80,6 +80,10 @@ struct Region {
     // cells (red-black phases of the FP32 cycles on levels with an even cell
     // count along every periodic axis); 0 everywhere else
     int wrap = 0;
+    // red-black phases: 1 takes every neighbour as 0 without reading it (the
+    // first phase from x = 0 of an FP32 cycle, whose other colour is never
+    // read before the next phase rewrites it)
+    int zeroNb = 0;
 };
 
 // Level-0 boundary data of the Neumann (gradient) and Dirichlet (value)

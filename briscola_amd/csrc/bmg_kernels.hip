@@ -417,8 +417,16 @@ __device__ __forceinline__ void rbgs_cell(const LevelDims& L,
     const int row = (I*L.P[1] + J)*L.Hz;
     const unsigned idx = unsigned(row + p);
     // issue every streamed load before the coefficient arithmetic
-    const T nb[6] = {ld(xo, idx - sI), ld(xo, idx + sI), ld(xo, idx - Hz),
-                     ld(xo, idx + Hz), ld(xo, idx - 1 + q), ld(xo, idx + q)};
+    T nb[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+    if (!rg.zeroNb)
+    {
+        nb[0] = ld(xo, idx - sI);
+        nb[1] = ld(xo, idx + sI);
+        nb[2] = ld(xo, idx - Hz);
+        nb[3] = ld(xo, idx + Hz);
+        nb[4] = ld(xo, idx - 1 + q);
+        nb[5] = ld(xo, idx + q);
+    }
     const T bv = ld(bcl, idx);
     T a[7];
     coeffs<T, OP>(L, op, colour, int(idx), row, p, q, I, J, K, a);
@@ -475,12 +483,20 @@ __device__ __forceinline__ void rbgs_cells(const LevelDims& L,
         row[c] = (I*L.P[1] + J[c])*L.Hz;
         idx[c] = unsigned(row[c] + p[c]);
         const unsigned i = idx[c];
-        nb[c][0] = ld(xo, i - sI);
-        nb[c][1] = ld(xo, i + sI);
-        nb[c][2] = ld(xo, i - Hz);
-        nb[c][3] = ld(xo, i + Hz);
-        nb[c][4] = ld(xo, i - 1 + q[c]);
-        nb[c][5] = ld(xo, i + q[c]);
+        if (rg.zeroNb)
+        {
+#pragma unroll
+            for (int e = 0; e < 6; e++) nb[c][e] = T(0);
+        }
+        else
+        {
+            nb[c][0] = ld(xo, i - sI);
+            nb[c][1] = ld(xo, i + sI);
+            nb[c][2] = ld(xo, i - Hz);
+            nb[c][3] = ld(xo, i + Hz);
+            nb[c][4] = ld(xo, i - 1 + q[c]);
+            nb[c][5] = ld(xo, i + q[c]);
+        }
         bv[c] = ld(bcl, i);
     }
     T a[R][7];
