@@ -459,11 +459,6 @@ void bmg_ctx::prolongFrom(int l, int skipColour)
                    X<T>(l), stream, skipColour);
 }
 
-// Colour of the first red-black phase of level l's post-smoothing when that
-// phase rewrites every interior cell of its colour -- no periodic faces, or
-// the wrapped phases over the whole interior -- so that the prolongation can
-// leave those cells alone (the phase reads only the other colour); -1
-// otherwise.
 // The first sweep of red-black smoothing on level l covers every interior
 // cell with at least two phases -- no periodic faces, or the wrapped phases --
 // so the second phase rewrites every cell of the colour the first one reads.
@@ -477,6 +472,11 @@ bool bmg_ctx::firstPhaseRewrites(int l) const
     return phase_colours(interior(l, wrap ? 0 : 1), cols) >= 2;
 }
 
+// Colour of the first red-black phase of level l's post-smoothing when that
+// phase rewrites every interior cell of its colour -- no periodic faces, or
+// the wrapped phases over the whole interior -- so that the prolongation can
+// leave those cells alone (the phase reads only the other colour); -1
+// otherwise.
 template <typename T>
 int bmg_ctx::postPhaseColour(int l) const
 {
@@ -538,15 +538,15 @@ void bmg_ctx::cycle(std::vector<int>& visits, int l, bool zeroX)
         // 62.6 us in one launch, whose linear element order reuses the
         // face coefficients less)
         const long cells = long(q.L.N[0])*q.L.N[1]*q.L.N[2];
-        if (pre && std::is_same<T, float>::value && !launch_recorder() &&
-            firstPhaseRewrites<T>(l))
+        if (pre && std::is_same<T, float>::value && firstPhaseRewrites<T>(l))
         {
             // FP32 cycles: no fill; the first phase takes its neighbours as
             // 0 and the second rewrites every cell of the other colour
-            // before anything reads it (no bitwise contract here: ghost
-            // values behind eliminated faces, multiplied by 0, may differ
-            // in the sign of a zero); recorded (batched) cycles keep the
-            // fill
+            // before anything reads it.  Ghosts behind eliminated faces keep
+            // what an earlier cycle left (finite: every solve start zeroes
+            // the FP32 fields, clearStaleCorrections) and are multiplied by
+            // 0, so results may differ from the filled form in the sign of
+            // a zero; no bitwise contract here
             firstZero = true;
         }
         else if (pre && opts.smoother == BMG_RBGS && !fusedLevel<T>(l) &&
@@ -1648,6 +1648,7 @@ void bmg_ctx::prepareFCG()
 void bmg_ctx::enqueueFCGStart()
 {
     enqueueBoundaryTerms();
+    clearStaleCorrections();
     Level& q0 = lv[0];
     const long n = q0.L.ncol;
     Pair<double> X{{kx[0], kx[1]}}, Bv{{kb[0], kb[1]}}, Rv{{kr[0], kr[1]}};
@@ -1774,9 +1775,22 @@ void bmg_ctx::enqueueBoundaryTerms()
     foldPending = false;
 }
 
+// The FP32 correction fields of every level zeroed, ghosts included.  The
+// FP32 cycles leave the ghosts behind eliminated faces as the previous cycle
+// left them (they are read only through zero couplings), so values that are
+// not finite -- a diverged solve, timing runs on uninitialised fields --
+// would otherwise survive into the next solve: 0 times NaN is NaN.
+void bmg_ctx::clearStaleCorrections()
+{
+    if (!mixedReady) return;
+    for (int l = 0; l < nLevels(); l++)
+        launch_fill<float>(pr(lv[l].xf), lv[l].L.ncol, 0.0f, stream);
+}
+
 void bmg_ctx::enqueueSolveStart()
 {
     enqueueBoundaryTerms();
+    clearStaleCorrections();
     Level& q0 = lv[0];
     launch_fill<double>(pr(q0.r), q0.L.ncol, 0.0, stream);
     int nb = 0;
@@ -3183,15 +3197,30 @@ static double smoothBytes(const bmg_ctx* c, int l, int sweeps, size_t w)
     return bytes;
 }
 
-static double cycleBytes(const bmg_ctx* c, int l, size_t w)
+static double cycleBytes(const bmg_ctx* c, int l, size_t w, bool zeroX)
 {
     const Level& q = c->lv[l];
     const int lc = c->coarseLevel();
     const double n = double(q.cells());
     const bool direct = c->opts.coarse_mode == BMG_COARSE_DIRECT;
+    const bool pre = c->opts.n_sweeps_pre > 0 && !(l == lc && direct);
     double bytes = 0.0;
-    if (l > 0) bytes += n*double(w);
-    if (c->opts.n_sweeps_pre > 0 && !(l == lc && direct))
+    if (l > 0 || zeroX)
+    {
+        // x = 0: the FP32 cycles skip the fill where the first sweep
+        // rewrites every cell, and that first phase reads no neighbour;
+        // elsewhere a fill (or the zeroing first phase, same bytes less the
+        // neighbour reads)
+        const bool fp32 = w == sizeof(float);
+        const bool skip = pre && fp32 && c->firstPhaseRewrites<float>(l);
+        const bool fused = fp32 ? c->fusedLevel<float>(l)
+                                : c->fusedLevel<double>(l);
+        const bool zp = pre && !skip && c->opts.smoother == BMG_RBGS &&
+                        !fused && n <= double(1L << 18);
+        if (!skip) bytes += n*double(w);
+        if (skip || zp) bytes -= 0.5*n*double(w);
+    }
+    if (pre)
         bytes += smoothBytes(c, l, c->opts.n_sweeps_pre, w);
     if (l < lc)
     {
@@ -3199,7 +3228,7 @@ static double cycleBytes(const bmg_ctx* c, int l, size_t w)
         const double nc = double(qc.cells());
         if (l > 0) bytes += n*cellBytesFull(q, w);
         bytes += n*double(w) + nc*double(w);
-        bytes += cycleBytes(c, l + 1, w);
+        bytes += cycleBytes(c, l + 1, w, true);
         bytes += 2.0*ghostCells(qc)*double(w);
         // prolongation: the fine cells read and written, half of them when
         // the first post-smoothing phase rewrites the other colour
@@ -3222,8 +3251,12 @@ static double iterationBytes(const bmg_ctx* c, bool mixed)
 {
     const Level& q0 = c->lv[0];
     const double n = double(q0.cells());
-    double bytes = cycleBytes(c, 0, mixed ? sizeof(float) : sizeof(double));
-    if (mixed) bytes += (12.0 + 4.0 + 20.0)*n;
+    // the FP32 cycle solves for a correction from 0 (its zeroing counted in
+    // cycleBytes); the FP64 cycle starts from the iterate
+    double bytes = cycleBytes(c, 0, mixed ? sizeof(float) : sizeof(double),
+                              mixed);
+    // r to FP32 (8 B read, 4 written), x += e (x read and written, e read)
+    if (mixed) bytes += (12.0 + 20.0)*n;
     bytes += n*cellBytesFull(q0, sizeof(double));
     return bytes;
 }

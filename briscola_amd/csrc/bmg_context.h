@@ -451,6 +451,8 @@ struct bmg_ctx
     // b[0] -= the level-0 boundary terms of eliminateGhosts, once per
     // assembled (or renewed) system
     void enqueueBoundaryTerms();
+    // the FP32 correction fields zeroed, ghosts included (solve starts)
+    void clearStaleCorrections();
     void enqueueSolveStart();
     void enqueueNormTerms(double y);
     void enqueueSolveEnd();
