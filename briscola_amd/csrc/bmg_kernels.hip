@@ -100,11 +100,13 @@ __device__ __forceinline__ void st(T* base, unsigned i, T v)
 // Stencil coefficients [C,L,R,B,T,A,F] of padded cell (I,J,K), as assembled
 // by linearGaussLaplacianScheme.C:61-94 (centre = 0 - v_xl - v_xu - v_yl -
 // v_yu - v_zl - v_zu, faces in loop order), before ghost elimination.
+// withDdt false leaves out the time term's diagonal (ddt_centre).
 template <typename T, int OP>
 __device__ __forceinline__ void raw_coeffs(const LevelDims& L,
                                            const OpView<T>& op, int colour,
                                            int idx, int row, int p, int q,
-                                           int I, int J, int K, T a[7])
+                                           int I, int J, int K, T a[7],
+                                           bool withDdt = true)
 {
     if constexpr (OP == OP_STENCIL)
     {
@@ -155,7 +157,7 @@ __device__ __forceinline__ void raw_coeffs(const LevelDims& L,
         c0 = c0 - a[6];
         // sys -= ddt(x)/dtFrac lowers the centre by the Euler diagonal
         // (defaultPoissonSolver.C:91-92, EulerDdtScheme.C:53)
-        if (op.dg[0]) c0 = c0 - ld(op.dg[colour], idx);
+        if (withDdt && op.dg[0]) c0 = c0 - ld(op.dg[colour], idx);
         a[0] = c0;
     }
 }
@@ -318,18 +320,32 @@ __device__ __forceinline__ void pair_Ax(const LevelDims& L,
     const T nB[6] = {ld(xa, i + 1 - sI), ld(xa, i + 1 + sI),
                      ld(xa, i + 1 - Hz), ld(xa, i + 1 + Hz), uA,
                      ld(xa, i + 1)};
-    T a[7];
-    coeffs<T, OP>(L, op, c.cA, int(i), c.row, c.p, 1, c.I, c.J, 2*c.p + 1, a);
+    // the coefficients of both cells, B's also where the pair has no cell B
+    // (odd N along z: B is then the ghost K = N+1, whose entries and those of
+    // its neighbours lie inside the padded arrays) and its product is
+    // dropped; the time term's diagonals of both are loaded under one
+    // branch after every other load, so no load waits for a branch
+    T a[7], b[7];
+    raw_coeffs<T, OP>(L, op, c.cA, int(i), c.row, c.p, 1, c.I, c.J, 2*c.p + 1,
+                      a, false);
+    raw_coeffs<T, OP>(L, op, c.cB, int(i) + 1, c.row, c.p + 1, 0, c.I, c.J,
+                      2*c.p + 2, b, false);
+    if constexpr (OP != OP_STENCIL)
+    {
+        if (op.dg[0])
+        {
+            const T dA = ld(op.dg[c.cA], i), dB = ld(op.dg[c.cB], i + 1);
+            a[0] = a[0] - dA;
+            b[0] = b[0] - dB;
+        }
+        eliminate_cell(L, c.I, c.J, 2*c.p + 1, a);
+        eliminate_cell(L, c.I, c.J, 2*c.p + 2, b);
+    }
     Ax[0] = a[0]*uA + a[1]*nA[0] + a[2]*nA[1] + a[3]*nA[2] + a[4]*nA[3]
           + a[5]*nA[4] + a[6]*nA[5];
-    Ax[1] = T(0);
-    if (c.vB)
-    {
-        coeffs<T, OP>(L, op, c.cB, int(i) + 1, c.row, c.p + 1, 0, c.I, c.J,
-                      2*c.p + 2, a);
-        Ax[1] = a[0]*uB + a[1]*nB[0] + a[2]*nB[1] + a[3]*nB[2] + a[4]*nB[3]
-              + a[5]*nB[4] + a[6]*nB[5];
-    }
+    const T axB = b[0]*uB + b[1]*nB[0] + b[2]*nB[1] + b[3]*nB[2] + b[4]*nB[3]
+                + b[5]*nB[4] + b[6]*nB[5];
+    Ax[1] = c.vB ? axB : T(0);
 }
 
 // The Gauss-Seidel update x = (b - P)/A0, P = A1 x_-i + A2 x_+i + A3 x_-j
@@ -1278,16 +1294,30 @@ __device__ __forceinline__ void restrict_impl(const Bid bid,
         rv[h][cz][cy][cx] = ld(col(rf, (fi[cx] + fj[cy] + fk) & 1),
                                unsigned(cidx(Lf, fi[cx], fj[cy], fk)));
     }
+    // every weight factor loaded up front as well: no load under a branch,
+    // so a term with w = 0 is dropped by a select (adding it would leave
+    // the sum as it is for finite values; the select also for the others)
+    T vf0[2], vf1[2], vf2[2][2], uz[2], cw2[2];
+#pragma unroll
+    for (int k = 0; k < 2; k++)
+    {
+        vf0[k] = tv.fw[0][fi[k]];
+        vf1[k] = tv.fw[1][fj[k]];
+        uz[k] = tv.ru[2][Kh[k]];
+        cw2[k] = tv.cw[2][Kh[k]];
+#pragma unroll
+        for (int cz = 0; cz < 2; cz++)
+            vf2[k][cz] = tv.fw[2][(Kh[k] - 1)*R2 + cz*oz + 1];
+    }
     const T ux = tv.ru[0][I], uy = tv.ru[1][J];
+    const T cw0 = tv.cw[0][I], cw1 = tv.cw[1][J];
     const T fx[2] = {T(1.0) - ux, ux};
     const T fy[2] = {T(1.0) - uy, uy};
     T out[2];
 #pragma unroll
     for (int h = 0; h < 2; h++)
     {
-        const int K = Kh[h];
-        const T uz = tv.ru[2][K];
-        const T fz[2] = {T(1.0) - uz, uz};
+        const T fz[2] = {T(1.0) - uz[h], uz[h]};
         T sum = T(0);
 #pragma unroll
         for (int cz = 0; cz < 2; cz++)
@@ -1299,19 +1329,16 @@ __device__ __forceinline__ void restrict_impl(const Bid bid,
             T w = fx[cx]*fy[cy];
             w = w*fz[cz];
             if (w <= T(1e-12)) w = T(0);
-            if (w != T(0))
-            {
-                const int fk = (K - 1)*R2 + cz*oz + 1;
-                T cvf = tv.fw[0][fi[cx]]*tv.fw[1][fj[cy]];
-                cvf = cvf*tv.fw[2][fk];
-                cvf = cvf > T(1e-16) ? cvf : T(1e-16);
-                const T icvf = recip(cvf);
-                const T t = w*rv[h][cz][cy][cx];
-                sum = madd(t, icvf, sum);
-            }
+            T cvf = vf0[cx]*vf1[cy];
+            cvf = cvf*vf2[h][cz];
+            cvf = cvf > T(1e-16) ? cvf : T(1e-16);
+            const T icvf = recip(cvf);
+            const T t = w*rv[h][cz][cy][cx];
+            const T next = madd(t, icvf, sum);
+            sum = w != T(0) ? next : sum;
         }
-        T cvc = tv.cw[0][I]*tv.cw[1][J];
-        cvc = cvc*tv.cw[2][K];
+        T cvc = cw0*cw1;
+        cvc = cvc*cw2[h];
         cvc = cvc > T(1e-16) ? cvc : T(1e-16);
         out[h] = sum*cvc;
     }
@@ -1558,6 +1585,24 @@ __device__ __forceinline__ void prolong8_impl(const Bid bid,
         ck = 1 + bx*BX + int(threadIdx.x);
     }
     if (cj > Lc.N[1] || ck > Lc.N[2]) return;
+    // the children first: K = 2ck-1 (c = 0) and K = 2ck (c = 1) of the
+    // fine rows (I, J) = (2ci-1+a, 2cj-1+b); children of colour `skip` are
+    // left as they are (the caller's next phase overwrites every cell of
+    // that colour).  Their loads are issued before the coarse block's, so
+    // the memory latency of both overlaps.
+    T old[2][2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+    {
+        const int I = 2*ci - 1 + a, J = 2*cj - 1 + b;
+        const int cB = (I + J) & 1;
+        const unsigned row = unsigned((I*Lf.P[1] + J)*Lf.Hz);
+        old[a][b][0] = (cB ^ 1) != skip
+                           ? ld(col(xf, cB ^ 1), row + unsigned(ck - 1)) : T(0);
+        old[a][b][1] = cB != skip ? ld(col(xf, cB), row + unsigned(ck)) : T(0);
+    }
     // coarse block [x][y][z] = (ci-1+x, cj-1+y, ck-1+z)
     T cv[3][3][3];
 #pragma unroll
@@ -1589,16 +1634,12 @@ __device__ __forceinline__ void prolong8_impl(const Bid bid,
     for (int b = 0; b < 2; b++)
     {
         const int I = 2*ci - 1 + a, J = 2*cj - 1 + b;
-        // children K = 2ck-1 (c = 0) and K = 2ck (c = 1) of row (I, J)
         const int cB = (I + J) & 1;
         const unsigned row = unsigned((I*Lf.P[1] + J)*Lf.Hz);
         T* xa = col(xf, cB ^ 1);
         T* xb = col(xf, cB);
         const unsigned ia = row + unsigned(ck - 1), ib = row + unsigned(ck);
-        // children of colour `skip` are left as they are (the caller's next
-        // phase overwrites every cell of that colour)
         const bool doA = (cB ^ 1) != skip, doB = cB != skip;
-        const T old[2] = {doA ? ld(xa, ia) : T(0), doB ? ld(xb, ib) : T(0)};
         // parent (abc 0) and neighbour (abc 1) positions in the block:
         // parent 1, neighbour 0 (offset 0) or 2 (offset 1)
         const int sx[2] = {1, a ? 2 : 0};
@@ -1653,8 +1694,8 @@ __device__ __forceinline__ void prolong8_impl(const Bid bid,
             }
             add[c] = value;
         }
-        if (doA) st(xa, ia, old[0] + add[0]);
-        if (doB) st(xb, ib, old[1] + add[1]);
+        if (doA) st(xa, ia, old[a][b][0] + add[0]);
+        if (doB) st(xb, ib, old[a][b][1] + add[1]);
     }
 }
 
@@ -3149,23 +3190,44 @@ struct KNfTerms
 };
 
 
+// Sum of row[0, n) by one workgroup of 1024 threads, in a fixed order:
+// thread t adds row[t + 1024 m] for m = k mod 16 into accumulator k, 16
+// loads in flight per thread (a row of 32768 partials takes two rounds of
+// memory latency instead of eight), then a fixed tree over the 16
+// accumulators and block_sum over the threads.
+__device__ __forceinline__ double row_sum_1024(const double* row, int n,
+                                               double* sm)
+{
+    constexpr int U = 16;
+    double a[U];
+#pragma unroll
+    for (int k = 0; k < U; k++) a[k] = 0.0;
+    int i = threadIdx.x;
+    for (; i + (U - 1)*1024 < n; i += U*1024)
+    {
+        double v[U];
+#pragma unroll
+        for (int k = 0; k < U; k++) v[k] = row[i + k*1024];
+#pragma unroll
+        for (int k = 0; k < U; k++) a[k] += v[k];
+    }
+#pragma unroll
+    for (int k = 0; k < U - 1; k++)
+        if (i + k*1024 < n) a[k] += row[i + k*1024];
+#pragma unroll
+    for (int w = U/2; w > 0; w >>= 1)
+#pragma unroll
+        for (int k = 0; k < w; k++) a[k] += a[k + w];
+    return block_sum(a[0], sm);
+}
+
 __device__ __forceinline__ void reduce_final_body(const Bid bid,
                                                   const double* partials,
                                                   int n, double* out,
                                                   double* hostOut)
 {
     __shared__ double sm[16];
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    int i = threadIdx.x;
-    for (; i + 3*1024 < n; i += 4*1024)
-    {
-        a0 += partials[i];
-        a1 += partials[i + 1024];
-        a2 += partials[i + 2*1024];
-        a3 += partials[i + 3*1024];
-    }
-    for (; i < n; i += 1024) a0 += partials[i];
-    const double s = block_sum((a0 + a1) + (a2 + a3), sm);
+    const double s = row_sum_1024(partials, n, sm);
     if (threadIdx.x == 0)
     {
         out[0] = s;
@@ -3417,13 +3479,19 @@ __device__ __forceinline__ void dot_zq_body(const Bid bid, const LevelDims& L,
     double acc = 0.0;
     if (pair_pos(L, bx, by, bz, c))
     {
+        // both cells' values loaded before any arithmetic (position idx+1
+        // lies inside the padded row also where the pair has no cell B)
+        double zv[2], qv[2];
         for (int h = 0; h < 2; h++)
         {
-            if (h == 1 && !c.vB) continue;
             const int k = h ? c.cB : c.cA;
             const unsigned i = c.idx + h;
-            acc += double(ld(col(z, k), i))*ld(col(q, k), i);
+            zv[h] = double(ld(col(z, k), i));
+            qv[h] = ld(col(q, k), i);
         }
+        acc += zv[0]*qv[0];
+        const double accB = acc + zv[1]*qv[1];
+        acc = c.vB ? accB : acc;
     }
     const double s = block_sum(acc, sm);
     if (threadIdx.x == 0 && threadIdx.y == 0)
@@ -3476,13 +3544,18 @@ __device__ __forceinline__ void update_p_body(const Bid bid,
     const double beta = -sc[3]/sc[0];
     PairPos c;
     if (!pair_pos(L, bx, by, bz, c)) return;
+    // both cells' values loaded before any store (position idx+1 lies
+    // inside the padded row also where the pair has no cell B)
+    double zv[2], pv[2];
     for (int h = 0; h < 2; h++)
     {
-        if (h == 1 && !c.vB) continue;
         const int k = h ? c.cB : c.cA;
         const unsigned i = c.idx + h;
-        st(col(p, k), i, double(ld(col(z, k), i)) + beta*ld(col(p, k), i));
+        zv[h] = double(ld(col(z, k), i));
+        pv[h] = ld(col(p, k), i);
     }
+    st(col(p, c.cA), c.idx, zv[0] + beta*pv[0]);
+    if (c.vB) st(col(p, c.cB), c.idx + 1, zv[1] + beta*pv[1]);
 }
 
 template <typename Z>
@@ -3599,18 +3672,7 @@ __device__ __forceinline__ void reduce_rows_body(const Bid bid,
                                                  double* hostOut)
 {
     __shared__ double sm[16];
-    const double* row = partials + size_t(bid.x)*n;
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    int i = threadIdx.x;
-    for (; i + 3*1024 < n; i += 4*1024)
-    {
-        a0 += row[i];
-        a1 += row[i + 1024];
-        a2 += row[i + 2*1024];
-        a3 += row[i + 3*1024];
-    }
-    for (; i < n; i += 1024) a0 += row[i];
-    const double s = block_sum((a0 + a1) + (a2 + a3), sm);
+    const double s = row_sum_1024(partials + size_t(bid.x)*n, n, sm);
     if (threadIdx.x == 0)
     {
         out[bid.x] = s;
