@@ -983,9 +983,10 @@ void bmg_ctx::enqueuePrecondition(bool mixed)
 }
 
 // One flexible-CG iteration with its scalars on the device (sc = dScalar):
-// q = A p with p.q, p.r; x += alpha p, r -= alpha q with r.r and r staged
-// for the preconditioner; z = M r; z.q; p = z + beta p; the four scalars
-// copied to the pinned host buffer.
+// q = A p with p.q, p.r; r -= alpha q with r.r and r staged for the
+// preconditioner; z = M r; z.q; x += alpha p and p = z + beta p in one pass
+// (x is complete again when the iteration ends); the four scalars copied to
+// the pinned host buffer.
 void bmg_ctx::enqueueFCGIteration(int mode)
 {
     Level& q0 = lv[0];
@@ -1004,8 +1005,8 @@ void bmg_ctx::enqueueFCGIteration(int mode)
                                      : Pair<float>{{nullptr, nullptr}};
     const Pair<double> r0 = mode == 0 ? pr(q0.r)
                                       : Pair<double>{{nullptr, nullptr}};
-    const int nbu = launch_fcg_update(q0.L, dScalar, X, Rv, cst(Pv), cst(Qv),
-                                      rf, r0, pr(q0.b), partials, stream);
+    const int nbu = launch_fcg_update(q0.L, dScalar, Rv, cst(Qv), rf, r0,
+                                      pr(q0.b), partials, stream);
     launch_reduce_rows(partials, nbu, 1, dScalar + 2, stream, hScalarDev + 2);
     if (mode == 2)
         enqueueLearned(dScalar + 2, 0.0, pr(q0.x));
@@ -1018,7 +1019,7 @@ void bmg_ctx::enqueueFCGIteration(int mode)
                                              partials, stream);
         launch_reduce_rows(partials, nbz, 1, dScalar + 3, stream,
                            hScalarDev + 3);
-        launch_update_p<float>(q0.L, dScalar, cpr(q0.xf), Pv, stream);
+        launch_update_p<float>(q0.L, dScalar, cpr(q0.xf), Pv, X, stream);
     }
     else
     {
@@ -1026,7 +1027,7 @@ void bmg_ctx::enqueueFCGIteration(int mode)
                                               partials, stream);
         launch_reduce_rows(partials, nbz, 1, dScalar + 3, stream,
                            hScalarDev + 3);
-        launch_update_p<double>(q0.L, dScalar, cpr(q0.x), Pv, stream);
+        launch_update_p<double>(q0.L, dScalar, cpr(q0.x), Pv, X, stream);
     }
     if (mode != 2 && deflating()) enqueueDeflate();
     HIPCK(hipGetLastError());

@@ -349,20 +349,21 @@ void launch_diag(const LevelDims& L, Pair<const double> a7, Pair<double> x,
 int launch_apply_dots(const LevelDims& L, const OpView<double>& op,
                       Pair<const double> p, Pair<double> q,
                       Pair<const double> r, double* partials, hipStream_t s);
-// x += alpha p, r -= alpha q (alpha = sc[1]/sc[0]), partials of r.r, and r
-// copied for the preconditioner: to FP32 rf when rf.c[0] is set, else to the
-// FP64 r0 and b0 when r0.c[0] is set, else nowhere.  Returns tiles.
-int launch_fcg_update(const LevelDims& L, const double* sc, Pair<double> x,
-                      Pair<double> r, Pair<const double> p,
+// r -= alpha q (alpha = sc[1]/sc[0]), partials of r.r, and r copied for the
+// preconditioner: to FP32 rf when rf.c[0] is set, else to the FP64 r0 and b0
+// when r0.c[0] is set, else nowhere.  Returns tiles.  (x += alpha p follows
+// in launch_update_p, which reads p anyway.)
+int launch_fcg_update(const LevelDims& L, const double* sc, Pair<double> r,
                       Pair<const double> q, Pair<float> rf, Pair<double> r0,
                       Pair<double> b0, double* partials, hipStream_t s);
 template <typename Z>
 int launch_dot_zq(const LevelDims& L, Pair<const Z> z, Pair<const double> q,
                   double* partials, hipStream_t s);
-// p = z + beta p on interior cells, beta = -sc[3]/sc[0]
+// x += alpha p and then p = z + beta p on interior cells, alpha =
+// sc[1]/sc[0] (the step of the direction p just used), beta = -sc[3]/sc[0]
 template <typename Z>
 void launch_update_p(const LevelDims& L, const double* sc, Pair<const Z> z,
-                     Pair<double> p, hipStream_t s);
+                     Pair<double> p, Pair<double> x, hipStream_t s);
 // level-0 interior <-> dense FP32 (reference cell order) for the learned
 // preconditioner: out = r / rms(r), z = in * rms / h, rms = sqrt(ss/cells)
 // with ss = *ssDev when ssDev is set, else ssHost

@@ -3365,9 +3365,7 @@ template <int STAGE>
 __device__ __forceinline__ void fcg_update_body(const Bid bid,
                                                 const LevelDims& L,
                                                 const double* sc,
-                                                Pair<double> x,
                                                 Pair<double> r,
-                                                Pair<const double> p,
                                                 Pair<const double> q,
                                                 Pair<float> rf,
                                                 Pair<double> r0,
@@ -3383,14 +3381,12 @@ __device__ __forceinline__ void fcg_update_body(const Bid bid,
     double acc = 0.0;
     if (pair_pos(L, bx, by, bz, c))
     {
-        double xv[2], rv[2], pv[2], qv[2];
+        double rv[2], qv[2];
         for (int h = 0; h < 2; h++)
         {
             const int k = h ? c.cB : c.cA;
             const unsigned i = c.idx + h;
-            xv[h] = ld(col(x, k), i);
             rv[h] = ld(col(r, k), i);
-            pv[h] = ld(col(p, k), i);
             qv[h] = ld(col(q, k), i);
         }
         for (int h = 0; h < 2; h++)
@@ -3398,7 +3394,6 @@ __device__ __forceinline__ void fcg_update_body(const Bid bid,
             if (h == 1 && !c.vB) continue;
             const int k = h ? c.cB : c.cA;
             const unsigned i = c.idx + h;
-            st(col(x, k), i, xv[h] + alpha*pv[h]);
             const double rn = rv[h] - alpha*qv[h];
             st(col(r, k), i, rn);
             if (STAGE == 1)
@@ -3419,9 +3414,7 @@ __device__ __forceinline__ void fcg_update_body(const Bid bid,
 template <int STAGE>
 __global__ __launch_bounds__(BX*BY) void k_fcg_update(LevelDims L,
                                                       const double* sc,
-                                                      Pair<double> x,
                                                       Pair<double> r,
-                                                      Pair<const double> p,
                                                       Pair<const double> q,
                                                       Pair<float> rf,
                                                       Pair<double> r0,
@@ -3430,7 +3423,7 @@ __global__ __launch_bounds__(BX*BY) void k_fcg_update(LevelDims L,
                                                       double* partials)
 {
     const Bid bid{blockIdx.x, blockIdx.y, blockIdx.z};
-    fcg_update_body<STAGE>(bid, L, sc, x, r, p, q, rf, r0, b0, tl, partials);
+    fcg_update_body<STAGE>(bid, L, sc, r, q, rf, r0, b0, tl, partials);
 }
 
 template <int STAGE>
@@ -3442,9 +3435,7 @@ struct KFcgUpdate
     {
         LevelDims L;
         const double* sc;
-        Pair<double> x;
         Pair<double> r;
-        Pair<const double> p;
         Pair<const double> q;
         Pair<float> rf;
         Pair<double> r0;
@@ -3454,14 +3445,13 @@ struct KFcgUpdate
     };
     __device__ static void run(const Bid bid, const Args& a)
     {
-        fcg_update_body<STAGE>(bid, a.L, a.sc, a.x, a.r, a.p, a.q, a.rf, a.r0,
-                               a.b0, a.tl, a.partials);
+        fcg_update_body<STAGE>(bid, a.L, a.sc, a.r, a.q, a.rf, a.r0, a.b0,
+                               a.tl, a.partials);
     }
     static void launch(dim3 g, size_t shm, hipStream_t s, const Args& a)
     {
         hipLaunchKernelGGL((k_fcg_update<STAGE>), g, dim3(bx, by), shm, s, a.L,
-                           a.sc, a.x, a.r, a.p, a.q, a.rf, a.r0, a.b0, a.tl,
-                           a.partials);
+                           a.sc, a.r, a.q, a.rf, a.r0, a.b0, a.tl, a.partials);
     }
 };
 
@@ -3537,35 +3527,44 @@ __device__ __forceinline__ void update_p_body(const Bid bid,
                                               const LevelDims& L,
                                               const double* sc,
                                               Pair<const Z> z,
-                                              Pair<double> p, const Tiles& tl)
+                                              Pair<double> p, Pair<double> x,
+                                              const Tiles& tl)
 {
     int bx, by, bz;
     if (!tile_of(tl, bid.x, bx, by, bz)) return;
+    const double alpha = sc[1]/sc[0];
     const double beta = -sc[3]/sc[0];
     PairPos c;
     if (!pair_pos(L, bx, by, bz, c)) return;
     // both cells' values loaded before any store (position idx+1 lies
     // inside the padded row also where the pair has no cell B)
-    double zv[2], pv[2];
+    double zv[2], pv[2], xv[2];
     for (int h = 0; h < 2; h++)
     {
         const int k = h ? c.cB : c.cA;
         const unsigned i = c.idx + h;
         zv[h] = double(ld(col(z, k), i));
         pv[h] = ld(col(p, k), i);
+        xv[h] = ld(col(x, k), i);
     }
+    st(col(x, c.cA), c.idx, xv[0] + alpha*pv[0]);
     st(col(p, c.cA), c.idx, zv[0] + beta*pv[0]);
-    if (c.vB) st(col(p, c.cB), c.idx + 1, zv[1] + beta*pv[1]);
+    if (c.vB)
+    {
+        st(col(x, c.cB), c.idx + 1, xv[1] + alpha*pv[1]);
+        st(col(p, c.cB), c.idx + 1, zv[1] + beta*pv[1]);
+    }
 }
 
 template <typename Z>
 __global__ __launch_bounds__(BX*BY) void k_update_p(LevelDims L,
                                                     const double* sc,
                                                     Pair<const Z> z,
-                                                    Pair<double> p, Tiles tl)
+                                                    Pair<double> p,
+                                                    Pair<double> x, Tiles tl)
 {
     const Bid bid{blockIdx.x, blockIdx.y, blockIdx.z};
-    update_p_body<Z>(bid, L, sc, z, p, tl);
+    update_p_body<Z>(bid, L, sc, z, p, x, tl);
 }
 
 template <typename Z>
@@ -3579,16 +3578,17 @@ struct KUpdateP
         const double* sc;
         Pair<const Z> z;
         Pair<double> p;
+        Pair<double> x;
         Tiles tl;
     };
     __device__ static void run(const Bid bid, const Args& a)
     {
-        update_p_body<Z>(bid, a.L, a.sc, a.z, a.p, a.tl);
+        update_p_body<Z>(bid, a.L, a.sc, a.z, a.p, a.x, a.tl);
     }
     static void launch(dim3 g, size_t shm, hipStream_t s, const Args& a)
     {
         hipLaunchKernelGGL((k_update_p<Z>), g, dim3(bx, by), shm, s, a.L, a.sc,
-                           a.z, a.p, a.tl);
+                           a.z, a.p, a.x, a.tl);
     }
 };
 
@@ -5297,22 +5297,18 @@ int launch_apply_dots(const LevelDims& L, const OpView<double>& op,
     return tiles_count(tl);
 }
 
-int launch_fcg_update(const LevelDims& L, const double* sc, Pair<double> x,
-                      Pair<double> r, Pair<const double> p,
+int launch_fcg_update(const LevelDims& L, const double* sc, Pair<double> r,
                       Pair<const double> q, Pair<float> rf, Pair<double> r0,
                       Pair<double> b0, double* partials, hipStream_t s)
 {
     const Tiles tl = pair_tiles(L);
     const dim3 g = tiles_grid(tl);
     if (rf.c[0])
-        dispatch<KFcgUpdate<1>>(g, 0, s,
-                                {L, sc, x, r, p, q, rf, r0, b0, tl, partials});
+        dispatch<KFcgUpdate<1>>(g, 0, s, {L, sc, r, q, rf, r0, b0, tl, partials});
     else if (r0.c[0])
-        dispatch<KFcgUpdate<0>>(g, 0, s,
-                                {L, sc, x, r, p, q, rf, r0, b0, tl, partials});
+        dispatch<KFcgUpdate<0>>(g, 0, s, {L, sc, r, q, rf, r0, b0, tl, partials});
     else
-        dispatch<KFcgUpdate<2>>(g, 0, s,
-                                {L, sc, x, r, p, q, rf, r0, b0, tl, partials});
+        dispatch<KFcgUpdate<2>>(g, 0, s, {L, sc, r, q, rf, r0, b0, tl, partials});
     return tiles_count(tl);
 }
 
@@ -5359,17 +5355,17 @@ template int launch_dot_zq<float>(const LevelDims&, Pair<const float>,
 
 template <typename Z>
 void launch_update_p(const LevelDims& L, const double* sc, Pair<const Z> z,
-                     Pair<double> p, hipStream_t s)
+                     Pair<double> p, Pair<double> x, hipStream_t s)
 {
     const Tiles tl = pair_tiles(L);
-    dispatch<KUpdateP<Z>>(tiles_grid(tl), 0, s, {L, sc, z, p, tl});
+    dispatch<KUpdateP<Z>>(tiles_grid(tl), 0, s, {L, sc, z, p, x, tl});
 }
 template void launch_update_p<double>(const LevelDims&, const double*,
                                       Pair<const double>, Pair<double>,
-                                      hipStream_t);
+                                      Pair<double>, hipStream_t);
 template void launch_update_p<float>(const LevelDims&, const double*,
                                      Pair<const float>, Pair<double>,
-                                     hipStream_t);
+                                     Pair<double>, hipStream_t);
 
 void launch_reduce_rows(const double* partials, int n, int nrows, double* out,
                         hipStream_t s, double* hostOut)
