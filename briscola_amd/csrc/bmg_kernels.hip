@@ -554,7 +554,10 @@ __device__ __forceinline__ void rbgs_body(const Bid bid, const LevelDims& L,
     int bx, by, bz;
     if (!tile_of(tl, bid.x, bx, by, bz)) return;
     const int I = rg.S[0] + bz;
-    if (tl.rpt > 1)
+    // several rows per thread in FP32 only (rbgs_tiles): the FP64 kernels
+    // are compiled without that path, which would set their register count
+    // (and so their occupancy) for every launch
+    if (std::is_same<T, float>::value && tl.rpt > 1)
     {
         // rpt rows per thread, BY wavefront rows apart
         const int ws = tl.ws;
