@@ -256,6 +256,34 @@ void bmg_ctx::ensureSmootherBuffers(bool mixed)
 // only the other colour, which did not change, so it would rewrite the same
 // values.  The boundary slabs of each periodic face pair run as one
 // temporally blocked launch (k_rbgs_slabs).
+// The red-black phases ph (colours in order) over region rg of level l, the
+// first from x = 0 without reading its neighbours when firstZero: on levels
+// of at most 16^3 cells one launch of one workgroup for all of them (the
+// same bits), else one launch each.  Recorded (batched) sequences keep one
+// launch each.
+template <typename T>
+void bmg_ctx::runPhases(int l, const Region& rg, const std::vector<int>& ph,
+                        bool firstZero)
+{
+    Level& q = lv[l];
+    const OpView<T> o = op<T>(l);
+    Pair<T> x = X<T>(l);
+    Pair<const T> b = cst(B<T>(l));
+    if (ph.empty()) return;
+    if (!launch_recorder() && q.cells() <= 4096 && ph.size() <= 8)
+    {
+        launch_rbgs_chain(q.L, o, x, b, rg, ph.data(), int(ph.size()),
+                          firstZero ? 1 : 0, stream);
+        return;
+    }
+    for (size_t i = 0; i < ph.size(); i++)
+    {
+        Region r = rg;
+        r.zeroNb = i == 0 && firstZero;
+        launch_rbgs(q.L, o, x, b, ph[i], r, stream);
+    }
+}
+
 template <typename T>
 void bmg_ctx::smooth(int l, int sweeps, bool firstDone, bool firstZero)
 {
@@ -313,6 +341,7 @@ void bmg_ctx::smooth(int l, int sweeps, bool firstDone, bool firstZero)
         // ghosts of its own colour, which is all the second one reads
         if (!firstDone && !firstZero)
             launch_bc_fill(q.L, o, x, 0, l == 0, &bcval, stream);
+        std::vector<int> ph;
         int prev = -1;
         for (int s = 0; s < sweeps; s++)
         {
@@ -325,15 +354,31 @@ void bmg_ctx::smooth(int l, int sweeps, bool firstDone, bool firstZero)
                     prev = cols[0];
                     continue;
                 }
-                if (cols[i] != prev)
-                {
-                    Region r = rg;
-                    r.zeroNb = s == 0 && i == 0 && firstZero;
-                    launch_rbgs(q.L, o, x, b, cols[i], r, stream);
-                }
+                if (cols[i] != prev) ph.push_back(cols[i]);
                 prev = cols[i];
             }
         }
+        runPhases<T>(l, rg, ph, firstZero);
+        return;
+    }
+    if (!anyPeriodic())
+    {
+        // no periodic face: the phases of every sweep over the interior
+        const Region rg = interior(l, 1);
+        std::vector<int> ph;
+        int prev = -1;
+        for (int s = 0; s < sweeps; s++)
+        {
+            int cols[4];
+            const int nc = colours(rg, cols);
+            for (int i = 0; i < nc; i++)
+            {
+                const bool done = s == 0 && i == 0 && firstDone;
+                if (!done && cols[i] != prev) ph.push_back(cols[i]);
+                prev = cols[i];
+            }
+        }
+        runPhases<T>(l, rg, ph, firstZero);
         return;
     }
     // the previous phase launched on this level: a phase that repeats it on

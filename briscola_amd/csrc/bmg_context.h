@@ -391,6 +391,9 @@ struct bmg_ctx
     void smooth(int l, int sweeps, bool firstDone = false,
                 bool firstZero = false);
     template <typename T>
+    void runPhases(int l, const Region& rg, const std::vector<int>& ph,
+                   bool firstZero);
+    template <typename T>
     bool firstPhaseRewrites(int l) const;
     template <typename T>
     bool fusedLevel(int l) const;

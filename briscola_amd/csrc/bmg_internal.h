@@ -160,6 +160,14 @@ template <typename T>
 void launch_rbgs(const LevelDims& L, const OpView<T>& op, Pair<T> x,
                  Pair<const T> b, int colour, const Region& rg,
                  hipStream_t s);
+// n (1..8) red-black phases over rg, colours cols[0..n), in one launch of
+// one workgroup with a barrier between phases (the first phase with
+// rg.zeroNb when zeroFirst): the same bits as n launch_rbgs calls.  For
+// small levels; not batchable.
+template <typename T>
+void launch_rbgs_chain(const LevelDims& L, const OpView<T>& op, Pair<T> x,
+                       Pair<const T> b, const Region& rg, const int* cols,
+                       int n, int zeroFirst, hipStream_t s);
 // The phase and, with snap, the periodic ghost fill (mode 0) with its
 // slab-plane snapshots in the same launch (the same bits as launch_rbgs
 // followed by launch_bc_fill_snap).
