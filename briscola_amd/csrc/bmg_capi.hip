@@ -1052,28 +1052,37 @@ void bmg_ctx::enqueueFCGIteration(int mode)
                                       : Pair<double>{{nullptr, nullptr}};
     const int nbu = launch_fcg_update(q0.L, dScalar, Rv, cst(Qv), rf, r0,
                                       pr(q0.b), partials, stream);
-    launch_reduce_rows(partials, nbu, 1, dScalar + 2, stream, hScalarDev + 2);
+    // r.r is needed on the device only by the learned preconditioner; else
+    // its partials wait (the cycle takes no norms) and are summed together
+    // with those of z.q, one launch
     if (mode == 2)
+    {
+        launch_reduce_rows(partials, nbu, 1, dScalar + 2, stream,
+                           hScalarDev + 2);
         enqueueLearned(dScalar + 2, 0.0, pr(q0.x));
+    }
     else
         enqueuePrecondition(mode == 1);
     record_tag(3000);
-    if (mode == 1)
-    {
-        const int nbz = launch_dot_zq<float>(q0.L, cpr(q0.xf), cst(Qv),
-                                             partials, stream);
+    double* const zqPart = mode == 2 ? partials : partials + nbu;
+    const int nbz =
+        mode == 1 ? launch_dot_zq<float>(q0.L, cpr(q0.xf), cst(Qv), zqPart,
+                                         stream)
+                  : launch_dot_zq<double>(q0.L, cpr(q0.x), cst(Qv), zqPart,
+                                          stream);
+    if (mode == 2)
         launch_reduce_rows(partials, nbz, 1, dScalar + 3, stream,
                            hScalarDev + 3);
-        launch_update_p<float>(q0.L, dScalar, cpr(q0.xf), Pv, X, stream);
-    }
     else
     {
-        const int nbz = launch_dot_zq<double>(q0.L, cpr(q0.x), cst(Qv),
-                                              partials, stream);
-        launch_reduce_rows(partials, nbz, 1, dScalar + 3, stream,
-                           hScalarDev + 3);
-        launch_update_p<double>(q0.L, dScalar, cpr(q0.x), Pv, X, stream);
+        if (nbz != nbu) throw Error(BMG_ERR_STATE, "FCG partial rows differ");
+        launch_reduce_rows(partials, nbu, 2, dScalar + 2, stream,
+                           hScalarDev + 2);
     }
+    if (mode == 1)
+        launch_update_p<float>(q0.L, dScalar, cpr(q0.xf), Pv, X, stream);
+    else
+        launch_update_p<double>(q0.L, dScalar, cpr(q0.x), Pv, X, stream);
     if (mode != 2 && deflating()) enqueueDeflate();
     HIPCK(hipGetLastError());
 }
