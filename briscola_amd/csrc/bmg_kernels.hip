@@ -3238,8 +3238,8 @@ struct KNfTerms
 
 // Sum of row[0, n) by one workgroup of 1024 threads, in a fixed order:
 // thread t adds row[t + 1024 m] for m = k mod 16 into accumulator k, 16
-// loads in flight per thread (a row of 32768 partials takes two rounds of
-// memory latency instead of eight), then a fixed tree over the 16
+// loads in flight per thread (a row of 32768 partials waits for memory
+// twice instead of eight times), then a fixed tree over the 16
 // accumulators and block_sum over the threads.
 __device__ __forceinline__ double row_sum_1024(const double* row, int n,
                                                double* sm)
