@@ -7,6 +7,7 @@ axis of 7 ``[C,L,R,B,T,A,F]`` coefficients.
 """
 import ctypes
 import os
+import sys
 
 import numpy as np
 
@@ -90,6 +91,15 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is missing: run python briscola_amd/build.py")
+    # PyTorch for ROCm bundles its own HIP runtime under the soname this
+    # library links; the runtime loaded first serves both, and torch finds
+    # no device on the system one.  Where torch is installed it is therefore
+    # loaded first, whatever order the caller imports the two in.
+    if "torch" not in sys.modules:
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
     lib = ctypes.CDLL(LIB_PATH)
     P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
     DP = ctypes.POINTER(ctypes.c_double)
