@@ -3402,11 +3402,11 @@ struct KApplyDots
     }
 };
 
-// x += alpha p, r -= alpha q with alpha = sc[1]/sc[0], per-tile partials of
-// r.r, and r handed to the preconditioner: STAGE 1 as FP32 into rf (mixed
-// cycle), STAGE 0 as FP64 into both r0 and b0 (the level-0 right-hand side
-// and restricted residual of an FP64 cycle), STAGE 2 nowhere (the learned
-// preconditioner reads r itself).
+// r -= alpha q with alpha = sc[1]/sc[0], per-tile partials of r.r, and r
+// handed to the preconditioner: STAGE 1 as FP32 into rf (mixed cycle),
+// STAGE 0 as FP64 into both r0 and b0 (the level-0 right-hand side and
+// restricted residual of an FP64 cycle), STAGE 2 nowhere (the learned
+// preconditioner reads r itself).  x += alpha p is done by update_p_body.
 template <int STAGE>
 __device__ __forceinline__ void fcg_update_body(const Bid bid,
                                                 const LevelDims& L,
